@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""Benchmark: SIMPLE outer iterations of the steady mixed-convection cavity on one MI355X.
+
+Workload (default): 1025 x 1025 nodes, skew amplitude 0.1 with tanh wall
+clustering beta = 1.5, Re = 400, Ri = 1, Pr = 0.71, FP64, relaxation 0.7/0.3,
+8 predictor sweeps, p' BiCGStab + multigrid to 1e-2 per outer iteration.
+One step = one outer iteration over the whole case (fields resident in HBM).
+
+    python bench.py [--steps K] [--warmup W] [--n 1025] [--cpu-iters 6] [--no-cpu]
+
+Prints one JSON line (value = cell-iterations/s, a cell = an interior node).
+The roofline object is for the kernel with the largest device time per outer
+iteration, timed live with HIP events on the case's stream during the timed
+steps; its algorithmic bytes per launch are (arrays read + arrays written) x
+interior nodes x 8 B.  cpu_baseline times the CPU oracle (a serial FP64 port of
+the same iteration) on this host for a bounded number of outer iterations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SIMPLE cell-iters/s at 1025² FP64 vs mixed; served solves/s + p99 at 1e-8 res"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic words per node per launch: arrays read + arrays written
+KERNEL_WORDS = {
+    "p_apply": 7 + 1,     # x, xi_x, xi_y, eta_x, eta_y, J, d -> y
+    "jacobi": 13 + 3,     # R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
+    "residual": 13 + 6,   # u,v,T,p, 4 metrics, alph,beta,gamma,P,Q -> R_u,R_v,R_T,d,U,V
+    "face_div": 9 + 1,    # u^,v^,p,d, 4 metrics, J -> q
+    "mg_smooth": 9 + 1,   # x,b,1/diag, 4 metrics, J, d -> x
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1025)
+    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    return ap.parse_args()
+
+
+def cpu_baseline(grid, n_iters, kw):
+    """Serial CPU oracle on the same workload for a bounded number of outer iterations."""
+    from oracle.oracle import CavityOracle, build, cav_params
+    build()
+    c = CavityOracle(grid, cav_params(**kw))
+    c.step()  # first iteration sets the normalisers; not timed
+    t0 = time.perf_counter()
+    for _ in range(n_iters):
+        c.step()
+    dt = time.perf_counter() - t0
+    cells = (grid.n0 - 2) * (grid.n1 - 2)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": cells * n_iters / dt, "unit": "cell-iter/s", "cores": 1, "kind": "port",
+            "sample": f"{n_iters} outer iterations of the same {grid.n0}x{grid.n1} case, serial FP64 "
+                      f"C oracle (gcc -O3), 1 of {os.cpu_count()} host threads ({model})"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is not "
+              "built in this version", file=sys.stderr)
+        sys.exit(2)
+    from cfd_solver_amd.cavity import CavityCase
+    from cfd_solver_amd.grid import cavity_grid
+
+    n = a.n
+    grid = cavity_grid(n, skew=0.1, beta=1.5)
+    phys = dict(Re=400.0, Ri=1.0, Pr=0.71)
+    case = CavityCase(grid, **phys)
+    for _ in range(a.warmup):
+        case.step()
+    case.profile(True)
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(a.steps):
+        st = case.step()
+    elapsed = time.perf_counter() - t0
+    prof = case.profile_read()
+    case.profile(False)
+
+    cells = (n - 2) * (n - 2)
+    value = cells * a.steps / elapsed
+    # dominant kernel: largest device time per outer iteration
+    name, (ms_tot, launches) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_ms = ms_tot / max(launches, 1)
+    nodes = n * n
+    bytes_launch = KERNEL_WORDS[name] * nodes * 8
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_ms": round(avg_ms, 5), "launches": launches,
+            "bytes_per_launch": bytes_launch,
+            "per_kernel_ms_per_iter": {k: round(v[0] / a.steps, 4) for k, v in prof.items()}}
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, "
+                               f"Ri 1, Pr 0.71, FP64 SIMPLE (0.7/0.3), 1 outer iteration per step",
+                   "grid": [n, n], "precision": a.precision, "mass_res_last": st.mass_res if st else None,
+                   "p_iters_last": st.p_iters if st else None},
+        "roofline": roof, "cpu_baseline": cpu, "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+"""Device-resident steady mixed-convection cavity (SIMPLE outer iteration on gfx950)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, Stats, call, f64
+from .grid import Grid, cavity_grid
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    _lib.load().cfd_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class CavityCase:
+    """One cavity case held in HBM; fields start at rest with the wall BCs applied."""
+
+    def __init__(self, grid: Grid, params: Params | None = None, **kw):
+        self.grid = grid
+        self.params = params if params is not None else default_params(**kw)
+        self.n0, self.n1 = grid.n0, grid.n1
+        g = grid
+        self._h = C.c_void_p()
+        self._keep = [f64(a) for a in (g.x, g.y, g.xix, g.xiy, g.etx, g.ety, g.alph, g.beta, g.gamma,
+                                       g.ajac, g.p1, g.q1)]
+        call("cfd_case_create", self.n0, self.n1, g.dxi, g.deta, *self._keep[:10],
+             self._keep[10].ctypes.data, self._keep[11].ctypes.data, C.byref(self.params),
+             C.byref(self._h))
+
+    @classmethod
+    def from_config(cls, n, skew=0.05, beta=None, **kw):
+        return cls(cavity_grid(n, skew=skew, beta=beta), **kw)
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.load().cfd_case_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, u, v, T, p):
+        call("cfd_case_set_fields", self._h, f64(u), f64(v), f64(T), f64(p))
+
+    def fields(self):
+        out = [np.zeros((self.n1, self.n0)) for _ in range(4)]
+        call("cfd_case_get_fields", self._h, *out)
+        return dict(u=out[0], v=out[1], T=out[2], p=out[3])
+
+    def apply_bcs(self):
+        call("cfd_case_apply_bcs", self._h)
+
+    def step(self, n=1) -> Stats:
+        st = Stats()
+        call("cfd_case_outer", self._h, int(n), C.byref(st))
+        return st
+
+    def solve(self, tol=1e-8, max_outer=100000) -> tuple[Stats, bool]:
+        st = Stats()
+        code = _lib.load().cfd_case_solve(self._h, tol, int(max_outer), C.byref(st))
+        if code not in (0, -4):
+            _lib.check("cfd_case_solve", code)
+        return st, code == 0
+
+    def nbytes(self):
+        b = C.c_size_t()
+        call("cfd_case_bytes", self._h, C.byref(b))
+        return b.value
+
+    def residuals(self):
+        o = [np.zeros((self.n1, self.n0)) for _ in range(4)]
+        call("cfd_case_residuals", self._h, *o)
+        return o
+
+    def p_apply(self, d, x):
+        y = np.zeros((self.n1, self.n0))
+        call("cfd_case_p_apply", self._h, f64(d), f64(x), y)
+        return y
+
+    def p_solve(self, d, q, rtol=1e-10, maxit=200):
+        x = np.zeros((self.n1, self.n0))
+        it, rr = C.c_int(), C.c_double()
+        call("cfd_case_p_solve", self._h, f64(d), f64(q), x, rtol, maxit, C.byref(it), C.byref(rr))
+        return x, it.value, rr.value
+
+    KERNELS = ("p_apply", "jacobi", "residual", "face_div", "mg_smooth")
+
+    def profile(self, enable=True):
+        call("cfd_case_profile", self._h, 1 if enable else 0)
+
+    def profile_read(self):
+        n = len(self.KERNELS)
+        ms = (C.c_double * n)()
+        cnt = (C.c_long * n)()
+        call("cfd_case_profile_read", self._h, ms, cnt, n)
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNELS)}
+
+    def time_kernel(self, kernel, reps=20):
+        ms = C.c_double()
+        call("cfd_case_time_kernel", self._h, int(kernel), int(reps), C.byref(ms))
+        return ms.value

@@ -1,0 +1,1441 @@
+// Cavity SIMPLE path on gfx950: device-resident case, per-stage kernels,
+// BiCGStab + geometric multigrid for the pressure correction, and the outer
+// iteration driver behind the cfd_case_* C ABI.
+//
+// One outer iteration (all on the case's stream):
+//   k_residual      reference RHS (hybrid convection, grad p, buoyancy) minus the
+//                   reference implicit operator applied to the current fields
+//                   -> R_u, R_v, R_T, the d factors, U, V      (solver.cpp:1062-1236, 751-848)
+//   k_jacobi x m    (A_lo/alpha) delta = R, Jacobi sweeps for u, v, T
+//   k_star          u* = u + du, T += dT, u-hat = u* + d grad p
+//   k_face_div      q = J-weighted divergence of the face velocities (solver.cpp:1389-1448)
+//   p' solve        BiCGStab, right-preconditioned by a multigrid V-cycle
+//   k_correct       u = u* - d grad p', p += alpha_p p'         (solver.cpp:1504-1541)
+//   BC kernels      adiabatic sides, wall pressure from normal momentum (solver.cpp:1667-1728)
+//   reductions      sum|q|, sum|R_T|, dmax (solver.cpp:1806-1852), wall Nusselt (1951-1982)
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cfd_cavity_kernels.h"
+
+namespace cfd {
+
+constexpr int MAXLEV = 16;
+constexpr int CFD_PROF_KINDS = 8;
+constexpr int PROF_EVENTS = 4096;
+constexpr int NT = BX * BY;  // threads per 2-D block
+
+// device scalar slots
+// (pairs reduced together must be adjacent: BB/RR, TS/TT, ENERGY/MOM)
+enum {
+    S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RV, S_TS, S_TT, S_SS, S_BB, S_RR, S_SUM, S_R0R,
+    S_MASS, S_ENERGY, S_MOM, S_DMAX, S_NUH, S_NUC, S_NSCAL
+};
+
+template <typename T>
+struct Level {
+    int n0 = 0, n1 = 0;
+    double h1 = 0, h2 = 0;
+    T *xix = nullptr, *xiy = nullptr, *etx = nullptr, *ety = nullptr, *jac = nullptr;
+    T *d = nullptr, *invd = nullptr;
+    T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;
+    bool owns_metrics = false;
+};
+
+struct Case {
+    int n0 = 0, n1 = 0;
+    size_t N = 0;
+    double h1 = 0, h2 = 0;
+    cfd_params prm{};
+    hipStream_t s = nullptr;
+    int device = 0;
+    // grid
+    double *xg, *yg, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *ajac, *p1, *q1;
+    // state and work
+    double *u, *v, *T, *p;
+    double *U, *V, *Ru, *Rv, *RT, *d;
+    double *jb[6];  // Jacobi ping-pong: du dv dT | tu tv tT
+    double *us, *vs, *uh, *vh, *q, *pc, *scratch;
+    double *bi[8];  // r r0 pv vv s t ph sh
+    Level<double> lev[MAXLEV];
+    int nlev = 0;
+    double *part = nullptr;  // reduction partials: 4 x pstride
+    int pstride = 0;
+    double *scal = nullptr;  // device scalars
+    double *hscal = nullptr; // pinned host mirror
+    double wall_len_hot = 0, wall_len_cold = 0;
+    double mass0 = 0, energy0 = 0;
+    int iter = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+    // live kernel timing: event pairs around level-0 launches of the hot kernels
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_kind;   // kind of each pending pair
+    int ev_used = 0;            // events recorded since the last drain
+    double prof_ms[CFD_PROF_KINDS] = {0};
+    long prof_n[CFD_PROF_KINDS] = {0};
+};
+
+// ------------------------------------------------------------------------
+// reductions
+
+// sum of `nparts` partials for each of `nv` arrays (stride pstride) -> scal[dst + k]
+__global__ void k_reduce_sum(const double *__restrict__ part, int pstride, int nparts, int nv,
+                             double *__restrict__ scal, int dst) {
+    const int k = blockIdx.x;
+    if (k >= nv) return;
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nparts; t += blockDim.x) s += part[(size_t)k * pstride + t];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) scal[dst + k] = s;
+}
+
+__global__ void k_reduce_max(const double *__restrict__ part, int nparts, double *__restrict__ scal,
+                             int dst) {
+    double s = 0.0;  // dmax is floored at 0 (solver.cpp:1806)
+    for (int t = threadIdx.x; t < nparts; t += blockDim.x) s = fmax(s, part[t]);
+    s = block_max<256>(s);
+    if (threadIdx.x == 0) scal[dst] = fmax(s, 0.0);
+}
+
+__device__ __forceinline__ int block_lin() { return blockIdx.x + blockIdx.y * gridDim.x; }
+
+// ------------------------------------------------------------------------
+// residual + d factors (oracle: or_cav_residuals)
+
+struct Phys {
+    double Re, Ri, Pr, dt, alpha_u, alpha_t, alpha_p;
+};
+
+__global__ __launch_bounds__(NT) void k_residual(
+    KGrid g, Phys ph, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ T, const double *__restrict__ p, const double *__restrict__ xix,
+    const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
+    const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
+    const double *__restrict__ p1, const double *__restrict__ q1, double *__restrict__ Ru,
+    double *__restrict__ Rv, double *__restrict__ RT, double *__restrict__ dd, double *__restrict__ Ua,
+    double *__restrict__ Va, double *__restrict__ part, int pstride) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    const double h1 = g.h1, h2 = g.h2;
+    double e_abs = 0.0, m_abs = 0.0;
+    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+        const size_t P = idx(i, j, n0);
+        const bool fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
+        const bool hyb = fit_i && fit_j;
+        const int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
+        const int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+        const double al = alph[P], be = beta[P], ga = gamma[P], pp = p1[P], qq = q1[P];
+        const double UP = xix[P] * u[P] + xiy[P] * v[P];
+        const double VP = etx[P] * u[P] + ety[P] * v[P];
+        const double *fl[3] = {u, v, T};
+        double conv[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double *f = fl[k];
+            const double sc = (k <= 1) ? ph.Re : ph.Re * ph.Pr;
+            const double pec1 = UP * sc * h1 / al;
+            const double pec2 = VP * sc * h2 / ga;
+            const double fm2i = fit_i ? f[P - 2] : 0.0, fp2i = fit_i ? f[P + 2] : 0.0;
+            const double fm2j = fit_j ? f[idx(i, j - 2, n0)] : 0.0, fp2j = fit_j ? f[idx(i, j + 2, n0)] : 0.0;
+            const double dxi = conv_deriv(mode_i, fm2i, f[P - 1], f[P], f[P + 1], fp2i, UP, pec1, h1);
+            const double det = conv_deriv(mode_j, fm2j, f[idx(i, j - 1, n0)], f[P], f[idx(i, j + 1, n0)],
+                                          fp2j, VP, pec2, h2);
+            conv[k] = UP * dxi + VP * det;
+        }
+        const double dp_dxi = (p[P + 1] - p[P - 1]) / (2.0 * h1);
+        const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) / (2.0 * h2);
+        const double dp_dx = xix[P] * dp_dxi + etx[P] * dp_de;
+        const double dp_dy = xiy[P] * dp_dxi + ety[P] * dp_de;
+        const double dt = ph.dt;
+        const double ru = dt * (-conv[0] - dp_dx) - aref_offdiag(u, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re, hyb);
+        const double rv = dt * (-conv[1] - dp_dy + ph.Ri * T[P]) -
+                          aref_offdiag(v, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re, hyb);
+        const double rt = -dt * conv[2] - aref_offdiag(T, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re * ph.Pr, hyb);
+        Ru[P] = ru;
+        Rv[P] = rv;
+        RT[P] = rt;
+        Ua[P] = UP;
+        Va[P] = VP;
+        double cl[9];
+        alo_coefs<double>(al, be, ga, pp, qq, UP, VP, h1, h2, dt, ph.Re, cl);
+        dd[P] = dt * ph.alpha_u / cl[4];
+        e_abs = fabs(rt);
+        m_abs = fabs(ru) + fabs(rv);
+    }
+    const double se = block_sum<NT>(e_abs);
+    const double sm = block_sum<NT>(m_abs);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[block_lin()] = se;
+        part[pstride + block_lin()] = sm;
+    }
+}
+
+// d on the border = nearest interior value (corners from the diagonal)
+__global__ void k_d_border(int n0, int n1, double *__restrict__ d) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n0) {
+        const int i = t;
+        if (i >= 1 && i <= n0 - 2) {
+            d[idx(i, 0, n0)] = d[idx(i, 1, n0)];
+            d[idx(i, n1 - 1, n0)] = d[idx(i, n1 - 2, n0)];
+        }
+    }
+    if (t < n1) {
+        const int j = t;
+        const int jj = j == 0 ? 1 : (j == n1 - 1 ? n1 - 2 : j);
+        d[idx(0, j, n0)] = d[idx(1, jj, n0)];
+        d[idx(n0 - 1, j, n0)] = d[idx(n0 - 2, jj, n0)];
+    }
+}
+
+// one Jacobi sweep of (A_lo/alpha) delta = R for u, v (shared operator) and T
+template <bool FIRST>
+__global__ __launch_bounds__(NT) void k_jacobi(
+    KGrid g, Phys ph, const double *__restrict__ Ru, const double *__restrict__ Rv,
+    const double *__restrict__ RT, const double *__restrict__ Ua, const double *__restrict__ Va,
+    const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
+    const double *__restrict__ p1, const double *__restrict__ q1, const double *__restrict__ du,
+    const double *__restrict__ dv, const double *__restrict__ dT, double *__restrict__ nu,
+    double *__restrict__ nv, double *__restrict__ nT) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const size_t P = idx(i, j, n0);
+    double cu[9], ct[9];
+    alo_coefs<double>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], g.h1, g.h2, ph.dt, ph.Re, cu);
+    alo_coefs<double>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], g.h1, g.h2, ph.dt,
+                      ph.Re * ph.Pr, ct);
+    double su = Ru[P], sv = Rv[P], st = RT[P];
+    if (!FIRST) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+            for (int di = -1; di <= 1; di++) {
+                if (di == 0 && dj == 0) continue;
+                const int k = (dj + 1) * 3 + (di + 1);
+                const size_t Q = idx(i + di, j + dj, n0);
+                su -= cu[k] * du[Q];
+                sv -= cu[k] * dv[Q];
+                st -= ct[k] * dT[Q];
+            }
+    }
+    nu[P] = su * ph.alpha_u / cu[4];
+    nv[P] = sv * ph.alpha_u / cu[4];
+    nT[P] = st * ph.alpha_t / ct[4];
+}
+
+// u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u
+__global__ __launch_bounds__(NT) void k_star(
+    KGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ du,
+    const double *__restrict__ dv, const double *__restrict__ dT, double *__restrict__ T,
+    const double *__restrict__ p, const double *__restrict__ d, const double *__restrict__ xix,
+    const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
+    double *__restrict__ us, double *__restrict__ vs, double *__restrict__ uh, double *__restrict__ vh) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = idx(i, j, n0);
+    if (i == 0 || j == 0 || i == n0 - 1 || j == n1 - 1) {
+        us[P] = u[P];
+        vs[P] = v[P];
+        return;
+    }
+    const double a = u[P] + du[P], b = v[P] + dv[P];
+    us[P] = a;
+    vs[P] = b;
+    T[P] = T[P] + dT[P];
+    const double dp_dxi = (p[P + 1] - p[P - 1]) / (2.0 * g.h1);
+    const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) / (2.0 * g.h2);
+    uh[P] = a + d[P] * (xix[P] * dp_dxi + etx[P] * dp_de);
+    vh[P] = b + d[P] * (xiy[P] * dp_dxi + ety[P] * dp_de);
+}
+
+// q = J-weighted divergence of the face velocities (oracle: face_div)
+__global__ __launch_bounds__(NT) void k_face_div(
+    KGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
+    const double *__restrict__ d, const double *__restrict__ xix, const double *__restrict__ xiy,
+    const double *__restrict__ etx, const double *__restrict__ ety, const double *__restrict__ jac,
+    double *__restrict__ q, double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    const double h1 = g.h1, h2 = g.h2;
+    double aq = 0.0;
+    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+        const int ipp = i + 1, inn = i - 1, jpp = j + 1, jnn = j - 1;
+        const size_t P = idx(i, j, n0), E = idx(ipp, j, n0), W = idx(inn, j, n0), Nn = idx(i, jpp, n0),
+                     S = idx(i, jnn, n0);
+        double fe = 0.0, fw = 0.0, gn = 0.0, gs = 0.0;
+        if (ipp != n0 - 1) {
+            const double de = 0.5 * (d[P] + d[E]);
+            const double dpdxi = (p[E] - p[P]) / h1;
+            const double dpde = (p[idx(ipp, jpp, n0)] + p[Nn] - p[S] - p[idx(ipp, jnn, n0)]) / (4.0 * h2);
+            const double uf = 0.5 * (uh[P] + uh[E]) - 0.5 * de * ((xix[P] + xix[E]) * dpdxi + (etx[P] + etx[E]) * dpde);
+            const double vf = 0.5 * (vh[P] + vh[E]) - 0.5 * de * ((xiy[P] + xiy[E]) * dpdxi + (ety[P] + ety[E]) * dpde);
+            fe = 0.5 * (jac[P] * xix[P] + jac[E] * xix[E]) * uf + 0.5 * (jac[P] * xiy[P] + jac[E] * xiy[E]) * vf;
+        }
+        if (inn != 0) {
+            const double dw = 0.5 * (d[P] + d[W]);
+            const double dpdxi = (p[P] - p[W]) / h1;
+            const double dpde = (p[Nn] + p[idx(inn, jpp, n0)] - p[S] - p[idx(inn, jnn, n0)]) / (4.0 * h2);
+            const double uf = 0.5 * (uh[P] + uh[W]) - 0.5 * dw * ((xix[P] + xix[W]) * dpdxi + (etx[P] + etx[W]) * dpde);
+            const double vf = 0.5 * (vh[P] + vh[W]) - 0.5 * dw * ((xiy[P] + xiy[W]) * dpdxi + (ety[P] + ety[W]) * dpde);
+            fw = 0.5 * (jac[P] * xix[P] + jac[W] * xix[W]) * uf + 0.5 * (jac[P] * xiy[P] + jac[W] * xiy[W]) * vf;
+        }
+        if (jpp != n1 - 1) {
+            const double dn = 0.5 * (d[P] + d[Nn]);
+            const double dpdxi = (p[idx(ipp, jpp, n0)] - p[idx(inn, jpp, n0)] + p[E] - p[W]) / (4.0 * h1);
+            const double dpde = (p[Nn] - p[P]) / h2;
+            const double uf = 0.5 * (uh[P] + uh[Nn]) - 0.5 * dn * ((xix[P] + xix[Nn]) * dpdxi + (etx[P] + etx[Nn]) * dpde);
+            const double vf = 0.5 * (vh[P] + vh[Nn]) - 0.5 * dn * ((xiy[P] + xiy[Nn]) * dpdxi + (ety[P] + ety[Nn]) * dpde);
+            gn = 0.5 * (jac[P] * etx[P] + jac[Nn] * etx[Nn]) * uf + 0.5 * (jac[P] * ety[P] + jac[Nn] * ety[Nn]) * vf;
+        }
+        if (jnn != 0) {
+            const double ds = 0.5 * (d[P] + d[S]);
+            const double dpdxi = (p[E] - p[W] + p[idx(ipp, jnn, n0)] - p[idx(inn, jnn, n0)]) / (4.0 * h1);
+            const double dpde = (p[P] - p[S]) / h2;
+            const double uf = 0.5 * (uh[P] + uh[S]) - 0.5 * ds * ((xix[P] + xix[S]) * dpdxi + (etx[P] + etx[S]) * dpde);
+            const double vf = 0.5 * (vh[P] + vh[S]) - 0.5 * ds * ((xiy[P] + xiy[S]) * dpdxi + (ety[P] + ety[S]) * dpde);
+            gs = 0.5 * (jac[P] * etx[P] + jac[S] * etx[S]) * uf + 0.5 * (jac[P] * ety[P] + jac[S] * ety[S]) * vf;
+        }
+        const double qq = (fe - fw) / h1 + (gn - gs) / h2;
+        q[P] = qq;
+        aq = fabs(qq);
+    }
+    const double s = block_sum<NT>(aq);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
+}
+
+// u = u* - d grad p' (p' read clamped = wall copy), p += alpha_p p'
+__global__ __launch_bounds__(NT) void k_correct(
+    KGrid g, double alpha_p, const double *__restrict__ us, const double *__restrict__ vs,
+    const double *__restrict__ pc, const double *__restrict__ d, const double *__restrict__ xix,
+    const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
+    double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const size_t P = idx(i, j, n0);
+    const double pe = pc[idx(clampi(i + 1, 1, n0 - 2), j, n0)], pw = pc[idx(clampi(i - 1, 1, n0 - 2), j, n0)];
+    const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)], ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)];
+    const double dpx = 0.5 * (pe - pw) / g.h1;
+    const double dpe = 0.5 * (pn - ps) / g.h2;
+    u[P] = us[P] - d[P] * (xix[P] * dpx + etx[P] * dpe);
+    v[P] = vs[P] - d[P] * (xiy[P] * dpx + ety[P] * dpe);
+    p[P] = p[P] + alpha_p * pc[P];
+}
+
+// adiabatic side walls (oracle: temperature_sides): new values into scratch
+__global__ void k_t_sides(KGrid g, const double *__restrict__ T, const double *__restrict__ alph,
+                          const double *__restrict__ beta, double *__restrict__ scratch) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = g.n0, n1 = g.n1;
+    if (j < 1 || j > n1 - 2) return;
+    size_t P = idx(0, j, n0);
+    double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
+    double txi = beta[P] / alph[P] * te;
+    scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
+    P = idx(n0 - 1, j, n0);
+    te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
+    txi = beta[P] / alph[P] * te;
+    scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
+}
+__global__ void k_t_sides_write(KGrid g, const double *__restrict__ scratch, double *__restrict__ T) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < 1 || j > g.n1 - 2) return;
+    T[idx(0, j, g.n0)] = scratch[j];
+    T[idx(g.n0 - 1, j, g.n0)] = scratch[g.n1 + j];
+}
+
+// wall pressure from the normal momentum equation (oracle: wall_p_value)
+struct WallIn {
+    const double *u, *v, *T, *p, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *p1, *q1, *ajac;
+};
+
+__device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j) {
+    const int n0 = g.n0;
+    const double h1 = g.h1, h2 = g.h2;
+    const size_t P = idx(i, j, n0);
+    const double U = w.xix[P] * w.u[P] + w.xiy[P] * w.v[P];
+    const double V = w.etx[P] * w.u[P] + w.ety[P] * w.v[P];
+    double dpx = 0.0, dpy = 0.0;
+    for (int k = 0; k < 2; k++) {
+        const double *f = k == 0 ? w.u : w.v;
+#define F(ii, jj) f[idx(ii, jj, n0)]
+        double d2u, conv;
+        if (side <= 1) {
+            const int s = (side == 0) ? 1 : -1;
+            const double aa = w.alph[P] * (F(i + 1, j) + F(i - 1, j) - 2 * F(i, j)) / (h1 * h1);
+            const double gg = w.gamma[P] * (F(i, j + 2 * s) + F(i, j) - 2 * F(i, j + s)) / (h2 * h2);
+            double bb;
+            if (side == 0)
+                bb = w.beta[P] * (F(i + 1, j + 1) + F(i - 1, j) - F(i - 1, j + 1) - F(i + 1, j)) / (2 * h1 * h2);
+            else
+                bb = w.beta[P] * (F(i + 1, j) + F(i - 1, j - 1) - F(i + 1, j - 1) - F(i - 1, j)) / (2 * h1 * h2);
+            const double qqq = w.q1[P] * s * (-3 * F(i, j) + 4 * F(i, j + s) - F(i, j + 2 * s)) / (2 * h2);
+            d2u = aa + gg - 2 * bb + qqq;
+            conv = U * 0.5 * (F(i + 1, j) - F(i - 1, j)) / h1;
+            conv = conv + V * s * (-3 * F(i, j) + 4 * F(i, j + s) - F(i, j + 2 * s)) / (2 * h2);
+        } else {
+            const int s = (side == 2) ? 1 : -1;
+            const double aa = w.gamma[P] * (F(i, j + 1) + F(i, j - 1) - 2 * F(i, j)) / (h2 * h2);
+            const double gg = w.alph[P] * (F(i + 2 * s, j) + F(i, j) - 2 * F(i + s, j)) / (h1 * h1);
+            double bb;
+            if (side == 2)
+                bb = w.beta[P] * (F(i + 1, j + 1) + F(i, j - 1) - F(i + 1, j - 1) - F(i, j + 1)) / (2 * h1 * h2);
+            else
+                bb = w.beta[P] * (F(i, j + 1) + F(i - 1, j - 1) - F(i, j - 1) - F(i - 1, j + 1)) / (2 * h1 * h2);
+            const double ppp = w.p1[P] * s * (-3 * F(i, j) + 4 * F(i + s, j) - F(i + 2 * s, j)) / (2 * h1);
+            d2u = aa + gg - 2 * bb + ppp;
+            conv = V * 0.5 * (F(i, j + 1) - F(i, j - 1)) / h2;
+            conv = conv + U * s * (-3 * F(i, j) + 4 * F(i + s, j) - F(i + 2 * s, j)) / (2 * h1);
+        }
+#undef F
+        if (k == 0) dpx = d2u / Re - conv;
+        else dpy = d2u / Re - conv + Ri * w.T[P];
+    }
+    const double J = w.ajac[P];
+    if (side <= 1) {
+        const double tx = -w.xiy[P] * J, ty = w.xix[P] * J;
+        const double dpn = (dpx * tx + dpy * ty) * h2;
+        return side == 0 ? w.p[idx(i, j + 1, n0)] - dpn : w.p[idx(i, j - 1, n0)] + dpn;
+    }
+    const double tx = w.ety[P] * J, ty = -w.etx[P] * J;
+    const double dpn = (dpx * tx + dpy * ty) * h1;
+    return side == 2 ? w.p[idx(i + 1, j, n0)] - dpn : w.p[idx(i - 1, j, n0)] + dpn;
+}
+
+// scratch layout: [bottom n0 | top n0 | left n1 | right n1]
+__global__ void k_p_walls(KGrid g, WallIn w, double Re, double Ri, double *__restrict__ scratch) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = g.n0, n1 = g.n1;
+    if (t >= 1 && t <= n0 - 2) {
+        scratch[t] = wall_p_value(w, g, Re, Ri, 0, t, 0);
+        scratch[n0 + t] = wall_p_value(w, g, Re, Ri, 1, t, n1 - 1);
+    }
+    if (t >= 1 && t <= n1 - 2) {
+        scratch[2 * n0 + t] = wall_p_value(w, g, Re, Ri, 2, 0, t);
+        scratch[2 * n0 + n1 + t] = wall_p_value(w, g, Re, Ri, 3, n0 - 1, t);
+    }
+}
+__global__ void k_p_walls_write(KGrid g, const double *__restrict__ scratch, double *__restrict__ p) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = g.n0, n1 = g.n1;
+    if (t >= 1 && t <= n0 - 2) {
+        p[idx(t, 0, n0)] = scratch[t];
+        p[idx(t, n1 - 1, n0)] = scratch[n0 + t];
+    }
+    if (t >= 1 && t <= n1 - 2) {
+        p[idx(0, t, n0)] = scratch[2 * n0 + t];
+        p[idx(n0 - 1, t, n0)] = scratch[2 * n0 + n1 + t];
+    }
+    if (t == 0) {
+        p[idx(0, 0, n0)] = 0.5 * (scratch[1] + scratch[2 * n0 + 1]);
+        p[idx(n0 - 1, 0, n0)] = 0.5 * (scratch[n0 - 2] + scratch[2 * n0 + n1 + 1]);
+        p[idx(0, n1 - 1, n0)] = 0.5 * (scratch[n0 + 1] + scratch[2 * n0 + n1 - 2]);
+        p[idx(n0 - 1, n1 - 1, n0)] = 0.5 * (scratch[2 * n0 - 2] + scratch[2 * n0 + 2 * n1 - 2]);
+    }
+}
+
+// velocity and temperature wall values (oracle: velocity_temperature_bcs)
+__global__ void k_uvt_walls(KGrid g, double lid, double th, double tc, double *__restrict__ u,
+                            double *__restrict__ v, double *__restrict__ T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = g.n0, n1 = g.n1;
+    if (t < n0) {
+        u[idx(t, 0, n0)] = 0.0;
+        v[idx(t, 0, n0)] = 0.0;
+        u[idx(t, n1 - 1, n0)] = (t == 0 || t == n0 - 1) ? 0.0 : lid;
+        v[idx(t, n1 - 1, n0)] = 0.0;
+        T[idx(t, 0, n0)] = th;
+        T[idx(t, n1 - 1, n0)] = tc;
+    }
+    if (t < n1) {
+        u[idx(0, t, n0)] = 0.0;
+        v[idx(0, t, n0)] = 0.0;
+        u[idx(n0 - 1, t, n0)] = 0.0;
+        v[idx(n0 - 1, t, n0)] = 0.0;
+    }
+}
+
+// signed max of the nodal dilation (solver.cpp:1821-1850) -> partials
+__global__ __launch_bounds__(NT) void k_dmax(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
+                                             const double *__restrict__ xix, const double *__restrict__ xiy,
+                                             const double *__restrict__ etx, const double *__restrict__ ety,
+                                             double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1;
+    double dm = 0.0;
+    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+        const size_t P = idx(i, j, n0);
+        dm = xix[P] * (u[P + 1] - u[P - 1]) / (2 * g.h1) +
+             etx[P] * (u[idx(i, j + 1, n0)] - u[idx(i, j - 1, n0)]) / (2 * g.h2) +
+             ety[P] * (v[idx(i, j + 1, n0)] - v[idx(i, j - 1, n0)]) / (2 * g.h2) +
+             xiy[P] * (v[P + 1] - v[P - 1]) / (2 * g.h1);
+    }
+    const double s = block_max<NT>(dm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
+}
+
+// wall Nusselt numbers, one block (oracle: nusselt)
+__global__ void k_nusselt(KGrid g, const double *__restrict__ T, const double *__restrict__ ajac,
+                          const double *__restrict__ etx, const double *__restrict__ ety, double len_hot,
+                          double len_cold, double dT, double *__restrict__ scal) {
+    const int n0 = g.n0, n1 = g.n1;
+    for (int wall = 0; wall < 2; wall++) {
+        const int j = wall ? n1 - 1 : 0, s = wall ? -1 : 1;
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < n0 - 1; i += blockDim.x) {
+            const int ip = i + 1;
+            const size_t A = idx(i, j, n0), B = idx(ip, j, n0);
+            const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
+            const double tj2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
+            const double fh1 = tj1 * (4 * T[idx(i, j + s, n0)] - 3 * T[A] - T[idx(i, j + 2 * s, n0)]) / (2 * g.h2);
+            const double fh2 = tj2 * (4 * T[idx(ip, j + s, n0)] - 3 * T[B] - T[idx(ip, j + 2 * s, n0)]) / (2 * g.h2);
+            acc += 0.5 * (fh1 + fh2) * g.h1;
+        }
+        acc = block_sum<256>(acc);
+        if (threadIdx.x == 0) {
+            const double len = wall ? len_cold : len_hot;
+            const double nu = -acc / (len * (dT != 0.0 ? dT : 1.0));
+            scal[wall ? S_NUC : S_NUH] = wall ? -nu : nu;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// pressure-correction operator, multigrid and BiCGStab (template on precision)
+
+template <typename T>
+struct LevArgs {
+    int n0, n1;
+    T h1, h2;
+    const T *xix, *xiy, *etx, *ety, *jac, *d;
+};
+
+template <typename T>
+LevArgs<T> largs(const Level<T> &L) {
+    return LevArgs<T>{L.n0, L.n1, (T)L.h1, (T)L.h2, L.xix, L.xiy, L.etx, L.ety, L.jac, L.d};
+}
+
+template <typename T>
+__device__ __forceinline__ T p9_row(const LevArgs<T> &a, const T *__restrict__ x, int i, int j) {
+    T c[9];
+    p9_coefs<T>(a.n0, a.n1, a.xix, a.xiy, a.etx, a.ety, a.jac, a.d, a.h1, a.h2, i, j, c);
+    T s = T(0);
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            const int ii = clampi(i + di, 1, a.n0 - 2), jj = clampi(j + dj, 1, a.n1 - 2);
+            s += c[(dj + 1) * 3 + (di + 1)] * x[idx(ii, jj, a.n0)];
+        }
+    return s;
+}
+
+// y = L x (interior), 0 on the border
+template <typename T>
+__global__ __launch_bounds__(NT) void k_p_apply(LevArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.n0 || j >= a.n1) return;
+    const size_t P = idx(i, j, a.n0);
+    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { y[P] = T(0); return; }
+    y[P] = p9_row<T>(a, x, i, j);
+}
+
+// r = b - L x (interior), 0 on the border
+template <typename T>
+__global__ __launch_bounds__(NT) void k_p_resid(LevArgs<T> a, const T *__restrict__ x, const T *__restrict__ b,
+                                                T *__restrict__ r) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.n0 || j >= a.n1) return;
+    const size_t P = idx(i, j, a.n0);
+    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { r[P] = T(0); return; }
+    r[P] = b[P] - p9_row<T>(a, x, i, j);
+}
+
+// inverse diagonal of the clamped operator (oracle: p9_diag)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_p_invdiag(LevArgs<T> a, T *__restrict__ invd) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > a.n0 - 2 || j < 1 || j > a.n1 - 2) return;
+    T c[9];
+    p9_coefs<T>(a.n0, a.n1, a.xix, a.xiy, a.etx, a.ety, a.jac, a.d, a.h1, a.h2, i, j, c);
+    T s = T(0);
+    for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++)
+            if (clampi(i + di, 1, a.n0 - 2) == i && clampi(j + dj, 1, a.n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
+    invd[idx(i, j, a.n0)] = T(1) / s;
+}
+
+// Jacobi sweep: xo = xi + w (b - L xi) invd (FIRST: xo = w b invd)
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(NT) void k_p_smooth(LevArgs<T> a, T w, const T *__restrict__ xi,
+                                                 const T *__restrict__ b, const T *__restrict__ invd,
+                                                 T *__restrict__ xo) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.n0 || j >= a.n1) return;
+    const size_t P = idx(i, j, a.n0);
+    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { xo[P] = T(0); return; }
+    if (FIRST) {
+        xo[P] = w * b[P] * invd[P];
+    } else {
+        xo[P] = xi[P] + w * (b[P] - p9_row<T>(a, xi, i, j)) * invd[P];
+    }
+}
+
+// full weighting of the fine residual into the coarse right-hand side
+template <typename T>
+__global__ __launch_bounds__(NT) void k_restrict(int n0, const T *__restrict__ r, int m0, int m1,
+                                                 T *__restrict__ bc) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= m0 || J >= m1) return;
+    const size_t Pc = idx(I, J, m0);
+    if (I == 0 || J == 0 || I == m0 - 1 || J == m1 - 1) { bc[Pc] = T(0); return; }
+    const int i = 2 * I, j = 2 * J;
+    bc[Pc] = (T(4) * r[idx(i, j, n0)] +
+              T(2) * (r[idx(i - 1, j, n0)] + r[idx(i + 1, j, n0)] + r[idx(i, j - 1, n0)] + r[idx(i, j + 1, n0)]) +
+              (r[idx(i - 1, j - 1, n0)] + r[idx(i + 1, j - 1, n0)] + r[idx(i - 1, j + 1, n0)] +
+               r[idx(i + 1, j + 1, n0)])) /
+             T(16);
+}
+
+// bilinear prolongation of the coarse correction, added on the fine interior
+template <typename T>
+__global__ __launch_bounds__(NT) void k_prolong_add(int n0, int n1, T *__restrict__ x, int m0, int m1,
+                                                    const T *__restrict__ xc) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const int I = i >> 1, J = j >> 1;
+#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
+    T e;
+    if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
+    else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
+    else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
+    else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
+#undef XC
+    x[idx(i, j, n0)] += e;
+}
+
+// interior sum of a vector -> partials
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sum_int(int n0, int n1, const T *__restrict__ v, double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double s = 0.0;
+    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) s = (double)v[idx(i, j, n0)];
+    s = block_sum<NT>(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
+}
+
+// v -= scal[slot] / count on the interior
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sub_mean(int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
+                                                 int slot) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
+    v[idx(i, j, n0)] = (T)((double)v[idx(i, j, n0)] - m);
+}
+
+template <typename T>
+__global__ void k_inject(int n0, const T *__restrict__ f, int m0, int m1, T *__restrict__ c) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= m0 || J >= m1) return;
+    c[idx(I, J, m0)] = f[idx(2 * I, 2 * J, n0)];
+}
+
+// --- BiCGStab vector kernels (FP64 vectors; scalars read from device memory)
+
+// dot products over the interior -> partials (up to 2 pairs)
+__global__ __launch_bounds__(NT) void k_dot2(int n0, int n1, const double *__restrict__ a, const double *__restrict__ b,
+                                             const double *__restrict__ c, const double *__restrict__ d2,
+                                             double *__restrict__ part, int pstride) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double s1 = 0.0, s2 = 0.0;
+    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+        const size_t P = idx(i, j, n0);
+        s1 = a[P] * b[P];
+        if (c) s2 = c[P] * d2[P];
+    }
+    s1 = block_sum<NT>(s1);
+    s2 = block_sum<NT>(s2);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[block_lin()] = s1;
+        part[pstride + block_lin()] = s2;
+    }
+}
+
+// pv = r (first) or r + beta (pv - omega vv)
+__global__ __launch_bounds__(NT) void k_bicg_p(int n0, int n1, bool first, const double *__restrict__ scal,
+                                               const double *__restrict__ r, const double *__restrict__ vv,
+                                               double *__restrict__ pv) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = idx(i, j, n0);
+    if (first) { pv[P] = r[P]; return; }
+    const double beta = (scal[S_RHO] / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+    pv[P] = r[P] + beta * (pv[P] - scal[S_OMEGA] * vv[P]);
+}
+
+// alpha = rho / (r0, v) (from scal[S_RV]); s = r - alpha v; partial (s, s)
+__global__ __launch_bounds__(NT) void k_bicg_s(int n0, int n1, double *__restrict__ scal, const double *__restrict__ r,
+                                               const double *__restrict__ vv, double *__restrict__ s,
+                                               double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const double alpha = scal[S_RHO] / scal[S_RV];
+    double ss = 0.0;
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        const double sv = r[P] - alpha * vv[P];
+        s[P] = sv;
+        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) ss = sv * sv;
+    }
+    ss = block_sum<NT>(ss);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = ss;
+    if (block_lin() == 0 && threadIdx.x == 0 && threadIdx.y == 0) scal[S_ALPHA] = alpha;
+}
+
+// x += alpha ph (early exit on ||s||)
+__global__ __launch_bounds__(NT) void k_bicg_x1(int n0, int n1, const double *__restrict__ scal,
+                                                const double *__restrict__ ph, double *__restrict__ x) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = idx(i, j, n0);
+    x[P] += scal[S_ALPHA] * ph[P];
+}
+
+// omega = (t,s)/(t,t); x += alpha ph + omega sh; r = s - omega t; partials (r,r), (r0,r)
+__global__ __launch_bounds__(NT) void k_bicg_xr(int n0, int n1, const double *__restrict__ scal,
+                                                const double *__restrict__ ph, const double *__restrict__ sh,
+                                                const double *__restrict__ s, const double *__restrict__ t,
+                                                const double *__restrict__ r0, double *__restrict__ x,
+                                                double *__restrict__ r, double *__restrict__ part, int pstride) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const double tt = scal[S_TT];
+    const double omega = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
+    const double alpha = scal[S_ALPHA];
+    double rr = 0.0, r0r = 0.0;
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        x[P] += alpha * ph[P] + omega * sh[P];
+        const double rv = s[P] - omega * t[P];
+        r[P] = rv;
+        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+            rr = rv * rv;
+            r0r = r0[P] * rv;
+        }
+    }
+    rr = block_sum<NT>(rr);
+    r0r = block_sum<NT>(r0r);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[block_lin()] = rr;
+        part[pstride + block_lin()] = r0r;
+    }
+}
+
+__global__ void k_set_omega(double *__restrict__ scal) {
+    const double tt = scal[S_TT];
+    scal[S_OMEGA] = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
+}
+
+// rho_old = rho; rho = (r0, r) of the new r
+__global__ void k_rotate_rho(double *__restrict__ scal) {
+    scal[S_RHO_OLD] = scal[S_RHO];
+    scal[S_RHO] = scal[S_R0R];
+}
+
+// start values of the BiCGStab scalars: rho = (r, r), rho_old = alpha = omega = 1
+__global__ void k_init_scal(double *__restrict__ scal) {
+    scal[S_RHO] = scal[S_RR];
+    scal[S_RHO_OLD] = 1.0;
+    scal[S_ALPHA] = 1.0;
+    scal[S_OMEGA] = 1.0;
+}
+
+}  // namespace cfd
+
+// ==========================================================================
+// host side
+
+using namespace cfd;
+
+namespace {
+
+template <typename T>
+int dalloc(Case *c, T **p, size_t n) {
+    void *ptr = nullptr;
+    if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) return CFD_E_OOM;
+    if (hipMemsetAsync(ptr, 0, n * sizeof(T), c->s) != hipSuccess) return CFD_E_HIP;
+    c->allocs.push_back(ptr);
+    c->bytes += n * sizeof(T);
+    *p = static_cast<T *>(ptr);
+    return CFD_OK;
+}
+
+int nblocks2d(int n0, int n1) {
+    dim3 g = grid2d(n0, n1);
+    return (int)(g.x * g.y);
+}
+
+// reduce `nv` partial arrays (stride pstride) into scal[dst..]
+int reduce_sum(Case *c, int nparts, int nv, int dst) {
+    k_reduce_sum<<<nv, 256, 0, c->s>>>(c->part, c->pstride, nparts, nv, c->scal, dst);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+// kernel kinds timed live: 0 p' apply/residual (fine level), 1 predictor Jacobi sweep,
+// 2 residual/RHS, 3 face divergence, 4 multigrid smoothing sweep (fine level)
+void prof_begin(Case *c, int kind) {
+    if (!c->prof || c->ev_used + 2 > (int)c->ev.size()) return;
+    (void)hipEventRecord(c->ev[c->ev_used], c->s);
+    c->ev_kind.push_back(kind);
+}
+void prof_end(Case *c) {
+    if (!c->prof || c->ev_kind.size() * 2 != (size_t)c->ev_used + 1) return;
+    (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
+    c->ev_used += 2;
+}
+// after a stream synchronisation: fold the recorded pairs into the totals
+void prof_drain(Case *c) {
+    if (!c->prof) return;
+    for (int k = 0; k + 1 < c->ev_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+            const int kind = c->ev_kind[k / 2];
+            c->prof_ms[kind] += ms;
+            c->prof_n[kind] += 1;
+        }
+    }
+    c->ev_used = 0;
+    c->ev_kind.clear();
+}
+// a begin without its end (pool exhausted mid-pair) is dropped at the next begin
+#define PROF_B(c, kind)                                                        \
+    do {                                                                       \
+        if ((c)->prof && (c)->ev_kind.size() * 2 > (size_t)(c)->ev_used)       \
+            (c)->ev_kind.pop_back();                                           \
+        prof_begin((c), (kind));                                               \
+    } while (0)
+#define PROF_E(c) prof_end(c)
+
+int sync_scal(Case *c) {
+    CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    prof_drain(c);
+    return CFD_OK;
+}
+
+#define RET(x)                      \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != CFD_OK) return r_; \
+    } while (0)
+
+template <typename T>
+int project_mean(Case *c, int n0, int n1, T *v) {
+    const int nb = nblocks2d(n0, n1);
+    k_sum_int<T><<<grid2d(n0, n1), dim3(BX, BY), 0, c->s>>>(n0, n1, v, c->part);
+    CFD_LAUNCH_CHECK();
+    RET(reduce_sum(c, nb, 1, S_SUM));
+    k_sub_mean<T><<<grid2d(n0, n1), dim3(BX, BY), 0, c->s>>>(n0, n1, v, c->scal, S_SUM);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+// smoothing sweeps on level L; x holds the result (x may be swapped with t)
+int mg_smooth(Case *c, Level<double> &L, int sweeps, bool from_zero) {
+    const double w = c->prm.mg_omega;
+    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
+    const LevArgs<double> a = largs(L);
+    for (int s = 0; s < sweeps; s++) {
+        if (from_zero && s == 0) {
+            k_p_smooth<double, true><<<gr, bl, 0, c->s>>>(a, w, L.x, L.b, L.invd, L.x);
+        } else {
+            const bool fine = (&L == &c->lev[0]);
+            if (fine) PROF_B(c, 4);
+            k_p_smooth<double, false><<<gr, bl, 0, c->s>>>(a, w, L.x, L.b, L.invd, L.t);
+            if (fine) PROF_E(c);
+            std::swap(L.x, L.t);
+        }
+        CFD_LAUNCH_CHECK();
+    }
+    return CFD_OK;
+}
+
+int mg_vcycle(Case *c, int l) {
+    Level<double> &L = c->lev[l];
+    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
+    if (l == c->nlev - 1) {
+        if (c->prm.mg_coarse <= 0) {
+            CFD_HIP(hipMemsetAsync(L.x, 0, (size_t)L.n0 * L.n1 * sizeof(double), c->s));
+            return CFD_OK;
+        }
+        return mg_smooth(c, L, c->prm.mg_coarse, true);
+    }
+    if (c->prm.mg_pre > 0) {
+        RET(mg_smooth(c, L, c->prm.mg_pre, true));
+    } else {
+        CFD_HIP(hipMemsetAsync(L.x, 0, (size_t)L.n0 * L.n1 * sizeof(double), c->s));
+    }
+    k_p_resid<double><<<gr, bl, 0, c->s>>>(largs(L), L.x, L.b, L.r);
+    CFD_LAUNCH_CHECK();
+    Level<double> &C = c->lev[l + 1];
+    k_restrict<double><<<grid2d(C.n0, C.n1), bl, 0, c->s>>>(L.n0, L.r, C.n0, C.n1, C.b);
+    CFD_LAUNCH_CHECK();
+    RET(project_mean(c, C.n0, C.n1, C.b));
+    RET(mg_vcycle(c, l + 1));
+    k_prolong_add<double><<<gr, bl, 0, c->s>>>(L.n0, L.n1, L.x, C.n0, C.n1, C.x);
+    CFD_LAUNCH_CHECK();
+    return mg_smooth(c, L, c->prm.mg_post, false);
+}
+
+// z = M^-1 y (V-cycle on level 0)
+int precond(Case *c, const double *y, double *z) {
+    Level<double> &L = c->lev[0];
+    CFD_HIP(hipMemcpyAsync(L.b, y, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    RET(mg_vcycle(c, 0));
+    CFD_HIP(hipMemcpyAsync(z, L.x, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    return CFD_OK;
+}
+
+// refresh d on the coarse levels and the inverse diagonals of every level
+int mg_setup(Case *c) {
+    const dim3 bl(BX, BY);
+    for (int l = 1; l < c->nlev; l++) {
+        Level<double> &F = c->lev[l - 1], &L = c->lev[l];
+        k_inject<double><<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(F.n0, F.d, L.n0, L.n1, L.d);
+        CFD_LAUNCH_CHECK();
+    }
+    for (int l = 0; l < c->nlev; l++) {
+        Level<double> &L = c->lev[l];
+        k_p_invdiag<double><<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(largs(L), L.invd);
+        CFD_LAUNCH_CHECK();
+    }
+    return CFD_OK;
+}
+
+// BiCGStab, right-preconditioned (oracle: bicgstab); x must be 0 on entry
+int bicgstab(Case *c, const double *b, double *x, double rtol, int maxit, int *its_out, double *relres) {
+    const int n0 = c->n0, n1 = c->n1;
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const int nb = nblocks2d(n0, n1);
+    double *r = c->bi[0], *r0 = c->bi[1], *pv = c->bi[2], *vv = c->bi[3];
+    double *s = c->bi[4], *t = c->bi[5], *ph = c->bi[6], *sh = c->bi[7];
+    const LevArgs<double> a0 = largs(c->lev[0]);
+    // r = b - A x (x = 0), projected; r0 = r
+    k_p_resid<double><<<gr, bl, 0, c->s>>>(a0, x, b, r);
+    CFD_LAUNCH_CHECK();
+    RET(project_mean(c, n0, n1, r));
+    CFD_HIP(hipMemcpyAsync(r0, r, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    CFD_HIP(hipMemsetAsync(pv, 0, c->N * sizeof(double), c->s));
+    CFD_HIP(hipMemsetAsync(vv, 0, c->N * sizeof(double), c->s));
+    k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, b, b, r, r, c->part, c->pstride);
+    CFD_LAUNCH_CHECK();
+    RET(reduce_sum(c, nb, 2, S_BB));  // S_BB, S_RR
+    k_init_scal<<<1, 1, 0, c->s>>>(c->scal);
+    CFD_LAUNCH_CHECK();
+    RET(sync_scal(c));
+    const double bnorm = std::sqrt(c->hscal[S_BB]);
+    double rnorm = std::sqrt(c->hscal[S_RR]);
+    *its_out = 0;
+    if (bnorm == 0.0) { *relres = 0.0; return CFD_OK; }
+    const double tol = rtol * bnorm;
+    if (rnorm <= tol) { *relres = rnorm / bnorm; return CFD_OK; }
+    double rho = c->hscal[S_RR];  // (r0, r) with r0 = r
+    int it;
+    for (it = 1; it <= maxit; it++) {
+        if (rho == 0.0) break;
+        k_bicg_p<<<gr, bl, 0, c->s>>>(n0, n1, it == 1, c->scal, r, vv, pv);
+        CFD_LAUNCH_CHECK();
+        RET(precond(c, pv, ph));
+        PROF_B(c, 0);
+        k_p_apply<double><<<gr, bl, 0, c->s>>>(a0, ph, vv);
+        PROF_E(c);
+        CFD_LAUNCH_CHECK();
+        k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, r0, vv, nullptr, nullptr, c->part, c->pstride);
+        CFD_LAUNCH_CHECK();
+        RET(reduce_sum(c, nb, 1, S_RV));
+        k_bicg_s<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, r, vv, s, c->part);
+        CFD_LAUNCH_CHECK();
+        RET(reduce_sum(c, nb, 1, S_SS));
+        RET(sync_scal(c));
+        const double snorm = std::sqrt(c->hscal[S_SS]);
+        if (snorm <= tol) {
+            k_bicg_x1<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, ph, x);
+            CFD_LAUNCH_CHECK();
+            rnorm = snorm;
+            break;
+        }
+        RET(precond(c, s, sh));
+        PROF_B(c, 0);
+        k_p_apply<double><<<gr, bl, 0, c->s>>>(a0, sh, t);
+        PROF_E(c);
+        CFD_LAUNCH_CHECK();
+        k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, t, s, t, t, c->part, c->pstride);
+        CFD_LAUNCH_CHECK();
+        RET(reduce_sum(c, nb, 2, S_TS));  // S_TS, S_TT
+        k_set_omega<<<1, 1, 0, c->s>>>(c->scal);
+        CFD_LAUNCH_CHECK();
+        k_bicg_xr<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, ph, sh, s, t, r0, x, r, c->part, c->pstride);
+        CFD_LAUNCH_CHECK();
+        RET(reduce_sum(c, nb, 2, S_RR));  // S_RR, then (r0, r) into S_SUM
+        k_reduce_sum<<<1, 256, 0, c->s>>>(c->part + c->pstride, c->pstride, nb, 1, c->scal, S_R0R);
+        CFD_LAUNCH_CHECK();
+        k_rotate_rho<<<1, 1, 0, c->s>>>(c->scal);
+        CFD_LAUNCH_CHECK();
+        RET(sync_scal(c));
+        rnorm = std::sqrt(c->hscal[S_RR]);
+        const double omega = c->hscal[S_OMEGA];
+        if (rnorm <= tol || omega == 0.0) break;
+        rho = c->hscal[S_RHO];
+    }
+    *its_out = it > maxit ? maxit : it;
+    *relres = rnorm / bnorm;
+    return CFD_OK;
+}
+
+int build_levels(Case *c) {
+    const int want = c->prm.mg_levels < 1 ? 1 : c->prm.mg_levels;
+    Level<double> &L0 = c->lev[0];
+    L0.n0 = c->n0; L0.n1 = c->n1; L0.h1 = c->h1; L0.h2 = c->h2;
+    L0.xix = c->xix; L0.xiy = c->xiy; L0.etx = c->etx; L0.ety = c->ety; L0.jac = c->ajac; L0.d = c->d;
+    RET(dalloc(c, &L0.invd, c->N));
+    RET(dalloc(c, &L0.x, c->N));
+    RET(dalloc(c, &L0.b, c->N));
+    RET(dalloc(c, &L0.r, c->N));
+    RET(dalloc(c, &L0.t, c->N));
+    int l = 1;
+    while (l < want && l < MAXLEV) {
+        const Level<double> &F = c->lev[l - 1];
+        if ((F.n0 & 1) == 0 || (F.n1 & 1) == 0 || F.n0 < 9 || F.n1 < 9) break;
+        Level<double> &L = c->lev[l];
+        L.n0 = (F.n0 + 1) / 2; L.n1 = (F.n1 + 1) / 2;
+        L.h1 = 2.0 * F.h1; L.h2 = 2.0 * F.h2;
+        const size_t M = (size_t)L.n0 * L.n1;
+        RET(dalloc(c, &L.xix, M)); RET(dalloc(c, &L.xiy, M)); RET(dalloc(c, &L.etx, M));
+        RET(dalloc(c, &L.ety, M)); RET(dalloc(c, &L.jac, M)); RET(dalloc(c, &L.d, M));
+        RET(dalloc(c, &L.invd, M)); RET(dalloc(c, &L.x, M)); RET(dalloc(c, &L.b, M));
+        RET(dalloc(c, &L.r, M)); RET(dalloc(c, &L.t, M));
+        L.owns_metrics = true;
+        const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
+        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.xix, L.n0, L.n1, L.xix);
+        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.xiy, L.n0, L.n1, L.xiy);
+        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.etx, L.n0, L.n1, L.etx);
+        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.ety, L.n0, L.n1, L.ety);
+        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.jac, L.n0, L.n1, L.jac);
+        CFD_LAUNCH_CHECK();
+        l++;
+    }
+    c->nlev = l;
+    return CFD_OK;
+}
+
+KGrid kg(const Case *c) { return KGrid{c->n0, c->n1, c->h1, c->h2}; }
+Phys phys(const Case *c) {
+    return Phys{c->prm.Re, c->prm.Ri, c->prm.Pr, c->prm.dt, c->prm.alpha_u, c->prm.alpha_t, c->prm.alpha_p};
+}
+WallIn wall_in(const Case *c) {
+    return WallIn{c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
+                  c->p1, c->q1, c->ajac};
+}
+
+int launch_residual(Case *c) {
+    PROF_B(c, 2);
+    k_residual<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(
+        kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
+        c->p1, c->q1, c->Ru, c->Rv, c->RT, c->d, c->U, c->V, c->part, c->pstride);
+    PROF_E(c);
+    CFD_LAUNCH_CHECK();
+    const int m = std::max(c->n0, c->n1);
+    k_d_border<<<(m + 255) / 256, 256, 0, c->s>>>(c->n0, c->n1, c->d);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+// predictor sweeps; returns pointers of the final du, dv, dT
+int predictor(Case *c, double **du, double **dv, double **dT) {
+    double *a[3] = {c->jb[0], c->jb[1], c->jb[2]};
+    double *b[3] = {c->jb[3], c->jb[4], c->jb[5]};
+    const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
+    for (int s = 0; s < c->prm.mom_sweeps; s++) {
+        if (s == 0)
+            k_jacobi<true><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
+                                                c->gamma, c->p1, c->q1, a[0], a[1], a[2], b[0], b[1], b[2]);
+        else {
+            PROF_B(c, 1);
+            k_jacobi<false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
+                                                 c->gamma, c->p1, c->q1, a[0], a[1], a[2], b[0], b[1], b[2]);
+            PROF_E(c);
+        }
+        CFD_LAUNCH_CHECK();
+        for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
+    }
+    if (c->prm.mom_sweeps <= 0) {
+        // no predictor: zero correction
+        for (int k = 0; k < 3; k++) CFD_HIP(hipMemsetAsync(a[k], 0, c->N * sizeof(double), c->s));
+    }
+    *du = a[0]; *dv = a[1]; *dT = a[2];
+    return CFD_OK;
+}
+
+int apply_bcs_dev(Case *c) {
+    const int m = std::max(c->n0, c->n1);
+    const int nb = (m + 255) / 256;
+    k_uvt_walls<<<nb, 256, 0, c->s>>>(kg(c), c->prm.lid_u, c->prm.t_hot, c->prm.t_cold, c->u, c->v, c->T);
+    CFD_LAUNCH_CHECK();
+    k_t_sides<<<nb, 256, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
+    k_t_sides_write<<<nb, 256, 0, c->s>>>(kg(c), c->scratch, c->T);
+    k_p_walls<<<nb, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->scratch);
+    k_p_walls_write<<<nb, 256, 0, c->s>>>(kg(c), c->scratch, c->p);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+int outer_once(Case *c, cfd_stats *st) {
+    const int n0 = c->n0, n1 = c->n1;
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const int nb = nblocks2d(n0, n1);
+    // 1-2 residuals and d
+    RET(launch_residual(c));
+    RET(reduce_sum(c, nb, 2, S_ENERGY));  // S_ENERGY, S_MOM
+    // 3 predictor
+    double *du, *dv, *dT;
+    RET(predictor(c, &du, &dv, &dT));
+    // 4 u*, T, u-hat, face divergence
+    k_star<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, du, dv, dT, c->T, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
+                                c->us, c->vs, c->uh, c->vh);
+    CFD_LAUNCH_CHECK();
+    PROF_B(c, 3);
+    k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
+                                    c->q, c->part);
+    PROF_E(c);
+    CFD_LAUNCH_CHECK();
+    RET(reduce_sum(c, nb, 1, S_MASS));
+    // 5 p' solve
+    RET(mg_setup(c));
+    CFD_HIP(hipMemsetAsync(c->pc, 0, c->N * sizeof(double), c->s));
+    int its = 0;
+    double relres = 0.0;
+    RET(bicgstab(c, c->q, c->pc, c->prm.p_rtol, c->prm.p_maxit, &its, &relres));
+    RET(project_mean(c, n0, n1, c->pc));
+    // 6-7 correction and BCs
+    k_correct<<<gr, bl, 0, c->s>>>(kg(c), c->prm.alpha_p, c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx,
+                                   c->ety, c->u, c->v, c->p);
+    CFD_LAUNCH_CHECK();
+    {
+        const int m = std::max(n0, n1);
+        const int nbl = (m + 255) / 256;
+        k_t_sides<<<nbl, 256, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
+        k_t_sides_write<<<nbl, 256, 0, c->s>>>(kg(c), c->scratch, c->T);
+        k_p_walls<<<nbl, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->scratch);
+        k_p_walls_write<<<nbl, 256, 0, c->s>>>(kg(c), c->scratch, c->p);
+        CFD_LAUNCH_CHECK();
+    }
+    // 8 reductions
+    k_dmax<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->part);
+    CFD_LAUNCH_CHECK();
+    k_reduce_max<<<1, 256, 0, c->s>>>(c->part, nb, c->scal, S_DMAX);
+    CFD_LAUNCH_CHECK();
+    k_nusselt<<<1, 256, 0, c->s>>>(kg(c), c->T, c->ajac, c->etx, c->ety, c->wall_len_hot, c->wall_len_cold,
+                                   c->prm.t_hot - c->prm.t_cold, c->scal);
+    CFD_LAUNCH_CHECK();
+    RET(sync_scal(c));
+    const double *h = c->hscal;
+    c->iter++;
+    if (c->iter == 1) {
+        c->mass0 = h[S_MASS];
+        c->energy0 = h[S_ENERGY];
+    }
+    if (st) {
+        st->mass_l1 = h[S_MASS];
+        st->energy_l1 = h[S_ENERGY];
+        st->mom_l1 = h[S_MOM];
+        st->dmax = h[S_DMAX];
+        st->nu_hot = h[S_NUH];
+        st->nu_cold = h[S_NUC];
+        st->p_iters = its;
+        st->p_relres = relres;
+        st->mass_res = c->mass0 > 0 ? h[S_MASS] / c->mass0 : 0.0;
+        st->energy_res = c->energy0 > 0 ? h[S_ENERGY] / c->energy0 : 0.0;
+        st->iter = c->iter;
+    }
+    if (!std::isfinite(h[S_MASS]) || !std::isfinite(h[S_ENERGY])) return CFD_E_NAN;
+    return CFD_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+
+extern "C" {
+
+void cfd_params_default(cfd_params *p) {
+    if (!p) return;
+    p->Re = 100.0; p->Ri = 1.0; p->Pr = 0.71; p->dt = 1.0e4;
+    p->alpha_u = 0.7; p->alpha_p = 0.3; p->alpha_t = 1.0;
+    p->lid_u = 1.0; p->t_hot = 1.0; p->t_cold = 0.0;
+    p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 50;
+    p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 30; p->mg_omega = 0.8;
+    p->precision = CFD_FP64;
+}
+
+int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, const double *y,
+                    const double *xix, const double *xiy, const double *etx, const double *ety,
+                    const double *alph, const double *beta, const double *gamma, const double *ajac,
+                    const double *p1, const double *q1, const cfd_params *params, cfd_case *out) {
+    if (!out || !params || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac)
+        return CFD_E_ARG;
+    if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0)) return CFD_E_ARG;
+    if (params->precision != CFD_FP64) return CFD_E_ARG;  // the mixed path is a separate build stage
+    *out = nullptr;
+    Case *c = new Case();
+    c->n0 = n0; c->n1 = n1; c->N = (size_t)n0 * n1; c->h1 = dxi; c->h2 = deta; c->prm = *params;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
+    const size_t N = c->N;
+    int rc = CFD_OK;
+    double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
+                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->U, &c->V, &c->Ru,
+                         &c->Rv, &c->RT, &c->d, &c->us, &c->vs, &c->uh, &c->vh, &c->q, &c->pc};
+    for (double **f : fields)
+        if (rc == CFD_OK) rc = dalloc(c, f, N);
+    for (int k = 0; k < 6 && rc == CFD_OK; k++) rc = dalloc(c, &c->jb[k], N);
+    for (int k = 0; k < 8 && rc == CFD_OK; k++) rc = dalloc(c, &c->bi[k], N);
+    if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)2 * (n0 + n1) + 8);
+    c->pstride = (nblocks2d(n0, n1) + 255) / 256 * 256;
+    if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)4 * c->pstride);
+    if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)S_NSCAL + 8);
+    if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (S_NSCAL + 8) * sizeof(double)) != hipSuccess)
+        rc = CFD_E_OOM;
+    const size_t B = N * sizeof(double);
+    const double *src[] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac};
+    double *dst[] = {c->xg, c->yg, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->ajac};
+    for (int k = 0; k < 10 && rc == CFD_OK; k++)
+        if (hipMemcpyAsync(dst[k], src[k], B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc == CFD_OK && p1 && hipMemcpyAsync(c->p1, p1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc == CFD_OK && q1 && hipMemcpyAsync(c->q1, q1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc == CFD_OK) rc = build_levels(c);
+    if (rc == CFD_OK) {
+        c->wall_len_hot = c->wall_len_cold = 0.0;
+        for (int i = 0; i < n0 - 1; i++) {
+            const size_t a = (size_t)i, b = (size_t)i + 1;
+            const size_t a1 = (size_t)(n1 - 1) * n0 + i, b1 = a1 + 1;
+            c->wall_len_hot += std::hypot(x[b] - x[a], y[b] - y[a]);
+            c->wall_len_cold += std::hypot(x[b1] - x[a1], y[b1] - y[a1]);
+        }
+        rc = apply_bcs_dev(c);
+    }
+    if (rc == CFD_OK && hipStreamSynchronize(c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc != CFD_OK) {
+        cfd_case_destroy(reinterpret_cast<cfd_case>(c));
+        return rc;
+    }
+    *out = reinterpret_cast<cfd_case>(c);
+    return CFD_OK;
+}
+
+int cfd_case_destroy(cfd_case h) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_OK;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return CFD_OK;
+}
+
+int cfd_case_set_fields(cfd_case h, const double *u, const double *v, const double *T, const double *p) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !u || !v || !T || !p) return CFD_E_ARG;
+    const size_t B = c->N * sizeof(double);
+    CFD_HIP(hipMemcpyAsync(c->u, u, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->v, v, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->T, T, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->p, p, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    c->iter = 0;
+    c->mass0 = c->energy0 = 0.0;
+    return CFD_OK;
+}
+
+int cfd_case_get_fields(cfd_case h, double *u, double *v, double *T, double *p) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    const size_t B = c->N * sizeof(double);
+    if (u) CFD_HIP(hipMemcpyAsync(u, c->u, B, hipMemcpyDeviceToHost, c->s));
+    if (v) CFD_HIP(hipMemcpyAsync(v, c->v, B, hipMemcpyDeviceToHost, c->s));
+    if (T) CFD_HIP(hipMemcpyAsync(T, c->T, B, hipMemcpyDeviceToHost, c->s));
+    if (p) CFD_HIP(hipMemcpyAsync(p, c->p, B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_case_apply_bcs(cfd_case h) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    RET(apply_bcs_dev(c));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_case_outer(cfd_case h, int n, cfd_stats *st) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || n < 0) return CFD_E_ARG;
+    cfd_stats tmp{};
+    for (int k = 0; k < n; k++) RET(outer_once(c, st ? st : &tmp));
+    return CFD_OK;
+}
+
+int cfd_case_solve(cfd_case h, double tol, int max_outer, cfd_stats *st) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || max_outer < 1) return CFD_E_ARG;
+    cfd_stats s{};
+    for (int k = 0; k < max_outer; k++) {
+        RET(outer_once(c, &s));
+        if (s.mass_res <= tol && s.energy_res <= tol) {
+            if (st) *st = s;
+            return CFD_OK;
+        }
+    }
+    if (st) *st = s;
+    return CFD_E_NOTCONV;
+}
+
+int cfd_case_bytes(cfd_case h, size_t *bytes) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !bytes) return CFD_E_ARG;
+    *bytes = c->bytes;
+    return CFD_OK;
+}
+
+int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    RET(launch_residual(c));
+    const size_t B = c->N * sizeof(double);
+    if (Ru) CFD_HIP(hipMemcpyAsync(Ru, c->Ru, B, hipMemcpyDeviceToHost, c->s));
+    if (Rv) CFD_HIP(hipMemcpyAsync(Rv, c->Rv, B, hipMemcpyDeviceToHost, c->s));
+    if (RT) CFD_HIP(hipMemcpyAsync(RT, c->RT, B, hipMemcpyDeviceToHost, c->s));
+    if (d) CFD_HIP(hipMemcpyAsync(d, c->d, B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_case_p_apply(cfd_case h, const double *d, const double *x, double *y) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !d || !x || !y) return CFD_E_ARG;
+    const size_t B = c->N * sizeof(double);
+    CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->bi[6], x, B, hipMemcpyHostToDevice, c->s));
+    k_p_apply<double><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(largs(c->lev[0]), c->bi[6], c->bi[7]);
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipMemcpyAsync(y, c->bi[7], B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, double rtol, int maxit,
+                     int *iters, double *relres) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !d || !q || !x) return CFD_E_ARG;
+    const size_t B = c->N * sizeof(double);
+    CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->q, q, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemsetAsync(c->pc, 0, B, c->s));
+    RET(mg_setup(c));
+    int its = 0;
+    double rr = 0.0;
+    RET(bicgstab(c, c->q, c->pc, rtol, maxit, &its, &rr));
+    CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    if (iters) *iters = its;
+    if (relres) *relres = rr;
+    return CFD_OK;
+}
+
+int cfd_case_profile(cfd_case h, int enable) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    CFD_HIP(hipStreamSynchronize(c->s));
+    if (enable && c->ev.empty()) {
+        c->ev.resize(PROF_EVENTS);
+        for (auto &e : c->ev) CFD_HIP(hipEventCreate(&e));
+    }
+    c->prof = enable != 0;
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    for (int k = 0; k < CFD_PROF_KINDS; k++) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    return CFD_OK;
+}
+
+int cfd_case_profile_read(cfd_case h, double *ms_total, long *launches, int nkinds) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !ms_total || !launches || nkinds < 1) return CFD_E_ARG;
+    CFD_HIP(hipStreamSynchronize(c->s));
+    prof_drain(c);
+    for (int k = 0; k < nkinds; k++) {
+        ms_total[k] = k < CFD_PROF_KINDS ? c->prof_ms[k] : 0.0;
+        launches[k] = k < CFD_PROF_KINDS ? c->prof_n[k] : 0;
+    }
+    return CFD_OK;
+}
+
+int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || reps < 1 || !ms_per_launch) return CFD_E_ARG;
+    const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
+    hipEvent_t e0, e1;
+    CFD_HIP(hipEventCreate(&e0));
+    CFD_HIP(hipEventCreate(&e1));
+    CFD_HIP(hipEventRecord(e0, c->s));
+    for (int k = 0; k < reps; k++) {
+        switch (kernel) {
+        case 0:
+            k_p_apply<double><<<gr, bl, 0, c->s>>>(largs(c->lev[0]), c->bi[6], c->bi[7]);
+            break;
+        case 1:
+            k_jacobi<false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
+                                                 c->gamma, c->p1, c->q1, c->jb[0], c->jb[1], c->jb[2], c->jb[3],
+                                                 c->jb[4], c->jb[5]);
+            break;
+        case 2:
+            k_residual<<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety,
+                                            c->alph, c->beta, c->gamma, c->p1, c->q1, c->Ru, c->Rv, c->RT, c->d,
+                                            c->U, c->V, c->part, c->pstride);
+            break;
+        case 3:
+            k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
+                                            c->ajac, c->q, c->part);
+            break;
+        case 4:
+            k_p_smooth<double, false><<<gr, bl, 0, c->s>>>(largs(c->lev[0]), 0.8, c->lev[0].x, c->lev[0].b,
+                                                           c->lev[0].invd, c->lev[0].t);
+            break;
+        default:
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return CFD_E_ARG;
+        }
+    }
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipEventRecord(e1, c->s));
+    CFD_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CFD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = (double)ms / reps;
+    return CFD_OK;
+}
+
+}  // extern "C"

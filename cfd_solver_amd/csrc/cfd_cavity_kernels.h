@@ -1,0 +1,154 @@
+// Device-side stencil formulas of the cavity SIMPLE path (gfx950).
+//
+// Every formula here is the one the CPU oracle uses, node for node:
+//  * conv_deriv   -- hybrid 4th-order central / Kawamura-Kuwahara upwind switch
+//                    on the cell Peclet number (solver.cpp:1120-1173);
+//  * aref_offdiag -- A_ref phi - phi of the implicit diffusion operator
+//                    I - dt/Re L (coefficients of solver.cpp:764-847);
+//  * alo_coefs    -- low-order predictor operator (the solver.cpp:764-786
+//                    second-order row + first-order upwind convection);
+//  * p9_coefs     -- the J-weighted flux form of the pressure-correction
+//                    operator built from the face velocity of solver.cpp:1402-1436.
+#pragma once
+
+#include "cfd_common.h"
+
+namespace cfd {
+
+struct KGrid {
+    int n0, n1;
+    double h1, h2;
+};
+
+__device__ __forceinline__ size_t idx(int i, int j, int n0) { return (size_t)j * (size_t)n0 + (size_t)i; }
+
+// mode 2: hybrid, 1: 4th-order central, 0: 2nd-order central (solver.cpp:1124-1172)
+__device__ __forceinline__ double conv_deriv(int mode, double fm2, double fm1, double f0, double fp1,
+                                             double fp2, double vel, double pec, double h) {
+    if (mode == 0) return 0.5 * (fp1 - fm1) / h;
+    if (mode == 1 || (pec <= 2 && pec > -2)) {
+        const double xpp = 8.0 * (fp1 - fm1);
+        const double xnn = fp2 - fm2;
+        return (1.0 / 12.0) * (xpp - xnn) / h;
+    }
+    double ak1 = vel * (-fp2 + 8 * fp1 - 8 * fm1 + fm2) / (12.0 * h);
+    const double ak2 = fabs(vel) * (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) / (4.0 * h);
+    ak1 = ak1 + ak2;
+    return ak1 / vel;
+}
+
+// (A_ref - I) phi at interior node (i, j); nd = Re (u, v) or Re*Pr (T)
+__device__ __forceinline__ double aref_offdiag(const double *__restrict__ f, int n0, int i, int j,
+                                               double al, double be, double ga, double pp, double qq,
+                                               double h1, double h2, double dt, double nd, bool wide) {
+#define F(di, dj) f[idx(i + (di), j + (dj), n0)]
+    if (!wide) {
+        const double ce = -dt * (al / (h1 * h1) + pp / (2.0 * h1)) / nd;
+        const double cw = -dt * (al / (h1 * h1) - pp / (2.0 * h1)) / nd;
+        const double cn = -dt * (ga / (h2 * h2) + qq / (2.0 * h2)) / nd;
+        const double cs = -dt * (ga / (h2 * h2) - qq / (2.0 * h2)) / nd;
+        const double cx = dt * be / (2.0 * h1 * h2 * nd);
+        const double cp = dt * 2.0 * (al / (h1 * h1) + ga / (h2 * h2)) / nd;
+        return cp * F(0, 0) + ce * F(1, 0) + cw * F(-1, 0) + cn * F(0, 1) + cs * F(0, -1) +
+               cx * (F(1, 1) + F(-1, -1) - F(-1, 1) - F(1, -1));
+    }
+    const double ce = (-dt) * ((4.0 * al) / (3.0 * h1 * h1) + (2.0 * pp) / (3.0 * h1)) / nd;
+    const double cw = (-dt) * ((4.0 * al) / (3.0 * h1 * h1) - (2.0 * pp) / (3.0 * h1)) / nd;
+    const double cn = (-dt) * ((4.0 * ga) / (3.0 * h2 * h2) + (2.0 * qq) / (3.0 * h2)) / nd;
+    const double cs = (-dt) * ((4.0 * ga) / (3.0 * h2 * h2) - (2.0 * qq) / (3.0 * h2)) / nd;
+    const double cne = (-dt) * (-8.0 * be / (9.0 * h1 * h2)) / nd;
+    const double cnw = (-dt) * (8.0 * be / (9.0 * h1 * h2)) / nd;
+    const double cnn = (-dt) * (-ga / (12.0 * h2 * h2) - qq / (12.0 * h2)) / nd;
+    const double css = (-dt) * (-ga / (12.0 * h2 * h2) + qq / (12.0 * h2)) / nd;
+    const double cee = (-dt) * (-al / (12.0 * h1 * h1) - pp / (12.0 * h1)) / nd;
+    const double cww = (-dt) * (-al / (12.0 * h1 * h1) + pp / (12.0 * h1)) / nd;
+    const double c22 = (-dt) * (-be / (72.0 * h1 * h2)) / nd;
+    const double c2m = (-dt) * (be / (72.0 * h1 * h2)) / nd;
+    const double c12 = (-dt) * (be / (9.0 * h1 * h2)) / nd;
+    const double c1m = (-dt) * (-be / (9.0 * h1 * h2)) / nd;
+    const double cp = dt * (5.0 * al / (2.0 * h1 * h1) + 5.0 * ga / (2.0 * h2 * h2)) / nd;
+    return cp * F(0, 0) + ce * F(1, 0) + cw * F(-1, 0) + cn * F(0, 1) + cs * F(0, -1) +
+           cne * (F(1, 1) + F(-1, -1)) + cnw * (F(-1, 1) + F(1, -1)) +
+           cnn * F(0, 2) + css * F(0, -2) + cee * F(2, 0) + cww * F(-2, 0) +
+           c22 * (F(2, 2) + F(-2, -2)) + c2m * (F(-2, 2) + F(2, -2)) +
+           c12 * (F(1, 2) + F(-1, -2) + F(2, 1) + F(-2, -1)) +
+           c1m * (F(-1, 2) + F(1, -2) + F(-2, 1) + F(2, -1));
+#undef F
+}
+
+// Low-order predictor operator row: c[(dj+1)*3+(di+1)], diagonal c[4]
+template <typename R>
+__device__ __forceinline__ void alo_coefs(R al, R be, R ga, R pp, R qq, R U, R V, R h1, R h2, R dt,
+                                          R nd, R c[9]) {
+    const R zero = R(0);
+    c[5] = -dt * (al / (h1 * h1) + pp / (R(2) * h1)) / nd + dt * fmin(U, zero) / h1;
+    c[3] = -dt * (al / (h1 * h1) - pp / (R(2) * h1)) / nd - dt * fmax(U, zero) / h1;
+    c[7] = -dt * (ga / (h2 * h2) + qq / (R(2) * h2)) / nd + dt * fmin(V, zero) / h2;
+    c[1] = -dt * (ga / (h2 * h2) - qq / (R(2) * h2)) / nd - dt * fmax(V, zero) / h2;
+    const R cx = dt * be / (R(2) * h1 * h2 * nd);
+    c[8] = cx;
+    c[0] = cx;
+    c[6] = -cx;
+    c[2] = -cx;
+    c[4] = R(1) + dt * R(2) * (al / (h1 * h1) + ga / (h2 * h2)) / nd + dt * (fabs(U) / h1 + fabs(V) / h2);
+}
+
+// J-weighted flux-form pressure-correction row (see the oracle's p9_coefs).
+// Faces touching a wall node carry no flux; neighbours are read clamped into
+// the interior by the caller.
+template <typename T>
+__device__ __forceinline__ void p9_coefs(int n0, int n1, const T *__restrict__ xix,
+                                         const T *__restrict__ xiy, const T *__restrict__ etx,
+                                         const T *__restrict__ ety, const T *__restrict__ jac,
+                                         const T *__restrict__ d, T h1, T h2, int i, int j, T c[9]) {
+    const size_t P = idx(i, j, n0), E = idx(i + 1, j, n0), W = idx(i - 1, j, n0);
+    const size_t Nn = idx(i, j + 1, n0), S = idx(i, j - 1, n0);
+    const bool we = (i + 1 == n0 - 1), ww = (i - 1 == 0), wn = (j + 1 == n1 - 1), ws = (j - 1 == 0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = T(0);
+    const T xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P], dP = d[P];
+    const T jxP = jP * xP, jyP = jP * yP, kxP = jP * eP, kyP = jP * fP;
+    const T half = T(0.5), four = T(4);
+    if (!we) {
+        const T xE = xix[E], yE = xiy[E], eE = etx[E], fE = ety[E], jE = jac[E];
+        const T mx = half * (jxP + jE * xE), my = half * (jyP + jE * yE);
+        const T a = mx * (xP + xE) + my * (yP + yE);
+        const T b = mx * (eP + eE) + my * (fP + fE);
+        const T de = half * (dP + d[E]);
+        const T kn = half * de * a / (h1 * h1), kt = half * de * b / (four * h1 * h2);
+        c[5] += kn; c[4] -= kn;
+        c[8] += kt; c[7] += kt; c[1] -= kt; c[2] -= kt;
+    }
+    if (!ww) {
+        const T xW = xix[W], yW = xiy[W], eW = etx[W], fW = ety[W], jW = jac[W];
+        const T mx = half * (jxP + jW * xW), my = half * (jyP + jW * yW);
+        const T a = mx * (xP + xW) + my * (yP + yW);
+        const T b = mx * (eP + eW) + my * (fP + fW);
+        const T dw = half * (dP + d[W]);
+        const T kn = half * dw * a / (h1 * h1), kt = half * dw * b / (four * h1 * h2);
+        c[4] -= kn; c[3] += kn;
+        c[7] -= kt; c[6] -= kt; c[1] += kt; c[0] += kt;
+    }
+    if (!wn) {
+        const T xN = xix[Nn], yN = xiy[Nn], eN = etx[Nn], fN = ety[Nn], jN = jac[Nn];
+        const T mx = half * (kxP + jN * eN), my = half * (kyP + jN * fN);
+        const T a = mx * (eP + eN) + my * (fP + fN);
+        const T b = mx * (xP + xN) + my * (yP + yN);
+        const T dn = half * (dP + d[Nn]);
+        const T kn = half * dn * a / (h2 * h2), kt = half * dn * b / (four * h1 * h2);
+        c[7] += kn; c[4] -= kn;
+        c[8] += kt; c[6] -= kt; c[5] += kt; c[3] -= kt;
+    }
+    if (!ws) {
+        const T xS = xix[S], yS = xiy[S], eS = etx[S], fS = ety[S], jS = jac[S];
+        const T mx = half * (kxP + jS * eS), my = half * (kyP + jS * fS);
+        const T a = mx * (eP + eS) + my * (fP + fS);
+        const T b = mx * (xP + xS) + my * (yP + yS);
+        const T ds = half * (dP + d[S]);
+        const T kn = half * ds * a / (h2 * h2), kt = half * ds * b / (four * h1 * h2);
+        c[4] -= kn; c[1] += kn;
+        c[5] -= kt; c[3] += kt; c[2] -= kt; c[0] += kt;
+    }
+}
+
+}  // namespace cfd

@@ -1,0 +1,89 @@
+// Shared device helpers for the gfx950 kernels of the SIMPLE / pressure-correction path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+
+#include "../../include/cfd_gpu.h"
+
+#define CFD_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "cfd_gpu: %s failed at %s:%d: %s\n", #call, __FILE__, __LINE__, \
+                    hipGetErrorString(e_));                                                \
+            return CFD_E_HIP;                                                              \
+        }                                                                                  \
+    } while (0)
+
+#define CFD_LAUNCH_CHECK()                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "cfd_gpu: launch failed at %s:%d: %s\n", __FILE__, __LINE__,   \
+                    hipGetErrorString(e_));                                                \
+            return CFD_E_HIP;                                                              \
+        }                                                                                  \
+    } while (0)
+
+#define CFD_RET(x)                   \
+    do {                             \
+        int r_ = (x);                \
+        if (r_ != CFD_OK) return r_; \
+    } while (0)
+
+namespace cfd {
+
+// 2-D launch shape: 64 lanes along xi (one wavefront per row segment), 4 rows per block.
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+inline dim3 grid2d(int n0, int n1) { return dim3((n0 + BX - 1) / BX, (n1 + BY - 1) / BY); }
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Wavefront (64 lanes) sum of a double.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+
+// Block reduction (sum) for up to 1024 threads; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double s_part[NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    v = wave_sum(v);
+    if ((lane & 63) == 0) s_part[lane >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (lane < 64) {
+        r = (lane < NT / 64) ? s_part[lane] : 0.0;
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double s_part[NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    v = wave_max(v);
+    if ((lane & 63) == 0) s_part[lane >> 6] = v;
+    __syncthreads();
+    double r = -1.0e300;
+    if (lane < 64) {
+        r = (lane < NT / 64) ? s_part[lane] : -1.0e300;
+        r = wave_max(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+}  // namespace cfd

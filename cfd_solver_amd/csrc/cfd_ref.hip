@@ -1,0 +1,806 @@
+// Reference-stage kernels (periodic O-grid) behind the cfd_ref_* / cfd_sip9p /
+// cfd_gauss C ABI.  Each entry point takes host arrays in the reference's
+// layout (converted to j*n0 + i), runs one gfx950 kernel (or a GPU BiCGStab
+// for the two solvers) and returns host arrays.  Column n0-1 duplicates
+// column 0; neighbour columns wrap with period n0-1 exactly as the reference's
+// inn/ipp/inn2/ipp2 maps do (solver.cpp:1076-1102), except that the literal
+// column n0-1 is addressed where the reference addresses it.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cfd_common.h"
+
+namespace cfd {
+namespace ref {
+
+__device__ __forceinline__ size_t ix(int i, int j, int n0) { return (size_t)j * n0 + i; }
+// neighbour column i+di of column i in [0, n0-2]
+__device__ __forceinline__ int pw(int k, int n0) { return k < 0 ? k + n0 - 1 : (k > n0 - 1 ? k - (n0 - 1) : k); }
+
+constexpr int K25(int di, int dj) { return (dj + 2) * 5 + (di + 2); }
+constexpr int K9(int di, int dj) { return (dj + 1) * 3 + (di + 1); }
+
+// ---------------------------------------------------------------- a2
+__global__ void k_asm_mom(int n0, int n1, double h1, double h2, const double *__restrict__ alph,
+                          const double *__restrict__ beta, const double *__restrict__ gamma,
+                          const double *__restrict__ p1, const double *__restrict__ q1, double dt, double Re,
+                          double Pr, double *__restrict__ AU, double *__restrict__ AT, double *__restrict__ BU,
+                          double *__restrict__ BT) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const size_t N = (size_t)n0 * n1, id = ix(i, j, n0);
+    const double al = alph[id], be = beta[id], ga = gamma[id], pp = p1[id], qq = q1[id];
+    double u[25], t[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) u[k] = t[k] = 0.0;
+    if (j == 1 || j == n1 - 2) {
+        u[K25(1, 0)] = -dt * (al / (h1 * h1) + pp / (2.0 * h1)) / Re;
+        u[K25(-1, 0)] = -dt * (al / (h1 * h1) - pp / (2.0 * h1)) / Re;
+        u[K25(0, 1)] = -dt * (ga / (h2 * h2) + qq / (2.0 * h2)) / Re;
+        u[K25(0, -1)] = -dt * (ga / (h2 * h2) - qq / (2.0 * h2)) / Re;
+        u[K25(1, 1)] = dt * be / (2.0 * h1 * h2 * Re);
+        u[K25(-1, -1)] = u[K25(1, 1)];
+        u[K25(-1, 1)] = -dt * be / (2.0 * h1 * h2 * Re);
+        u[K25(1, -1)] = u[K25(-1, 1)];
+        u[K25(0, 0)] = 1 + dt * 2.0 * (al / (h1 * h1) + ga / (h2 * h2)) / Re;
+        t[K25(1, 0)] = -dt * (al / (h1 * h1) + pp / (2.0 * h1)) / (Re * Pr);
+        t[K25(-1, 0)] = -dt * (al / (h1 * h1) - pp / (2.0 * h1)) / (Re * Pr);
+        t[K25(0, 1)] = -dt * (ga / (h2 * h2) + qq / (2.0 * h2)) / (Re * Pr);
+        t[K25(0, -1)] = -dt * (ga / (h2 * h2) - qq / (2.0 * h2)) / (Re * Pr);
+        t[K25(1, 1)] = dt * (be / (2.0 * h1 * h2)) / (Re * Pr);
+        t[K25(-1, -1)] = t[K25(1, 1)];
+        t[K25(-1, 1)] = -dt * (be / (2.0 * h1 * h2)) / (Re * Pr);
+        t[K25(1, -1)] = t[K25(-1, 1)];
+        t[K25(0, 0)] = 1 + dt * 2.0 * (al / (h1 * h1) + ga / (h2 * h2)) / (Re * Pr);
+    } else {
+        u[K25(1, 0)] = (-dt) * ((4.0 * al) / (3.0 * h1 * h1) + (2.0 * pp) / (3.0 * h1)) / Re;
+        u[K25(-1, 0)] = (-dt) * ((4.0 * al) / (3.0 * h1 * h1) - (2.0 * pp) / (3.0 * h1)) / Re;
+        u[K25(0, 1)] = (-dt) * ((4.0 * ga) / (3.0 * h2 * h2) + (2.0 * qq) / (3.0 * h2)) / Re;
+        u[K25(0, -1)] = (-dt) * ((4.0 * ga) / (3.0 * h2 * h2) - (2.0 * qq) / (3.0 * h2)) / Re;
+        u[K25(1, 1)] = (-dt) * (-8.0 * be / (9.0 * h1 * h2)) / Re;
+        u[K25(-1, 1)] = (-dt) * (8.0 * be / (9.0 * h1 * h2)) / Re;
+        u[K25(1, -1)] = u[K25(-1, 1)];
+        u[K25(-1, -1)] = u[K25(1, 1)];
+        u[K25(0, 2)] = (-dt) * (-ga / (12.0 * h2 * h2) - qq / (12.0 * h2)) / Re;
+        u[K25(0, -2)] = (-dt) * (-ga / (12.0 * h2 * h2) + qq / (12.0 * h2)) / Re;
+        u[K25(2, 0)] = (-dt) * (-al / (12.0 * h1 * h1) - pp / (12.0 * h1)) / Re;
+        u[K25(-2, 0)] = (-dt) * (-al / (12.0 * h1 * h1) + pp / (12.0 * h1)) / Re;
+        u[K25(2, 2)] = (-dt) * (-be / (72.0 * h1 * h2)) / Re;
+        u[K25(-2, 2)] = (-dt) * (be / (72.0 * h1 * h2)) / Re;
+        u[K25(2, -2)] = u[K25(-2, 2)];
+        u[K25(-2, -2)] = u[K25(2, 2)];
+        u[K25(1, 2)] = (-dt) * (be / (9.0 * h1 * h2)) / Re;
+        u[K25(-1, 2)] = (-dt) * (-be / (9.0 * h1 * h2)) / Re;
+        u[K25(1, -2)] = u[K25(-1, 2)];
+        u[K25(-1, -2)] = u[K25(1, 2)];
+        u[K25(2, 1)] = u[K25(1, 2)];
+        u[K25(-2, 1)] = u[K25(-1, 2)];
+        u[K25(2, -1)] = u[K25(-1, 2)];
+        u[K25(-2, -1)] = u[K25(1, 2)];
+        u[K25(0, 0)] = 1 + dt * (5.0 * al / (2.0 * h1 * h1) + 5.0 * ga / (2.0 * h2 * h2)) / Re;
+#pragma unroll
+        for (int k = 0; k < 25; k++) t[k] = u[k] / Pr;
+        t[K25(0, 0)] = 1 + dt * (5.0 * al / (2.0 * h1 * h1) + 5.0 * ga / (2.0 * h2 * h2)) / (Re * Pr);
+    }
+    if (j == 1) {
+        BU[0 * n0 + i] = u[K25(0, -1)]; BU[1 * n0 + i] = u[K25(1, -1)]; BU[2 * n0 + i] = u[K25(-1, -1)];
+        BT[0 * n0 + i] = t[K25(0, -1)]; BT[1 * n0 + i] = t[K25(1, -1)]; BT[2 * n0 + i] = t[K25(-1, -1)];
+        u[K25(0, -1)] = u[K25(1, -1)] = u[K25(-1, -1)] = 0;
+        t[K25(0, -1)] = t[K25(1, -1)] = t[K25(-1, -1)] = 0;
+    }
+    if (j == n1 - 2) {
+        BU[3 * n0 + i] = u[K25(0, 1)]; BU[4 * n0 + i] = u[K25(1, 1)]; BU[5 * n0 + i] = u[K25(-1, 1)];
+        BT[3 * n0 + i] = t[K25(0, 1)]; BT[4 * n0 + i] = t[K25(1, 1)]; BT[5 * n0 + i] = t[K25(-1, 1)];
+        u[K25(0, 1)] = u[K25(1, 1)] = u[K25(-1, 1)] = 0;
+        t[K25(0, 1)] = t[K25(1, 1)] = t[K25(-1, 1)] = 0;
+    }
+    const size_t dup = ix(n0 - 1, j, n0);
+#pragma unroll
+    for (int k = 0; k < 25; k++) {
+        AU[(size_t)k * N + id] = u[k];
+        AT[(size_t)k * N + id] = t[k];
+        if (i == 0) {
+            AU[(size_t)k * N + dup] = u[k];
+            AT[(size_t)k * N + dup] = t[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- a3
+__global__ void k_asm_p9(int n0, int n1, double h1, double h2, const double *__restrict__ xix,
+                         const double *__restrict__ xiy, const double *__restrict__ etx,
+                         const double *__restrict__ ety, double *__restrict__ A9) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const size_t N = (size_t)n0 * n1;
+    const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+    const size_t P = ix(i, j, n0), E = ix(ipp, j, n0), W = ix(inn, j, n0), Nn = ix(i, j + 1, n0), S = ix(i, j - 1, n0);
+    const double h11 = 2.0 * h1 * h1, h22 = 2.0 * h2 * h2, h12 = 8.0 * h1 * h2;
+    double c[9];
+    c[K9(1, 0)] = (xix[P] / h11) * (xix[P] + xix[E]) + (etx[P] / h12) * (xix[Nn] - xix[S]) +
+                  (xiy[P] / h11) * (xiy[P] + xiy[E]) + (ety[P] / h12) * (xiy[Nn] - xiy[S]);
+    c[K9(-1, 0)] = (xix[P] / h11) * (xix[P] + xix[W]) + (etx[P] / h12) * (xix[S] - xix[Nn]) +
+                   (xiy[P] / h11) * (xiy[P] + xiy[W]) + (ety[P] / h12) * (xiy[S] - xiy[Nn]);
+    c[K9(0, 1)] = (xix[P] / h12) * (etx[E] - etx[W]) + (etx[P] / h22) * (etx[P] + etx[Nn]) +
+                  (xiy[P] / h12) * (ety[E] - ety[W]) + (ety[P] / h22) * (ety[P] + ety[Nn]);
+    c[K9(0, -1)] = (xix[P] / h12) * (etx[W] - etx[E]) + (etx[P] / h22) * (etx[P] + etx[S]) +
+                   (xiy[P] / h12) * (ety[W] - ety[E]) + (ety[P] / h22) * (ety[P] + ety[S]);
+    c[K9(1, 1)] = (xix[P] / h12) * (etx[P] + etx[E]) + (etx[P] / h12) * (xix[P] + xix[Nn]) +
+                  (xiy[P] / h12) * (ety[P] + ety[E]) + (ety[P] / h12) * (xiy[P] + xiy[Nn]);
+    c[K9(-1, -1)] = (xix[P] / h12) * (etx[P] + etx[W]) + (etx[P] / h12) * (xix[P] + xix[S]) +
+                    (xiy[P] / h12) * (ety[P] + ety[W]) + (ety[P] / h12) * (xiy[P] + xiy[S]);
+    c[K9(-1, 1)] = -(xix[P] / h12) * (etx[P] + etx[W]) + -(etx[P] / h12) * (xix[P] + xix[Nn]) +
+                   -(xiy[P] / h12) * (ety[P] + ety[W]) + -(ety[P] / h12) * (xiy[P] + xiy[Nn]);
+    c[K9(1, -1)] = -(xix[P] / h12) * (etx[P] + etx[E]) + -(etx[P] / h12) * (xix[P] + xix[S]) +
+                   -(xiy[P] / h12) * (ety[P] + ety[E]) + -(ety[P] / h12) * (xiy[P] + xiy[S]);
+    {
+        const double pxi = 1.0 / (2. * h1 * h1), pet = 1.0 / (2. * h2 * h2);
+        const double ap = -xix[P] * (2. * xix[P] + xix[W] + xix[E]);
+        const double bp = -etx[P] * (2. * etx[P] + etx[S] + etx[Nn]);
+        const double cp = -xiy[P] * (2. * xiy[P] + xiy[W] + xiy[E]);
+        const double dp = -ety[P] * (2. * ety[P] + ety[S] + ety[Nn]);
+        c[K9(0, 0)] = ap * pxi + bp * pet + cp * pxi + dp * pet;
+    }
+    const size_t dup = ix(n0 - 1, j, n0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        A9[(size_t)k * N + P] = c[k];
+        if (i == 0) A9[(size_t)k * N + dup] = c[k];
+    }
+}
+
+// ---------------------------------------------------------------- a4 + a5
+__global__ void k_contra(int n0, int n1, const double *__restrict__ u, const double *__restrict__ v,
+                         const double *__restrict__ xix, const double *__restrict__ xiy,
+                         const double *__restrict__ etx, const double *__restrict__ ety, double *__restrict__ uxi,
+                         double *__restrict__ uet) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t id = ix(i, j, n0);
+    uxi[id] = xix[id] * u[id] + xiy[id] * v[id];
+    uet[id] = etx[id] * u[id] + ety[id] * v[id];
+}
+
+__device__ __forceinline__ double hyb(double fm2, double fm1, double f0, double fp1, double fp2, double vel,
+                                      double pec, double h) {
+    if (pec <= 2 && pec > -2) {
+        const double xpp = 8.0 * (fp1 - fm1);
+        const double xnn = fp2 - fm2;
+        return (1.0 / 12.0) * (xpp - xnn) / h;
+    }
+    double ak1 = vel * (-fp2 + 8 * fp1 - 8 * fm1 + fm2) / (12.0 * h);
+    const double ak2 = fabs(vel) * (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) / (4.0 * h);
+    ak1 = ak1 + ak2;
+    return ak1 / vel;
+}
+
+struct RhsIn {
+    const double *u, *v, *T, *up, *vp, *p, *xix, *xiy, *etx, *ety, *alph, *gamma, *BU, *BT, *uxi, *uet;
+};
+struct RhsOut {
+    double *qu, *qv, *qt, *qup, *qvp;
+};
+
+__global__ void k_rhs(int n0, int n1, double h1, double h2, RhsIn a, RhsOut o, double Re, double Pr, double Ri,
+                      double dt) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const int inn = pw(i - 1, n0), ipp = i + 1, inn2 = pw(i - 2, n0), ipp2 = pw(i + 2, n0);
+    const size_t P = ix(i, j, n0);
+    const bool interior = (j >= 2 && j <= n1 - 3);
+    const double *fl[3] = {a.u, a.v, a.T};
+    const double U = a.uxi[P], V = a.uet[P];
+    double conv[3];
+    for (int k = 0; k < 3; k++) {
+        const double *f = fl[k];
+        double pec1, pec2;
+        if (k <= 1) {
+            pec1 = U * Re * h1 / a.alph[P];
+            pec2 = V * Re * h2 / a.gamma[P];
+        } else {
+            pec1 = U * Re * Pr * h1 / a.alph[P];
+            pec2 = V * Re * Pr * h2 / a.gamma[P];
+        }
+        double dxi, det;
+        if (interior) {
+            dxi = hyb(f[ix(inn2, j, n0)], f[ix(inn, j, n0)], f[P], f[ix(ipp, j, n0)], f[ix(ipp2, j, n0)], U, pec1, h1);
+            det = hyb(f[ix(i, j - 2, n0)], f[ix(i, j - 1, n0)], f[P], f[ix(i, j + 1, n0)], f[ix(i, j + 2, n0)], V, pec2,
+                      h2);
+        } else {
+            const double xpp = 8.0 * (f[ix(ipp, j, n0)] - f[ix(inn, j, n0)]);
+            const double xnn = f[ix(ipp2, j, n0)] - f[ix(inn2, j, n0)];
+            dxi = (1.0 / 12.0) * (xpp - xnn) / h1;
+            det = 0.5 * (f[ix(i, j + 1, n0)] - f[ix(i, j - 1, n0)]) / h2;
+        }
+        conv[k] = U * dxi + V * det;
+    }
+    const double *p = a.p;
+    const double dp_dxi = (p[ix(ipp, j, n0)] - p[ix(inn, j, n0)]) / (2.0 * h1);
+    const double dp_de = (p[ix(i, j + 1, n0)] - p[ix(i, j - 1, n0)]) / (2.0 * h2);
+    const double dp_dx = a.xix[P] * dp_dxi + a.etx[P] * dp_de;
+    const double dp_dy = a.xiy[P] * dp_dxi + a.ety[P] * dp_de;
+    double qu = dt * (-conv[0] - dp_dx) + a.u[P];
+    double qv = dt * (-conv[1] - dp_dy + Ri * a.T[P]) + a.v[P];
+    double qt = -dt * conv[2] + a.T[P];
+    double qup = qu + dt * dp_dx;
+    double qvp = qv + dt * dp_dy;
+    if (j == 1) {
+        const size_t s = ix(i, j - 1, n0), se = ix(ipp, j - 1, n0), sw = ix(inn, j - 1, n0);
+        const double bs = a.BU[i], bse = a.BU[n0 + i], bsw = a.BU[2 * n0 + i];
+        const double ts = a.BT[i], tse = a.BT[n0 + i], tsw = a.BT[2 * n0 + i];
+        qu = qu - (bs * a.u[s] + bse * a.u[se] + bsw * a.u[sw]);
+        qv = qv - (bs * a.v[s] + bse * a.v[se] + bsw * a.v[sw]);
+        qt = qt - (ts * a.T[s] + tse * a.T[se] + tsw * a.T[sw]);
+        qup = qup - (bs * a.up[s] + bse * a.up[se] + bsw * a.up[sw]);
+        qvp = qvp - (bs * a.vp[s] + bse * a.vp[se] + bsw * a.vp[sw]);
+    }
+    if (j == n1 - 2) {
+        const size_t nn = ix(i, j + 1, n0), ne = ix(ipp, j + 1, n0), nw = ix(inn, j + 1, n0);
+        const double bn = a.BU[3 * n0 + i], bne = a.BU[4 * n0 + i], bnw = a.BU[5 * n0 + i];
+        const double tn = a.BT[3 * n0 + i], tne = a.BT[4 * n0 + i], tnw = a.BT[5 * n0 + i];
+        qu = qu - (bn * a.u[nn] + bne * a.u[ne] + bnw * a.u[nw]);
+        qv = qv - (bn * a.v[nn] + bne * a.v[ne] + bnw * a.v[nw]);
+        qt = qt - (tn * a.T[nn] + tne * a.T[ne] + tnw * a.T[nw]);
+        qup = qup - (bn * a.up[nn] + bne * a.up[ne] + bnw * a.up[nw]);
+        qvp = qvp - (bn * a.vp[nn] + bne * a.vp[ne] + bnw * a.vp[nw]);
+    }
+    o.qu[P] = qu; o.qv[P] = qv; o.qt[P] = qt; o.qup[P] = qup; o.qvp[P] = qvp;
+    if (i == 0) {
+        const size_t dup = ix(n0 - 1, j, n0);
+        o.qu[dup] = qu; o.qv[dup] = qv; o.qt[dup] = qt; o.qup[dup] = qup; o.qvp[dup] = qvp;
+    }
+}
+
+// ---------------------------------------------------------------- a7
+__global__ void k_face_div_ref(int n0, int n1, double h1, double h2, const double *__restrict__ up,
+                               const double *__restrict__ vp, const double *__restrict__ p,
+                               const double *__restrict__ xix, const double *__restrict__ xiy,
+                               const double *__restrict__ etx, const double *__restrict__ ety, double dt,
+                               double *__restrict__ q) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1, jpp = j + 1, jnn = j - 1;
+    const size_t P = ix(i, j, n0), E = ix(ipp, j, n0), W = ix(inn, j, n0), Nn = ix(i, jpp, n0), S = ix(i, jnn, n0);
+    const double dpdxi_ip = (p[E] - p[P]) / h1;
+    const double dpde_ip = (p[ix(ipp, jpp, n0)] + p[Nn] - p[S] - p[ix(ipp, jnn, n0)]) / (4.0 * h2);
+    const double dpdxi_in = (p[P] - p[W]) / h1;
+    const double dpde_in = (p[Nn] + p[ix(inn, jpp, n0)] - p[S] - p[ix(inn, jnn, n0)]) / (4.0 * h2);
+    const double dpdxi_jp = (p[ix(ipp, jpp, n0)] - p[ix(inn, jpp, n0)] + p[E] - p[W]) / (4.0 * h1);
+    const double dpde_jp = (p[Nn] - p[P]) / h2;
+    const double dpdxi_jn = (p[E] - p[W] + p[ix(ipp, jnn, n0)] - p[ix(inn, jnn, n0)]) / (4.0 * h1);
+    const double dpde_jn = (p[P] - p[S]) / h2;
+    const double us_ip = 0.5 * (up[P] + up[E]) - 0.5 * dt * ((xix[P] + xix[E]) * dpdxi_ip + (etx[P] + etx[E]) * dpde_ip);
+    const double us_in = 0.5 * (up[P] + up[W]) - 0.5 * dt * ((xix[P] + xix[W]) * dpdxi_in + (etx[P] + etx[W]) * dpde_in);
+    const double us_jp = 0.5 * (up[P] + up[Nn]) - 0.5 * dt * ((xix[P] + xix[Nn]) * dpdxi_jp + (etx[P] + etx[Nn]) * dpde_jp);
+    const double us_jn = 0.5 * (up[P] + up[S]) - 0.5 * dt * ((xix[P] + xix[S]) * dpdxi_jn + (etx[P] + etx[S]) * dpde_jn);
+    const double vs_ip = 0.5 * (vp[P] + vp[E]) - 0.5 * dt * ((xiy[P] + xiy[E]) * dpdxi_ip + (ety[P] + ety[E]) * dpde_ip);
+    const double vs_in = 0.5 * (vp[P] + vp[W]) - 0.5 * dt * ((xiy[P] + xiy[W]) * dpdxi_in + (ety[P] + ety[W]) * dpde_in);
+    const double vs_jp = 0.5 * (vp[P] + vp[Nn]) - 0.5 * dt * ((xiy[P] + xiy[Nn]) * dpdxi_jp + (ety[P] + ety[Nn]) * dpde_jp);
+    const double vs_jn = 0.5 * (vp[P] + vp[S]) - 0.5 * dt * ((xiy[P] + xiy[S]) * dpdxi_jn + (ety[P] + ety[S]) * dpde_jn);
+    const double qq = (xix[P] * ((us_ip - us_in) / h1)) + (etx[P] * ((us_jp - us_jn) / h2)) +
+                      (xiy[P] * ((vs_ip - vs_in) / h1)) + (ety[P] * ((vs_jp - vs_jn) / h2));
+    q[P] = qq / dt;
+}
+
+// ---------------------------------------------------------------- a9, a10
+__global__ void k_pcor_bc(int n0, int n1, double *__restrict__ pc, const double *__restrict__ xnox,
+                          const double *__restrict__ xnoy, double uinf, double vinf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n0 - 2) return;
+    pc[ix(i, 0, n0)] = pc[ix(i, 1, n0)];
+    const int j = n1 - 1;
+    const double vnn = uinf * xnox[i] + vinf * xnoy[i];
+    const double val = (vnn >= 0) ? pc[ix(i, j - 1, n0)] : 0.0;
+    pc[ix(i, j, n0)] = val;
+    if (i == 0) {
+        pc[ix(n0 - 1, 0, n0)] = pc[ix(0, 1, n0)];
+        pc[ix(n0 - 1, j, n0)] = val;
+    }
+}
+
+__global__ void k_correct_ref(int n0, int n1, double h1, double h2, const double *__restrict__ us,
+                              const double *__restrict__ vs, const double *__restrict__ pc,
+                              const double *__restrict__ xix, const double *__restrict__ xiy,
+                              const double *__restrict__ etx, const double *__restrict__ ety, double dt,
+                              double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+    const size_t P = ix(i, j, n0);
+    const double dpx = 0.5 * (pc[ix(ipp, j, n0)] - pc[ix(inn, j, n0)]) / h1;
+    const double dpe = 0.5 * (pc[ix(i, j + 1, n0)] - pc[ix(i, j - 1, n0)]) / h2;
+    const double uu = us[P] - dt * (xix[P] * dpx + etx[P] * dpe);
+    const double vv = vs[P] - dt * (xiy[P] * dpx + ety[P] * dpe);
+    const double pp = p[P] + pc[P];
+    u[P] = uu; v[P] = vv; p[P] = pp;
+    if (i == 0) {
+        const size_t dup = ix(n0 - 1, j, n0);
+        u[dup] = uu; v[dup] = vv; p[dup] = pp;
+    }
+}
+
+// ---------------------------------------------------------------- a11
+struct WallRef {
+    const double *u, *v, *T, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *q1, *ajac, *alc_u, *alc_v;
+};
+
+__global__ void k_wall_p_ref(int n0, int n1, double h1, double h2, WallRef w, double *__restrict__ p, double Re,
+                             double Ri) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n0 - 2) return;
+    const int j = 0;
+    const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+    const size_t P = ix(i, j, n0);
+    const double uxi = w.xix[P] * w.u[P] + w.xiy[P] * w.v[P];
+    const double uet = w.etx[P] * w.u[P] + w.ety[P] * w.v[P];
+    double dp_dx = 0.0, dp_dy = 0.0;
+    for (int k = 0; k < 2; k++) {
+        const double *f = k == 0 ? w.u : w.v;
+        const double aa = w.alph[P] * (f[ix(ipp, j, n0)] + f[ix(inn, j, n0)] - 2 * f[P]) / (h1 * h1);
+        const double gg = w.gamma[P] * (f[ix(i, j + 2, n0)] + f[P] - 2 * f[ix(i, j + 1, n0)]) / (h2 * h2);
+        const double bb = w.beta[P] * (f[ix(ipp, j + 1, n0)] + f[ix(inn, j, n0)] - f[ix(inn, j + 1, n0)] -
+                                       f[ix(ipp, j, n0)]) / (2 * h1 * h2);
+        const double qqq = w.q1[P] * (-3 * f[P] + 4 * f[ix(i, j + 1, n0)] - f[ix(i, j + 2, n0)]) / (2 * h2);
+        const double d2u = aa + gg - 2 * bb + qqq;
+        double conv = uxi * 0.5 * (f[ix(ipp, j, n0)] - f[ix(inn, j, n0)]) / h1;
+        conv = conv + uet * (f[ix(i, j + 1, n0)] - f[P]) / h2;
+        if (k == 0) dp_dx = 1.0 * d2u / Re - conv - (w.alc_u ? w.alc_u[i] : 0.0);
+        else dp_dy = 1.0 * d2u / Re - conv - (w.alc_v ? w.alc_v[i] : 0.0) + Ri * w.T[P];
+    }
+    const double val = p[ix(i, j + 1, n0)] - (dp_dx * (-w.xiy[P] * w.ajac[P]) + dp_dy * (w.xix[P] * w.ajac[P])) * h2;
+    p[P] = val;
+    if (i == 0) p[ix(n0 - 1, j, n0)] = val;
+}
+
+// ---------------------------------------------------------------- a12, a13
+__global__ void k_dil_ref(int n0, int n1, double h1, double h2, const double *__restrict__ u,
+                          const double *__restrict__ v, const double *__restrict__ xix,
+                          const double *__restrict__ xiy, const double *__restrict__ etx,
+                          const double *__restrict__ ety, double *__restrict__ dil, double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double d = 0.0;
+    if (i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+        const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+        const size_t P = ix(i, j, n0);
+        d = xix[P] * (u[ix(ipp, j, n0)] - u[ix(inn, j, n0)]) / (2 * h1) +
+            etx[P] * (u[ix(i, j + 1, n0)] - u[ix(i, j - 1, n0)]) / (2 * h2) +
+            ety[P] * (v[ix(i, j + 1, n0)] - v[ix(i, j - 1, n0)]) / (2 * h2) +
+            xiy[P] * (v[ix(ipp, j, n0)] - v[ix(inn, j, n0)]) / (2 * h1);
+        dil[P] = d;
+        if (i == 0) dil[ix(n0 - 1, j, n0)] = d;
+    }
+    const double m = block_max<BX * BY>(d);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = m;
+}
+
+__global__ void k_max_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    double m = 0.0;  // floor at 0 as solver.cpp:1806 starts dmax at 0
+    for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmax(m, part[t]);
+    m = block_max<256>(m);
+    if (threadIdx.x == 0) *out = m;
+}
+
+__global__ void k_nusselt_ref(int n0, double h1, double h2, const double *__restrict__ T,
+                              const double *__restrict__ ajac, const double *__restrict__ etx,
+                              const double *__restrict__ ety, double ar, double *__restrict__ out,
+                              double *__restrict__ dthdn) {
+    double acc = 0.0;
+    const int j = 0;
+    for (int i = threadIdx.x; i < n0 - 1; i += blockDim.x) {
+        const int ip = i + 1;
+        const size_t A = ix(i, j, n0), B = ix(ip, j, n0);
+        const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
+        const double tj2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
+        const double fh1 = tj1 * (4 * T[ix(i, j + 1, n0)] - 3 * T[A] - T[ix(i, j + 2, n0)]) / (2 * h2);
+        const double fh2 = tj2 * (4 * T[ix(ip, j + 1, n0)] - 3 * T[B] - T[ix(ip, j + 2, n0)]) / (2 * h2);
+        acc += 0.5 * (fh1 + fh2) * h1;
+    }
+    for (int i = threadIdx.x; i < n0; i += blockDim.x) {
+        const double d = -(4 * T[ix(i, 1, n0)] - 3 * T[ix(i, 0, n0)] - T[ix(i, 2, n0)]) / (2 * h2);
+        dthdn[i] = d * sqrt(etx[ix(i, 0, n0)] * etx[ix(i, 0, n0)] + ety[ix(i, 0, n0)] * ety[ix(i, 0, n0)]);
+    }
+    acc = block_sum<256>(acc);
+    if (threadIdx.x == 0) {
+        const double Pi = acos(-1.0);
+        *out = (2 * acc) / (Pi * (3 * (1 + (1.0 / ar)) - sqrt((3 + (1.0 / ar)) * ((3.0 / ar) + 1))));
+    }
+}
+
+// ---------------------------------------------------------------- stored-operator BiCGStab
+// Unknowns: columns 0..n0-2, rows 1..n1-2 of the periodic grid.  The operator is
+// `npts` stored planes (9 or 25); rows 1 and n1-2 of a 25-point operator use only
+// their 9-point planes, as Solver::gauss does (solver.cpp:2331-2353).
+
+__device__ __forceinline__ bool is_unk(int i, int j, int n0, int n1) { return i <= n0 - 2 && j >= 1 && j <= n1 - 2; }
+
+template <int R>
+__global__ void k_sop_apply(int n0, int n1, const double *__restrict__ A, const double *__restrict__ x,
+                            const double *__restrict__ q, double *__restrict__ y) {
+    // y = q - A x on the unknowns when q != nullptr, else y = A x; 0 elsewhere
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = ix(i, j, n0);
+    if (!is_unk(i, j, n0, n1)) { y[P] = 0.0; return; }
+    const size_t N = (size_t)n0 * n1;
+    const int r = (R == 2 && (j == 1 || j == n1 - 2)) ? 1 : R;
+    double s = 0.0;
+    for (int dj = -r; dj <= r; dj++)
+        for (int di = -r; di <= r; di++) {
+            const int k = (R == 2) ? (dj + 2) * 5 + (di + 2) : (dj + 1) * 3 + (di + 1);
+            // wrap with period n0-1 (column n0-1 is read as column 0)
+            int ii = i + di;
+            if (ii < 0) ii += n0 - 1;
+            if (ii > n0 - 2) ii -= n0 - 1;
+            s += A[(size_t)k * N + P] * x[ix(ii, j + dj, n0)];
+        }
+    y[P] = q ? q[P] - s : s;
+}
+
+__global__ void k_dot_part(int n0, int n1, const double *__restrict__ a, const double *__restrict__ b,
+                           double *__restrict__ part) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double s = 0.0;
+    if (i < n0 && j < n1 && is_unk(i, j, n0, n1)) s = a[ix(i, j, n0)] * b[ix(i, j, n0)];
+    s = block_sum<BX * BY>(s);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = s;
+}
+
+__global__ void k_sum_final(const double *__restrict__ part, int n, double *__restrict__ out) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) s += part[t];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// z = invdiag * y
+__global__ void k_jac(size_t n, const double *__restrict__ y, const double *__restrict__ invd, double *__restrict__ z) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) z[k] = y[k] * invd[k];
+}
+// y = a + s1*b + s2*c
+__global__ void k_lin3(size_t n, double *__restrict__ y, const double *__restrict__ a, double s1,
+                       const double *__restrict__ b, double s2, const double *__restrict__ c) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = a[k] + s1 * b[k] + (c ? s2 * c[k] : 0.0);
+}
+__global__ void k_invdiag(int n0, int n1, int center, const double *__restrict__ A, double *__restrict__ invd) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = ix(i, j, n0);
+    invd[P] = is_unk(i, j, n0, n1) ? 1.0 / A[(size_t)center * n0 * n1 + P] : 0.0;
+}
+// duplicate column n0-1 := column 0 on the unknown rows
+__global__ void k_dupcol(int n0, int n1, double *__restrict__ x) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 1 && j <= n1 - 2) x[ix(n0 - 1, j, n0)] = x[ix(0, j, n0)];
+}
+
+}  // namespace ref
+}  // namespace cfd
+
+// ==========================================================================
+using namespace cfd;
+using namespace cfd::ref;
+
+namespace {
+
+// RAII bundle of device buffers for one stage call
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs() {
+        for (void *q : p) (void)hipFree(q);
+    }
+    double *get(size_t n) {
+        void *q = nullptr;
+        if (hipMalloc(&q, n * sizeof(double)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return static_cast<double *>(q);
+    }
+    double *up(const double *h, size_t n) {
+        double *d = get(n);
+        if (d && h && hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        if (d && !h && hipMemset(d, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+        return d;
+    }
+};
+
+#define NEED(x)                          \
+    do {                                 \
+        if (!(x)) return CFD_E_OOM;      \
+    } while (0)
+
+int down(double *h, const double *d, size_t n) {
+    if (!h) return CFD_OK;
+    CFD_HIP(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+    return CFD_OK;
+}
+
+bool bad_grid(int n0, int n1) { return n0 < 5 || n1 < 5; }
+
+// GPU BiCGStab (Jacobi-preconditioned) on a stored 9/25-point operator
+template <int R>
+int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double *phi, const double *q,
+              double rtol, int maxit, int *iters, double *relres) {
+    if (bad_grid(n0, n1) || !phi || !q) return CFD_E_ARG;
+    for (int k = 0; k < npts; k++)
+        if (!planes_host[k]) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *A = B.get((size_t)npts * N);
+    NEED(A);
+    for (int k = 0; k < npts; k++)
+        CFD_HIP(hipMemcpy(A + (size_t)k * N, planes_host[k], N * sizeof(double), hipMemcpyHostToDevice));
+    double *x = B.up(phi, N), *qd = B.up(q, N);
+    double *r = B.get(N), *r0 = B.get(N), *pv = B.up(nullptr, N), *vv = B.up(nullptr, N), *s = B.get(N);
+    double *t = B.get(N), *ph = B.get(N), *sh = B.get(N), *dx = B.up(nullptr, N), *invd = B.get(N);
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const int nb = (int)(gr.x * gr.y);
+    double *part = B.get((size_t)nb + 8), *dsc = B.get(8);
+    NEED(x && qd && r && r0 && pv && vv && s && t && ph && sh && dx && invd && part && dsc);
+    const int center = (R == 2) ? 12 : 4;
+    k_invdiag<<<gr, bl>>>(n0, n1, center, A, invd);
+    CFD_LAUNCH_CHECK();
+    auto dot = [&](const double *a, const double *b, double *out) -> int {
+        k_dot_part<<<gr, bl>>>(n0, n1, a, b, part);
+        k_sum_final<<<1, 256>>>(part, nb, dsc);
+        CFD_LAUNCH_CHECK();
+        CFD_HIP(hipMemcpy(out, dsc, sizeof(double), hipMemcpyDeviceToHost));
+        return CFD_OK;
+    };
+    const int lb = 256;
+    const int lg = (int)((N + lb - 1) / lb);
+    // r = q - A phi (boundary rows of phi enter here), solve A dx = r
+    k_sop_apply<R><<<gr, bl>>>(n0, n1, A, x, qd, r);
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipMemcpy(r0, r, N * sizeof(double), hipMemcpyDeviceToDevice));
+    double rr0 = 0.0;
+    int rc = dot(r, r, &rr0);
+    if (rc) return rc;
+    const double r0n = std::sqrt(rr0);
+    double rn = r0n;
+    int it = 0;
+    if (r0n > 0.0) {
+        const double tol = rtol * r0n;
+        double rho_old = 1.0, alpha = 1.0, omega = 1.0, rho = rr0;
+        for (it = 1; it <= maxit; it++) {
+            if (rho == 0.0) break;
+            if (it == 1) {
+                CFD_HIP(hipMemcpy(pv, r, N * sizeof(double), hipMemcpyDeviceToDevice));
+            } else {
+                const double beta = (rho / rho_old) * (alpha / omega);
+                // pv = r + beta (pv - omega vv)
+                k_lin3<<<lg, lb>>>(N, pv, pv, -omega, vv, 0.0, nullptr);
+                k_lin3<<<lg, lb>>>(N, pv, r, beta, pv, 0.0, nullptr);
+            }
+            k_jac<<<lg, lb>>>(N, pv, invd, ph);
+            k_sop_apply<R><<<gr, bl>>>(n0, n1, A, ph, nullptr, vv);
+            CFD_LAUNCH_CHECK();
+            double rv = 0.0;
+            if ((rc = dot(r0, vv, &rv))) return rc;
+            alpha = rho / rv;
+            k_lin3<<<lg, lb>>>(N, s, r, -alpha, vv, 0.0, nullptr);
+            double ss = 0.0;
+            if ((rc = dot(s, s, &ss))) return rc;
+            if (std::sqrt(ss) <= tol) {
+                k_lin3<<<lg, lb>>>(N, dx, dx, alpha, ph, 0.0, nullptr);
+                rn = std::sqrt(ss);
+                break;
+            }
+            k_jac<<<lg, lb>>>(N, s, invd, sh);
+            k_sop_apply<R><<<gr, bl>>>(n0, n1, A, sh, nullptr, t);
+            CFD_LAUNCH_CHECK();
+            double ts = 0.0, tt = 0.0;
+            if ((rc = dot(t, s, &ts))) return rc;
+            if ((rc = dot(t, t, &tt))) return rc;
+            omega = tt > 0.0 ? ts / tt : 0.0;
+            k_lin3<<<lg, lb>>>(N, dx, dx, alpha, ph, omega, sh);
+            k_lin3<<<lg, lb>>>(N, r, s, -omega, t, 0.0, nullptr);
+            double rr = 0.0;
+            if ((rc = dot(r, r, &rr))) return rc;
+            rn = std::sqrt(rr);
+            if (rn <= tol || omega == 0.0) break;
+            rho_old = rho;
+            if ((rc = dot(r0, r, &rho))) return rc;
+        }
+        if (it > maxit) it = maxit;
+        k_lin3<<<lg, lb>>>(N, x, x, 1.0, dx, 0.0, nullptr);
+        k_dupcol<<<(n1 + 255) / 256, 256>>>(n0, n1, x);
+        CFD_LAUNCH_CHECK();
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = r0n > 0.0 ? rn / r0n : 0.0;
+    return down(phi, x, N);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfd_ref_assemble_mom(int n0, int n1, double dxi, double deta, const double *alph, const double *beta,
+                         const double *gamma, const double *p1, const double *q1, double dt, double Re, double Pr,
+                         double *AU, double *AT, double *BU, double *BT) {
+    if (bad_grid(n0, n1) || !alph || !beta || !gamma || !AU || !AT || !BU || !BT) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(alph, N), *b = B.up(beta, N), *g = B.up(gamma, N), *pp = B.up(p1, N), *qq = B.up(q1, N);
+    double *au = B.up(nullptr, 25 * N), *at = B.up(nullptr, 25 * N), *bu = B.up(nullptr, 6 * (size_t)n0),
+           *bt = B.up(nullptr, 6 * (size_t)n0);
+    NEED(a && b && g && pp && qq && au && at && bu && bt);
+    k_asm_mom<<<grid2d(n0, n1), dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, g, pp, qq, dt, Re, Pr, au, at, bu, bt);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(AU, au, 25 * N));
+    CFD_RET(down(AT, at, 25 * N));
+    CFD_RET(down(BU, bu, 6 * (size_t)n0));
+    return down(BT, bt, 6 * (size_t)n0);
+}
+
+int cfd_ref_assemble_p9(int n0, int n1, double dxi, double deta, const double *xix, const double *xiy,
+                        const double *etx, const double *ety, double *A9) {
+    if (bad_grid(n0, n1) || !xix || !xiy || !etx || !ety || !A9) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(xix, N), *b = B.up(xiy, N), *c = B.up(etx, N), *d = B.up(ety, N), *o = B.up(nullptr, 9 * N);
+    NEED(a && b && c && d && o);
+    k_asm_p9<<<grid2d(n0, n1), dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, c, d, o);
+    CFD_LAUNCH_CHECK();
+    return down(A9, o, 9 * N);
+}
+
+int cfd_ref_rhs(int n0, int n1, double dxi, double deta, const double *u, const double *v, const double *T,
+                const double *up, const double *vp, const double *p, const double *xix, const double *xiy,
+                const double *etx, const double *ety, const double *alph, const double *gamma, const double *BU,
+                const double *BT, double Re, double Pr, double Ri, double dt, double *uxi, double *uet, double *qu,
+                double *qv, double *qt, double *qup, double *qvp) {
+    if (bad_grid(n0, n1) || !u || !v || !T || !up || !vp || !p || !xix || !xiy || !etx || !ety || !alph ||
+        !gamma || !BU || !BT)
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    RhsIn a{B.up(u, N), B.up(v, N), B.up(T, N), B.up(up, N), B.up(vp, N), B.up(p, N), B.up(xix, N),
+            B.up(xiy, N), B.up(etx, N), B.up(ety, N), B.up(alph, N), B.up(gamma, N), B.up(BU, 6 * (size_t)n0),
+            B.up(BT, 6 * (size_t)n0), B.up(nullptr, N), B.up(nullptr, N)};
+    RhsOut o{B.up(nullptr, N), B.up(nullptr, N), B.up(nullptr, N), B.up(nullptr, N), B.up(nullptr, N)};
+    NEED(a.u && a.v && a.T && a.up && a.vp && a.p && a.xix && a.xiy && a.etx && a.ety && a.alph && a.gamma &&
+         a.BU && a.BT && a.uxi && a.uet && o.qu && o.qv && o.qt && o.qup && o.qvp);
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    k_contra<<<gr, bl>>>(n0, n1, a.u, a.v, a.xix, a.xiy, a.etx, a.ety, (double *)a.uxi, (double *)a.uet);
+    k_rhs<<<gr, bl>>>(n0, n1, dxi, deta, a, o, Re, Pr, Ri, dt);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(uxi, a.uxi, N));
+    CFD_RET(down(uet, a.uet, N));
+    CFD_RET(down(qu, o.qu, N));
+    CFD_RET(down(qv, o.qv, N));
+    CFD_RET(down(qt, o.qt, N));
+    CFD_RET(down(qup, o.qup, N));
+    return down(qvp, o.qvp, N);
+}
+
+int cfd_gauss(int n0, int n1, const double *ap, const double *ae, const double *as, const double *an,
+              const double *aw, const double *ase, const double *asw, const double *ane, const double *anw,
+              const double *ass, const double *assee, const double *assww, const double *asse, const double *assw,
+              const double *asee, const double *asww, const double *ann, const double *annee,
+              const double *annww, const double *anne, const double *annw, const double *anee, const double *anww,
+              const double *aee, const double *aww, double *phi, const double *q, double rtol, int maxit,
+              int *iters, double *relres) {
+    // planes in (dj+2)*5 + (di+2) order
+    const double *pl[25] = {assww, assw, ass, asse, assee,
+                            asww, asw, as, ase, asee,
+                            aww, aw, ap, ae, aee,
+                            anww, anw, an, ane, anee,
+                            annww, annw, ann, anne, annee};
+    return sop_solve<2>(n0, n1, pl, 25, phi, q, rtol, maxit, iters, relres);
+}
+
+int cfd_sip9p(int n0, int n1, const double *ap, const double *ae, const double *as, const double *an,
+              const double *aw, const double *ase, const double *asw, const double *ane, const double *anw,
+              double *phi, const double *q, double rtol, int maxit, int *iters, double *relres) {
+    const double *pl[9] = {asw, as, ase, aw, ap, ae, anw, an, ane};
+    return sop_solve<1>(n0, n1, pl, 9, phi, q, rtol, maxit, iters, relres);
+}
+
+int cfd_ref_face_div(int n0, int n1, double dxi, double deta, const double *up, const double *vp, const double *p,
+                     const double *xix, const double *xiy, const double *etx, const double *ety, double dt,
+                     double *q) {
+    if (bad_grid(n0, n1) || !up || !vp || !p || !xix || !xiy || !etx || !ety || !q) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(up, N), *b = B.up(vp, N), *c = B.up(p, N), *d = B.up(xix, N), *e = B.up(xiy, N),
+           *f = B.up(etx, N), *g = B.up(ety, N), *o = B.up(q, N);
+    NEED(a && b && c && d && e && f && g && o);
+    k_face_div_ref<<<grid2d(n0, n1), dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, c, d, e, f, g, dt, o);
+    CFD_LAUNCH_CHECK();
+    return down(q, o, N);
+}
+
+int cfd_ref_pcor_bc(int n0, int n1, double *pcor, const double *xnox, const double *xnoy, double uinf,
+                    double vinf) {
+    if (bad_grid(n0, n1) || !pcor || !xnox || !xnoy) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *pc = B.up(pcor, N), *a = B.up(xnox, n0), *b = B.up(xnoy, n0);
+    NEED(pc && a && b);
+    k_pcor_bc<<<(n0 + 255) / 256, 256>>>(n0, n1, pc, a, b, uinf, vinf);
+    CFD_LAUNCH_CHECK();
+    return down(pcor, pc, N);
+}
+
+int cfd_ref_correct(int n0, int n1, double dxi, double deta, const double *us, const double *vs,
+                    const double *pcor, const double *xix, const double *xiy, const double *etx, const double *ety,
+                    double dt, double *u, double *v, double *p) {
+    if (bad_grid(n0, n1) || !us || !vs || !pcor || !xix || !xiy || !etx || !ety || !u || !v || !p)
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(us, N), *b = B.up(vs, N), *c = B.up(pcor, N), *d = B.up(xix, N), *e = B.up(xiy, N),
+           *f = B.up(etx, N), *g = B.up(ety, N), *uu = B.up(u, N), *vv = B.up(v, N), *pp = B.up(p, N);
+    NEED(a && b && c && d && e && f && g && uu && vv && pp);
+    k_correct_ref<<<grid2d(n0, n1), dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, c, d, e, f, g, dt, uu, vv, pp);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(u, uu, N));
+    CFD_RET(down(v, vv, N));
+    return down(p, pp, N);
+}
+
+int cfd_ref_wall_pressure(int n0, int n1, double dxi, double deta, const double *u, const double *v,
+                          const double *T, double *p, const double *xix, const double *xiy, const double *etx,
+                          const double *ety, const double *alph, const double *beta, const double *gamma,
+                          const double *q1, const double *ajac, const double *alc_u, const double *alc_v, double Re,
+                          double Ri) {
+    if (bad_grid(n0, n1) || !u || !v || !T || !p || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma ||
+        !ajac)
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    WallRef w{B.up(u, N), B.up(v, N), B.up(T, N), B.up(xix, N), B.up(xiy, N), B.up(etx, N), B.up(ety, N),
+              B.up(alph, N), B.up(beta, N), B.up(gamma, N), B.up(q1, N), B.up(ajac, N),
+              alc_u ? B.up(alc_u, n0) : nullptr, alc_v ? B.up(alc_v, n0) : nullptr};
+    double *pp = B.up(p, N);
+    NEED(w.u && w.v && w.T && w.xix && w.xiy && w.etx && w.ety && w.alph && w.beta && w.gamma && w.q1 && w.ajac && pp);
+    k_wall_p_ref<<<(n0 + 255) / 256, 256>>>(n0, n1, dxi, deta, w, pp, Re, Ri);
+    CFD_LAUNCH_CHECK();
+    return down(p, pp, N);
+}
+
+int cfd_ref_dilation(int n0, int n1, double dxi, double deta, const double *u, const double *v, const double *xix,
+                     const double *xiy, const double *etx, const double *ety, double *dil, double *dmax) {
+    if (bad_grid(n0, n1) || !u || !v || !xix || !xiy || !etx || !ety) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    const dim3 gr = grid2d(n0, n1);
+    const int nb = (int)(gr.x * gr.y);
+    DevBufs B;
+    double *a = B.up(u, N), *b = B.up(v, N), *c = B.up(xix, N), *d = B.up(xiy, N), *e = B.up(etx, N),
+           *f = B.up(ety, N), *o = B.up(dil, N), *part = B.get((size_t)nb), *m = B.get(1);
+    NEED(a && b && c && d && e && f && o && part && m);
+    k_dil_ref<<<gr, dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, c, d, e, f, o, part);
+    k_max_final<<<1, 256>>>(part, nb, m);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(dil, o, N));
+    return down(dmax, m, 1);
+}
+
+int cfd_ref_nusselt(int n0, int n1, double dxi, double deta, const double *T, const double *ajac, const double *etx,
+                    const double *ety, double ar, double *nu, double *dthdn) {
+    if (bad_grid(n0, n1) || !T || !ajac || !etx || !ety || !nu) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(T, N), *b = B.up(ajac, N), *c = B.up(etx, N), *d = B.up(ety, N), *o = B.get(1),
+           *prof = B.get((size_t)n0);
+    NEED(a && b && c && d && o && prof);
+    k_nusselt_ref<<<1, 256>>>(n0, dxi, deta, a, b, c, d, ar, o, prof);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(nu, o, 1));
+    return down(dthdn, prof, (size_t)n0);
+}
+
+const char *cfd_version(void) { return "cfd_gpu 0.1 (gfx950)"; }
+
+int cfd_device_count(int *count) {
+    if (!count) return CFD_E_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return CFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+/*
+ * cfd_gpu.h -- C ABI of the MI355X (gfx950) SIMPLE / pressure-correction path.
+ *
+ * All arrays are FP64, contiguous, n1 rows of n0 values: node (i, j) at
+ * j*n0 + i (xi fastest).  Host pointers unless stated otherwise.  Every
+ * function returns CFD_OK (0) or a negative CFD_E_* code; nothing throws.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Reference stages (cfd_ref_*, cfd_sip9p, cfd_gauss) -- one call per
+ *     stage of the reference solver's time step on its periodic O-grid
+ *     (column n0-1 duplicates column 0).  Each replaces the cited region of
+ *     solver.cpp (PSolver.cpp is the same code one line lower) and takes the
+ *     same arrays the reference's Solver members hold, converted from the
+ *     reference's double** rows (a[i][j]) to the contiguous layout above.
+ *     25-point operators are 25 planes of n0*n1 values, plane (dj+2)*5+(di+2)
+ *     holding the coefficient of phi(i+di, j+dj); 9-point ones 9 planes,
+ *     plane (dj+1)*3+(di+1).
+ *
+ *  2. Cavity cases (cfd_case_*) -- a device-resident steady mixed-convection
+ *     cavity solved by the SIMPLE-type outer iteration built from the same
+ *     operators; the case owns its HBM buffers and its HIP stream.
+ *
+ * Threading: one case per host thread at a time; stage entry points are
+ * re-entrant (they allocate their own device scratch per call).
+ */
+#ifndef CFD_GPU_H
+#define CFD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_E_ARG (-1)      /* bad size / null pointer / unsupported shape        */
+#define CFD_E_OOM (-2)      /* device allocation failed                           */
+#define CFD_E_NAN (-3)      /* a residual or field became NaN/Inf                 */
+#define CFD_E_NOTCONV (-4)  /* iteration cap reached before the tolerance         */
+#define CFD_E_HIP (-5)      /* HIP runtime error (message on stderr)              */
+#define CFD_E_STATE (-6)    /* call out of order (e.g. fields never set)          */
+
+/* Library / device information. */
+const char *cfd_version(void);
+int cfd_device_count(int *count);
+
+/* ------------------------------------------------------------------ */
+/* 1. Reference stages                                                 */
+/* ------------------------------------------------------------------ */
+
+/* Momentum / temperature operators I - dt/Re L, second order at j = 1 and
+ * j = n1-2, fourth order inside; wall/far-field couplings of those rows moved
+ * into BU/BT (6 arrays of n0: s, se, sw, n, ne, nw).
+ * Replaces solver.cpp:751-939. */
+int cfd_ref_assemble_mom(int n0, int n1, double dxi, double deta,
+                         const double *alph, const double *beta, const double *gamma,
+                         const double *p1, const double *q1,
+                         double dt, double Re, double Pr,
+                         double *AU, double *AT, double *BU, double *BT);
+
+/* Pressure-correction 9-point operator.  Replaces solver.cpp:943-1039. */
+int cfd_ref_assemble_p9(int n0, int n1, double dxi, double deta,
+                        const double *xix, const double *xiy,
+                        const double *etx, const double *ety, double *A9);
+
+/* Contravariant velocities and the u/v/T right-hand sides (hybrid convection,
+ * pressure gradient, buoyancy, boundary-row sums).
+ * Replaces solver.cpp:1062-1236. */
+int cfd_ref_rhs(int n0, int n1, double dxi, double deta,
+                const double *u, const double *v, const double *T,
+                const double *up, const double *vp, const double *p,
+                const double *xix, const double *xiy, const double *etx, const double *ety,
+                const double *alph, const double *gamma,
+                const double *BU, const double *BT,
+                double Re, double Pr, double Ri, double dt,
+                double *uxi, double *uet,
+                double *qu, double *qv, double *qt, double *qup, double *qvp);
+
+/* Solve A phi = q for the interior rows 1..n1-2 of the periodic O-grid with a
+ * 25-point operator given as the reference's 25 arrays, in the argument order
+ * of Solver::gauss (solver.cpp:2279-2285).  phi: in = initial guess and the
+ * fixed boundary rows, out = solution.  Solved on the GPU by BiCGStab to
+ * ||q - A phi||_2 <= rtol ||q - A phi0||_2 (replaces the Gauss-Seidel sweeps
+ * of solver.cpp:2279-2445, whose fixed point it shares). */
+int cfd_gauss(int n0, int n1,
+              const double *ap, const double *ae, const double *as, const double *an,
+              const double *aw, const double *ase, const double *asw, const double *ane,
+              const double *anw, const double *ass, const double *assee, const double *assww,
+              const double *asse, const double *assw, const double *asee, const double *asww,
+              const double *ann, const double *annee, const double *annww, const double *anne,
+              const double *annw, const double *anee, const double *anww, const double *aee,
+              const double *aww, double *phi, const double *q,
+              double rtol, int maxit, int *iters, double *relres);
+
+/* Face-velocity divergence q = div(u_f)/dt.  Replaces solver.cpp:1389-1448. */
+int cfd_ref_face_div(int n0, int n1, double dxi, double deta,
+                     const double *up, const double *vp, const double *p,
+                     const double *xix, const double *xiy, const double *etx, const double *ety,
+                     double dt, double *q);
+
+/* Solve the 9-point pressure-correction system, argument order of
+ * Solver::sip9p (solver.cpp:2066-2068).  Same conventions as cfd_gauss;
+ * replaces Stone's SIP of solver.cpp:2066-2277. */
+int cfd_sip9p(int n0, int n1,
+              const double *ap, const double *ae, const double *as, const double *an,
+              const double *aw, const double *ase, const double *asw, const double *ane,
+              const double *anw, double *phi, const double *q,
+              double rtol, int maxit, int *iters, double *relres);
+
+/* p' boundary rows after the solve: wall copy and far-field 0/copy by the
+ * sign of the free-stream normal velocity.  Replaces solver.cpp:1474-1497. */
+int cfd_ref_pcor_bc(int n0, int n1, double *pcor, const double *xnox, const double *xnoy,
+                    double uinf, double vinf);
+
+/* u = us - dt grad p', v likewise, p += p'.  Replaces solver.cpp:1504-1541. */
+int cfd_ref_correct(int n0, int n1, double dxi, double deta,
+                    const double *us, const double *vs, const double *pcor,
+                    const double *xix, const double *xiy, const double *etx, const double *ety,
+                    double dt, double *u, double *v, double *p);
+
+/* Wall (j = 0) pressure from the normal momentum equation; alc_u/alc_v are the
+ * per-node local accelerations (may be NULL for zero).
+ * Replaces solver.cpp:1667-1728. */
+int cfd_ref_wall_pressure(int n0, int n1, double dxi, double deta,
+                          const double *u, const double *v, const double *T, double *p,
+                          const double *xix, const double *xiy, const double *etx, const double *ety,
+                          const double *alph, const double *beta, const double *gamma,
+                          const double *q1, const double *ajac,
+                          const double *alc_u, const double *alc_v, double Re, double Ri);
+
+/* Dilation field and its signed maximum floored at 0 (the mass-imbalance
+ * monitor).  Replaces solver.cpp:1806-1852. */
+int cfd_ref_dilation(int n0, int n1, double dxi, double deta,
+                     const double *u, const double *v,
+                     const double *xix, const double *xiy, const double *etx, const double *ety,
+                     double *dil, double *dmax);
+
+/* Mean wall Nusselt number of the j = 0 wall, normalised by the ellipse
+ * perimeter of aspect ratio ar, and the local dT/dn profile (may be NULL).
+ * Replaces solver.cpp:1951-1982 and 2031-2038. */
+int cfd_ref_nusselt(int n0, int n1, double dxi, double deta, const double *T,
+                    const double *ajac, const double *etx, const double *ety,
+                    double ar, double *nu, double *dthdn);
+
+/* ------------------------------------------------------------------ */
+/* 2. Cavity cases                                                     */
+/* ------------------------------------------------------------------ */
+
+#define CFD_FP64 0   /* every stage in FP64                                       */
+#define CFD_MIXED 1  /* FP32 p' Krylov/multigrid and predictor sweeps, FP64
+                        residuals, refinement, accumulation and reductions     */
+
+typedef struct {
+    double Re, Ri, Pr;
+    double dt;          /* pseudo-time step of the predictor (large = steady SIMPLE) */
+    double alpha_u;     /* implicit relaxation of u, v                           */
+    double alpha_p;     /* p += alpha_p p'                                       */
+    double alpha_t;     /* implicit relaxation of T                              */
+    double lid_u;       /* lid (j = n1-1) velocity                               */
+    double t_hot, t_cold;
+    int mom_sweeps;     /* Jacobi sweeps of the low-order predictor              */
+    double p_rtol;      /* p' relative tolerance (2-norm)                        */
+    int p_maxit;        /* p' BiCGStab iteration cap                             */
+    int mg_levels;      /* multigrid levels of the p' preconditioner             */
+    int mg_pre, mg_post, mg_coarse;
+    double mg_omega;    /* damping of the Jacobi smoother                        */
+    int precision;      /* CFD_FP64 or CFD_MIXED                                 */
+} cfd_params;
+
+typedef struct {
+    double mass_l1;     /* sum |q| (J-weighted face divergence) this iteration  */
+    double energy_l1;   /* sum |R_T|                                             */
+    double mom_l1;      /* sum |R_u| + |R_v|                                     */
+    double dmax;        /* max nodal dilation, floored at 0                      */
+    double nu_hot, nu_cold;
+    int p_iters;
+    double p_relres;
+    double mass_res;    /* mass_l1 / mass_l1 of the first iteration              */
+    double energy_res;  /* energy_l1 / energy_l1 of the first iteration          */
+    int iter;           /* outer iterations done so far                          */
+} cfd_stats;
+
+typedef struct cfd_case_s *cfd_case;
+
+/* Fill *p with the defaults used by the cavity configurations. */
+void cfd_params_default(cfd_params *p);
+
+/* Create a cavity case on the current device from INP.DAT-style grid arrays
+ * (x, y, metrics, alph/beta/gamma, Jacobian) plus the Laplacian control
+ * functions p1 = lap(xi), q1 = lap(eta) (NULL = 0, as the reference reads them).
+ * Fields start at rest with the wall conditions applied. */
+int cfd_case_create(int n0, int n1, double dxi, double deta,
+                    const double *x, const double *y,
+                    const double *xix, const double *xiy, const double *etx, const double *ety,
+                    const double *alph, const double *beta, const double *gamma,
+                    const double *ajac, const double *p1, const double *q1,
+                    const cfd_params *params, cfd_case *out);
+int cfd_case_destroy(cfd_case c);
+int cfd_case_set_fields(cfd_case c, const double *u, const double *v, const double *T,
+                        const double *p);
+int cfd_case_get_fields(cfd_case c, double *u, double *v, double *T, double *p);
+int cfd_case_apply_bcs(cfd_case c);
+/* Run n outer iterations; *st (may be NULL) receives the last iteration's stats. */
+int cfd_case_outer(cfd_case c, int n, cfd_stats *st);
+/* Iterate until mass_res <= tol (and energy_res <= tol) or max_outer. */
+int cfd_case_solve(cfd_case c, double tol, int max_outer, cfd_stats *st);
+/* Bytes of HBM the case holds. */
+int cfd_case_bytes(cfd_case c, size_t *bytes);
+
+/* Stage-level access for tests (host arrays). */
+int cfd_case_residuals(cfd_case c, double *Ru, double *Rv, double *RT, double *d);
+int cfd_case_p_apply(cfd_case c, const double *d, const double *x, double *y);
+int cfd_case_p_solve(cfd_case c, const double *d, const double *q, double *x,
+                     double rtol, int maxit, int *iters, double *relres);
+
+/* Live kernel timing.  While enabled, every fine-level launch of the hot
+ * kernels inside cfd_case_outer/solve is bracketed by HIP events on the
+ * case's stream; cfd_case_profile_read returns, per kind, the summed device
+ * milliseconds and the launch count since the last cfd_case_profile call.
+ * Kinds: 0 p' operator apply, 1 predictor Jacobi sweep, 2 residual/RHS,
+ * 3 face divergence, 4 multigrid smoothing sweep. */
+int cfd_case_profile(cfd_case c, int enable);
+int cfd_case_profile_read(cfd_case c, double *ms_total, long *launches, int nkinds);
+
+/* Average device time of one launch of a hot kernel on the case's stream,
+ * measured with HIP events around `reps` back-to-back launches on current
+ * data.  kernel: 0 = p' operator apply, 1 = predictor Jacobi sweep,
+ * 2 = residual/RHS, 3 = face divergence, 4 = multigrid smoothing sweep. */
+int cfd_case_time_kernel(cfd_case c, int kernel, int reps, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,39 @@
+"""The C ABI: the library loads without a GPU and exports every declared symbol."""
+import os
+import re
+
+from cfd_solver_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "cfd_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cfd_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_symbols_exported():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES)
+
+
+def test_version_and_device_count_without_gpu():
+    import ctypes as C
+    lib = _lib.load()
+    assert lib.cfd_version().startswith(b"cfd_gpu")
+    n = C.c_int(-1)
+    assert lib.cfd_device_count(C.byref(n)) == 0
+    assert n.value >= 0
+
+
+def test_argument_errors_need_no_gpu():
+    import numpy as np
+    lib = _lib.load()
+    z = np.zeros((3, 3))
+    # grids below 5 x 5 are rejected before any device work
+    assert lib.cfd_ref_assemble_p9(3, 3, 0.5, 0.5, z, z, z, z, np.zeros((9, 3, 3))) == -1

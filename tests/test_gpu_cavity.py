@@ -1,0 +1,79 @@
+"""Cavity path on the GPU against the CPU oracle (FP64; tolerance 1e-9 relative L2)."""
+import numpy as np
+import pytest
+
+from cfd_solver_amd.grid import cavity_grid
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(n, skew=0.05, beta=None, **kw):
+    from cfd_solver_amd.cavity import CavityCase, default_params
+    from oracle.oracle import CavityOracle, cav_params
+    g = cavity_grid(n, skew=skew, beta=beta)
+    gp = default_params(**kw)
+    op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields()})
+    return CavityCase(g, gp), CavityOracle(g, op), g
+
+
+def op_fields():
+    from oracle.oracle import CavParams
+    return CavParams._fields_
+
+
+def test_p_apply_and_solve_match_oracle():
+    print("\n[gpu] p' operator 33x33 skew 0.15 tanh 1.5", flush=True)
+    gpu, cpu, g = pair(33, skew=0.15, beta=1.5)
+    rng = np.random.default_rng(7)
+    d = 0.5 + rng.random((33, 33))
+    x = rng.standard_normal((33, 33))
+    assert rel_l2(gpu.p_apply(d, x), cpu.p_apply(d, x)) < 1e-13
+    q = np.zeros((33, 33))
+    q[1:-1, 1:-1] = rng.standard_normal((31, 31))
+    q[1:-1, 1:-1] -= q[1:-1, 1:-1].mean()
+    xg, itg, rg = gpu.p_solve(d, q, rtol=1e-10, maxit=100)
+    xc, itc, rc = cpu.p_solve(d, q, rtol=1e-10, maxit=100)
+    assert rg <= 1e-10 and itg == itc
+    xg = xg[1:-1, 1:-1] - xg[1:-1, 1:-1].mean()
+    xc = xc[1:-1, 1:-1] - xc[1:-1, 1:-1].mean()
+    assert rel_l2(xg, xc) < 1e-9
+
+
+def test_residuals_match_oracle():
+    print("\n[gpu] residual kernel 65x65", flush=True)
+    gpu, cpu, g = pair(65, skew=0.1, beta=1.5, Re=400.0)
+    rng = np.random.default_rng(3)
+    f = [rng.standard_normal((65, 65)) * s for s in (0.3, 0.3, 1.0, 0.5)]
+    gpu.set_fields(*f)
+    cpu.set_fields(*f)
+    for a, b, name in zip(gpu.residuals(), cpu.residuals(), ["Ru", "Rv", "RT", "d"]):
+        assert rel_l2(a[1:-1, 1:-1], b[1:-1, 1:-1]) < 1e-13, name
+
+
+def test_config1_500_iterations_match_oracle():
+    """Config 1: 65x65, skew 0.05, Re 100, Ri 1, Pr 0.71, alpha 0.7/0.3, 500 outer iterations."""
+    print("\n[gpu] cavity 65x65, 500 outer iterations", flush=True)
+    gpu, cpu, g = pair(65, skew=0.05, Re=100.0, Ri=1.0, Pr=0.71)
+    for k in range(500):
+        sg = gpu.step()
+        sc = cpu.step()
+        assert sg.p_iters == sc.p_iters, k
+    fg, fc = gpu.fields(), cpu.fields()
+    for k in ("u", "v", "T", "p"):
+        assert rel_l2(fg[k], fc[k]) < 1e-9, k
+    assert abs(sg.nu_hot - sc.nu_hot) < 1e-9 * abs(sc.nu_hot)
+    assert abs(sg.nu_cold - sc.nu_cold) < 1e-9 * abs(sc.nu_cold)
+    assert abs(sg.mass_l1 - sc.mass_l1) < 1e-6 * sc.mass_l1
+
+
+def test_large_grid_runs_and_conserves():
+    """1025x1025 (config 2 grid): a few outer iterations stay finite and decrease the mass residual."""
+    print("\n[gpu] cavity 1025x1025, 5 outer iterations", flush=True)
+    from cfd_solver_amd.cavity import CavityCase
+    c = CavityCase(cavity_grid(1025, skew=0.1, beta=1.5), Re=400.0, Ri=1.0)
+    s = [c.step() for _ in range(5)]
+    assert all(np.isfinite(x.mass_l1) and np.isfinite(x.nu_hot) for x in s)
+    assert s[-1].mass_res < 1.0
+    f = c.fields()
+    assert np.isfinite(f["u"]).all() and np.abs(f["u"]).max() < 10

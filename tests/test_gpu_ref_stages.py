@@ -1,0 +1,108 @@
+"""GPU reference stages (cfd_ref_*, cfd_sip9p, cfd_gauss) against the compiled reference's arrays."""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13  # FP64; FMA contraction on the GPU changes the last bits only
+
+
+@pytest.fixture(scope="module")
+def st(golden):
+    from cfd_solver_amd.stages import RefStages
+    print(f"\n[gpu] RefStages {golden['n0']}x{golden['n1']}", flush=True)
+    return RefStages(golden["n0"], golden["n1"], golden["dxi"], golden["deta"])
+
+
+def _g(d):
+    return {k: d["grid_" + k] for k in ["xix", "xiy", "etx", "ety", "alph", "beta", "gamma", "ajac"]}
+
+
+def test_assemble_p9(golden, st):
+    g = _g(golden)
+    A9 = st.assemble_p9(g["xix"], g["xiy"], g["etx"], g["ety"])
+    assert rel_l2(A9[:, 1:-1], golden["A9"][:, 1:-1]) < TOL
+
+
+def test_assemble_mom(golden, st):
+    d, g = golden, _g(golden)
+    z = np.zeros((d["n1"], d["n0"]))
+    AU, AT, BU, BT = st.assemble_mom(g["alph"], g["beta"], g["gamma"], z, z, d["dt"], d["Re"], d["Pr"])
+    assert rel_l2(AU[:, 1:-1], d["AU"][:, 1:-1]) < TOL
+    assert rel_l2(AT[:, 1:-1], d["AT"][:, 1:-1]) < TOL
+    assert rel_l2(BU[:, :-1], d["BU"][:, :-1]) < TOL
+    assert rel_l2(BT[:, :-1], d["BT"][:, :-1]) < TOL
+
+
+def test_rhs(golden, st):
+    d, g = golden, _g(golden)
+    o = st.rhs(d["s2_uold"], d["s2_vold"], d["s2_Told"], d["s1_up"], d["s1_vp"], d["s1_p"], g["xix"],
+               g["xiy"], g["etx"], g["ety"], g["alph"], g["gamma"], d["BU"], d["BT"], d["Re"], d["Pr"],
+               float(d["s2_Ri"][0]), d["dt"])
+    for k in ("qu", "qv", "qt", "qup", "qvp"):
+        assert rel_l2(o[k][1:-1], d["s2_" + k][1:-1]) < TOL, k
+
+
+def test_face_div(golden, st):
+    d, g = golden, _g(golden)
+    q = st.face_div(d["s2_up"], d["s2_vp"], d["s1_p"], g["xix"], g["xiy"], g["etx"], g["ety"], d["dt"])
+    assert rel_l2(q[1:-1, :-1], d["s2_q"][1:-1, :-1]) < 1e-11  # q = div/dt amplifies rounding
+
+
+def test_sip9p_converged_solution(golden, st):
+    """Same 9-point system as the reference's sip9p, solved to a tight tolerance."""
+    d = golden
+    phi = np.zeros((d["n1"], d["n0"]))
+    it, rr = st.sip9p(d["A9"], phi, d["s2_q"], rtol=1e-13, maxit=500)
+    assert rr <= 1e-13
+    ref = d["s2_pcor_conv"]
+    assert rel_l2(phi[1:-1], ref[1:-1]) < 1e-10
+    np.testing.assert_array_equal(phi[:, -1], phi[:, 0])
+
+
+def test_gauss_converged_solution(golden, st):
+    d = golden
+    phi = d["s2_uold"].copy()
+    it, rr = st.gauss(d["AU"], phi, d["s2_qu"], rtol=1e-14, maxit=200)
+    assert rel_l2(phi[1:-1], d["s2_u_conv"][1:-1]) < 1e-11
+    # temperature call with the reference's (-1,+1)-for-(-1,+2) argument
+    phi = d["s2_Told"].copy()
+    st.gauss(d["AT"], phi, d["s2_qt"], rtol=1e-14, maxit=200, nw_for_nnw=True)
+    assert rel_l2(phi[1:-1], d["s2_T_conv"][1:-1]) < 1e-11
+
+
+def test_pcor_bc_correct(golden, st):
+    d, g = golden, _g(golden)
+    pc = st.pcor_bc(d["s2_pcor_one"], d["grid_xnox"], d["grid_xnoy"], d["uinf"], d["vinf"])
+    np.testing.assert_array_equal(pc, d["s2_pcor"])
+    z = np.zeros((d["n1"], d["n0"]))
+    u, v, p = st.correct(d["s2_us"], d["s2_vs"], d["s2_pcor"], g["xix"], g["xiy"], g["etx"], g["ety"],
+                         d["dt"], z, z, d["s1_p"])
+    for a, k in ((u, "u"), (v, "v"), (p, "p")):
+        assert rel_l2(a[1:-1], d["s2_" + k][1:-1]) < TOL, k
+
+
+def test_wall_pressure(golden, st):
+    d, g = golden, _g(golden)
+    p = d["s2_p"].copy()
+    p[0] = 0.0
+    z = np.zeros((d["n1"], d["n0"]))
+    out = st.wall_pressure(d["s2_u"], d["s2_v"], d["s2_T"], p, g["xix"], g["xiy"], g["etx"], g["ety"],
+                           g["alph"], g["beta"], g["gamma"], z, g["ajac"], d["Re"], float(d["s2_Ri"][0]))
+    assert rel_l2(out[0], d["s2_p"][0]) < 1e-12
+
+
+def test_dilation(golden, st):
+    d, g = golden, _g(golden)
+    dil, dmax = st.dilation(d["s2_u"], d["s2_v"], g["xix"], g["xiy"], g["etx"], g["ety"])
+    assert rel_l2(dil[1:-1], d["s2_dil"][1:-1]) < 1e-12
+    assert abs(dmax - d["s2_dmax"][0]) <= 1e-12 * abs(d["s2_dmax"][0])
+
+
+def test_nusselt(golden, st):
+    d, g = golden, _g(golden)
+    nu, prof = st.nusselt(d["s0_T"], g["ajac"], g["etx"], g["ety"], d["ar"])
+    assert abs(nu - d["s0_nu_8dp"][0]) <= 5e-9
+    np.testing.assert_allclose(prof, d["s0_dthdn_6sig"], rtol=1e-5, atol=1e-12)

@@ -1,0 +1,116 @@
+"""The cavity oracle pinned by closed forms and invariants (the reference has no cavity).
+
+- the pressure-correction operator: zero row and column sums (conservative,
+  Neumann), the 5-point Laplacian on a Cartesian grid, and its solver against
+  a dense solve;
+- the residual operator: exact on quadratics (fourth-order diffusion, hybrid
+  convection, central pressure gradient), zero on constants;
+- pure conduction gives Nu = 1 on both walls.
+"""
+import numpy as np
+import pytest
+
+from cfd_solver_amd.grid import cavity_grid
+from oracle.oracle import CavityOracle, cav_params
+
+
+def op_matrix(c, d):
+    n1, n0 = c.n1, c.n0
+    m0, m1 = n0 - 2, n1 - 2
+    M = np.zeros((m0 * m1, m0 * m1))
+    for k in range(m0 * m1):
+        e = np.zeros((n1, n0))
+        e[1 + k // m0, 1 + k % m0] = 1.0
+        M[:, k] = c.p_apply(d, e)[1:-1, 1:-1].ravel()
+    return M
+
+
+def test_p_operator_conservative_neumann():
+    g = cavity_grid(17, skew=0.15, beta=1.5)
+    c = CavityOracle(g, cav_params())
+    d = 0.5 + np.random.default_rng(1).random((17, 17))
+    M = op_matrix(c, d)
+    scale = np.abs(M).max()
+    assert np.abs(M.sum(axis=1)).max() < 1e-12 * scale
+    assert np.abs(M.sum(axis=0)).max() < 1e-12 * scale
+    assert np.linalg.matrix_rank(M, tol=1e-9 * scale) == M.shape[0] - 1
+
+
+def test_p_operator_cartesian_is_5point_neumann():
+    n = 9
+    g = cavity_grid(n, skew=0.0, beta=None)
+    c = CavityOracle(g, cav_params())
+    M = op_matrix(c, np.ones((n, n)))
+    h = 1.0 / (n - 1)
+    m = n - 2
+    T = np.zeros((m, m))
+    for k in range(m):
+        if k > 0:
+            T[k, k - 1] = 1
+        if k < m - 1:
+            T[k, k + 1] = 1
+        T[k, k] = -(int(k > 0) + int(k < m - 1))
+    expect = (np.kron(np.eye(m), T) + np.kron(T, np.eye(m))) / h ** 2
+    np.testing.assert_allclose(M, expect, rtol=1e-12, atol=1e-9)
+
+
+def test_p_solve_matches_dense():
+    n = 17
+    g = cavity_grid(n, skew=0.1, beta=1.5)
+    c = CavityOracle(g, cav_params())
+    rng = np.random.default_rng(20250101)
+    d = 0.5 + rng.random((n, n))
+    q = np.zeros((n, n))
+    q[1:-1, 1:-1] = rng.standard_normal((n - 2, n - 2))
+    q[1:-1, 1:-1] -= q[1:-1, 1:-1].mean()
+    x, it, rr = c.p_solve(d, q, rtol=1e-12, maxit=200)
+    assert rr <= 1e-12
+    M = op_matrix(c, d)
+    xd = np.linalg.lstsq(M, q[1:-1, 1:-1].ravel(), rcond=None)[0]
+    xs = x[1:-1, 1:-1].ravel()
+    xs = xs - xs.mean()
+    xd = xd - xd.mean()
+    assert np.linalg.norm(xs - xd) <= 1e-9 * np.linalg.norm(xd)
+
+
+def test_residual_exact_on_quadratic():
+    n = 33
+    g = cavity_grid(n, skew=0.0, beta=None, control=False)
+    prm = cav_params(Re=50.0, dt=0.3, Ri=0.0)
+    c = CavityOracle(g, prm)
+    X, Y = g.x, g.y
+    u = X ** 2 + Y ** 2
+    z = np.zeros_like(u)
+    c.set_fields(u, z, z, z)
+    Ru, Rv, RT, d = c.residuals()
+    expect = prm.dt * (-u * 2 * X + 4.0 / prm.Re)
+    np.testing.assert_allclose(Ru[1:-1, 1:-1], expect[1:-1, 1:-1], rtol=1e-9, atol=1e-12)
+    assert np.abs(Rv).max() == 0.0
+    # constants: every row of the reference operator sums to one
+    c.set_fields(np.full_like(u, 0.3), z, np.full_like(u, 0.7), z)
+    Ru, Rv, RT, d = c.residuals()
+    assert np.abs(Ru[1:-1, 1:-1]).max() < 1e-13
+    assert np.abs(RT[1:-1, 1:-1]).max() < 1e-13
+
+
+@pytest.mark.parametrize("skew,beta,tol", [(0.0, None, 1e-9), (0.1, 1.5, 1e-2)])
+def test_pure_conduction_nusselt(skew, beta, tol):
+    g = cavity_grid(17, skew=skew, beta=beta)
+    c = CavityOracle(g, cav_params(Ri=0.0, lid_u=0.0, mom_sweeps=8))
+    for _ in range(600):
+        st = c.step()
+    f = c.fields()
+    assert np.abs(f["u"]).max() < 1e-12 and np.abs(f["v"]).max() < 1e-12
+    assert abs(st.nu_hot - 1.0) < tol and abs(st.nu_cold - 1.0) < tol
+    if beta is None:
+        np.testing.assert_allclose(f["T"][:, 8], 1.0 - g.y[:, 8], atol=1e-9)
+
+
+def test_cavity_converges_config1_like():
+    g = cavity_grid(33, skew=0.05)
+    c = CavityOracle(g, cav_params(Re=100.0, Ri=1.0))
+    first = c.step()
+    for _ in range(399):
+        st = c.step()
+    assert st.mass_l1 / first.mass_l1 < 1e-4
+    assert np.isfinite(st.nu_hot) and 1.5 < st.nu_hot < 4.0

@@ -1,0 +1,155 @@
+"""The oracle's reference-mode restatement against the compiled reference's own arrays.
+
+Each stage is fed the reference's inputs from the fixture and must reproduce
+the reference's outputs to rounding (in practice bit for bit).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import lib
+
+INT = np.s_[1:-1, :]  # rows the reference writes
+
+
+def _g(d):
+    return {k: d["grid_" + k] for k in ["xix", "xiy", "etx", "ety", "alph", "beta", "gamma", "ajac", "x", "y"]}
+
+
+def test_assemble_p9(golden):
+    d = golden
+    g = _g(d)
+    A9 = np.zeros((9, d["n1"], d["n0"]))
+    lib().or_assemble_p9_ref(d["n0"], d["n1"], d["dxi"], d["deta"], g["xix"], g["xiy"], g["etx"], g["ety"], A9)
+    assert np.array_equal(A9[:, 1:-1, :], d["A9"][:, 1:-1, :])
+
+
+def test_assemble_mom(golden):
+    d = golden
+    g = _g(d)
+    n0, n1 = d["n0"], d["n1"]
+    AU, AT = np.zeros((25, n1, n0)), np.zeros((25, n1, n0))
+    BU, BT = np.zeros((6, n0)), np.zeros((6, n0))
+    z = np.zeros((n1, n0))
+    lib().or_assemble_mom_ref(n0, n1, d["dxi"], d["deta"], g["alph"], g["beta"], g["gamma"], z, z,
+                              d["dt"], d["Re"], d["Pr"], AU, AT, BU, BT)
+    assert np.array_equal(AU[:, 1:-1, :], d["AU"][:, 1:-1, :])
+    assert np.array_equal(AT[:, 1:-1, :], d["AT"][:, 1:-1, :])
+    assert np.array_equal(BU[:, :n0 - 1], d["BU"][:, :n0 - 1])
+    assert np.array_equal(BT[:, :n0 - 1], d["BT"][:, :n0 - 1])
+
+
+def test_rhs_step2(golden):
+    d = golden
+    g = _g(d)
+    n0, n1 = d["n0"], d["n1"]
+    out = [np.zeros((n1, n0)) for _ in range(7)]
+    lib().or_rhs_ref(n0, n1, d["dxi"], d["deta"], d["s2_uold"], d["s2_vold"], d["s2_Told"], d["s1_up"],
+                     d["s1_vp"], d["s1_p"], g["xix"], g["xiy"], g["etx"], g["ety"], g["alph"], g["gamma"],
+                     d["BU"], d["BT"], d["Re"], d["Pr"], float(d["s2_Ri"][0]), d["dt"], *out)
+    for k, o in zip(["uxi", "uet", "qu", "qv", "qt", "qup", "qvp"], out):
+        if k in ("uxi", "uet"):
+            continue  # the reference recomputes these later in the step
+        assert np.array_equal(o[INT], d["s2_" + k][INT]), k
+
+
+def test_gauss_single_call(golden):
+    d = golden
+    n0, n1 = d["n0"], d["n1"]
+    r = C.c_double()
+    for name, A, q, start, quirk, ref in [("u", d["AU"], d["s2_qu"], d["s2_uold"], 0, d["s2_us"]),
+                                          ("v", d["AU"], d["s2_qv"], d["s2_vold"], 0, d["s2_vs"]),
+                                          ("T", d["AT"], d["s2_qt"], d["s2_Told"], 1, d["s2_T"])]:
+        phi = start.copy()
+        lib().or_gauss_ref(n0, n1, np.ascontiguousarray(A), quirk, phi, q, 0.75e-2, 100000, C.byref(r))
+        assert np.array_equal(phi[INT], ref[INT]), name
+
+
+def test_face_div(golden):
+    d = golden
+    g = _g(d)
+    q = np.zeros((d["n1"], d["n0"]))
+    lib().or_face_div_ref(d["n0"], d["n1"], d["dxi"], d["deta"], d["s2_up"], d["s2_vp"], d["s1_p"],
+                          g["xix"], g["xiy"], g["etx"], g["ety"], d["dt"], q)
+    assert np.array_equal(q[1:-1, :-1], d["s2_q"][1:-1, :-1])
+
+
+def test_sip9p_single_call_and_converged(golden):
+    d = golden
+    n0, n1 = d["n0"], d["n1"]
+    r = C.c_double()
+    phi = np.zeros((n1, n0))
+    lib().or_sip9p_ref(n0, n1, d["A9"], phi, d["s2_q"], 0.75e-2, 100000, 0.92, C.byref(r))
+    assert np.array_equal(phi, d["s2_pcor_one"])
+    assert np.array_equal(phi[1:-1, :], d["s2_pcor"][1:-1, :])
+    phi = np.zeros((n1, n0))
+    for _ in range(7):
+        lib().or_sip9p_ref(n0, n1, d["A9"], phi, d["s2_q"], 0.75e-2, 100000, 0.92, C.byref(r))
+    assert np.array_equal(phi, d["s2_pcor_conv"])
+
+
+def test_pcor_bc_and_correct(golden):
+    d = golden
+    g = _g(d)
+    n0, n1 = d["n0"], d["n1"]
+    pc = d["s2_pcor_one"].copy()
+    lib().or_pcor_bc_ref(n0, n1, pc, d["grid_xnox"], d["grid_xnoy"], d["uinf"], d["vinf"])
+    assert np.array_equal(pc, d["s2_pcor"])
+    u, v, p = np.zeros((n1, n0)), np.zeros((n1, n0)), d["s1_p"].copy()
+    lib().or_correct_ref(n0, n1, d["dxi"], d["deta"], d["s2_us"], d["s2_vs"], d["s2_pcor"], g["xix"],
+                         g["xiy"], g["etx"], g["ety"], d["dt"], u, v, p)
+    assert np.array_equal(u[INT], d["s2_u"][INT])
+    assert np.array_equal(v[INT], d["s2_v"][INT])
+    assert np.array_equal(p[INT], d["s2_p"][INT])
+
+
+def test_wall_pressure(golden):
+    d = golden
+    g = _g(d)
+    n0, n1 = d["n0"], d["n1"]
+    p = d["s2_p"].copy()
+    p[0, :] = 0.0
+    z = np.zeros((n1, n0))
+    z0 = np.zeros(n0)
+    lib().or_wall_p_ref(n0, n1, d["dxi"], d["deta"], d["s2_u"], d["s2_v"], d["s2_T"], p, g["xix"], g["xiy"],
+                        g["etx"], g["ety"], g["alph"], g["beta"], g["gamma"], z, g["ajac"], z0, z0,
+                        d["Re"], float(d["s2_Ri"][0]))
+    assert np.array_equal(p[0], d["s2_p"][0])
+
+
+def test_dilation(golden):
+    d = golden
+    g = _g(d)
+    dil = np.zeros((d["n1"], d["n0"]))
+    dm = lib().or_dil_ref(d["n0"], d["n1"], d["dxi"], d["deta"], d["s2_u"], d["s2_v"], g["xix"], g["xiy"],
+                          g["etx"], g["ety"], dil)
+    assert np.array_equal(dil[INT], d["s2_dil"][INT])
+    assert dm == d["s2_dmax"][0]
+
+
+def test_nusselt(golden):
+    d = golden
+    g = _g(d)
+    prof = np.zeros(d["n0"])
+    nu = lib().or_nusselt_ref(d["n0"], d["n1"], d["dxi"], d["deta"], d["s0_T"], g["ajac"], g["etx"],
+                              g["ety"], d["ar"], prof.ctypes.data)
+    # COEFF_HIS.dat prints Nu with 8 decimals, SURF_DIST.dat dT/dn with 6 significant digits
+    assert abs(nu - d["s0_nu_8dp"][0]) <= 5e-9
+    np.testing.assert_allclose(prof, d["s0_dthdn_6sig"], rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.parametrize("mistake", ["sign", "index"])
+def test_golden_catches_planted_mistake(golden, mistake):
+    """A planted oracle-side mistake must break the golden comparison."""
+    d = golden
+    g = _g(d)
+    q = np.zeros((d["n1"], d["n0"]))
+    up = d["s2_up"].copy()
+    if mistake == "sign":
+        vp = -d["s2_vp"]
+    else:
+        vp = np.roll(d["s2_vp"], 1, axis=1)
+    lib().or_face_div_ref(d["n0"], d["n1"], d["dxi"], d["deta"], up, vp, d["s1_p"], g["xix"], g["xiy"],
+                          g["etx"], g["ety"], d["dt"], q)
+    assert not np.allclose(q[1:-1, :-1], d["s2_q"][1:-1, :-1], rtol=1e-6)
