@@ -153,3 +153,26 @@ def test_golden_catches_planted_mistake(golden, mistake):
     lib().or_face_div_ref(d["n0"], d["n1"], d["dxi"], d["deta"], up, vp, d["s1_p"], g["xix"], g["xiy"],
                           g["etx"], g["ety"], d["dt"], q)
     assert not np.allclose(q[1:-1, :-1], d["s2_q"][1:-1, :-1], rtol=1e-6)
+
+
+def test_golden_pcor_conv_equals_dense_solve(golden):
+    """The reference's repeated-SIP solution is the exact solution of its 9-point system."""
+    d = golden
+    A9, q, conv = d["A9"], d["s2_q"], d["s2_pcor_conv"]
+    n1, n0 = q.shape
+    P = n0 - 1
+
+    def k(i, j):
+        return (j - 1) * P + i
+
+    M = np.zeros((P * (n1 - 2), P * (n1 - 2)))
+    b = np.zeros(P * (n1 - 2))
+    for j in range(1, n1 - 1):
+        for i in range(P):
+            b[k(i, j)] = q[j, i]
+            for dj in (-1, 0, 1):
+                for di in (-1, 0, 1):
+                    if 1 <= j + dj <= n1 - 2:
+                        M[k(i, j), k((i + di) % P, j + dj)] += A9[(dj + 1) * 3 + di + 1, j, i]
+    x = np.linalg.solve(M, b).reshape(n1 - 2, P)
+    assert np.linalg.norm(x - conv[1:-1, :-1]) <= 1e-13 * np.linalg.norm(x)
