@@ -104,7 +104,9 @@ def main():
     value = cells * a.steps / elapsed
     # dominant kernel: largest device time per outer iteration
     name, (ms_tot, launches) = max(prof.items(), key=lambda kv: kv[1][0])
-    avg_ms = ms_tot / max(launches, 1)
+    if launches <= 0 or ms_tot <= 0.0:
+        raise RuntimeError(f"live kernel timing recorded nothing: {prof}")
+    avg_ms = ms_tot / launches
     nodes = n * n
     bytes_launch = KERNEL_WORDS[name] * nodes * 8
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9

@@ -53,7 +53,7 @@ SIGNATURES = {
     "cfd_device_count": ([C.POINTER(_i)], _i),
     "cfd_ref_assemble_mom": ([_i, _i, _d, _d, dp, dp, dp, dp, dp, _d, _d, _d, dp, dp, dp, dp], _i),
     "cfd_ref_assemble_p9": ([_i, _i, _d, _d, dp, dp, dp, dp, dp], _i),
-    "cfd_ref_rhs": ([_i, _i, _d, _d] + [dp] * 16 + [_d] * 4 + [dp] * 7, _i),
+    "cfd_ref_rhs": ([_i, _i, _d, _d] + [dp] * 14 + [_d] * 4 + [dp] * 7, _i),
     "cfd_gauss": ([_i, _i] + [dp] * 25 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
     "cfd_ref_face_div": ([_i, _i, _d, _d] + [dp] * 7 + [_d, dp], _i),
     "cfd_sip9p": ([_i, _i] + [dp] * 9 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),

@@ -795,7 +795,8 @@ void prof_begin(Case *c, int kind) {
     c->ev_kind.push_back(kind);
 }
 void prof_end(Case *c) {
-    if (!c->prof || c->ev_kind.size() * 2 != (size_t)c->ev_used + 1) return;
+    // a begin is pending when there is one more kind than recorded pairs
+    if (!c->prof || c->ev_kind.size() * 2 != (size_t)c->ev_used + 2) return;
     (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
     c->ev_used += 2;
 }

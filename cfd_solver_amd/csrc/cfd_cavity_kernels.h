@@ -151,4 +151,64 @@ __device__ __forceinline__ void p9_coefs(int n0, int n1, const T *__restrict__ x
     }
 }
 
+// Coefficients (normal kn, tangential kt) of the face between node P=(i, j) and its
+// east neighbour (dir 0) or north neighbour (dir 1): the per-face terms of
+// p9_coefs, in the same arithmetic.  A face touching a wall node is zero.
+template <typename T>
+__device__ __forceinline__ void p9_face(int n0, int n1, const T *__restrict__ xix, const T *__restrict__ xiy,
+                                        const T *__restrict__ etx, const T *__restrict__ ety,
+                                        const T *__restrict__ jac, const T *__restrict__ d, T h1, T h2, int i,
+                                        int j, int dir, T &kn, T &kt) {
+    kn = T(0);
+    kt = T(0);
+    const T half = T(0.5), four = T(4);
+    const size_t P = idx(i, j, n0);
+    const T xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P], dP = d[P];
+    if (dir == 0) {
+        if (i < 1 || i > n0 - 3 || j < 1 || j > n1 - 2) return;
+        const size_t E = P + 1;
+        const T xE = xix[E], yE = xiy[E], eE = etx[E], fE = ety[E], jE = jac[E];
+        const T mx = half * (jP * xP + jE * xE), my = half * (jP * yP + jE * yE);
+        const T a = mx * (xP + xE) + my * (yP + yE);
+        const T b = mx * (eP + eE) + my * (fP + fE);
+        const T de = half * (dP + d[E]);
+        kn = half * de * a / (h1 * h1);
+        kt = half * de * b / (four * h1 * h2);
+    } else {
+        if (j < 1 || j > n1 - 3 || i < 1 || i > n0 - 2) return;
+        const size_t Nn = P + n0;
+        const T xN = xix[Nn], yN = xiy[Nn], eN = etx[Nn], fN = ety[Nn], jN = jac[Nn];
+        const T mx = half * (jP * eP + jN * eN), my = half * (jP * fP + jN * fN);
+        const T a = mx * (eP + eN) + my * (fP + fN);
+        const T b = mx * (xP + xN) + my * (yP + yN);
+        const T dn = half * (dP + d[Nn]);
+        kn = half * dn * a / (h2 * h2);
+        kt = half * dn * b / (four * h1 * h2);
+    }
+}
+
+// Row (i, j) of the pressure-correction operator from stored face coefficients
+// (fe_*: face to the east of a node, fn_*: face to the north), accumulated in
+// the order of p9_coefs.
+template <typename T>
+__device__ __forceinline__ void p9_row_faces(int n0, const T *__restrict__ fen, const T *__restrict__ fet,
+                                             const T *__restrict__ fnn, const T *__restrict__ fnt, int i, int j,
+                                             T c[9]) {
+    const size_t P = idx(i, j, n0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = T(0);
+    T kn = fen[P], kt = fet[P];
+    c[5] += kn; c[4] -= kn;
+    c[8] += kt; c[7] += kt; c[1] -= kt; c[2] -= kt;
+    kn = fen[P - 1]; kt = fet[P - 1];
+    c[4] -= kn; c[3] += kn;
+    c[7] -= kt; c[6] -= kt; c[1] += kt; c[0] += kt;
+    kn = fnn[P]; kt = fnt[P];
+    c[7] += kn; c[4] -= kn;
+    c[8] += kt; c[6] -= kt; c[5] += kt; c[3] -= kt;
+    kn = fnn[P - n0]; kt = fnt[P - n0];
+    c[4] -= kn; c[1] += kn;
+    c[5] -= kt; c[3] += kt; c[2] -= kt; c[0] += kt;
+}
+
 }  // namespace cfd
