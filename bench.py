@@ -20,7 +20,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -30,14 +29,16 @@ sys.path.insert(0, ROOT)
 METRIC = "SIMPLE cell-iters/s at 1025² FP64 vs mixed; served solves/s + p99 at 1e-8 res"
 HBM_PEAK_GBS = 8000.0
 
-# algorithmic words per node per launch: arrays read + arrays written
-KERNEL_WORDS = {
-    "p_apply": 7 + 1,     # x, xi_x, xi_y, eta_x, eta_y, J, d -> y
-    "jacobi": 13 + 3,     # R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
-    "residual": 13 + 6,   # u,v,T,p, 4 metrics, alph,beta,gamma,P,Q -> R_u,R_v,R_T,d,U,V
-    "face_div": 9 + 1,    # u^,v^,p,d, 4 metrics, J -> q
-    "mg_smooth": 9 + 1,   # x,b,1/diag, 4 metrics, J, d -> x
-}
+# algorithmic bytes per node per launch: (arrays read + arrays written) x element size
+def kernel_bytes(name, mixed):
+    w = 4 if mixed else 8  # predictor and p' inner solve precision
+    return {
+        "p_apply": 6 * w,          # x, 4 face coefficients -> y
+        "mg_smooth": 8 * w,        # x, b, 1/diag, 4 face coefficients -> x
+        "jacobi": 16 * w,          # R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
+        "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
+        "face_div": 10 * 8,        # u^,v^,p,d, 4 metrics, J -> q (FP64)
+    }[name]
 
 
 def parse():
@@ -48,7 +49,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1025)
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     return ap.parse_args()
 
 
@@ -76,57 +77,69 @@ def cpu_baseline(grid, n_iters, kw):
                       f"C oracle (gcc -O3), 1 of {os.cpu_count()} host threads ({model})"}
 
 
+def run_gpu(grid, mixed, steps, warmup):
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
+    from cfd_solver_amd.cavity import CavityCase
+    case = CavityCase(grid, Re=400.0, Ri=1.0, Pr=0.71, precision=CFD_MIXED if mixed else CFD_FP64)
+    for _ in range(warmup):
+        case.step()
+    case.profile(True)
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(steps):
+        st = case.step()
+    elapsed = time.perf_counter() - t0
+    prof = case.profile_read()
+    case.profile(False)
+    n0, n1 = grid.n0, grid.n1
+    value = (n0 - 2) * (n1 - 2) * steps / elapsed
+    name, (ms_tot, launches) = max(prof.items(), key=lambda kv: kv[1][0])
+    if launches <= 0 or ms_tot <= 0.0:
+        raise RuntimeError(f"live kernel timing recorded nothing: {prof}")
+    avg_ms = ms_tot / launches
+    bytes_launch = kernel_bytes(name, mixed) * n0 * n1
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_ms": round(avg_ms, 5), "launches": launches, "bytes_per_launch": bytes_launch,
+            "per_kernel_ms_per_step": {k: round(v[0] / steps, 4) for k, v in prof.items()},
+            "per_kernel_GBs": {k: round(kernel_bytes(k, mixed) * n0 * n1 / (v[0] / v[1] * 1e-3) / 1e9, 1)
+                               for k, v in prof.items() if v[1] > 0 and v[0] > 0}}
+    info = {"mass_res_last": st.mass_res, "p_iters_last": st.p_iters, "nu_hot_last": st.nu_hot}
+    case.close()
+    return value, elapsed / steps * 1e3, roof, info
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU sharding is not "
               "built in this version", file=sys.stderr)
         sys.exit(2)
-    from cfd_solver_amd.cavity import CavityCase
     from cfd_solver_amd.grid import cavity_grid
-
     n = a.n
     grid = cavity_grid(n, skew=0.1, beta=1.5)
-    phys = dict(Re=400.0, Ri=1.0, Pr=0.71)
-    case = CavityCase(grid, **phys)
-    for _ in range(a.warmup):
-        case.step()
-    case.profile(True)
-    t0 = time.perf_counter()
-    st = None
-    for _ in range(a.steps):
-        st = case.step()
-    elapsed = time.perf_counter() - t0
-    prof = case.profile_read()
-    case.profile(False)
-
-    cells = (n - 2) * (n - 2)
-    value = cells * a.steps / elapsed
-    # dominant kernel: largest device time per outer iteration
-    name, (ms_tot, launches) = max(prof.items(), key=lambda kv: kv[1][0])
-    if launches <= 0 or ms_tot <= 0.0:
-        raise RuntimeError(f"live kernel timing recorded nothing: {prof}")
-    avg_ms = ms_tot / launches
-    nodes = n * n
-    bytes_launch = KERNEL_WORDS[name] * nodes * 8
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-            "avg_launch_ms": round(avg_ms, 5), "launches": launches,
-            "bytes_per_launch": bytes_launch,
-            "per_kernel_ms_per_iter": {k: round(v[0] / a.steps, 4) for k, v in prof.items()}}
+    res = {}
+    for prec in (["fp64", "mixed"] if a.precision == "both" else [a.precision]):
+        res[prec] = run_gpu(grid, prec == "mixed", a.steps, a.warmup)
+    main_prec = "mixed" if "mixed" in res else "fp64"
+    value, ms_step, roof, info = res[main_prec]
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))
+    cfg = {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, Ri 1, Pr 0.71, "
+                       f"SIMPLE (0.7/0.3), one outer iteration per step, {main_prec} path",
+           "grid": [n, n], "precision": main_prec, **info}
+    if "fp64" in res and "mixed" in res:
+        cfg["fp64_value"] = res["fp64"][0]
+        cfg["fp64_ms_per_step"] = res["fp64"][1]
+        cfg["fp64_roofline"] = res["fp64"][2]
+        cfg["mixed_over_fp64"] = res["mixed"][0] / res["fp64"][0]
     out = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, "
-                               f"Ri 1, Pr 0.71, FP64 SIMPLE (0.7/0.3), 1 outer iteration per step",
-                   "grid": [n, n], "precision": a.precision, "mass_res_last": st.mass_res if st else None,
-                   "p_iters_last": st.p_iters if st else None},
-        "roofline": roof, "cpu_baseline": cpu, "host": platform.node(),
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if main_prec == "mixed" else "f64", "data": "synthetic",
+        "config": cfg, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -18,49 +18,40 @@
 #include <cstring>
 #include <vector>
 
-#include "cfd_cavity_kernels.h"
+#include "cfd_psolve.h"
 
 namespace cfd {
 
-constexpr int MAXLEV = 16;
+constexpr int MAXLEV = PS_MAXLEV;
 constexpr int CFD_PROF_KINDS = 8;
 constexpr int PROF_EVENTS = 4096;
 constexpr int NT = BX * BY;  // threads per 2-D block
-
-// device scalar slots
-// (pairs reduced together must be adjacent: BB/RR, TS/TT, ENERGY/MOM)
-enum {
-    S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RV, S_TS, S_TT, S_SS, S_BB, S_RR, S_SUM, S_R0R,
-    S_MASS, S_ENERGY, S_MOM, S_DMAX, S_NUH, S_NUC, S_NSCAL
-};
-
-template <typename T>
-struct Level {
-    int n0 = 0, n1 = 0;
-    double h1 = 0, h2 = 0;
-    T *xix = nullptr, *xiy = nullptr, *etx = nullptr, *ety = nullptr, *jac = nullptr;
-    T *d = nullptr, *invd = nullptr;
-    T *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;
-    bool owns_metrics = false;
-};
 
 struct Case {
     int n0 = 0, n1 = 0;
     size_t N = 0;
     double h1 = 0, h2 = 0;
     cfd_params prm{};
+    bool mixed = false;
     hipStream_t s = nullptr;
     int device = 0;
     // grid
     double *xg, *yg, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *ajac, *p1, *q1;
-    // state and work
+    // state and FP64 work
     double *u, *v, *T, *p;
-    double *U, *V, *Ru, *Rv, *RT, *d;
-    double *jb[6];  // Jacobi ping-pong: du dv dT | tu tv tT
+    double *d;
     double *us, *vs, *uh, *vh, *q, *pc, *scratch;
-    double *bi[8];  // r r0 pv vv s t ph sh
-    Level<double> lev[MAXLEV];
+    // predictor arrays in the predictor precision (double, or float in the mixed path)
+    void *Ru, *Rv, *RT, *U, *V;
+    void *jb[6];                           // Jacobi ping-pong: du dv dT | tu tv tT
+    float *a32[5] = {nullptr};             // alph, beta, gamma, p1, q1 in FP32 (mixed)
+    // pressure correction
+    Geom geom[MAXLEV];
     int nlev = 0;
+    PSolver<double> *ps64 = nullptr;
+    PSolver<float> *ps32 = nullptr;
+    float *rf = nullptr, *ef = nullptr;    // mixed: FP32 right-hand side / correction
+    double *r64 = nullptr;                 // mixed: FP64 defect
     double *part = nullptr;  // reduction partials: 4 x pstride
     int pstride = 0;
     double *scal = nullptr;  // device scalars
@@ -70,7 +61,7 @@ struct Case {
     int iter = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
-    // live kernel timing: event pairs around level-0 launches of the hot kernels
+    // live kernel timing: event pairs around fine-level launches of the hot kernels
     bool prof = false;
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_kind;   // kind of each pending pair
@@ -81,17 +72,6 @@ struct Case {
 
 // ------------------------------------------------------------------------
 // reductions
-
-// sum of `nparts` partials for each of `nv` arrays (stride pstride) -> scal[dst + k]
-__global__ void k_reduce_sum(const double *__restrict__ part, int pstride, int nparts, int nv,
-                             double *__restrict__ scal, int dst) {
-    const int k = blockIdx.x;
-    if (k >= nv) return;
-    double s = 0.0;
-    for (int t = threadIdx.x; t < nparts; t += blockDim.x) s += part[(size_t)k * pstride + t];
-    s = block_sum<256>(s);
-    if (threadIdx.x == 0) scal[dst + k] = s;
-}
 
 __global__ void k_reduce_max(const double *__restrict__ part, int nparts, double *__restrict__ scal,
                              int dst) {
@@ -110,14 +90,15 @@ struct Phys {
     double Re, Ri, Pr, dt, alpha_u, alpha_t, alpha_p;
 };
 
+template <typename TR>
 __global__ __launch_bounds__(NT) void k_residual(
     KGrid g, Phys ph, const double *__restrict__ u, const double *__restrict__ v,
     const double *__restrict__ T, const double *__restrict__ p, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
-    const double *__restrict__ p1, const double *__restrict__ q1, double *__restrict__ Ru,
-    double *__restrict__ Rv, double *__restrict__ RT, double *__restrict__ dd, double *__restrict__ Ua,
-    double *__restrict__ Va, double *__restrict__ part, int pstride) {
+    const double *__restrict__ p1, const double *__restrict__ q1, TR *__restrict__ Ru,
+    TR *__restrict__ Rv, TR *__restrict__ RT, double *__restrict__ dd, TR *__restrict__ Ua,
+    TR *__restrict__ Va, double *__restrict__ part, int pstride) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     const double h1 = g.h1, h2 = g.h2;
@@ -155,11 +136,11 @@ __global__ __launch_bounds__(NT) void k_residual(
         const double rv = dt * (-conv[1] - dp_dy + ph.Ri * T[P]) -
                           aref_offdiag(v, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re, hyb);
         const double rt = -dt * conv[2] - aref_offdiag(T, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re * ph.Pr, hyb);
-        Ru[P] = ru;
-        Rv[P] = rv;
-        RT[P] = rt;
-        Ua[P] = UP;
-        Va[P] = VP;
+        Ru[P] = (TR)ru;
+        Rv[P] = (TR)rv;
+        RT[P] = (TR)rt;
+        Ua[P] = (TR)UP;
+        Va[P] = (TR)VP;
         double cl[9];
         alo_coefs<double>(al, be, ga, pp, qq, UP, VP, h1, h2, dt, ph.Re, cl);
         dd[P] = dt * ph.alpha_u / cl[4];
@@ -192,24 +173,24 @@ __global__ void k_d_border(int n0, int n1, double *__restrict__ d) {
     }
 }
 
-// one Jacobi sweep of (A_lo/alpha) delta = R for u, v (shared operator) and T
-template <bool FIRST>
+// one Jacobi sweep of (A_lo/alpha) delta = R for u, v (shared operator) and T,
+// in the predictor precision TR
+template <typename TR, bool FIRST>
 __global__ __launch_bounds__(NT) void k_jacobi(
-    KGrid g, Phys ph, const double *__restrict__ Ru, const double *__restrict__ Rv,
-    const double *__restrict__ RT, const double *__restrict__ Ua, const double *__restrict__ Va,
-    const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
-    const double *__restrict__ p1, const double *__restrict__ q1, const double *__restrict__ du,
-    const double *__restrict__ dv, const double *__restrict__ dT, double *__restrict__ nu,
-    double *__restrict__ nv, double *__restrict__ nT) {
+    KGrid g, Phys ph, const TR *__restrict__ Ru, const TR *__restrict__ Rv, const TR *__restrict__ RT,
+    const TR *__restrict__ Ua, const TR *__restrict__ Va, const TR *__restrict__ alph, const TR *__restrict__ beta,
+    const TR *__restrict__ gamma, const TR *__restrict__ p1, const TR *__restrict__ q1, const TR *__restrict__ du,
+    const TR *__restrict__ dv, const TR *__restrict__ dT, TR *__restrict__ nu, TR *__restrict__ nv,
+    TR *__restrict__ nT) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
     const size_t P = idx(i, j, n0);
-    double cu[9], ct[9];
-    alo_coefs<double>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], g.h1, g.h2, ph.dt, ph.Re, cu);
-    alo_coefs<double>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], g.h1, g.h2, ph.dt,
-                      ph.Re * ph.Pr, ct);
-    double su = Ru[P], sv = Rv[P], st = RT[P];
+    const TR h1 = (TR)g.h1, h2 = (TR)g.h2, dt = (TR)ph.dt;
+    TR cu[9], ct[9];
+    alo_coefs<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], h1, h2, dt, (TR)ph.Re, cu);
+    alo_coefs<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], h1, h2, dt, (TR)(ph.Re * ph.Pr), ct);
+    TR su = Ru[P], sv = Rv[P], st = RT[P];
     if (!FIRST) {
 #pragma unroll
         for (int dj = -1; dj <= 1; dj++)
@@ -223,15 +204,16 @@ __global__ __launch_bounds__(NT) void k_jacobi(
                 st -= ct[k] * dT[Q];
             }
     }
-    nu[P] = su * ph.alpha_u / cu[4];
-    nv[P] = sv * ph.alpha_u / cu[4];
-    nT[P] = st * ph.alpha_t / ct[4];
+    nu[P] = su * (TR)ph.alpha_u / cu[4];
+    nv[P] = sv * (TR)ph.alpha_u / cu[4];
+    nT[P] = st * (TR)ph.alpha_t / ct[4];
 }
 
 // u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u
+template <typename TR>
 __global__ __launch_bounds__(NT) void k_star(
-    KGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ du,
-    const double *__restrict__ dv, const double *__restrict__ dT, double *__restrict__ T,
+    KGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
+    const TR *__restrict__ dv, const TR *__restrict__ dT, double *__restrict__ T,
     const double *__restrict__ p, const double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     double *__restrict__ us, double *__restrict__ vs, double *__restrict__ uh, double *__restrict__ vh) {
@@ -244,10 +226,10 @@ __global__ __launch_bounds__(NT) void k_star(
         vs[P] = v[P];
         return;
     }
-    const double a = u[P] + du[P], b = v[P] + dv[P];
+    const double a = u[P] + (double)du[P], b = v[P] + (double)dv[P];
     us[P] = a;
     vs[P] = b;
-    T[P] = T[P] + dT[P];
+    T[P] = T[P] + (double)dT[P];
     const double dp_dxi = (p[P + 1] - p[P - 1]) / (2.0 * g.h1);
     const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) / (2.0 * g.h2);
     uh[P] = a + d[P] * (xix[P] * dp_dxi + etx[P] * dp_de);
@@ -507,25 +489,19 @@ __global__ void k_nusselt(KGrid g, const double *__restrict__ T, const double *_
 }
 
 // ------------------------------------------------------------------------
-// pressure-correction operator, multigrid and BiCGStab (template on precision)
+// FP64 pressure-correction operator straight from the metrics (the defect of
+// the mixed path's iterative refinement, and cfd_case_p_apply)
 
-template <typename T>
-struct LevArgs {
+struct MArgs {
     int n0, n1;
-    T h1, h2;
-    const T *xix, *xiy, *etx, *ety, *jac, *d;
+    double h1, h2;
+    const double *xix, *xiy, *etx, *ety, *jac, *d;
 };
 
-template <typename T>
-LevArgs<T> largs(const Level<T> &L) {
-    return LevArgs<T>{L.n0, L.n1, (T)L.h1, (T)L.h2, L.xix, L.xiy, L.etx, L.ety, L.jac, L.d};
-}
-
-template <typename T>
-__device__ __forceinline__ T p9_row(const LevArgs<T> &a, const T *__restrict__ x, int i, int j) {
-    T c[9];
-    p9_coefs<T>(a.n0, a.n1, a.xix, a.xiy, a.etx, a.ety, a.jac, a.d, a.h1, a.h2, i, j, c);
-    T s = T(0);
+__device__ __forceinline__ double mrow(const MArgs &a, const double *__restrict__ x, int i, int j) {
+    double c[9];
+    p9_coefs<double>(a.n0, a.n1, a.xix, a.xiy, a.etx, a.ety, a.jac, a.d, a.h1, a.h2, i, j, c);
+    double s = 0.0;
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++)
 #pragma unroll
@@ -536,223 +512,15 @@ __device__ __forceinline__ T p9_row(const LevArgs<T> &a, const T *__restrict__ x
     return s;
 }
 
-// y = L x (interior), 0 on the border
-template <typename T>
-__global__ __launch_bounds__(NT) void k_p_apply(LevArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
+// y = L x, or y = b - L x when b != nullptr; 0 on the border
+__global__ __launch_bounds__(NT) void k_mapply(MArgs a, const double *__restrict__ x, const double *__restrict__ b,
+                                               double *__restrict__ y) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { y[P] = T(0); return; }
-    y[P] = p9_row<T>(a, x, i, j);
-}
-
-// r = b - L x (interior), 0 on the border
-template <typename T>
-__global__ __launch_bounds__(NT) void k_p_resid(LevArgs<T> a, const T *__restrict__ x, const T *__restrict__ b,
-                                                T *__restrict__ r) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.n0 || j >= a.n1) return;
-    const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { r[P] = T(0); return; }
-    r[P] = b[P] - p9_row<T>(a, x, i, j);
-}
-
-// inverse diagonal of the clamped operator (oracle: p9_diag)
-template <typename T>
-__global__ __launch_bounds__(NT) void k_p_invdiag(LevArgs<T> a, T *__restrict__ invd) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > a.n0 - 2 || j < 1 || j > a.n1 - 2) return;
-    T c[9];
-    p9_coefs<T>(a.n0, a.n1, a.xix, a.xiy, a.etx, a.ety, a.jac, a.d, a.h1, a.h2, i, j, c);
-    T s = T(0);
-    for (int dj = -1; dj <= 1; dj++)
-        for (int di = -1; di <= 1; di++)
-            if (clampi(i + di, 1, a.n0 - 2) == i && clampi(j + dj, 1, a.n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
-    invd[idx(i, j, a.n0)] = T(1) / s;
-}
-
-// Jacobi sweep: xo = xi + w (b - L xi) invd (FIRST: xo = w b invd)
-template <typename T, bool FIRST>
-__global__ __launch_bounds__(NT) void k_p_smooth(LevArgs<T> a, T w, const T *__restrict__ xi,
-                                                 const T *__restrict__ b, const T *__restrict__ invd,
-                                                 T *__restrict__ xo) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.n0 || j >= a.n1) return;
-    const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { xo[P] = T(0); return; }
-    if (FIRST) {
-        xo[P] = w * b[P] * invd[P];
-    } else {
-        xo[P] = xi[P] + w * (b[P] - p9_row<T>(a, xi, i, j)) * invd[P];
-    }
-}
-
-// full weighting of the fine residual into the coarse right-hand side
-template <typename T>
-__global__ __launch_bounds__(NT) void k_restrict(int n0, const T *__restrict__ r, int m0, int m1,
-                                                 T *__restrict__ bc) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= m0 || J >= m1) return;
-    const size_t Pc = idx(I, J, m0);
-    if (I == 0 || J == 0 || I == m0 - 1 || J == m1 - 1) { bc[Pc] = T(0); return; }
-    const int i = 2 * I, j = 2 * J;
-    bc[Pc] = (T(4) * r[idx(i, j, n0)] +
-              T(2) * (r[idx(i - 1, j, n0)] + r[idx(i + 1, j, n0)] + r[idx(i, j - 1, n0)] + r[idx(i, j + 1, n0)]) +
-              (r[idx(i - 1, j - 1, n0)] + r[idx(i + 1, j - 1, n0)] + r[idx(i - 1, j + 1, n0)] +
-               r[idx(i + 1, j + 1, n0)])) /
-             T(16);
-}
-
-// bilinear prolongation of the coarse correction, added on the fine interior
-template <typename T>
-__global__ __launch_bounds__(NT) void k_prolong_add(int n0, int n1, T *__restrict__ x, int m0, int m1,
-                                                    const T *__restrict__ xc) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
-    const int I = i >> 1, J = j >> 1;
-#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
-    T e;
-    if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
-    else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
-    else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
-    else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
-#undef XC
-    x[idx(i, j, n0)] += e;
-}
-
-// interior sum of a vector -> partials
-template <typename T>
-__global__ __launch_bounds__(NT) void k_sum_int(int n0, int n1, const T *__restrict__ v, double *__restrict__ part) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    double s = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) s = (double)v[idx(i, j, n0)];
-    s = block_sum<NT>(s);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
-}
-
-// v -= scal[slot] / count on the interior
-template <typename T>
-__global__ __launch_bounds__(NT) void k_sub_mean(int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
-                                                 int slot) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
-    const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
-    v[idx(i, j, n0)] = (T)((double)v[idx(i, j, n0)] - m);
-}
-
-template <typename T>
-__global__ void k_inject(int n0, const T *__restrict__ f, int m0, int m1, T *__restrict__ c) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= m0 || J >= m1) return;
-    c[idx(I, J, m0)] = f[idx(2 * I, 2 * J, n0)];
-}
-
-// --- BiCGStab vector kernels (FP64 vectors; scalars read from device memory)
-
-// dot products over the interior -> partials (up to 2 pairs)
-__global__ __launch_bounds__(NT) void k_dot2(int n0, int n1, const double *__restrict__ a, const double *__restrict__ b,
-                                             const double *__restrict__ c, const double *__restrict__ d2,
-                                             double *__restrict__ part, int pstride) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    double s1 = 0.0, s2 = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-        const size_t P = idx(i, j, n0);
-        s1 = a[P] * b[P];
-        if (c) s2 = c[P] * d2[P];
-    }
-    s1 = block_sum<NT>(s1);
-    s2 = block_sum<NT>(s2);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[block_lin()] = s1;
-        part[pstride + block_lin()] = s2;
-    }
-}
-
-// pv = r (first) or r + beta (pv - omega vv)
-__global__ __launch_bounds__(NT) void k_bicg_p(int n0, int n1, bool first, const double *__restrict__ scal,
-                                               const double *__restrict__ r, const double *__restrict__ vv,
-                                               double *__restrict__ pv) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    if (first) { pv[P] = r[P]; return; }
-    const double beta = (scal[S_RHO] / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
-    pv[P] = r[P] + beta * (pv[P] - scal[S_OMEGA] * vv[P]);
-}
-
-// alpha = rho / (r0, v) (from scal[S_RV]); s = r - alpha v; partial (s, s)
-__global__ __launch_bounds__(NT) void k_bicg_s(int n0, int n1, double *__restrict__ scal, const double *__restrict__ r,
-                                               const double *__restrict__ vv, double *__restrict__ s,
-                                               double *__restrict__ part) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const double alpha = scal[S_RHO] / scal[S_RV];
-    double ss = 0.0;
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
-        const double sv = r[P] - alpha * vv[P];
-        s[P] = sv;
-        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) ss = sv * sv;
-    }
-    ss = block_sum<NT>(ss);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = ss;
-    if (block_lin() == 0 && threadIdx.x == 0 && threadIdx.y == 0) scal[S_ALPHA] = alpha;
-}
-
-// x += alpha ph (early exit on ||s||)
-__global__ __launch_bounds__(NT) void k_bicg_x1(int n0, int n1, const double *__restrict__ scal,
-                                                const double *__restrict__ ph, double *__restrict__ x) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    x[P] += scal[S_ALPHA] * ph[P];
-}
-
-// omega = (t,s)/(t,t); x += alpha ph + omega sh; r = s - omega t; partials (r,r), (r0,r)
-__global__ __launch_bounds__(NT) void k_bicg_xr(int n0, int n1, const double *__restrict__ scal,
-                                                const double *__restrict__ ph, const double *__restrict__ sh,
-                                                const double *__restrict__ s, const double *__restrict__ t,
-                                                const double *__restrict__ r0, double *__restrict__ x,
-                                                double *__restrict__ r, double *__restrict__ part, int pstride) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const double tt = scal[S_TT];
-    const double omega = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
-    const double alpha = scal[S_ALPHA];
-    double rr = 0.0, r0r = 0.0;
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
-        x[P] += alpha * ph[P] + omega * sh[P];
-        const double rv = s[P] - omega * t[P];
-        r[P] = rv;
-        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-            rr = rv * rv;
-            r0r = r0[P] * rv;
-        }
-    }
-    rr = block_sum<NT>(rr);
-    r0r = block_sum<NT>(r0r);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[block_lin()] = rr;
-        part[pstride + block_lin()] = r0r;
-    }
-}
-
-__global__ void k_set_omega(double *__restrict__ scal) {
-    const double tt = scal[S_TT];
-    scal[S_OMEGA] = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
-}
-
-// rho_old = rho; rho = (r0, r) of the new r
-__global__ void k_rotate_rho(double *__restrict__ scal) {
-    scal[S_RHO_OLD] = scal[S_RHO];
-    scal[S_RHO] = scal[S_R0R];
-}
-
-// start values of the BiCGStab scalars: rho = (r, r), rho_old = alpha = omega = 1
-__global__ void k_init_scal(double *__restrict__ scal) {
-    scal[S_RHO] = scal[S_RR];
-    scal[S_RHO_OLD] = 1.0;
-    scal[S_ALPHA] = 1.0;
-    scal[S_OMEGA] = 1.0;
+    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { y[P] = 0.0; return; }
+    const double v = mrow(a, x, i, j);
+    y[P] = b ? b[P] - v : v;
 }
 
 }  // namespace cfd
@@ -775,22 +543,15 @@ int dalloc(Case *c, T **p, size_t n) {
     return CFD_OK;
 }
 
-int nblocks2d(int n0, int n1) {
-    dim3 g = grid2d(n0, n1);
-    return (int)(g.x * g.y);
-}
+#define RET(x) CFD_RET(x)
 
-// reduce `nv` partial arrays (stride pstride) into scal[dst..]
-int reduce_sum(Case *c, int nparts, int nv, int dst) {
-    k_reduce_sum<<<nv, 256, 0, c->s>>>(c->part, c->pstride, nparts, nv, c->scal, dst);
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
-}
-
-// kernel kinds timed live: 0 p' apply/residual (fine level), 1 predictor Jacobi sweep,
+// ---- live kernel timing
+// kinds: 0 p' operator apply (fine level), 1 predictor Jacobi sweep,
 // 2 residual/RHS, 3 face divergence, 4 multigrid smoothing sweep (fine level)
 void prof_begin(Case *c, int kind) {
-    if (!c->prof || c->ev_used + 2 > (int)c->ev.size()) return;
+    if (!c->prof) return;
+    if (c->ev_kind.size() * 2 > (size_t)c->ev_used) c->ev_kind.pop_back();  // drop a dangling begin
+    if (c->ev_used + 2 > (int)c->ev.size()) return;
     (void)hipEventRecord(c->ev[c->ev_used], c->s);
     c->ev_kind.push_back(kind);
 }
@@ -800,7 +561,6 @@ void prof_end(Case *c) {
     (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
     c->ev_used += 2;
 }
-// after a stream synchronisation: fold the recorded pairs into the totals
 void prof_drain(Case *c) {
     if (!c->prof) return;
     for (int k = 0; k + 1 < c->ev_used; k += 2) {
@@ -814,224 +574,124 @@ void prof_drain(Case *c) {
     c->ev_used = 0;
     c->ev_kind.clear();
 }
-// a begin without its end (pool exhausted mid-pair) is dropped at the next begin
-#define PROF_B(c, kind)                                                        \
-    do {                                                                       \
-        if ((c)->prof && (c)->ev_kind.size() * 2 > (size_t)(c)->ev_used)       \
-            (c)->ev_kind.pop_back();                                           \
-        prof_begin((c), (kind));                                               \
-    } while (0)
-#define PROF_E(c) prof_end(c)
+void cb_begin(void *o, int kind) { prof_begin(static_cast<Case *>(o), kind); }
+void cb_end(void *o) { prof_end(static_cast<Case *>(o)); }
+void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
+
+PsCtx ctx(Case *c) {
+    return PsCtx{c->s, c->part, c->pstride, c->scal, c->hscal, cb_begin, cb_end, cb_drain, c, &c->allocs, &c->bytes};
+}
+
+int reduce_sum(Case *c, int nparts, int nv, int dst) {
+    PsCtx cx = ctx(c);
+    return ps_reduce(cx, nparts, nv, dst);
+}
 
 int sync_scal(Case *c) {
-    CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    CFD_HIP(hipStreamSynchronize(c->s));
-    prof_drain(c);
-    return CFD_OK;
+    PsCtx cx = ctx(c);
+    return ps_sync(cx);
 }
 
-#define RET(x)                      \
-    do {                            \
-        int r_ = (x);               \
-        if (r_ != CFD_OK) return r_; \
-    } while (0)
-
-template <typename T>
-int project_mean(Case *c, int n0, int n1, T *v) {
-    const int nb = nblocks2d(n0, n1);
-    k_sum_int<T><<<grid2d(n0, n1), dim3(BX, BY), 0, c->s>>>(n0, n1, v, c->part);
-    CFD_LAUNCH_CHECK();
-    RET(reduce_sum(c, nb, 1, S_SUM));
-    k_sub_mean<T><<<grid2d(n0, n1), dim3(BX, BY), 0, c->s>>>(n0, n1, v, c->scal, S_SUM);
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
+MArgs margs(const Case *c) {
+    return MArgs{c->n0, c->n1, c->h1, c->h2, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->d};
 }
 
-// smoothing sweeps on level L; x holds the result (x may be swapped with t)
-int mg_smooth(Case *c, Level<double> &L, int sweeps, bool from_zero) {
-    const double w = c->prm.mg_omega;
-    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-    const LevArgs<double> a = largs(L);
-    for (int s = 0; s < sweeps; s++) {
-        if (from_zero && s == 0) {
-            k_p_smooth<double, true><<<gr, bl, 0, c->s>>>(a, w, L.x, L.b, L.invd, L.x);
-        } else {
-            const bool fine = (&L == &c->lev[0]);
-            if (fine) PROF_B(c, 4);
-            k_p_smooth<double, false><<<gr, bl, 0, c->s>>>(a, w, L.x, L.b, L.invd, L.t);
-            if (fine) PROF_E(c);
-            std::swap(L.x, L.t);
-        }
-        CFD_LAUNCH_CHECK();
-    }
-    return CFD_OK;
-}
-
-int mg_vcycle(Case *c, int l) {
-    Level<double> &L = c->lev[l];
-    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-    if (l == c->nlev - 1) {
-        if (c->prm.mg_coarse <= 0) {
-            CFD_HIP(hipMemsetAsync(L.x, 0, (size_t)L.n0 * L.n1 * sizeof(double), c->s));
-            return CFD_OK;
-        }
-        return mg_smooth(c, L, c->prm.mg_coarse, true);
-    }
-    if (c->prm.mg_pre > 0) {
-        RET(mg_smooth(c, L, c->prm.mg_pre, true));
-    } else {
-        CFD_HIP(hipMemsetAsync(L.x, 0, (size_t)L.n0 * L.n1 * sizeof(double), c->s));
-    }
-    k_p_resid<double><<<gr, bl, 0, c->s>>>(largs(L), L.x, L.b, L.r);
-    CFD_LAUNCH_CHECK();
-    Level<double> &C = c->lev[l + 1];
-    k_restrict<double><<<grid2d(C.n0, C.n1), bl, 0, c->s>>>(L.n0, L.r, C.n0, C.n1, C.b);
-    CFD_LAUNCH_CHECK();
-    RET(project_mean(c, C.n0, C.n1, C.b));
-    RET(mg_vcycle(c, l + 1));
-    k_prolong_add<double><<<gr, bl, 0, c->s>>>(L.n0, L.n1, L.x, C.n0, C.n1, C.x);
-    CFD_LAUNCH_CHECK();
-    return mg_smooth(c, L, c->prm.mg_post, false);
-}
-
-// z = M^-1 y (V-cycle on level 0)
-int precond(Case *c, const double *y, double *z) {
-    Level<double> &L = c->lev[0];
-    CFD_HIP(hipMemcpyAsync(L.b, y, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-    RET(mg_vcycle(c, 0));
-    CFD_HIP(hipMemcpyAsync(z, L.x, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-    return CFD_OK;
-}
-
-// refresh d on the coarse levels and the inverse diagonals of every level
-int mg_setup(Case *c) {
-    const dim3 bl(BX, BY);
-    for (int l = 1; l < c->nlev; l++) {
-        Level<double> &F = c->lev[l - 1], &L = c->lev[l];
-        k_inject<double><<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(F.n0, F.d, L.n0, L.n1, L.d);
-        CFD_LAUNCH_CHECK();
-    }
-    for (int l = 0; l < c->nlev; l++) {
-        Level<double> &L = c->lev[l];
-        k_p_invdiag<double><<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(largs(L), L.invd);
-        CFD_LAUNCH_CHECK();
-    }
-    return CFD_OK;
-}
-
-// BiCGStab, right-preconditioned (oracle: bicgstab); x must be 0 on entry
-int bicgstab(Case *c, const double *b, double *x, double rtol, int maxit, int *its_out, double *relres) {
-    const int n0 = c->n0, n1 = c->n1;
-    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = nblocks2d(n0, n1);
-    double *r = c->bi[0], *r0 = c->bi[1], *pv = c->bi[2], *vv = c->bi[3];
-    double *s = c->bi[4], *t = c->bi[5], *ph = c->bi[6], *sh = c->bi[7];
-    const LevArgs<double> a0 = largs(c->lev[0]);
-    // r = b - A x (x = 0), projected; r0 = r
-    k_p_resid<double><<<gr, bl, 0, c->s>>>(a0, x, b, r);
-    CFD_LAUNCH_CHECK();
-    RET(project_mean(c, n0, n1, r));
-    CFD_HIP(hipMemcpyAsync(r0, r, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
-    CFD_HIP(hipMemsetAsync(pv, 0, c->N * sizeof(double), c->s));
-    CFD_HIP(hipMemsetAsync(vv, 0, c->N * sizeof(double), c->s));
-    k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, b, b, r, r, c->part, c->pstride);
-    CFD_LAUNCH_CHECK();
-    RET(reduce_sum(c, nb, 2, S_BB));  // S_BB, S_RR
-    k_init_scal<<<1, 1, 0, c->s>>>(c->scal);
-    CFD_LAUNCH_CHECK();
-    RET(sync_scal(c));
-    const double bnorm = std::sqrt(c->hscal[S_BB]);
-    double rnorm = std::sqrt(c->hscal[S_RR]);
-    *its_out = 0;
-    if (bnorm == 0.0) { *relres = 0.0; return CFD_OK; }
-    const double tol = rtol * bnorm;
-    if (rnorm <= tol) { *relres = rnorm / bnorm; return CFD_OK; }
-    double rho = c->hscal[S_RR];  // (r0, r) with r0 = r
-    int it;
-    for (it = 1; it <= maxit; it++) {
-        if (rho == 0.0) break;
-        k_bicg_p<<<gr, bl, 0, c->s>>>(n0, n1, it == 1, c->scal, r, vv, pv);
-        CFD_LAUNCH_CHECK();
-        RET(precond(c, pv, ph));
-        PROF_B(c, 0);
-        k_p_apply<double><<<gr, bl, 0, c->s>>>(a0, ph, vv);
-        PROF_E(c);
-        CFD_LAUNCH_CHECK();
-        k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, r0, vv, nullptr, nullptr, c->part, c->pstride);
-        CFD_LAUNCH_CHECK();
-        RET(reduce_sum(c, nb, 1, S_RV));
-        k_bicg_s<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, r, vv, s, c->part);
-        CFD_LAUNCH_CHECK();
-        RET(reduce_sum(c, nb, 1, S_SS));
-        RET(sync_scal(c));
-        const double snorm = std::sqrt(c->hscal[S_SS]);
-        if (snorm <= tol) {
-            k_bicg_x1<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, ph, x);
-            CFD_LAUNCH_CHECK();
-            rnorm = snorm;
-            break;
-        }
-        RET(precond(c, s, sh));
-        PROF_B(c, 0);
-        k_p_apply<double><<<gr, bl, 0, c->s>>>(a0, sh, t);
-        PROF_E(c);
-        CFD_LAUNCH_CHECK();
-        k_dot2<<<gr, bl, 0, c->s>>>(n0, n1, t, s, t, t, c->part, c->pstride);
-        CFD_LAUNCH_CHECK();
-        RET(reduce_sum(c, nb, 2, S_TS));  // S_TS, S_TT
-        k_set_omega<<<1, 1, 0, c->s>>>(c->scal);
-        CFD_LAUNCH_CHECK();
-        k_bicg_xr<<<gr, bl, 0, c->s>>>(n0, n1, c->scal, ph, sh, s, t, r0, x, r, c->part, c->pstride);
-        CFD_LAUNCH_CHECK();
-        RET(reduce_sum(c, nb, 2, S_RR));  // S_RR, then (r0, r) into S_SUM
-        k_reduce_sum<<<1, 256, 0, c->s>>>(c->part + c->pstride, c->pstride, nb, 1, c->scal, S_R0R);
-        CFD_LAUNCH_CHECK();
-        k_rotate_rho<<<1, 1, 0, c->s>>>(c->scal);
-        CFD_LAUNCH_CHECK();
-        RET(sync_scal(c));
-        rnorm = std::sqrt(c->hscal[S_RR]);
-        const double omega = c->hscal[S_OMEGA];
-        if (rnorm <= tol || omega == 0.0) break;
-        rho = c->hscal[S_RHO];
-    }
-    *its_out = it > maxit ? maxit : it;
-    *relres = rnorm / bnorm;
-    return CFD_OK;
-}
-
+// coarse geometry (metrics injected once) and the solver(s) of the case's precision
 int build_levels(Case *c) {
     const int want = c->prm.mg_levels < 1 ? 1 : c->prm.mg_levels;
-    Level<double> &L0 = c->lev[0];
-    L0.n0 = c->n0; L0.n1 = c->n1; L0.h1 = c->h1; L0.h2 = c->h2;
-    L0.xix = c->xix; L0.xiy = c->xiy; L0.etx = c->etx; L0.ety = c->ety; L0.jac = c->ajac; L0.d = c->d;
-    RET(dalloc(c, &L0.invd, c->N));
-    RET(dalloc(c, &L0.x, c->N));
-    RET(dalloc(c, &L0.b, c->N));
-    RET(dalloc(c, &L0.r, c->N));
-    RET(dalloc(c, &L0.t, c->N));
+    Geom &G0 = c->geom[0];
+    G0.n0 = c->n0; G0.n1 = c->n1; G0.h1 = c->h1; G0.h2 = c->h2;
+    G0.xix = c->xix; G0.xiy = c->xiy; G0.etx = c->etx; G0.ety = c->ety; G0.jac = c->ajac; G0.d = c->d;
     int l = 1;
     while (l < want && l < MAXLEV) {
-        const Level<double> &F = c->lev[l - 1];
+        const Geom &F = c->geom[l - 1];
         if ((F.n0 & 1) == 0 || (F.n1 & 1) == 0 || F.n0 < 9 || F.n1 < 9) break;
-        Level<double> &L = c->lev[l];
+        Geom &L = c->geom[l];
         L.n0 = (F.n0 + 1) / 2; L.n1 = (F.n1 + 1) / 2;
         L.h1 = 2.0 * F.h1; L.h2 = 2.0 * F.h2;
         const size_t M = (size_t)L.n0 * L.n1;
         RET(dalloc(c, &L.xix, M)); RET(dalloc(c, &L.xiy, M)); RET(dalloc(c, &L.etx, M));
         RET(dalloc(c, &L.ety, M)); RET(dalloc(c, &L.jac, M)); RET(dalloc(c, &L.d, M));
-        RET(dalloc(c, &L.invd, M)); RET(dalloc(c, &L.x, M)); RET(dalloc(c, &L.b, M));
-        RET(dalloc(c, &L.r, M)); RET(dalloc(c, &L.t, M));
-        L.owns_metrics = true;
         const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.xix, L.n0, L.n1, L.xix);
-        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.xiy, L.n0, L.n1, L.xiy);
-        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.etx, L.n0, L.n1, L.etx);
-        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.ety, L.n0, L.n1, L.ety);
-        k_inject<double><<<gr, bl, 0, c->s>>>(F.n0, F.jac, L.n0, L.n1, L.jac);
+        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xix, L.n0, L.n1, L.xix);
+        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xiy, L.n0, L.n1, L.xiy);
+        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.etx, L.n0, L.n1, L.etx);
+        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.ety, L.n0, L.n1, L.ety);
+        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.jac, L.n0, L.n1, L.jac);
         CFD_LAUNCH_CHECK();
         l++;
     }
     c->nlev = l;
+    PsCtx cx = ctx(c);
+    if (c->mixed) {
+        c->ps32 = new PSolver<float>();
+        RET(c->ps32->init(cx, c->geom, c->nlev, c->prm));
+    } else {
+        c->ps64 = new PSolver<double>();
+        RET(c->ps64->init(cx, c->geom, c->nlev, c->prm));
+    }
+    return CFD_OK;
+}
+
+// per outer iteration: d onto the coarse levels, then faces and diagonals
+int mg_setup(Case *c) {
+    const dim3 bl(BX, BY);
+    for (int l = 1; l < c->nlev; l++) {
+        Geom &F = c->geom[l - 1], &L = c->geom[l];
+        k_inject<<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(F.n0, F.d, L.n0, L.n1, L.d);
+        CFD_LAUNCH_CHECK();
+    }
+    PsCtx cx = ctx(c);
+    if (c->mixed) return c->ps32->setup(cx, c->geom);
+    return c->ps64->setup(cx, c->geom);
+}
+
+// p' solve into c->pc (zero on entry).  FP64: BiCGStab + multigrid in FP64.
+// Mixed: FP64 iterative refinement around an FP32 BiCGStab + multigrid solve,
+// with the FP64 defect r = q - L p' (projected to zero mean) deciding convergence.
+int psolve(Case *c, double rtol, int maxit, int *its_out, double *relres) {
+    PsCtx cx = ctx(c);
+    if (!c->mixed) return c->ps64->solve(cx, c->q, c->pc, rtol, maxit, its_out, relres);
+    const int n0 = c->n0, n1 = c->n1;
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const int nb = nblk(n0, n1);
+    const size_t N = c->N;
+    const int lb = 256, lg = (int)((N + lb - 1) / lb);
+    // ||q|| and the first defect (p' = 0 -> r = q, projected)
+    CFD_HIP(hipMemcpyAsync(c->r64, c->q, N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    RET(ps_project_mean<double>(cx, n0, n1, c->r64));
+    k_dot2<double><<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
+    CFD_LAUNCH_CHECK();
+    RET(reduce_sum(c, nb, 2, S_BB));
+    RET(sync_scal(c));
+    const double bnorm = std::sqrt(c->hscal[S_BB]);
+    double rnorm = std::sqrt(c->hscal[S_RR]);
+    int total = 0;
+    *its_out = 0;
+    if (bnorm == 0.0) { *relres = 0.0; return CFD_OK; }
+    const double tol = rtol * bnorm;
+    for (int refine = 0; refine < 6 && rnorm > tol && total < maxit; refine++) {
+        // inner FP32 solve of L e = r to a relative tolerance that leaves room for rounding
+        const double inner = std::max(0.5 * tol / rnorm, 1e-5);
+        k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, c->r64, c->rf);
+        CFD_HIP(hipMemsetAsync(c->ef, 0, N * sizeof(float), c->s));
+        int its = 0;
+        double rr = 0.0;
+        RET(c->ps32->solve(cx, c->rf, c->ef, inner, maxit - total, &its, &rr));
+        total += its > 0 ? its : 1;
+        k_acc<float><<<lg, lb, 0, c->s>>>(N, c->ef, c->pc);
+        // FP64 defect
+        k_mapply<<<gr, bl, 0, c->s>>>(margs(c), c->pc, c->q, c->r64);
+        CFD_LAUNCH_CHECK();
+        RET(ps_project_mean<double>(cx, n0, n1, c->r64));
+        k_dot2<double><<<gr, bl, 0, c->s>>>(n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
+        CFD_LAUNCH_CHECK();
+        RET(reduce_sum(c, nb, 1, S_RR));
+        RET(sync_scal(c));
+        rnorm = std::sqrt(c->hscal[S_RR]);
+    }
+    *its_out = total;
+    *relres = rnorm / bnorm;
     return CFD_OK;
 }
 
@@ -1044,41 +704,51 @@ WallIn wall_in(const Case *c) {
                   c->p1, c->q1, c->ajac};
 }
 
-int launch_residual(Case *c) {
-    PROF_B(c, 2);
-    k_residual<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(
+template <typename TR>
+int launch_residual_t(Case *c) {
+    prof_begin(c, 2);
+    k_residual<TR><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(
         kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
-        c->p1, c->q1, c->Ru, c->Rv, c->RT, c->d, c->U, c->V, c->part, c->pstride);
-    PROF_E(c);
+        c->p1, c->q1, (TR *)c->Ru, (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, c->part, c->pstride);
+    prof_end(c);
     CFD_LAUNCH_CHECK();
     const int m = std::max(c->n0, c->n1);
     k_d_border<<<(m + 255) / 256, 256, 0, c->s>>>(c->n0, c->n1, c->d);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
+int launch_residual(Case *c) { return c->mixed ? launch_residual_t<float>(c) : launch_residual_t<double>(c); }
 
-// predictor sweeps; returns pointers of the final du, dv, dT
-int predictor(Case *c, double **du, double **dv, double **dT) {
-    double *a[3] = {c->jb[0], c->jb[1], c->jb[2]};
-    double *b[3] = {c->jb[3], c->jb[4], c->jb[5]};
+// predictor sweeps; *du.. receive the buffers holding the final corrections
+template <typename TR>
+int predictor_t(Case *c, void **du, void **dv, void **dT) {
+    TR *a[3] = {(TR *)c->jb[0], (TR *)c->jb[1], (TR *)c->jb[2]};
+    TR *b[3] = {(TR *)c->jb[3], (TR *)c->jb[4], (TR *)c->jb[5]};
+    const TR *co[5];
+    if (c->mixed) {
+        for (int k = 0; k < 5; k++) co[k] = (const TR *)c->a32[k];
+    } else {
+        const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
+        for (int k = 0; k < 5; k++) co[k] = (const TR *)m[k];
+    }
+    const TR *Ru = (const TR *)c->Ru, *Rv = (const TR *)c->Rv, *RT = (const TR *)c->RT;
+    const TR *U = (const TR *)c->U, *V = (const TR *)c->V;
     const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
     for (int s = 0; s < c->prm.mom_sweeps; s++) {
-        if (s == 0)
-            k_jacobi<true><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
-                                                c->gamma, c->p1, c->q1, a[0], a[1], a[2], b[0], b[1], b[2]);
-        else {
-            PROF_B(c, 1);
-            k_jacobi<false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
-                                                 c->gamma, c->p1, c->q1, a[0], a[1], a[2], b[0], b[1], b[2]);
-            PROF_E(c);
+        if (s == 0) {
+            k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(kg(c), phys(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4],
+                                                    a[0], a[1], a[2], b[0], b[1], b[2]);
+        } else {
+            prof_begin(c, 1);
+            k_jacobi<TR, false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3],
+                                                     co[4], a[0], a[1], a[2], b[0], b[1], b[2]);
+            prof_end(c);
         }
         CFD_LAUNCH_CHECK();
         for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
     }
-    if (c->prm.mom_sweeps <= 0) {
-        // no predictor: zero correction
-        for (int k = 0; k < 3; k++) CFD_HIP(hipMemsetAsync(a[k], 0, c->N * sizeof(double), c->s));
-    }
+    if (c->prm.mom_sweeps <= 0)
+        for (int k = 0; k < 3; k++) CFD_HIP(hipMemsetAsync(a[k], 0, c->N * sizeof(TR), c->s));
     *du = a[0]; *dv = a[1]; *dT = a[2];
     return CFD_OK;
 }
@@ -1096,24 +766,33 @@ int apply_bcs_dev(Case *c) {
     return CFD_OK;
 }
 
+template <typename TR>
+int star_t(Case *c, void *du, void *dv, void *dT) {
+    k_star<TR><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(kg(c), c->u, c->v, (TR *)du, (TR *)dv, (TR *)dT,
+                                                                 c->T, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
+                                                                 c->us, c->vs, c->uh, c->vh);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
 int outer_once(Case *c, cfd_stats *st) {
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = nblocks2d(n0, n1);
-    // 1-2 residuals and d
+    const int nb = nblk(n0, n1);
+    // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
     RET(reduce_sum(c, nb, 2, S_ENERGY));  // S_ENERGY, S_MOM
     // 3 predictor
-    double *du, *dv, *dT;
-    RET(predictor(c, &du, &dv, &dT));
-    // 4 u*, T, u-hat, face divergence
-    k_star<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, du, dv, dT, c->T, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
-                                c->us, c->vs, c->uh, c->vh);
-    CFD_LAUNCH_CHECK();
-    PROF_B(c, 3);
+    void *du, *dv, *dT;
+    if (c->mixed) RET(predictor_t<float>(c, &du, &dv, &dT));
+    else RET(predictor_t<double>(c, &du, &dv, &dT));
+    // 4 u*, T, u-hat, face divergence (FP64)
+    if (c->mixed) RET(star_t<float>(c, du, dv, dT));
+    else RET(star_t<double>(c, du, dv, dT));
+    prof_begin(c, 3);
     k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
                                     c->q, c->part);
-    PROF_E(c);
+    prof_end(c);
     CFD_LAUNCH_CHECK();
     RET(reduce_sum(c, nb, 1, S_MASS));
     // 5 p' solve
@@ -1121,9 +800,12 @@ int outer_once(Case *c, cfd_stats *st) {
     CFD_HIP(hipMemsetAsync(c->pc, 0, c->N * sizeof(double), c->s));
     int its = 0;
     double relres = 0.0;
-    RET(bicgstab(c, c->q, c->pc, c->prm.p_rtol, c->prm.p_maxit, &its, &relres));
-    RET(project_mean(c, n0, n1, c->pc));
-    // 6-7 correction and BCs
+    RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, &its, &relres));
+    {
+        PsCtx cx = ctx(c);
+        RET(ps_project_mean<double>(cx, n0, n1, c->pc));
+    }
+    // 6-7 correction and BCs (FP64 accumulation)
     k_correct<<<gr, bl, 0, c->s>>>(kg(c), c->prm.alpha_p, c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx,
                                    c->ety, c->u, c->v, c->p);
     CFD_LAUNCH_CHECK();
@@ -1136,7 +818,7 @@ int outer_once(Case *c, cfd_stats *st) {
         k_p_walls_write<<<nbl, 256, 0, c->s>>>(kg(c), c->scratch, c->p);
         CFD_LAUNCH_CHECK();
     }
-    // 8 reductions
+    // 8 reductions (FP64)
     k_dmax<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->part);
     CFD_LAUNCH_CHECK();
     k_reduce_max<<<1, 256, 0, c->s>>>(c->part, nb, c->scal, S_DMAX);
@@ -1168,6 +850,14 @@ int outer_once(Case *c, cfd_stats *st) {
     return CFD_OK;
 }
 
+template <typename T>
+void time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
+    OpLevel<T> &L = ps->lev[0];
+    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
+    if (kernel == 0) k_fapply<T><<<gr, bl, 0, c->s>>>(fargs(L), L.x, L.t);
+    else k_fsmooth<T, false><<<gr, bl, 0, c->s>>>(fargs(L), (T)0.8, L.x, L.b, L.invd, L.t);
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -1192,23 +882,35 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
     if (!out || !params || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac)
         return CFD_E_ARG;
     if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0)) return CFD_E_ARG;
-    if (params->precision != CFD_FP64) return CFD_E_ARG;  // the mixed path is a separate build stage
+    if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
     *out = nullptr;
     Case *c = new Case();
     c->n0 = n0; c->n1 = n1; c->N = (size_t)n0 * n1; c->h1 = dxi; c->h2 = deta; c->prm = *params;
+    c->mixed = params->precision == CFD_MIXED;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
     const size_t N = c->N;
     int rc = CFD_OK;
     double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
-                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->U, &c->V, &c->Ru,
-                         &c->Rv, &c->RT, &c->d, &c->us, &c->vs, &c->uh, &c->vh, &c->q, &c->pc};
+                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->d, &c->us, &c->vs,
+                         &c->uh, &c->vh, &c->q, &c->pc};
     for (double **f : fields)
         if (rc == CFD_OK) rc = dalloc(c, f, N);
-    for (int k = 0; k < 6 && rc == CFD_OK; k++) rc = dalloc(c, &c->jb[k], N);
-    for (int k = 0; k < 8 && rc == CFD_OK; k++) rc = dalloc(c, &c->bi[k], N);
+    void **pred[] = {&c->Ru, &c->Rv, &c->RT, &c->U, &c->V, &c->jb[0], &c->jb[1], &c->jb[2], &c->jb[3],
+                     &c->jb[4], &c->jb[5]};
+    for (void **f : pred) {
+        if (rc != CFD_OK) break;
+        if (c->mixed) rc = dalloc(c, (float **)f, N);
+        else rc = dalloc(c, (double **)f, N);
+    }
+    if (c->mixed) {
+        for (int k = 0; k < 5 && rc == CFD_OK; k++) rc = dalloc(c, &c->a32[k], N);
+        if (rc == CFD_OK) rc = dalloc(c, &c->rf, N);
+        if (rc == CFD_OK) rc = dalloc(c, &c->ef, N);
+        if (rc == CFD_OK) rc = dalloc(c, &c->r64, N);
+    }
     if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)2 * (n0 + n1) + 8);
-    c->pstride = (nblocks2d(n0, n1) + 255) / 256 * 256;
+    c->pstride = (nblk(n0, n1) + 255) / 256 * 256;
     if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)4 * c->pstride);
     if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)S_NSCAL + 8);
     if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (S_NSCAL + 8) * sizeof(double)) != hipSuccess)
@@ -1220,6 +922,12 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
         if (hipMemcpyAsync(dst[k], src[k], B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
     if (rc == CFD_OK && p1 && hipMemcpyAsync(c->p1, p1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
     if (rc == CFD_OK && q1 && hipMemcpyAsync(c->q1, q1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc == CFD_OK && c->mixed) {
+        const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
+        const int lb = 256, lg = (int)((N + lb - 1) / lb);
+        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, m[k], c->a32[k]);
+        if (hipGetLastError() != hipSuccess) rc = CFD_E_HIP;
+    }
     if (rc == CFD_OK) rc = build_levels(c);
     if (rc == CFD_OK) {
         c->wall_len_hot = c->wall_len_cold = 0.0;
@@ -1248,6 +956,8 @@ int cfd_case_destroy(cfd_case h) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->s) (void)hipStreamDestroy(c->s);
+    delete c->ps64;
+    delete c->ps32;
     delete c;
     return CFD_OK;
 }
@@ -1316,14 +1026,26 @@ int cfd_case_bytes(cfd_case h, size_t *bytes) {
     return CFD_OK;
 }
 
+// residual arrays are returned in FP64 whatever the predictor precision
 int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c) return CFD_E_ARG;
     RET(launch_residual(c));
-    const size_t B = c->N * sizeof(double);
-    if (Ru) CFD_HIP(hipMemcpyAsync(Ru, c->Ru, B, hipMemcpyDeviceToHost, c->s));
-    if (Rv) CFD_HIP(hipMemcpyAsync(Rv, c->Rv, B, hipMemcpyDeviceToHost, c->s));
-    if (RT) CFD_HIP(hipMemcpyAsync(RT, c->RT, B, hipMemcpyDeviceToHost, c->s));
+    const size_t N = c->N, B = N * sizeof(double);
+    void *src[3] = {c->Ru, c->Rv, c->RT};
+    double *dst[3] = {Ru, Rv, RT};
+    for (int k = 0; k < 3; k++) {
+        if (!dst[k]) continue;
+        if (c->mixed) {
+            const int lb = 256, lg = (int)((N + lb - 1) / lb);
+            k_cvt<float, double><<<lg, lb, 0, c->s>>>(N, (const float *)src[k], c->uh);
+            CFD_LAUNCH_CHECK();
+            CFD_HIP(hipMemcpyAsync(dst[k], c->uh, B, hipMemcpyDeviceToHost, c->s));
+            CFD_HIP(hipStreamSynchronize(c->s));
+        } else {
+            CFD_HIP(hipMemcpyAsync(dst[k], src[k], B, hipMemcpyDeviceToHost, c->s));
+        }
+    }
     if (d) CFD_HIP(hipMemcpyAsync(d, c->d, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     return CFD_OK;
@@ -1334,10 +1056,10 @@ int cfd_case_p_apply(cfd_case h, const double *d, const double *x, double *y) {
     if (!c || !d || !x || !y) return CFD_E_ARG;
     const size_t B = c->N * sizeof(double);
     CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemcpyAsync(c->bi[6], x, B, hipMemcpyHostToDevice, c->s));
-    k_p_apply<double><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(largs(c->lev[0]), c->bi[6], c->bi[7]);
+    CFD_HIP(hipMemcpyAsync(c->uh, x, B, hipMemcpyHostToDevice, c->s));
+    k_mapply<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(margs(c), c->uh, nullptr, c->vh);
     CFD_LAUNCH_CHECK();
-    CFD_HIP(hipMemcpyAsync(y, c->bi[7], B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipMemcpyAsync(y, c->vh, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     return CFD_OK;
 }
@@ -1353,7 +1075,7 @@ int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, do
     RET(mg_setup(c));
     int its = 0;
     double rr = 0.0;
-    RET(bicgstab(c, c->q, c->pc, rtol, maxit, &its, &rr));
+    RET(psolve(c, rtol, maxit, &its, &rr));
     CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     if (iters) *iters = its;
@@ -1393,7 +1115,7 @@ int cfd_case_profile_read(cfd_case h, double *ms_total, long *launches, int nkin
 
 int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch) {
     Case *c = reinterpret_cast<Case *>(h);
-    if (!c || reps < 1 || !ms_per_launch) return CFD_E_ARG;
+    if (!c || reps < 1 || !ms_per_launch || kernel < 0 || kernel > 4) return CFD_E_ARG;
     const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
     hipEvent_t e0, e1;
     CFD_HIP(hipEventCreate(&e0));
@@ -1402,30 +1124,26 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
     for (int k = 0; k < reps; k++) {
         switch (kernel) {
         case 0:
-            k_p_apply<double><<<gr, bl, 0, c->s>>>(largs(c->lev[0]), c->bi[6], c->bi[7]);
+        case 4:
+            if (c->mixed) time_solver_kernel<float>(c, c->ps32, kernel);
+            else time_solver_kernel<double>(c, c->ps64, kernel);
             break;
         case 1:
-            k_jacobi<false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->Ru, c->Rv, c->RT, c->U, c->V, c->alph, c->beta,
-                                                 c->gamma, c->p1, c->q1, c->jb[0], c->jb[1], c->jb[2], c->jb[3],
-                                                 c->jb[4], c->jb[5]);
+            if (c->mixed) {
+                void *a, *b, *t;
+                (void)predictor_t<float>(c, &a, &b, &t);
+            } else {
+                void *a, *b, *t;
+                (void)predictor_t<double>(c, &a, &b, &t);
+            }
             break;
         case 2:
-            k_residual<<<gr, bl, 0, c->s>>>(kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety,
-                                            c->alph, c->beta, c->gamma, c->p1, c->q1, c->Ru, c->Rv, c->RT, c->d,
-                                            c->U, c->V, c->part, c->pstride);
+            (void)launch_residual(c);
             break;
         case 3:
             k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
                                             c->ajac, c->q, c->part);
             break;
-        case 4:
-            k_p_smooth<double, false><<<gr, bl, 0, c->s>>>(largs(c->lev[0]), 0.8, c->lev[0].x, c->lev[0].b,
-                                                           c->lev[0].invd, c->lev[0].t);
-            break;
-        default:
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return CFD_E_ARG;
         }
     }
     CFD_LAUNCH_CHECK();

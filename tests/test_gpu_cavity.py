@@ -13,7 +13,7 @@ def pair(n, skew=0.05, beta=None, **kw):
     from oracle.oracle import CavityOracle, cav_params
     g = cavity_grid(n, skew=skew, beta=beta)
     gp = default_params(**kw)
-    op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields()})
+    op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields()})  # precision has no oracle field
     return CavityCase(g, gp), CavityOracle(g, op), g
 
 
@@ -77,3 +77,45 @@ def test_large_grid_runs_and_conserves():
     assert s[-1].mass_res < 1.0
     f = c.fields()
     assert np.isfinite(f["u"]).all() and np.abs(f["u"]).max() < 10
+
+
+def test_mixed_p_solve_refines_to_fp64_accuracy():
+    """FP32 BiCGStab + multigrid inside FP64 refinement reaches an FP64-level tolerance."""
+    print("\n[gpu] mixed p' solve 65x65", flush=True)
+    from cfd_solver_amd._lib import CFD_MIXED
+    gpu, cpu, g = pair(65, skew=0.15, beta=1.5, precision=CFD_MIXED)
+    rng = np.random.default_rng(11)
+    d = 0.5 + rng.random((65, 65))
+    q = np.zeros((65, 65))
+    q[1:-1, 1:-1] = rng.standard_normal((63, 63))
+    q[1:-1, 1:-1] -= q[1:-1, 1:-1].mean()
+    xg, itg, rg = gpu.p_solve(d, q, rtol=1e-10, maxit=200)
+    xc, itc, rc = cpu.p_solve(d, q, rtol=1e-12, maxit=200)
+    assert rg <= 1e-10, rg
+    xg = xg[1:-1, 1:-1] - xg[1:-1, 1:-1].mean()
+    xc = xc[1:-1, 1:-1] - xc[1:-1, 1:-1].mean()
+    assert rel_l2(xg, xc) < 1e-8
+
+
+def test_mixed_converged_fields_match_fp64_oracle():
+    """Refined mixed path vs the FP64 oracle on converged fields (33x33, both to 1e-9)."""
+    print("\n[gpu] mixed vs FP64 oracle, 33x33 to 1e-9", flush=True)
+    from cfd_solver_amd._lib import CFD_MIXED
+    gpu, cpu, g = pair(33, skew=0.1, beta=1.5, Re=100.0, Ri=1.0, precision=CFD_MIXED)
+    st, ok = gpu.solve(tol=1e-9, max_outer=4000)
+    assert ok, (st.mass_res, st.energy_res, st.iter)
+    first = None
+    for k in range(4000):
+        sc = cpu.step()
+        first = first or sc.mass_l1
+        e0 = sc.energy_l1 if k == 0 else e0
+        if sc.mass_l1 <= 1e-9 * first and sc.energy_l1 <= 1e-9 * e0:
+            break
+    fg, fc = gpu.fields(), cpu.fields()
+    errs = {k: rel_l2(fg[k], fc[k]) for k in ("u", "v", "T")}
+    pg, pc = fg["p"] - fg["p"][1:-1, 1:-1].mean(), fc["p"] - fc["p"][1:-1, 1:-1].mean()
+    errs["p"] = rel_l2(pg, pc)
+    errs["Nu"] = abs(st.nu_hot - sc.nu_hot) / abs(sc.nu_hot)
+    print("[gpu] mixed vs FP64 per-field rel L2:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v < 1e-6, (k, v)
