@@ -321,6 +321,145 @@ __global__ void k_reduce_sum(const double *__restrict__ part, int pstride, int n
 }
 
 // ---------------------------------------------------------------------------
+// Coarse-level V-cycle in one workgroup.  Levels at or below a few thousand
+// nodes are latency-bound: one launch per sweep would cost more than the sweep.
+// This kernel runs the whole V-cycle from level l0 down to the coarsest and
+// back in a single 1024-thread workgroup, separating the phases with block
+// barriers (which also order the global-memory traffic of the workgroup).
+// The arithmetic per node is the same device code as the per-level kernels.
+
+constexpr int CW = 1024;  // threads of the coarse workgroup
+
+template <typename T>
+struct CLev {
+    int n0, n1;
+    T *fen, *fet, *fnn, *fnt, *invd, *x, *b, *r, *t;
+};
+template <typename T>
+struct CoarseArgs {
+    int nlev;
+    int pre, post, coarse;
+    T omega;
+    CLev<T> L[PS_MAXLEV];
+};
+
+template <typename T>
+__device__ __forceinline__ void cw_smooth(const CLev<T> &L, const T *xi, T *xo, T w, bool first) {
+    const FArgs<T> a{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt};
+    const int n = L.n0 * L.n1;
+    for (int k = threadIdx.x; k < n; k += CW) {
+        const int i = k % L.n0, j = k / L.n0;
+        if (i == 0 || j == 0 || i == L.n0 - 1 || j == L.n1 - 1) { xo[k] = T(0); continue; }
+        if (first) xo[k] = w * L.b[k] * L.invd[k];
+        else xo[k] = xi[k] + w * (L.b[k] - frow<T>(a, xi, i, j)) * L.invd[k];
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(CW) void k_mg_coarse(CoarseArgs<T> A, int l0) {
+    __shared__ double red[CW / 64];
+    T *xc[PS_MAXLEV], *tc[PS_MAXLEV];
+    for (int l = 0; l < A.nlev; l++) { xc[l] = A.L[l].x; tc[l] = A.L[l].t; }
+    const T w = A.omega;
+    const int last = A.nlev - 1;
+    // sweeps on level l starting from zero or from xc[l]; result in xc[l]
+    auto sweeps = [&](int l, int count, bool from_zero) {
+        for (int s = 0; s < count; s++) {
+            if (from_zero && s == 0) {
+                cw_smooth<T>(A.L[l], xc[l], xc[l], w, true);
+            } else {
+                cw_smooth<T>(A.L[l], xc[l], tc[l], w, false);
+                T *tmp = xc[l]; xc[l] = tc[l]; tc[l] = tmp;
+            }
+        }
+    };
+    for (int l = l0; l < last; l++) {
+        const CLev<T> &L = A.L[l];
+        const int n = L.n0 * L.n1;
+        if (A.pre > 0) {
+            sweeps(l, A.pre, true);
+        } else {
+            for (int k = threadIdx.x; k < n; k += CW) xc[l][k] = T(0);
+            __syncthreads();
+        }
+        // residual
+        const FArgs<T> a{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt};
+        for (int k = threadIdx.x; k < n; k += CW) {
+            const int i = k % L.n0, j = k / L.n0;
+            if (i == 0 || j == 0 || i == L.n0 - 1 || j == L.n1 - 1) { L.r[k] = T(0); continue; }
+            L.r[k] = L.b[k] - frow<T>(a, xc[l], i, j);
+        }
+        __syncthreads();
+        // full weighting onto level l+1 and the interior sum for the mean projection
+        const CLev<T> &C = A.L[l + 1];
+        const int m = C.n0 * C.n1;
+        double part = 0.0;
+        for (int k = threadIdx.x; k < m; k += CW) {
+            const int I = k % C.n0, J = k / C.n0;
+            if (I == 0 || J == 0 || I == C.n0 - 1 || J == C.n1 - 1) { C.b[k] = T(0); continue; }
+            const int i = 2 * I, j = 2 * J, f = L.n0;
+            const T *r = L.r;
+            const T v = (T(4) * r[j * f + i] +
+                         T(2) * (r[j * f + i - 1] + r[j * f + i + 1] + r[(j - 1) * f + i] + r[(j + 1) * f + i]) +
+                         (r[(j - 1) * f + i - 1] + r[(j - 1) * f + i + 1] + r[(j + 1) * f + i - 1] +
+                          r[(j + 1) * f + i + 1])) /
+                        T(16);
+            C.b[k] = v;
+            part += (double)v;
+        }
+        // block sum (same shape as block_sum, sized for CW threads)
+        part = wave_sum(part);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+        __syncthreads();
+        double tot = 0.0;
+        for (int q = 0; q < CW / 64; q++) tot += red[q];
+        __syncthreads();
+        const double mean = tot / ((double)(C.n0 - 2) * (double)(C.n1 - 2));
+        for (int k = threadIdx.x; k < m; k += CW) {
+            const int I = k % C.n0, J = k / C.n0;
+            if (I == 0 || J == 0 || I == C.n0 - 1 || J == C.n1 - 1) continue;
+            C.b[k] = (T)((double)C.b[k] - mean);
+        }
+        __syncthreads();
+    }
+    // coarsest level
+    if (A.coarse > 0) {
+        sweeps(last, A.coarse, true);
+    } else {
+        const int n = A.L[last].n0 * A.L[last].n1;
+        for (int k = threadIdx.x; k < n; k += CW) xc[last][k] = T(0);
+        __syncthreads();
+    }
+    for (int l = last - 1; l >= l0; l--) {
+        const CLev<T> &L = A.L[l];
+        const CLev<T> &C = A.L[l + 1];
+        const int n = L.n0 * L.n1;
+        const T *xcc = xc[l + 1];
+        for (int k = threadIdx.x; k < n; k += CW) {
+            const int i = k % L.n0, j = k / L.n0;
+            if (i < 1 || i > L.n0 - 2 || j < 1 || j > L.n1 - 2) continue;
+            const int I = i >> 1, J = j >> 1, m0 = C.n0, m1 = C.n1;
+#define XC(a_, b_) xcc[clampi((b_), 1, m1 - 2) * m0 + clampi((a_), 1, m0 - 2)]
+            T e;
+            if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
+            else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
+            else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
+            else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
+#undef XC
+            xc[l][k] += e;
+        }
+        __syncthreads();
+        sweeps(l, A.post, false);
+    }
+    // hand the level-l0 result back in its x buffer
+    if (xc[l0] != A.L[l0].x) {
+        const int n = A.L[l0].n0 * A.L[l0].n1;
+        for (int k = threadIdx.x; k < n; k += CW) A.L[l0].x[k] = xc[l0][k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 // what the solver needs from its owner (stream, reductions, timing hooks)
@@ -382,6 +521,9 @@ struct PSolver {
     T *bi[8] = {nullptr};  // r r0 pv vv s t ph sh (level-0 size)
     int pre = 2, post = 2, coarse = 30;
     double omega = 0.8;
+    int lc = 0;                  // first level handled by the single-workgroup kernel
+    CoarseArgs<T> cargs{};
+    static constexpr int COARSE_NODES = 4225;  // levels with at most 65 x 65 nodes
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
         nlev = nlev_;
@@ -399,6 +541,17 @@ struct PSolver {
         }
         const size_t M0 = (size_t)lev[0].n0 * lev[0].n1 * sizeof(T);
         for (int k = 0; k < 8; k++) CFD_RET(ps_alloc(cx, (void **)&bi[k], M0));
+        lc = nlev - 1;
+        while (lc > 0 && (size_t)lev[lc - 1].n0 * lev[lc - 1].n1 <= COARSE_NODES) lc--;
+        cargs.nlev = nlev;
+        cargs.pre = pre;
+        cargs.post = post;
+        cargs.coarse = coarse;
+        cargs.omega = (T)omega;
+        for (int l = 0; l < nlev; l++) {
+            const OpLevel<T> &L = lev[l];
+            cargs.L[l] = CLev<T>{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt, L.invd, L.x, L.b, L.r, L.t};
+        }
         return CFD_OK;
     }
 
@@ -438,6 +591,12 @@ struct PSolver {
 
     int vcycle(PsCtx &cx, int l) {
         OpLevel<T> &L = lev[l];
+        if (l == lc) {
+            // the rest of the cycle in one workgroup; the level's x/t roles are fixed there
+            k_mg_coarse<T><<<1, CW, 0, cx.s>>>(cargs, l);
+            CFD_LAUNCH_CHECK();
+            return CFD_OK;
+        }
         const size_t M = (size_t)L.n0 * L.n1 * sizeof(T);
         const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
         if (l == nlev - 1) {
