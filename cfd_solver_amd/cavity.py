@@ -97,7 +97,9 @@ class CavityCase:
     KERNELS = ("p_apply", "jacobi", "residual", "face_div", "mg_smooth")
 
     def profile(self, enable=True):
-        call("cfd_case_profile", self._h, 1 if enable else 0)
+        """True: time hot kernels live (recaptures the graph); False: stop; 2: reset counters."""
+        mode = 2 if enable == 2 and enable is not True else (1 if enable else 0)
+        call("cfd_case_profile", self._h, mode)
 
     def profile_read(self):
         n = len(self.KERNELS)

@@ -68,6 +68,12 @@ struct Case {
     int ev_used = 0;            // events recorded since the last drain
     double prof_ms[CFD_PROF_KINDS] = {0};
     long prof_n[CFD_PROF_KINDS] = {0};
+    // captured outer iteration
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool capturing = false;
+    std::vector<int> gkind;     // event-pair kinds recorded inside the graph
+    int gev_used = 0;
 };
 
 // ------------------------------------------------------------------------
@@ -513,14 +519,43 @@ __device__ __forceinline__ double mrow(const MArgs &a, const double *__restrict_
 }
 
 // y = L x, or y = b - L x when b != nullptr; 0 on the border
-__global__ __launch_bounds__(NT) void k_mapply(MArgs a, const double *__restrict__ x, const double *__restrict__ b,
-                                               double *__restrict__ y) {
+__global__ __launch_bounds__(NT) void k_mapply(const double *gd, MArgs a, const double *__restrict__ x,
+                                               const double *__restrict__ b, double *__restrict__ y) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
     if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { y[P] = 0.0; return; }
     const double v = mrow(a, x, i, j);
     y[P] = b ? b[P] - v : v;
+}
+
+// device-side control of the captured mixed-precision refinement
+__global__ void k_r_init(double *scal, double rtol) {
+    const double bn = sqrt(scal[S_BB64]);
+    const double rn = sqrt(scal[S_RR64]);
+    scal[S_BN64] = bn;
+    scal[S_TOL64] = rtol * bn;
+    scal[S_RN64] = rn;
+    scal[S_ITTOT] = 0.0;
+    scal[S_RDONE] = (bn == 0.0 || rn <= rtol * bn) ? 1.0 : 0.0;
+}
+__global__ void k_r_prep(double *scal) {
+    if (scal[S_RDONE] != 0.0) return;
+    scal[S_RTOL] = fmax(0.5 * scal[S_TOL64] / scal[S_RN64], 1e-5);
+}
+__global__ void k_r_its(double *scal) {
+    if (scal[S_RDONE] != 0.0) return;
+    scal[S_ITTOT] += fmax(scal[S_IT], 1.0);
+}
+__global__ void k_r_check(double *scal) {
+    if (scal[S_RDONE] != 0.0) return;
+    const double rn = sqrt(scal[S_RR64]);
+    scal[S_RN64] = rn;
+    if (rn <= scal[S_TOL64]) scal[S_RDONE] = 1.0;
+}
+__global__ void k_r_final(double *scal) {
+    scal[S_RELRES64] = scal[S_BN64] > 0.0 ? scal[S_RN64] / scal[S_BN64] : 0.0;
 }
 
 }  // namespace cfd
@@ -658,9 +693,9 @@ int psolve(Case *c, double rtol, int maxit, int *its_out, double *relres) {
     const size_t N = c->N;
     const int lb = 256, lg = (int)((N + lb - 1) / lb);
     // ||q|| and the first defect (p' = 0 -> r = q, projected)
-    CFD_HIP(hipMemcpyAsync(c->r64, c->q, N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    RET(ps_copy<double>(cx, c->q, c->r64, N));
     RET(ps_project_mean<double>(cx, n0, n1, c->r64));
-    k_dot2<double><<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
+    k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
     CFD_LAUNCH_CHECK();
     RET(reduce_sum(c, nb, 2, S_BB));
     RET(sync_scal(c));
@@ -673,18 +708,18 @@ int psolve(Case *c, double rtol, int maxit, int *its_out, double *relres) {
     for (int refine = 0; refine < 6 && rnorm > tol && total < maxit; refine++) {
         // inner FP32 solve of L e = r to a relative tolerance that leaves room for rounding
         const double inner = std::max(0.5 * tol / rnorm, 1e-5);
-        k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, c->r64, c->rf);
-        CFD_HIP(hipMemsetAsync(c->ef, 0, N * sizeof(float), c->s));
+        k_cvt<double, float><<<lg, lb, 0, c->s>>>(nullptr, N, c->r64, c->rf);
+        RET(ps_fill<float>(cx, c->ef, N));
         int its = 0;
         double rr = 0.0;
         RET(c->ps32->solve(cx, c->rf, c->ef, inner, maxit - total, &its, &rr));
         total += its > 0 ? its : 1;
-        k_acc<float><<<lg, lb, 0, c->s>>>(N, c->ef, c->pc);
+        k_acc<float><<<lg, lb, 0, c->s>>>(nullptr, N, c->ef, c->pc);
         // FP64 defect
-        k_mapply<<<gr, bl, 0, c->s>>>(margs(c), c->pc, c->q, c->r64);
+        k_mapply<<<gr, bl, 0, c->s>>>(nullptr, margs(c), c->pc, c->q, c->r64);
         CFD_LAUNCH_CHECK();
         RET(ps_project_mean<double>(cx, n0, n1, c->r64));
-        k_dot2<double><<<gr, bl, 0, c->s>>>(n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
+        k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
         CFD_LAUNCH_CHECK();
         RET(reduce_sum(c, nb, 1, S_RR));
         RET(sync_scal(c));
@@ -692,6 +727,47 @@ int psolve(Case *c, double rtol, int maxit, int *its_out, double *relres) {
     }
     *its_out = total;
     *relres = rnorm / bnorm;
+    return CFD_OK;
+}
+
+// Capturable p' solve (no host synchronisation): FP64 runs p_maxit unrolled
+// BiCGStab iterations; mixed runs two FP64 refinement steps around FP32 solves.
+// Iterations and the relative residual are left in device scalars.
+int psolve_graph(Case *c, double rtol, int maxit) {
+    PsCtx cx = ctx(c);
+    if (!c->mixed) return c->ps64->solve_graph(cx, c->q, c->pc, rtol, -1, maxit, nullptr);
+    const int n0 = c->n0, n1 = c->n1;
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const int nb = nblk(n0, n1);
+    const size_t N = c->N;
+    const int lb = 256, lg = (int)((N + lb - 1) / lb);
+    const double *rd = c->scal + S_RDONE;
+    RET(ps_copy<double>(cx, c->q, c->r64, N));
+    RET(ps_project_mean<double>(cx, n0, n1, c->r64));
+    k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
+    CFD_LAUNCH_CHECK();
+    RET(ps_reduce(cx, nb, 2, S_BB64));
+    k_r_init<<<1, 1, 0, c->s>>>(c->scal, rtol);
+    CFD_LAUNCH_CHECK();
+    for (int refine = 0; refine < 2; refine++) {
+        k_r_prep<<<1, 1, 0, c->s>>>(c->scal);
+        k_cvt<double, float><<<lg, lb, 0, c->s>>>(rd, N, c->r64, c->rf);
+        CFD_LAUNCH_CHECK();
+        RET(ps_fill<float>(cx, c->ef, N, rd));
+        RET(c->ps32->solve_graph(cx, c->rf, c->ef, 0.0, S_RTOL, maxit, rd));
+        k_r_its<<<1, 1, 0, c->s>>>(c->scal);
+        k_acc<float><<<lg, lb, 0, c->s>>>(rd, N, c->ef, c->pc);
+        k_mapply<<<gr, bl, 0, c->s>>>(rd, margs(c), c->pc, c->q, c->r64);
+        CFD_LAUNCH_CHECK();
+        RET(ps_project_mean<double>(cx, n0, n1, c->r64, rd));
+        k_dot2<double><<<gr, bl, 0, c->s>>>(rd, n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
+        CFD_LAUNCH_CHECK();
+        RET(ps_reduce(cx, nb, 1, S_RR64, rd));
+        k_r_check<<<1, 1, 0, c->s>>>(c->scal);
+        CFD_LAUNCH_CHECK();
+    }
+    k_r_final<<<1, 1, 0, c->s>>>(c->scal);
+    CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 
@@ -747,8 +823,10 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
         CFD_LAUNCH_CHECK();
         for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
     }
-    if (c->prm.mom_sweeps <= 0)
-        for (int k = 0; k < 3; k++) CFD_HIP(hipMemsetAsync(a[k], 0, c->N * sizeof(TR), c->s));
+    if (c->prm.mom_sweeps <= 0) {
+        PsCtx cx = ctx(c);
+        for (int k = 0; k < 3; k++) RET(ps_fill<TR>(cx, a[k], c->N));
+    }
     *du = a[0]; *dv = a[1]; *dT = a[2];
     return CFD_OK;
 }
@@ -775,10 +853,14 @@ int star_t(Case *c, void *du, void *dv, void *dT) {
     return CFD_OK;
 }
 
-int outer_once(Case *c, cfd_stats *st) {
+// Everything of one outer iteration that runs on the stream.  graph: capturable
+// (device-side p' control); otherwise the p' solve synchronises per iteration and
+// returns its iteration count and residual in *its / *relres.
+int outer_body(Case *c, bool graph, int *its, double *relres) {
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
     const int nb = nblk(n0, n1);
+    PsCtx cx = ctx(c);
     // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
     RET(reduce_sum(c, nb, 2, S_ENERGY));  // S_ENERGY, S_MOM
@@ -797,14 +879,13 @@ int outer_once(Case *c, cfd_stats *st) {
     RET(reduce_sum(c, nb, 1, S_MASS));
     // 5 p' solve
     RET(mg_setup(c));
-    CFD_HIP(hipMemsetAsync(c->pc, 0, c->N * sizeof(double), c->s));
-    int its = 0;
-    double relres = 0.0;
-    RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, &its, &relres));
-    {
-        PsCtx cx = ctx(c);
-        RET(ps_project_mean<double>(cx, n0, n1, c->pc));
+    RET(ps_fill<double>(cx, c->pc, c->N));
+    if (graph) {
+        RET(psolve_graph(c, c->prm.p_rtol, c->prm.p_maxit));
+    } else {
+        RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, its, relres));
     }
+    RET(ps_project_mean<double>(cx, n0, n1, c->pc));
     // 6-7 correction and BCs (FP64 accumulation)
     k_correct<<<gr, bl, 0, c->s>>>(kg(c), c->prm.alpha_p, c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx,
                                    c->ety, c->u, c->v, c->p);
@@ -826,7 +907,74 @@ int outer_once(Case *c, cfd_stats *st) {
     k_nusselt<<<1, 256, 0, c->s>>>(kg(c), c->T, c->ajac, c->etx, c->ety, c->wall_len_hot, c->wall_len_cold,
                                    c->prm.t_hot - c->prm.t_cold, c->scal);
     CFD_LAUNCH_CHECK();
-    RET(sync_scal(c));
+    return CFD_OK;
+}
+
+void drop_graph(Case *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gkind.clear();
+    c->gev_used = 0;
+}
+
+// capture the outer iteration once (with the event pairs of live timing when enabled)
+int capture(Case *c) {
+    drop_graph(c);
+    CFD_HIP(hipStreamSynchronize(c->s));
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    CFD_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    const int rc = outer_body(c, true, nullptr, nullptr);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->s, &g);
+    if (rc != CFD_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        fprintf(stderr, "cfd_gpu: outer-iteration capture failed (rc %d, %s)\n", rc, hipGetErrorString(e));
+        c->ev_used = 0;
+        c->ev_kind.clear();
+        return rc != CFD_OK ? rc : CFD_E_HIP;
+    }
+    c->graph = g;
+    CFD_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->gkind = c->ev_kind;
+    c->gev_used = c->ev_used;
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    return CFD_OK;
+}
+
+// after a replay: fold the graph's event pairs into the totals (events stay in the graph)
+void drain_graph_events(Case *c) {
+    if (!c->prof) return;
+    for (int k = 0; k + 1 < c->gev_used; k += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+            const int kind = c->gkind[k / 2];
+            c->prof_ms[kind] += ms;
+            c->prof_n[kind] += 1;
+        }
+    }
+}
+
+int outer_once(Case *c, cfd_stats *st) {
+    int its = 0;
+    double relres = 0.0;
+    if (c->prm.graph) {
+        if (!c->gexec) RET(capture(c));
+        CFD_HIP(hipGraphLaunch(c->gexec, c->s));
+        CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        CFD_HIP(hipStreamSynchronize(c->s));
+        drain_graph_events(c);
+        its = c->mixed ? (int)c->hscal[S_ITTOT] : (int)c->hscal[S_IT];
+        relres = c->mixed ? c->hscal[S_RELRES64] : c->hscal[S_RELRES];
+    } else {
+        RET(outer_body(c, false, &its, &relres));
+        RET(sync_scal(c));
+    }
     const double *h = c->hscal;
     c->iter++;
     if (c->iter == 1) {
@@ -854,8 +1002,8 @@ template <typename T>
 void time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
     OpLevel<T> &L = ps->lev[0];
     const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-    if (kernel == 0) k_fapply<T><<<gr, bl, 0, c->s>>>(fargs(L), L.x, L.t);
-    else k_fsmooth<T, false><<<gr, bl, 0, c->s>>>(fargs(L), (T)0.8, L.x, L.b, L.invd, L.t);
+    if (kernel == 0) k_fapply<T><<<gr, bl, 0, c->s>>>(nullptr, fargs(L), L.x, L.t);
+    else k_fsmooth<T, false><<<gr, bl, 0, c->s>>>(nullptr, fargs(L), (T)0.8, L.x, L.b, L.invd, L.t);
 }
 
 }  // namespace
@@ -870,9 +1018,10 @@ void cfd_params_default(cfd_params *p) {
     p->Re = 100.0; p->Ri = 1.0; p->Pr = 0.71; p->dt = 1.0e4;
     p->alpha_u = 0.7; p->alpha_p = 0.3; p->alpha_t = 1.0;
     p->lid_u = 1.0; p->t_hot = 1.0; p->t_cold = 0.0;
-    p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 50;
+    p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 4;
     p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 30; p->mg_omega = 0.8;
     p->precision = CFD_FP64;
+    p->graph = 1;
 }
 
 int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, const double *y,
@@ -925,7 +1074,7 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
     if (rc == CFD_OK && c->mixed) {
         const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
         const int lb = 256, lg = (int)((N + lb - 1) / lb);
-        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, m[k], c->a32[k]);
+        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(nullptr, N, m[k], c->a32[k]);
         if (hipGetLastError() != hipSuccess) rc = CFD_E_HIP;
     }
     if (rc == CFD_OK) rc = build_levels(c);
@@ -952,6 +1101,7 @@ int cfd_case_destroy(cfd_case h) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c) return CFD_OK;
     if (c->s) (void)hipStreamSynchronize(c->s);
+    drop_graph(c);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
@@ -1038,7 +1188,7 @@ int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d
         if (!dst[k]) continue;
         if (c->mixed) {
             const int lb = 256, lg = (int)((N + lb - 1) / lb);
-            k_cvt<float, double><<<lg, lb, 0, c->s>>>(N, (const float *)src[k], c->uh);
+            k_cvt<float, double><<<lg, lb, 0, c->s>>>(nullptr, N, (const float *)src[k], c->uh);
             CFD_LAUNCH_CHECK();
             CFD_HIP(hipMemcpyAsync(dst[k], c->uh, B, hipMemcpyDeviceToHost, c->s));
             CFD_HIP(hipStreamSynchronize(c->s));
@@ -1057,7 +1207,7 @@ int cfd_case_p_apply(cfd_case h, const double *d, const double *x, double *y) {
     const size_t B = c->N * sizeof(double);
     CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
     CFD_HIP(hipMemcpyAsync(c->uh, x, B, hipMemcpyHostToDevice, c->s));
-    k_mapply<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(margs(c), c->uh, nullptr, c->vh);
+    k_mapply<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(nullptr, margs(c), c->uh, nullptr, c->vh);
     CFD_LAUNCH_CHECK();
     CFD_HIP(hipMemcpyAsync(y, c->vh, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
@@ -1091,7 +1241,11 @@ int cfd_case_profile(cfd_case h, int enable) {
         c->ev.resize(PROF_EVENTS);
         for (auto &e : c->ev) CFD_HIP(hipEventCreate(&e));
     }
-    c->prof = enable != 0;
+    if (enable != 2) {
+        const bool want = enable != 0;
+        if (want != c->prof) drop_graph(c);  // recapture with or without the event pairs
+        c->prof = want;
+    }
     c->ev_used = 0;
     c->ev_kind.clear();
     for (int k = 0; k < CFD_PROF_KINDS; k++) {

@@ -26,7 +26,12 @@ constexpr int PS_NT = BX * BY;
 // device scalar slots used by the solver (pairs reduced together are adjacent)
 enum {
     S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RV, S_TS, S_TT, S_SS, S_BB, S_RR, S_SUM, S_R0R,
-    S_MASS, S_ENERGY, S_MOM, S_DMAX, S_NUH, S_NUC, S_NSCAL
+    S_MASS, S_ENERGY, S_MOM, S_DMAX, S_NUH, S_NUC,
+    // device-side control of the captured (graph) solve
+    S_DONE, S_SCONV, S_TOL, S_IT, S_RN, S_BN, S_RELRES, S_RTOL,
+    // mixed-precision refinement (FP64 defect)
+    S_BB64, S_RR64, S_TOL64, S_RN64, S_BN64, S_RDONE, S_ITTOT, S_RELRES64,
+    S_NSCAL
 };
 
 // FP64 geometry of one multigrid level (level 0 aliases the case's arrays)
@@ -98,7 +103,8 @@ __device__ __forceinline__ T frow(const FArgs<T> &a, const T *__restrict__ x, in
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_fapply(FArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
+__global__ __launch_bounds__(PS_NT) void k_fapply(const double *gd, FArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
@@ -107,8 +113,9 @@ __global__ __launch_bounds__(PS_NT) void k_fapply(FArgs<T> a, const T *__restric
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_fresid(FArgs<T> a, const T *__restrict__ x, const T *__restrict__ b,
+__global__ __launch_bounds__(PS_NT) void k_fresid(const double *gd, FArgs<T> a, const T *__restrict__ x, const T *__restrict__ b,
                                                   T *__restrict__ r) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
@@ -131,8 +138,9 @@ __global__ __launch_bounds__(PS_NT) void k_finvdiag(FArgs<T> a, T *__restrict__ 
 
 // Jacobi sweep: xo = xi + w (b - L xi) invd (FIRST: xo = w b invd)
 template <typename T, bool FIRST>
-__global__ __launch_bounds__(PS_NT) void k_fsmooth(FArgs<T> a, T w, const T *__restrict__ xi, const T *__restrict__ b,
+__global__ __launch_bounds__(PS_NT) void k_fsmooth(const double *gd, FArgs<T> a, T w, const T *__restrict__ xi, const T *__restrict__ b,
                                                    const T *__restrict__ invd, T *__restrict__ xo) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
@@ -142,7 +150,8 @@ __global__ __launch_bounds__(PS_NT) void k_fsmooth(FArgs<T> a, T w, const T *__r
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_restrict(int n0, const T *__restrict__ r, int m0, int m1, T *__restrict__ bc) {
+__global__ __launch_bounds__(PS_NT) void k_restrict(const double *gd, int n0, const T *__restrict__ r, int m0, int m1, T *__restrict__ bc) {
+    if (gd && *gd != 0.0) return;
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= m0 || J >= m1) return;
     const size_t Pc = idx(I, J, m0);
@@ -156,8 +165,9 @@ __global__ __launch_bounds__(PS_NT) void k_restrict(int n0, const T *__restrict_
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_prolong_add(int n0, int n1, T *__restrict__ x, int m0, int m1,
+__global__ __launch_bounds__(PS_NT) void k_prolong_add(const double *gd, int n0, int n1, T *__restrict__ x, int m0, int m1,
                                                        const T *__restrict__ xc) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
     const int I = i >> 1, J = j >> 1;
@@ -172,7 +182,8 @@ __global__ __launch_bounds__(PS_NT) void k_prolong_add(int n0, int n1, T *__rest
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_sum_int(int n0, int n1, const T *__restrict__ v, double *__restrict__ part) {
+__global__ __launch_bounds__(PS_NT) void k_sum_int(const double *gd, int n0, int n1, const T *__restrict__ v, double *__restrict__ part) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double s = 0.0;
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) s = (double)v[idx(i, j, n0)];
@@ -181,8 +192,9 @@ __global__ __launch_bounds__(PS_NT) void k_sum_int(int n0, int n1, const T *__re
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_sub_mean(int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
+__global__ __launch_bounds__(PS_NT) void k_sub_mean(const double *gd, int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
                                                     int slot) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
     const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
@@ -196,23 +208,26 @@ __global__ void k_inject(int n0, const double *__restrict__ f, int m0, int m1, d
 }
 
 template <typename A, typename B>
-__global__ void k_cvt(size_t n, const A *__restrict__ a, B *__restrict__ b) {
+__global__ void k_cvt(const double *gd, size_t n, const A *__restrict__ a, B *__restrict__ b) {
+    if (gd && *gd != 0.0) return;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) b[k] = (B)a[k];
 }
 
 // acc += (double) e on every node
 template <typename T>
-__global__ void k_acc(size_t n, const T *__restrict__ e, double *__restrict__ acc) {
+__global__ void k_acc(const double *gd, size_t n, const T *__restrict__ e, double *__restrict__ acc) {
+    if (gd && *gd != 0.0) return;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) acc[k] += (double)e[k];
 }
 
 // --- BiCGStab vector kernels
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_dot2(int n0, int n1, const T *__restrict__ a, const T *__restrict__ b,
+__global__ __launch_bounds__(PS_NT) void k_dot2(const double *gd, int n0, int n1, const T *__restrict__ a, const T *__restrict__ b,
                                                 const T *__restrict__ c, const T *__restrict__ d2,
                                                 double *__restrict__ part, int pstride) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double s1 = 0.0, s2 = 0.0;
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
@@ -229,8 +244,9 @@ __global__ __launch_bounds__(PS_NT) void k_dot2(int n0, int n1, const T *__restr
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_p(int n0, int n1, bool first, const double *__restrict__ scal,
+__global__ __launch_bounds__(PS_NT) void k_bicg_p(const double *gd, int n0, int n1, bool first, const double *__restrict__ scal,
                                                   const T *__restrict__ r, const T *__restrict__ vv, T *__restrict__ pv) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= n0 || j >= n1) return;
     const size_t P = idx(i, j, n0);
@@ -240,8 +256,9 @@ __global__ __launch_bounds__(PS_NT) void k_bicg_p(int n0, int n1, bool first, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_s(int n0, int n1, double *__restrict__ scal, const T *__restrict__ r,
+__global__ __launch_bounds__(PS_NT) void k_bicg_s(const double *gd, int n0, int n1, double *__restrict__ scal, const T *__restrict__ r,
                                                   const T *__restrict__ vv, T *__restrict__ s, double *__restrict__ part) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const double alpha = scal[S_RHO] / scal[S_RV];
     double ss = 0.0;
@@ -266,11 +283,12 @@ __global__ __launch_bounds__(PS_NT) void k_bicg_x1(int n0, int n1, const double 
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_xr(int n0, int n1, const double *__restrict__ scal,
+__global__ __launch_bounds__(PS_NT) void k_bicg_xr(const double *gd, int n0, int n1, const double *__restrict__ scal,
                                                    const T *__restrict__ ph, const T *__restrict__ sh,
                                                    const T *__restrict__ s, const T *__restrict__ t,
                                                    const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r,
                                                    double *__restrict__ part, int pstride) {
+    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const double tt = scal[S_TT];
     const double omega = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
@@ -294,7 +312,8 @@ __global__ __launch_bounds__(PS_NT) void k_bicg_xr(int n0, int n1, const double 
     }
 }
 
-__global__ void k_set_omega(double *__restrict__ scal) {
+__global__ void k_set_omega(const double *gd, double *__restrict__ scal) {
+    if (gd && *gd != 0.0) return;
     const double tt = scal[S_TT];
     scal[S_OMEGA] = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
 }
@@ -310,8 +329,9 @@ __global__ void k_init_scal(double *__restrict__ scal) {
 }
 
 // sum of `nparts` partials for each of `nv` arrays (stride pstride) -> scal[dst + k]
-__global__ void k_reduce_sum(const double *__restrict__ part, int pstride, int nparts, int nv, double *__restrict__ scal,
+__global__ void k_reduce_sum(const double *gd, const double *__restrict__ part, int pstride, int nparts, int nv, double *__restrict__ scal,
                              int dst) {
+    if (gd && *gd != 0.0) return;
     const int k = blockIdx.x;
     if (k >= nv) return;
     double s = 0.0;
@@ -357,7 +377,8 @@ __device__ __forceinline__ void cw_smooth(const CLev<T> &L, const T *xi, T *xo, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(CW) void k_mg_coarse(CoarseArgs<T> A, int l0) {
+__global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T> A, int l0) {
+    if (gd && *gd != 0.0) return;
     __shared__ double red[CW / 64];
     T *xc[PS_MAXLEV], *tc[PS_MAXLEV];
     for (int l = 0; l < A.nlev; l++) { xc[l] = A.L[l].x; tc[l] = A.L[l].t; }
@@ -459,6 +480,76 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(CoarseArgs<T> A, int l0) {
     }
 }
 
+// --- small guarded helpers and the device-side control of the captured solve
+template <typename T>
+__global__ void k_fill(const double *gd, size_t n, T *__restrict__ a, T v) {
+    if (gd && *gd != 0.0) return;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+template <typename T>
+__global__ void k_copy(const double *gd, size_t n, const T *__restrict__ a, T *__restrict__ b) {
+    if (gd && *gd != 0.0) return;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = a[k];
+}
+// x += alpha ph, only when this iteration stopped at the ||s|| test
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bicg_x1g(int n0, int n1, const double *scal, const T *__restrict__ ph,
+                                                    T *__restrict__ x) {
+    if (scal[S_SCONV] == 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i >= n0 || j >= n1) return;
+    const size_t P = idx(i, j, n0);
+    x[P] = (T)((double)x[P] + scal[S_ALPHA] * (double)ph[P]);
+}
+// after (b,b), (r,r): tolerance, initial test and start values; an outer guard
+// (the refinement's done flag) marks the whole solve as done
+__global__ void k_g_init(double *scal, double rtol, int rtol_slot, const double *outer) {
+    const double bn = sqrt(scal[S_BB]);
+    const double rt = rtol_slot >= 0 ? scal[rtol_slot] : rtol;
+    const double tol = rt * bn;
+    const double rn = sqrt(scal[S_RR]);
+    scal[S_BN] = bn;
+    scal[S_TOL] = tol;
+    scal[S_RN] = rn;
+    scal[S_IT] = 0.0;
+    scal[S_SCONV] = 0.0;
+    scal[S_RHO] = scal[S_RR];
+    scal[S_RHO_OLD] = 1.0;
+    scal[S_ALPHA] = 1.0;
+    scal[S_OMEGA] = 1.0;
+    scal[S_DONE] = (bn == 0.0 || rn <= tol || (outer && *outer != 0.0)) ? 1.0 : 0.0;
+}
+__global__ void k_g_begin(double *scal, int k) {
+    if (scal[S_DONE] != 0.0) return;
+    scal[S_IT] = (double)k;
+    if (scal[S_RHO] == 0.0) scal[S_DONE] = 1.0;
+}
+__global__ void k_g_check_s(double *scal) {
+    if (scal[S_DONE] != 0.0) { scal[S_SCONV] = 0.0; return; }
+    const double sn = sqrt(scal[S_SS]);
+    if (sn <= scal[S_TOL]) {
+        scal[S_SCONV] = 1.0;
+        scal[S_DONE] = 1.0;
+        scal[S_RN] = sn;
+    } else {
+        scal[S_SCONV] = 0.0;
+    }
+}
+__global__ void k_g_check_r(double *scal) {
+    if (scal[S_DONE] != 0.0) return;
+    const double rn = sqrt(scal[S_RR]);
+    scal[S_RN] = rn;
+    if (rn <= scal[S_TOL] || scal[S_OMEGA] == 0.0) {
+        scal[S_DONE] = 1.0;
+    } else {
+        scal[S_RHO_OLD] = scal[S_RHO];
+        scal[S_RHO] = scal[S_R0R];
+    }
+}
+__global__ void k_g_final(double *scal) { scal[S_RELRES] = scal[S_BN] > 0.0 ? scal[S_RN] / scal[S_BN] : 0.0; }
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -490,8 +581,8 @@ inline int nblk(int n0, int n1) {
     return (int)(g.x * g.y);
 }
 
-inline int ps_reduce(PsCtx &cx, int nparts, int nv, int dst) {
-    k_reduce_sum<<<nv, 256, 0, cx.s>>>(cx.part, cx.pstride, nparts, nv, cx.scal, dst);
+inline int ps_reduce(PsCtx &cx, int nparts, int nv, int dst, const double *gd = nullptr) {
+    k_reduce_sum<<<nv, 256, 0, cx.s>>>(gd, cx.part, cx.pstride, nparts, nv, cx.scal, dst);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -504,11 +595,24 @@ inline int ps_sync(PsCtx &cx) {
 }
 
 template <typename T>
-int ps_project_mean(PsCtx &cx, int n0, int n1, T *v) {
-    k_sum_int<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(n0, n1, v, cx.part);
+int ps_project_mean(PsCtx &cx, int n0, int n1, T *v, const double *gd = nullptr) {
+    k_sum_int<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(gd, n0, n1, v, cx.part);
     CFD_LAUNCH_CHECK();
-    CFD_RET(ps_reduce(cx, nblk(n0, n1), 1, S_SUM));
-    k_sub_mean<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(n0, n1, v, cx.scal, S_SUM);
+    CFD_RET(ps_reduce(cx, nblk(n0, n1), 1, S_SUM, gd));
+    k_sub_mean<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(gd, n0, n1, v, cx.scal, S_SUM);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+template <typename T>
+int ps_fill(PsCtx &cx, T *a, size_t n, const double *gd = nullptr) {
+    k_fill<T><<<(unsigned)((n + 255) / 256), 256, 0, cx.s>>>(gd, n, a, T(0));
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+template <typename T>
+int ps_copy(PsCtx &cx, const T *a, T *b, size_t n, const double *gd = nullptr) {
+    k_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, cx.s>>>(gd, n, a, b);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -569,18 +673,18 @@ struct PSolver {
         return CFD_OK;
     }
 
-    // sweeps on level l with x -> result in L.x
-    int smooth(PsCtx &cx, int l, int sweeps, bool from_zero) {
+    // sweeps on level l; result in L.x (host-side ping-pong of x and t)
+    int smooth(PsCtx &cx, int l, int sweeps, bool from_zero, const double *gd) {
         OpLevel<T> &L = lev[l];
         const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
         const FArgs<T> a = fargs(L);
         const T w = (T)omega;
         for (int s = 0; s < sweeps; s++) {
             if (from_zero && s == 0) {
-                k_fsmooth<T, true><<<gr, bl, 0, cx.s>>>(a, w, L.x, L.b, L.invd, L.x);
+                k_fsmooth<T, true><<<gr, bl, 0, cx.s>>>(gd, a, w, L.x, L.b, L.invd, L.x);
             } else {
                 if (l == 0) cx.prof_b(cx.owner, 4);
-                k_fsmooth<T, false><<<gr, bl, 0, cx.s>>>(a, w, L.x, L.b, L.invd, L.t);
+                k_fsmooth<T, false><<<gr, bl, 0, cx.s>>>(gd, a, w, L.x, L.b, L.invd, L.t);
                 if (l == 0) cx.prof_e(cx.owner);
                 std::swap(L.x, L.t);
             }
@@ -589,69 +693,71 @@ struct PSolver {
         return CFD_OK;
     }
 
-    int vcycle(PsCtx &cx, int l) {
+    int vcycle(PsCtx &cx, int l, const double *gd) {
         OpLevel<T> &L = lev[l];
         if (l == lc) {
             // the rest of the cycle in one workgroup; the level's x/t roles are fixed there
-            k_mg_coarse<T><<<1, CW, 0, cx.s>>>(cargs, l);
+            k_mg_coarse<T><<<1, CW, 0, cx.s>>>(gd, cargs, l);
             CFD_LAUNCH_CHECK();
             return CFD_OK;
         }
-        const size_t M = (size_t)L.n0 * L.n1 * sizeof(T);
+        const size_t M = (size_t)L.n0 * L.n1;
         const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-        if (l == nlev - 1) {
-            if (coarse <= 0) {
-                CFD_HIP(hipMemsetAsync(L.x, 0, M, cx.s));
-                return CFD_OK;
-            }
-            return smooth(cx, l, coarse, true);
-        }
         if (pre > 0) {
-            CFD_RET(smooth(cx, l, pre, true));
+            CFD_RET(smooth(cx, l, pre, true, gd));
         } else {
-            CFD_HIP(hipMemsetAsync(L.x, 0, M, cx.s));
+            CFD_RET(ps_fill<T>(cx, L.x, M, gd));
         }
-        k_fresid<T><<<gr, bl, 0, cx.s>>>(fargs(L), L.x, L.b, L.r);
+        k_fresid<T><<<gr, bl, 0, cx.s>>>(gd, fargs(L), L.x, L.b, L.r);
         CFD_LAUNCH_CHECK();
         OpLevel<T> &C = lev[l + 1];
-        k_restrict<T><<<grid2d(C.n0, C.n1), bl, 0, cx.s>>>(L.n0, L.r, C.n0, C.n1, C.b);
+        k_restrict<T><<<grid2d(C.n0, C.n1), bl, 0, cx.s>>>(gd, L.n0, L.r, C.n0, C.n1, C.b);
         CFD_LAUNCH_CHECK();
-        CFD_RET(ps_project_mean<T>(cx, C.n0, C.n1, C.b));
-        CFD_RET(vcycle(cx, l + 1));
-        k_prolong_add<T><<<gr, bl, 0, cx.s>>>(L.n0, L.n1, L.x, C.n0, C.n1, C.x);
+        CFD_RET(ps_project_mean<T>(cx, C.n0, C.n1, C.b, gd));
+        CFD_RET(vcycle(cx, l + 1, gd));
+        k_prolong_add<T><<<gr, bl, 0, cx.s>>>(gd, L.n0, L.n1, L.x, C.n0, C.n1, C.x);
         CFD_LAUNCH_CHECK();
-        return smooth(cx, l, post, false);
+        return smooth(cx, l, post, false, gd);
     }
 
-    int precond(PsCtx &cx, const T *y, T *z) {
+    int precond(PsCtx &cx, const T *y, T *z, const double *gd) {
         OpLevel<T> &L = lev[0];
-        const size_t M = (size_t)L.n0 * L.n1 * sizeof(T);
-        CFD_HIP(hipMemcpyAsync(L.b, y, M, hipMemcpyDeviceToDevice, cx.s));
-        CFD_RET(vcycle(cx, 0));
-        CFD_HIP(hipMemcpyAsync(z, L.x, M, hipMemcpyDeviceToDevice, cx.s));
+        const size_t M = (size_t)L.n0 * L.n1;
+        CFD_RET(ps_copy<T>(cx, y, L.b, M, gd));
+        CFD_RET(vcycle(cx, 0, gd));
+        CFD_RET(ps_copy<T>(cx, L.x, z, M, gd));
         return CFD_OK;
     }
 
-    // solve L x = b to ||r|| <= rtol ||b|| (b's interior is projected to zero mean
-    // first); x must be zero on entry.  Mirrors the oracle's bicgstab().
+    // r = b - A x (x = 0 on entry), projected; r0 = r; (b,b) and (r,r) into S_BB, S_RR
+    int start(PsCtx &cx, const T *b, T *x, const double *gd) {
+        OpLevel<T> &L0 = lev[0];
+        const int n0 = L0.n0, n1 = L0.n1;
+        const size_t M = (size_t)n0 * n1;
+        const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+        T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3];
+        k_fresid<T><<<gr, bl, 0, cx.s>>>(gd, fargs(L0), x, b, r);
+        CFD_LAUNCH_CHECK();
+        CFD_RET(ps_project_mean<T>(cx, n0, n1, r, gd));
+        CFD_RET(ps_copy<T>(cx, r, r0, M, gd));
+        CFD_RET(ps_fill<T>(cx, pv, M, gd));
+        CFD_RET(ps_fill<T>(cx, vv, M, gd));
+        k_dot2<T><<<gr, bl, 0, cx.s>>>(gd, n0, n1, b, b, r, r, cx.part, cx.pstride);
+        CFD_LAUNCH_CHECK();
+        return ps_reduce(cx, nblk(n0, n1), 2, S_BB, gd);
+    }
+
+    // Synchronous solve: L x = b to ||r|| <= rtol ||b|| with host-side tests,
+    // iterate for iterate the oracle's bicgstab(); x must be zero on entry.
     int solve(PsCtx &cx, const T *b, T *x, double rtol, int maxit, int *its_out, double *relres) {
         OpLevel<T> &L0 = lev[0];
         const int n0 = L0.n0, n1 = L0.n1;
-        const size_t M = (size_t)n0 * n1 * sizeof(T);
         const dim3 gr = grid2d(n0, n1), bl(BX, BY);
         const int nb = nblk(n0, n1);
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
-        k_fresid<T><<<gr, bl, 0, cx.s>>>(a0, x, b, r);
-        CFD_LAUNCH_CHECK();
-        CFD_RET(ps_project_mean<T>(cx, n0, n1, r));
-        CFD_HIP(hipMemcpyAsync(r0, r, M, hipMemcpyDeviceToDevice, cx.s));
-        CFD_HIP(hipMemsetAsync(pv, 0, M, cx.s));
-        CFD_HIP(hipMemsetAsync(vv, 0, M, cx.s));
-        k_dot2<T><<<gr, bl, 0, cx.s>>>(n0, n1, b, b, r, r, cx.part, cx.pstride);
-        CFD_LAUNCH_CHECK();
-        CFD_RET(ps_reduce(cx, nb, 2, S_BB));
-        k_init_scal<<<1, 1, 0, cx.s>>>(cx.scal);
+        CFD_RET(start(cx, b, x, nullptr));
+        k_g_init<<<1, 1, 0, cx.s>>>(cx.scal, rtol, -1, nullptr);
         CFD_LAUNCH_CHECK();
         CFD_RET(ps_sync(cx));
         const double bnorm = std::sqrt(cx.hscal[S_BB]);
@@ -664,17 +770,17 @@ struct PSolver {
         int it;
         for (it = 1; it <= maxit; it++) {
             if (rho == 0.0) break;
-            k_bicg_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it == 1, cx.scal, r, vv, pv);
+            k_bicg_p<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, it == 1, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
-            CFD_RET(precond(cx, pv, ph));
+            CFD_RET(precond(cx, pv, ph, nullptr));
             cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(a0, ph, vv);
+            k_fapply<T><<<gr, bl, 0, cx.s>>>(nullptr, a0, ph, vv);
             cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(n0, n1, r0, vv, nullptr, nullptr, cx.part, cx.pstride);
+            k_dot2<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, r0, vv, nullptr, nullptr, cx.part, cx.pstride);
             CFD_LAUNCH_CHECK();
             CFD_RET(ps_reduce(cx, nb, 1, S_RV));
-            k_bicg_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, cx.scal, r, vv, s, cx.part);
+            k_bicg_s<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, cx.scal, r, vv, s, cx.part);
             CFD_LAUNCH_CHECK();
             CFD_RET(ps_reduce(cx, nb, 1, S_SS));
             CFD_RET(ps_sync(cx));
@@ -685,20 +791,20 @@ struct PSolver {
                 rnorm = snorm;
                 break;
             }
-            CFD_RET(precond(cx, s, sh));
+            CFD_RET(precond(cx, s, sh, nullptr));
             cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(a0, sh, t);
+            k_fapply<T><<<gr, bl, 0, cx.s>>>(nullptr, a0, sh, t);
             cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(n0, n1, t, s, t, t, cx.part, cx.pstride);
+            k_dot2<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, t, s, t, t, cx.part, cx.pstride);
             CFD_LAUNCH_CHECK();
             CFD_RET(ps_reduce(cx, nb, 2, S_TS));
-            k_set_omega<<<1, 1, 0, cx.s>>>(cx.scal);
+            k_set_omega<<<1, 1, 0, cx.s>>>(nullptr, cx.scal);
             CFD_LAUNCH_CHECK();
-            k_bicg_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, cx.scal, ph, sh, s, t, r0, x, r, cx.part, cx.pstride);
+            k_bicg_xr<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, cx.scal, ph, sh, s, t, r0, x, r, cx.part, cx.pstride);
             CFD_LAUNCH_CHECK();
             CFD_RET(ps_reduce(cx, nb, 1, S_RR));
-            k_reduce_sum<<<1, 256, 0, cx.s>>>(cx.part + cx.pstride, cx.pstride, nb, 1, cx.scal, S_R0R);
+            k_reduce_sum<<<1, 256, 0, cx.s>>>(nullptr, cx.part + cx.pstride, cx.pstride, nb, 1, cx.scal, S_R0R);
             CFD_LAUNCH_CHECK();
             k_rotate_rho<<<1, 1, 0, cx.s>>>(cx.scal);
             CFD_LAUNCH_CHECK();
@@ -710,6 +816,59 @@ struct PSolver {
         }
         *its_out = it > maxit ? maxit : it;
         *relres = rnorm / bnorm;
+        return CFD_OK;
+    }
+
+    // Capturable solve: exactly `maxit` unrolled iterations whose kernels turn into
+    // no-ops once the device-side tests (the same ones as solve()) have fired; no
+    // host synchronisation.  Result: iterations in S_IT, relative residual in
+    // S_RELRES.  rtol comes from `rtol` or, if rtol_slot >= 0, from scal[rtol_slot];
+    // `outer` (may be null) skips the whole solve when nonzero.
+    int solve_graph(PsCtx &cx, const T *b, T *x, double rtol, int rtol_slot, int maxit, const double *outer) {
+        OpLevel<T> &L0 = lev[0];
+        const int n0 = L0.n0, n1 = L0.n1;
+        const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+        const int nb = nblk(n0, n1);
+        T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
+        const FArgs<T> a0 = fargs(L0);
+        const double *dn = cx.scal + S_DONE;
+        CFD_RET(start(cx, b, x, outer));
+        k_g_init<<<1, 1, 0, cx.s>>>(cx.scal, rtol, rtol_slot, outer);
+        CFD_LAUNCH_CHECK();
+        for (int it = 1; it <= maxit; it++) {
+            k_g_begin<<<1, 1, 0, cx.s>>>(cx.scal, it);
+            k_bicg_p<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, it == 1, cx.scal, r, vv, pv);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(precond(cx, pv, ph, dn));
+            cx.prof_b(cx.owner, 0);
+            k_fapply<T><<<gr, bl, 0, cx.s>>>(dn, a0, ph, vv);
+            cx.prof_e(cx.owner);
+            k_dot2<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, r0, vv, nullptr, nullptr, cx.part, cx.pstride);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(ps_reduce(cx, nb, 1, S_RV, dn));
+            k_bicg_s<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, cx.scal, r, vv, s, cx.part);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(ps_reduce(cx, nb, 1, S_SS, dn));
+            k_g_check_s<<<1, 1, 0, cx.s>>>(cx.scal);
+            k_bicg_x1g<T><<<gr, bl, 0, cx.s>>>(n0, n1, cx.scal, ph, x);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(precond(cx, s, sh, dn));
+            cx.prof_b(cx.owner, 0);
+            k_fapply<T><<<gr, bl, 0, cx.s>>>(dn, a0, sh, t);
+            cx.prof_e(cx.owner);
+            k_dot2<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, t, s, t, t, cx.part, cx.pstride);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(ps_reduce(cx, nb, 2, S_TS, dn));
+            k_set_omega<<<1, 1, 0, cx.s>>>(dn, cx.scal);
+            k_bicg_xr<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, cx.scal, ph, sh, s, t, r0, x, r, cx.part, cx.pstride);
+            CFD_LAUNCH_CHECK();
+            CFD_RET(ps_reduce(cx, nb, 1, S_RR, dn));
+            k_reduce_sum<<<1, 256, 0, cx.s>>>(dn, cx.part + cx.pstride, cx.pstride, nb, 1, cx.scal, S_R0R);
+            k_g_check_r<<<1, 1, 0, cx.s>>>(cx.scal);
+            CFD_LAUNCH_CHECK();
+        }
+        k_g_final<<<1, 1, 0, cx.s>>>(cx.scal);
+        CFD_LAUNCH_CHECK();
         return CFD_OK;
     }
 };

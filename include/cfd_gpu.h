@@ -164,6 +164,9 @@ typedef struct {
     int mg_pre, mg_post, mg_coarse;
     double mg_omega;    /* damping of the Jacobi smoother                        */
     int precision;      /* CFD_FP64 or CFD_MIXED                                 */
+    int graph;          /* 1: capture one outer iteration as a hipGraph and replay it;
+                           the p' BiCGStab then runs p_maxit unrolled iterations with
+                           device-side convergence tests (no host synchronisation)  */
 } cfd_params;
 
 typedef struct {

@@ -119,3 +119,35 @@ def test_mixed_converged_fields_match_fp64_oracle():
     print("[gpu] mixed vs FP64 per-field rel L2:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v < 1e-6, (k, v)
+
+
+def test_graph_replays_match_eager():
+    """The captured outer iteration, replayed with host work and syncs in between, equals eager runs."""
+    print("\n[gpu] graph vs eager, 65x65, 30 outer iterations", flush=True)
+    from cfd_solver_amd.cavity import CavityCase, default_params
+    g = cavity_grid(65, skew=0.1, beta=1.5)
+    for prec in (0, 1):
+        eager = CavityCase(g, default_params(Re=400.0, Ri=1.0, graph=0, precision=prec))
+        graph = CavityCase(g, default_params(Re=400.0, Ri=1.0, graph=1, precision=prec))
+        for k in range(30):
+            se, sg = eager.step(), graph.step()
+            fe, fg = eager.fields(), graph.fields()   # host work between replays
+            for name in ("u", "v", "T", "p"):
+                assert rel_l2(fg[name], fe[name]) < 1e-13, (prec, k, name)
+            assert se.p_iters == sg.p_iters, (prec, k)
+            assert abs(se.mass_l1 - sg.mass_l1) <= 1e-12 * se.mass_l1
+
+
+def test_live_timing_in_graph_mode():
+    print("\n[gpu] live kernel timing inside the replayed graph", flush=True)
+    from cfd_solver_amd.cavity import CavityCase
+    c = CavityCase(cavity_grid(129, skew=0.1, beta=1.5), Re=400.0, Ri=1.0)
+    c.step()
+    c.profile(True)
+    c.step()          # recapture with the event pairs
+    c.profile(2)      # reset the counters, keep the graph
+    for _ in range(3):
+        c.step()
+    prof = c.profile_read()
+    assert prof["jacobi"][1] == 3 * 7 and prof["jacobi"][0] > 0
+    assert prof["residual"][1] == 3 and prof["face_div"][1] == 3
