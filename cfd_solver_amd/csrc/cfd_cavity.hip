@@ -72,8 +72,6 @@ struct Case {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool capturing = false;
-    std::vector<int> gkind;     // event-pair kinds recorded inside the graph
-    int gev_used = 0;
 };
 
 // ------------------------------------------------------------------------
@@ -584,7 +582,7 @@ int dalloc(Case *c, T **p, size_t n) {
 // kinds: 0 p' operator apply (fine level), 1 predictor Jacobi sweep,
 // 2 residual/RHS, 3 face divergence, 4 multigrid smoothing sweep (fine level)
 void prof_begin(Case *c, int kind) {
-    if (!c->prof) return;
+    if (!c->prof || c->capturing) return;
     if (c->ev_kind.size() * 2 > (size_t)c->ev_used) c->ev_kind.pop_back();  // drop a dangling begin
     if (c->ev_used + 2 > (int)c->ev.size()) return;
     (void)hipEventRecord(c->ev[c->ev_used], c->s);
@@ -606,6 +604,7 @@ void prof_drain(Case *c) {
             c->prof_n[kind] += 1;
         }
     }
+    (void)hipGetLastError();  // a failed query must not surface at the next launch check
     c->ev_used = 0;
     c->ev_kind.clear();
 }
@@ -913,10 +912,9 @@ int outer_body(Case *c, bool graph, int *its, double *relres) {
 void drop_graph(Case *c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    (void)hipGetLastError();
     c->gexec = nullptr;
     c->graph = nullptr;
-    c->gkind.clear();
-    c->gev_used = 0;
 }
 
 // capture the outer iteration once (with the event pairs of live timing when enabled)
@@ -940,35 +938,18 @@ int capture(Case *c) {
     }
     c->graph = g;
     CFD_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-    c->gkind = c->ev_kind;
-    c->gev_used = c->ev_used;
-    c->ev_used = 0;
-    c->ev_kind.clear();
     return CFD_OK;
-}
-
-// after a replay: fold the graph's event pairs into the totals (events stay in the graph)
-void drain_graph_events(Case *c) {
-    if (!c->prof) return;
-    for (int k = 0; k + 1 < c->gev_used; k += 2) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
-            const int kind = c->gkind[k / 2];
-            c->prof_ms[kind] += ms;
-            c->prof_n[kind] += 1;
-        }
-    }
 }
 
 int outer_once(Case *c, cfd_stats *st) {
     int its = 0;
     double relres = 0.0;
-    if (c->prm.graph) {
+    // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
+    if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
         CFD_HIP(hipGraphLaunch(c->gexec, c->s));
         CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, c->s));
         CFD_HIP(hipStreamSynchronize(c->s));
-        drain_graph_events(c);
         its = c->mixed ? (int)c->hscal[S_ITTOT] : (int)c->hscal[S_IT];
         relres = c->mixed ? c->hscal[S_RELRES64] : c->hscal[S_RELRES];
     } else {
@@ -1109,6 +1090,7 @@ int cfd_case_destroy(cfd_case h) {
     delete c->ps64;
     delete c->ps32;
     delete c;
+    (void)hipGetLastError();
     return CFD_OK;
 }
 
@@ -1241,11 +1223,7 @@ int cfd_case_profile(cfd_case h, int enable) {
         c->ev.resize(PROF_EVENTS);
         for (auto &e : c->ev) CFD_HIP(hipEventCreate(&e));
     }
-    if (enable != 2) {
-        const bool want = enable != 0;
-        if (want != c->prof) drop_graph(c);  // recapture with or without the event pairs
-        c->prof = want;
-    }
+    if (enable != 2) c->prof = enable != 0;
     c->ev_used = 0;
     c->ev_kind.clear();
     for (int k = 0; k < CFD_PROF_KINDS; k++) {

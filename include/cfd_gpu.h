@@ -215,10 +215,12 @@ int cfd_case_p_apply(cfd_case c, const double *d, const double *x, double *y);
 int cfd_case_p_solve(cfd_case c, const double *d, const double *q, double *x,
                      double rtol, int maxit, int *iters, double *relres);
 
-/* Live kernel timing.  While enabled, every fine-level launch of the hot
- * kernels inside cfd_case_outer/solve is bracketed by HIP events on the
- * case's stream; cfd_case_profile_read returns, per kind, the summed device
- * milliseconds and the launch count since the last cfd_case_profile call.
+/* Live kernel timing.  While enabled (enable = 1), outer iterations run
+ * eagerly (not from the captured graph) and every fine-level launch of the
+ * hot kernels is bracketed by HIP events on the case's stream;
+ * cfd_case_profile_read returns, per kind, the summed device milliseconds and
+ * the launch count since the last cfd_case_profile call (enable = 2 only
+ * resets the counters).
  * Kinds: 0 p' operator apply, 1 predictor Jacobi sweep, 2 residual/RHS,
  * 3 face divergence, 4 multigrid smoothing sweep. */
 int cfd_case_profile(cfd_case c, int enable);

@@ -142,10 +142,10 @@ def test_live_timing_in_graph_mode():
     print("\n[gpu] live kernel timing inside the replayed graph", flush=True)
     from cfd_solver_amd.cavity import CavityCase
     c = CavityCase(cavity_grid(129, skew=0.1, beta=1.5), Re=400.0, Ri=1.0)
+    c.step()          # captured graph
+    c.profile(True)   # profiled iterations run eagerly
     c.step()
-    c.profile(True)
-    c.step()          # recapture with the event pairs
-    c.profile(2)      # reset the counters, keep the graph
+    c.profile(2)      # reset the counters
     for _ in range(3):
         c.step()
     prof = c.profile_read()
