@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 def kernel_bytes(name, mixed):
     w = 4 if mixed else 8  # predictor and p' inner solve precision
     return {
-        "p_apply": 6 * w,          # x, 4 face coefficients -> y
-        "mg_smooth": 8 * w,        # x, b, 1/diag, 4 face coefficients -> x
+        "p_apply": 7 * w,          # x, 4 face coefficients, r0 (or s) -> y; dot product in the kernel
+        "mg_down": 7.25 * w,       # b, 1/diag, 4 face coefficients -> x, restricted residual (1/4 node)
+        "mg_up": 8.25 * w,         # x, b, 1/diag, 4 faces, coarse x (1/4 node) -> x
         "jacobi": 16 * w,          # R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
         "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
         "face_div": 10 * 8,        # u^,v^,p,d, 4 metrics, J -> q (FP64)
@@ -106,7 +107,7 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
     if launches <= 0 or ms_tot <= 0.0:
         raise RuntimeError(f"live kernel timing recorded nothing: {prof}")
     avg_ms = ms_tot / launches
-    bytes_launch = kernel_bytes(name, mixed) * n0 * n1
+    bytes_launch = int(kernel_bytes(name, mixed) * n0 * n1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,

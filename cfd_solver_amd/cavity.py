@@ -94,7 +94,7 @@ class CavityCase:
         call("cfd_case_p_solve", self._h, f64(d), f64(q), x, rtol, maxit, C.byref(it), C.byref(rr))
         return x, it.value, rr.value
 
-    KERNELS = ("p_apply", "jacobi", "residual", "face_div", "mg_smooth")
+    KERNELS = ("p_apply", "jacobi", "residual", "face_div", "mg_down", "mg_up")
 
     def profile(self, enable=True):
         """True: time hot kernels live (recaptures the graph); False: stop; 2: reset counters."""

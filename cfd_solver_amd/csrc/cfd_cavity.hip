@@ -50,8 +50,9 @@ struct Case {
     int nlev = 0;
     PSolver<double> *ps64 = nullptr;
     PSolver<float> *ps32 = nullptr;
-    float *rf = nullptr, *ef = nullptr;    // mixed: FP32 right-hand side / correction
+    float *ef = nullptr;                   // mixed: FP32 correction of one refinement step
     double *r64 = nullptr;                 // mixed: FP64 defect
+    double *f64[4] = {nullptr};            // mixed: level-0 face coefficients in FP64 (defect)
     double *part = nullptr;  // reduction partials: 4 x pstride
     int pstride = 0;
     double *scal = nullptr;  // device scalars
@@ -77,16 +78,6 @@ struct Case {
 // ------------------------------------------------------------------------
 // reductions
 
-__global__ void k_reduce_max(const double *__restrict__ part, int nparts, double *__restrict__ scal,
-                             int dst) {
-    double s = 0.0;  // dmax is floored at 0 (solver.cpp:1806)
-    for (int t = threadIdx.x; t < nparts; t += blockDim.x) s = fmax(s, part[t]);
-    s = block_max<256>(s);
-    if (threadIdx.x == 0) scal[dst] = fmax(s, 0.0);
-}
-
-__device__ __forceinline__ int block_lin() { return blockIdx.x + blockIdx.y * gridDim.x; }
-
 // ------------------------------------------------------------------------
 // residual + d factors (oracle: or_cav_residuals)
 
@@ -102,7 +93,7 @@ __global__ __launch_bounds__(NT) void k_residual(
     const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
     const double *__restrict__ p1, const double *__restrict__ q1, TR *__restrict__ Ru,
     TR *__restrict__ Rv, TR *__restrict__ RT, double *__restrict__ dd, TR *__restrict__ Ua,
-    TR *__restrict__ Va, double *__restrict__ part, int pstride) {
+    TR *__restrict__ Va, RedArgs ra) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     const double h1 = g.h1, h2 = g.h2;
@@ -151,12 +142,8 @@ __global__ __launch_bounds__(NT) void k_residual(
         e_abs = fabs(rt);
         m_abs = fabs(ru) + fabs(rv);
     }
-    const double se = block_sum<NT>(e_abs);
-    const double sm = block_sum<NT>(m_abs);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[block_lin()] = se;
-        part[pstride + block_lin()] = sm;
-    }
+    double sums[2] = {e_abs, m_abs};
+    block_partials<2>(sums, ra.part, ra.pstride);
 }
 
 // d on the border = nearest interior value (corners from the diagonal)
@@ -213,12 +200,13 @@ __global__ __launch_bounds__(NT) void k_jacobi(
     nT[P] = st * (TR)ph.alpha_t / ct[4];
 }
 
-// u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u
+// u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u.
+// Also completes d on the border (written there by no other kernel of the iteration).
 template <typename TR>
 __global__ __launch_bounds__(NT) void k_star(
     KGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
     const TR *__restrict__ dv, const TR *__restrict__ dT, double *__restrict__ T,
-    const double *__restrict__ p, const double *__restrict__ d, const double *__restrict__ xix,
+    const double *__restrict__ p, double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     double *__restrict__ us, double *__restrict__ vs, double *__restrict__ uh, double *__restrict__ vh) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -228,6 +216,8 @@ __global__ __launch_bounds__(NT) void k_star(
     if (i == 0 || j == 0 || i == n0 - 1 || j == n1 - 1) {
         us[P] = u[P];
         vs[P] = v[P];
+        // d on the border = nearest interior value (corners from the diagonal)
+        d[P] = d[idx(clampi(i, 1, n0 - 2), clampi(j, 1, n1 - 2), n0)];
         return;
     }
     const double a = u[P] + (double)du[P], b = v[P] + (double)dv[P];
@@ -245,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_face_div(
     KGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
     const double *__restrict__ d, const double *__restrict__ xix, const double *__restrict__ xiy,
     const double *__restrict__ etx, const double *__restrict__ ety, const double *__restrict__ jac,
-    double *__restrict__ q, double *__restrict__ part) {
+    double *__restrict__ q, RedArgs ra) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     const double h1 = g.h1, h2 = g.h2;
@@ -291,49 +281,52 @@ __global__ __launch_bounds__(NT) void k_face_div(
         q[P] = qq;
         aq = fabs(qq);
     }
-    const double s = block_sum<NT>(aq);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
+    double v[1] = {aq};
+    block_partials<1>(v, ra.part, ra.pstride);
 }
 
-// u = u* - d grad p' (p' read clamped = wall copy), p += alpha_p p'
+// u = u* - d grad p' (p' read clamped = wall copy), p += alpha_p p', with p' taken
+// with its interior mean (scal[S_PSUM] / interior count) removed
 __global__ __launch_bounds__(NT) void k_correct(
     KGrid g, double alpha_p, const double *__restrict__ us, const double *__restrict__ vs,
     const double *__restrict__ pc, const double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
-    double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const double *__restrict__ scal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const double m = scal[S_PSUM] / ((double)(n0 - 2) * (double)(n1 - 2));
     const size_t P = idx(i, j, n0);
-    const double pe = pc[idx(clampi(i + 1, 1, n0 - 2), j, n0)], pw = pc[idx(clampi(i - 1, 1, n0 - 2), j, n0)];
-    const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)], ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)];
+    const double pe = pc[idx(clampi(i + 1, 1, n0 - 2), j, n0)] - m, pw = pc[idx(clampi(i - 1, 1, n0 - 2), j, n0)] - m;
+    const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)] - m, ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)] - m;
     const double dpx = 0.5 * (pe - pw) / g.h1;
     const double dpe = 0.5 * (pn - ps) / g.h2;
     u[P] = us[P] - d[P] * (xix[P] * dpx + etx[P] * dpe);
     v[P] = vs[P] - d[P] * (xiy[P] * dpx + ety[P] * dpe);
-    p[P] = p[P] + alpha_p * pc[P];
+    p[P] = p[P] + alpha_p * (pc[P] - m);
 }
 
-// adiabatic side walls (oracle: temperature_sides): new values into scratch
-__global__ void k_t_sides(KGrid g, const double *__restrict__ T, const double *__restrict__ alph,
-                          const double *__restrict__ beta, double *__restrict__ scratch) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// adiabatic side walls (oracle: temperature_sides), one workgroup: every new value
+// is computed from the old wall column (into scratch) before any is written
+constexpr int TS_NT = 1024;
+__global__ __launch_bounds__(TS_NT) void k_t_sides(KGrid g, double *__restrict__ T, const double *__restrict__ alph,
+                                                   const double *__restrict__ beta, double *__restrict__ scratch) {
     const int n0 = g.n0, n1 = g.n1;
-    if (j < 1 || j > n1 - 2) return;
-    size_t P = idx(0, j, n0);
-    double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
-    double txi = beta[P] / alph[P] * te;
-    scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
-    P = idx(n0 - 1, j, n0);
-    te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
-    txi = beta[P] / alph[P] * te;
-    scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
-}
-__global__ void k_t_sides_write(KGrid g, const double *__restrict__ scratch, double *__restrict__ T) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < 1 || j > g.n1 - 2) return;
-    T[idx(0, j, g.n0)] = scratch[j];
-    T[idx(g.n0 - 1, j, g.n0)] = scratch[g.n1 + j];
+    for (int j = 1 + (int)threadIdx.x; j <= n1 - 2; j += TS_NT) {
+        size_t P = idx(0, j, n0);
+        double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
+        double txi = beta[P] / alph[P] * te;
+        scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
+        P = idx(n0 - 1, j, n0);
+        te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
+        txi = beta[P] / alph[P] * te;
+        scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
+    }
+    __syncthreads();
+    for (int j = 1 + (int)threadIdx.x; j <= n1 - 2; j += TS_NT) {
+        T[idx(0, j, n0)] = scratch[j];
+        T[idx(n0 - 1, j, n0)] = scratch[n1 + j];
+    }
 }
 
 // wall pressure from the normal momentum equation (oracle: wall_p_value)
@@ -394,35 +387,28 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
     return side == 2 ? w.p[idx(i + 1, j, n0)] - dpn : w.p[idx(i - 1, j, n0)] + dpn;
 }
 
-// scratch layout: [bottom n0 | top n0 | left n1 | right n1]
-__global__ void k_p_walls(KGrid g, WallIn w, double Re, double Ri, double *__restrict__ scratch) {
+// Wall pressure written in place: wall_p_value reads p only at interior nodes, so
+// no thread reads a value another one writes.  A corner is the mean of the two
+// adjacent wall values (recomputed by the thread that writes it).
+__global__ void k_p_walls(KGrid g, WallIn w, double Re, double Ri, double *__restrict__ p) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n0 = g.n0, n1 = g.n1;
     if (t >= 1 && t <= n0 - 2) {
-        scratch[t] = wall_p_value(w, g, Re, Ri, 0, t, 0);
-        scratch[n0 + t] = wall_p_value(w, g, Re, Ri, 1, t, n1 - 1);
+        p[idx(t, 0, n0)] = wall_p_value(w, g, Re, Ri, 0, t, 0);
+        p[idx(t, n1 - 1, n0)] = wall_p_value(w, g, Re, Ri, 1, t, n1 - 1);
     }
     if (t >= 1 && t <= n1 - 2) {
-        scratch[2 * n0 + t] = wall_p_value(w, g, Re, Ri, 2, 0, t);
-        scratch[2 * n0 + n1 + t] = wall_p_value(w, g, Re, Ri, 3, n0 - 1, t);
-    }
-}
-__global__ void k_p_walls_write(KGrid g, const double *__restrict__ scratch, double *__restrict__ p) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n0 = g.n0, n1 = g.n1;
-    if (t >= 1 && t <= n0 - 2) {
-        p[idx(t, 0, n0)] = scratch[t];
-        p[idx(t, n1 - 1, n0)] = scratch[n0 + t];
-    }
-    if (t >= 1 && t <= n1 - 2) {
-        p[idx(0, t, n0)] = scratch[2 * n0 + t];
-        p[idx(n0 - 1, t, n0)] = scratch[2 * n0 + n1 + t];
+        p[idx(0, t, n0)] = wall_p_value(w, g, Re, Ri, 2, 0, t);
+        p[idx(n0 - 1, t, n0)] = wall_p_value(w, g, Re, Ri, 3, n0 - 1, t);
     }
     if (t == 0) {
-        p[idx(0, 0, n0)] = 0.5 * (scratch[1] + scratch[2 * n0 + 1]);
-        p[idx(n0 - 1, 0, n0)] = 0.5 * (scratch[n0 - 2] + scratch[2 * n0 + n1 + 1]);
-        p[idx(0, n1 - 1, n0)] = 0.5 * (scratch[n0 + 1] + scratch[2 * n0 + n1 - 2]);
-        p[idx(n0 - 1, n1 - 1, n0)] = 0.5 * (scratch[2 * n0 - 2] + scratch[2 * n0 + 2 * n1 - 2]);
+        p[idx(0, 0, n0)] = 0.5 * (wall_p_value(w, g, Re, Ri, 0, 1, 0) + wall_p_value(w, g, Re, Ri, 2, 0, 1));
+        p[idx(n0 - 1, 0, n0)] =
+            0.5 * (wall_p_value(w, g, Re, Ri, 0, n0 - 2, 0) + wall_p_value(w, g, Re, Ri, 3, n0 - 1, 1));
+        p[idx(0, n1 - 1, n0)] =
+            0.5 * (wall_p_value(w, g, Re, Ri, 1, 1, n1 - 1) + wall_p_value(w, g, Re, Ri, 2, 0, n1 - 2));
+        p[idx(n0 - 1, n1 - 1, n0)] =
+            0.5 * (wall_p_value(w, g, Re, Ri, 1, n0 - 2, n1 - 1) + wall_p_value(w, g, Re, Ri, 3, n0 - 1, n1 - 2));
     }
 }
 
@@ -447,7 +433,26 @@ __global__ void k_uvt_walls(KGrid g, double lid, double th, double tc, double *_
     }
 }
 
-// signed max of the nodal dilation (solver.cpp:1821-1850) -> partials
+// wall Nusselt number of the bottom (wall 0) or top (wall 1) wall (oracle: nusselt),
+// summed by the calling block
+__device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const double *__restrict__ ajac,
+                               const double *__restrict__ etx, const double *__restrict__ ety, int wall) {
+    const int n0 = g.n0, n1 = g.n1;
+    const int j = wall ? n1 - 1 : 0, s = wall ? -1 : 1;
+    double acc = 0.0;
+    for (int i = tlin(); i < n0 - 1; i += NT) {
+        const int ip = i + 1;
+        const size_t A = idx(i, j, n0), B = idx(ip, j, n0);
+        const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
+        const double tj2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
+        const double fh1 = tj1 * (4 * T[idx(i, j + s, n0)] - 3 * T[A] - T[idx(i, j + 2 * s, n0)]) / (2 * g.h2);
+        const double fh2 = tj2 * (4 * T[idx(ip, j + s, n0)] - 3 * T[B] - T[idx(ip, j + 2 * s, n0)]) / (2 * g.h2);
+        acc += 0.5 * (fh1 + fh2) * g.h1;
+    }
+    return block_sum<NT>(acc);
+}
+
+// signed max of the nodal dilation (solver.cpp:1821-1850) -> block partials
 __global__ __launch_bounds__(NT) void k_dmax(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
                                              const double *__restrict__ xix, const double *__restrict__ xiy,
                                              const double *__restrict__ etx, const double *__restrict__ ety,
@@ -462,33 +467,27 @@ __global__ __launch_bounds__(NT) void k_dmax(KGrid g, const double *__restrict__
              ety[P] * (v[idx(i, j + 1, n0)] - v[idx(i, j - 1, n0)]) / (2 * g.h2) +
              xiy[P] * (v[P + 1] - v[P - 1]) / (2 * g.h1);
     }
-    const double s = block_max<NT>(dm);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_lin()] = s;
+    dm = block_max<NT>(dm);
+    if (tlin() == 0) part[blin()] = dm;
 }
 
-// wall Nusselt numbers, one block (oracle: nusselt)
-__global__ void k_nusselt(KGrid g, const double *__restrict__ T, const double *__restrict__ ajac,
-                          const double *__restrict__ etx, const double *__restrict__ ety, double len_hot,
-                          double len_cold, double dT, double *__restrict__ scal) {
-    const int n0 = g.n0, n1 = g.n1;
-    for (int wall = 0; wall < 2; wall++) {
-        const int j = wall ? n1 - 1 : 0, s = wall ? -1 : 1;
-        double acc = 0.0;
-        for (int i = threadIdx.x; i < n0 - 1; i += blockDim.x) {
-            const int ip = i + 1;
-            const size_t A = idx(i, j, n0), B = idx(ip, j, n0);
-            const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
-            const double tj2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
-            const double fh1 = tj1 * (4 * T[idx(i, j + s, n0)] - 3 * T[A] - T[idx(i, j + 2 * s, n0)]) / (2 * g.h2);
-            const double fh2 = tj2 * (4 * T[idx(ip, j + s, n0)] - 3 * T[B] - T[idx(ip, j + 2 * s, n0)]) / (2 * g.h2);
-            acc += 0.5 * (fh1 + fh2) * g.h1;
-        }
-        acc = block_sum<256>(acc);
-        if (threadIdx.x == 0) {
-            const double len = wall ? len_cold : len_hot;
-            const double nu = -acc / (len * (dT != 0.0 ? dT : 1.0));
-            scal[wall ? S_NUC : S_NUH] = wall ? -nu : nu;
-        }
+// one block: dmax from the partials, floored at 0 (solver.cpp:1806), and both
+// wall Nusselt numbers
+__global__ __launch_bounds__(NT) void k_dmax_nu_fin(KGrid g, const double *__restrict__ part, int nparts,
+                                                    const double *__restrict__ T, const double *__restrict__ ajac,
+                                                    const double *__restrict__ etx, const double *__restrict__ ety,
+                                                    double len_hot, double len_cold, double dT,
+                                                    double *__restrict__ scal) {
+    double m = -1.0e300;
+    for (int t = tlin(); t < nparts; t += NT) m = fmax(m, part[t]);
+    m = block_max<NT>(m);
+    const double nh = nusselt_wall(g, T, ajac, etx, ety, 0);
+    const double nc = nusselt_wall(g, T, ajac, etx, ety, 1);
+    if (tlin() == 0) {
+        const double den = dT != 0.0 ? dT : 1.0;
+        scal[S_DMAX] = fmax(m, 0.0);
+        scal[S_NUH] = -nh / (len_hot * den);
+        scal[S_NUC] = nc / (len_cold * den);
     }
 }
 
@@ -517,9 +516,8 @@ __device__ __forceinline__ double mrow(const MArgs &a, const double *__restrict_
 }
 
 // y = L x, or y = b - L x when b != nullptr; 0 on the border
-__global__ __launch_bounds__(NT) void k_mapply(const double *gd, MArgs a, const double *__restrict__ x,
+__global__ __launch_bounds__(NT) void k_mapply(MArgs a, const double *__restrict__ x,
                                                const double *__restrict__ b, double *__restrict__ y) {
-    if (gd && *gd != 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
@@ -528,32 +526,112 @@ __global__ __launch_bounds__(NT) void k_mapply(const double *gd, MArgs a, const 
     y[P] = b ? b[P] - v : v;
 }
 
-// device-side control of the captured mixed-precision refinement
-__global__ void k_r_init(double *scal, double rtol) {
-    const double bn = sqrt(scal[S_BB64]);
-    const double rn = sqrt(scal[S_RR64]);
-    scal[S_BN64] = bn;
-    scal[S_TOL64] = rtol * bn;
-    scal[S_RN64] = rn;
-    scal[S_ITTOT] = 0.0;
-    scal[S_RDONE] = (bn == 0.0 || rn <= rtol * bn) ? 1.0 : 0.0;
+// ------------------------------------------------------------------------
+// mixed path: FP64 iterative refinement around the FP32 solve, and the p' sum
+
+// ||q||, ||q - mean(q)|| (one pass: sum q^2 - sum(q) mean), p' = 0 and the
+// refinement's start: tolerance, done flag, first inner tolerance
+__global__ __launch_bounds__(NT) void k_q_norms(int n0, int n1, const double *__restrict__ q, double *__restrict__ pc,
+                                                RedArgs ra) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[2] = {0.0, 0.0};
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        pc[P] = 0.0;
+        if (inner_node(i, j, n0, n1)) {
+            v[0] = q[P];
+            v[1] = q[P] * q[P];
+        }
+    }
+    block_partials<2>(v, ra.part, ra.pstride);
 }
-__global__ void k_r_prep(double *scal) {
-    if (scal[S_RDONE] != 0.0) return;
-    scal[S_RTOL] = fmax(0.5 * scal[S_TOL64] / scal[S_RN64], 1e-5);
+
+// FP64 defect r = q - L (p' + e) (faces stored in FP64), its norm with the mean
+// removed, the refinement test and the next inner tolerance
+__global__ __launch_bounds__(NT) void k_defect(const double *rdone, FArgs<double> a, const double *__restrict__ q,
+                                               const double *__restrict__ pc, const float *__restrict__ ef,
+                                               double *__restrict__ r, RedArgs ra) {
+    if (*rdone != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[2] = {0.0, 0.0};
+    if (i < a.n0 && j < a.n1) {
+        const size_t P = idx(i, j, a.n0);
+        double y = 0.0;
+        if (inner_node(i, j, a.n0, a.n1)) {
+            y = q[P] - frow_f<double>(a, i, j, [&](int ii, int jj) {
+                    const size_t Q = idx(ii, jj, a.n0);
+                    return pc[Q] + (double)ef[Q];
+                });
+            v[0] = y;
+            v[1] = y * y;
+        }
+        r[P] = y;
+    }
+    block_partials<2>(v, ra.part, ra.pstride);
 }
-__global__ void k_r_its(double *scal) {
-    if (scal[S_RDONE] != 0.0) return;
-    scal[S_ITTOT] += fmax(scal[S_IT], 1.0);
+
+// end of the p' solve: (mixed) p' += e, e = 0; interior sum of p' -> S_PSUM
+template <bool ACC>
+__global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restrict__ pc, float *__restrict__ ef,
+                                                 RedArgs ra) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[1] = {0.0};
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        double x = pc[P];
+        if (ACC) {
+            x += (double)ef[P];
+            pc[P] = x;
+            ef[P] = 0.0f;
+        }
+        if (inner_node(i, j, n0, n1)) v[0] = x;
+    }
+    block_partials<1>(v, ra.part, ra.pstride);
 }
-__global__ void k_r_check(double *scal) {
-    if (scal[S_RDONE] != 0.0) return;
-    const double rn = sqrt(scal[S_RR64]);
-    scal[S_RN64] = rn;
-    if (rn <= scal[S_TOL64]) scal[S_RDONE] = 1.0;
-}
-__global__ void k_r_final(double *scal) {
-    scal[S_RELRES64] = scal[S_BN64] > 0.0 ? scal[S_RN64] / scal[S_BN64] : 0.0;
+
+// totals of the case-level reductions and what follows from them (one block)
+enum { CF_RES, CF_MASS, CF_Q, CF_DEFECT, CF_PSUM };
+template <int OP>
+__global__ __launch_bounds__(NT) void k_cav_fin(const double *guard, RedArgs ra, int nparts, int n0, int n1,
+                                                double rtol) {
+    if (guard && *guard != 0.0) return;
+    constexpr int NV = (OP == CF_MASS || OP == CF_PSUM) ? 1 : 2;
+    double tot[NV];
+    sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
+    if (tlin() != 0) return;
+    double *s = ra.scal;
+    const double nin = (double)(n0 - 2) * (double)(n1 - 2);
+    if (OP == CF_RES) {
+        s[S_ENERGY] = tot[0];
+        s[S_MOM] = tot[NV - 1];
+    } else if (OP == CF_MASS) {
+        s[S_MASS] = tot[0];
+    } else if (OP == CF_PSUM) {
+        s[S_PSUM] = tot[0];
+    } else if (OP == CF_Q) {
+        // ||q|| and ||q - mean(q)|| = sqrt(sum q^2 - sum(q) mean); refinement start
+        const double rr = fmax(tot[NV - 1] - tot[0] * (tot[0] / nin), 0.0);
+        const double bn = sqrt(tot[NV - 1]), rn = sqrt(rr);
+        s[S_BB64] = tot[NV - 1];
+        s[S_RR64] = rr;
+        s[S_BN64] = bn;
+        s[S_TOL64] = rtol * bn;
+        s[S_RN64] = rn;
+        s[S_RELRES64] = bn > 0.0 ? rn / bn : 0.0;
+        s[S_ITTOT] = 0.0;
+        s[S_RDONE] = (bn == 0.0 || rn <= rtol * bn) ? 1.0 : 0.0;
+        s[S_RTOL] = rn > 0.0 ? fmax(0.5 * rtol * bn / rn, 1e-5) : 1e-5;
+    } else {
+        // refinement test on the FP64 defect (mean removed) and the next inner tolerance
+        s[S_ITTOT] += fmax(s[S_IT], 1.0);
+        const double rr = fmax(tot[NV - 1] - tot[0] * (tot[0] / nin), 0.0);
+        const double rn = sqrt(rr);
+        s[S_RR64] = rr;
+        s[S_RN64] = rn;
+        s[S_RELRES64] = s[S_BN64] > 0.0 ? rn / s[S_BN64] : 0.0;
+        if (rn <= s[S_TOL64]) s[S_RDONE] = 1.0;
+        else s[S_RTOL] = fmax(0.5 * s[S_TOL64] / rn, 1e-5);
+    }
 }
 
 }  // namespace cfd
@@ -613,12 +691,17 @@ void cb_end(void *o) { prof_end(static_cast<Case *>(o)); }
 void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 
 PsCtx ctx(Case *c) {
-    return PsCtx{c->s, c->part, c->pstride, c->scal, c->hscal, cb_begin, cb_end, cb_drain, c, &c->allocs, &c->bytes};
+    return PsCtx{c->s, c->part, c->pstride, c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
+                 &c->allocs, &c->bytes};
 }
+RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, c->scal}; }
 
-int reduce_sum(Case *c, int nparts, int nv, int dst) {
-    PsCtx cx = ctx(c);
-    return ps_reduce(cx, nparts, nv, dst);
+// one-block finish of a case-level reduction over the n0 x n1 grid's block partials
+template <int OP>
+int cav_fin(Case *c, const double *guard = nullptr, double rtol = 0.0) {
+    k_cav_fin<OP><<<1, NT, 0, c->s>>>(guard, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
 }
 
 int sync_scal(Case *c) {
@@ -630,12 +713,12 @@ MArgs margs(const Case *c) {
     return MArgs{c->n0, c->n1, c->h1, c->h2, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->d};
 }
 
-// coarse geometry (metrics injected once) and the solver(s) of the case's precision
+// coarse geometry (metrics injected once) and the solver of the case's precision
 int build_levels(Case *c) {
     const int want = c->prm.mg_levels < 1 ? 1 : c->prm.mg_levels;
     Geom &G0 = c->geom[0];
     G0.n0 = c->n0; G0.n1 = c->n1; G0.h1 = c->h1; G0.h2 = c->h2;
-    G0.xix = c->xix; G0.xiy = c->xiy; G0.etx = c->etx; G0.ety = c->ety; G0.jac = c->ajac; G0.d = c->d;
+    G0.xix = c->xix; G0.xiy = c->xiy; G0.etx = c->etx; G0.ety = c->ety; G0.jac = c->ajac;
     int l = 1;
     while (l < want && l < MAXLEV) {
         const Geom &F = c->geom[l - 1];
@@ -645,7 +728,7 @@ int build_levels(Case *c) {
         L.h1 = 2.0 * F.h1; L.h2 = 2.0 * F.h2;
         const size_t M = (size_t)L.n0 * L.n1;
         RET(dalloc(c, &L.xix, M)); RET(dalloc(c, &L.xiy, M)); RET(dalloc(c, &L.etx, M));
-        RET(dalloc(c, &L.ety, M)); RET(dalloc(c, &L.jac, M)); RET(dalloc(c, &L.d, M));
+        RET(dalloc(c, &L.ety, M)); RET(dalloc(c, &L.jac, M));
         const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
         k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xix, L.n0, L.n1, L.xix);
         k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xiy, L.n0, L.n1, L.xiy);
@@ -667,107 +750,55 @@ int build_levels(Case *c) {
     return CFD_OK;
 }
 
-// per outer iteration: d onto the coarse levels, then faces and diagonals
+// per outer iteration: faces and diagonals of every level from the current d
 int mg_setup(Case *c) {
-    const dim3 bl(BX, BY);
-    for (int l = 1; l < c->nlev; l++) {
-        Geom &F = c->geom[l - 1], &L = c->geom[l];
-        k_inject<<<grid2d(L.n0, L.n1), bl, 0, c->s>>>(F.n0, F.d, L.n0, L.n1, L.d);
-        CFD_LAUNCH_CHECK();
-    }
     PsCtx cx = ctx(c);
-    if (c->mixed) return c->ps32->setup(cx, c->geom);
-    return c->ps64->setup(cx, c->geom);
+    if (c->mixed) return c->ps32->setup(cx, c->d, c->f64);
+    return c->ps64->setup(cx, c->d, nullptr);
 }
 
-// p' solve into c->pc (zero on entry).  FP64: BiCGStab + multigrid in FP64.
-// Mixed: FP64 iterative refinement around an FP32 BiCGStab + multigrid solve,
-// with the FP64 defect r = q - L p' (projected to zero mean) deciding convergence.
-int psolve(Case *c, double rtol, int maxit, int *its_out, double *relres) {
+// p' solve into c->pc (cleared first), ending with the interior sum of p' in S_PSUM.
+// FP64: BiCGStab + multigrid in FP64.  Mixed: FP64 iterative refinement around
+// FP32 BiCGStab + multigrid solves, the FP64 defect (mean removed) deciding
+// convergence; one refinement step when rtol >= 1e-4 (the FP32 solve alone can
+// reach it), up to six otherwise.  graph: capturable (no host synchronisation).
+// Iterations and relative residual: S_IT / S_RELRES (FP64), S_ITTOT / S_RELRES64.
+int psolve(Case *c, double rtol, int maxit, bool graph) {
     PsCtx cx = ctx(c);
-    if (!c->mixed) return c->ps64->solve(cx, c->q, c->pc, rtol, maxit, its_out, relres);
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = nblk(n0, n1);
-    const size_t N = c->N;
-    const int lb = 256, lg = (int)((N + lb - 1) / lb);
-    // ||q|| and the first defect (p' = 0 -> r = q, projected)
-    RET(ps_copy<double>(cx, c->q, c->r64, N));
-    RET(ps_project_mean<double>(cx, n0, n1, c->r64));
-    k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
-    CFD_LAUNCH_CHECK();
-    RET(reduce_sum(c, nb, 2, S_BB));
-    RET(sync_scal(c));
-    const double bnorm = std::sqrt(c->hscal[S_BB]);
-    double rnorm = std::sqrt(c->hscal[S_RR]);
-    int total = 0;
-    *its_out = 0;
-    if (bnorm == 0.0) { *relres = 0.0; return CFD_OK; }
-    const double tol = rtol * bnorm;
-    for (int refine = 0; refine < 6 && rnorm > tol && total < maxit; refine++) {
-        // inner FP32 solve of L e = r to a relative tolerance that leaves room for rounding
-        const double inner = std::max(0.5 * tol / rnorm, 1e-5);
-        k_cvt<double, float><<<lg, lb, 0, c->s>>>(nullptr, N, c->r64, c->rf);
-        RET(ps_fill<float>(cx, c->ef, N));
-        int its = 0;
-        double rr = 0.0;
-        RET(c->ps32->solve(cx, c->rf, c->ef, inner, maxit - total, &its, &rr));
-        total += its > 0 ? its : 1;
-        k_acc<float><<<lg, lb, 0, c->s>>>(nullptr, N, c->ef, c->pc);
-        // FP64 defect
-        k_mapply<<<gr, bl, 0, c->s>>>(nullptr, margs(c), c->pc, c->q, c->r64);
+    if (!c->mixed) {
+        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, nullptr, graph));
+        k_pc_final<false><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
-        RET(ps_project_mean<double>(cx, n0, n1, c->r64));
-        k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
-        CFD_LAUNCH_CHECK();
-        RET(reduce_sum(c, nb, 1, S_RR));
-        RET(sync_scal(c));
-        rnorm = std::sqrt(c->hscal[S_RR]);
+        return cav_fin<CF_PSUM>(c);
     }
-    *its_out = total;
-    *relres = rnorm / bnorm;
-    return CFD_OK;
-}
-
-// Capturable p' solve (no host synchronisation): FP64 runs p_maxit unrolled
-// BiCGStab iterations; mixed runs two FP64 refinement steps around FP32 solves.
-// Iterations and the relative residual are left in device scalars.
-int psolve_graph(Case *c, double rtol, int maxit) {
-    PsCtx cx = ctx(c);
-    if (!c->mixed) return c->ps64->solve_graph(cx, c->q, c->pc, rtol, -1, maxit, nullptr);
-    const int n0 = c->n0, n1 = c->n1;
-    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = nblk(n0, n1);
-    const size_t N = c->N;
-    const int lb = 256, lg = (int)((N + lb - 1) / lb);
+    const int R = rtol >= 1e-4 ? 1 : 6;
     const double *rd = c->scal + S_RDONE;
-    RET(ps_copy<double>(cx, c->q, c->r64, N));
-    RET(ps_project_mean<double>(cx, n0, n1, c->r64));
-    k_dot2<double><<<gr, bl, 0, c->s>>>(nullptr, n0, n1, c->q, c->q, c->r64, c->r64, c->part, c->pstride);
+    const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
+    k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
     CFD_LAUNCH_CHECK();
-    RET(ps_reduce(cx, nb, 2, S_BB64));
-    k_r_init<<<1, 1, 0, c->s>>>(c->scal, rtol);
-    CFD_LAUNCH_CHECK();
-    for (int refine = 0; refine < 2; refine++) {
-        k_r_prep<<<1, 1, 0, c->s>>>(c->scal);
-        k_cvt<double, float><<<lg, lb, 0, c->s>>>(rd, N, c->r64, c->rf);
+    RET(cav_fin<CF_Q>(c, nullptr, rtol));
+    for (int k = 0; k < R; k++) {
+        if (!graph && k > 0) {
+            RET(ps_sync(cx));
+            if (c->hscal[S_RDONE] != 0.0) break;
+        }
+        RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, rd,
+                           graph));
+        k_defect<<<gr, bl, 0, c->s>>>(rd, a64, c->q, c->pc, c->ef, c->r64, red(c));
         CFD_LAUNCH_CHECK();
-        RET(ps_fill<float>(cx, c->ef, N, rd));
-        RET(c->ps32->solve_graph(cx, c->rf, c->ef, 0.0, S_RTOL, maxit, rd));
-        k_r_its<<<1, 1, 0, c->s>>>(c->scal);
-        k_acc<float><<<lg, lb, 0, c->s>>>(rd, N, c->ef, c->pc);
-        k_mapply<<<gr, bl, 0, c->s>>>(rd, margs(c), c->pc, c->q, c->r64);
-        CFD_LAUNCH_CHECK();
-        RET(ps_project_mean<double>(cx, n0, n1, c->r64, rd));
-        k_dot2<double><<<gr, bl, 0, c->s>>>(rd, n0, n1, c->r64, c->r64, nullptr, nullptr, c->part, c->pstride);
-        CFD_LAUNCH_CHECK();
-        RET(ps_reduce(cx, nb, 1, S_RR64, rd));
-        k_r_check<<<1, 1, 0, c->s>>>(c->scal);
-        CFD_LAUNCH_CHECK();
+        RET(cav_fin<CF_DEFECT>(c, rd));
     }
-    k_r_final<<<1, 1, 0, c->s>>>(c->scal);
+    k_pc_final<true><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, c->ef, red(c));
     CFD_LAUNCH_CHECK();
-    return CFD_OK;
+    return cav_fin<CF_PSUM>(c);
+}
+
+void psolve_result(const Case *c, int *its, double *relres) {
+    const double *h = c->hscal;
+    *its = c->mixed ? (int)h[S_ITTOT] : (int)h[S_IT];
+    *relres = c->mixed ? h[S_RELRES64] : h[S_RELRES];
 }
 
 KGrid kg(const Case *c) { return KGrid{c->n0, c->n1, c->h1, c->h2}; }
@@ -784,13 +815,10 @@ int launch_residual_t(Case *c) {
     prof_begin(c, 2);
     k_residual<TR><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(
         kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
-        c->p1, c->q1, (TR *)c->Ru, (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, c->part, c->pstride);
+        c->p1, c->q1, (TR *)c->Ru, (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c));
     prof_end(c);
     CFD_LAUNCH_CHECK();
-    const int m = std::max(c->n0, c->n1);
-    k_d_border<<<(m + 255) / 256, 256, 0, c->s>>>(c->n0, c->n1, c->d);
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
+    return cav_fin<CF_RES>(c);
 }
 int launch_residual(Case *c) { return c->mixed ? launch_residual_t<float>(c) : launch_residual_t<double>(c); }
 
@@ -823,24 +851,29 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
         for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
     }
     if (c->prm.mom_sweeps <= 0) {
-        PsCtx cx = ctx(c);
-        for (int k = 0; k < 3; k++) RET(ps_fill<TR>(cx, a[k], c->N));
+        const unsigned g = (unsigned)((c->N + 255) / 256);
+        for (int k = 0; k < 3; k++) k_fill<TR><<<g, 256, 0, c->s>>>(c->N, a[k], TR(0));
+        CFD_LAUNCH_CHECK();
     }
     *du = a[0]; *dv = a[1]; *dT = a[2];
     return CFD_OK;
 }
 
-int apply_bcs_dev(Case *c) {
+// adiabatic sides, then the wall pressure (which reads the new wall T)
+int wall_bcs(Case *c) {
     const int m = std::max(c->n0, c->n1);
-    const int nb = (m + 255) / 256;
-    k_uvt_walls<<<nb, 256, 0, c->s>>>(kg(c), c->prm.lid_u, c->prm.t_hot, c->prm.t_cold, c->u, c->v, c->T);
-    CFD_LAUNCH_CHECK();
-    k_t_sides<<<nb, 256, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
-    k_t_sides_write<<<nb, 256, 0, c->s>>>(kg(c), c->scratch, c->T);
-    k_p_walls<<<nb, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->scratch);
-    k_p_walls_write<<<nb, 256, 0, c->s>>>(kg(c), c->scratch, c->p);
+    k_t_sides<<<1, TS_NT, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
+    k_p_walls<<<(m + 255) / 256, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->p);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
+}
+
+int apply_bcs_dev(Case *c) {
+    const int m = std::max(c->n0, c->n1);
+    k_uvt_walls<<<(m + 255) / 256, 256, 0, c->s>>>(kg(c), c->prm.lid_u, c->prm.t_hot, c->prm.t_cold, c->u, c->v,
+                                                   c->T);
+    CFD_LAUNCH_CHECK();
+    return wall_bcs(c);
 }
 
 template <typename TR>
@@ -853,16 +886,13 @@ int star_t(Case *c, void *du, void *dv, void *dT) {
 }
 
 // Everything of one outer iteration that runs on the stream.  graph: capturable
-// (device-side p' control); otherwise the p' solve synchronises per iteration and
-// returns its iteration count and residual in *its / *relres.
-int outer_body(Case *c, bool graph, int *its, double *relres) {
+// (the p' solve is controlled on the device); otherwise the p' solve reads its
+// convergence flags on the host between half-iterations.
+int outer_body(Case *c, bool graph) {
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = nblk(n0, n1);
-    PsCtx cx = ctx(c);
     // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
-    RET(reduce_sum(c, nb, 2, S_ENERGY));  // S_ENERGY, S_MOM
     // 3 predictor
     void *du, *dv, *dT;
     if (c->mixed) RET(predictor_t<float>(c, &du, &dv, &dT));
@@ -872,39 +902,22 @@ int outer_body(Case *c, bool graph, int *its, double *relres) {
     else RET(star_t<double>(c, du, dv, dT));
     prof_begin(c, 3);
     k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
-                                    c->q, c->part);
+                                    c->q, red(c));
     prof_end(c);
     CFD_LAUNCH_CHECK();
-    RET(reduce_sum(c, nb, 1, S_MASS));
+    RET(cav_fin<CF_MASS>(c));
     // 5 p' solve
     RET(mg_setup(c));
-    RET(ps_fill<double>(cx, c->pc, c->N));
-    if (graph) {
-        RET(psolve_graph(c, c->prm.p_rtol, c->prm.p_maxit));
-    } else {
-        RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, its, relres));
-    }
-    RET(ps_project_mean<double>(cx, n0, n1, c->pc));
-    // 6-7 correction and BCs (FP64 accumulation)
+    RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph));
+    // 6-7 correction (p' mean removed) and BCs (FP64 accumulation)
     k_correct<<<gr, bl, 0, c->s>>>(kg(c), c->prm.alpha_p, c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx,
-                                   c->ety, c->u, c->v, c->p);
+                                   c->ety, c->scal, c->u, c->v, c->p);
     CFD_LAUNCH_CHECK();
-    {
-        const int m = std::max(n0, n1);
-        const int nbl = (m + 255) / 256;
-        k_t_sides<<<nbl, 256, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
-        k_t_sides_write<<<nbl, 256, 0, c->s>>>(kg(c), c->scratch, c->T);
-        k_p_walls<<<nbl, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->scratch);
-        k_p_walls_write<<<nbl, 256, 0, c->s>>>(kg(c), c->scratch, c->p);
-        CFD_LAUNCH_CHECK();
-    }
-    // 8 reductions (FP64)
+    RET(wall_bcs(c));
+    // 8 dmax and the wall Nusselt numbers (FP64)
     k_dmax<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->part);
-    CFD_LAUNCH_CHECK();
-    k_reduce_max<<<1, 256, 0, c->s>>>(c->part, nb, c->scal, S_DMAX);
-    CFD_LAUNCH_CHECK();
-    k_nusselt<<<1, 256, 0, c->s>>>(kg(c), c->T, c->ajac, c->etx, c->ety, c->wall_len_hot, c->wall_len_cold,
-                                   c->prm.t_hot - c->prm.t_cold, c->scal);
+    k_dmax_nu_fin<<<1, NT, 0, c->s>>>(kg(c), c->part, nblk(n0, n1), c->T, c->ajac, c->etx, c->ety, c->wall_len_hot,
+                                      c->wall_len_cold, c->prm.t_hot - c->prm.t_cold, c->scal);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -917,7 +930,7 @@ void drop_graph(Case *c) {
     c->graph = nullptr;
 }
 
-// capture the outer iteration once (with the event pairs of live timing when enabled)
+// capture the outer iteration once
 int capture(Case *c) {
     drop_graph(c);
     CFD_HIP(hipStreamSynchronize(c->s));
@@ -925,7 +938,7 @@ int capture(Case *c) {
     c->ev_kind.clear();
     CFD_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    const int rc = outer_body(c, true, nullptr, nullptr);
+    const int rc = outer_body(c, true);
     c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->s, &g);
@@ -942,20 +955,17 @@ int capture(Case *c) {
 }
 
 int outer_once(Case *c, cfd_stats *st) {
-    int its = 0;
-    double relres = 0.0;
     // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
     if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
         CFD_HIP(hipGraphLaunch(c->gexec, c->s));
-        CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, c->s));
-        CFD_HIP(hipStreamSynchronize(c->s));
-        its = c->mixed ? (int)c->hscal[S_ITTOT] : (int)c->hscal[S_IT];
-        relres = c->mixed ? c->hscal[S_RELRES64] : c->hscal[S_RELRES];
     } else {
-        RET(outer_body(c, false, &its, &relres));
-        RET(sync_scal(c));
+        RET(outer_body(c, false));
     }
+    RET(sync_scal(c));
+    int its = 0;
+    double relres = 0.0;
+    psolve_result(c, &its, &relres);
     const double *h = c->hscal;
     c->iter++;
     if (c->iter == 1) {
@@ -979,12 +989,28 @@ int outer_once(Case *c, cfd_stats *st) {
     return CFD_OK;
 }
 
+// one launch of a fine-level solver kernel on the solver's own buffers (timing only)
 template <typename T>
-void time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
+int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
     OpLevel<T> &L = ps->lev[0];
     const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-    if (kernel == 0) k_fapply<T><<<gr, bl, 0, c->s>>>(nullptr, fargs(L), L.x, L.t);
-    else k_fsmooth<T, false><<<gr, bl, 0, c->s>>>(nullptr, fargs(L), (T)0.8, L.x, L.b, L.invd, L.t);
+    if (kernel == 0) {
+        k_fapply<T><<<gr, bl, 0, c->s>>>(fargs(L), L.x, L.t);
+    } else if (ps->lc == 0) {
+        return CFD_E_ARG;  // no fine multigrid level on this grid
+    } else if (kernel == 4) {
+        OpLevel<T> &C = ps->lev[1];
+        DownArgs<T> a{fargs(L), L.invd, L.b, nullptr, 0, nullptr, L.x, C.n0, C.n1, C.t, ps->lpart, (T)ps->omega};
+        const dim3 g2((C.n0 + MG_CX - 1) / MG_CX, (C.n1 + MG_CY - 1) / MG_CY);
+        k_mg_down<T, 2><<<g2, PS_NT, 0, c->s>>>(nullptr, a);
+    } else {
+        OpLevel<T> &C = ps->lev[1];
+        UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega};
+        const dim3 g2((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY);
+        k_mg_up<T, 2><<<g2, PS_NT, 0, c->s>>>(nullptr, a);
+    }
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
 }
 
 }  // namespace
@@ -1035,9 +1061,9 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
     }
     if (c->mixed) {
         for (int k = 0; k < 5 && rc == CFD_OK; k++) rc = dalloc(c, &c->a32[k], N);
-        if (rc == CFD_OK) rc = dalloc(c, &c->rf, N);
         if (rc == CFD_OK) rc = dalloc(c, &c->ef, N);
         if (rc == CFD_OK) rc = dalloc(c, &c->r64, N);
+        for (int k = 0; k < 4 && rc == CFD_OK; k++) rc = dalloc(c, &c->f64[k], N);
     }
     if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)2 * (n0 + n1) + 8);
     c->pstride = (nblk(n0, n1) + 255) / 256 * 256;
@@ -1055,7 +1081,7 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
     if (rc == CFD_OK && c->mixed) {
         const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
         const int lb = 256, lg = (int)((N + lb - 1) / lb);
-        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(nullptr, N, m[k], c->a32[k]);
+        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, m[k], c->a32[k]);
         if (hipGetLastError() != hipSuccess) rc = CFD_E_HIP;
     }
     if (rc == CFD_OK) rc = build_levels(c);
@@ -1163,6 +1189,11 @@ int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d
     Case *c = reinterpret_cast<Case *>(h);
     if (!c) return CFD_E_ARG;
     RET(launch_residual(c));
+    {
+        const int m = std::max(c->n0, c->n1);
+        k_d_border<<<(m + 255) / 256, 256, 0, c->s>>>(c->n0, c->n1, c->d);
+        CFD_LAUNCH_CHECK();
+    }
     const size_t N = c->N, B = N * sizeof(double);
     void *src[3] = {c->Ru, c->Rv, c->RT};
     double *dst[3] = {Ru, Rv, RT};
@@ -1170,7 +1201,7 @@ int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d
         if (!dst[k]) continue;
         if (c->mixed) {
             const int lb = 256, lg = (int)((N + lb - 1) / lb);
-            k_cvt<float, double><<<lg, lb, 0, c->s>>>(nullptr, N, (const float *)src[k], c->uh);
+            k_cvt<float, double><<<lg, lb, 0, c->s>>>(N, (const float *)src[k], c->uh);
             CFD_LAUNCH_CHECK();
             CFD_HIP(hipMemcpyAsync(dst[k], c->uh, B, hipMemcpyDeviceToHost, c->s));
             CFD_HIP(hipStreamSynchronize(c->s));
@@ -1189,7 +1220,7 @@ int cfd_case_p_apply(cfd_case h, const double *d, const double *x, double *y) {
     const size_t B = c->N * sizeof(double);
     CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
     CFD_HIP(hipMemcpyAsync(c->uh, x, B, hipMemcpyHostToDevice, c->s));
-    k_mapply<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(nullptr, margs(c), c->uh, nullptr, c->vh);
+    k_mapply<<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(margs(c), c->uh, nullptr, c->vh);
     CFD_LAUNCH_CHECK();
     CFD_HIP(hipMemcpyAsync(y, c->vh, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
@@ -1203,11 +1234,12 @@ int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, do
     const size_t B = c->N * sizeof(double);
     CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
     CFD_HIP(hipMemcpyAsync(c->q, q, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemsetAsync(c->pc, 0, B, c->s));
     RET(mg_setup(c));
+    RET(psolve(c, rtol, maxit, false));
+    RET(sync_scal(c));
     int its = 0;
     double rr = 0.0;
-    RET(psolve(c, rtol, maxit, &its, &rr));
+    psolve_result(c, &its, &rr);
     CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     if (iters) *iters = its;
@@ -1247,7 +1279,7 @@ int cfd_case_profile_read(cfd_case h, double *ms_total, long *launches, int nkin
 
 int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch) {
     Case *c = reinterpret_cast<Case *>(h);
-    if (!c || reps < 1 || !ms_per_launch || kernel < 0 || kernel > 4) return CFD_E_ARG;
+    if (!c || reps < 1 || !ms_per_launch || kernel < 0 || kernel > 5) return CFD_E_ARG;
     const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
     hipEvent_t e0, e1;
     CFD_HIP(hipEventCreate(&e0));
@@ -1257,8 +1289,9 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
         switch (kernel) {
         case 0:
         case 4:
-            if (c->mixed) time_solver_kernel<float>(c, c->ps32, kernel);
-            else time_solver_kernel<double>(c, c->ps64, kernel);
+        case 5:
+            RET(c->mixed ? time_solver_kernel<float>(c, c->ps32, kernel)
+                         : time_solver_kernel<double>(c, c->ps64, kernel));
             break;
         case 1:
             if (c->mixed) {
@@ -1274,7 +1307,7 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
             break;
         case 3:
             k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
-                                            c->ajac, c->q, c->part);
+                                            c->ajac, c->q, red(c));
             break;
         }
     }

@@ -5,11 +5,26 @@
 // live in device memory as double.
 //
 // The operator is the J-weighted flux form of the face velocity of
-// solver.cpp:1402-1436 (see p9_coefs).  Per level and per outer iteration the
-// four face coefficients (east/north, normal/tangential) are computed once in
-// FP64 from the level's metrics and d and stored in T; a row is assembled from
+// solver.cpp:1402-1436 (see p9_coefs).  Per outer iteration the four face
+// coefficients (east/north, normal/tangential) of every level are computed once
+// in FP64 from the level's metrics and d and stored in T; a row is assembled from
 // the faces of P, W and S, so an operator application reads 4 coefficient
 // arrays and the vector instead of 5 metric arrays, d and the vector.
+//
+// Launch structure (one BiCGStab iteration: 9 + 2 x (2 x fine levels + 1) launches):
+//  * a fine multigrid level costs two launches: k_mg_down (pre-smoothing from
+//    zero, residual, full-weighting restriction and the coarse interior sum, on
+//    an LDS tile) and k_mg_up (bilinear prolongation and post-smoothing on an
+//    LDS tile); levels of at most 33 x 33 nodes run in one single-workgroup
+//    kernel resident in LDS (k_mg_coarse);
+//  * every BiCGStab reduction is finished by a one-block kernel (k_bi_fin) that
+//    also evaluates the solver's scalar recurrences and convergence tests; the
+//    coarse-level means of the V-cycle are summed from the block partials by the
+//    kernel that consumes them;
+//  * the convergence state lives in device scalars: the same kernel sequence
+//    runs eagerly (with host reads of the flags between half-iterations) or
+//    captured in a graph (all iterations unrolled; kernels after convergence
+//    return at their first instruction).
 #pragma once
 
 #include <algorithm>
@@ -23,14 +38,16 @@ namespace cfd {
 constexpr int PS_MAXLEV = 16;
 constexpr int PS_NT = BX * BY;
 
-// device scalar slots used by the solver (pairs reduced together are adjacent)
+// device scalar slots
 enum {
     S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_RV, S_TS, S_TT, S_SS, S_BB, S_RR, S_SUM, S_R0R,
     S_MASS, S_ENERGY, S_MOM, S_DMAX, S_NUH, S_NUC,
-    // device-side control of the captured (graph) solve
+    // BiCGStab control: done flag, iteration that stopped at the ||s|| test,
+    // tolerance, iteration count, norms, relative residual, tolerance source
     S_DONE, S_SCONV, S_TOL, S_IT, S_RN, S_BN, S_RELRES, S_RTOL,
     // mixed-precision refinement (FP64 defect)
     S_BB64, S_RR64, S_TOL64, S_RN64, S_BN64, S_RDONE, S_ITTOT, S_RELRES64,
+    S_PSUM,   // interior sum of p' (its mean is removed by the correction)
     S_NSCAL
 };
 
@@ -38,7 +55,7 @@ enum {
 struct Geom {
     int n0 = 0, n1 = 0;
     double h1 = 0, h2 = 0;
-    double *xix = nullptr, *xiy = nullptr, *etx = nullptr, *ety = nullptr, *jac = nullptr, *d = nullptr;
+    double *xix = nullptr, *xiy = nullptr, *etx = nullptr, *ety = nullptr, *jac = nullptr;
 };
 
 template <typename T>
@@ -59,36 +76,54 @@ FArgs<T> fargs(const OpLevel<T> &L) {
 }
 
 __device__ __forceinline__ int blin() { return blockIdx.x + blockIdx.y * gridDim.x; }
-
-// face coefficients of a level, computed in FP64 and stored as T
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_p_faces(int n0, int n1, double h1, double h2, const double *__restrict__ xix,
-                                                   const double *__restrict__ xiy, const double *__restrict__ etx,
-                                                   const double *__restrict__ ety, const double *__restrict__ jac,
-                                                   const double *__restrict__ d, T *__restrict__ fen,
-                                                   T *__restrict__ fet, T *__restrict__ fnn, T *__restrict__ fnt) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    double kn, kt;
-    if (i < n0 - 1) {
-        p9_face<double>(n0, n1, xix, xiy, etx, ety, jac, d, h1, h2, i, j, 0, kn, kt);
-    } else {
-        kn = kt = 0.0;
-    }
-    fen[P] = (T)kn;
-    fet[P] = (T)kt;
-    if (j < n1 - 1) {
-        p9_face<double>(n0, n1, xix, xiy, etx, ety, jac, d, h1, h2, i, j, 1, kn, kt);
-    } else {
-        kn = kt = 0.0;
-    }
-    fnn[P] = (T)kn;
-    fnt[P] = (T)kt;
+__device__ __forceinline__ int tlin() { return threadIdx.x + threadIdx.y * blockDim.x; }
+__device__ __forceinline__ bool inner_node(int i, int j, int n0, int n1) {
+    return i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2;
 }
 
-template <typename T>
-__device__ __forceinline__ T frow(const FArgs<T> &a, const T *__restrict__ x, int i, int j) {
+// ---------------------------------------------------------------------------
+// Reductions in two launches: a kernel stores one partial per block and value
+// (part[k * pstride + block]), a one-block kernel adds them in block order (a
+// deterministic result) and evaluates what depends on the totals.  (A last-block
+// finish inside the first kernel needs a device-scope fence per block, which
+// costs more on gfx950 than the extra launch.)
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *part, int pstride) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = block_sum<PS_NT>(v[k]);
+    if (tlin() == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) part[(size_t)k * pstride + blin()] = v[k];
+    }
+}
+
+// totals of NV partial arrays, summed by the whole (PS_NT-thread) block in a fixed
+// order (four independent chains per thread); valid in thread 0
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *part, int pstride, int nparts, double (&tot)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double *p = part + (size_t)k * pstride;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int t = tlin();
+        for (; t + 3 * PS_NT < nparts; t += 4 * PS_NT) {
+            s0 += p[t];
+            s1 += p[t + PS_NT];
+            s2 += p[t + 2 * PS_NT];
+            s3 += p[t + 3 * PS_NT];
+        }
+        for (; t < nparts; t += PS_NT) s0 += p[t];
+        tot[k] = block_sum<PS_NT>((s0 + s1) + (s2 + s3));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// operator rows
+
+// Row (i, j) of the stored operator applied to x, with x read through `xat`
+// at the neighbours clamped into the interior (Neumann ghost).
+template <typename T, typename F>
+__device__ __forceinline__ T frow_f(const FArgs<T> &a, int i, int j, F xat) {
     T c[9];
     p9_row_faces<T>(a.n0, a.fen, a.fet, a.fnn, a.fnt, i, j, c);
     T s = T(0);
@@ -97,263 +132,387 @@ __device__ __forceinline__ T frow(const FArgs<T> &a, const T *__restrict__ x, in
 #pragma unroll
         for (int di = -1; di <= 1; di++) {
             const int ii = clampi(i + di, 1, a.n0 - 2), jj = clampi(j + dj, 1, a.n1 - 2);
-            s += c[(dj + 1) * 3 + (di + 1)] * x[idx(ii, jj, a.n0)];
+            s += c[(dj + 1) * 3 + (di + 1)] * xat(ii, jj);
         }
     return s;
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_fapply(const double *gd, FArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
-    if (gd && *gd != 0.0) return;
+__device__ __forceinline__ T frow(const FArgs<T> &a, const T *__restrict__ x, int i, int j) {
+    return frow_f<T>(a, i, j, [&](int ii, int jj) { return x[idx(ii, jj, a.n0)]; });
+}
+
+// y = L x (interior), 0 on the border
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_fapply(FArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
-    const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { y[P] = T(0); return; }
-    y[P] = frow<T>(a, x, i, j);
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_fresid(const double *gd, FArgs<T> a, const T *__restrict__ x, const T *__restrict__ b,
-                                                  T *__restrict__ r) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.n0 || j >= a.n1) return;
-    const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { r[P] = T(0); return; }
-    r[P] = b[P] - frow<T>(a, x, i, j);
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_finvdiag(FArgs<T> a, T *__restrict__ invd) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > a.n0 - 2 || j < 1 || j > a.n1 - 2) return;
-    T c[9];
-    p9_row_faces<T>(a.n0, a.fen, a.fet, a.fnn, a.fnt, i, j, c);
-    T s = T(0);
-    for (int dj = -1; dj <= 1; dj++)
-        for (int di = -1; di <= 1; di++)
-            if (clampi(i + di, 1, a.n0 - 2) == i && clampi(j + dj, 1, a.n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
-    invd[idx(i, j, a.n0)] = T(1) / s;
-}
-
-// Jacobi sweep: xo = xi + w (b - L xi) invd (FIRST: xo = w b invd)
-template <typename T, bool FIRST>
-__global__ __launch_bounds__(PS_NT) void k_fsmooth(const double *gd, FArgs<T> a, T w, const T *__restrict__ xi, const T *__restrict__ b,
-                                                   const T *__restrict__ invd, T *__restrict__ xo) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.n0 || j >= a.n1) return;
-    const size_t P = idx(i, j, a.n0);
-    if (i == 0 || j == 0 || i == a.n0 - 1 || j == a.n1 - 1) { xo[P] = T(0); return; }
-    if (FIRST) xo[P] = w * b[P] * invd[P];
-    else xo[P] = xi[P] + w * (b[P] - frow<T>(a, xi, i, j)) * invd[P];
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_restrict(const double *gd, int n0, const T *__restrict__ r, int m0, int m1, T *__restrict__ bc) {
-    if (gd && *gd != 0.0) return;
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= m0 || J >= m1) return;
-    const size_t Pc = idx(I, J, m0);
-    if (I == 0 || J == 0 || I == m0 - 1 || J == m1 - 1) { bc[Pc] = T(0); return; }
-    const int i = 2 * I, j = 2 * J;
-    bc[Pc] = (T(4) * r[idx(i, j, n0)] +
-              T(2) * (r[idx(i - 1, j, n0)] + r[idx(i + 1, j, n0)] + r[idx(i, j - 1, n0)] + r[idx(i, j + 1, n0)]) +
-              (r[idx(i - 1, j - 1, n0)] + r[idx(i + 1, j - 1, n0)] + r[idx(i - 1, j + 1, n0)] +
-               r[idx(i + 1, j + 1, n0)])) /
-             T(16);
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_prolong_add(const double *gd, int n0, int n1, T *__restrict__ x, int m0, int m1,
-                                                       const T *__restrict__ xc) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
-    const int I = i >> 1, J = j >> 1;
-#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
-    T e;
-    if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
-    else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
-    else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
-    else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
-#undef XC
-    x[idx(i, j, n0)] += e;
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_sum_int(const double *gd, int n0, int n1, const T *__restrict__ v, double *__restrict__ part) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    double s = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) s = (double)v[idx(i, j, n0)];
-    s = block_sum<PS_NT>(s);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blin()] = s;
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_sub_mean(const double *gd, int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
-                                                    int slot) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
-    const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
-    v[idx(i, j, n0)] = (T)((double)v[idx(i, j, n0)] - m);
-}
-
-__global__ void k_inject(int n0, const double *__restrict__ f, int m0, int m1, double *__restrict__ c) {
-    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
-    if (I >= m0 || J >= m1) return;
-    c[idx(I, J, m0)] = f[idx(2 * I, 2 * J, n0)];
-}
-
-template <typename A, typename B>
-__global__ void k_cvt(const double *gd, size_t n, const A *__restrict__ a, B *__restrict__ b) {
-    if (gd && *gd != 0.0) return;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) b[k] = (B)a[k];
-}
-
-// acc += (double) e on every node
-template <typename T>
-__global__ void k_acc(const double *gd, size_t n, const T *__restrict__ e, double *__restrict__ acc) {
-    if (gd && *gd != 0.0) return;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) acc[k] += (double)e[k];
-}
-
-// --- BiCGStab vector kernels
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_dot2(const double *gd, int n0, int n1, const T *__restrict__ a, const T *__restrict__ b,
-                                                const T *__restrict__ c, const T *__restrict__ d2,
-                                                double *__restrict__ part, int pstride) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    double s1 = 0.0, s2 = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-        const size_t P = idx(i, j, n0);
-        s1 = (double)a[P] * (double)b[P];
-        if (c) s2 = (double)c[P] * (double)d2[P];
-    }
-    s1 = block_sum<PS_NT>(s1);
-    s2 = block_sum<PS_NT>(s2);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[blin()] = s1;
-        part[pstride + blin()] = s2;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_p(const double *gd, int n0, int n1, bool first, const double *__restrict__ scal,
-                                                  const T *__restrict__ r, const T *__restrict__ vv, T *__restrict__ pv) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    if (first) { pv[P] = r[P]; return; }
-    const double beta = (scal[S_RHO] / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
-    pv[P] = (T)((double)r[P] + beta * ((double)pv[P] - scal[S_OMEGA] * (double)vv[P]));
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_s(const double *gd, int n0, int n1, double *__restrict__ scal, const T *__restrict__ r,
-                                                  const T *__restrict__ vv, T *__restrict__ s, double *__restrict__ part) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const double alpha = scal[S_RHO] / scal[S_RV];
-    double ss = 0.0;
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
-        const T sv = (T)((double)r[P] - alpha * (double)vv[P]);
-        s[P] = sv;
-        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) ss = (double)sv * (double)sv;
-    }
-    ss = block_sum<PS_NT>(ss);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blin()] = ss;
-    if (blin() == 0 && threadIdx.x == 0 && threadIdx.y == 0) scal[S_ALPHA] = alpha;
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_x1(int n0, int n1, const double *__restrict__ scal,
-                                                   const T *__restrict__ ph, T *__restrict__ x) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    x[P] = (T)((double)x[P] + scal[S_ALPHA] * (double)ph[P]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_xr(const double *gd, int n0, int n1, const double *__restrict__ scal,
-                                                   const T *__restrict__ ph, const T *__restrict__ sh,
-                                                   const T *__restrict__ s, const T *__restrict__ t,
-                                                   const T *__restrict__ r0, T *__restrict__ x, T *__restrict__ r,
-                                                   double *__restrict__ part, int pstride) {
-    if (gd && *gd != 0.0) return;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const double tt = scal[S_TT];
-    const double omega = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
-    const double alpha = scal[S_ALPHA];
-    double rr = 0.0, r0r = 0.0;
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
-        x[P] = (T)((double)x[P] + alpha * (double)ph[P] + omega * (double)sh[P]);
-        const T rv = (T)((double)s[P] - omega * (double)t[P]);
-        r[P] = rv;
-        if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-            rr = (double)rv * (double)rv;
-            r0r = (double)r0[P] * (double)rv;
-        }
-    }
-    rr = block_sum<PS_NT>(rr);
-    r0r = block_sum<PS_NT>(r0r);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[blin()] = rr;
-        part[pstride + blin()] = r0r;
-    }
-}
-
-__global__ void k_set_omega(const double *gd, double *__restrict__ scal) {
-    if (gd && *gd != 0.0) return;
-    const double tt = scal[S_TT];
-    scal[S_OMEGA] = (tt > 0.0) ? scal[S_TS] / tt : 0.0;
-}
-__global__ void k_rotate_rho(double *__restrict__ scal) {
-    scal[S_RHO_OLD] = scal[S_RHO];
-    scal[S_RHO] = scal[S_R0R];
-}
-__global__ void k_init_scal(double *__restrict__ scal) {
-    scal[S_RHO] = scal[S_RR];
-    scal[S_RHO_OLD] = 1.0;
-    scal[S_ALPHA] = 1.0;
-    scal[S_OMEGA] = 1.0;
-}
-
-// sum of `nparts` partials for each of `nv` arrays (stride pstride) -> scal[dst + k]
-__global__ void k_reduce_sum(const double *gd, const double *__restrict__ part, int pstride, int nparts, int nv, double *__restrict__ scal,
-                             int dst) {
-    if (gd && *gd != 0.0) return;
-    const int k = blockIdx.x;
-    if (k >= nv) return;
-    double s = 0.0;
-    for (int t = threadIdx.x; t < nparts; t += blockDim.x) s += part[(size_t)k * pstride + t];
-    s = block_sum<256>(s);
-    if (threadIdx.x == 0) scal[dst + k] = s;
+    y[idx(i, j, a.n0)] = inner_node(i, j, a.n0, a.n1) ? frow<T>(a, x, i, j) : T(0);
 }
 
 // ---------------------------------------------------------------------------
-// Coarse-level V-cycle in one workgroup.  Levels at or below a few thousand
-// nodes are latency-bound: one launch per sweep would cost more than the sweep.
-// This kernel runs the whole V-cycle from level l0 down to the coarsest and
-// back in a single 1024-thread workgroup, separating the phases with block
-// barriers (which also order the global-memory traffic of the workgroup).
-// The arithmetic per node is the same device code as the per-level kernels.
+// per-outer-iteration set-up of every level in one launch each
 
-constexpr int CW = 1024;  // threads of the coarse workgroup
+// Face (normal kn, tangential kt) between node (i, j) of level `lv` and its east
+// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face; d is level 0's
+// d read at (i << lv, j << lv) (injection).  Zero on faces touching a wall node.
+__device__ __forceinline__ void p9_face_lv(int n0, int n1, const double *__restrict__ xix,
+                                           const double *__restrict__ xiy, const double *__restrict__ etx,
+                                           const double *__restrict__ ety, const double *__restrict__ jac,
+                                           const double *__restrict__ d0, int dn0, int lv, double h1, double h2,
+                                           int i, int j, int dir, double &kn, double &kt) {
+    kn = 0.0;
+    kt = 0.0;
+    const double half = 0.5, four = 4.0;
+    const size_t P = idx(i, j, n0);
+    if (dir == 0) {
+        if (i < 1 || i > n0 - 3 || j < 1 || j > n1 - 2) return;
+    } else {
+        if (j < 1 || j > n1 - 3 || i < 1 || i > n0 - 2) return;
+    }
+    const double xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P];
+    const double dP = d0[idx(i << lv, j << lv, dn0)];
+    if (dir == 0) {
+        const size_t E = P + 1;
+        const double xE = xix[E], yE = xiy[E], eE = etx[E], fE = ety[E], jE = jac[E];
+        const double mx = half * (jP * xP + jE * xE), my = half * (jP * yP + jE * yE);
+        const double a = mx * (xP + xE) + my * (yP + yE);
+        const double b = mx * (eP + eE) + my * (fP + fE);
+        const double de = half * (dP + d0[idx((i + 1) << lv, j << lv, dn0)]);
+        kn = half * de * a / (h1 * h1);
+        kt = half * de * b / (four * h1 * h2);
+    } else {
+        const size_t Nn = P + n0;
+        const double xN = xix[Nn], yN = xiy[Nn], eN = etx[Nn], fN = ety[Nn], jN = jac[Nn];
+        const double mx = half * (jP * eP + jN * eN), my = half * (jP * fP + jN * fN);
+        const double a = mx * (eP + eN) + my * (fP + fN);
+        const double b = mx * (xP + xN) + my * (yP + yN);
+        const double dn = half * (dP + d0[idx(i << lv, (j + 1) << lv, dn0)]);
+        kn = half * dn * a / (h2 * h2);
+        kt = half * dn * b / (four * h1 * h2);
+    }
+}
+
+// all levels of one solver, 1-D block numbering (block b belongs to the last
+// level whose first block is <= b)
+template <typename T>
+struct LevTab {
+    int nlev;
+    int bofs[PS_MAXLEV + 1];
+    int bx[PS_MAXLEV];
+    int n0[PS_MAXLEV], n1[PS_MAXLEV];
+    double h1[PS_MAXLEV], h2[PS_MAXLEV];
+    const double *xix[PS_MAXLEV], *xiy[PS_MAXLEV], *etx[PS_MAXLEV], *ety[PS_MAXLEV], *jac[PS_MAXLEV];
+    T *fen[PS_MAXLEV], *fet[PS_MAXLEV], *fnn[PS_MAXLEV], *fnt[PS_MAXLEV], *invd[PS_MAXLEV];
+    const double *d0;
+    int dn0;
+    double *f64[4];  // optional FP64 copy of level 0's faces (en, et, nn, nt)
+};
+
+template <typename T>
+__device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i, int &j) {
+    const int b = blockIdx.x;
+    l = 0;
+    while (l + 1 < A.nlev && A.bofs[l + 1] <= b) l++;
+    const int lb = b - A.bofs[l];
+    i = (lb % A.bx[l]) * BX + (threadIdx.x % BX);
+    j = (lb / A.bx[l]) * BY + (threadIdx.x / BX);
+    return i < A.n0[l] && j < A.n1[l];
+}
+
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_faces_all(LevTab<T> A) {
+    int l, i, j;
+    if (!lev_of_block(A, l, i, j)) return;
+    const int n0 = A.n0[l], n1 = A.n1[l];
+    const size_t P = idx(i, j, n0);
+    double en, et, nn, nt;
+    p9_face_lv(n0, n1, A.xix[l], A.xiy[l], A.etx[l], A.ety[l], A.jac[l], A.d0, A.dn0, l, A.h1[l], A.h2[l], i, j, 0,
+               en, et);
+    p9_face_lv(n0, n1, A.xix[l], A.xiy[l], A.etx[l], A.ety[l], A.jac[l], A.d0, A.dn0, l, A.h1[l], A.h2[l], i, j, 1,
+               nn, nt);
+    A.fen[l][P] = (T)en;
+    A.fet[l][P] = (T)et;
+    A.fnn[l][P] = (T)nn;
+    A.fnt[l][P] = (T)nt;
+    if (l == 0 && A.f64[0]) {
+        A.f64[0][P] = en;
+        A.f64[1][P] = et;
+        A.f64[2][P] = nn;
+        A.f64[3][P] = nt;
+    }
+}
+
+// inverse of the clamped operator's diagonal (every coefficient whose neighbour
+// clamps onto P), all levels
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_invd_all(LevTab<T> A) {
+    int l, i, j;
+    if (!lev_of_block(A, l, i, j)) return;
+    const int n0 = A.n0[l], n1 = A.n1[l];
+    if (!inner_node(i, j, n0, n1)) return;
+    T c[9];
+    p9_row_faces<T>(n0, A.fen[l], A.fet[l], A.fnn[l], A.fnt[l], i, j, c);
+    T s = T(0);
+    for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++)
+            if (clampi(i + di, 1, n0 - 2) == i && clampi(j + dj, 1, n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
+    A.invd[l][idx(i, j, n0)] = T(1) / s;
+}
+
+// ---------------------------------------------------------------------------
+// Fine multigrid levels.  Both kernels work on a 64 x 16 tile of fine nodes per
+// 256-thread block and stage their intermediate sweeps in LDS with the halo the
+// next stencil needs, so a level's pre-smoothing, residual and restriction read
+// the level's arrays once, and so do its prolongation and post-smoothing.
+// Jacobi sweep: x <- x + w (b - L x) / diag; the first sweep from zero is
+// x = w b / diag (no neighbours).
+
+constexpr int MG_FX = 64, MG_FY = 16;  // fine tile
+constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
+
+template <typename T>
+struct DownArgs {
+    FArgs<T> a;
+    const T *invd;
+    const T *bsrc;       // right-hand side (restricted, or the preconditioner input on level 0)
+    const double *bpart; // block partials of bsrc's interior sum, whose mean is removed first (null: none)
+    int nbp;             // number of those partials
+    T *bout;             // bsrc with its mean removed, kept for the post-smoothing (null: not kept)
+    T *x;                // smoothed x
+    int m0, m1;          // coarse level
+    T *craw;             // restricted residual on the coarse level (mean not removed)
+    double *cpart;       // block partials of craw's interior sum (one per block)
+    T w;
+};
+
+template <int NPRE>
+struct DownTile {
+    static constexpr int HB = NPRE + 1;  // halo of the staged right-hand side
+    static constexpr int BWX = MG_FX + 2 * HB, BWY = MG_FY + 2 * HB;
+    static constexpr int SX = MG_FX + 4, SY = MG_FY + 4;  // smoothed x (halo 2)
+    static constexpr int RX = MG_FX + 2, RY = MG_FY + 2;  // residual (halo 1)
+    static constexpr int NB = BWX * BWY;
+    static constexpr int NX1 = NPRE == 2 ? NB : 0;        // first sweep (reused for the residual)
+    static constexpr int NXS = NPRE >= 1 ? SX * SY : 0;
+    static constexpr int NR = NPRE == 2 ? 0 : (NPRE == 1 ? RX * RY : 0);
+    static constexpr int TOTAL = NB + NX1 + NXS + NR;
+};
+
+template <typename T, int NPRE>
+__global__ __launch_bounds__(PS_NT) void k_mg_down(const double *gd, DownArgs<T> A) {
+    if (gd && *gd != 0.0) return;
+    using TL = DownTile<NPRE>;
+    __shared__ T lds[TL::TOTAL];
+    T *s_b = lds;
+    T *s_x1 = lds + TL::NB;
+    T *s_xs = s_x1 + TL::NX1;
+    T *s_r = NPRE == 2 ? s_x1 : (NPRE == 1 ? s_xs + TL::NXS : s_b);
+    const int tid = tlin();
+    const int n0 = A.a.n0, n1 = A.a.n1;
+    const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
+    const T w = A.w;
+    double mean = 0.0;
+    if (A.bpart) {
+        double tot[1];
+        sum_partials<1>(A.bpart, 0, A.nbp, tot);
+        __shared__ double s_mean;
+        if (tid == 0) s_mean = tot[0] / ((double)(n0 - 2) * (double)(n1 - 2));
+        __syncthreads();
+        mean = s_mean;
+    }
+    // right-hand side with its mean removed, halo HB
+    for (int k = tid; k < TL::NB; k += PS_NT) {
+        const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            const T raw = A.bsrc[idx(gi, gj, n0)];
+            v = A.bpart ? (T)((double)raw - mean) : raw;
+        }
+        s_b[k] = v;
+    }
+    __syncthreads();
+    if (NPRE == 2) {
+        for (int k = tid; k < TL::NB; k += PS_NT) {
+            const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
+            s_x1[k] = inner_node(gi, gj, n0, n1) ? w * s_b[k] * A.invd[idx(gi, gj, n0)] : T(0);
+        }
+        __syncthreads();
+    }
+    if (NPRE >= 1) {
+        constexpr int HB = TL::HB, BWX = TL::BWX;
+        for (int k = tid; k < TL::SX * TL::SY; k += PS_NT) {
+            const int li = k % TL::SX, lj = k / TL::SX;
+            const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
+            const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
+            T v = T(0);
+            if (inner_node(gi, gj, n0, n1)) {
+                const T id = A.invd[idx(gi, gj, n0)];
+                if (NPRE == 1) {
+                    v = w * s_b[kb] * id;
+                } else {
+                    const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                        return s_x1[(jj - fj0 + HB) * BWX + (ii - fi0 + HB)];
+                    });
+                    v = s_x1[kb] + w * (s_b[kb] - ax) * id;
+                }
+            }
+            s_xs[k] = v;
+        }
+        __syncthreads();
+    }
+    // residual, halo 1
+    for (int k = tid; k < TL::RX * TL::RY; k += PS_NT) {
+        const int li = k % TL::RX, lj = k / TL::RX;
+        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
+        const int kb = (lj + TL::HB - 1) * TL::BWX + (li + TL::HB - 1);
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            if (NPRE == 0) {
+                v = s_b[kb];
+            } else {
+                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                    return s_xs[(jj - fj0 + 2) * TL::SX + (ii - fi0 + 2)];
+                });
+                v = s_b[kb] - ax;
+            }
+        }
+        if (NPRE != 0) s_r[k] = v;
+    }
+    // owned nodes: smoothed x and the mean-free right-hand side
+    for (int k = tid; k < MG_FX * MG_FY; k += PS_NT) {
+        const int li = k % MG_FX, lj = k / MG_FX;
+        const int gi = fi0 + li, gj = fj0 + lj;
+        if (gi >= n0 || gj >= n1) continue;
+        const size_t P = idx(gi, gj, n0);
+        A.x[P] = NPRE >= 1 ? s_xs[(lj + 2) * TL::SX + (li + 2)] : T(0);
+        if (A.bout) A.bout[P] = s_b[(lj + TL::HB) * TL::BWX + (li + TL::HB)];
+    }
+    __syncthreads();
+    // full weighting onto the coarse tile, one coarse node per thread
+    double cs[1] = {0.0};
+    {
+        const int ci = tid % MG_CX, cj = tid / MG_CX;
+        const int I = blockIdx.x * MG_CX + ci, J = blockIdx.y * MG_CY + cj;
+        if (I < A.m0 && J < A.m1) {
+            T v = T(0);
+            if (inner_node(I, J, A.m0, A.m1)) {
+                // residual-region index of fine node (2I + di, 2J + dj)
+                auto R = [&](int di, int dj) {
+                    const int li = 2 * ci + 1 + di, lj = 2 * cj + 1 + dj;
+                    return NPRE == 0 ? s_b[(lj + TL::HB - 1) * TL::BWX + (li + TL::HB - 1)] : s_r[lj * TL::RX + li];
+                };
+                v = (T(4) * R(0, 0) + T(2) * (R(-1, 0) + R(1, 0) + R(0, -1) + R(0, 1)) +
+                     (R(-1, -1) + R(1, -1) + R(-1, 1) + R(1, 1))) /
+                    T(16);
+                cs[0] = (double)v;
+            }
+            A.craw[idx(I, J, A.m0)] = v;
+        }
+    }
+    block_partials<1>(cs, A.cpart, 0);
+}
+
+template <typename T>
+struct UpArgs {
+    FArgs<T> a;
+    const T *invd;
+    const T *b;    // right-hand side of the level (mean removed)
+    const T *x;    // pre-smoothed x
+    int m0, m1;
+    const T *xc;   // coarse-level correction
+    T *out;        // x + prolongation, post-smoothed
+    T w;
+};
+
+template <typename T, int NPOST>
+__global__ __launch_bounds__(PS_NT) void k_mg_up(const double *gd, UpArgs<T> A) {
+    if (gd && *gd != 0.0) return;
+    constexpr int YX = MG_FX + 2 * NPOST, YY = MG_FY + 2 * NPOST;  // x + e, halo NPOST
+    constexpr int ZX = MG_FX + 2, ZY = MG_FY + 2;                  // first post sweep (NPOST == 2)
+    __shared__ T s_y[YX * YY];
+    __shared__ T s_z[NPOST == 2 ? ZX * ZY : 1];
+    const int tid = tlin();
+    const int n0 = A.a.n0, n1 = A.a.n1, m0 = A.m0, m1 = A.m1;
+    const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
+    const T w = A.w;
+    const T *xc = A.xc;
+    for (int k = tid; k < YX * YY; k += PS_NT) {
+        const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            const int I = gi >> 1, J = gj >> 1;
+#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
+            T e;
+            if ((gi & 1) == 0 && (gj & 1) == 0) e = XC(I, J);
+            else if ((gj & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
+            else if ((gi & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
+            else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
+#undef XC
+            v = A.x[idx(gi, gj, n0)] + e;
+        }
+        s_y[k] = v;
+    }
+    __syncthreads();
+    if (NPOST == 2) {
+        for (int k = tid; k < ZX * ZY; k += PS_NT) {
+            const int li = k % ZX, lj = k / ZX;
+            const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
+            T v = T(0);
+            if (inner_node(gi, gj, n0, n1)) {
+                const size_t P = idx(gi, gj, n0);
+                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                    return s_y[(jj - fj0 + 2) * YX + (ii - fi0 + 2)];
+                });
+                v = s_y[(lj + 1) * YX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+            }
+            s_z[k] = v;
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < MG_FX * MG_FY; k += PS_NT) {
+        const int li = k % MG_FX, lj = k / MG_FX;
+        const int gi = fi0 + li, gj = fj0 + lj;
+        if (gi >= n0 || gj >= n1) continue;
+        const size_t P = idx(gi, gj, n0);
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            if (NPOST == 0) {
+                v = s_y[lj * YX + li];
+            } else if (NPOST == 1) {
+                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                    return s_y[(jj - fj0 + 1) * YX + (ii - fi0 + 1)];
+                });
+                v = s_y[(lj + 1) * YX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+            } else {
+                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                    return s_z[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
+                });
+                v = s_z[(lj + 1) * ZX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+            }
+        }
+        A.out[P] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-level V-cycle in one workgroup, resident in LDS.  Levels of at most
+// 33 x 33 nodes are latency-bound: a launch per sweep would cost more than the
+// sweep, and so would a global-memory round trip per sweep.  This kernel loads
+// the coefficients of every level from l0 down (and level l0's right-hand side)
+// into LDS once, runs the V-cycle there with block barriers between the phases,
+// and writes level l0's result back.  The arithmetic per node is the same device
+// code as the fine-level kernels.
+
+constexpr int CW = 1024;          // threads of the coarse workgroup
+constexpr int CW_NODES = 1089;    // first level handled: at most 33 x 33 nodes
+constexpr int CN_MAX = 1536;      // nodes of all LDS-resident levels together
+constexpr int CW_ARR = 8;         // per level: fen fet fnn fnt invd x t b
 
 template <typename T>
 struct CLev {
     int n0, n1;
-    T *fen, *fet, *fnn, *fnt, *invd, *x, *b, *r, *t;
+    const T *fen, *fet, *fnn, *fnt, *invd;
 };
 template <typename T>
 struct CoarseArgs {
@@ -361,106 +520,151 @@ struct CoarseArgs {
     int pre, post, coarse;
     T omega;
     CLev<T> L[PS_MAXLEV];
+    int off[PS_MAXLEV];  // LDS offset of each level's arrays (levels >= l0)
 };
 
-template <typename T>
-__device__ __forceinline__ void cw_smooth(const CLev<T> &L, const T *xi, T *xo, T w, bool first) {
-    const FArgs<T> a{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt};
-    const int n = L.n0 * L.n1;
-    for (int k = threadIdx.x; k < n; k += CW) {
-        const int i = k % L.n0, j = k / L.n0;
-        if (i == 0 || j == 0 || i == L.n0 - 1 || j == L.n1 - 1) { xo[k] = T(0); continue; }
-        if (first) xo[k] = w * L.b[k] * L.invd[k];
-        else xo[k] = xi[k] + w * (L.b[k] - frow<T>(a, xi, i, j)) * L.invd[k];
-    }
+__device__ __forceinline__ double cw_sum(double v) {
+    __shared__ double red[CW / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
+    double tot = 0.0;
+    for (int q = 0; q < CW / 64; q++) tot += red[q];
+    __syncthreads();
+    return tot;
 }
 
+// b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
+// whose mean is removed; null: used as is); the level-l0 result is written to x0out.
 template <typename T>
-__global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T> A, int l0) {
+__global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T> A, int l0, const T *b0,
+                                                  const double *b0part, int nb0p, T *x0out) {
     if (gd && *gd != 0.0) return;
-    __shared__ double red[CW / 64];
-    T *xc[PS_MAXLEV], *tc[PS_MAXLEV];
-    for (int l = 0; l < A.nlev; l++) { xc[l] = A.L[l].x; tc[l] = A.L[l].t; }
+    __shared__ T sh[CN_MAX * CW_ARR];
+    const int tid = threadIdx.x;
     const T w = A.omega;
     const int last = A.nlev - 1;
-    // sweeps on level l starting from zero or from xc[l]; result in xc[l]
+    auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
+    auto ARR = [&](int l, int k) { return sh + A.off[l] + k * NN(l); };
+    auto FA = [&](int l) {
+        return FArgs<T>{A.L[l].n0, A.L[l].n1, ARR(l, 0), ARR(l, 1), ARR(l, 2), ARR(l, 3)};
+    };
+    // coefficients of every level, right-hand side of level l0
+    for (int l = l0; l <= last; l++) {
+        const int n = NN(l);
+        const T *src[5] = {A.L[l].fen, A.L[l].fet, A.L[l].fnn, A.L[l].fnt, A.L[l].invd};
+        for (int q = 0; q < 5; q++) {
+            T *dst = ARR(l, q);
+            for (int k = tid; k < n; k += CW) dst[k] = src[q][k];
+        }
+    }
+    {
+        const int n0 = A.L[l0].n0, n1 = A.L[l0].n1, n = NN(l0);
+        double part = 0.0;
+        if (b0part)
+            for (int t = tid; t < nb0p; t += CW) part += b0part[t];
+        const double mean = cw_sum(part) / ((double)(n0 - 2) * (double)(n1 - 2));
+        T *b = ARR(l0, 7);
+        for (int k = tid; k < n; k += CW) {
+            T v = T(0);
+            if (inner_node(k % n0, k / n0, n0, n1)) v = b0part ? (T)((double)b0[k] - mean) : b0[k];
+            b[k] = v;
+        }
+    }
+    __syncthreads();
+    // `count` Jacobi sweeps on level l (from zero when from_zero); result in x
     auto sweeps = [&](int l, int count, bool from_zero) {
+        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
+        const FArgs<T> a = FA(l);
+        const T *invd = ARR(l, 4), *b = ARR(l, 7);
+        T *x = ARR(l, 5), *t = ARR(l, 6);
+        T *src = x, *dst = t;
         for (int s = 0; s < count; s++) {
             if (from_zero && s == 0) {
-                cw_smooth<T>(A.L[l], xc[l], xc[l], w, true);
+                for (int k = tid; k < n; k += CW) x[k] = inner_node(k % n0, k / n0, n0, n1) ? w * b[k] * invd[k] : T(0);
             } else {
-                cw_smooth<T>(A.L[l], xc[l], tc[l], w, false);
-                T *tmp = xc[l]; xc[l] = tc[l]; tc[l] = tmp;
+                for (int k = tid; k < n; k += CW) {
+                    const int i = k % n0, j = k / n0;
+                    T v = T(0);
+                    if (inner_node(i, j, n0, n1))
+                        v = src[k] + w * (b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return src[jj * n0 + ii]; })) *
+                                         invd[k];
+                    dst[k] = v;
+                }
+                T *tmp = src; src = dst; dst = tmp;
             }
+            __syncthreads();
+        }
+        if (src != x) {
+            for (int k = tid; k < n; k += CW) x[k] = src[k];
+            __syncthreads();
         }
     };
     for (int l = l0; l < last; l++) {
-        const CLev<T> &L = A.L[l];
-        const int n = L.n0 * L.n1;
+        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
         if (A.pre > 0) {
             sweeps(l, A.pre, true);
         } else {
-            for (int k = threadIdx.x; k < n; k += CW) xc[l][k] = T(0);
+            T *x = ARR(l, 5);
+            for (int k = tid; k < n; k += CW) x[k] = T(0);
             __syncthreads();
         }
-        // residual
-        const FArgs<T> a{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt};
-        for (int k = threadIdx.x; k < n; k += CW) {
-            const int i = k % L.n0, j = k / L.n0;
-            if (i == 0 || j == 0 || i == L.n0 - 1 || j == L.n1 - 1) { L.r[k] = T(0); continue; }
-            L.r[k] = L.b[k] - frow<T>(a, xc[l], i, j);
+        // residual into t
+        {
+            const FArgs<T> a = FA(l);
+            const T *x = ARR(l, 5), *b = ARR(l, 7);
+            T *r = ARR(l, 6);
+            for (int k = tid; k < n; k += CW) {
+                const int i = k % n0, j = k / n0;
+                T v = T(0);
+                if (inner_node(i, j, n0, n1)) v = b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return x[jj * n0 + ii]; });
+                r[k] = v;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        // full weighting onto level l+1 and the interior sum for the mean projection
-        const CLev<T> &C = A.L[l + 1];
-        const int m = C.n0 * C.n1;
-        double part = 0.0;
-        for (int k = threadIdx.x; k < m; k += CW) {
-            const int I = k % C.n0, J = k / C.n0;
-            if (I == 0 || J == 0 || I == C.n0 - 1 || J == C.n1 - 1) { C.b[k] = T(0); continue; }
-            const int i = 2 * I, j = 2 * J, f = L.n0;
-            const T *r = L.r;
-            const T v = (T(4) * r[j * f + i] +
+        // full weighting onto level l+1, interior mean removed
+        {
+            const int m0 = A.L[l + 1].n0, m1 = A.L[l + 1].n1, m = NN(l + 1);
+            const T *r = ARR(l, 6);
+            T *bc = ARR(l + 1, 7);
+            double part = 0.0;
+            for (int k = tid; k < m; k += CW) {
+                const int I = k % m0, J = k / m0;
+                T v = T(0);
+                if (inner_node(I, J, m0, m1)) {
+                    const int i = 2 * I, j = 2 * J, f = n0;
+                    v = (T(4) * r[j * f + i] +
                          T(2) * (r[j * f + i - 1] + r[j * f + i + 1] + r[(j - 1) * f + i] + r[(j + 1) * f + i]) +
                          (r[(j - 1) * f + i - 1] + r[(j - 1) * f + i + 1] + r[(j + 1) * f + i - 1] +
                           r[(j + 1) * f + i + 1])) /
                         T(16);
-            C.b[k] = v;
-            part += (double)v;
+                    part += (double)v;
+                }
+                bc[k] = v;
+            }
+            const double mean = cw_sum(part) / ((double)(m0 - 2) * (double)(m1 - 2));
+            for (int k = tid; k < m; k += CW)
+                if (inner_node(k % m0, k / m0, m0, m1)) bc[k] = (T)((double)bc[k] - mean);
+            __syncthreads();
         }
-        // block sum (same shape as block_sum, sized for CW threads)
-        part = wave_sum(part);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
-        __syncthreads();
-        double tot = 0.0;
-        for (int q = 0; q < CW / 64; q++) tot += red[q];
-        __syncthreads();
-        const double mean = tot / ((double)(C.n0 - 2) * (double)(C.n1 - 2));
-        for (int k = threadIdx.x; k < m; k += CW) {
-            const int I = k % C.n0, J = k / C.n0;
-            if (I == 0 || J == 0 || I == C.n0 - 1 || J == C.n1 - 1) continue;
-            C.b[k] = (T)((double)C.b[k] - mean);
-        }
-        __syncthreads();
     }
     // coarsest level
     if (A.coarse > 0) {
         sweeps(last, A.coarse, true);
     } else {
-        const int n = A.L[last].n0 * A.L[last].n1;
-        for (int k = threadIdx.x; k < n; k += CW) xc[last][k] = T(0);
+        T *x = ARR(last, 5);
+        for (int k = tid; k < NN(last); k += CW) x[k] = T(0);
         __syncthreads();
     }
     for (int l = last - 1; l >= l0; l--) {
-        const CLev<T> &L = A.L[l];
-        const CLev<T> &C = A.L[l + 1];
-        const int n = L.n0 * L.n1;
-        const T *xcc = xc[l + 1];
-        for (int k = threadIdx.x; k < n; k += CW) {
-            const int i = k % L.n0, j = k / L.n0;
-            if (i < 1 || i > L.n0 - 2 || j < 1 || j > L.n1 - 2) continue;
-            const int I = i >> 1, J = j >> 1, m0 = C.n0, m1 = C.n1;
+        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
+        const int m0 = A.L[l + 1].n0, m1 = A.L[l + 1].n1;
+        const T *xcc = ARR(l + 1, 5);
+        T *x = ARR(l, 5);
+        for (int k = tid; k < n; k += CW) {
+            const int i = k % n0, j = k / n0;
+            if (!inner_node(i, j, n0, n1)) continue;
+            const int I = i >> 1, J = j >> 1;
 #define XC(a_, b_) xcc[clampi((b_), 1, m1 - 2) * m0 + clampi((a_), 1, m0 - 2)]
             T e;
             if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
@@ -468,87 +672,276 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T
             else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
             else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
 #undef XC
-            xc[l][k] += e;
+            x[k] += e;
         }
         __syncthreads();
         sweeps(l, A.post, false);
     }
-    // hand the level-l0 result back in its x buffer
-    if (xc[l0] != A.L[l0].x) {
-        const int n = A.L[l0].n0 * A.L[l0].n1;
-        for (int k = threadIdx.x; k < n; k += CW) A.L[l0].x[k] = xc[l0][k];
+    {
+        const T *x = ARR(l0, 5);
+        for (int k = tid; k < NN(l0); k += CW) x0out[k] = x[k];
     }
 }
 
-// --- small guarded helpers and the device-side control of the captured solve
+// ---------------------------------------------------------------------------
+// BiCGStab (right preconditioned): one kernel per vector phase writing block
+// partials of its dot products, then k_bi_fin<phase> with the totals and the
+// scalar recurrence / convergence test of that phase.
+
+struct RedArgs {
+    double *part;
+    int pstride;
+    double *scal;
+};
+
+// x (and, if acc != null, acc += x first) <- 0; r = b (interior); partials of
+// sum(r), (b, b).  outer != null and set: the solve is skipped (S_DONE = 1),
+// x is still cleared.
 template <typename T>
-__global__ void k_fill(const double *gd, size_t n, T *__restrict__ a, T v) {
-    if (gd && *gd != 0.0) return;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = v;
+__global__ __launch_bounds__(PS_NT) void k_bi_start1(const double *outer, int n0, int n1, const double *__restrict__ b,
+                                                     T *__restrict__ x, double *__restrict__ acc, T *__restrict__ r,
+                                                     RedArgs ra) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const bool in = i < n0 && j < n1;
+    const size_t P = in ? idx(i, j, n0) : 0;
+    if (in) {
+        if (acc) acc[P] += (double)x[P];
+        x[P] = T(0);
+    }
+    if (outer && *outer != 0.0) {
+        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
+        return;
+    }
+    double v[2] = {0.0, 0.0};
+    if (in) {
+        T rv = T(0);
+        if (inner_node(i, j, n0, n1)) {
+            rv = (T)b[P];
+            v[0] = (double)rv;
+            v[1] = (double)rv * (double)rv;
+        }
+        r[P] = rv;
+    }
+    block_partials<2>(v, ra.part, ra.pstride);
 }
+
+// r -= interior mean (S_SUM), r0 = r; partials of (r, r)
 template <typename T>
-__global__ void k_copy(const double *gd, size_t n, const T *__restrict__ a, T *__restrict__ b) {
-    if (gd && *gd != 0.0) return;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) b[k] = a[k];
+__global__ __launch_bounds__(PS_NT) void k_bi_start2(const double *outer, int n0, int n1, T *__restrict__ r,
+                                                     T *__restrict__ r0, RedArgs ra) {
+    if (outer && *outer != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const double mean = ra.scal[S_SUM] / ((double)(n0 - 2) * (double)(n1 - 2));
+    double v[1] = {0.0};
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        T rv = T(0);
+        if (inner_node(i, j, n0, n1)) {
+            rv = (T)((double)r[P] - mean);
+            r[P] = rv;
+            v[0] = (double)rv * (double)rv;
+        }
+        r0[P] = rv;
+    }
+    block_partials<1>(v, ra.part, ra.pstride);
 }
-// x += alpha ph, only when this iteration stopped at the ||s|| test
+
+// p = r (first iteration) or r + beta (p - omega v); stops the solve when rho = 0
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bicg_x1g(int n0, int n1, const double *scal, const T *__restrict__ ph,
-                                                    T *__restrict__ x) {
-    if (scal[S_SCONV] == 0.0) return;
+__global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *scal, const T *__restrict__ r,
+                                                const T *__restrict__ vv, T *__restrict__ pv) {
+    if (scal[S_DONE] != 0.0) return;
+    const double rho = scal[S_RHO];
+    if (blin() == 0 && tlin() == 0) {
+        scal[S_IT] = (double)it;
+        if (rho == 0.0) scal[S_DONE] = 1.0;
+    }
+    if (rho == 0.0) return;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= n0 || j >= n1) return;
     const size_t P = idx(i, j, n0);
-    x[P] = (T)((double)x[P] + scal[S_ALPHA] * (double)ph[P]);
+    if (it == 1) { pv[P] = r[P]; return; }
+    const double beta = (rho / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+    pv[P] = (T)((double)r[P] + beta * ((double)pv[P] - scal[S_OMEGA] * (double)vv[P]));
 }
-// after (b,b), (r,r): tolerance, initial test and start values; an outer guard
-// (the refinement's done flag) marks the whole solve as done
-__global__ void k_g_init(double *scal, double rtol, int rtol_slot, const double *outer) {
-    const double bn = sqrt(scal[S_BB]);
-    const double rt = rtol_slot >= 0 ? scal[rtol_slot] : rtol;
-    const double tol = rt * bn;
-    const double rn = sqrt(scal[S_RR]);
-    scal[S_BN] = bn;
-    scal[S_TOL] = tol;
-    scal[S_RN] = rn;
-    scal[S_IT] = 0.0;
-    scal[S_SCONV] = 0.0;
-    scal[S_RHO] = scal[S_RR];
-    scal[S_RHO_OLD] = 1.0;
-    scal[S_ALPHA] = 1.0;
-    scal[S_OMEGA] = 1.0;
-    scal[S_DONE] = (bn == 0.0 || rn <= tol || (outer && *outer != 0.0)) ? 1.0 : 0.0;
+
+// v = L ph; partials of (r0, v)
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict__ ph, T *__restrict__ vv,
+                                                const T *__restrict__ r0, RedArgs ra) {
+    if (ra.scal[S_DONE] != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[1] = {0.0};
+    if (i < a.n0 && j < a.n1) {
+        const size_t P = idx(i, j, a.n0);
+        T y = T(0);
+        if (inner_node(i, j, a.n0, a.n1)) {
+            y = frow<T>(a, ph, i, j);
+            v[0] = (double)r0[P] * (double)y;
+        }
+        vv[P] = y;
+    }
+    block_partials<1>(v, ra.part, ra.pstride);
 }
-__global__ void k_g_begin(double *scal, int k) {
-    if (scal[S_DONE] != 0.0) return;
-    scal[S_IT] = (double)k;
-    if (scal[S_RHO] == 0.0) scal[S_DONE] = 1.0;
+
+// s = r - alpha v; partials of (s, s)
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restrict__ r, const T *__restrict__ vv,
+                                                T *__restrict__ s, RedArgs ra) {
+    if (ra.scal[S_DONE] != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const double alpha = ra.scal[S_ALPHA];
+    double v[1] = {0.0};
+    if (i < n0 && j < n1) {
+        const size_t P = idx(i, j, n0);
+        const T sv = (T)((double)r[P] - alpha * (double)vv[P]);
+        s[P] = sv;
+        if (inner_node(i, j, n0, n1)) v[0] = (double)sv * (double)sv;
+    }
+    block_partials<1>(v, ra.part, ra.pstride);
 }
-__global__ void k_g_check_s(double *scal) {
-    if (scal[S_DONE] != 0.0) { scal[S_SCONV] = 0.0; return; }
-    const double sn = sqrt(scal[S_SS]);
-    if (sn <= scal[S_TOL]) {
-        scal[S_SCONV] = 1.0;
-        scal[S_DONE] = 1.0;
-        scal[S_RN] = sn;
+
+// t = L sh; partials of (t, s), (t, t)
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bi_t(FArgs<T> a, const T *__restrict__ sh, T *__restrict__ t,
+                                                const T *__restrict__ s, RedArgs ra) {
+    if (ra.scal[S_DONE] != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[2] = {0.0, 0.0};
+    if (i < a.n0 && j < a.n1) {
+        const size_t P = idx(i, j, a.n0);
+        T y = T(0);
+        if (inner_node(i, j, a.n0, a.n1)) {
+            y = frow<T>(a, sh, i, j);
+            v[0] = (double)y * (double)s[P];
+            v[1] = (double)y * (double)y;
+        }
+        t[P] = y;
+    }
+    block_partials<2>(v, ra.part, ra.pstride);
+}
+
+// x += alpha ph + omega sh, r = s - omega t, partials of (r, r), (r0, r); in the
+// iteration that stopped at the ||s|| test only x += alpha ph
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T *__restrict__ ph,
+                                                 const T *__restrict__ sh, const T *__restrict__ s,
+                                                 const T *__restrict__ t, const T *__restrict__ r0, T *__restrict__ x,
+                                                 T *__restrict__ r, RedArgs ra) {
+    const double *sc = ra.scal;
+    const bool sconv = sc[S_SCONV] == (double)it;
+    if (!sconv && sc[S_DONE] != 0.0) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const bool in = i < n0 && j < n1;
+    const size_t P = in ? idx(i, j, n0) : 0;
+    const double alpha = sc[S_ALPHA];
+    if (sconv) {
+        if (in) x[P] = (T)((double)x[P] + alpha * (double)ph[P]);
+        return;
+    }
+    const double omega = sc[S_OMEGA];
+    double v[2] = {0.0, 0.0};
+    if (in) {
+        x[P] = (T)((double)x[P] + alpha * (double)ph[P] + omega * (double)sh[P]);
+        const T rv = (T)((double)s[P] - omega * (double)t[P]);
+        r[P] = rv;
+        if (inner_node(i, j, n0, n1)) {
+            v[0] = (double)rv * (double)rv;
+            v[1] = (double)r0[P] * (double)rv;
+        }
+    }
+    block_partials<2>(v, ra.part, ra.pstride);
+}
+
+enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
+
+// totals of a phase's partials and what follows from them; skipped while *guard
+// is set (the phase's kernel did not run)
+template <int OP>
+__global__ __launch_bounds__(PS_NT) void k_bi_fin(const double *guard, RedArgs ra, int nparts, int it, double rtol,
+                                                  int rtol_slot) {
+    if (guard && *guard != 0.0) return;
+    constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
+    double tot[NV];
+    sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
+    if (tlin() != 0) return;
+    double *s = ra.scal;
+    if (OP == FIN_START1) {
+        s[S_SUM] = tot[0];
+        s[S_BB] = tot[NV - 1];
+        s[S_DONE] = 0.0;
+        s[S_IT] = 0.0;
+        s[S_SCONV] = 0.0;
+    } else if (OP == FIN_START2) {
+        const double bn = sqrt(s[S_BB]);
+        const double rt = rtol_slot >= 0 ? s[rtol_slot] : rtol;
+        const double tol = rt * bn;
+        const double rn = sqrt(tot[0]);
+        s[S_RR] = tot[0];
+        s[S_BN] = bn;
+        s[S_TOL] = tol;
+        s[S_RN] = rn;
+        s[S_RELRES] = bn > 0.0 ? rn / bn : 0.0;
+        s[S_RHO] = tot[0];
+        s[S_RHO_OLD] = 1.0;
+        s[S_ALPHA] = 1.0;
+        s[S_OMEGA] = 1.0;
+        s[S_DONE] = (bn == 0.0 || rn <= tol) ? 1.0 : 0.0;
+    } else if (OP == FIN_V) {
+        s[S_RV] = tot[0];
+        s[S_ALPHA] = s[S_RHO] / tot[0];
+    } else if (OP == FIN_S) {
+        s[S_SS] = tot[0];
+        const double sn = sqrt(tot[0]);
+        if (sn <= s[S_TOL]) {
+            s[S_SCONV] = (double)it;
+            s[S_DONE] = 1.0;
+            s[S_RN] = sn;
+            s[S_RELRES] = s[S_BN] > 0.0 ? sn / s[S_BN] : 0.0;
+        }
+    } else if (OP == FIN_T) {
+        s[S_TS] = tot[0];
+        s[S_TT] = tot[NV - 1];
+        s[S_OMEGA] = tot[NV - 1] > 0.0 ? tot[0] / tot[NV - 1] : 0.0;
     } else {
-        scal[S_SCONV] = 0.0;
+        s[S_RR] = tot[0];
+        s[S_R0R] = tot[NV - 1];
+        const double rn = sqrt(tot[0]);
+        s[S_RN] = rn;
+        s[S_RELRES] = s[S_BN] > 0.0 ? rn / s[S_BN] : 0.0;
+        if (rn <= s[S_TOL] || s[S_OMEGA] == 0.0) {
+            s[S_DONE] = 1.0;
+        } else {
+            s[S_RHO_OLD] = s[S_RHO];
+            s[S_RHO] = tot[NV - 1];
+        }
     }
 }
-__global__ void k_g_check_r(double *scal) {
-    if (scal[S_DONE] != 0.0) return;
-    const double rn = sqrt(scal[S_RR]);
-    scal[S_RN] = rn;
-    if (rn <= scal[S_TOL] || scal[S_OMEGA] == 0.0) {
-        scal[S_DONE] = 1.0;
-    } else {
-        scal[S_RHO_OLD] = scal[S_RHO];
-        scal[S_RHO] = scal[S_R0R];
-    }
+
+template <typename A, typename B>
+__global__ void k_cvt(size_t n, const A *__restrict__ a, B *__restrict__ b) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = (B)a[k];
 }
-__global__ void k_g_final(double *scal) { scal[S_RELRES] = scal[S_BN] > 0.0 ? scal[S_RN] / scal[S_BN] : 0.0; }
+template <typename T>
+__global__ void k_fill(size_t n, T *__restrict__ a, T v) {
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+__global__ void k_inject(int n0, const double *__restrict__ f, int m0, int m1, double *__restrict__ c) {
+    const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+    if (I >= m0 || J >= m1) return;
+    c[idx(I, J, m0)] = f[idx(2 * I, 2 * J, n0)];
+}
+// v -= interior mean (scal[slot] holds the interior sum)
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_sub_mean(int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
+                                                    int slot) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (!inner_node(i, j, n0, n1)) return;
+    const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
+    v[idx(i, j, n0)] = (T)((double)v[idx(i, j, n0)] - m);
+}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -566,7 +959,11 @@ struct PsCtx {
     void *owner;
     std::vector<void *> *allocs;
     size_t *bytes;
+    RedArgs ra() const { return RedArgs{part, pstride, scal}; }
 };
+
+// live-timing kinds of the solver's fine-level kernels
+enum { PK_APPLY = 0, PK_MG_DOWN = 4, PK_MG_UP = 5 };
 
 inline int ps_alloc(PsCtx &cx, void **p, size_t bytes) {
     if (hipMalloc(p, bytes) != hipSuccess) return CFD_E_OOM;
@@ -581,39 +978,10 @@ inline int nblk(int n0, int n1) {
     return (int)(g.x * g.y);
 }
 
-inline int ps_reduce(PsCtx &cx, int nparts, int nv, int dst, const double *gd = nullptr) {
-    k_reduce_sum<<<nv, 256, 0, cx.s>>>(gd, cx.part, cx.pstride, nparts, nv, cx.scal, dst);
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
-}
-
 inline int ps_sync(PsCtx &cx) {
     CFD_HIP(hipMemcpyAsync(cx.hscal, cx.scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, cx.s));
     CFD_HIP(hipStreamSynchronize(cx.s));
     if (cx.drain) cx.drain(cx.owner);
-    return CFD_OK;
-}
-
-template <typename T>
-int ps_project_mean(PsCtx &cx, int n0, int n1, T *v, const double *gd = nullptr) {
-    k_sum_int<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(gd, n0, n1, v, cx.part);
-    CFD_LAUNCH_CHECK();
-    CFD_RET(ps_reduce(cx, nblk(n0, n1), 1, S_SUM, gd));
-    k_sub_mean<T><<<grid2d(n0, n1), dim3(BX, BY), 0, cx.s>>>(gd, n0, n1, v, cx.scal, S_SUM);
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
-}
-
-template <typename T>
-int ps_fill(PsCtx &cx, T *a, size_t n, const double *gd = nullptr) {
-    k_fill<T><<<(unsigned)((n + 255) / 256), 256, 0, cx.s>>>(gd, n, a, T(0));
-    CFD_LAUNCH_CHECK();
-    return CFD_OK;
-}
-template <typename T>
-int ps_copy(PsCtx &cx, const T *a, T *b, size_t n, const double *gd = nullptr) {
-    k_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, cx.s>>>(gd, n, a, b);
-    CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 
@@ -627,7 +995,11 @@ struct PSolver {
     double omega = 0.8;
     int lc = 0;                  // first level handled by the single-workgroup kernel
     CoarseArgs<T> cargs{};
-    static constexpr int COARSE_NODES = 4225;  // levels with at most 65 x 65 nodes
+    LevTab<T> tab{};
+    int tab_blocks = 0;
+    double *lpart = nullptr;       // block partials of the restricted right-hand sides' sums
+    int lofs[PS_MAXLEV] = {0};     // level l's partials at lpart + lofs[l] (l >= 1)
+    int lnp[PS_MAXLEV] = {0};      // and their number
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
         nlev = nlev_;
@@ -635,240 +1007,190 @@ struct PSolver {
         post = prm.mg_post;
         coarse = prm.mg_coarse;
         omega = prm.mg_omega;
+        if (pre < 0 || pre > 2 || post < 0 || post > 2 || coarse < 0) return CFD_E_ARG;
+        // the LDS-resident coarse kernel takes every level from the first one with
+        // at most CW_NODES nodes whose levels together fit its LDS arrays
+        lc = nlev - 1;
+        while (lc > 0 && g[lc - 1].n0 * g[lc - 1].n1 <= CW_NODES) lc--;
+        for (;;) {
+            int tot = 0;
+            for (int l = lc; l < nlev; l++) tot += g[l].n0 * g[l].n1;
+            if (tot <= CN_MAX) break;
+            if (lc + 1 >= nlev) return CFD_E_ARG;
+            lc++;
+        }
         for (int l = 0; l < nlev; l++) {
             OpLevel<T> &L = lev[l];
             L.n0 = g[l].n0;
             L.n1 = g[l].n1;
             const size_t M = (size_t)L.n0 * L.n1 * sizeof(T);
-            T **arr[] = {&L.fen, &L.fet, &L.fnn, &L.fnt, &L.invd, &L.x, &L.b, &L.r, &L.t};
+            T **arr[] = {&L.fen, &L.fet, &L.fnn, &L.fnt, &L.invd, &L.x, &L.b, &L.t};
             for (T **a : arr) CFD_RET(ps_alloc(cx, (void **)a, M));
         }
         const size_t M0 = (size_t)lev[0].n0 * lev[0].n1 * sizeof(T);
         for (int k = 0; k < 8; k++) CFD_RET(ps_alloc(cx, (void **)&bi[k], M0));
-        lc = nlev - 1;
-        while (lc > 0 && (size_t)lev[lc - 1].n0 * lev[lc - 1].n1 <= COARSE_NODES) lc--;
+        int np = 0;
+        for (int l = 1; l <= lc; l++) {
+            lofs[l] = np;
+            lnp[l] = down_grid(l - 1).x * down_grid(l - 1).y;
+            np += lnp[l];
+        }
+        CFD_RET(ps_alloc(cx, (void **)&lpart, (size_t)(np + 1) * sizeof(double)));
         cargs.nlev = nlev;
         cargs.pre = pre;
         cargs.post = post;
         cargs.coarse = coarse;
         cargs.omega = (T)omega;
+        int off = 0;
         for (int l = 0; l < nlev; l++) {
             const OpLevel<T> &L = lev[l];
-            cargs.L[l] = CLev<T>{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt, L.invd, L.x, L.b, L.r, L.t};
+            cargs.L[l] = CLev<T>{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt, L.invd};
+            cargs.off[l] = off;
+            if (l >= lc) off += CW_ARR * L.n0 * L.n1;
         }
+        tab.nlev = nlev;
+        int b = 0;
+        for (int l = 0; l < nlev; l++) {
+            const dim3 gr = grid2d(g[l].n0, g[l].n1);
+            tab.bofs[l] = b;
+            tab.bx[l] = (int)gr.x;
+            b += (int)(gr.x * gr.y);
+            tab.n0[l] = g[l].n0;
+            tab.n1[l] = g[l].n1;
+            tab.h1[l] = g[l].h1;
+            tab.h2[l] = g[l].h2;
+            tab.xix[l] = g[l].xix;
+            tab.xiy[l] = g[l].xiy;
+            tab.etx[l] = g[l].etx;
+            tab.ety[l] = g[l].ety;
+            tab.jac[l] = g[l].jac;
+            tab.fen[l] = lev[l].fen;
+            tab.fet[l] = lev[l].fet;
+            tab.fnn[l] = lev[l].fnn;
+            tab.fnt[l] = lev[l].fnt;
+            tab.invd[l] = lev[l].invd;
+        }
+        tab.bofs[nlev] = b;
+        tab_blocks = b;
+        tab.dn0 = g[0].n0;
         return CFD_OK;
     }
 
     // per outer iteration: face coefficients and inverse diagonals of every level
-    int setup(PsCtx &cx, const Geom *g) {
-        const dim3 bl(BX, BY);
-        for (int l = 0; l < nlev; l++) {
-            OpLevel<T> &L = lev[l];
-            const dim3 gr = grid2d(L.n0, L.n1);
-            k_p_faces<T><<<gr, bl, 0, cx.s>>>(L.n0, L.n1, g[l].h1, g[l].h2, g[l].xix, g[l].xiy, g[l].etx, g[l].ety,
-                                              g[l].jac, g[l].d, L.fen, L.fet, L.fnn, L.fnt);
-            k_finvdiag<T><<<gr, bl, 0, cx.s>>>(fargs(L), L.invd);
-            CFD_LAUNCH_CHECK();
-        }
+    // from level 0's d; f64 (optional) receives level 0's faces in FP64
+    int setup(PsCtx &cx, const double *d0, double *const f64[4]) {
+        LevTab<T> t = tab;
+        t.d0 = d0;
+        for (int k = 0; k < 4; k++) t.f64[k] = f64 ? f64[k] : nullptr;
+        k_faces_all<T><<<tab_blocks, PS_NT, 0, cx.s>>>(t);
+        k_invd_all<T><<<tab_blocks, PS_NT, 0, cx.s>>>(t);
+        CFD_LAUNCH_CHECK();
         return CFD_OK;
     }
 
-    // sweeps on level l; result in L.x (host-side ping-pong of x and t)
-    int smooth(PsCtx &cx, int l, int sweeps, bool from_zero, const double *gd) {
-        OpLevel<T> &L = lev[l];
-        const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-        const FArgs<T> a = fargs(L);
-        const T w = (T)omega;
-        for (int s = 0; s < sweeps; s++) {
-            if (from_zero && s == 0) {
-                k_fsmooth<T, true><<<gr, bl, 0, cx.s>>>(gd, a, w, L.x, L.b, L.invd, L.x);
-            } else {
-                if (l == 0) cx.prof_b(cx.owner, 4);
-                k_fsmooth<T, false><<<gr, bl, 0, cx.s>>>(gd, a, w, L.x, L.b, L.invd, L.t);
-                if (l == 0) cx.prof_e(cx.owner);
-                std::swap(L.x, L.t);
-            }
-            CFD_LAUNCH_CHECK();
-        }
-        return CFD_OK;
+    // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
+    dim3 down_grid(int l) const {
+        return dim3((lev[l + 1].n0 + MG_CX - 1) / MG_CX, (lev[l + 1].n1 + MG_CY - 1) / MG_CY);
     }
 
-    int vcycle(PsCtx &cx, int l, const double *gd) {
-        OpLevel<T> &L = lev[l];
-        if (l == lc) {
-            // the rest of the cycle in one workgroup; the level's x/t roles are fixed there
-            k_mg_coarse<T><<<1, CW, 0, cx.s>>>(gd, cargs, l);
-            CFD_LAUNCH_CHECK();
-            return CFD_OK;
-        }
-        const size_t M = (size_t)L.n0 * L.n1;
-        const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-        if (pre > 0) {
-            CFD_RET(smooth(cx, l, pre, true, gd));
-        } else {
-            CFD_RET(ps_fill<T>(cx, L.x, M, gd));
-        }
-        k_fresid<T><<<gr, bl, 0, cx.s>>>(gd, fargs(L), L.x, L.b, L.r);
-        CFD_LAUNCH_CHECK();
-        OpLevel<T> &C = lev[l + 1];
-        k_restrict<T><<<grid2d(C.n0, C.n1), bl, 0, cx.s>>>(gd, L.n0, L.r, C.n0, C.n1, C.b);
-        CFD_LAUNCH_CHECK();
-        CFD_RET(ps_project_mean<T>(cx, C.n0, C.n1, C.b, gd));
-        CFD_RET(vcycle(cx, l + 1, gd));
-        k_prolong_add<T><<<gr, bl, 0, cx.s>>>(gd, L.n0, L.n1, L.x, C.n0, C.n1, C.x);
-        CFD_LAUNCH_CHECK();
-        return smooth(cx, l, post, false, gd);
-    }
-
+    // z = M^-1 y: one V-cycle from zero with right-hand side y
     int precond(PsCtx &cx, const T *y, T *z, const double *gd) {
-        OpLevel<T> &L = lev[0];
-        const size_t M = (size_t)L.n0 * L.n1;
-        CFD_RET(ps_copy<T>(cx, y, L.b, M, gd));
-        CFD_RET(vcycle(cx, 0, gd));
-        CFD_RET(ps_copy<T>(cx, L.x, z, M, gd));
-        return CFD_OK;
-    }
-
-    // r = b - A x (x = 0 on entry), projected; r0 = r; (b,b) and (r,r) into S_BB, S_RR
-    int start(PsCtx &cx, const T *b, T *x, const double *gd) {
-        OpLevel<T> &L0 = lev[0];
-        const int n0 = L0.n0, n1 = L0.n1;
-        const size_t M = (size_t)n0 * n1;
-        const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-        T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3];
-        k_fresid<T><<<gr, bl, 0, cx.s>>>(gd, fargs(L0), x, b, r);
-        CFD_LAUNCH_CHECK();
-        CFD_RET(ps_project_mean<T>(cx, n0, n1, r, gd));
-        CFD_RET(ps_copy<T>(cx, r, r0, M, gd));
-        CFD_RET(ps_fill<T>(cx, pv, M, gd));
-        CFD_RET(ps_fill<T>(cx, vv, M, gd));
-        k_dot2<T><<<gr, bl, 0, cx.s>>>(gd, n0, n1, b, b, r, r, cx.part, cx.pstride);
-        CFD_LAUNCH_CHECK();
-        return ps_reduce(cx, nblk(n0, n1), 2, S_BB, gd);
-    }
-
-    // Synchronous solve: L x = b to ||r|| <= rtol ||b|| with host-side tests,
-    // iterate for iterate the oracle's bicgstab(); x must be zero on entry.
-    int solve(PsCtx &cx, const T *b, T *x, double rtol, int maxit, int *its_out, double *relres) {
-        OpLevel<T> &L0 = lev[0];
-        const int n0 = L0.n0, n1 = L0.n1;
-        const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-        const int nb = nblk(n0, n1);
-        T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
-        const FArgs<T> a0 = fargs(L0);
-        CFD_RET(start(cx, b, x, nullptr));
-        k_g_init<<<1, 1, 0, cx.s>>>(cx.scal, rtol, -1, nullptr);
-        CFD_LAUNCH_CHECK();
-        CFD_RET(ps_sync(cx));
-        const double bnorm = std::sqrt(cx.hscal[S_BB]);
-        double rnorm = std::sqrt(cx.hscal[S_RR]);
-        *its_out = 0;
-        if (bnorm == 0.0) { *relres = 0.0; return CFD_OK; }
-        const double tol = rtol * bnorm;
-        if (rnorm <= tol) { *relres = rnorm / bnorm; return CFD_OK; }
-        double rho = cx.hscal[S_RR];
-        int it;
-        for (it = 1; it <= maxit; it++) {
-            if (rho == 0.0) break;
-            k_bicg_p<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, it == 1, cx.scal, r, vv, pv);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(precond(cx, pv, ph, nullptr));
-            cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(nullptr, a0, ph, vv);
-            cx.prof_e(cx.owner);
-            CFD_LAUNCH_CHECK();
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, r0, vv, nullptr, nullptr, cx.part, cx.pstride);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_RV));
-            k_bicg_s<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, cx.scal, r, vv, s, cx.part);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_SS));
-            CFD_RET(ps_sync(cx));
-            const double snorm = std::sqrt(cx.hscal[S_SS]);
-            if (snorm <= tol) {
-                k_bicg_x1<T><<<gr, bl, 0, cx.s>>>(n0, n1, cx.scal, ph, x);
-                CFD_LAUNCH_CHECK();
-                rnorm = snorm;
-                break;
+        const T w = (T)omega;
+        for (int l = 0; l < lc; l++) {
+            OpLevel<T> &L = lev[l];
+            OpLevel<T> &C = lev[l + 1];
+            DownArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.t, l == 0 ? nullptr : lpart + lofs[l], lnp[l],
+                          l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w};
+            const dim3 gr = down_grid(l);
+            if (l == 0) cx.prof_b(cx.owner, PK_MG_DOWN);
+            switch (pre) {
+            case 0: k_mg_down<T, 0><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            case 1: k_mg_down<T, 1><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            default: k_mg_down<T, 2><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
             }
-            CFD_RET(precond(cx, s, sh, nullptr));
-            cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(nullptr, a0, sh, t);
-            cx.prof_e(cx.owner);
+            if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, t, s, t, t, cx.part, cx.pstride);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 2, S_TS));
-            k_set_omega<<<1, 1, 0, cx.s>>>(nullptr, cx.scal);
-            CFD_LAUNCH_CHECK();
-            k_bicg_xr<T><<<gr, bl, 0, cx.s>>>(nullptr, n0, n1, cx.scal, ph, sh, s, t, r0, x, r, cx.part, cx.pstride);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_RR));
-            k_reduce_sum<<<1, 256, 0, cx.s>>>(nullptr, cx.part + cx.pstride, cx.pstride, nb, 1, cx.scal, S_R0R);
-            CFD_LAUNCH_CHECK();
-            k_rotate_rho<<<1, 1, 0, cx.s>>>(cx.scal);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_sync(cx));
-            rnorm = std::sqrt(cx.hscal[S_RR]);
-            const double omega_h = cx.hscal[S_OMEGA];
-            if (rnorm <= tol || omega_h == 0.0) break;
-            rho = cx.hscal[S_RHO];
         }
-        *its_out = it > maxit ? maxit : it;
-        *relres = rnorm / bnorm;
+        T *cout = lc == 0 ? z : lev[lc].x;
+        k_mg_coarse<T><<<1, CW, 0, cx.s>>>(gd, cargs, lc, lc == 0 ? y : lev[lc].t,
+                                           lc == 0 ? nullptr : lpart + lofs[lc], lnp[lc], cout);
+        CFD_LAUNCH_CHECK();
+        for (int l = lc - 1; l >= 0; l--) {
+            OpLevel<T> &L = lev[l];
+            OpLevel<T> &C = lev[l + 1];
+            UpArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.b, L.x, C.n0, C.n1, l + 1 == lc ? C.x : C.t,
+                        l == 0 ? z : L.t, w};
+            const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY);
+            if (l == 0) cx.prof_b(cx.owner, PK_MG_UP);
+            switch (post) {
+            case 0: k_mg_up<T, 0><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            case 1: k_mg_up<T, 1><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            default: k_mg_up<T, 2><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            }
+            if (l == 0) cx.prof_e(cx.owner);
+            CFD_LAUNCH_CHECK();
+        }
         return CFD_OK;
     }
 
-    // Capturable solve: exactly `maxit` unrolled iterations whose kernels turn into
-    // no-ops once the device-side tests (the same ones as solve()) have fired; no
-    // host synchronisation.  Result: iterations in S_IT, relative residual in
-    // S_RELRES.  rtol comes from `rtol` or, if rtol_slot >= 0, from scal[rtol_slot];
-    // `outer` (may be null) skips the whole solve when nonzero.
-    int solve_graph(PsCtx &cx, const T *b, T *x, double rtol, int rtol_slot, int maxit, const double *outer) {
+    // L x = b to ||r|| <= rtol ||b|| (rtol from scal[rtol_slot] when rtol_slot >= 0),
+    // x cleared first (acc += x before clearing when acc != null); skipped when
+    // *outer != 0.  Iteration for iteration the oracle's bicgstab().  graph: no
+    // host synchronisation, `maxit` iterations launched; otherwise the host reads
+    // the device flags after each half-iteration and stops launching once done.
+    // The iteration count and relative residual are left in S_IT / S_RELRES.
+    int solve(PsCtx &cx, const double *b, T *x, double *acc, double rtol, int rtol_slot, int maxit,
+              const double *outer, bool graph) {
         OpLevel<T> &L0 = lev[0];
         const int n0 = L0.n0, n1 = L0.n1;
         const dim3 gr = grid2d(n0, n1), bl(BX, BY);
         const int nb = nblk(n0, n1);
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
+        const RedArgs ra = cx.ra();
         const double *dn = cx.scal + S_DONE;
-        CFD_RET(start(cx, b, x, outer));
-        k_g_init<<<1, 1, 0, cx.s>>>(cx.scal, rtol, rtol_slot, outer);
+        k_bi_start1<T><<<gr, bl, 0, cx.s>>>(outer, n0, n1, b, x, acc, r, ra);
+        k_bi_fin<FIN_START1><<<1, PS_NT, 0, cx.s>>>(outer, ra, nb, 0, 0.0, -1);
+        k_bi_start2<T><<<gr, bl, 0, cx.s>>>(outer, n0, n1, r, r0, ra);
+        k_bi_fin<FIN_START2><<<1, PS_NT, 0, cx.s>>>(outer, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
+        if (!graph) {
+            CFD_RET(ps_sync(cx));
+            if (cx.hscal[S_DONE] != 0.0) return CFD_OK;
+        }
         for (int it = 1; it <= maxit; it++) {
-            k_g_begin<<<1, 1, 0, cx.s>>>(cx.scal, it);
-            k_bicg_p<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, it == 1, cx.scal, r, vv, pv);
+            k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
             CFD_RET(precond(cx, pv, ph, dn));
-            cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(dn, a0, ph, vv);
+            cx.prof_b(cx.owner, PK_APPLY);
+            k_bi_v<T><<<gr, bl, 0, cx.s>>>(a0, ph, vv, r0, ra);
             cx.prof_e(cx.owner);
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, r0, vv, nullptr, nullptr, cx.part, cx.pstride);
+            k_bi_fin<FIN_V><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
+            k_bi_fin<FIN_S><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_RV, dn));
-            k_bicg_s<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, cx.scal, r, vv, s, cx.part);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_SS, dn));
-            k_g_check_s<<<1, 1, 0, cx.s>>>(cx.scal);
-            k_bicg_x1g<T><<<gr, bl, 0, cx.s>>>(n0, n1, cx.scal, ph, x);
-            CFD_LAUNCH_CHECK();
+            if (!graph) {
+                CFD_RET(ps_sync(cx));
+                if (cx.hscal[S_DONE] != 0.0) {
+                    if (cx.hscal[S_SCONV] == (double)it)
+                        k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
+                    CFD_LAUNCH_CHECK();
+                    return CFD_OK;
+                }
+            }
             CFD_RET(precond(cx, s, sh, dn));
-            cx.prof_b(cx.owner, 0);
-            k_fapply<T><<<gr, bl, 0, cx.s>>>(dn, a0, sh, t);
+            cx.prof_b(cx.owner, PK_APPLY);
+            k_bi_t<T><<<gr, bl, 0, cx.s>>>(a0, sh, t, s, ra);
             cx.prof_e(cx.owner);
-            k_dot2<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, t, s, t, t, cx.part, cx.pstride);
+            k_bi_fin<FIN_T><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
+            k_bi_fin<FIN_XR><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 2, S_TS, dn));
-            k_set_omega<<<1, 1, 0, cx.s>>>(dn, cx.scal);
-            k_bicg_xr<T><<<gr, bl, 0, cx.s>>>(dn, n0, n1, cx.scal, ph, sh, s, t, r0, x, r, cx.part, cx.pstride);
-            CFD_LAUNCH_CHECK();
-            CFD_RET(ps_reduce(cx, nb, 1, S_RR, dn));
-            k_reduce_sum<<<1, 256, 0, cx.s>>>(dn, cx.part + cx.pstride, cx.pstride, nb, 1, cx.scal, S_R0R);
-            k_g_check_r<<<1, 1, 0, cx.s>>>(cx.scal);
-            CFD_LAUNCH_CHECK();
+            if (!graph) {
+                CFD_RET(ps_sync(cx));
+                if (cx.hscal[S_DONE] != 0.0) return CFD_OK;
+            }
         }
-        k_g_final<<<1, 1, 0, cx.s>>>(cx.scal);
-        CFD_LAUNCH_CHECK();
         return CFD_OK;
     }
 };
