@@ -3,7 +3,8 @@ mixed-convection flow, with the operators of mhsuhail00/CFD_Solver.
 
   cfd_solver_amd.grid      grids, metrics, INP.DAT format
   cfd_solver_amd.stages    RefStages: the reference's stage functions on the GPU
-  cfd_solver_amd.cavity    CavityCase: device-resident steady cavity solves
+  cfd_solver_amd.cavity    CavityCase / CavityBatch: device-resident steady cavity solves,
+                           one case or many equally sized cases per launch
   cfd_solver_amd.build     compiles libcfd_gpu.so for gfx950
 
 The HIP library (libcfd_gpu.so) is required; nothing falls back to the CPU.
@@ -17,7 +18,7 @@ def __getattr__(name):
     if name in ("RefStages",):
         from .stages import RefStages
         return RefStages
-    if name in ("CavityCase", "default_params"):
+    if name in ("CavityCase", "CavityBatch", "default_params"):
         from . import cavity
         return getattr(cavity, name)
     raise AttributeError(name)

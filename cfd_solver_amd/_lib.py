@@ -77,6 +77,16 @@ SIGNATURES = {
     "cfd_case_time_kernel": ([_p, _i, _i, C.POINTER(_d)], _i),
     "cfd_case_profile": ([_p, _i], _i),
     "cfd_case_profile_read": ([_p, C.POINTER(_d), C.POINTER(C.c_long), _i], _i),
+    "cfd_batch_create": ([_i, _i, _d, _d, _i, C.POINTER(Params), C.POINTER(_p)], _i),
+    "cfd_batch_destroy": ([_p], _i),
+    "cfd_batch_set_case": ([_p, _i] + [dp] * 10 + [_p, _p, C.POINTER(Params)], _i),
+    "cfd_batch_release": ([_p, _i], _i),
+    "cfd_batch_outer": ([_p, _i, _p], _i),
+    "cfd_batch_launch": ([_p], _i),
+    "cfd_batch_finish": ([_p, _p], _i),
+    "cfd_batch_get_fields": ([_p, _i, dp, dp, dp, dp], _i),
+    "cfd_batch_set_fields": ([_p, _i, dp, dp, dp, dp], _i),
+    "cfd_batch_bytes": ([_p, C.POINTER(C.c_size_t)], _i),
 }
 
 _LIB = None

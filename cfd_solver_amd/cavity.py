@@ -112,3 +112,78 @@ class CavityCase:
         ms = C.c_double()
         call("cfd_case_time_kernel", self._h, int(kernel), int(reps), C.byref(ms))
         return ms.value
+
+
+def _slot_params(base: Params, **phys) -> Params:
+    p = Params()
+    C.memmove(C.byref(p), C.byref(base), C.sizeof(Params))
+    for k, v in phys.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class CavityBatch:
+    """`slots` cavity cases of one grid shape advanced together on one stream.
+
+    Batch-wide settings (predictor sweeps, p' solver, multigrid, precision,
+    graph) come from `params`; each slot's physics from the keywords of
+    `set_case` (Re, Ri, Pr, dt, alpha_u, alpha_p, alpha_t, lid_u, t_hot, t_cold).
+    """
+
+    def __init__(self, n0: int, n1: int, dxi: float, deta: float, slots: int, params: Params | None = None, **kw):
+        self.n0, self.n1, self.slots = n0, n1, slots
+        self.params = params if params is not None else default_params(**kw)
+        self._h = C.c_void_p()
+        call("cfd_batch_create", n0, n1, dxi, deta, slots, C.byref(self.params), C.byref(self._h))
+        self._stats = (Stats * slots)()
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.load().cfd_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_case(self, slot: int, grid: Grid, **phys):
+        if (grid.n0, grid.n1) != (self.n0, self.n1):
+            raise ValueError(f"grid {grid.n0}x{grid.n1} does not fit a {self.n0}x{self.n1} batch")
+        g = grid
+        arrs = [f64(a) for a in (g.x, g.y, g.xix, g.xiy, g.etx, g.ety, g.alph, g.beta, g.gamma, g.ajac,
+                                 g.p1, g.q1)]
+        sp = _slot_params(self.params, **phys)
+        call("cfd_batch_set_case", self._h, int(slot), *arrs[:10], arrs[10].ctypes.data, arrs[11].ctypes.data,
+             C.byref(sp))
+
+    def release(self, slot: int):
+        call("cfd_batch_release", self._h, int(slot))
+
+    def step(self, n: int = 1):
+        """n outer iterations of every active slot; returns the per-slot Stats array."""
+        call("cfd_batch_outer", self._h, int(n), C.cast(self._stats, C.c_void_p))
+        return self._stats
+
+    def launch(self):
+        call("cfd_batch_launch", self._h)
+
+    def finish(self):
+        call("cfd_batch_finish", self._h, C.cast(self._stats, C.c_void_p))
+        return self._stats
+
+    def fields(self, slot: int):
+        out = [np.zeros((self.n1, self.n0)) for _ in range(4)]
+        call("cfd_batch_get_fields", self._h, int(slot), *out)
+        return dict(u=out[0], v=out[1], T=out[2], p=out[3])
+
+    def set_fields(self, slot: int, u, v, T, p):
+        call("cfd_batch_set_fields", self._h, int(slot), f64(u), f64(v), f64(T), f64(p))
+
+    def nbytes(self):
+        b = C.c_size_t()
+        call("cfd_batch_bytes", self._h, C.byref(b))
+        return b.value

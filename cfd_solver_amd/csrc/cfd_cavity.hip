@@ -1,8 +1,14 @@
-// Cavity SIMPLE path on gfx950: device-resident case, per-stage kernels,
-// BiCGStab + geometric multigrid for the pressure correction, and the outer
-// iteration driver behind the cfd_case_* C ABI.
+// Cavity SIMPLE path on gfx950: device-resident batches of equally sized cases,
+// per-stage kernels, BiCGStab + geometric multigrid for the pressure correction,
+// and the outer-iteration driver behind the cfd_batch_* / cfd_case_* C ABI.
 //
-// One outer iteration (all on the case's stream):
+// A batch holds `slots` cases of n0 x n1 nodes.  Every kernel runs blockIdx.z
+// over the slots: slot z's arrays start at z x n0 x n1, its physics (Re, Ri, Pr,
+// relaxation, wall values) is SlotP z, its device scalars start at z x SS.  A
+// slot that holds no case (SlotP::active = 0) costs one early return per block.
+// A single case (cfd_case_*) is a batch of one slot.
+//
+// One outer iteration (all on the batch's stream):
 //   k_residual      reference RHS (hybrid convection, grad p, buoyancy) minus the
 //                   reference implicit operator applied to the current fields
 //                   -> R_u, R_v, R_T, the d factors, U, V      (solver.cpp:1062-1236, 751-848)
@@ -27,15 +33,38 @@ constexpr int CFD_PROF_KINDS = 8;
 constexpr int PROF_EVENTS = 4096;
 constexpr int NT = BX * BY;  // threads per 2-D block
 
+// per-slot physics and wall data (device array, one entry per slot)
+struct SlotP {
+    double Re, Ri, Pr, dt, alpha_u, alpha_t, alpha_p;
+    double lid, t_hot, t_cold, len_hot, len_cold;
+    double active;  // 1: the slot holds a case
+};
+
+// grid shape shared by the slots, and the slots' physics
+struct CGrid {
+    int n0, n1;
+    double h1, h2;
+    size_t N;          // nodes per slot
+    const SlotP *sp;   // sp[blockIdx.z]
+    __host__ __device__ KGrid kg() const { return KGrid{n0, n1, h1, h2}; }
+};
+
 struct Case {
     int n0 = 0, n1 = 0;
-    size_t N = 0;
+    int B = 1;           // slots
+    size_t N = 0;        // nodes per slot
     double h1 = 0, h2 = 0;
     cfd_params prm{};
     bool mixed = false;
     hipStream_t s = nullptr;
     int device = 0;
-    // grid
+    // per-slot physics (device copy, host copy) and host-side per-slot state
+    SlotP *spd = nullptr;
+    std::vector<SlotP> sph;
+    std::vector<double> mass0, energy0;
+    std::vector<int> iter;
+    size_t scr = 0;      // scratch per slot
+    // grid (every array: slots x N)
     double *xg, *yg, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *ajac, *p1, *q1;
     // state and FP64 work
     double *u, *v, *T, *p;
@@ -53,13 +82,10 @@ struct Case {
     float *ef = nullptr;                   // mixed: FP32 correction of one refinement step
     double *r64 = nullptr;                 // mixed: FP64 defect
     double *f64[4] = {nullptr};            // mixed: level-0 face coefficients in FP64 (defect)
-    double *part = nullptr;  // reduction partials: 4 x pstride
+    double *part = nullptr;  // reduction partials: per slot 4 x pstride
     int pstride = 0;
-    double *scal = nullptr;  // device scalars
+    double *scal = nullptr;  // device scalars, SS per slot
     double *hscal = nullptr; // pinned host mirror
-    double wall_len_hot = 0, wall_len_cold = 0;
-    double mass0 = 0, energy0 = 0;
-    int iter = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
     // live kernel timing: event pairs around fine-level launches of the hot kernels
@@ -81,19 +107,24 @@ struct Case {
 // ------------------------------------------------------------------------
 // residual + d factors (oracle: or_cav_residuals)
 
-struct Phys {
-    double Re, Ri, Pr, dt, alpha_u, alpha_t, alpha_p;
+// the grid arrays of one case, offset to a slot
+struct GridPtrs {
+    const double *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *p1, *q1;
 };
 
 template <typename TR>
 __global__ __launch_bounds__(NT) void k_residual(
-    KGrid g, Phys ph, const double *__restrict__ u, const double *__restrict__ v,
-    const double *__restrict__ T, const double *__restrict__ p, const double *__restrict__ xix,
-    const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
-    const double *__restrict__ alph, const double *__restrict__ beta, const double *__restrict__ gamma,
-    const double *__restrict__ p1, const double *__restrict__ q1, TR *__restrict__ Ru,
-    TR *__restrict__ Rv, TR *__restrict__ RT, double *__restrict__ dd, TR *__restrict__ Ua,
-    TR *__restrict__ Va, RedArgs ra) {
+    CGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ T,
+    const double *__restrict__ p, GridPtrs gp, TR *__restrict__ Ru, TR *__restrict__ Rv, TR *__restrict__ RT,
+    double *__restrict__ dd, TR *__restrict__ Ua, TR *__restrict__ Va, RedArgs ra) {
+    const SlotP ph = g.sp[blockIdx.z];
+    if (ph.active == 0.0) return;
+    ra = rz(ra);
+    const size_t o = zslot() * g.N;
+    u += o; v += o; T += o; p += o; Ru += o; Rv += o; RT += o; dd += o; Ua += o; Va += o;
+    const double *__restrict__ xix = gp.xix + o, *__restrict__ xiy = gp.xiy + o, *__restrict__ etx = gp.etx + o,
+                 *__restrict__ ety = gp.ety + o, *__restrict__ alph = gp.alph + o, *__restrict__ beta = gp.beta + o,
+                 *__restrict__ gamma = gp.gamma + o, *__restrict__ p1 = gp.p1 + o, *__restrict__ q1 = gp.q1 + o;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     const double h1 = g.h1, h2 = g.h2;
@@ -148,6 +179,7 @@ __global__ __launch_bounds__(NT) void k_residual(
 
 // d on the border = nearest interior value (corners from the diagonal)
 __global__ void k_d_border(int n0, int n1, double *__restrict__ d) {
+    d += zslot() * n0 * n1;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n0) {
         const int i = t;
@@ -168,11 +200,18 @@ __global__ void k_d_border(int n0, int n1, double *__restrict__ d) {
 // in the predictor precision TR
 template <typename TR, bool FIRST>
 __global__ __launch_bounds__(NT) void k_jacobi(
-    KGrid g, Phys ph, const TR *__restrict__ Ru, const TR *__restrict__ Rv, const TR *__restrict__ RT,
+    CGrid g, const TR *__restrict__ Ru, const TR *__restrict__ Rv, const TR *__restrict__ RT,
     const TR *__restrict__ Ua, const TR *__restrict__ Va, const TR *__restrict__ alph, const TR *__restrict__ beta,
     const TR *__restrict__ gamma, const TR *__restrict__ p1, const TR *__restrict__ q1, const TR *__restrict__ du,
     const TR *__restrict__ dv, const TR *__restrict__ dT, TR *__restrict__ nu, TR *__restrict__ nv,
     TR *__restrict__ nT) {
+    const SlotP ph = g.sp[blockIdx.z];
+    if (ph.active == 0.0) return;
+    {
+        const size_t o = zslot() * g.N;
+        Ru += o; Rv += o; RT += o; Ua += o; Va += o; alph += o; beta += o; gamma += o; p1 += o; q1 += o;
+        du += o; dv += o; dT += o; nu += o; nv += o; nT += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
@@ -204,11 +243,17 @@ __global__ __launch_bounds__(NT) void k_jacobi(
 // Also completes d on the border (written there by no other kernel of the iteration).
 template <typename TR>
 __global__ __launch_bounds__(NT) void k_star(
-    KGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
+    CGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
     const TR *__restrict__ dv, const TR *__restrict__ dT, double *__restrict__ T,
     const double *__restrict__ p, double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     double *__restrict__ us, double *__restrict__ vs, double *__restrict__ uh, double *__restrict__ vh) {
+    if (g.sp[blockIdx.z].active == 0.0) return;
+    {
+        const size_t o = zslot() * g.N;
+        u += o; v += o; du += o; dv += o; dT += o; T += o; p += o; d += o; xix += o; xiy += o; etx += o;
+        ety += o; us += o; vs += o; uh += o; vh += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     if (i >= n0 || j >= n1) return;
@@ -232,10 +277,16 @@ __global__ __launch_bounds__(NT) void k_star(
 
 // q = J-weighted divergence of the face velocities (oracle: face_div)
 __global__ __launch_bounds__(NT) void k_face_div(
-    KGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
+    CGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
     const double *__restrict__ d, const double *__restrict__ xix, const double *__restrict__ xiy,
     const double *__restrict__ etx, const double *__restrict__ ety, const double *__restrict__ jac,
     double *__restrict__ q, RedArgs ra) {
+    if (g.sp[blockIdx.z].active == 0.0) return;
+    ra = rz(ra);
+    {
+        const size_t o = zslot() * g.N;
+        uh += o; vh += o; p += o; d += o; xix += o; xiy += o; etx += o; ety += o; jac += o; q += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     const double h1 = g.h1, h2 = g.h2;
@@ -288,10 +339,18 @@ __global__ __launch_bounds__(NT) void k_face_div(
 // u = u* - d grad p' (p' read clamped = wall copy), p += alpha_p p', with p' taken
 // with its interior mean (scal[S_PSUM] / interior count) removed
 __global__ __launch_bounds__(NT) void k_correct(
-    KGrid g, double alpha_p, const double *__restrict__ us, const double *__restrict__ vs,
+    CGrid g, const double *__restrict__ us, const double *__restrict__ vs,
     const double *__restrict__ pc, const double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     const double *__restrict__ scal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const SlotP &sp = g.sp[blockIdx.z];
+    if (sp.active == 0.0) return;
+    const double alpha_p = sp.alpha_p;
+    scal += zslot() * SS;
+    {
+        const size_t o = zslot() * g.N;
+        us += o; vs += o; pc += o; d += o; xix += o; xiy += o; etx += o; ety += o; u += o; v += o; p += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
@@ -309,8 +368,15 @@ __global__ __launch_bounds__(NT) void k_correct(
 // adiabatic side walls (oracle: temperature_sides), one workgroup: every new value
 // is computed from the old wall column (into scratch) before any is written
 constexpr int TS_NT = 1024;
-__global__ __launch_bounds__(TS_NT) void k_t_sides(KGrid g, double *__restrict__ T, const double *__restrict__ alph,
-                                                   const double *__restrict__ beta, double *__restrict__ scratch) {
+__global__ __launch_bounds__(TS_NT) void k_t_sides(CGrid g, double *__restrict__ T, const double *__restrict__ alph,
+                                                   const double *__restrict__ beta, double *__restrict__ scratch,
+                                                   size_t scr) {
+    if (g.sp[blockIdx.z].active == 0.0) return;
+    {
+        const size_t o = zslot() * g.N;
+        T += o; alph += o; beta += o;
+        scratch += zslot() * scr;
+    }
     const int n0 = g.n0, n1 = g.n1;
     for (int j = 1 + (int)threadIdx.x; j <= n1 - 2; j += TS_NT) {
         size_t P = idx(0, j, n0);
@@ -333,6 +399,11 @@ __global__ __launch_bounds__(TS_NT) void k_t_sides(KGrid g, double *__restrict__
 struct WallIn {
     const double *u, *v, *T, *p, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *p1, *q1, *ajac;
 };
+__device__ __forceinline__ WallIn wall_at(WallIn w, size_t o) {
+    w.u += o; w.v += o; w.T += o; w.p += o; w.xix += o; w.xiy += o; w.etx += o; w.ety += o;
+    w.alph += o; w.beta += o; w.gamma += o; w.p1 += o; w.q1 += o; w.ajac += o;
+    return w;
+}
 
 __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j) {
     const int n0 = g.n0;
@@ -390,7 +461,13 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
 // Wall pressure written in place: wall_p_value reads p only at interior nodes, so
 // no thread reads a value another one writes.  A corner is the mean of the two
 // adjacent wall values (recomputed by the thread that writes it).
-__global__ void k_p_walls(KGrid g, WallIn w, double Re, double Ri, double *__restrict__ p) {
+__global__ void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p) {
+    const SlotP &sp = cg.sp[blockIdx.z];
+    if (sp.active == 0.0) return;
+    const double Re = sp.Re, Ri = sp.Ri;
+    const KGrid g = cg.kg();
+    w = wall_at(w, zslot() * cg.N);
+    p += zslot() * cg.N;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n0 = g.n0, n1 = g.n1;
     if (t >= 1 && t <= n0 - 2) {
@@ -413,8 +490,14 @@ __global__ void k_p_walls(KGrid g, WallIn w, double Re, double Ri, double *__res
 }
 
 // velocity and temperature wall values (oracle: velocity_temperature_bcs)
-__global__ void k_uvt_walls(KGrid g, double lid, double th, double tc, double *__restrict__ u,
-                            double *__restrict__ v, double *__restrict__ T) {
+__global__ void k_uvt_walls(CGrid g, double *__restrict__ u, double *__restrict__ v, double *__restrict__ T) {
+    const SlotP &sp = g.sp[blockIdx.z];
+    if (sp.active == 0.0) return;
+    const double lid = sp.lid, th = sp.t_hot, tc = sp.t_cold;
+    {
+        const size_t o = zslot() * g.N;
+        u += o; v += o; T += o;
+    }
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int n0 = g.n0, n1 = g.n1;
     if (t < n0) {
@@ -453,10 +536,17 @@ __device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const doub
 }
 
 // signed max of the nodal dilation (solver.cpp:1821-1850) -> block partials
-__global__ __launch_bounds__(NT) void k_dmax(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
+__global__ __launch_bounds__(NT) void k_dmax(CGrid g, const double *__restrict__ u, const double *__restrict__ v,
                                              const double *__restrict__ xix, const double *__restrict__ xiy,
                                              const double *__restrict__ etx, const double *__restrict__ ety,
-                                             double *__restrict__ part) {
+                                             RedArgs ra) {
+    if (g.sp[blockIdx.z].active == 0.0) return;
+    ra = rz(ra);
+    double *part = ra.part;
+    {
+        const size_t o = zslot() * g.N;
+        u += o; v += o; xix += o; xiy += o; etx += o; ety += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
     double dm = 0.0;
@@ -473,11 +563,20 @@ __global__ __launch_bounds__(NT) void k_dmax(KGrid g, const double *__restrict__
 
 // one block: dmax from the partials, floored at 0 (solver.cpp:1806), and both
 // wall Nusselt numbers
-__global__ __launch_bounds__(NT) void k_dmax_nu_fin(KGrid g, const double *__restrict__ part, int nparts,
-                                                    const double *__restrict__ T, const double *__restrict__ ajac,
-                                                    const double *__restrict__ etx, const double *__restrict__ ety,
-                                                    double len_hot, double len_cold, double dT,
-                                                    double *__restrict__ scal) {
+__global__ __launch_bounds__(NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, const double *__restrict__ T,
+                                                    const double *__restrict__ ajac, const double *__restrict__ etx,
+                                                    const double *__restrict__ ety) {
+    const SlotP &sp = cg.sp[blockIdx.z];
+    if (sp.active == 0.0) return;
+    ra = rz(ra);
+    const double *part = ra.part;
+    double *scal = ra.scal;
+    const double len_hot = sp.len_hot, len_cold = sp.len_cold, dT = sp.t_hot - sp.t_cold;
+    const KGrid g = cg.kg();
+    {
+        const size_t o = zslot() * cg.N;
+        T += o; ajac += o; etx += o; ety += o;
+    }
     double m = -1.0e300;
     for (int t = tlin(); t < nparts; t += NT) m = fmax(m, part[t]);
     m = block_max<NT>(m);
@@ -518,6 +617,12 @@ __device__ __forceinline__ double mrow(const MArgs &a, const double *__restrict_
 // y = L x, or y = b - L x when b != nullptr; 0 on the border
 __global__ __launch_bounds__(NT) void k_mapply(MArgs a, const double *__restrict__ x,
                                                const double *__restrict__ b, double *__restrict__ y) {
+    {
+        const size_t o = zslot() * a.n0 * a.n1;
+        a.xix += o; a.xiy += o; a.etx += o; a.ety += o; a.jac += o; a.d += o;
+        x += o; y += o;
+        if (b) b += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     const size_t P = idx(i, j, a.n0);
@@ -533,6 +638,12 @@ __global__ __launch_bounds__(NT) void k_mapply(MArgs a, const double *__restrict
 // refinement's start: tolerance, done flag, first inner tolerance
 __global__ __launch_bounds__(NT) void k_q_norms(int n0, int n1, const double *__restrict__ q, double *__restrict__ pc,
                                                 RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, -1)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        q += o; pc += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < n0 && j < n1) {
@@ -548,10 +659,15 @@ __global__ __launch_bounds__(NT) void k_q_norms(int n0, int n1, const double *__
 
 // FP64 defect r = q - L (p' + e) (faces stored in FP64), its norm with the mean
 // removed, the refinement test and the next inner tolerance
-__global__ __launch_bounds__(NT) void k_defect(const double *rdone, FArgs<double> a, const double *__restrict__ q,
+__global__ __launch_bounds__(NT) void k_defect(FArgs<double> a, const double *__restrict__ q,
                                                const double *__restrict__ pc, const float *__restrict__ ef,
                                                double *__restrict__ r, RedArgs ra) {
-    if (*rdone != 0.0) return;
+    ra = rz(ra);
+    if (slot_skip(ra.scal, S_RDONE)) return;
+    {
+        const size_t o = zslot() * a.n0 * a.n1;
+        a = fz(a, o); q += o; pc += o; ef += o; r += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < a.n0 && j < a.n1) {
@@ -574,6 +690,13 @@ __global__ __launch_bounds__(NT) void k_defect(const double *rdone, FArgs<double
 template <bool ACC>
 __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restrict__ pc, float *__restrict__ ef,
                                                  RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, -1)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        pc += o;
+        if (ACC) ef += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v[1] = {0.0};
     if (i < n0 && j < n1) {
@@ -592,9 +715,9 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
 // totals of the case-level reductions and what follows from them (one block)
 enum { CF_RES, CF_MASS, CF_Q, CF_DEFECT, CF_PSUM };
 template <int OP>
-__global__ __launch_bounds__(NT) void k_cav_fin(const double *guard, RedArgs ra, int nparts, int n0, int n1,
-                                                double rtol) {
-    if (guard && *guard != 0.0) return;
+__global__ __launch_bounds__(NT) void k_cav_fin(int gs, RedArgs ra, int nparts, int n0, int n1, double rtol) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, gs)) return;
     constexpr int NV = (OP == CF_MASS || OP == CF_PSUM) ? 1 : 2;
     double tot[NV];
     sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
@@ -658,7 +781,7 @@ int dalloc(Case *c, T **p, size_t n) {
 
 // ---- live kernel timing
 // kinds: 0 p' operator apply (fine level), 1 predictor Jacobi sweep,
-// 2 residual/RHS, 3 face divergence, 4 multigrid smoothing sweep (fine level)
+// 2 residual/RHS, 3 face divergence, 4 multigrid down (fine level), 5 multigrid up
 void prof_begin(Case *c, int kind) {
     if (!c->prof || c->capturing) return;
     if (c->ev_kind.size() * 2 > (size_t)c->ev_used) c->ev_kind.pop_back();  // drop a dangling begin
@@ -690,16 +813,23 @@ void cb_begin(void *o, int kind) { prof_begin(static_cast<Case *>(o), kind); }
 void cb_end(void *o) { prof_end(static_cast<Case *>(o)); }
 void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 
+size_t psz(const Case *c) { return (size_t)4 * c->pstride; }
 PsCtx ctx(Case *c) {
-    return PsCtx{c->s, c->part, c->pstride, c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
+    return PsCtx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
                  &c->allocs, &c->bytes};
 }
-RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, c->scal}; }
+RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, psz(c), c->scal}; }
+CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd}; }
+dim3 grid_b(const Case *c) {
+    dim3 g = grid2d(c->n0, c->n1);
+    g.z = c->B;
+    return g;
+}
 
-// one-block finish of a case-level reduction over the n0 x n1 grid's block partials
+// one-block-per-slot finish of a case-level reduction over the grid's block partials
 template <int OP>
-int cav_fin(Case *c, const double *guard = nullptr, double rtol = 0.0) {
-    k_cav_fin<OP><<<1, NT, 0, c->s>>>(guard, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol);
+int cav_fin(Case *c, int gs = -1, double rtol = 0.0) {
+    k_cav_fin<OP><<<dim3(1, 1, c->B), NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -713,7 +843,7 @@ MArgs margs(const Case *c) {
     return MArgs{c->n0, c->n1, c->h1, c->h2, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->d};
 }
 
-// coarse geometry (metrics injected once) and the solver of the case's precision
+// coarse-level geometry arrays and the solver of the batch's precision
 int build_levels(Case *c) {
     const int want = c->prm.mg_levels < 1 ? 1 : c->prm.mg_levels;
     Geom &G0 = c->geom[0];
@@ -726,16 +856,9 @@ int build_levels(Case *c) {
         Geom &L = c->geom[l];
         L.n0 = (F.n0 + 1) / 2; L.n1 = (F.n1 + 1) / 2;
         L.h1 = 2.0 * F.h1; L.h2 = 2.0 * F.h2;
-        const size_t M = (size_t)L.n0 * L.n1;
+        const size_t M = (size_t)c->B * L.n0 * L.n1;
         RET(dalloc(c, &L.xix, M)); RET(dalloc(c, &L.xiy, M)); RET(dalloc(c, &L.etx, M));
         RET(dalloc(c, &L.ety, M)); RET(dalloc(c, &L.jac, M));
-        const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
-        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xix, L.n0, L.n1, L.xix);
-        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.xiy, L.n0, L.n1, L.xiy);
-        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.etx, L.n0, L.n1, L.etx);
-        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.ety, L.n0, L.n1, L.ety);
-        k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.jac, L.n0, L.n1, L.jac);
-        CFD_LAUNCH_CHECK();
         l++;
     }
     c->nlev = l;
@@ -746,6 +869,22 @@ int build_levels(Case *c) {
     } else {
         c->ps64 = new PSolver<double>();
         RET(c->ps64->init(cx, c->geom, c->nlev, c->prm));
+    }
+    return CFD_OK;
+}
+
+// coarse metrics of slot z by repeated injection
+int inject_slot(Case *c, int z) {
+    const dim3 bl(BX, BY);
+    for (int l = 1; l < c->nlev; l++) {
+        const Geom &F = c->geom[l - 1];
+        const Geom &L = c->geom[l];
+        const size_t of = (size_t)z * F.n0 * F.n1, oc = (size_t)z * L.n0 * L.n1;
+        const dim3 gr = grid2d(L.n0, L.n1);
+        const double *src[5] = {F.xix, F.xiy, F.etx, F.ety, F.jac};
+        double *dst[5] = {L.xix, L.xiy, L.etx, L.ety, L.jac};
+        for (int k = 0; k < 5; k++) k_inject<<<gr, bl, 0, c->s>>>(F.n0, F.n1, src[k] + of, L.n0, L.n1, dst[k] + oc);
+        CFD_LAUNCH_CHECK();
     }
     return CFD_OK;
 }
@@ -766,44 +905,47 @@ int mg_setup(Case *c) {
 int psolve(Case *c, double rtol, int maxit, bool graph) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
-    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const dim3 gr = grid_b(c), bl(BX, BY);
     if (!c->mixed) {
-        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, nullptr, graph));
+        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph));
         k_pc_final<false><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
         return cav_fin<CF_PSUM>(c);
     }
     const int R = rtol >= 1e-4 ? 1 : 6;
-    const double *rd = c->scal + S_RDONE;
     const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
     k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
     CFD_LAUNCH_CHECK();
-    RET(cav_fin<CF_Q>(c, nullptr, rtol));
+    RET(cav_fin<CF_Q>(c, -1, rtol));
     for (int k = 0; k < R; k++) {
         if (!graph && k > 0) {
             RET(ps_sync(cx));
-            if (c->hscal[S_RDONE] != 0.0) break;
+            bool all = true;
+            for (int z = 0; z < c->B; z++) {
+                const double *h = c->hscal + (size_t)z * SS;
+                if (h[S_ACTIVE] != 0.0 && h[S_RDONE] == 0.0) all = false;
+            }
+            if (all) break;
         }
-        RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, rd,
+        RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, S_RDONE,
                            graph));
-        k_defect<<<gr, bl, 0, c->s>>>(rd, a64, c->q, c->pc, c->ef, c->r64, red(c));
+        k_defect<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, c->r64, red(c));
         CFD_LAUNCH_CHECK();
-        RET(cav_fin<CF_DEFECT>(c, rd));
+        RET(cav_fin<CF_DEFECT>(c, S_RDONE));
     }
     k_pc_final<true><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, c->ef, red(c));
     CFD_LAUNCH_CHECK();
     return cav_fin<CF_PSUM>(c);
 }
 
-void psolve_result(const Case *c, int *its, double *relres) {
-    const double *h = c->hscal;
+void psolve_result(const Case *c, int z, int *its, double *relres) {
+    const double *h = c->hscal + (size_t)z * SS;
     *its = c->mixed ? (int)h[S_ITTOT] : (int)h[S_IT];
     *relres = c->mixed ? h[S_RELRES64] : h[S_RELRES];
 }
 
-KGrid kg(const Case *c) { return KGrid{c->n0, c->n1, c->h1, c->h2}; }
-Phys phys(const Case *c) {
-    return Phys{c->prm.Re, c->prm.Ri, c->prm.Pr, c->prm.dt, c->prm.alpha_u, c->prm.alpha_t, c->prm.alpha_p};
+GridPtrs gptrs(const Case *c) {
+    return GridPtrs{c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->p1, c->q1};
 }
 WallIn wall_in(const Case *c) {
     return WallIn{c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
@@ -813,9 +955,9 @@ WallIn wall_in(const Case *c) {
 template <typename TR>
 int launch_residual_t(Case *c) {
     prof_begin(c, 2);
-    k_residual<TR><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(
-        kg(c), phys(c), c->u, c->v, c->T, c->p, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma,
-        c->p1, c->q1, (TR *)c->Ru, (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c));
+    k_residual<TR><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), (TR *)c->Ru,
+                                                          (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V,
+                                                          red(c));
     prof_end(c);
     CFD_LAUNCH_CHECK();
     return cav_fin<CF_RES>(c);
@@ -836,51 +978,55 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
     }
     const TR *Ru = (const TR *)c->Ru, *Rv = (const TR *)c->Rv, *RT = (const TR *)c->RT;
     const TR *U = (const TR *)c->U, *V = (const TR *)c->V;
-    const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
+    const dim3 gr = grid_b(c), bl(BX, BY);
     for (int s = 0; s < c->prm.mom_sweeps; s++) {
         if (s == 0) {
-            k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(kg(c), phys(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4],
-                                                    a[0], a[1], a[2], b[0], b[1], b[2]);
+            k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
+                                                    a[1], a[2], b[0], b[1], b[2]);
         } else {
             prof_begin(c, 1);
-            k_jacobi<TR, false><<<gr, bl, 0, c->s>>>(kg(c), phys(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3],
-                                                     co[4], a[0], a[1], a[2], b[0], b[1], b[2]);
+            k_jacobi<TR, false><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
+                                                     a[1], a[2], b[0], b[1], b[2]);
             prof_end(c);
         }
         CFD_LAUNCH_CHECK();
         for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
     }
     if (c->prm.mom_sweeps <= 0) {
-        const unsigned g = (unsigned)((c->N + 255) / 256);
-        for (int k = 0; k < 3; k++) k_fill<TR><<<g, 256, 0, c->s>>>(c->N, a[k], TR(0));
+        const size_t n = (size_t)c->B * c->N;
+        const unsigned g = (unsigned)((n + 255) / 256);
+        for (int k = 0; k < 3; k++) k_fill<TR><<<g, 256, 0, c->s>>>(n, a[k], TR(0));
         CFD_LAUNCH_CHECK();
     }
     *du = a[0]; *dv = a[1]; *dT = a[2];
     return CFD_OK;
 }
 
-// adiabatic sides, then the wall pressure (which reads the new wall T)
-int wall_bcs(Case *c) {
+// Wall conditions of the slots [z0, z0 + nz): velocity/temperature values
+// (uvt), adiabatic sides, then the wall pressure (which reads the new wall T).
+// Launched on pointers offset to slot z0, so other slots are untouched.
+int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     const int m = std::max(c->n0, c->n1);
-    k_t_sides<<<1, TS_NT, 0, c->s>>>(kg(c), c->T, c->alph, c->beta, c->scratch);
-    k_p_walls<<<(m + 255) / 256, 256, 0, c->s>>>(kg(c), wall_in(c), c->prm.Re, c->prm.Ri, c->p);
+    const size_t o = (size_t)z0 * c->N;
+    CGrid g = cg(c);
+    g.sp += z0;
+    WallIn w = wall_in(c);
+    w.u += o; w.v += o; w.T += o; w.p += o; w.xix += o; w.xiy += o; w.etx += o; w.ety += o;
+    w.alph += o; w.beta += o; w.gamma += o; w.p1 += o; w.q1 += o; w.ajac += o;
+    const dim3 g1((m + 255) / 256, 1, nz);
+    if (uvt) k_uvt_walls<<<g1, 256, 0, c->s>>>(g, c->u + o, c->v + o, c->T + o);
+    k_t_sides<<<dim3(1, 1, nz), TS_NT, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr,
+                                                   c->scr);
+    k_p_walls<<<g1, 256, 0, c->s>>>(g, w, c->p + o);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 
-int apply_bcs_dev(Case *c) {
-    const int m = std::max(c->n0, c->n1);
-    k_uvt_walls<<<(m + 255) / 256, 256, 0, c->s>>>(kg(c), c->prm.lid_u, c->prm.t_hot, c->prm.t_cold, c->u, c->v,
-                                                   c->T);
-    CFD_LAUNCH_CHECK();
-    return wall_bcs(c);
-}
-
 template <typename TR>
 int star_t(Case *c, void *du, void *dv, void *dT) {
-    k_star<TR><<<grid2d(c->n0, c->n1), dim3(BX, BY), 0, c->s>>>(kg(c), c->u, c->v, (TR *)du, (TR *)dv, (TR *)dT,
-                                                                 c->T, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
-                                                                 c->us, c->vs, c->uh, c->vh);
+    k_star<TR><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (TR *)du, (TR *)dv, (TR *)dT, c->T, c->p,
+                                                      c->d, c->xix, c->xiy, c->etx, c->ety, c->us, c->vs, c->uh,
+                                                      c->vh);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -889,8 +1035,7 @@ int star_t(Case *c, void *du, void *dv, void *dT) {
 // (the p' solve is controlled on the device); otherwise the p' solve reads its
 // convergence flags on the host between half-iterations.
 int outer_body(Case *c, bool graph) {
-    const int n0 = c->n0, n1 = c->n1;
-    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    const dim3 gr = grid_b(c), bl(BX, BY);
     // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
     // 3 predictor
@@ -901,7 +1046,7 @@ int outer_body(Case *c, bool graph) {
     if (c->mixed) RET(star_t<float>(c, du, dv, dT));
     else RET(star_t<double>(c, du, dv, dT));
     prof_begin(c, 3);
-    k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
+    k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
                                     c->q, red(c));
     prof_end(c);
     CFD_LAUNCH_CHECK();
@@ -910,14 +1055,14 @@ int outer_body(Case *c, bool graph) {
     RET(mg_setup(c));
     RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph));
     // 6-7 correction (p' mean removed) and BCs (FP64 accumulation)
-    k_correct<<<gr, bl, 0, c->s>>>(kg(c), c->prm.alpha_p, c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx,
-                                   c->ety, c->scal, c->u, c->v, c->p);
+    k_correct<<<gr, bl, 0, c->s>>>(cg(c), c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u,
+                                   c->v, c->p);
     CFD_LAUNCH_CHECK();
-    RET(wall_bcs(c));
+    RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax and the wall Nusselt numbers (FP64)
-    k_dmax<<<gr, bl, 0, c->s>>>(kg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->part);
-    k_dmax_nu_fin<<<1, NT, 0, c->s>>>(kg(c), c->part, nblk(n0, n1), c->T, c->ajac, c->etx, c->ety, c->wall_len_hot,
-                                      c->wall_len_cold, c->prm.t_hot - c->prm.t_cold, c->scal);
+    k_dmax<<<gr, bl, 0, c->s>>>(cg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, red(c));
+    k_dmax_nu_fin<<<dim3(1, 1, c->B), NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), c->T, c->ajac, c->etx,
+                                                     c->ety);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -954,64 +1099,213 @@ int capture(Case *c) {
     return CFD_OK;
 }
 
-int outer_once(Case *c, cfd_stats *st) {
+// launch one outer iteration of every active slot (no synchronisation)
+int outer_launch(Case *c) {
     // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
     if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
         CFD_HIP(hipGraphLaunch(c->gexec, c->s));
-    } else {
-        RET(outer_body(c, false));
+        return CFD_OK;
     }
+    return outer_body(c, false);
+}
+
+// after an outer iteration: per-slot statistics from the scalars (host mirror current)
+int outer_stats(Case *c, cfd_stats *st) {
+    int rc = CFD_OK;
+    for (int z = 0; z < c->B; z++) {
+        const double *h = c->hscal + (size_t)z * SS;
+        if (h[S_ACTIVE] == 0.0) {
+            if (st) st[z] = cfd_stats{};
+            continue;
+        }
+        c->iter[z]++;
+        if (c->iter[z] == 1) {
+            c->mass0[z] = h[S_MASS];
+            c->energy0[z] = h[S_ENERGY];
+        }
+        if (st) {
+            cfd_stats &s = st[z];
+            psolve_result(c, z, &s.p_iters, &s.p_relres);
+            s.mass_l1 = h[S_MASS];
+            s.energy_l1 = h[S_ENERGY];
+            s.mom_l1 = h[S_MOM];
+            s.dmax = h[S_DMAX];
+            s.nu_hot = h[S_NUH];
+            s.nu_cold = h[S_NUC];
+            s.mass_res = c->mass0[z] > 0 ? h[S_MASS] / c->mass0[z] : 0.0;
+            s.energy_res = c->energy0[z] > 0 ? h[S_ENERGY] / c->energy0[z] : 0.0;
+            s.iter = c->iter[z];
+        }
+        if (!std::isfinite(h[S_MASS]) || !std::isfinite(h[S_ENERGY])) rc = CFD_E_NAN;
+    }
+    return rc;
+}
+
+int outer_once(Case *c, cfd_stats *st) {
+    RET(outer_launch(c));
     RET(sync_scal(c));
-    int its = 0;
-    double relres = 0.0;
-    psolve_result(c, &its, &relres);
-    const double *h = c->hscal;
-    c->iter++;
-    if (c->iter == 1) {
-        c->mass0 = h[S_MASS];
-        c->energy0 = h[S_ENERGY];
-    }
-    if (st) {
-        st->mass_l1 = h[S_MASS];
-        st->energy_l1 = h[S_ENERGY];
-        st->mom_l1 = h[S_MOM];
-        st->dmax = h[S_DMAX];
-        st->nu_hot = h[S_NUH];
-        st->nu_cold = h[S_NUC];
-        st->p_iters = its;
-        st->p_relres = relres;
-        st->mass_res = c->mass0 > 0 ? h[S_MASS] / c->mass0 : 0.0;
-        st->energy_res = c->energy0 > 0 ? h[S_ENERGY] / c->energy0 : 0.0;
-        st->iter = c->iter;
-    }
-    if (!std::isfinite(h[S_MASS]) || !std::isfinite(h[S_ENERGY])) return CFD_E_NAN;
-    return CFD_OK;
+    return outer_stats(c, st);
 }
 
 // one launch of a fine-level solver kernel on the solver's own buffers (timing only)
 template <typename T>
 int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
     OpLevel<T> &L = ps->lev[0];
-    const dim3 gr = grid2d(L.n0, L.n1), bl(BX, BY);
+    dim3 gr = grid2d(L.n0, L.n1);
+    gr.z = c->B;
+    const dim3 bl(BX, BY);
     if (kernel == 0) {
         k_fapply<T><<<gr, bl, 0, c->s>>>(fargs(L), L.x, L.t);
     } else if (ps->lc == 0) {
         return CFD_E_ARG;  // no fine multigrid level on this grid
     } else if (kernel == 4) {
         OpLevel<T> &C = ps->lev[1];
-        DownArgs<T> a{fargs(L), L.invd, L.b, nullptr, 0, nullptr, L.x, C.n0, C.n1, C.t, ps->lpart, (T)ps->omega};
-        const dim3 g2((C.n0 + MG_CX - 1) / MG_CX, (C.n1 + MG_CY - 1) / MG_CY);
-        k_mg_down<T, 2><<<g2, PS_NT, 0, c->s>>>(nullptr, a);
+        DownArgs<T> a{fargs(L), L.invd, L.b, nullptr, 0, nullptr, L.x, C.n0, C.n1, C.t, ps->lpart, (T)ps->omega,
+                      ps->lps, c->scal, -1};
+        dim3 g2 = ps->down_grid(0);
+        g2.z = c->B;
+        k_mg_down<T, 2><<<g2, PS_NT, 0, c->s>>>(a);
     } else {
         OpLevel<T> &C = ps->lev[1];
-        UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega};
-        const dim3 g2((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY);
-        k_mg_up<T, 2><<<g2, PS_NT, 0, c->s>>>(nullptr, a);
+        UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega, c->scal, -1};
+        const dim3 g2((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, c->B);
+        k_mg_up<T, 2><<<g2, PS_NT, 0, c->s>>>(a);
     }
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
+
+// device-side slot state: SlotP and the S_ACTIVE scalar
+int put_slot(Case *c, int z) {
+    CFD_HIP(hipMemcpyAsync(c->spd + z, &c->sph[z], sizeof(SlotP), hipMemcpyHostToDevice, c->s));
+    c->hscal[(size_t)z * SS + S_ACTIVE] = c->sph[z].active;
+    CFD_HIP(hipMemcpyAsync(c->scal + (size_t)z * SS + S_ACTIVE, c->hscal + (size_t)z * SS + S_ACTIVE, sizeof(double),
+                           hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+void batch_destroy(Case *c) {
+    if (!c) return;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    drop_graph(c);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c->ps64;
+    delete c->ps32;
+    delete c;
+    (void)hipGetLastError();
+}
+
+int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params, Case **out) {
+    if (!out || !params) return CFD_E_ARG;
+    if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0) || slots < 1 || slots > 65535) return CFD_E_ARG;
+    if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
+    *out = nullptr;
+    Case *c = new Case();
+    c->n0 = n0; c->n1 = n1; c->B = slots; c->N = (size_t)n0 * n1; c->h1 = dxi; c->h2 = deta; c->prm = *params;
+    c->mixed = params->precision == CFD_MIXED;
+    c->sph.assign(slots, SlotP{});
+    c->mass0.assign(slots, 0.0);
+    c->energy0.assign(slots, 0.0);
+    c->iter.assign(slots, 0);
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
+    const size_t BN = (size_t)slots * c->N;
+    int rc = CFD_OK;
+    double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
+                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->d, &c->us, &c->vs,
+                         &c->uh, &c->vh, &c->q, &c->pc};
+    for (double **f : fields)
+        if (rc == CFD_OK) rc = dalloc(c, f, BN);
+    void **pred[] = {&c->Ru, &c->Rv, &c->RT, &c->U, &c->V, &c->jb[0], &c->jb[1], &c->jb[2], &c->jb[3],
+                     &c->jb[4], &c->jb[5]};
+    for (void **f : pred) {
+        if (rc != CFD_OK) break;
+        if (c->mixed) rc = dalloc(c, (float **)f, BN);
+        else rc = dalloc(c, (double **)f, BN);
+    }
+    if (c->mixed) {
+        for (int k = 0; k < 5 && rc == CFD_OK; k++) rc = dalloc(c, &c->a32[k], BN);
+        if (rc == CFD_OK) rc = dalloc(c, &c->ef, BN);
+        if (rc == CFD_OK) rc = dalloc(c, &c->r64, BN);
+        for (int k = 0; k < 4 && rc == CFD_OK; k++) rc = dalloc(c, &c->f64[k], BN);
+    }
+    c->scr = (size_t)2 * (n0 + n1) + 8;
+    if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)slots * c->scr);
+    c->pstride = (nblk(n0, n1) + 255) / 256 * 256;
+    if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)slots * psz(c));
+    if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)slots * SS);
+    if (rc == CFD_OK) rc = dalloc(c, &c->spd, (size_t)slots);
+    if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (size_t)slots * SS * sizeof(double)) != hipSuccess)
+        rc = CFD_E_OOM;
+    if (rc == CFD_OK) std::memset(c->hscal, 0, (size_t)slots * SS * sizeof(double));
+    if (rc == CFD_OK) rc = build_levels(c);
+    if (rc == CFD_OK && hipStreamSynchronize(c->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc != CFD_OK) {
+        batch_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return CFD_OK;
+}
+
+// Load a case into slot z: grid arrays (host), coarse metrics, physics; fields at
+// rest with the wall conditions; the slot becomes active.
+int batch_set(Case *c, int z, const double *x, const double *y, const double *xix, const double *xiy,
+              const double *etx, const double *ety, const double *alph, const double *beta, const double *gamma,
+              const double *ajac, const double *p1, const double *q1, const cfd_params *phys) {
+    if (!c || z < 0 || z >= c->B || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac ||
+        !phys)
+        return CFD_E_ARG;
+    const int n0 = c->n0, n1 = c->n1;
+    const size_t N = c->N, B = N * sizeof(double), o = (size_t)z * N;
+    // the slot's kernels must not run while its arrays are replaced
+    c->sph[z].active = 0.0;
+    RET(put_slot(c, z));
+    const double *src[] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac};
+    double *dst[] = {c->xg, c->yg, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->ajac};
+    for (int k = 0; k < 10; k++) CFD_HIP(hipMemcpyAsync(dst[k] + o, src[k], B, hipMemcpyHostToDevice, c->s));
+    const double *pq[2] = {p1, q1};
+    double *pqd[2] = {c->p1, c->q1};
+    for (int k = 0; k < 2; k++) {
+        if (pq[k]) CFD_HIP(hipMemcpyAsync(pqd[k] + o, pq[k], B, hipMemcpyHostToDevice, c->s));
+        else CFD_HIP(hipMemsetAsync(pqd[k] + o, 0, B, c->s));
+    }
+    double *state[] = {c->u, c->v, c->T, c->p, c->pc};
+    for (double *f : state) CFD_HIP(hipMemsetAsync(f + o, 0, B, c->s));
+    if (c->mixed) {
+        const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
+        const int lb = 256, lg = (int)((N + lb - 1) / lb);
+        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, m[k] + o, c->a32[k] + o);
+        CFD_HIP(hipMemsetAsync(c->ef + o, 0, N * sizeof(float), c->s));
+        CFD_LAUNCH_CHECK();
+    }
+    RET(inject_slot(c, z));
+    SlotP &sp = c->sph[z];
+    sp.Re = phys->Re; sp.Ri = phys->Ri; sp.Pr = phys->Pr; sp.dt = phys->dt;
+    sp.alpha_u = phys->alpha_u; sp.alpha_t = phys->alpha_t; sp.alpha_p = phys->alpha_p;
+    sp.lid = phys->lid_u; sp.t_hot = phys->t_hot; sp.t_cold = phys->t_cold;
+    sp.len_hot = sp.len_cold = 0.0;
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t a = (size_t)i, b = (size_t)i + 1;
+        const size_t a1 = (size_t)(n1 - 1) * n0 + i, b1 = a1 + 1;
+        sp.len_hot += std::hypot(x[b] - x[a], y[b] - y[a]);
+        sp.len_cold += std::hypot(x[b1] - x[a1], y[b1] - y[a1]);
+    }
+    sp.active = 1.0;
+    RET(put_slot(c, z));
+    c->iter[z] = 0;
+    c->mass0[z] = c->energy0[z] = 0.0;
+    RET(wall_bcs(c, z, 1, true));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int batch_check_slot(const Case *c, int z) { return (!c || z < 0 || z >= c->B) ? CFD_E_ARG : CFD_OK; }
 
 }  // namespace
 
@@ -1031,73 +1325,106 @@ void cfd_params_default(cfd_params *p) {
     p->graph = 1;
 }
 
+// ---- batches
+
+int cfd_batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params,
+                     cfd_batch *out) {
+    if (!out) return CFD_E_ARG;
+    Case *c = nullptr;
+    RET(batch_create(n0, n1, dxi, deta, slots, params, &c));
+    *out = reinterpret_cast<cfd_batch>(c);
+    return CFD_OK;
+}
+
+int cfd_batch_destroy(cfd_batch h) {
+    batch_destroy(reinterpret_cast<Case *>(h));
+    return CFD_OK;
+}
+
+int cfd_batch_set_case(cfd_batch h, int slot, const double *x, const double *y, const double *xix,
+                       const double *xiy, const double *etx, const double *ety, const double *alph,
+                       const double *beta, const double *gamma, const double *ajac, const double *p1,
+                       const double *q1, const cfd_params *params) {
+    return batch_set(reinterpret_cast<Case *>(h), slot, x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1,
+                     params);
+}
+
+int cfd_batch_release(cfd_batch h, int slot) {
+    Case *c = reinterpret_cast<Case *>(h);
+    RET(batch_check_slot(c, slot));
+    c->sph[slot].active = 0.0;
+    return put_slot(c, slot);
+}
+
+int cfd_batch_outer(cfd_batch h, int n, cfd_stats *st) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || n < 0) return CFD_E_ARG;
+    for (int k = 0; k < n; k++) RET(outer_once(c, st));
+    return CFD_OK;
+}
+
+int cfd_batch_launch(cfd_batch h) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    return outer_launch(c);
+}
+
+int cfd_batch_finish(cfd_batch h, cfd_stats *st) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c) return CFD_E_ARG;
+    RET(sync_scal(c));
+    return outer_stats(c, st);
+}
+
+int cfd_batch_get_fields(cfd_batch h, int slot, double *u, double *v, double *T, double *p) {
+    Case *c = reinterpret_cast<Case *>(h);
+    RET(batch_check_slot(c, slot));
+    const size_t B = c->N * sizeof(double), o = (size_t)slot * c->N;
+    if (u) CFD_HIP(hipMemcpyAsync(u, c->u + o, B, hipMemcpyDeviceToHost, c->s));
+    if (v) CFD_HIP(hipMemcpyAsync(v, c->v + o, B, hipMemcpyDeviceToHost, c->s));
+    if (T) CFD_HIP(hipMemcpyAsync(T, c->T + o, B, hipMemcpyDeviceToHost, c->s));
+    if (p) CFD_HIP(hipMemcpyAsync(p, c->p + o, B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_batch_set_fields(cfd_batch h, int slot, const double *u, const double *v, const double *T,
+                         const double *p) {
+    Case *c = reinterpret_cast<Case *>(h);
+    RET(batch_check_slot(c, slot));
+    if (!u || !v || !T || !p) return CFD_E_ARG;
+    const size_t B = c->N * sizeof(double), o = (size_t)slot * c->N;
+    CFD_HIP(hipMemcpyAsync(c->u + o, u, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->v + o, v, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->T + o, T, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipMemcpyAsync(c->p + o, p, B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    c->iter[slot] = 0;
+    c->mass0[slot] = c->energy0[slot] = 0.0;
+    return CFD_OK;
+}
+
+int cfd_batch_bytes(cfd_batch h, size_t *bytes) {
+    Case *c = reinterpret_cast<Case *>(h);
+    if (!c || !bytes) return CFD_E_ARG;
+    *bytes = c->bytes;
+    return CFD_OK;
+}
+
+// ---- single cases: a batch of one slot
+
 int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, const double *y,
                     const double *xix, const double *xiy, const double *etx, const double *ety,
                     const double *alph, const double *beta, const double *gamma, const double *ajac,
                     const double *p1, const double *q1, const cfd_params *params, cfd_case *out) {
     if (!out || !params || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac)
         return CFD_E_ARG;
-    if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0)) return CFD_E_ARG;
-    if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
     *out = nullptr;
-    Case *c = new Case();
-    c->n0 = n0; c->n1 = n1; c->N = (size_t)n0 * n1; c->h1 = dxi; c->h2 = deta; c->prm = *params;
-    c->mixed = params->precision == CFD_MIXED;
-    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
-    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
-    const size_t N = c->N;
-    int rc = CFD_OK;
-    double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
-                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->d, &c->us, &c->vs,
-                         &c->uh, &c->vh, &c->q, &c->pc};
-    for (double **f : fields)
-        if (rc == CFD_OK) rc = dalloc(c, f, N);
-    void **pred[] = {&c->Ru, &c->Rv, &c->RT, &c->U, &c->V, &c->jb[0], &c->jb[1], &c->jb[2], &c->jb[3],
-                     &c->jb[4], &c->jb[5]};
-    for (void **f : pred) {
-        if (rc != CFD_OK) break;
-        if (c->mixed) rc = dalloc(c, (float **)f, N);
-        else rc = dalloc(c, (double **)f, N);
-    }
-    if (c->mixed) {
-        for (int k = 0; k < 5 && rc == CFD_OK; k++) rc = dalloc(c, &c->a32[k], N);
-        if (rc == CFD_OK) rc = dalloc(c, &c->ef, N);
-        if (rc == CFD_OK) rc = dalloc(c, &c->r64, N);
-        for (int k = 0; k < 4 && rc == CFD_OK; k++) rc = dalloc(c, &c->f64[k], N);
-    }
-    if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)2 * (n0 + n1) + 8);
-    c->pstride = (nblk(n0, n1) + 255) / 256 * 256;
-    if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)4 * c->pstride);
-    if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)S_NSCAL + 8);
-    if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (S_NSCAL + 8) * sizeof(double)) != hipSuccess)
-        rc = CFD_E_OOM;
-    const size_t B = N * sizeof(double);
-    const double *src[] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac};
-    double *dst[] = {c->xg, c->yg, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->ajac};
-    for (int k = 0; k < 10 && rc == CFD_OK; k++)
-        if (hipMemcpyAsync(dst[k], src[k], B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
-    if (rc == CFD_OK && p1 && hipMemcpyAsync(c->p1, p1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
-    if (rc == CFD_OK && q1 && hipMemcpyAsync(c->q1, q1, B, hipMemcpyHostToDevice, c->s) != hipSuccess) rc = CFD_E_HIP;
-    if (rc == CFD_OK && c->mixed) {
-        const double *m[5] = {c->alph, c->beta, c->gamma, c->p1, c->q1};
-        const int lb = 256, lg = (int)((N + lb - 1) / lb);
-        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, c->s>>>(N, m[k], c->a32[k]);
-        if (hipGetLastError() != hipSuccess) rc = CFD_E_HIP;
-    }
-    if (rc == CFD_OK) rc = build_levels(c);
-    if (rc == CFD_OK) {
-        c->wall_len_hot = c->wall_len_cold = 0.0;
-        for (int i = 0; i < n0 - 1; i++) {
-            const size_t a = (size_t)i, b = (size_t)i + 1;
-            const size_t a1 = (size_t)(n1 - 1) * n0 + i, b1 = a1 + 1;
-            c->wall_len_hot += std::hypot(x[b] - x[a], y[b] - y[a]);
-            c->wall_len_cold += std::hypot(x[b1] - x[a1], y[b1] - y[a1]);
-        }
-        rc = apply_bcs_dev(c);
-    }
-    if (rc == CFD_OK && hipStreamSynchronize(c->s) != hipSuccess) rc = CFD_E_HIP;
+    Case *c = nullptr;
+    RET(batch_create(n0, n1, dxi, deta, 1, params, &c));
+    const int rc = batch_set(c, 0, x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1, params);
     if (rc != CFD_OK) {
-        cfd_case_destroy(reinterpret_cast<cfd_case>(c));
+        batch_destroy(c);
         return rc;
     }
     *out = reinterpret_cast<cfd_case>(c);
@@ -1105,51 +1432,22 @@ int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, co
 }
 
 int cfd_case_destroy(cfd_case h) {
-    Case *c = reinterpret_cast<Case *>(h);
-    if (!c) return CFD_OK;
-    if (c->s) (void)hipStreamSynchronize(c->s);
-    drop_graph(c);
-    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-    for (void *p : c->allocs) (void)hipFree(p);
-    if (c->hscal) (void)hipHostFree(c->hscal);
-    if (c->s) (void)hipStreamDestroy(c->s);
-    delete c->ps64;
-    delete c->ps32;
-    delete c;
-    (void)hipGetLastError();
+    batch_destroy(reinterpret_cast<Case *>(h));
     return CFD_OK;
 }
 
 int cfd_case_set_fields(cfd_case h, const double *u, const double *v, const double *T, const double *p) {
-    Case *c = reinterpret_cast<Case *>(h);
-    if (!c || !u || !v || !T || !p) return CFD_E_ARG;
-    const size_t B = c->N * sizeof(double);
-    CFD_HIP(hipMemcpyAsync(c->u, u, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemcpyAsync(c->v, v, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemcpyAsync(c->T, T, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemcpyAsync(c->p, p, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipStreamSynchronize(c->s));
-    c->iter = 0;
-    c->mass0 = c->energy0 = 0.0;
-    return CFD_OK;
+    return cfd_batch_set_fields(reinterpret_cast<cfd_batch>(h), 0, u, v, T, p);
 }
 
 int cfd_case_get_fields(cfd_case h, double *u, double *v, double *T, double *p) {
-    Case *c = reinterpret_cast<Case *>(h);
-    if (!c) return CFD_E_ARG;
-    const size_t B = c->N * sizeof(double);
-    if (u) CFD_HIP(hipMemcpyAsync(u, c->u, B, hipMemcpyDeviceToHost, c->s));
-    if (v) CFD_HIP(hipMemcpyAsync(v, c->v, B, hipMemcpyDeviceToHost, c->s));
-    if (T) CFD_HIP(hipMemcpyAsync(T, c->T, B, hipMemcpyDeviceToHost, c->s));
-    if (p) CFD_HIP(hipMemcpyAsync(p, c->p, B, hipMemcpyDeviceToHost, c->s));
-    CFD_HIP(hipStreamSynchronize(c->s));
-    return CFD_OK;
+    return cfd_batch_get_fields(reinterpret_cast<cfd_batch>(h), 0, u, v, T, p);
 }
 
 int cfd_case_apply_bcs(cfd_case h) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c) return CFD_E_ARG;
-    RET(apply_bcs_dev(c));
+    RET(wall_bcs(c, 0, c->B, true));
     CFD_HIP(hipStreamSynchronize(c->s));
     return CFD_OK;
 }
@@ -1177,12 +1475,7 @@ int cfd_case_solve(cfd_case h, double tol, int max_outer, cfd_stats *st) {
     return CFD_E_NOTCONV;
 }
 
-int cfd_case_bytes(cfd_case h, size_t *bytes) {
-    Case *c = reinterpret_cast<Case *>(h);
-    if (!c || !bytes) return CFD_E_ARG;
-    *bytes = c->bytes;
-    return CFD_OK;
-}
+int cfd_case_bytes(cfd_case h, size_t *bytes) { return cfd_batch_bytes(reinterpret_cast<cfd_batch>(h), bytes); }
 
 // residual arrays are returned in FP64 whatever the predictor precision
 int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d) {
@@ -1191,7 +1484,7 @@ int cfd_case_residuals(cfd_case h, double *Ru, double *Rv, double *RT, double *d
     RET(launch_residual(c));
     {
         const int m = std::max(c->n0, c->n1);
-        k_d_border<<<(m + 255) / 256, 256, 0, c->s>>>(c->n0, c->n1, c->d);
+        k_d_border<<<dim3((m + 255) / 256, 1, c->B), 256, 0, c->s>>>(c->n0, c->n1, c->d);
         CFD_LAUNCH_CHECK();
     }
     const size_t N = c->N, B = N * sizeof(double);
@@ -1239,7 +1532,7 @@ int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, do
     RET(sync_scal(c));
     int its = 0;
     double rr = 0.0;
-    psolve_result(c, &its, &rr);
+    psolve_result(c, 0, &its, &rr);
     CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     if (iters) *iters = its;
@@ -1280,7 +1573,7 @@ int cfd_case_profile_read(cfd_case h, double *ms_total, long *launches, int nkin
 int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c || reps < 1 || !ms_per_launch || kernel < 0 || kernel > 5) return CFD_E_ARG;
-    const dim3 gr = grid2d(c->n0, c->n1), bl(BX, BY);
+    const dim3 gr = grid_b(c), bl(BX, BY);
     hipEvent_t e0, e1;
     CFD_HIP(hipEventCreate(&e0));
     CFD_HIP(hipEventCreate(&e1));
@@ -1306,7 +1599,7 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
             (void)launch_residual(c);
             break;
         case 3:
-            k_face_div<<<gr, bl, 0, c->s>>>(kg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
+            k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
                                             c->ajac, c->q, red(c));
             break;
         }

@@ -25,6 +25,12 @@
 //    runs eagerly (with host reads of the flags between half-iterations) or
 //    captured in a graph (all iterations unrolled; kernels after convergence
 //    return at their first instruction).
+//
+// Slots: every kernel runs blockIdx.z over the slots of a batch of equally sized
+// cases (grid2d(...).z = slots).  Slot z's arrays start at z x (the array's size
+// per slot), its device scalars at z x SS; a block of a slot that is inactive,
+// or whose guard flag is set, returns at once, so slots converge and are
+// refilled independently.  A single case is a batch of one slot.
 #pragma once
 
 #include <algorithm>
@@ -48,8 +54,17 @@ enum {
     // mixed-precision refinement (FP64 defect)
     S_BB64, S_RR64, S_TOL64, S_RN64, S_BN64, S_RDONE, S_ITTOT, S_RELRES64,
     S_PSUM,   // interior sum of p' (its mean is removed by the correction)
+    S_ACTIVE, // slot holds a case (set by the host)
     S_NSCAL
 };
+constexpr int SS = 64;  // device scalars per slot
+static_assert(S_NSCAL <= SS, "scalar slots");
+
+__device__ __forceinline__ size_t zslot() { return (size_t)blockIdx.z; }
+// slot's scalars; true when the block has nothing to do (inactive slot, or guard set)
+__device__ __forceinline__ bool slot_skip(const double *sc, int gs) {
+    return sc[S_ACTIVE] == 0.0 || (gs >= 0 && sc[gs] != 0.0);
+}
 
 // FP64 geometry of one multigrid level (level 0 aliases the case's arrays)
 struct Geom {
@@ -73,6 +88,11 @@ struct FArgs {
 template <typename T>
 FArgs<T> fargs(const OpLevel<T> &L) {
     return FArgs<T>{L.n0, L.n1, L.fen, L.fet, L.fnn, L.fnt};
+}
+template <typename T>
+__device__ __forceinline__ FArgs<T> fz(FArgs<T> a, size_t o) {
+    a.fen += o; a.fet += o; a.fnn += o; a.fnt += o;
+    return a;
 }
 
 __device__ __forceinline__ int blin() { return blockIdx.x + blockIdx.y * gridDim.x; }
@@ -145,6 +165,8 @@ __device__ __forceinline__ T frow(const FArgs<T> &a, const T *__restrict__ x, in
 // y = L x (interior), 0 on the border
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_fapply(FArgs<T> a, const T *__restrict__ x, T *__restrict__ y) {
+    const size_t o = zslot() * a.n0 * a.n1;
+    a = fz(a, o); x += o; y += o;
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= a.n0 || j >= a.n1) return;
     y[idx(i, j, a.n0)] = inner_node(i, j, a.n0, a.n1) ? frow<T>(a, x, i, j) : T(0);
@@ -221,16 +243,19 @@ __device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i,
 }
 
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_faces_all(LevTab<T> A) {
+__global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<T> A) {
+    if (slot_skip(scal + zslot() * SS, -1)) return;
     int l, i, j;
     if (!lev_of_block(A, l, i, j)) return;
     const int n0 = A.n0[l], n1 = A.n1[l];
-    const size_t P = idx(i, j, n0);
+    const size_t o = zslot() * n0 * n1, o0 = zslot() * A.dn0 * A.n1[0];
+    const size_t P = o + idx(i, j, n0);
+    const double *d0 = A.d0 + o0;
     double en, et, nn, nt;
-    p9_face_lv(n0, n1, A.xix[l], A.xiy[l], A.etx[l], A.ety[l], A.jac[l], A.d0, A.dn0, l, A.h1[l], A.h2[l], i, j, 0,
-               en, et);
-    p9_face_lv(n0, n1, A.xix[l], A.xiy[l], A.etx[l], A.ety[l], A.jac[l], A.d0, A.dn0, l, A.h1[l], A.h2[l], i, j, 1,
-               nn, nt);
+    p9_face_lv(n0, n1, A.xix[l] + o, A.xiy[l] + o, A.etx[l] + o, A.ety[l] + o, A.jac[l] + o, d0, A.dn0, l, A.h1[l],
+               A.h2[l], i, j, 0, en, et);
+    p9_face_lv(n0, n1, A.xix[l] + o, A.xiy[l] + o, A.etx[l] + o, A.ety[l] + o, A.jac[l] + o, d0, A.dn0, l, A.h1[l],
+               A.h2[l], i, j, 1, nn, nt);
     A.fen[l][P] = (T)en;
     A.fet[l][P] = (T)et;
     A.fnn[l][P] = (T)nn;
@@ -246,18 +271,20 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(LevTab<T> A) {
 // inverse of the clamped operator's diagonal (every coefficient whose neighbour
 // clamps onto P), all levels
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_invd_all(LevTab<T> A) {
+__global__ __launch_bounds__(PS_NT) void k_invd_all(const double *scal, LevTab<T> A) {
+    if (slot_skip(scal + zslot() * SS, -1)) return;
     int l, i, j;
     if (!lev_of_block(A, l, i, j)) return;
     const int n0 = A.n0[l], n1 = A.n1[l];
     if (!inner_node(i, j, n0, n1)) return;
+    const size_t o = zslot() * n0 * n1;
     T c[9];
-    p9_row_faces<T>(n0, A.fen[l], A.fet[l], A.fnn[l], A.fnt[l], i, j, c);
+    p9_row_faces<T>(n0, A.fen[l] + o, A.fet[l] + o, A.fnn[l] + o, A.fnt[l] + o, i, j, c);
     T s = T(0);
     for (int dj = -1; dj <= 1; dj++)
         for (int di = -1; di <= 1; di++)
             if (clampi(i + di, 1, n0 - 2) == i && clampi(j + dj, 1, n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
-    A.invd[l][idx(i, j, n0)] = T(1) / s;
+    A.invd[l][o + idx(i, j, n0)] = T(1) / s;
 }
 
 // ---------------------------------------------------------------------------
@@ -284,6 +311,9 @@ struct DownArgs {
     T *craw;             // restricted residual on the coarse level (mean not removed)
     double *cpart;       // block partials of craw's interior sum (one per block)
     T w;
+    size_t lps;          // per-slot stride of the partial arrays
+    const double *scal;  // device scalars (guard)
+    int gs;              // guard slot (-1: none)
 };
 
 template <int NPRE>
@@ -300,8 +330,19 @@ struct DownTile {
 };
 
 template <typename T, int NPRE>
-__global__ __launch_bounds__(PS_NT) void k_mg_down(const double *gd, DownArgs<T> A) {
-    if (gd && *gd != 0.0) return;
+__global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
+    if (slot_skip(A.scal + zslot() * SS, A.gs)) return;
+    {
+        const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1, op = zslot() * A.lps;
+        A.a = fz(A.a, of);
+        A.invd += of;
+        A.bsrc += of;
+        if (A.bout) A.bout += of;
+        A.x += of;
+        A.craw += oc;
+        if (A.bpart) A.bpart += op;
+        A.cpart += op;
+    }
     using TL = DownTile<NPRE>;
     __shared__ T lds[TL::TOTAL];
     T *s_b = lds;
@@ -423,11 +464,22 @@ struct UpArgs {
     const T *xc;   // coarse-level correction
     T *out;        // x + prolongation, post-smoothed
     T w;
+    const double *scal;  // device scalars (guard)
+    int gs;              // guard slot (-1: none)
 };
 
 template <typename T, int NPOST>
-__global__ __launch_bounds__(PS_NT) void k_mg_up(const double *gd, UpArgs<T> A) {
-    if (gd && *gd != 0.0) return;
+__global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
+    if (slot_skip(A.scal + zslot() * SS, A.gs)) return;
+    {
+        const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1;
+        A.a = fz(A.a, of);
+        A.invd += of;
+        A.b += of;
+        A.x += of;
+        A.out += of;
+        A.xc += oc;
+    }
     constexpr int YX = MG_FX + 2 * NPOST, YY = MG_FY + 2 * NPOST;  // x + e, halo NPOST
     constexpr int ZX = MG_FX + 2, ZY = MG_FY + 2;                  // first post sweep (NPOST == 2)
     __shared__ T s_y[YX * YY];
@@ -537,9 +589,15 @@ __device__ __forceinline__ double cw_sum(double v) {
 // b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
 // whose mean is removed; null: used as is); the level-l0 result is written to x0out.
 template <typename T>
-__global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T> A, int l0, const T *b0,
-                                                  const double *b0part, int nb0p, T *x0out) {
-    if (gd && *gd != 0.0) return;
+__global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, CoarseArgs<T> A, int l0, const T *b0,
+                                                  const double *b0part, int nb0p, size_t lps, T *x0out) {
+    if (slot_skip(scal + zslot() * SS, gs)) return;
+    {
+        const size_t o0 = zslot() * A.L[l0].n0 * A.L[l0].n1;
+        b0 += o0;
+        x0out += o0;
+        if (b0part) b0part += zslot() * lps;
+    }
     __shared__ T sh[CN_MAX * CW_ARR];
     const int tid = threadIdx.x;
     const T w = A.omega;
@@ -552,7 +610,8 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T
     // coefficients of every level, right-hand side of level l0
     for (int l = l0; l <= last; l++) {
         const int n = NN(l);
-        const T *src[5] = {A.L[l].fen, A.L[l].fet, A.L[l].fnn, A.L[l].fnt, A.L[l].invd};
+        const size_t o = zslot() * n;
+        const T *src[5] = {A.L[l].fen + o, A.L[l].fet + o, A.L[l].fnn + o, A.L[l].fnt + o, A.L[l].invd + o};
         for (int q = 0; q < 5; q++) {
             T *dst = ARR(l, q);
             for (int k = tid; k < n; k += CW) dst[k] = src[q][k];
@@ -689,18 +748,32 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *gd, CoarseArgs<T
 // scalar recurrence / convergence test of that phase.
 
 struct RedArgs {
-    double *part;
+    double *part;   // partials: per slot NV x pstride, slot stride psz
     int pstride;
-    double *scal;
+    size_t psz;
+    double *scal;   // device scalars, slot stride SS
 };
+// the block's slot view of the reduction buffers
+__device__ __forceinline__ RedArgs rz(RedArgs ra) {
+    ra.part += zslot() * ra.psz;
+    ra.scal += zslot() * SS;
+    return ra;
+}
 
 // x (and, if acc != null, acc += x first) <- 0; r = b (interior); partials of
 // sum(r), (b, b).  outer != null and set: the solve is skipped (S_DONE = 1),
 // x is still cleared.
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bi_start1(const double *outer, int n0, int n1, const double *__restrict__ b,
+__global__ __launch_bounds__(PS_NT) void k_bi_start1(int og, int n0, int n1, const double *__restrict__ b,
                                                      T *__restrict__ x, double *__restrict__ acc, T *__restrict__ r,
                                                      RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, -1)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        b += o; x += o; r += o;
+        if (acc) acc += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const bool in = i < n0 && j < n1;
     const size_t P = in ? idx(i, j, n0) : 0;
@@ -708,7 +781,7 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start1(const double *outer, int n0
         if (acc) acc[P] += (double)x[P];
         x[P] = T(0);
     }
-    if (outer && *outer != 0.0) {
+    if (og >= 0 && ra.scal[og] != 0.0) {
         if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
         return;
     }
@@ -727,9 +800,14 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start1(const double *outer, int n0
 
 // r -= interior mean (S_SUM), r0 = r; partials of (r, r)
 template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_bi_start2(const double *outer, int n0, int n1, T *__restrict__ r,
-                                                     T *__restrict__ r0, RedArgs ra) {
-    if (outer && *outer != 0.0) return;
+__global__ __launch_bounds__(PS_NT) void k_bi_start2(int og, int n0, int n1, T *__restrict__ r, T *__restrict__ r0,
+                                                     RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, og)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        r += o; r0 += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const double mean = ra.scal[S_SUM] / ((double)(n0 - 2) * (double)(n1 - 2));
     double v[1] = {0.0};
@@ -750,7 +828,12 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start2(const double *outer, int n0
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *scal, const T *__restrict__ r,
                                                 const T *__restrict__ vv, T *__restrict__ pv) {
-    if (scal[S_DONE] != 0.0) return;
+    scal += zslot() * SS;
+    if (slot_skip(scal, S_DONE)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        r += o; vv += o; pv += o;
+    }
     const double rho = scal[S_RHO];
     if (blin() == 0 && tlin() == 0) {
         scal[S_IT] = (double)it;
@@ -769,7 +852,12 @@ __global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict__ ph, T *__restrict__ vv,
                                                 const T *__restrict__ r0, RedArgs ra) {
-    if (ra.scal[S_DONE] != 0.0) return;
+    ra = rz(ra);
+    if (slot_skip(ra.scal, S_DONE)) return;
+    {
+        const size_t o = zslot() * a.n0 * a.n1;
+        a = fz(a, o); ph += o; vv += o; r0 += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v[1] = {0.0};
     if (i < a.n0 && j < a.n1) {
@@ -788,7 +876,12 @@ __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict_
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restrict__ r, const T *__restrict__ vv,
                                                 T *__restrict__ s, RedArgs ra) {
-    if (ra.scal[S_DONE] != 0.0) return;
+    ra = rz(ra);
+    if (slot_skip(ra.scal, S_DONE)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        r += o; vv += o; s += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const double alpha = ra.scal[S_ALPHA];
     double v[1] = {0.0};
@@ -805,7 +898,12 @@ __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restr
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_t(FArgs<T> a, const T *__restrict__ sh, T *__restrict__ t,
                                                 const T *__restrict__ s, RedArgs ra) {
-    if (ra.scal[S_DONE] != 0.0) return;
+    ra = rz(ra);
+    if (slot_skip(ra.scal, S_DONE)) return;
+    {
+        const size_t o = zslot() * a.n0 * a.n1;
+        a = fz(a, o); sh += o; t += o; s += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < a.n0 && j < a.n1) {
@@ -828,9 +926,15 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
                                                  const T *__restrict__ sh, const T *__restrict__ s,
                                                  const T *__restrict__ t, const T *__restrict__ r0, T *__restrict__ x,
                                                  T *__restrict__ r, RedArgs ra) {
+    ra = rz(ra);
     const double *sc = ra.scal;
+    if (sc[S_ACTIVE] == 0.0) return;
     const bool sconv = sc[S_SCONV] == (double)it;
     if (!sconv && sc[S_DONE] != 0.0) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        ph += o; sh += o; s += o; t += o; r0 += o; x += o; r += o;
+    }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const bool in = i < n0 && j < n1;
     const size_t P = in ? idx(i, j, n0) : 0;
@@ -858,9 +962,9 @@ enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
 // totals of a phase's partials and what follows from them; skipped while *guard
 // is set (the phase's kernel did not run)
 template <int OP>
-__global__ __launch_bounds__(PS_NT) void k_bi_fin(const double *guard, RedArgs ra, int nparts, int it, double rtol,
-                                                  int rtol_slot) {
-    if (guard && *guard != 0.0) return;
+__global__ __launch_bounds__(PS_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, gs)) return;
     constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
     double tot[NV];
     sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
@@ -928,19 +1032,11 @@ __global__ void k_fill(size_t n, T *__restrict__ a, T v) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) a[k] = v;
 }
-__global__ void k_inject(int n0, const double *__restrict__ f, int m0, int m1, double *__restrict__ c) {
+// injection of the fine level's values onto the coarse level, slot z = blockIdx.z
+__global__ void k_inject(int n0, int n1, const double *__restrict__ f, int m0, int m1, double *__restrict__ c) {
     const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
     if (I >= m0 || J >= m1) return;
-    c[idx(I, J, m0)] = f[idx(2 * I, 2 * J, n0)];
-}
-// v -= interior mean (scal[slot] holds the interior sum)
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_sub_mean(int n0, int n1, T *__restrict__ v, const double *__restrict__ scal,
-                                                    int slot) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (!inner_node(i, j, n0, n1)) return;
-    const double m = scal[slot] / ((double)(n0 - 2) * (double)(n1 - 2));
-    v[idx(i, j, n0)] = (T)((double)v[idx(i, j, n0)] - m);
+    c[zslot() * m0 * m1 + idx(I, J, m0)] = f[zslot() * n0 * n1 + idx(2 * I, 2 * J, n0)];
 }
 
 // ---------------------------------------------------------------------------
@@ -949,17 +1045,35 @@ __global__ __launch_bounds__(PS_NT) void k_sub_mean(int n0, int n1, T *__restric
 // what the solver needs from its owner (stream, reductions, timing hooks)
 struct PsCtx {
     hipStream_t s;
-    double *part;
+    int B;          // slots
+    double *part;   // B x psz
     int pstride;
-    double *scal;
-    double *hscal;
+    size_t psz;
+    double *scal;   // B x SS
+    double *hscal;  // pinned host mirror of scal
     void (*prof_b)(void *, int);
     void (*prof_e)(void *);
     void (*drain)(void *);
     void *owner;
     std::vector<void *> *allocs;
     size_t *bytes;
-    RedArgs ra() const { return RedArgs{part, pstride, scal}; }
+    RedArgs ra() const { return RedArgs{part, pstride, psz, scal}; }
+    // every active slot has its S_DONE flag set (host mirror)
+    bool all_done() const {
+        for (int z = 0; z < B; z++) {
+            const double *h = hscal + (size_t)z * SS;
+            if (h[S_ACTIVE] != 0.0 && h[S_DONE] == 0.0) return false;
+        }
+        return true;
+    }
+    // some active slot stopped at the ||s|| test of iteration it (host mirror)
+    bool any_sconv(int it) const {
+        for (int z = 0; z < B; z++) {
+            const double *h = hscal + (size_t)z * SS;
+            if (h[S_ACTIVE] != 0.0 && h[S_SCONV] == (double)it) return true;
+        }
+        return false;
+    }
 };
 
 // live-timing kinds of the solver's fine-level kernels
@@ -979,7 +1093,7 @@ inline int nblk(int n0, int n1) {
 }
 
 inline int ps_sync(PsCtx &cx) {
-    CFD_HIP(hipMemcpyAsync(cx.hscal, cx.scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, cx.s));
+    CFD_HIP(hipMemcpyAsync(cx.hscal, cx.scal, (size_t)cx.B * SS * sizeof(double), hipMemcpyDeviceToHost, cx.s));
     CFD_HIP(hipStreamSynchronize(cx.s));
     if (cx.drain) cx.drain(cx.owner);
     return CFD_OK;
@@ -998,8 +1112,10 @@ struct PSolver {
     LevTab<T> tab{};
     int tab_blocks = 0;
     double *lpart = nullptr;       // block partials of the restricted right-hand sides' sums
-    int lofs[PS_MAXLEV] = {0};     // level l's partials at lpart + lofs[l] (l >= 1)
+    int lofs[PS_MAXLEV] = {0};     // level l's partials at lpart + lofs[l] (l >= 1), per slot
     int lnp[PS_MAXLEV] = {0};      // and their number
+    size_t lps = 0;                // per-slot stride of lpart
+    int B = 1;                     // slots
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
         nlev = nlev_;
@@ -1019,15 +1135,16 @@ struct PSolver {
             if (lc + 1 >= nlev) return CFD_E_ARG;
             lc++;
         }
+        B = cx.B;
         for (int l = 0; l < nlev; l++) {
             OpLevel<T> &L = lev[l];
             L.n0 = g[l].n0;
             L.n1 = g[l].n1;
-            const size_t M = (size_t)L.n0 * L.n1 * sizeof(T);
+            const size_t M = (size_t)B * L.n0 * L.n1 * sizeof(T);
             T **arr[] = {&L.fen, &L.fet, &L.fnn, &L.fnt, &L.invd, &L.x, &L.b, &L.t};
             for (T **a : arr) CFD_RET(ps_alloc(cx, (void **)a, M));
         }
-        const size_t M0 = (size_t)lev[0].n0 * lev[0].n1 * sizeof(T);
+        const size_t M0 = (size_t)B * lev[0].n0 * lev[0].n1 * sizeof(T);
         for (int k = 0; k < 8; k++) CFD_RET(ps_alloc(cx, (void **)&bi[k], M0));
         int np = 0;
         for (int l = 1; l <= lc; l++) {
@@ -1035,7 +1152,8 @@ struct PSolver {
             lnp[l] = down_grid(l - 1).x * down_grid(l - 1).y;
             np += lnp[l];
         }
-        CFD_RET(ps_alloc(cx, (void **)&lpart, (size_t)(np + 1) * sizeof(double)));
+        lps = (size_t)np + 1;
+        CFD_RET(ps_alloc(cx, (void **)&lpart, (size_t)B * lps * sizeof(double)));
         cargs.nlev = nlev;
         cargs.pre = pre;
         cargs.post = post;
@@ -1082,8 +1200,9 @@ struct PSolver {
         LevTab<T> t = tab;
         t.d0 = d0;
         for (int k = 0; k < 4; k++) t.f64[k] = f64 ? f64[k] : nullptr;
-        k_faces_all<T><<<tab_blocks, PS_NT, 0, cx.s>>>(t);
-        k_invd_all<T><<<tab_blocks, PS_NT, 0, cx.s>>>(t);
+        const dim3 gr(tab_blocks, 1, B);
+        k_faces_all<T><<<gr, PS_NT, 0, cx.s>>>(cx.scal, t);
+        k_invd_all<T><<<gr, PS_NT, 0, cx.s>>>(cx.scal, t);
         CFD_LAUNCH_CHECK();
         return CFD_OK;
     }
@@ -1093,39 +1212,41 @@ struct PSolver {
         return dim3((lev[l + 1].n0 + MG_CX - 1) / MG_CX, (lev[l + 1].n1 + MG_CY - 1) / MG_CY);
     }
 
-    // z = M^-1 y: one V-cycle from zero with right-hand side y
-    int precond(PsCtx &cx, const T *y, T *z, const double *gd) {
+    // z = M^-1 y: one V-cycle from zero with right-hand side y (slots whose
+    // scalar `gs` is set are skipped)
+    int precond(PsCtx &cx, const T *y, T *z, int gs) {
         const T w = (T)omega;
         for (int l = 0; l < lc; l++) {
             OpLevel<T> &L = lev[l];
             OpLevel<T> &C = lev[l + 1];
             DownArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.t, l == 0 ? nullptr : lpart + lofs[l], lnp[l],
-                          l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w};
-            const dim3 gr = down_grid(l);
+                          l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w, lps, cx.scal, gs};
+            dim3 gr = down_grid(l);
+            gr.z = B;
             if (l == 0) cx.prof_b(cx.owner, PK_MG_DOWN);
             switch (pre) {
-            case 0: k_mg_down<T, 0><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
-            case 1: k_mg_down<T, 1><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
-            default: k_mg_down<T, 2><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            case 0: k_mg_down<T, 0><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            case 1: k_mg_down<T, 1><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            default: k_mg_down<T, 2><<<gr, PS_NT, 0, cx.s>>>(a); break;
             }
             if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
         }
         T *cout = lc == 0 ? z : lev[lc].x;
-        k_mg_coarse<T><<<1, CW, 0, cx.s>>>(gd, cargs, lc, lc == 0 ? y : lev[lc].t,
-                                           lc == 0 ? nullptr : lpart + lofs[lc], lnp[lc], cout);
+        k_mg_coarse<T><<<dim3(1, 1, B), CW, 0, cx.s>>>(cx.scal, gs, cargs, lc, lc == 0 ? y : lev[lc].t,
+                                                       lc == 0 ? nullptr : lpart + lofs[lc], lnp[lc], lps, cout);
         CFD_LAUNCH_CHECK();
         for (int l = lc - 1; l >= 0; l--) {
             OpLevel<T> &L = lev[l];
             OpLevel<T> &C = lev[l + 1];
             UpArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.b, L.x, C.n0, C.n1, l + 1 == lc ? C.x : C.t,
-                        l == 0 ? z : L.t, w};
-            const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY);
+                        l == 0 ? z : L.t, w, cx.scal, gs};
+            const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
             if (l == 0) cx.prof_b(cx.owner, PK_MG_UP);
             switch (post) {
-            case 0: k_mg_up<T, 0><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
-            case 1: k_mg_up<T, 1><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
-            default: k_mg_up<T, 2><<<gr, PS_NT, 0, cx.s>>>(gd, a); break;
+            case 0: k_mg_up<T, 0><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            case 1: k_mg_up<T, 1><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            default: k_mg_up<T, 2><<<gr, PS_NT, 0, cx.s>>>(a); break;
             }
             if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
@@ -1134,61 +1255,62 @@ struct PSolver {
     }
 
     // L x = b to ||r|| <= rtol ||b|| (rtol from scal[rtol_slot] when rtol_slot >= 0),
-    // x cleared first (acc += x before clearing when acc != null); skipped when
-    // *outer != 0.  Iteration for iteration the oracle's bicgstab().  graph: no
-    // host synchronisation, `maxit` iterations launched; otherwise the host reads
-    // the device flags after each half-iteration and stops launching once done.
-    // The iteration count and relative residual are left in S_IT / S_RELRES.
-    int solve(PsCtx &cx, const double *b, T *x, double *acc, double rtol, int rtol_slot, int maxit,
-              const double *outer, bool graph) {
+    // per slot, x cleared first (acc += x before clearing when acc != null);
+    // skipped for slots whose scalar `og` (>= 0) is set.  Iteration for iteration
+    // the oracle's bicgstab().  graph: no host synchronisation, `maxit` iterations
+    // launched; otherwise the host reads the device flags after each
+    // half-iteration and stops launching once every slot is done.  The iteration
+    // count and relative residual are left in S_IT / S_RELRES of each slot.
+    int solve(PsCtx &cx, const double *b, T *x, double *acc, double rtol, int rtol_slot, int maxit, int og,
+              bool graph) {
         OpLevel<T> &L0 = lev[0];
         const int n0 = L0.n0, n1 = L0.n1;
-        const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+        dim3 gr = grid2d(n0, n1);
+        gr.z = B;
+        const dim3 bl(BX, BY), g1(1, 1, B);
         const int nb = nblk(n0, n1);
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
-        const double *dn = cx.scal + S_DONE;
-        k_bi_start1<T><<<gr, bl, 0, cx.s>>>(outer, n0, n1, b, x, acc, r, ra);
-        k_bi_fin<FIN_START1><<<1, PS_NT, 0, cx.s>>>(outer, ra, nb, 0, 0.0, -1);
-        k_bi_start2<T><<<gr, bl, 0, cx.s>>>(outer, n0, n1, r, r0, ra);
-        k_bi_fin<FIN_START2><<<1, PS_NT, 0, cx.s>>>(outer, ra, nb, 0, rtol, rtol_slot);
+        k_bi_start1<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, b, x, acc, r, ra);
+        k_bi_fin<FIN_START1><<<g1, PS_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
+        k_bi_start2<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
+        k_bi_fin<FIN_START2><<<g1, PS_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
         if (!graph) {
             CFD_RET(ps_sync(cx));
-            if (cx.hscal[S_DONE] != 0.0) return CFD_OK;
+            if (cx.all_done()) return CFD_OK;
         }
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
-            CFD_RET(precond(cx, pv, ph, dn));
+            CFD_RET(precond(cx, pv, ph, S_DONE));
             cx.prof_b(cx.owner, PK_APPLY);
             k_bi_v<T><<<gr, bl, 0, cx.s>>>(a0, ph, vv, r0, ra);
             cx.prof_e(cx.owner);
-            k_bi_fin<FIN_V><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_V><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
-            k_bi_fin<FIN_S><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_S><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             if (!graph) {
                 CFD_RET(ps_sync(cx));
-                if (cx.hscal[S_DONE] != 0.0) {
-                    if (cx.hscal[S_SCONV] == (double)it)
-                        k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
+                if (cx.all_done()) {
+                    if (cx.any_sconv(it)) k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
                     CFD_LAUNCH_CHECK();
                     return CFD_OK;
                 }
             }
-            CFD_RET(precond(cx, s, sh, dn));
+            CFD_RET(precond(cx, s, sh, S_DONE));
             cx.prof_b(cx.owner, PK_APPLY);
             k_bi_t<T><<<gr, bl, 0, cx.s>>>(a0, sh, t, s, ra);
             cx.prof_e(cx.owner);
-            k_bi_fin<FIN_T><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_T><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
-            k_bi_fin<FIN_XR><<<1, PS_NT, 0, cx.s>>>(dn, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_XR><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             if (!graph) {
                 CFD_RET(ps_sync(cx));
-                if (cx.hscal[S_DONE] != 0.0) return CFD_OK;
+                if (cx.all_done()) return CFD_OK;
             }
         }
         return CFD_OK;

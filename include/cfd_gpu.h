@@ -183,6 +183,7 @@ typedef struct {
 } cfd_stats;
 
 typedef struct cfd_case_s *cfd_case;
+typedef struct cfd_batch_s *cfd_batch;
 
 /* Fill *p with the defaults used by the cavity configurations. */
 void cfd_params_default(cfd_params *p);
@@ -221,16 +222,55 @@ int cfd_case_p_solve(cfd_case c, const double *d, const double *q, double *x,
  * cfd_case_profile_read returns, per kind, the summed device milliseconds and
  * the launch count since the last cfd_case_profile call (enable = 2 only
  * resets the counters).
- * Kinds: 0 p' operator apply, 1 predictor Jacobi sweep, 2 residual/RHS,
- * 3 face divergence, 4 multigrid smoothing sweep. */
+ * Kinds: 0 p' operator apply (with its dot product), 1 predictor Jacobi sweep,
+ * 2 residual/RHS, 3 face divergence, 4 multigrid down (pre-smoothing, residual,
+ * restriction) and 5 multigrid up (prolongation, post-smoothing) on the fine level. */
 int cfd_case_profile(cfd_case c, int enable);
 int cfd_case_profile_read(cfd_case c, double *ms_total, long *launches, int nkinds);
 
 /* Average device time of one launch of a hot kernel on the case's stream,
  * measured with HIP events around `reps` back-to-back launches on current
- * data.  kernel: 0 = p' operator apply, 1 = predictor Jacobi sweep,
- * 2 = residual/RHS, 3 = face divergence, 4 = multigrid smoothing sweep. */
+ * data.  kernel: 0 = p' operator apply, 1 = predictor (all sweeps),
+ * 2 = residual/RHS, 3 = face divergence, 4 = multigrid down, 5 = multigrid up
+ * (fine level). */
 int cfd_case_time_kernel(cfd_case c, int kernel, int reps, double *ms_per_launch);
+
+/* ------------------------------------------------------------------ */
+/* 3. Batches: slots of equally sized cavity cases advanced together   */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one case per process, solver.cpp:2041-2062
+ * main).  A batch holds `slots` cases of n0 x n1 nodes with spacing (dxi, deta);
+ * every kernel of an outer iteration serves all slots in one launch.  A single
+ * cfd_case is a batch of one slot, and the cfd_case_* calls below apply to it.
+ * Batch-wide settings come from `params` at creation (predictor sweeps, p'
+ * solver, multigrid, precision, graph); each slot's physics (Re, Ri, Pr, dt,
+ * alpha_u, alpha_p, alpha_t, lid_u, t_hot, t_cold) from the params given to
+ * cfd_batch_set_case. */
+int cfd_batch_create(int n0, int n1, double dxi, double deta, int slots,
+                     const cfd_params *params, cfd_batch *out);
+int cfd_batch_destroy(cfd_batch b);
+/* Load a case into a slot (grid arrays as for cfd_case_create); its fields start
+ * at rest with the wall conditions and the slot becomes active.  Other slots are
+ * not touched. */
+int cfd_batch_set_case(cfd_batch b, int slot, const double *x, const double *y,
+                       const double *xix, const double *xiy, const double *etx, const double *ety,
+                       const double *alph, const double *beta, const double *gamma,
+                       const double *ajac, const double *p1, const double *q1,
+                       const cfd_params *params);
+/* The slot stops taking part in outer iterations (its arrays stay readable). */
+int cfd_batch_release(cfd_batch b, int slot);
+/* n outer iterations of every active slot; st (may be NULL) receives `slots`
+ * entries: the last iteration's statistics per slot (zeros for inactive slots). */
+int cfd_batch_outer(cfd_batch b, int n, cfd_stats *st);
+/* The two halves of one cfd_batch_outer iteration: enqueue it on the batch's
+ * stream without waiting, then wait and read the statistics (several batches
+ * can be launched before any is finished). */
+int cfd_batch_launch(cfd_batch b);
+int cfd_batch_finish(cfd_batch b, cfd_stats *st);
+int cfd_batch_get_fields(cfd_batch b, int slot, double *u, double *v, double *T, double *p);
+int cfd_batch_set_fields(cfd_batch b, int slot, const double *u, const double *v, const double *T,
+                         const double *p);
+int cfd_batch_bytes(cfd_batch b, size_t *bytes);
 
 #ifdef __cplusplus
 }

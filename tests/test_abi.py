@@ -37,3 +37,18 @@ def test_argument_errors_need_no_gpu():
     z = np.zeros((3, 3))
     # grids below 5 x 5 are rejected before any device work
     assert lib.cfd_ref_assemble_p9(3, 3, 0.5, 0.5, z, z, z, z, np.zeros((9, 3, 3))) == -1
+
+
+def test_batch_argument_errors_need_no_gpu():
+    import ctypes as C
+    lib = _lib.load()
+    prm = _lib.Params()
+    lib.cfd_params_default(C.byref(prm))
+    h = C.c_void_p()
+    # zero slots, tiny grids and a bad precision are rejected before any device work
+    assert lib.cfd_batch_create(65, 65, 0.1, 0.1, 0, C.byref(prm), C.byref(h)) == -1
+    assert lib.cfd_batch_create(4, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
+    prm.precision = 7
+    assert lib.cfd_batch_create(65, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
+    assert lib.cfd_batch_release(None, 0) == -1
+
