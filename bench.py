@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark: SIMPLE outer iterations of the steady mixed-convection cavity on one MI355X.
 
-Workload (default): 1025 x 1025 nodes, skew amplitude 0.1 with tanh wall
-clustering beta = 1.5, Re = 400, Ri = 1, Pr = 0.71, FP64, relaxation 0.7/0.3,
-8 predictor sweeps, p' BiCGStab + multigrid to 1e-2 per outer iteration.
-One step = one outer iteration over the whole case (fields resident in HBM).
+Headline (value): 1025 x 1025 nodes, skew amplitude 0.1 with tanh wall clustering
+beta = 1.5, Re = 400, Ri = 1, Pr = 0.71, relaxation 0.7/0.3, 8 predictor sweeps,
+p' BiCGStab + multigrid to 1e-2 per outer iteration; mixed path (FP32 predictor
+and p' inner solve, FP64 residuals, refinement and accumulation), with the FP64
+path beside it.  One step = one outer iteration over the whole case, replayed
+from its captured graph (fields resident in HBM).
 
     python bench.py [--steps K] [--warmup W] [--n 1025] [--cpu-iters 6] [--no-cpu]
+                    [--sections main,batch,serve,piso]
 
 Prints one JSON line (value = cell-iterations/s, a cell = an interior node).
 The roofline object is for the kernel with the largest device time per outer
-iteration, timed live with HIP events on the case's stream during the timed
-steps; its algorithmic bytes per launch are (arrays read + arrays written) x
-interior nodes x 8 B.  cpu_baseline times the CPU oracle (a serial FP64 port of
-the same iteration) on this host for a bounded number of outer iterations.
+iteration, timed live with HIP events on the case's stream; its algorithmic
+bytes per launch are (arrays read + arrays written) x nodes x element size.
+cpu_baseline times the CPU oracle (a serial FP64 port of the same iteration) on
+this host for a bounded number of outer iterations.  Further objects in config:
+  batched   256 cases (65..513 nodes per side, mixed physics) advanced as four
+            size-class batches, all slots per launch, vs one case at a time;
+  served    continuous batching of Poisson-arriving solves to 1e-8: solves/s,
+            latency p50/p99;
+  p_isolate the p' solve alone at 2049 x 2049 to 1e-10, FP64 vs mixed.
 """
 from __future__ import annotations
 
@@ -22,6 +30,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -51,6 +61,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--sections", default="main,batch,serve,piso")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="served arrivals per second")
+    ap.add_argument("--serve-count", type=int, default=200)
     return ap.parse_args()
 
 
@@ -122,6 +135,152 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
     return value, elapsed / steps * 1e3, roof, info
 
 
+def _single_ms(n, mixed, steps=15, warmup=3):
+    """ms per outer iteration of one n x n case alone (graph replay)."""
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
+    from cfd_solver_amd.cavity import CavityCase
+    from cfd_solver_amd.grid import cavity_grid
+    c = CavityCase(cavity_grid(n, skew=0.075), precision=CFD_MIXED if mixed else CFD_FP64)
+    c.step(warmup)
+    t0 = time.perf_counter()
+    c.step(steps)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    c.close()
+    return ms
+
+
+def run_batched(mixed, steps=10, warmup=2):
+    """The 256-case mix as one batch per grid size; every step advances all cases."""
+    from collections import defaultdict
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
+    from cfd_solver_amd.cavity import CavityBatch, default_params
+    from cfd_solver_amd.grid import cavity_grid
+    from cfd_solver_amd.workloads import batched_cases
+    cases = batched_cases()
+    by_n = defaultdict(list)
+    for n, skew, phys in cases:
+        by_n[n].append((skew, phys))
+    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64)
+    batches = []
+    for n in sorted(by_n):
+        g0 = cavity_grid(n)
+        b = CavityBatch(n, n, g0.dxi, g0.deta, len(by_n[n]), prm)
+        for z, (skew, phys) in enumerate(by_n[n]):
+            b.set_case(z, cavity_grid(n, skew=skew), **phys)
+        batches.append(b)
+    for _ in range(warmup):
+        for b in batches:
+            b.launch()
+        for b in batches:
+            b.finish()
+    t0 = time.perf_counter()
+    diverged = 0
+    for k in range(steps):
+        for b in batches:
+            b.launch()
+        for b in batches:
+            st = b.finish()
+            if any(s.iter == 0 for s in st):
+                raise RuntimeError("a batched case did not advance")
+            if k == steps - 1:
+                diverged += sum(1 for s in st if not np.isfinite(s.mass_l1) or not np.isfinite(s.energy_l1))
+    el = time.perf_counter() - t0
+    cells = sum((n - 2) ** 2 for n, _, _ in cases)
+    nbytes = sum(b.nbytes() for b in batches)
+    for b in batches:
+        b.close()
+    single = {n: _single_ms(n, mixed) for n in sorted(by_n)}
+    alone_ms = sum(single[n] * len(by_n[n]) for n in by_n)
+    ms = el / steps * 1e3
+    return {"value": cells * steps / el, "unit": "cell-iter/s", "ms_per_step": ms, "cases": len(cases),
+            "nodes": sum(n * n for n, _, _ in cases), "size_classes": {str(n): len(v) for n, v in sorted(by_n.items())},
+            "hbm_bytes": nbytes, "non_finite_cases": diverged, "one_at_a_time_ms_per_step": alone_ms, "speedup_vs_one_at_a_time": alone_ms / ms,
+            "single_case_ms_per_step": {str(n): round(v, 4) for n, v in single.items()},
+            "precision": "mixed" if mixed else "fp64",
+            "workload": "256 cases, n log-uniform in [65, 513] (2^k+1), A ~ U[0, 0.15], Re ~ U[50, 1000], "
+                        "Ri log-uniform [0.1, 10], Pr in {0.71, 7}, seed 4; one step = one outer iteration of "
+                        "every case"}
+
+
+def run_served(mixed, lam, count, kmin=5, kmax=6, tol=1e-8, slots=32, max_outer=10000):
+    """Poisson-arriving solves to tol, served by continuous batching."""
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
+    from cfd_solver_amd.cavity import default_params
+    from cfd_solver_amd.grid import cavity_grid
+    from cfd_solver_amd.serve import CavityServer
+    from cfd_solver_amd.workloads import served_cases
+    arrivals = served_cases(count, lam, kmin=kmin, kmax=kmax)
+    grids = [cavity_grid(n, skew=skew) for _, n, skew, _ in arrivals]
+    srv = CavityServer(default_params(precision=CFD_MIXED if mixed else CFD_FP64), hbm_budget=64 << 30,
+                       slots_per_batch=slots)
+    # warm the graphs of both size classes outside the timed run
+    warm = {srv.submit(cavity_grid(2 ** k + 1), tol=tol, max_outer=3) for k in range(kmin, kmax + 1)}
+    while warm:
+        srv.step()
+        warm = {t for t in warm if srv.poll(t) is None}
+    done, lat, its, states = 0, [], [], {}
+    pending = {}
+    nxt = 0
+    busy = 0.0
+    t0 = time.perf_counter()
+    while done < count:
+        now = time.perf_counter() - t0
+        while nxt < count and arrivals[nxt][0] <= now:
+            _, n, skew, phys = arrivals[nxt]
+            pending[srv.submit(grids[nxt], tol=tol, max_outer=max_outer, **phys)] = nxt
+            nxt += 1
+        if not pending:
+            time.sleep(max(0.0, min(arrivals[nxt][0] - now, 0.01)))
+            continue
+        ts = time.perf_counter()
+        srv.step()
+        busy += time.perf_counter() - ts
+        for t in list(pending):
+            r = srv.poll(t)
+            if r is not None:
+                del pending[t]
+                done += 1
+                lat.append(r["latency"])
+                its.append(r["stats"].iter)
+                states[r["state"]] = states.get(r["state"], 0) + 1
+    el = time.perf_counter() - t0
+    srv.close()
+    lat = np.sort(np.array(lat))
+    return {"solves_per_s": count / el, "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "mean_outer_iterations": float(np.mean(its)),
+            "requests": count, "arrival_rate_per_s": lam, "busy_fraction": busy / el, "states": states,
+            "elapsed_s": el, "tolerance": tol, "precision": "mixed" if mixed else "fp64",
+            "workload": f"Poisson arrivals (seed 5) at {lam}/s, n log-uniform over 2^k+1, k in [{kmin}, {kmax}], "
+                        "A ~ U[0, 0.15], Re ~ U[50, 150], Ri log-uniform [0.1, 1], Pr 0.71; each solve to mass "
+                        f"and energy residual {tol:g} (cap {max_outer} outer iterations); {slots} slots per size "
+                        "class"}
+
+
+def run_p_isolate(n=2049, reps=3):
+    """p' solve alone to 1e-10 on an n x n grid (skew 0.15), FP64 vs mixed, device-resident."""
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
+    from cfd_solver_amd.cavity import CavityCase
+    from cfd_solver_amd.grid import cavity_grid
+    from cfd_solver_amd.workloads import p_isolate
+    g = cavity_grid(n, skew=0.15)
+    u, v, q = p_isolate(g)
+    out = {"grid": [n, n], "rtol": 1e-10,
+           "workload": "u, v = 8 Fourier modes each (seed 20250101) setting d; q ~ N(0, 1) minus its mean"}
+    for name, prec in (("fp64", CFD_FP64), ("mixed", CFD_MIXED)):
+        c = CavityCase(g, precision=prec)
+        c.set_fields(u, v, np.zeros_like(u), np.zeros_like(u))
+        c.residuals()
+        _, its, rr = c.p_solve(None, q, rtol=1e-10, maxit=400, want_x=False)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, its, rr = c.p_solve(None, None, rtol=1e-10, maxit=400, want_x=False)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out[name] = {"ms_per_solve": ms, "iterations": its, "relres": rr}
+        c.close()
+    out["mixed_speedup"] = out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"]
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -129,16 +288,15 @@ def main():
               "built in this version", file=sys.stderr)
         sys.exit(2)
     from cfd_solver_amd.grid import cavity_grid
+    sections = set(a.sections.split(","))
     n = a.n
     grid = cavity_grid(n, skew=0.1, beta=1.5)
     res = {}
     for prec in (["fp64", "mixed"] if a.precision == "both" else [a.precision]):
         res[prec] = run_gpu(grid, prec == "mixed", a.steps, a.warmup)
+        print(f"[bench] {prec}: {res[prec][1]:.4f} ms/step", file=sys.stderr, flush=True)
     main_prec = "mixed" if "mixed" in res else "fp64"
     value, ms_step, roof, info = res[main_prec]
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))
     cfg = {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, Ri 1, Pr 0.71, "
                        f"SIMPLE (0.7/0.3), one outer iteration per step, {main_prec} path",
            "grid": [n, n], "precision": main_prec, **info}
@@ -147,6 +305,18 @@ def main():
         cfg["fp64_ms_per_step"] = res["fp64"][1]
         cfg["fp64_roofline"] = res["fp64"][2]
         cfg["mixed_over_fp64"] = res["mixed"][0] / res["fp64"][0]
+    if "batch" in sections:
+        cfg["batched"] = run_batched(main_prec == "mixed")
+        print(f"[bench] batched: {cfg['batched']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    if "serve" in sections:
+        cfg["served"] = run_served(main_prec == "mixed", a.serve_rate, a.serve_count)
+        print(f"[bench] served: {cfg['served']['solves_per_s']:.2f} solves/s", file=sys.stderr, flush=True)
+    if "piso" in sections:
+        cfg["p_isolate"] = run_p_isolate()
+        print("[bench] p' isolate done", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))
     out = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

@@ -73,7 +73,7 @@ SIGNATURES = {
     "cfd_case_bytes": ([_p, C.POINTER(C.c_size_t)], _i),
     "cfd_case_residuals": ([_p, dp, dp, dp, dp], _i),
     "cfd_case_p_apply": ([_p, dp, dp, dp], _i),
-    "cfd_case_p_solve": ([_p, dp, dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
+    "cfd_case_p_solve": ([_p, _p, _p, _p, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
     "cfd_case_time_kernel": ([_p, _i, _i, C.POINTER(_d)], _i),
     "cfd_case_profile": ([_p, _i], _i),
     "cfd_case_profile_read": ([_p, C.POINTER(_d), C.POINTER(C.c_long), _i], _i),
@@ -87,6 +87,14 @@ SIGNATURES = {
     "cfd_batch_get_fields": ([_p, _i, dp, dp, dp, dp], _i),
     "cfd_batch_set_fields": ([_p, _i, dp, dp, dp, dp], _i),
     "cfd_batch_bytes": ([_p, C.POINTER(C.c_size_t)], _i),
+    "cfd_batch_bytes_needed": ([_i, _i, _d, _d, _i, C.POINTER(Params), C.POINTER(C.c_size_t)], _i),
+    "cfd_server_create": ([C.POINTER(Params), C.c_size_t, _i, C.POINTER(_p)], _i),
+    "cfd_server_destroy": ([_p], _i),
+    "cfd_server_submit": ([_p, _i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), _d, _i,
+                          C.POINTER(C.c_long)], _i),
+    "cfd_server_step": ([_p, C.POINTER(_i), C.POINTER(_i)], _i),
+    "cfd_server_poll": ([_p, C.c_long, C.POINTER(_i), C.POINTER(Stats), C.POINTER(_d), _p, _p, _p, _p], _i),
+    "cfd_server_info": ([_p, C.POINTER(_i), C.POINTER(C.c_size_t), C.POINTER(C.c_long)], _i),
 }
 
 _LIB = None

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfd_gpu.so")
-SOURCES = ["cfd_cavity.hip", "cfd_ref.hip"]
+SOURCES = ["cfd_cavity.hip", "cfd_ref.hip", "cfd_server.cpp"]
 ARCH = os.environ.get("CFD_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -88,10 +88,15 @@ class CavityCase:
         call("cfd_case_p_apply", self._h, f64(d), f64(x), y)
         return y
 
-    def p_solve(self, d, q, rtol=1e-10, maxit=200):
-        x = np.zeros((self.n1, self.n0))
+    def p_solve(self, d, q, rtol=1e-10, maxit=200, want_x=True):
+        """Solve L x = q; d or q None keeps the device-resident values (d from the
+        last residual evaluation); want_x False leaves x on the device."""
+        x = np.zeros((self.n1, self.n0)) if want_x else None
         it, rr = C.c_int(), C.c_double()
-        call("cfd_case_p_solve", self._h, f64(d), f64(q), x, rtol, maxit, C.byref(it), C.byref(rr))
+        keep = [f64(a) if a is not None else None for a in (d, q)]
+        call("cfd_case_p_solve", self._h, None if d is None else keep[0].ctypes.data,
+             None if q is None else keep[1].ctypes.data, None if x is None else x.ctypes.data, rtol, maxit,
+             C.byref(it), C.byref(rr))
         return x, it.value, rr.value
 
     KERNELS = ("p_apply", "jacobi", "residual", "face_div", "mg_down", "mg_up")

@@ -87,6 +87,7 @@ struct Case {
     double *scal = nullptr;  // device scalars, SS per slot
     double *hscal = nullptr; // pinned host mirror
     size_t bytes = 0;
+    bool dry = false;    // size estimate only: no device work
     std::vector<void *> allocs;
     // live kernel timing: event pairs around fine-level launches of the hot kernels
     bool prof = false;
@@ -768,6 +769,11 @@ namespace {
 
 template <typename T>
 int dalloc(Case *c, T **p, size_t n) {
+    if (c->dry) {
+        *p = nullptr;
+        c->bytes += n * sizeof(T);
+        return CFD_OK;
+    }
     void *ptr = nullptr;
     if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) return CFD_E_OOM;
     if (hipMemsetAsync(ptr, 0, n * sizeof(T), c->s) != hipSuccess) return CFD_E_HIP;
@@ -816,7 +822,7 @@ void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 size_t psz(const Case *c) { return (size_t)4 * c->pstride; }
 PsCtx ctx(Case *c) {
     return PsCtx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
-                 &c->allocs, &c->bytes};
+                 &c->allocs, &c->bytes, c->dry};
 }
 RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, psz(c), c->scal}; }
 CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd}; }
@@ -1200,20 +1206,23 @@ void batch_destroy(Case *c) {
     (void)hipGetLastError();
 }
 
-int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params, Case **out) {
+// dry: only count the bytes the batch would hold (c->bytes), touching no device
+int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params, Case **out,
+                 bool dry = false) {
     if (!out || !params) return CFD_E_ARG;
     if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0) || slots < 1 || slots > 65535) return CFD_E_ARG;
     if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
     *out = nullptr;
     Case *c = new Case();
+    c->dry = dry;
     c->n0 = n0; c->n1 = n1; c->B = slots; c->N = (size_t)n0 * n1; c->h1 = dxi; c->h2 = deta; c->prm = *params;
     c->mixed = params->precision == CFD_MIXED;
     c->sph.assign(slots, SlotP{});
     c->mass0.assign(slots, 0.0);
     c->energy0.assign(slots, 0.0);
     c->iter.assign(slots, 0);
-    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
-    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
+    if (!dry && hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
+    if (!dry && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
     const size_t BN = (size_t)slots * c->N;
     int rc = CFD_OK;
     double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
@@ -1240,6 +1249,11 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)slots * psz(c));
     if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)slots * SS);
     if (rc == CFD_OK) rc = dalloc(c, &c->spd, (size_t)slots);
+    if (dry) {
+        if (rc == CFD_OK) rc = build_levels(c);
+        *out = c;
+        return rc;
+    }
     if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (size_t)slots * SS * sizeof(double)) != hipSuccess)
         rc = CFD_E_OOM;
     if (rc == CFD_OK) std::memset(c->hscal, 0, (size_t)slots * SS * sizeof(double));
@@ -1356,10 +1370,14 @@ int cfd_batch_release(cfd_batch h, int slot) {
     return put_slot(c, slot);
 }
 
+// a slot whose residuals went non-finite shows it in its statistics; the batch call succeeds
 int cfd_batch_outer(cfd_batch h, int n, cfd_stats *st) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c || n < 0) return CFD_E_ARG;
-    for (int k = 0; k < n; k++) RET(outer_once(c, st));
+    for (int k = 0; k < n; k++) {
+        const int rc = outer_once(c, st);
+        if (rc != CFD_OK && rc != CFD_E_NAN) return rc;
+    }
     return CFD_OK;
 }
 
@@ -1373,7 +1391,8 @@ int cfd_batch_finish(cfd_batch h, cfd_stats *st) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c) return CFD_E_ARG;
     RET(sync_scal(c));
-    return outer_stats(c, st);
+    const int rc = outer_stats(c, st);
+    return rc == CFD_E_NAN ? CFD_OK : rc;
 }
 
 int cfd_batch_get_fields(cfd_batch h, int slot, double *u, double *v, double *T, double *p) {
@@ -1409,6 +1428,20 @@ int cfd_batch_bytes(cfd_batch h, size_t *bytes) {
     if (!c || !bytes) return CFD_E_ARG;
     *bytes = c->bytes;
     return CFD_OK;
+}
+
+int cfd_batch_bytes_needed(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params,
+                           size_t *bytes) {
+    if (!bytes) return CFD_E_ARG;
+    Case *c = nullptr;
+    const int rc = batch_create(n0, n1, dxi, deta, slots, params, &c, true);
+    if (c) {
+        *bytes = c->bytes;
+        delete c->ps64;
+        delete c->ps32;
+        delete c;
+    }
+    return rc;
 }
 
 // ---- single cases: a batch of one slot
@@ -1523,17 +1556,17 @@ int cfd_case_p_apply(cfd_case h, const double *d, const double *x, double *y) {
 int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, double rtol, int maxit,
                      int *iters, double *relres) {
     Case *c = reinterpret_cast<Case *>(h);
-    if (!c || !d || !q || !x) return CFD_E_ARG;
+    if (!c) return CFD_E_ARG;
     const size_t B = c->N * sizeof(double);
-    CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
-    CFD_HIP(hipMemcpyAsync(c->q, q, B, hipMemcpyHostToDevice, c->s));
+    if (d) CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
+    if (q) CFD_HIP(hipMemcpyAsync(c->q, q, B, hipMemcpyHostToDevice, c->s));
     RET(mg_setup(c));
     RET(psolve(c, rtol, maxit, false));
     RET(sync_scal(c));
     int its = 0;
     double rr = 0.0;
     psolve_result(c, 0, &its, &rr);
-    CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
+    if (x) CFD_HIP(hipMemcpyAsync(x, c->pc, B, hipMemcpyDeviceToHost, c->s));
     CFD_HIP(hipStreamSynchronize(c->s));
     if (iters) *iters = its;
     if (relres) *relres = rr;

@@ -1057,6 +1057,7 @@ struct PsCtx {
     void *owner;
     std::vector<void *> *allocs;
     size_t *bytes;
+    bool dry;       // count bytes only (no device allocation)
     RedArgs ra() const { return RedArgs{part, pstride, psz, scal}; }
     // every active slot has its S_DONE flag set (host mirror)
     bool all_done() const {
@@ -1080,6 +1081,11 @@ struct PsCtx {
 enum { PK_APPLY = 0, PK_MG_DOWN = 4, PK_MG_UP = 5 };
 
 inline int ps_alloc(PsCtx &cx, void **p, size_t bytes) {
+    if (cx.dry) {
+        *p = nullptr;
+        *cx.bytes += bytes;
+        return CFD_OK;
+    }
     if (hipMalloc(p, bytes) != hipSuccess) return CFD_E_OOM;
     if (hipMemsetAsync(*p, 0, bytes, cx.s) != hipSuccess) return CFD_E_HIP;
     cx.allocs->push_back(*p);

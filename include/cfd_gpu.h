@@ -213,6 +213,11 @@ int cfd_case_bytes(cfd_case c, size_t *bytes);
 /* Stage-level access for tests (host arrays). */
 int cfd_case_residuals(cfd_case c, double *Ru, double *Rv, double *RT, double *d);
 int cfd_case_p_apply(cfd_case c, const double *d, const double *x, double *y);
+/* p' solve L x = q (x from zero) with the case's solver to ||r|| <= rtol ||q||
+ * (FP64 residual); maxit caps BiCGStab iterations (per refinement step in the
+ * mixed path).  d, q: host arrays, or NULL to keep the case's current device
+ * values (d from the last residual evaluation, q from the last call); x: host
+ * output, or NULL to leave the solution on the device. */
 int cfd_case_p_solve(cfd_case c, const double *d, const double *q, double *x,
                      double rtol, int maxit, int *iters, double *relres);
 
@@ -260,7 +265,9 @@ int cfd_batch_set_case(cfd_batch b, int slot, const double *x, const double *y,
 /* The slot stops taking part in outer iterations (its arrays stay readable). */
 int cfd_batch_release(cfd_batch b, int slot);
 /* n outer iterations of every active slot; st (may be NULL) receives `slots`
- * entries: the last iteration's statistics per slot (zeros for inactive slots). */
+ * entries: the last iteration's statistics per slot (zeros for inactive slots).
+ * A slot whose residuals turned non-finite shows it in its statistics (it keeps
+ * iterating until released); the call itself succeeds. */
 int cfd_batch_outer(cfd_batch b, int n, cfd_stats *st);
 /* The two halves of one cfd_batch_outer iteration: enqueue it on the batch's
  * stream without waiting, then wait and read the statistics (several batches
@@ -271,6 +278,50 @@ int cfd_batch_get_fields(cfd_batch b, int slot, double *u, double *v, double *T,
 int cfd_batch_set_fields(cfd_batch b, int slot, const double *u, const double *v, const double *T,
                          const double *p);
 int cfd_batch_bytes(cfd_batch b, size_t *bytes);
+/* HBM bytes a batch of this shape would hold (no device work). */
+int cfd_batch_bytes_needed(int n0, int n1, double dxi, double deta, int slots,
+                           const cfd_params *params, size_t *bytes);
+
+/* ------------------------------------------------------------------ */
+/* 4. Serving: continuous batching of cavity solves on one device      */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference.  Solves are queued by cfd_server_submit and
+ * advanced by cfd_server_step, one outer iteration of every running solve per
+ * step: queued requests take free slots of the batch of their grid shape
+ * (`slots_per_batch` slots; a new shape gets a batch when it fits
+ * `hbm_budget` bytes, empty batches being released to make room), all batches
+ * are launched before any is awaited, and a solve leaves its slot when its mass
+ * and energy residuals reach `tol`, at `max_outer` iterations, or on a
+ * non-finite residual.  A request's result does not depend on what else is
+ * served.  Batch-wide settings come from the server's params; each request's
+ * physics from its own. */
+typedef struct cfd_server_s *cfd_server;
+
+#define CFD_REQ_QUEUED 0
+#define CFD_REQ_RUNNING 1
+#define CFD_REQ_DONE 2      /* converged to tol                         */
+#define CFD_REQ_NOTCONV 3   /* stopped at max_outer                      */
+#define CFD_REQ_FAILED 4    /* non-finite residual or not servable       */
+
+int cfd_server_create(const cfd_params *params, size_t hbm_budget, int slots_per_batch,
+                      cfd_server *out);
+int cfd_server_destroy(cfd_server s);
+/* Queue a solve (the arrays are copied); *ticket identifies it. */
+int cfd_server_submit(cfd_server s, int n0, int n1, double dxi, double deta,
+                      const double *x, const double *y,
+                      const double *xix, const double *xiy, const double *etx, const double *ety,
+                      const double *alph, const double *beta, const double *gamma,
+                      const double *ajac, const double *p1, const double *q1,
+                      const cfd_params *params, double tol, int max_outer, long *ticket);
+/* One scheduling round; *running / *queued (may be NULL) receive the counts after it. */
+int cfd_server_step(cfd_server s, int *running, int *queued);
+/* State of a request; once finished (DONE/NOTCONV/FAILED) the statistics, the
+ * submit-to-finish latency in seconds and the fields (any pointer may be NULL)
+ * are returned once and the request is forgotten. */
+int cfd_server_poll(cfd_server s, long ticket, int *state, cfd_stats *st, double *latency_s,
+                    double *u, double *v, double *T, double *p);
+/* Batches held, HBM bytes held, requests finished so far (any pointer may be NULL). */
+int cfd_server_info(cfd_server s, int *batches, size_t *bytes, long *finished);
 
 #ifdef __cplusplus
 }

@@ -52,3 +52,19 @@ def test_batch_argument_errors_need_no_gpu():
     assert lib.cfd_batch_create(65, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
     assert lib.cfd_batch_release(None, 0) == -1
 
+
+
+def test_server_queue_needs_no_gpu():
+    """Submitting and polling only queue host copies; no device work happens."""
+    import ctypes as C
+    import numpy as np
+    from cfd_solver_amd.grid import cavity_grid
+    from cfd_solver_amd.serve import CavityServer
+    srv = CavityServer(hbm_budget=1 << 30, slots_per_batch=2)
+    t = srv.submit(cavity_grid(17), tol=1e-6, max_outer=10, Re=50.0)
+    assert srv.poll(t) is None
+    assert srv.info() == dict(batches=0, bytes=0, finished=0)
+    lib = _lib.load()
+    st = C.c_int()
+    assert lib.cfd_server_poll(srv._h, C.c_long(t + 99), C.byref(st), None, None, None, None, None, None) == -1
+    srv.close()
