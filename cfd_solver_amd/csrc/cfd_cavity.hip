@@ -38,6 +38,7 @@ struct SlotP {
     double Re, Ri, Pr, dt, alpha_u, alpha_t, alpha_p;
     double lid, t_hot, t_cold, len_hot, len_cold;
     double active;  // 1: the slot holds a case
+    double dnu, dnt;  // dt / Re, dt / (Re Pr)
 };
 
 // grid shape shared by the slots, and the slots' physics
@@ -46,6 +47,7 @@ struct CGrid {
     double h1, h2;
     size_t N;          // nodes per slot
     const SlotP *sp;   // sp[blockIdx.z]
+    RecipT<double> r;  // reciprocals of the grid constants
     __host__ __device__ KGrid kg() const { return KGrid{n0, n1, h1, h2}; }
 };
 
@@ -141,35 +143,36 @@ __global__ __launch_bounds__(NT) void k_residual(
         const double VP = etx[P] * u[P] + ety[P] * v[P];
         const double *fl[3] = {u, v, T};
         double conv[3];
+        const double ial = 1.0 / al, iga = 1.0 / ga;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const double *f = fl[k];
             const double sc = (k <= 1) ? ph.Re : ph.Re * ph.Pr;
-            const double pec1 = UP * sc * h1 / al;
-            const double pec2 = VP * sc * h2 / ga;
+            const double pec1 = UP * sc * h1 * ial;
+            const double pec2 = VP * sc * h2 * iga;
             const double fm2i = fit_i ? f[P - 2] : 0.0, fp2i = fit_i ? f[P + 2] : 0.0;
             const double fm2j = fit_j ? f[idx(i, j - 2, n0)] : 0.0, fp2j = fit_j ? f[idx(i, j + 2, n0)] : 0.0;
-            const double dxi = conv_deriv(mode_i, fm2i, f[P - 1], f[P], f[P + 1], fp2i, UP, pec1, h1);
-            const double det = conv_deriv(mode_j, fm2j, f[idx(i, j - 1, n0)], f[P], f[idx(i, j + 1, n0)],
-                                          fp2j, VP, pec2, h2);
+            const double dxi = conv_deriv_r(mode_i, fm2i, f[P - 1], f[P], f[P + 1], fp2i, UP, pec1, g.r.ih1);
+            const double det = conv_deriv_r(mode_j, fm2j, f[idx(i, j - 1, n0)], f[P], f[idx(i, j + 1, n0)],
+                                            fp2j, VP, pec2, g.r.ih2);
             conv[k] = UP * dxi + VP * det;
         }
-        const double dp_dxi = (p[P + 1] - p[P - 1]) / (2.0 * h1);
-        const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) / (2.0 * h2);
+        const double dp_dxi = (p[P + 1] - p[P - 1]) * g.r.i2h1;
+        const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) * g.r.i2h2;
         const double dp_dx = xix[P] * dp_dxi + etx[P] * dp_de;
         const double dp_dy = xiy[P] * dp_dxi + ety[P] * dp_de;
         const double dt = ph.dt;
-        const double ru = dt * (-conv[0] - dp_dx) - aref_offdiag(u, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re, hyb);
+        const double ru = dt * (-conv[0] - dp_dx) - aref_offdiag_r(u, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnu, hyb);
         const double rv = dt * (-conv[1] - dp_dy + ph.Ri * T[P]) -
-                          aref_offdiag(v, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re, hyb);
-        const double rt = -dt * conv[2] - aref_offdiag(T, n0, i, j, al, be, ga, pp, qq, h1, h2, dt, ph.Re * ph.Pr, hyb);
+                          aref_offdiag_r(v, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnu, hyb);
+        const double rt = -dt * conv[2] - aref_offdiag_r(T, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnt, hyb);
         Ru[P] = (TR)ru;
         Rv[P] = (TR)rv;
         RT[P] = (TR)rt;
         Ua[P] = (TR)UP;
         Va[P] = (TR)VP;
         double cl[9];
-        alo_coefs<double>(al, be, ga, pp, qq, UP, VP, h1, h2, dt, ph.Re, cl);
+        alo_coefs_r<double>(al, be, ga, pp, qq, UP, VP, g.r, dt, ph.dnu, cl);
         dd[P] = dt * ph.alpha_u / cl[4];
         e_abs = fabs(rt);
         m_abs = fabs(ru) + fabs(rv);
@@ -217,10 +220,11 @@ __global__ __launch_bounds__(NT) void k_jacobi(
     const int n0 = g.n0, n1 = g.n1;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
     const size_t P = idx(i, j, n0);
-    const TR h1 = (TR)g.h1, h2 = (TR)g.h2, dt = (TR)ph.dt;
+    const TR dt = (TR)ph.dt;
+    const RecipT<TR> r = recip_cast<TR>(g.r);
     TR cu[9], ct[9];
-    alo_coefs<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], h1, h2, dt, (TR)ph.Re, cu);
-    alo_coefs<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], h1, h2, dt, (TR)(ph.Re * ph.Pr), ct);
+    alo_coefs_r<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], r, dt, (TR)ph.dnu, cu);
+    alo_coefs_r<TR>(alph[P], beta[P], gamma[P], p1[P], q1[P], Ua[P], Va[P], r, dt, (TR)ph.dnt, ct);
     TR su = Ru[P], sv = Rv[P], st = RT[P];
     if (!FIRST) {
 #pragma unroll
@@ -235,9 +239,10 @@ __global__ __launch_bounds__(NT) void k_jacobi(
                 st -= ct[k] * dT[Q];
             }
     }
-    nu[P] = su * (TR)ph.alpha_u / cu[4];
-    nv[P] = sv * (TR)ph.alpha_u / cu[4];
-    nT[P] = st * (TR)ph.alpha_t / ct[4];
+    const TR iu = (TR)ph.alpha_u / cu[4], it = (TR)ph.alpha_t / ct[4];
+    nu[P] = su * iu;
+    nv[P] = sv * iu;
+    nT[P] = st * it;
 }
 
 // u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u.
@@ -270,8 +275,8 @@ __global__ __launch_bounds__(NT) void k_star(
     us[P] = a;
     vs[P] = b;
     T[P] = T[P] + (double)dT[P];
-    const double dp_dxi = (p[P + 1] - p[P - 1]) / (2.0 * g.h1);
-    const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) / (2.0 * g.h2);
+    const double dp_dxi = (p[P + 1] - p[P - 1]) * g.r.i2h1;
+    const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) * g.r.i2h2;
     uh[P] = a + d[P] * (xix[P] * dp_dxi + etx[P] * dp_de);
     vh[P] = b + d[P] * (xiy[P] * dp_dxi + ety[P] * dp_de);
 }
@@ -290,7 +295,6 @@ __global__ __launch_bounds__(NT) void k_face_div(
     }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
-    const double h1 = g.h1, h2 = g.h2;
     double aq = 0.0;
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
         const int ipp = i + 1, inn = i - 1, jpp = j + 1, jnn = j - 1;
@@ -299,37 +303,37 @@ __global__ __launch_bounds__(NT) void k_face_div(
         double fe = 0.0, fw = 0.0, gn = 0.0, gs = 0.0;
         if (ipp != n0 - 1) {
             const double de = 0.5 * (d[P] + d[E]);
-            const double dpdxi = (p[E] - p[P]) / h1;
-            const double dpde = (p[idx(ipp, jpp, n0)] + p[Nn] - p[S] - p[idx(ipp, jnn, n0)]) / (4.0 * h2);
+            const double dpdxi = (p[E] - p[P]) * g.r.ih1;
+            const double dpde = (p[idx(ipp, jpp, n0)] + p[Nn] - p[S] - p[idx(ipp, jnn, n0)]) * (0.25 * g.r.ih2);
             const double uf = 0.5 * (uh[P] + uh[E]) - 0.5 * de * ((xix[P] + xix[E]) * dpdxi + (etx[P] + etx[E]) * dpde);
             const double vf = 0.5 * (vh[P] + vh[E]) - 0.5 * de * ((xiy[P] + xiy[E]) * dpdxi + (ety[P] + ety[E]) * dpde);
             fe = 0.5 * (jac[P] * xix[P] + jac[E] * xix[E]) * uf + 0.5 * (jac[P] * xiy[P] + jac[E] * xiy[E]) * vf;
         }
         if (inn != 0) {
             const double dw = 0.5 * (d[P] + d[W]);
-            const double dpdxi = (p[P] - p[W]) / h1;
-            const double dpde = (p[Nn] + p[idx(inn, jpp, n0)] - p[S] - p[idx(inn, jnn, n0)]) / (4.0 * h2);
+            const double dpdxi = (p[P] - p[W]) * g.r.ih1;
+            const double dpde = (p[Nn] + p[idx(inn, jpp, n0)] - p[S] - p[idx(inn, jnn, n0)]) * (0.25 * g.r.ih2);
             const double uf = 0.5 * (uh[P] + uh[W]) - 0.5 * dw * ((xix[P] + xix[W]) * dpdxi + (etx[P] + etx[W]) * dpde);
             const double vf = 0.5 * (vh[P] + vh[W]) - 0.5 * dw * ((xiy[P] + xiy[W]) * dpdxi + (ety[P] + ety[W]) * dpde);
             fw = 0.5 * (jac[P] * xix[P] + jac[W] * xix[W]) * uf + 0.5 * (jac[P] * xiy[P] + jac[W] * xiy[W]) * vf;
         }
         if (jpp != n1 - 1) {
             const double dn = 0.5 * (d[P] + d[Nn]);
-            const double dpdxi = (p[idx(ipp, jpp, n0)] - p[idx(inn, jpp, n0)] + p[E] - p[W]) / (4.0 * h1);
-            const double dpde = (p[Nn] - p[P]) / h2;
+            const double dpdxi = (p[idx(ipp, jpp, n0)] - p[idx(inn, jpp, n0)] + p[E] - p[W]) * (0.25 * g.r.ih1);
+            const double dpde = (p[Nn] - p[P]) * g.r.ih2;
             const double uf = 0.5 * (uh[P] + uh[Nn]) - 0.5 * dn * ((xix[P] + xix[Nn]) * dpdxi + (etx[P] + etx[Nn]) * dpde);
             const double vf = 0.5 * (vh[P] + vh[Nn]) - 0.5 * dn * ((xiy[P] + xiy[Nn]) * dpdxi + (ety[P] + ety[Nn]) * dpde);
             gn = 0.5 * (jac[P] * etx[P] + jac[Nn] * etx[Nn]) * uf + 0.5 * (jac[P] * ety[P] + jac[Nn] * ety[Nn]) * vf;
         }
         if (jnn != 0) {
             const double ds = 0.5 * (d[P] + d[S]);
-            const double dpdxi = (p[E] - p[W] + p[idx(ipp, jnn, n0)] - p[idx(inn, jnn, n0)]) / (4.0 * h1);
-            const double dpde = (p[P] - p[S]) / h2;
+            const double dpdxi = (p[E] - p[W] + p[idx(ipp, jnn, n0)] - p[idx(inn, jnn, n0)]) * (0.25 * g.r.ih1);
+            const double dpde = (p[P] - p[S]) * g.r.ih2;
             const double uf = 0.5 * (uh[P] + uh[S]) - 0.5 * ds * ((xix[P] + xix[S]) * dpdxi + (etx[P] + etx[S]) * dpde);
             const double vf = 0.5 * (vh[P] + vh[S]) - 0.5 * ds * ((xiy[P] + xiy[S]) * dpdxi + (ety[P] + ety[S]) * dpde);
             gs = 0.5 * (jac[P] * etx[P] + jac[S] * etx[S]) * uf + 0.5 * (jac[P] * ety[P] + jac[S] * ety[S]) * vf;
         }
-        const double qq = (fe - fw) / h1 + (gn - gs) / h2;
+        const double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
         q[P] = qq;
         aq = fabs(qq);
     }
@@ -359,8 +363,8 @@ __global__ __launch_bounds__(NT) void k_correct(
     const size_t P = idx(i, j, n0);
     const double pe = pc[idx(clampi(i + 1, 1, n0 - 2), j, n0)] - m, pw = pc[idx(clampi(i - 1, 1, n0 - 2), j, n0)] - m;
     const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)] - m, ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)] - m;
-    const double dpx = 0.5 * (pe - pw) / g.h1;
-    const double dpe = 0.5 * (pn - ps) / g.h2;
+    const double dpx = 0.5 * (pe - pw) * g.r.ih1;
+    const double dpe = 0.5 * (pn - ps) * g.r.ih2;
     u[P] = us[P] - d[P] * (xix[P] * dpx + etx[P] * dpe);
     v[P] = vs[P] - d[P] * (xiy[P] * dpx + ety[P] * dpe);
     p[P] = p[P] + alpha_p * (pc[P] - m);
@@ -825,7 +829,7 @@ PsCtx ctx(Case *c) {
                  &c->allocs, &c->bytes, c->dry};
 }
 RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, psz(c), c->scal}; }
-CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd}; }
+CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd, make_recip(c->h1, c->h2)}; }
 dim3 grid_b(const Case *c) {
     dim3 g = grid2d(c->n0, c->n1);
     g.z = c->B;
@@ -1303,6 +1307,8 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
     sp.Re = phys->Re; sp.Ri = phys->Ri; sp.Pr = phys->Pr; sp.dt = phys->dt;
     sp.alpha_u = phys->alpha_u; sp.alpha_t = phys->alpha_t; sp.alpha_p = phys->alpha_p;
     sp.lid = phys->lid_u; sp.t_hot = phys->t_hot; sp.t_cold = phys->t_cold;
+    sp.dnu = phys->dt / phys->Re;
+    sp.dnt = phys->dt / (phys->Re * phys->Pr);
     sp.len_hot = sp.len_cold = 0.0;
     for (int i = 0; i < n0 - 1; i++) {
         const size_t a = (size_t)i, b = (size_t)i + 1;

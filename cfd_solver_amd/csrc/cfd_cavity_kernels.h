@@ -1,6 +1,10 @@
 // Device-side stencil formulas of the cavity SIMPLE path (gfx950).
 //
-// Every formula here is the one the CPU oracle uses, node for node:
+// Every formula here is the one the CPU oracle uses, node for node, with one
+// change of form: a division by a grid or physics constant (h1, h2, Re, Re Pr,
+// their products) is a multiplication by its reciprocal, computed once on the
+// host (Recip, and dt/nd per slot).  The results differ from the oracle's in the
+// last bits only; a division by a per-node value stays a division.
 //  * conv_deriv   -- hybrid 4th-order central / Kawamura-Kuwahara upwind switch
 //                    on the cell Peclet number (solver.cpp:1120-1173);
 //  * aref_offdiag -- A_ref phi - phi of the implicit diffusion operator
@@ -15,12 +19,103 @@
 
 namespace cfd {
 
+__device__ __forceinline__ size_t idx(int i, int j, int n0) { return (size_t)j * (size_t)n0 + (size_t)i; }
+
 struct KGrid {
     int n0, n1;
     double h1, h2;
 };
 
-__device__ __forceinline__ size_t idx(int i, int j, int n0) { return (size_t)j * (size_t)n0 + (size_t)i; }
+// reciprocals of the grid constants (host-computed)
+template <typename R>
+struct RecipT {
+    R ih1, ih2;       // 1/h1, 1/h2
+    R ih1sq, ih2sq;   // 1/h1^2, 1/h2^2
+    R i2h1, i2h2;     // 1/(2 h1), 1/(2 h2)
+    R ih1h2, i2h1h2;  // 1/(h1 h2), 1/(2 h1 h2)
+};
+template <typename R, typename S>
+__host__ __device__ inline RecipT<R> recip_cast(const RecipT<S> &a) {
+    return RecipT<R>{(R)a.ih1, (R)a.ih2, (R)a.ih1sq, (R)a.ih2sq, (R)a.i2h1, (R)a.i2h2, (R)a.ih1h2, (R)a.i2h1h2};
+}
+inline RecipT<double> make_recip(double h1, double h2) {
+    return RecipT<double>{1.0 / h1, 1.0 / h2, 1.0 / (h1 * h1), 1.0 / (h2 * h2), 1.0 / (2.0 * h1), 1.0 / (2.0 * h2),
+                          1.0 / (h1 * h2), 1.0 / (2.0 * h1 * h2)};
+}
+
+// conv_deriv with the reciprocal ih = 1/h; the upwind branch divides by vel
+// only symbolically: (vel D4 + |vel| D4') / vel = D4 + sign(vel) D4'
+__device__ __forceinline__ double conv_deriv_r(int mode, double fm2, double fm1, double f0, double fp1,
+                                               double fp2, double vel, double pec, double ih) {
+    if (mode == 0) return 0.5 * (fp1 - fm1) * ih;
+    if (mode == 1 || (pec <= 2 && pec > -2)) {
+        const double xpp = 8.0 * (fp1 - fm1);
+        const double xnn = fp2 - fm2;
+        return (1.0 / 12.0) * (xpp - xnn) * ih;
+    }
+    const double d4 = (-fp2 + 8 * fp1 - 8 * fm1 + fm2) * ((1.0 / 12.0) * ih);
+    const double d4p = (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) * (0.25 * ih);
+    return vel > 0.0 ? d4 + d4p : d4 - d4p;
+}
+
+// (A_ref - I) phi at interior node (i, j) with dn = dt / nd (nd = Re or Re Pr)
+__device__ __forceinline__ double aref_offdiag_r(const double *__restrict__ f, int n0, int i, int j, double al,
+                                                 double be, double ga, double pp, double qq,
+                                                 const RecipT<double> &r, double dn, bool wide) {
+#define F(di, dj) f[idx(i + (di), j + (dj), n0)]
+    const double A = al * r.ih1sq, G = ga * r.ih2sq;
+    if (!wide) {
+        const double Pp = pp * r.i2h1, Qq = qq * r.i2h2;
+        const double ce = -dn * (A + Pp), cw = -dn * (A - Pp);
+        const double cn = -dn * (G + Qq), cs = -dn * (G - Qq);
+        const double cx = dn * be * r.i2h1h2;
+        const double cp = 2.0 * dn * (A + G);
+        return cp * F(0, 0) + ce * F(1, 0) + cw * F(-1, 0) + cn * F(0, 1) + cs * F(0, -1) +
+               cx * (F(1, 1) + F(-1, -1) - F(-1, 1) - F(1, -1));
+    }
+    const double Pi = pp * r.ih1, Qi = qq * r.ih2, B = be * r.ih1h2;
+    const double ce = -dn * ((4.0 / 3.0) * A + (2.0 / 3.0) * Pi);
+    const double cw = -dn * ((4.0 / 3.0) * A - (2.0 / 3.0) * Pi);
+    const double cn = -dn * ((4.0 / 3.0) * G + (2.0 / 3.0) * Qi);
+    const double cs = -dn * ((4.0 / 3.0) * G - (2.0 / 3.0) * Qi);
+    const double cne = dn * (8.0 / 9.0) * B;
+    const double cnw = -cne;
+    const double cnn = dn * ((1.0 / 12.0) * G + (1.0 / 12.0) * Qi);
+    const double css = dn * ((1.0 / 12.0) * G - (1.0 / 12.0) * Qi);
+    const double cee = dn * ((1.0 / 12.0) * A + (1.0 / 12.0) * Pi);
+    const double cww = dn * ((1.0 / 12.0) * A - (1.0 / 12.0) * Pi);
+    const double c22 = dn * (1.0 / 72.0) * B;
+    const double c2m = -c22;
+    const double c12 = -dn * (1.0 / 9.0) * B;
+    const double c1m = -c12;
+    const double cp = 2.5 * dn * (A + G);
+    return cp * F(0, 0) + ce * F(1, 0) + cw * F(-1, 0) + cn * F(0, 1) + cs * F(0, -1) +
+           cne * (F(1, 1) + F(-1, -1)) + cnw * (F(-1, 1) + F(1, -1)) +
+           cnn * F(0, 2) + css * F(0, -2) + cee * F(2, 0) + cww * F(-2, 0) +
+           c22 * (F(2, 2) + F(-2, -2)) + c2m * (F(-2, 2) + F(2, -2)) +
+           c12 * (F(1, 2) + F(-1, -2) + F(2, 1) + F(-2, -1)) +
+           c1m * (F(-1, 2) + F(1, -2) + F(-2, 1) + F(2, -1));
+#undef F
+}
+
+// low-order predictor row with reciprocals: dn = dt / nd
+template <typename R>
+__device__ __forceinline__ void alo_coefs_r(R al, R be, R ga, R pp, R qq, R U, R V, const RecipT<R> &r, R dt, R dn,
+                                            R c[9]) {
+    const R zero = R(0);
+    const R A = al * r.ih1sq, G = ga * r.ih2sq, Pp = pp * r.i2h1, Qq = qq * r.i2h2;
+    c[5] = -dn * (A + Pp) + dt * fmin(U, zero) * r.ih1;
+    c[3] = -dn * (A - Pp) - dt * fmax(U, zero) * r.ih1;
+    c[7] = -dn * (G + Qq) + dt * fmin(V, zero) * r.ih2;
+    c[1] = -dn * (G - Qq) - dt * fmax(V, zero) * r.ih2;
+    const R cx = dn * be * r.i2h1h2;
+    c[8] = cx;
+    c[0] = cx;
+    c[6] = -cx;
+    c[2] = -cx;
+    c[4] = R(1) + R(2) * dn * (A + G) + dt * (fabs(U) * r.ih1 + fabs(V) * r.ih2);
+}
+
 
 // mode 2: hybrid, 1: 4th-order central, 0: 2nd-order central (solver.cpp:1124-1172)
 __device__ __forceinline__ double conv_deriv(int mode, double fm2, double fm1, double f0, double fp1,
