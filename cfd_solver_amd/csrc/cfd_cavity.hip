@@ -719,8 +719,11 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
 
 // totals of the case-level reductions and what follows from them (one block)
 enum { CF_RES, CF_MASS, CF_Q, CF_DEFECT, CF_PSUM };
+// margin: fraction of the remaining FP64 tolerance the first FP32 solve aims at
+// (1 when no further refinement step follows it)
 template <int OP>
-__global__ __launch_bounds__(NT) void k_cav_fin(int gs, RedArgs ra, int nparts, int n0, int n1, double rtol) {
+__global__ __launch_bounds__(NT) void k_cav_fin(int gs, RedArgs ra, int nparts, int n0, int n1, double rtol,
+                                                double margin) {
     ra = rz(ra);
     if (slot_skip(ra.scal, gs)) return;
     constexpr int NV = (OP == CF_MASS || OP == CF_PSUM) ? 1 : 2;
@@ -748,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_cav_fin(int gs, RedArgs ra, int nparts, 
         s[S_RELRES64] = bn > 0.0 ? rn / bn : 0.0;
         s[S_ITTOT] = 0.0;
         s[S_RDONE] = (bn == 0.0 || rn <= rtol * bn) ? 1.0 : 0.0;
-        s[S_RTOL] = rn > 0.0 ? fmax(0.5 * rtol * bn / rn, 1e-5) : 1e-5;
+        s[S_RTOL] = rn > 0.0 ? fmax(margin * rtol * bn / rn, 1e-5) : 1e-5;
     } else {
         // refinement test on the FP64 defect (mean removed) and the next inner tolerance
         s[S_ITTOT] += fmax(s[S_IT], 1.0);
@@ -838,8 +841,8 @@ dim3 grid_b(const Case *c) {
 
 // one-block-per-slot finish of a case-level reduction over the grid's block partials
 template <int OP>
-int cav_fin(Case *c, int gs = -1, double rtol = 0.0) {
-    k_cav_fin<OP><<<dim3(1, 1, c->B), NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol);
+int cav_fin(Case *c, int gs = -1, double rtol = 0.0, double margin = 0.5) {
+    k_cav_fin<OP><<<dim3(1, 1, c->B), NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol, margin);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -910,7 +913,9 @@ int mg_setup(Case *c) {
 // FP64: BiCGStab + multigrid in FP64.  Mixed: FP64 iterative refinement around
 // FP32 BiCGStab + multigrid solves, the FP64 defect (mean removed) deciding
 // convergence; one refinement step when rtol >= 1e-4 (the FP32 solve alone can
-// reach it), up to six otherwise.  graph: capturable (no host synchronisation).
+// reach it, so it aims at rtol itself), up to six otherwise (each FP32 solve
+// aiming at half the remaining tolerance).  graph: capturable (no host
+// synchronisation).
 // Iterations and relative residual: S_IT / S_RELRES (FP64), S_ITTOT / S_RELRES64.
 int psolve(Case *c, double rtol, int maxit, bool graph) {
     PsCtx cx = ctx(c);
@@ -926,7 +931,7 @@ int psolve(Case *c, double rtol, int maxit, bool graph) {
     const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
     k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
     CFD_LAUNCH_CHECK();
-    RET(cav_fin<CF_Q>(c, -1, rtol));
+    RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5));
     for (int k = 0; k < R; k++) {
         if (!graph && k > 0) {
             RET(ps_sync(cx));
