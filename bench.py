@@ -46,7 +46,7 @@ def kernel_bytes(name, mixed):
         "p_apply": 7 * w,          # x, 4 face coefficients, r0 (or s) -> y; dot product in the kernel
         "mg_down": 7.25 * w,       # b, 1/diag, 4 face coefficients -> x, restricted residual (1/4 node)
         "mg_up": 8.25 * w,         # x, b, 1/diag, 4 faces, coarse x (1/4 node) -> x
-        "jacobi": 16 * w,          # R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
+        "jacobi": 16 * w,          # two sweeps: R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
         "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
         "face_div": 10 * 8,        # u^,v^,p,d, 4 metrics, J -> q (FP64)
     }[name]
@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--sections", default="main,batch,serve,piso")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="served arrivals per second")
     ap.add_argument("--serve-count", type=int, default=200)
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
+                         "rocprof's FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -289,6 +292,14 @@ def main():
         sys.exit(2)
     from cfd_solver_amd.grid import cavity_grid
     sections = set(a.sections.split(","))
+    if a.copy_check > 0:
+        import torch
+        n = a.copy_check << 18  # doubles
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy check: {a.copy_check} MiB device-to-device", file=sys.stderr, flush=True)
     n = a.n
     grid = cavity_grid(n, skew=0.1, beta=1.5)
     res = {}

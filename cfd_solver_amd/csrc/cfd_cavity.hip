@@ -245,6 +245,97 @@ __global__ __launch_bounds__(NT) void k_jacobi(
     nT[P] = st * it;
 }
 
+// Two Jacobi sweeps per launch: the first sweep is computed on the block's tile
+// plus a one-node halo into LDS, the second on the tile from there, so a pair of
+// sweeps reads the coefficient arrays and the old correction once.  Node for node
+// the arithmetic of k_jacobi (same order), so a pair equals two single sweeps.
+template <typename TR>
+struct JArgs {
+    const TR *Ru, *Rv, *RT, *Ua, *Va, *alph, *beta, *gamma, *p1, *q1, *du, *dv, *dT;
+    TR *nu, *nv, *nT;
+};
+
+constexpr int JT_X = 64, JT_Y = 16;  // output tile of k_jacobi2
+
+// one sweep at interior node (i, j) = P; `dat(f, i, j)` reads the old correction
+// of field f (0 u, 1 v, 2 T) at a neighbour
+template <typename TR, bool FIRST, typename F>
+__device__ __forceinline__ void jac_node(const JArgs<TR> &a, const SlotP &ph, const RecipT<TR> &r, size_t P, int i,
+                                         int j, F dat, TR &ou, TR &ov, TR &ot) {
+    const TR dt = (TR)ph.dt;
+    TR cu[9], ct[9];
+    alo_coefs_r<TR>(a.alph[P], a.beta[P], a.gamma[P], a.p1[P], a.q1[P], a.Ua[P], a.Va[P], r, dt, (TR)ph.dnu, cu);
+    alo_coefs_r<TR>(a.alph[P], a.beta[P], a.gamma[P], a.p1[P], a.q1[P], a.Ua[P], a.Va[P], r, dt, (TR)ph.dnt, ct);
+    TR su = a.Ru[P], sv = a.Rv[P], st = a.RT[P];
+    if (!FIRST) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+            for (int di = -1; di <= 1; di++) {
+                if (di == 0 && dj == 0) continue;
+                const int k = (dj + 1) * 3 + (di + 1);
+                su -= cu[k] * dat(0, i + di, j + dj);
+                sv -= cu[k] * dat(1, i + di, j + dj);
+                st -= ct[k] * dat(2, i + di, j + dj);
+            }
+    }
+    const TR iu = (TR)ph.alpha_u / cu[4], it = (TR)ph.alpha_t / ct[4];
+    ou = su * iu;
+    ov = sv * iu;
+    ot = st * it;
+}
+
+template <typename TR, bool FIRST>
+__global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
+    const SlotP ph = g.sp[blockIdx.z];
+    if (ph.active == 0.0) return;
+    {
+        const size_t o = zslot() * g.N;
+        a.Ru += o; a.Rv += o; a.RT += o; a.Ua += o; a.Va += o; a.alph += o; a.beta += o; a.gamma += o;
+        a.p1 += o; a.q1 += o; a.du += o; a.dv += o; a.dT += o; a.nu += o; a.nv += o; a.nT += o;
+    }
+    constexpr int AX = JT_X + 4, AY = JT_Y + 4;  // old correction, halo 2
+    constexpr int MX = JT_X + 2, MY = JT_Y + 2;  // first sweep, halo 1
+    constexpr int NA = AX * AY, NM = MX * MY;
+    __shared__ TR s_old[FIRST ? 1 : 3 * NA];
+    __shared__ TR s_mid[3 * NM];
+    const int tid = tlin();
+    const int n0 = g.n0, n1 = g.n1;
+    const int fi0 = blockIdx.x * JT_X, fj0 = blockIdx.y * JT_Y;
+    const RecipT<TR> r = recip_cast<TR>(g.r);
+    if (!FIRST) {
+        const TR *src[3] = {a.du, a.dv, a.dT};
+        for (int k = tid; k < NA; k += NT) {
+            const int gi = fi0 - 2 + k % AX, gj = fj0 - 2 + k / AX;
+            const bool in = gi >= 0 && gi < n0 && gj >= 0 && gj < n1;
+#pragma unroll
+            for (int f = 0; f < 3; f++) s_old[f * NA + k] = in ? src[f][idx(gi, gj, n0)] : TR(0);
+        }
+        __syncthreads();
+    }
+    auto old_at = [&](int f, int ii, int jj) { return s_old[f * NA + (jj - fj0 + 2) * AX + (ii - fi0 + 2)]; };
+    for (int k = tid; k < NM; k += NT) {
+        const int gi = fi0 - 1 + k % MX, gj = fj0 - 1 + k / MX;
+        TR ou = TR(0), ov = TR(0), ot = TR(0);
+        if (inner_node(gi, gj, n0, n1)) jac_node<TR, FIRST>(a, ph, r, idx(gi, gj, n0), gi, gj, old_at, ou, ov, ot);
+        s_mid[k] = ou;
+        s_mid[NM + k] = ov;
+        s_mid[2 * NM + k] = ot;
+    }
+    __syncthreads();
+    auto mid_at = [&](int f, int ii, int jj) { return s_mid[f * NM + (jj - fj0 + 1) * MX + (ii - fi0 + 1)]; };
+    for (int k = tid; k < JT_X * JT_Y; k += NT) {
+        const int gi = fi0 + k % JT_X, gj = fj0 + k / JT_X;
+        if (!inner_node(gi, gj, n0, n1)) continue;
+        const size_t P = idx(gi, gj, n0);
+        TR ou, ov, ot;
+        jac_node<TR, false>(a, ph, r, P, gi, gj, mid_at, ou, ov, ot);
+        a.nu[P] = ou;
+        a.nv[P] = ov;
+        a.nT[P] = ot;
+    }
+}
+
 // u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u.
 // Also completes d on the border (written there by no other kernel of the iteration).
 template <typename TR>
@@ -994,15 +1085,27 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
     const TR *Ru = (const TR *)c->Ru, *Rv = (const TR *)c->Rv, *RT = (const TR *)c->RT;
     const TR *U = (const TR *)c->U, *V = (const TR *)c->V;
     const dim3 gr = grid_b(c), bl(BX, BY);
-    for (int s = 0; s < c->prm.mom_sweeps; s++) {
-        if (s == 0) {
+    const dim3 g2((c->n0 + JT_X - 1) / JT_X, (c->n1 + JT_Y - 1) / JT_Y, c->B);
+    const int m = c->prm.mom_sweeps;
+    for (int s = 0; s < m;) {
+        if (m - s >= 2) {  // two sweeps in one launch
+            const JArgs<TR> ja{Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0], a[1], a[2], b[0], b[1], b[2]};
+            if (s == 0) {
+                k_jacobi2<TR, true><<<g2, NT, 0, c->s>>>(cg(c), ja);
+            } else {
+                prof_begin(c, 1);
+                k_jacobi2<TR, false><<<g2, NT, 0, c->s>>>(cg(c), ja);
+                prof_end(c);
+            }
+            s += 2;
+        } else if (s == 0) {
             k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
                                                     a[1], a[2], b[0], b[1], b[2]);
+            s += 1;
         } else {
-            prof_begin(c, 1);
             k_jacobi<TR, false><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
                                                      a[1], a[2], b[0], b[1], b[2]);
-            prof_end(c);
+            s += 1;
         }
         CFD_LAUNCH_CHECK();
         for (int k = 0; k < 3; k++) std::swap(a[k], b[k]);
