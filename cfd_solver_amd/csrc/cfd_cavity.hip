@@ -1450,7 +1450,7 @@ void cfd_params_default(cfd_params *p) {
     p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 4;
     p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 30; p->mg_omega = 0.8;
     p->precision = CFD_FP64;
-    p->graph = 1;
+    p->graph = 0;
 }
 
 // ---- batches

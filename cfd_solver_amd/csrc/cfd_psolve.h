@@ -1067,14 +1067,6 @@ struct PsCtx {
         }
         return true;
     }
-    // some active slot stopped at the ||s|| test of iteration it (host mirror)
-    bool any_sconv(int it) const {
-        for (int z = 0; z < B; z++) {
-            const double *h = hscal + (size_t)z * SS;
-            if (h[S_ACTIVE] != 0.0 && h[S_SCONV] == (double)it) return true;
-        }
-        return false;
-    }
 };
 
 // live-timing kinds of the solver's fine-level kernels
@@ -1264,9 +1256,11 @@ struct PSolver {
     // per slot, x cleared first (acc += x before clearing when acc != null);
     // skipped for slots whose scalar `og` (>= 0) is set.  Iteration for iteration
     // the oracle's bicgstab().  graph: no host synchronisation, `maxit` iterations
-    // launched; otherwise the host reads the device flags after each
-    // half-iteration and stops launching once every slot is done.  The iteration
-    // count and relative residual are left in S_IT / S_RELRES of each slot.
+    // launched; otherwise the host reads the device flags at the end of each
+    // iteration and stops launching once every slot is done (a slot that stops
+    // at the start or at the ||s|| test turns the rest of its iteration's
+    // launches into no-ops).  The iteration count and relative residual are
+    // left in S_IT / S_RELRES of each slot.
     int solve(PsCtx &cx, const double *b, T *x, double *acc, double rtol, int rtol_slot, int maxit, int og,
               bool graph) {
         OpLevel<T> &L0 = lev[0];
@@ -1283,10 +1277,6 @@ struct PSolver {
         k_bi_start2<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
         k_bi_fin<FIN_START2><<<g1, PS_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
-        if (!graph) {
-            CFD_RET(ps_sync(cx));
-            if (cx.all_done()) return CFD_OK;
-        }
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
@@ -1298,14 +1288,6 @@ struct PSolver {
             k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
             k_bi_fin<FIN_S><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
-            if (!graph) {
-                CFD_RET(ps_sync(cx));
-                if (cx.all_done()) {
-                    if (cx.any_sconv(it)) k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
-                    CFD_LAUNCH_CHECK();
-                    return CFD_OK;
-                }
-            }
             CFD_RET(precond(cx, s, sh, S_DONE));
             cx.prof_b(cx.owner, PK_APPLY);
             k_bi_t<T><<<gr, bl, 0, cx.s>>>(a0, sh, t, s, ra);

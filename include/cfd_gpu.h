@@ -164,9 +164,12 @@ typedef struct {
     int mg_pre, mg_post, mg_coarse;
     double mg_omega;    /* damping of the Jacobi smoother                        */
     int precision;      /* CFD_FP64 or CFD_MIXED                                 */
-    int graph;          /* 1: capture one outer iteration as a hipGraph and replay it;
+    int graph;          /* 0 (default): launch eagerly, reading the p' solver's device
+                           flags once per BiCGStab iteration to stop launching;
+                           1: capture one outer iteration as a hipGraph and replay it;
                            the p' BiCGStab then runs p_maxit unrolled iterations with
-                           device-side convergence tests (no host synchronisation)  */
+                           device-side convergence tests (no host synchronisation,
+                           so batches on several streams overlap)                   */
 } cfd_params;
 
 typedef struct {
