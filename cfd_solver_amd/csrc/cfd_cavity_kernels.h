@@ -282,28 +282,47 @@ __device__ __forceinline__ void p9_face(int n0, int n1, const T *__restrict__ xi
     }
 }
 
-// Row (i, j) of the pressure-correction operator from stored face coefficients
-// (fe_*: face to the east of a node, fn_*: face to the north), accumulated in
-// the order of p9_coefs.
+// The eight face coefficients row (i, j) of the pressure-correction operator
+// uses (fe_*: face to the east of a node, fn_*: face to the north): the node's
+// own east and north faces and those of its west and south neighbours.
+template <typename T>
+struct RowFaces {
+    T en, et, wn, wt, nn, nt, sn, st;
+};
+
+template <typename T>
+__device__ __forceinline__ RowFaces<T> load_row_faces(int n0, const T *__restrict__ fen,
+                                                      const T *__restrict__ fet, const T *__restrict__ fnn,
+                                                      const T *__restrict__ fnt, int i, int j) {
+    const size_t P = idx(i, j, n0);
+    return {fen[P], fet[P], fen[P - 1], fet[P - 1], fnn[P], fnt[P], fnn[P - n0], fnt[P - n0]};
+}
+
+// Row coefficients from the eight faces, accumulated in the order of p9_coefs.
+template <typename T>
+__device__ __forceinline__ void p9_row_from(const RowFaces<T> &f, T c[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) c[k] = T(0);
+    T kn = f.en, kt = f.et;
+    c[5] += kn; c[4] -= kn;
+    c[8] += kt; c[7] += kt; c[1] -= kt; c[2] -= kt;
+    kn = f.wn; kt = f.wt;
+    c[4] -= kn; c[3] += kn;
+    c[7] -= kt; c[6] -= kt; c[1] += kt; c[0] += kt;
+    kn = f.nn; kt = f.nt;
+    c[7] += kn; c[4] -= kn;
+    c[8] += kt; c[6] -= kt; c[5] += kt; c[3] -= kt;
+    kn = f.sn; kt = f.st;
+    c[4] -= kn; c[1] += kn;
+    c[5] -= kt; c[3] += kt; c[2] -= kt; c[0] += kt;
+}
+
+// Row (i, j) of the pressure-correction operator from stored face coefficients.
 template <typename T>
 __device__ __forceinline__ void p9_row_faces(int n0, const T *__restrict__ fen, const T *__restrict__ fet,
                                              const T *__restrict__ fnn, const T *__restrict__ fnt, int i, int j,
                                              T c[9]) {
-    const size_t P = idx(i, j, n0);
-#pragma unroll
-    for (int k = 0; k < 9; k++) c[k] = T(0);
-    T kn = fen[P], kt = fet[P];
-    c[5] += kn; c[4] -= kn;
-    c[8] += kt; c[7] += kt; c[1] -= kt; c[2] -= kt;
-    kn = fen[P - 1]; kt = fet[P - 1];
-    c[4] -= kn; c[3] += kn;
-    c[7] -= kt; c[6] -= kt; c[1] += kt; c[0] += kt;
-    kn = fnn[P]; kt = fnt[P];
-    c[7] += kn; c[4] -= kn;
-    c[8] += kt; c[6] -= kt; c[5] += kt; c[3] -= kt;
-    kn = fnn[P - n0]; kt = fnt[P - n0];
-    c[4] -= kn; c[1] += kn;
-    c[5] -= kt; c[3] += kt; c[2] -= kt; c[0] += kt;
+    p9_row_from<T>(load_row_faces<T>(n0, fen, fet, fnn, fnt, i, j), c);
 }
 
 }  // namespace cfd

@@ -142,19 +142,59 @@ __device__ __forceinline__ void sum_partials(const double *part, int pstride, in
 
 // Row (i, j) of the stored operator applied to x, with x read through `xat`
 // at the neighbours clamped into the interior (Neumann ghost).
+template <typename T>
+__device__ __forceinline__ RowFaces<T> row_faces(const FArgs<T> &a, int i, int j) {
+    return load_row_faces<T>(a.n0, a.fen, a.fet, a.fnn, a.fnt, i, j);
+}
+
+// the same row from faces already loaded
 template <typename T, typename F>
-__device__ __forceinline__ T frow_f(const FArgs<T> &a, int i, int j, F xat) {
+__device__ __forceinline__ T frow_v(const RowFaces<T> &f, int n0, int n1, int i, int j, F xat) {
     T c[9];
-    p9_row_faces<T>(a.n0, a.fen, a.fet, a.fnn, a.fnt, i, j, c);
+    p9_row_from<T>(f, c);
     T s = T(0);
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++)
 #pragma unroll
         for (int di = -1; di <= 1; di++) {
-            const int ii = clampi(i + di, 1, a.n0 - 2), jj = clampi(j + dj, 1, a.n1 - 2);
+            const int ii = clampi(i + di, 1, n0 - 2), jj = clampi(j + dj, 1, n1 - 2);
             s += c[(dj + 1) * 3 + (di + 1)] * xat(ii, jj);
         }
     return s;
+}
+
+template <typename T, typename F>
+__device__ __forceinline__ T frow_f(const FArgs<T> &a, int i, int j, F xat) {
+    return frow_v<T>(row_faces<T>(a, i, j), a.n0, a.n1, i, j, xat);
+}
+
+// Loads of a strided block loop issued together: element k = tid + m * NT of a
+// COUNT-element region gets load(k) for every m before any is used, so a staging
+// phase costs one memory round trip instead of one per iteration.  The loads are
+// unconditional (no branch between them): load(k) must read valid addresses for
+// every k < COUNT (clamped ones where the element is unused), and k past the
+// region is clamped to COUNT - 1.  body(k, r) then runs for k < COUNT in order.
+template <typename R, int NIT>
+struct Batch {
+    R r[NIT];
+};
+
+template <int COUNT, int NT, typename L>
+__device__ __forceinline__ auto batch_load(int tid, L load) {
+    constexpr int NIT = (COUNT + NT - 1) / NT;
+    Batch<decltype(load(0)), NIT> b;
+#pragma unroll
+    for (int m = 0; m < NIT; m++) b.r[m] = load(min(tid + m * NT, COUNT - 1));
+    return b;
+}
+
+template <int COUNT, int NT, typename R, int NIT, typename B>
+__device__ __forceinline__ void batch_run(int tid, const Batch<R, NIT> &b, B body) {
+#pragma unroll
+    for (int m = 0; m < NIT; m++) {
+        const int k = tid + m * NT;
+        if (k < COUNT) body(k, b.r[m]);
+    }
 }
 
 template <typename T>
@@ -353,6 +393,15 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
     const int n0 = A.a.n0, n1 = A.a.n1;
     const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
     const T w = A.w;
+    // right-hand side (and 1/diag for a first sweep staged here), halo HB
+    struct BI {
+        T b, id;
+    };
+    const auto lb = batch_load<TL::NB, PS_NT>(tid, [&](int k) {
+        const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
+        const size_t P = idx(clampi(gi, 1, n0 - 2), clampi(gj, 1, n1 - 2), n0);
+        return BI{A.bsrc[P], NPRE == 2 ? A.invd[P] : T(0)};
+    });
     double mean = 0.0;
     if (A.bpart) {
         double tot[1];
@@ -362,63 +411,64 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         __syncthreads();
         mean = s_mean;
     }
-    // right-hand side with its mean removed, halo HB
-    for (int k = tid; k < TL::NB; k += PS_NT) {
+    // mean-free right-hand side; first sweep from zero (no neighbours) in place
+    batch_run<TL::NB, PS_NT>(tid, lb, [&](int k, const BI &r) {
         const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            const T raw = A.bsrc[idx(gi, gj, n0)];
-            v = A.bpart ? (T)((double)raw - mean) : raw;
-        }
-        s_b[k] = v;
-    }
+        const bool in = inner_node(gi, gj, n0, n1);
+        const T bv = in ? (A.bpart ? (T)((double)r.b - mean) : r.b) : T(0);
+        s_b[k] = bv;
+        if (NPRE == 2) s_x1[k] = in ? w * bv * r.id : T(0);
+    });
     __syncthreads();
-    if (NPRE == 2) {
-        for (int k = tid; k < TL::NB; k += PS_NT) {
-            const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
-            s_x1[k] = inner_node(gi, gj, n0, n1) ? w * s_b[k] * A.invd[idx(gi, gj, n0)] : T(0);
-        }
-        __syncthreads();
-    }
     if (NPRE >= 1) {
         constexpr int HB = TL::HB, BWX = TL::BWX;
-        for (int k = tid; k < TL::SX * TL::SY; k += PS_NT) {
+        struct XR {
+            RowFaces<T> f;
+            T id;
+        };
+        const auto lx = batch_load<TL::SX * TL::SY, PS_NT>(tid, [&](int k) {
+            const int gi = clampi(fi0 - 2 + k % TL::SX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / TL::SX, 1, n1 - 2);
+            XR r;
+            r.id = A.invd[idx(gi, gj, n0)];
+            if (NPRE == 2) r.f = row_faces<T>(A.a, gi, gj);
+            return r;
+        });
+        batch_run<TL::SX * TL::SY, PS_NT>(tid, lx, [&](int k, const XR &r) {
             const int li = k % TL::SX, lj = k / TL::SX;
             const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
             const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
             T v = T(0);
             if (inner_node(gi, gj, n0, n1)) {
-                const T id = A.invd[idx(gi, gj, n0)];
                 if (NPRE == 1) {
-                    v = w * s_b[kb] * id;
+                    v = w * s_b[kb] * r.id;
                 } else {
-                    const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                    const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                         return s_x1[(jj - fj0 + HB) * BWX + (ii - fi0 + HB)];
                     });
-                    v = s_x1[kb] + w * (s_b[kb] - ax) * id;
+                    v = s_x1[kb] + w * (s_b[kb] - ax) * r.id;
                 }
             }
             s_xs[k] = v;
-        }
+        });
         __syncthreads();
-    }
-    // residual, halo 1
-    for (int k = tid; k < TL::RX * TL::RY; k += PS_NT) {
-        const int li = k % TL::RX, lj = k / TL::RX;
-        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
-        const int kb = (lj + TL::HB - 1) * TL::BWX + (li + TL::HB - 1);
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            if (NPRE == 0) {
-                v = s_b[kb];
-            } else {
-                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+        // residual, halo 1
+        const auto lr = batch_load<TL::RX * TL::RY, PS_NT>(tid, [&](int k) {
+            const int gi = clampi(fi0 - 1 + k % TL::RX, 1, n0 - 2), gj = clampi(fj0 - 1 + k / TL::RX, 1, n1 - 2);
+            return row_faces<T>(A.a, gi, gj);
+        });
+        batch_run<TL::RX * TL::RY, PS_NT>(tid, lr, [&](int k, const RowFaces<T> &f) {
+            const int li = k % TL::RX, lj = k / TL::RX;
+            const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
+            const int kb = (lj + TL::HB - 1) * TL::BWX + (li + TL::HB - 1);
+            T v = T(0);
+            if (inner_node(gi, gj, n0, n1)) {
+                const T ax = frow_v<T>(f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_xs[(jj - fj0 + 2) * TL::SX + (ii - fi0 + 2)];
                 });
                 v = s_b[kb] - ax;
             }
-        }
-        if (NPRE != 0) s_r[k] = v;
+            s_r[k] = v;
+        });
     }
     // owned nodes: smoothed x and the mean-free right-hand side
     for (int k = tid; k < MG_FX * MG_FY; k += PS_NT) {
@@ -489,62 +539,85 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
     const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
     const T w = A.w;
     const T *xc = A.xc;
-    for (int k = tid; k < YX * YY; k += PS_NT) {
+    // x and the coarse values its bilinear interpolation uses, halo NPOST
+    struct YR {
+        T x, c00, c10, c01, c11;
+    };
+#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
+    const auto ly = batch_load<YX * YY, PS_NT>(tid, [&](int k) {
+        const int gi = clampi(fi0 - NPOST + k % YX, 1, n0 - 2), gj = clampi(fj0 - NPOST + k / YX, 1, n1 - 2);
+        const int I = gi >> 1, J = gj >> 1;
+        return YR{A.x[idx(gi, gj, n0)], XC(I, J), XC(I + 1, J), XC(I, J + 1), XC(I + 1, J + 1)};
+    });
+#undef XC
+    batch_run<YX * YY, PS_NT>(tid, ly, [&](int k, const YR &r) {
         const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
         T v = T(0);
         if (inner_node(gi, gj, n0, n1)) {
-            const int I = gi >> 1, J = gj >> 1;
-#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
             T e;
-            if ((gi & 1) == 0 && (gj & 1) == 0) e = XC(I, J);
-            else if ((gj & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
-            else if ((gi & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
-            else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
-#undef XC
-            v = A.x[idx(gi, gj, n0)] + e;
+            if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
+            else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
+            else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
+            else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
+            v = r.x + e;
         }
         s_y[k] = v;
-    }
+    });
     __syncthreads();
+    // faces, right-hand side and 1/diag of a post-smoothed node
+    struct PR {
+        RowFaces<T> f;
+        T b, id;
+    };
     if (NPOST == 2) {
-        for (int k = tid; k < ZX * ZY; k += PS_NT) {
+        const auto lz = batch_load<ZX * ZY, PS_NT>(tid, [&](int k) {
+            const int gi = clampi(fi0 - 1 + k % ZX, 1, n0 - 2), gj = clampi(fj0 - 1 + k / ZX, 1, n1 - 2);
+            const size_t P = idx(gi, gj, n0);
+            return PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P]};
+        });
+        batch_run<ZX * ZY, PS_NT>(tid, lz, [&](int k, const PR &r) {
             const int li = k % ZX, lj = k / ZX;
             const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
             T v = T(0);
             if (inner_node(gi, gj, n0, n1)) {
-                const size_t P = idx(gi, gj, n0);
-                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_y[(jj - fj0 + 2) * YX + (ii - fi0 + 2)];
                 });
-                v = s_y[(lj + 1) * YX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+                v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
             }
             s_z[k] = v;
-        }
+        });
         __syncthreads();
     }
-    for (int k = tid; k < MG_FX * MG_FY; k += PS_NT) {
+    const auto lo = batch_load<MG_FX * MG_FY, PS_NT>(tid, [&](int k) {
+        const int gi = clampi(fi0 + k % MG_FX, 1, n0 - 2), gj = clampi(fj0 + k / MG_FX, 1, n1 - 2);
+        const size_t P = idx(gi, gj, n0);
+        PR r{};
+        if (NPOST >= 1) r = PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P]};
+        return r;
+    });
+    batch_run<MG_FX * MG_FY, PS_NT>(tid, lo, [&](int k, const PR &r) {
         const int li = k % MG_FX, lj = k / MG_FX;
         const int gi = fi0 + li, gj = fj0 + lj;
-        if (gi >= n0 || gj >= n1) continue;
-        const size_t P = idx(gi, gj, n0);
+        if (gi >= n0 || gj >= n1) return;
         T v = T(0);
         if (inner_node(gi, gj, n0, n1)) {
             if (NPOST == 0) {
                 v = s_y[lj * YX + li];
             } else if (NPOST == 1) {
-                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_y[(jj - fj0 + 1) * YX + (ii - fi0 + 1)];
                 });
-                v = s_y[(lj + 1) * YX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+                v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
             } else {
-                const T ax = frow_f<T>(A.a, gi, gj, [&](int ii, int jj) {
+                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_z[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
                 });
-                v = s_z[(lj + 1) * ZX + (li + 1)] + w * (A.b[P] - ax) * A.invd[P];
+                v = s_z[(lj + 1) * ZX + (li + 1)] + w * (r.b - ax) * r.id;
             }
         }
-        A.out[P] = v;
-    }
+        A.out[idx(gi, gj, n0)] = v;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -560,6 +633,8 @@ constexpr int CW = 1024;          // threads of the coarse workgroup
 constexpr int CW_NODES = 1089;    // first level handled: at most 33 x 33 nodes
 constexpr int CN_MAX = 1536;      // nodes of all LDS-resident levels together
 constexpr int CW_ARR = 8;         // per level: fen fet fnn fnt invd x t b
+constexpr int CW_WAVE_NODES = 128;  // levels this small are swept by one wavefront
+constexpr int CW_LEVS = 6;        // levels whose set-up loads are batched
 
 template <typename T>
 struct CLev {
@@ -607,55 +682,132 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
     auto FA = [&](int l) {
         return FArgs<T>{A.L[l].n0, A.L[l].n1, ARR(l, 0), ARR(l, 1), ARR(l, 2), ARR(l, 3)};
     };
-    // coefficients of every level, right-hand side of level l0
-    for (int l = l0; l <= last; l++) {
-        const int n = NN(l);
-        const size_t o = zslot() * n;
-        const T *src[5] = {A.L[l].fen + o, A.L[l].fet + o, A.L[l].fnn + o, A.L[l].fnt + o, A.L[l].invd + o};
-        for (int q = 0; q < 5; q++) {
-            T *dst = ARR(l, q);
-            for (int k = tid; k < n; k += CW) dst[k] = src[q][k];
+    // Set-up: the coefficient arrays of every level from l0 down, level l0's
+    // right-hand side and its partial sums, all loads issued before any is used.
+    // Level l0 + L holds at most CW_IT[L] * CW / 5 nodes (levels shrink about
+    // fourfold from at most CW_NODES); elements beyond that are loaded by a plain loop.
+    constexpr int CW_IT[CW_LEVS] = {6, 2, 1, 1, 1, 1};
+    constexpr int CW_REG = 12;  // sum of CW_IT
+    static_assert(CW_NODES <= 2 * CW, "two right-hand-side nodes per thread");
+    T creg[CW_REG];
+    T braw[2];
+    auto coef_src = [&](int l, int q) {
+        const CLev<T> &L = A.L[l];
+        return q == 0 ? L.fen : q == 1 ? L.fet : q == 2 ? L.fnn : q == 3 ? L.fnt : L.invd;
+    };
+    {
+        const int n = NN(l0);
+#pragma unroll
+        for (int m = 0; m < 2; m++) braw[m] = b0[min(tid + m * CW, n - 1)];
+        int r = 0;
+#pragma unroll
+        for (int L = 0; L < CW_LEVS; L++) {
+            const int l = min(l0 + L, last);  // levels past the last reload it, unused
+            const int nl = NN(l);
+#pragma unroll
+            for (int m = 0; m < CW_IT[L]; m++) {
+                const int e = min(tid + m * CW, 5 * nl - 1);
+                const int q = e / nl, k = e - q * nl;
+                creg[r + m] = coef_src(l, q)[zslot() * nl + k];
+            }
+            r += CW_IT[L];
+        }
+    }
+    double part = 0.0;
+    if (b0part)
+        for (int t = tid; t < nb0p; t += CW) part += b0part[t];
+    {
+        int r = 0;
+#pragma unroll
+        for (int L = 0; L < CW_LEVS; L++) {
+            const int l = l0 + L;
+            const int nl = l <= last ? NN(l) : 0;
+#pragma unroll
+            for (int m = 0; m < CW_IT[L]; m++, r++) {
+                const int e = tid + m * CW;
+                if (e < 5 * nl) sh[A.off[l] + e] = creg[r];
+            }
+            for (int e = tid + CW_IT[L] * CW; e < 5 * nl; e += CW) {
+                const int q = e / nl, k = e - q * nl;
+                sh[A.off[l] + e] = coef_src(l, q)[zslot() * nl + k];
+            }
+        }
+        for (int l = l0 + CW_LEVS; l <= last; l++) {
+            const int nl = NN(l);
+            for (int e = tid; e < 5 * nl; e += CW) {
+                const int q = e / nl, k = e - q * nl;
+                sh[A.off[l] + e] = coef_src(l, q)[zslot() * nl + k];
+            }
         }
     }
     {
         const int n0 = A.L[l0].n0, n1 = A.L[l0].n1, n = NN(l0);
-        double part = 0.0;
-        if (b0part)
-            for (int t = tid; t < nb0p; t += CW) part += b0part[t];
         const double mean = cw_sum(part) / ((double)(n0 - 2) * (double)(n1 - 2));
         T *b = ARR(l0, 7);
-        for (int k = tid; k < n; k += CW) {
-            T v = T(0);
-            if (inner_node(k % n0, k / n0, n0, n1)) v = b0part ? (T)((double)b0[k] - mean) : b0[k];
-            b[k] = v;
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+            const int k = tid + m * CW;
+            if (k < n) {
+                T v = T(0);
+                if (inner_node(k % n0, k / n0, n0, n1)) v = b0part ? (T)((double)braw[m] - mean) : braw[m];
+                b[k] = v;
+            }
         }
     }
     __syncthreads();
-    // `count` Jacobi sweeps on level l (from zero when from_zero); result in x
+    // `count` Jacobi sweeps on level l (from zero when from_zero); result in x.
+    // The buffers alternate so that the last sweep writes x (a from-zero first
+    // sweep, which has no neighbours, starts in t when count is even).  A level of
+    // at most CW_WAVE_NODES nodes is swept by wavefront 0 alone, without block
+    // barriers: its LDS writes are ordered before its next sweep's reads by a
+    // wavefront-scope fence.
     auto sweeps = [&](int l, int count, bool from_zero) {
+        if (count <= 0) return;
         const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
         const FArgs<T> a = FA(l);
         const T *invd = ARR(l, 4), *b = ARR(l, 7);
         T *x = ARR(l, 5), *t = ARR(l, 6);
         T *src = x, *dst = t;
-        for (int s = 0; s < count; s++) {
-            if (from_zero && s == 0) {
-                for (int k = tid; k < n; k += CW) x[k] = inner_node(k % n0, k / n0, n0, n1) ? w * b[k] * invd[k] : T(0);
-            } else {
-                for (int k = tid; k < n; k += CW) {
-                    const int i = k % n0, j = k / n0;
-                    T v = T(0);
-                    if (inner_node(i, j, n0, n1))
-                        v = src[k] + w * (b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return src[jj * n0 + ii]; })) *
-                                         invd[k];
-                    dst[k] = v;
+        int s = 0;
+        auto first = [&](T *out) {
+            for (int k = tid; k < n; k += CW) out[k] = inner_node(k % n0, k / n0, n0, n1) ? w * b[k] * invd[k] : T(0);
+        };
+        auto sweep_node = [&](int k) {
+            const int i = k % n0, j = k / n0;
+            T v = T(0);
+            if (inner_node(i, j, n0, n1))
+                v = src[k] + w * (b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return src[jj * n0 + ii]; })) *
+                                 invd[k];
+            dst[k] = v;
+        };
+        if (from_zero) {
+            T *out = (count % 2 == 0) ? t : x;
+            first(out);
+            src = out;
+            dst = (out == x) ? t : x;
+            s = 1;
+        }
+        __syncthreads();
+        if (n <= CW_WAVE_NODES) {
+            if (tid < 64) {
+                for (; s < count; s++) {
+                    for (int k = tid; k < n; k += 64) sweep_node(k);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    T *tmp = src; src = dst; dst = tmp;
                 }
-                T *tmp = src; src = dst; dst = tmp;
             }
             __syncthreads();
+        } else {
+            for (; s < count; s++) {
+                for (int k = tid; k < n; k += CW) sweep_node(k);
+                __syncthreads();
+                T *tmp = src; src = dst; dst = tmp;
+            }
         }
-        if (src != x) {
-            for (int k = tid; k < n; k += CW) x[k] = src[k];
+        if (!from_zero && (count & 1)) {  // the last sweep wrote t
+            for (int k = tid; k < n; k += CW) x[k] = t[k];
             __syncthreads();
         }
     };
