@@ -621,20 +621,25 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
-// Coarse-level V-cycle in one workgroup, resident in LDS.  Levels of at most
-// 33 x 33 nodes are latency-bound: a launch per sweep would cost more than the
-// sweep, and so would a global-memory round trip per sweep.  This kernel loads
-// the coefficients of every level from l0 down (and level l0's right-hand side)
-// into LDS once, runs the V-cycle there with block barriers between the phases,
-// and writes level l0's result back.  The arithmetic per node is the same device
-// code as the fine-level kernels.
+// Coarse-level V-cycle in one workgroup.  Levels of at most 33 x 33 nodes are
+// latency-bound: a launch per sweep would cost more than the sweep, and so would
+// a global-memory round trip per sweep.  Every thread owns one interior node of
+// each level (interior node q = tid; at most 31 x 31 = 961 of them) and keeps
+// that node's operator row, 1/diag and right-hand side in registers for the whole
+// V-cycle; LDS holds only each level's iterate and its ping-pong buffer, whose
+// wall entries stay zero.  The row is the stored operator's with the Neumann
+// ghosts (neighbours clamped into the interior) folded into the interior
+// coefficients, so a sweep is nine LDS reads and nine multiply-adds per node.
+// Levels of at most 64 interior nodes are owned by wavefront 0 alone, whose
+// sweeps need no block barrier.  The set-up issues every global load (faces and
+// 1/diag of every level, level l0's right-hand side) before any is used.
 
 constexpr int CW = 1024;          // threads of the coarse workgroup
 constexpr int CW_NODES = 1089;    // first level handled: at most 33 x 33 nodes
 constexpr int CN_MAX = 1536;      // nodes of all LDS-resident levels together
-constexpr int CW_ARR = 8;         // per level: fen fet fnn fnt invd x t b
-constexpr int CW_WAVE_NODES = 128;  // levels this small are swept by one wavefront
-constexpr int CW_LEVS = 6;        // levels whose set-up loads are batched
+constexpr int CW_ARR = 2;         // LDS arrays per level: x and its ping-pong buffer
+constexpr int CW_LEVS = 4;        // at most this many levels in the kernel (a 33 x 33 level has four)
+constexpr int CW_WAVE = 64;       // levels with at most this many interior nodes run in wavefront 0
 
 template <typename T>
 struct CLev {
@@ -661,6 +666,53 @@ __device__ __forceinline__ double cw_sum(double v) {
     return tot;
 }
 
+// the interior node of one level a thread owns
+template <typename T>
+struct CNode {
+    int k;     // node index j * n0 + i; -1: none
+    int i, j;
+    T c[9];    // operator row, ghosts folded in
+    T id;      // 1 / diag
+    T b;       // right-hand side
+};
+
+// row (i, j) from its faces with every clamped neighbour's coefficient moved onto
+// the interior node it is clamped to: next to the west (east) wall the west (east)
+// column folds onto the centre column, next to the south (north) wall the south
+// (north) row onto the centre row.  Static indices only (the row stays in
+// registers); an interior row is unchanged.
+template <typename T>
+__device__ __forceinline__ void fold_row(const RowFaces<T> &f, int n0, int n1, int i, int j, T (&c)[9]) {
+    p9_row_from<T>(f, c);
+    const bool w_ = i == 1, e_ = i == n0 - 2, s_ = j == 1, n_ = j == n1 - 2;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        T &cw = c[3 * r], &cc = c[3 * r + 1], &cx = c[3 * r + 2];
+        cc = w_ ? cc + cw : cc;
+        cw = w_ ? T(0) : cw;
+        cc = e_ ? cc + cx : cc;
+        cx = e_ ? T(0) : cx;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        T &cs = c[q], &cc = c[3 + q], &cn = c[6 + q];
+        cc = s_ ? cc + cs : cc;
+        cs = s_ ? T(0) : cs;
+        cc = n_ ? cc + cn : cc;
+        cn = n_ ? T(0) : cn;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T cw_row(const CNode<T> &c, const T *x, int n0) {
+    T s = T(0);
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) s += c.c[(dj + 1) * 3 + (di + 1)] * x[c.k + dj * n0 + di];
+    return s;
+}
+
 // b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
 // whose mean is removed; null: used as is); the level-l0 result is written to x0out.
 template <typename T>
@@ -677,219 +729,138 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
     const int tid = threadIdx.x;
     const T w = A.omega;
     const int last = A.nlev - 1;
+    const int nl = last - l0 + 1;
     auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
-    auto ARR = [&](int l, int k) { return sh + A.off[l] + k * NN(l); };
-    auto FA = [&](int l) {
-        return FArgs<T>{A.L[l].n0, A.L[l].n1, ARR(l, 0), ARR(l, 1), ARR(l, 2), ARR(l, 3)};
-    };
-    // Set-up: the coefficient arrays of every level from l0 down, level l0's
-    // right-hand side and its partial sums, all loads issued before any is used.
-    // Level l0 + L holds at most CW_IT[L] * CW / 5 nodes (levels shrink about
-    // fourfold from at most CW_NODES); elements beyond that are loaded by a plain loop.
-    constexpr int CW_IT[CW_LEVS] = {6, 2, 1, 1, 1, 1};
-    constexpr int CW_REG = 12;  // sum of CW_IT
-    static_assert(CW_NODES <= 2 * CW, "two right-hand-side nodes per thread");
-    T creg[CW_REG];
-    T braw[2];
-    auto coef_src = [&](int l, int q) {
-        const CLev<T> &L = A.L[l];
-        return q == 0 ? L.fen : q == 1 ? L.fet : q == 2 ? L.fnn : q == 3 ? L.fnt : L.invd;
-    };
-    {
-        const int n = NN(l0);
+    auto XA = [&](int L) { return sh + A.off[l0 + L]; };             // iterate of level l0 + L
+    auto TA = [&](int L) { return sh + A.off[l0 + L] + NN(l0 + L); }; // its ping-pong buffer
+    auto INNER = [&](int L) { return (A.L[l0 + L].n0 - 2) * (A.L[l0 + L].n1 - 2); };
+
+    // set-up loads: row faces and 1/diag of the owned node of every level
+    CNode<T> nd[CW_LEVS];
+    RowFaces<T> rf[CW_LEVS];
 #pragma unroll
-        for (int m = 0; m < 2; m++) braw[m] = b0[min(tid + m * CW, n - 1)];
-        int r = 0;
-#pragma unroll
-        for (int L = 0; L < CW_LEVS; L++) {
-            const int l = min(l0 + L, last);  // levels past the last reload it, unused
-            const int nl = NN(l);
-#pragma unroll
-            for (int m = 0; m < CW_IT[L]; m++) {
-                const int e = min(tid + m * CW, 5 * nl - 1);
-                const int q = e / nl, k = e - q * nl;
-                creg[r + m] = coef_src(l, q)[zslot() * nl + k];
-            }
-            r += CW_IT[L];
-        }
+    for (int L = 0; L < CW_LEVS; L++) {
+        const int l = min(l0 + L, last);  // levels past the last reload it, unused
+        const CLev<T> &cl = A.L[l];
+        const int m0 = cl.n0 - 2, m1 = cl.n1 - 2;
+        const int q = min(tid, m0 * m1 - 1);
+        const int i = 1 + q % m0, j = 1 + q / m0;
+        const size_t o = zslot() * cl.n0 * cl.n1;
+        nd[L].i = i;
+        nd[L].j = j;
+        nd[L].k = (l0 + L <= last && tid < m0 * m1) ? j * cl.n0 + i : -1;
+        rf[L] = load_row_faces<T>(cl.n0, cl.fen + o, cl.fet + o, cl.fnn + o, cl.fnt + o, i, j);
+        nd[L].id = cl.invd[o + (size_t)j * cl.n0 + i];
+        nd[L].b = T(0);
     }
+    const T braw = b0[nd[0].j * A.L[l0].n0 + nd[0].i];
     double part = 0.0;
     if (b0part)
         for (int t = tid; t < nb0p; t += CW) part += b0part[t];
     {
-        int r = 0;
+        int tot = 0;
+        for (int l = l0; l <= last; l++) tot += CW_ARR * NN(l);
+        for (int k = tid; k < tot; k += CW) sh[k] = T(0);
+    }
 #pragma unroll
-        for (int L = 0; L < CW_LEVS; L++) {
-            const int l = l0 + L;
-            const int nl = l <= last ? NN(l) : 0;
-#pragma unroll
-            for (int m = 0; m < CW_IT[L]; m++, r++) {
-                const int e = tid + m * CW;
-                if (e < 5 * nl) sh[A.off[l] + e] = creg[r];
-            }
-            for (int e = tid + CW_IT[L] * CW; e < 5 * nl; e += CW) {
-                const int q = e / nl, k = e - q * nl;
-                sh[A.off[l] + e] = coef_src(l, q)[zslot() * nl + k];
-            }
-        }
-        for (int l = l0 + CW_LEVS; l <= last; l++) {
-            const int nl = NN(l);
-            for (int e = tid; e < 5 * nl; e += CW) {
-                const int q = e / nl, k = e - q * nl;
-                sh[A.off[l] + e] = coef_src(l, q)[zslot() * nl + k];
-            }
-        }
+    for (int L = 0; L < CW_LEVS; L++) {
+        const int l = min(l0 + L, last);
+        fold_row<T>(rf[L], A.L[l].n0, A.L[l].n1, nd[L].i, nd[L].j, nd[L].c);
     }
     {
-        const int n0 = A.L[l0].n0, n1 = A.L[l0].n1, n = NN(l0);
-        const double mean = cw_sum(part) / ((double)(n0 - 2) * (double)(n1 - 2));
-        T *b = ARR(l0, 7);
-#pragma unroll
-        for (int m = 0; m < 2; m++) {
-            const int k = tid + m * CW;
-            if (k < n) {
-                T v = T(0);
-                if (inner_node(k % n0, k / n0, n0, n1)) v = b0part ? (T)((double)braw[m] - mean) : braw[m];
-                b[k] = v;
-            }
-        }
+        const double mean = cw_sum(part) / (double)INNER(0);  // also orders the zero fill
+        nd[0].b = b0part ? (T)((double)braw - mean) : braw;
     }
-    __syncthreads();
-    // `count` Jacobi sweeps on level l (from zero when from_zero); result in x.
-    // The buffers alternate so that the last sweep writes x (a from-zero first
-    // sweep, which has no neighbours, starts in t when count is even).  A level of
-    // at most CW_WAVE_NODES nodes is swept by wavefront 0 alone, without block
-    // barriers: its LDS writes are ordered before its next sweep's reads by a
-    // wavefront-scope fence.
-    auto sweeps = [&](int l, int count, bool from_zero) {
+
+    // `count` Jacobi sweeps of level L (from zero when from_zero); the result is
+    // left in the level's iterate
+    auto sweeps = [&](const CNode<T> &c, int L, int count, bool from_zero) __attribute__((always_inline)) {
         if (count <= 0) return;
-        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
-        const FArgs<T> a = FA(l);
-        const T *invd = ARR(l, 4), *b = ARR(l, 7);
-        T *x = ARR(l, 5), *t = ARR(l, 6);
-        T *src = x, *dst = t;
-        int s = 0;
-        auto first = [&](T *out) {
-            for (int k = tid; k < n; k += CW) out[k] = inner_node(k % n0, k / n0, n0, n1) ? w * b[k] * invd[k] : T(0);
+        const int n0 = A.L[l0 + L].n0;
+        T *x = XA(L), *t = TA(L);
+        // output of sweep s: the buffers alternate so that the last one is x
+        // (from zero); otherwise sweep 0 reads x and writes t
+        auto out_of = [&](int s) __attribute__((always_inline)) { return from_zero ? (((count - 1 - s) & 1) ? t : x) : ((s & 1) ? x : t); };
+        auto one = [&](int s) __attribute__((always_inline)) {
+            if (c.k < 0) return;
+            T *out = out_of(s);
+            if (from_zero && s == 0) {
+                out[c.k] = w * c.b * c.id;
+            } else {
+                const T *in = (out == x) ? t : x;
+                out[c.k] = in[c.k] + w * (c.b - cw_row<T>(c, in, n0)) * c.id;
+            }
         };
-        auto sweep_node = [&](int k) {
-            const int i = k % n0, j = k / n0;
-            T v = T(0);
-            if (inner_node(i, j, n0, n1))
-                v = src[k] + w * (b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return src[jj * n0 + ii]; })) *
-                                 invd[k];
-            dst[k] = v;
-        };
-        if (from_zero) {
-            T *out = (count % 2 == 0) ? t : x;
-            first(out);
-            src = out;
-            dst = (out == x) ? t : x;
-            s = 1;
-        }
-        __syncthreads();
-        if (n <= CW_WAVE_NODES) {
+        if (INNER(L) <= CW_WAVE) {
             if (tid < 64) {
-                for (; s < count; s++) {
-                    for (int k = tid; k < n; k += 64) sweep_node(k);
+                for (int s = 0; s < count; s++) {
+                    one(s);
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    T *tmp = src; src = dst; dst = tmp;
                 }
             }
-            __syncthreads();
         } else {
-            for (; s < count; s++) {
-                for (int k = tid; k < n; k += CW) sweep_node(k);
-                __syncthreads();
-                T *tmp = src; src = dst; dst = tmp;
+            for (int s = 0; s < count; s++) {
+                one(s);
+                if (s + 1 < count) __syncthreads();
             }
         }
-        if (!from_zero && (count & 1)) {  // the last sweep wrote t
-            for (int k = tid; k < n; k += CW) x[k] = t[k];
-            __syncthreads();
-        }
+        if (!from_zero && (count & 1) && c.k >= 0) x[c.k] = t[c.k];  // the last sweep wrote t
+        __syncthreads();
     };
-    for (int l = l0; l < last; l++) {
-        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
-        if (A.pre > 0) {
-            sweeps(l, A.pre, true);
-        } else {
-            T *x = ARR(l, 5);
-            for (int k = tid; k < n; k += CW) x[k] = T(0);
+
+    // down: smoothing, residual, full weighting onto the next level (mean removed)
+#pragma unroll
+    for (int L = 0; L < CW_LEVS - 1; L++) {
+        if (L < nl - 1) {
+            const int n0 = A.L[l0 + L].n0;
+            const CNode<T> &c = nd[L];
+            T *x = XA(L), *r = TA(L);
+            if (A.pre > 0) sweeps(c, L, A.pre, true);  // with pre == 0 x stays zero
+            if (c.k >= 0) r[c.k] = c.b - cw_row<T>(c, x, n0);
             __syncthreads();
-        }
-        // residual into t
-        {
-            const FArgs<T> a = FA(l);
-            const T *x = ARR(l, 5), *b = ARR(l, 7);
-            T *r = ARR(l, 6);
-            for (int k = tid; k < n; k += CW) {
-                const int i = k % n0, j = k / n0;
-                T v = T(0);
-                if (inner_node(i, j, n0, n1)) v = b[k] - frow_f<T>(a, i, j, [&](int ii, int jj) { return x[jj * n0 + ii]; });
-                r[k] = v;
+            CNode<T> &cc = nd[L + 1];
+            T v = T(0);
+            if (cc.k >= 0) {
+                const int k = 2 * cc.j * n0 + 2 * cc.i;
+                v = (T(4) * r[k] + T(2) * (r[k - 1] + r[k + 1] + r[k - n0] + r[k + n0]) +
+                     (r[k - n0 - 1] + r[k - n0 + 1] + r[k + n0 - 1] + r[k + n0 + 1])) /
+                    T(16);
             }
-            __syncthreads();
-        }
-        // full weighting onto level l+1, interior mean removed
-        {
-            const int m0 = A.L[l + 1].n0, m1 = A.L[l + 1].n1, m = NN(l + 1);
-            const T *r = ARR(l, 6);
-            T *bc = ARR(l + 1, 7);
-            double part = 0.0;
-            for (int k = tid; k < m; k += CW) {
-                const int I = k % m0, J = k / m0;
-                T v = T(0);
-                if (inner_node(I, J, m0, m1)) {
-                    const int i = 2 * I, j = 2 * J, f = n0;
-                    v = (T(4) * r[j * f + i] +
-                         T(2) * (r[j * f + i - 1] + r[j * f + i + 1] + r[(j - 1) * f + i] + r[(j + 1) * f + i]) +
-                         (r[(j - 1) * f + i - 1] + r[(j - 1) * f + i + 1] + r[(j + 1) * f + i - 1] +
-                          r[(j + 1) * f + i + 1])) /
-                        T(16);
-                    part += (double)v;
-                }
-                bc[k] = v;
-            }
-            const double mean = cw_sum(part) / ((double)(m0 - 2) * (double)(m1 - 2));
-            for (int k = tid; k < m; k += CW)
-                if (inner_node(k % m0, k / m0, m0, m1)) bc[k] = (T)((double)bc[k] - mean);
-            __syncthreads();
+            const double mean = cw_sum(cc.k >= 0 ? (double)v : 0.0) / (double)INNER(L + 1);
+            cc.b = (T)((double)v - mean);
         }
     }
     // coarsest level
-    if (A.coarse > 0) {
-        sweeps(last, A.coarse, true);
-    } else {
-        T *x = ARR(last, 5);
-        for (int k = tid; k < NN(last); k += CW) x[k] = T(0);
-        __syncthreads();
-    }
-    for (int l = last - 1; l >= l0; l--) {
-        const int n0 = A.L[l].n0, n1 = A.L[l].n1, n = NN(l);
-        const int m0 = A.L[l + 1].n0, m1 = A.L[l + 1].n1;
-        const T *xcc = ARR(l + 1, 5);
-        T *x = ARR(l, 5);
-        for (int k = tid; k < n; k += CW) {
-            const int i = k % n0, j = k / n0;
-            if (!inner_node(i, j, n0, n1)) continue;
-            const int I = i >> 1, J = j >> 1;
+#pragma unroll
+    for (int L = 0; L < CW_LEVS; L++)
+        if (L == nl - 1) sweeps(nd[L], L, A.coarse, true);  // with coarse == 0 x stays zero
+    // up: bilinear prolongation of the correction, post-smoothing
+#pragma unroll
+    for (int L = CW_LEVS - 2; L >= 0; L--) {
+        if (L < nl - 1) {
+            const CNode<T> &c = nd[L];
+            const int m0 = A.L[l0 + L + 1].n0, m1 = A.L[l0 + L + 1].n1;
+            const T *xcc = XA(L + 1);
+            T *x = XA(L);
+            if (c.k >= 0) {
+                const int i = c.i, j = c.j, I = i >> 1, J = j >> 1;
 #define XC(a_, b_) xcc[clampi((b_), 1, m1 - 2) * m0 + clampi((a_), 1, m0 - 2)]
-            T e;
-            if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
-            else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
-            else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
-            else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
+                T e;
+                if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
+                else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
+                else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
+                else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
 #undef XC
-            x[k] += e;
+                x[c.k] += e;
+            }
+            __syncthreads();
+            sweeps(c, L, A.post, false);
         }
-        __syncthreads();
-        sweeps(l, A.post, false);
     }
     {
-        const T *x = ARR(l0, 5);
+        const T *x = XA(0);
         for (int k = tid; k < NN(l0); k += CW) x0out[k] = x[k];
     }
 }
@@ -1285,6 +1256,7 @@ struct PSolver {
             if (lc + 1 >= nlev) return CFD_E_ARG;
             lc++;
         }
+        if (nlev - lc > CW_LEVS) return CFD_E_ARG;
         B = cx.B;
         for (int l = 0; l < nlev; l++) {
             OpLevel<T> &L = lev[l];
