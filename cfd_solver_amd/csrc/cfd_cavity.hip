@@ -463,10 +463,13 @@ __global__ __launch_bounds__(NT) void k_correct(
 
 // adiabatic side walls (oracle: temperature_sides), one workgroup: every new value
 // is computed from the old wall column (into scratch) before any is written
-constexpr int TS_NT = 1024;
-__global__ __launch_bounds__(TS_NT) void k_t_sides(CGrid g, double *__restrict__ T, const double *__restrict__ alph,
-                                                   const double *__restrict__ beta, double *__restrict__ scratch,
-                                                   size_t scr) {
+// Adiabatic side walls in two launches spread over the chip (the column-strided
+// wall accesses of one block would be bound by that CU's outstanding misses): the
+// new wall values, computed from the old ones, go to scratch, then into T.
+constexpr int WB = 64;  // threads per block of the wall kernels
+__global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restrict__ T,
+                                                const double *__restrict__ alph, const double *__restrict__ beta,
+                                                double *__restrict__ scratch, size_t scr) {
     if (g.sp[blockIdx.z].active == 0.0) return;
     {
         const size_t o = zslot() * g.N;
@@ -474,21 +477,27 @@ __global__ __launch_bounds__(TS_NT) void k_t_sides(CGrid g, double *__restrict__
         scratch += zslot() * scr;
     }
     const int n0 = g.n0, n1 = g.n1;
-    for (int j = 1 + (int)threadIdx.x; j <= n1 - 2; j += TS_NT) {
-        size_t P = idx(0, j, n0);
-        double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
-        double txi = beta[P] / alph[P] * te;
-        scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
-        P = idx(n0 - 1, j, n0);
-        te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
-        txi = beta[P] / alph[P] * te;
-        scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
-    }
-    __syncthreads();
-    for (int j = 1 + (int)threadIdx.x; j <= n1 - 2; j += TS_NT) {
-        T[idx(0, j, n0)] = scratch[j];
-        T[idx(n0 - 1, j, n0)] = scratch[n1 + j];
-    }
+    const int j = 1 + (int)(blockIdx.x * WB + threadIdx.x);
+    if (j > n1 - 2) return;
+    size_t P = idx(0, j, n0);
+    double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
+    double txi = beta[P] / alph[P] * te;
+    scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
+    P = idx(n0 - 1, j, n0);
+    te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
+    txi = beta[P] / alph[P] * te;
+    scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
+}
+__global__ __launch_bounds__(WB) void k_t_sides_put(CGrid g, double *__restrict__ T,
+                                                    const double *__restrict__ scratch, size_t scr) {
+    if (g.sp[blockIdx.z].active == 0.0) return;
+    T += zslot() * g.N;
+    scratch += zslot() * scr;
+    const int n0 = g.n0, n1 = g.n1;
+    const int j = 1 + (int)(blockIdx.x * WB + threadIdx.x);
+    if (j > n1 - 2) return;
+    T[idx(0, j, n0)] = scratch[j];
+    T[idx(n0 - 1, j, n0)] = scratch[n1 + j];
 }
 
 // wall pressure from the normal momentum equation (oracle: wall_p_value)
@@ -555,34 +564,39 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
 }
 
 // Wall pressure written in place: wall_p_value reads p only at interior nodes, so
-// no thread reads a value another one writes.  A corner is the mean of the two
-// adjacent wall values (recomputed by the thread that writes it).
-__global__ void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p) {
+// no thread reads a value another one writes.  Block row y < 4 holds the nodes of
+// wall y (one per thread); row 4 the corners, each the mean of the two adjacent
+// wall values, computed by a lane pair.
+__global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p) {
     const SlotP &sp = cg.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     const double Re = sp.Re, Ri = sp.Ri;
     const KGrid g = cg.kg();
     w = wall_at(w, zslot() * cg.N);
     p += zslot() * cg.N;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int side = blockIdx.y;
+    const int t = blockIdx.x * WB + threadIdx.x;
     const int n0 = g.n0, n1 = g.n1;
-    if (t >= 1 && t <= n0 - 2) {
-        p[idx(t, 0, n0)] = wall_p_value(w, g, Re, Ri, 0, t, 0);
-        p[idx(t, n1 - 1, n0)] = wall_p_value(w, g, Re, Ri, 1, t, n1 - 1);
+    if (side < 4) {
+        const int len = side <= 1 ? n0 : n1;
+        if (t < 1 || t > len - 2) return;
+        const int i = side == 2 ? 0 : (side == 3 ? n0 - 1 : t);
+        const int j = side == 0 ? 0 : (side == 1 ? n1 - 1 : t);
+        p[idx(i, j, n0)] = wall_p_value(w, g, Re, Ri, side, i, j);
+        return;
     }
-    if (t >= 1 && t <= n1 - 2) {
-        p[idx(0, t, n0)] = wall_p_value(w, g, Re, Ri, 2, 0, t);
-        p[idx(n0 - 1, t, n0)] = wall_p_value(w, g, Re, Ri, 3, n0 - 1, t);
-    }
-    if (t == 0) {
-        p[idx(0, 0, n0)] = 0.5 * (wall_p_value(w, g, Re, Ri, 0, 1, 0) + wall_p_value(w, g, Re, Ri, 2, 0, 1));
-        p[idx(n0 - 1, 0, n0)] =
-            0.5 * (wall_p_value(w, g, Re, Ri, 0, n0 - 2, 0) + wall_p_value(w, g, Re, Ri, 3, n0 - 1, 1));
-        p[idx(0, n1 - 1, n0)] =
-            0.5 * (wall_p_value(w, g, Re, Ri, 1, 1, n1 - 1) + wall_p_value(w, g, Re, Ri, 2, 0, n1 - 2));
-        p[idx(n0 - 1, n1 - 1, n0)] =
-            0.5 * (wall_p_value(w, g, Re, Ri, 1, n0 - 2, n1 - 1) + wall_p_value(w, g, Re, Ri, 3, n0 - 1, n1 - 2));
-    }
+    if (blockIdx.x != 0 || threadIdx.x >= 8) return;
+    // corner c = threadIdx.x / 2 (bit 0: east, bit 1: north); lane 0 of the pair
+    // takes the bottom/top wall value next to it, lane 1 the side wall value
+    const int c = threadIdx.x >> 1, h = threadIdx.x & 1;
+    const bool east = c & 1, north = c & 2;
+    const int ci = east ? n0 - 1 : 0, cj = north ? n1 - 1 : 0;
+    const int ws = h == 0 ? (north ? 1 : 0) : (east ? 3 : 2);
+    const int wi = h == 0 ? (east ? n0 - 2 : 1) : ci;
+    const int wj = h == 0 ? cj : (north ? n1 - 2 : 1);
+    const double v = wall_p_value(w, g, Re, Ri, ws, wi, wj);
+    const double o = __shfl_xor(v, 1, 64);
+    if (h == 0) p[idx(ci, cj, n0)] = 0.5 * (v + o);
 }
 
 // velocity and temperature wall values (oracle: velocity_temperature_bcs)
@@ -613,13 +627,14 @@ __global__ void k_uvt_walls(CGrid g, double *__restrict__ u, double *__restrict_
 }
 
 // wall Nusselt number of the bottom (wall 0) or top (wall 1) wall (oracle: nusselt),
-// summed by the calling block
+// summed by the calling NTB-thread block
+template <int NTB>
 __device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const double *__restrict__ ajac,
                                const double *__restrict__ etx, const double *__restrict__ ety, int wall) {
     const int n0 = g.n0, n1 = g.n1;
     const int j = wall ? n1 - 1 : 0, s = wall ? -1 : 1;
     double acc = 0.0;
-    for (int i = tlin(); i < n0 - 1; i += NT) {
+    for (int i = tlin(); i < n0 - 1; i += NTB) {
         const int ip = i + 1;
         const size_t A = idx(i, j, n0), B = idx(ip, j, n0);
         const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
@@ -628,7 +643,7 @@ __device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const doub
         const double fh2 = tj2 * (4 * T[idx(ip, j + s, n0)] - 3 * T[B] - T[idx(ip, j + 2 * s, n0)]) / (2 * g.h2);
         acc += 0.5 * (fh1 + fh2) * g.h1;
     }
-    return block_sum<NT>(acc);
+    return block_sum<NTB>(acc);
 }
 
 // signed max of the nodal dilation (solver.cpp:1821-1850) -> block partials
@@ -659,7 +674,7 @@ __global__ __launch_bounds__(NT) void k_dmax(CGrid g, const double *__restrict__
 
 // one block: dmax from the partials, floored at 0 (solver.cpp:1806), and both
 // wall Nusselt numbers
-__global__ __launch_bounds__(NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, const double *__restrict__ T,
+__global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, const double *__restrict__ T,
                                                     const double *__restrict__ ajac, const double *__restrict__ etx,
                                                     const double *__restrict__ ety) {
     const SlotP &sp = cg.sp[blockIdx.z];
@@ -674,10 +689,10 @@ __global__ __launch_bounds__(NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int np
         T += o; ajac += o; etx += o; ety += o;
     }
     double m = -1.0e300;
-    for (int t = tlin(); t < nparts; t += NT) m = fmax(m, part[t]);
-    m = block_max<NT>(m);
-    const double nh = nusselt_wall(g, T, ajac, etx, ety, 0);
-    const double nc = nusselt_wall(g, T, ajac, etx, ety, 1);
+    for (int t = tlin(); t < nparts; t += FIN_NT) m = fmax(m, part[t]);
+    m = block_max<FIN_NT>(m);
+    const double nh = nusselt_wall<FIN_NT>(g, T, ajac, etx, ety, 0);
+    const double nc = nusselt_wall<FIN_NT>(g, T, ajac, etx, ety, 1);
     if (tlin() == 0) {
         const double den = dT != 0.0 ? dT : 1.0;
         scal[S_DMAX] = fmax(m, 0.0);
@@ -813,14 +828,14 @@ enum { CF_RES, CF_MASS, CF_Q, CF_DEFECT, CF_PSUM };
 // margin: fraction of the remaining FP64 tolerance the first FP32 solve aims at
 // (1 when no further refinement step follows it)
 template <int OP>
-__global__ __launch_bounds__(NT) void k_cav_fin(int gs, RedArgs ra, int nparts, int n0, int n1, double rtol,
-                                                double margin) {
+__global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int nparts, int n0, int n1, double rtol,
+                                                    double margin) {
     ra = rz(ra);
-    if (slot_skip(ra.scal, gs)) return;
+    const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
     constexpr int NV = (OP == CF_MASS || OP == CF_PSUM) ? 1 : 2;
     double tot[NV];
-    sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
-    if (tlin() != 0) return;
+    sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
+    if (skip || tlin() != 0) return;
     double *s = ra.scal;
     const double nin = (double)(n0 - 2) * (double)(n1 - 2);
     if (OP == CF_RES) {
@@ -933,7 +948,7 @@ dim3 grid_b(const Case *c) {
 // one-block-per-slot finish of a case-level reduction over the grid's block partials
 template <int OP>
 int cav_fin(Case *c, int gs = -1, double rtol = 0.0, double margin = 0.5) {
-    k_cav_fin<OP><<<dim3(1, 1, c->B), NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol, margin);
+    k_cav_fin<OP><<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol, margin);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1133,9 +1148,10 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     w.alph += o; w.beta += o; w.gamma += o; w.p1 += o; w.q1 += o; w.ajac += o;
     const dim3 g1((m + 255) / 256, 1, nz);
     if (uvt) k_uvt_walls<<<g1, 256, 0, c->s>>>(g, c->u + o, c->v + o, c->T + o);
-    k_t_sides<<<dim3(1, 1, nz), TS_NT, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr,
-                                                   c->scr);
-    k_p_walls<<<g1, 256, 0, c->s>>>(g, w, c->p + o);
+    const dim3 gs((c->n1 + WB - 1) / WB, 1, nz);
+    k_t_sides<<<gs, WB, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr, c->scr);
+    k_t_sides_put<<<gs, WB, 0, c->s>>>(g, c->T + o, c->scratch + z0 * c->scr, c->scr);
+    k_p_walls<<<dim3((m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1179,7 +1195,7 @@ int outer_body(Case *c, bool graph) {
     RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax and the wall Nusselt numbers (FP64)
     k_dmax<<<gr, bl, 0, c->s>>>(cg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, red(c));
-    k_dmax_nu_fin<<<dim3(1, 1, c->B), NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), c->T, c->ajac, c->etx,
+    k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), c->T, c->ajac, c->etx,
                                                      c->ety);
     CFD_LAUNCH_CHECK();
     return CFD_OK;

@@ -117,25 +117,38 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *part, in
     }
 }
 
-// totals of NV partial arrays, summed by the whole (PS_NT-thread) block in a fixed
-// order (four independent chains per thread); valid in thread 0
-template <int NV>
+// totals of NV partial arrays, summed by the whole NT-thread block in a fixed
+// order: each thread keeps eight running sums per value, and every load of a pass
+// over 8 * NT partials is issued before any is added (one memory round trip per
+// pass instead of one per partial); valid in thread 0
+template <int NV, int NT = PS_NT>
 __device__ __forceinline__ void sum_partials(const double *part, int pstride, int nparts, double (&tot)[NV]) {
+    constexpr int U = 8;
+    const int t = tlin();
+    double s[NV][U];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const double *p = part + (size_t)k * pstride;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int t = tlin();
-        for (; t + 3 * PS_NT < nparts; t += 4 * PS_NT) {
-            s0 += p[t];
-            s1 += p[t + PS_NT];
-            s2 += p[t + 2 * PS_NT];
-            s3 += p[t + 3 * PS_NT];
-        }
-        for (; t < nparts; t += PS_NT) s0 += p[t];
-        tot[k] = block_sum<PS_NT>((s0 + s1) + (s2 + s3));
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int u = 0; u < U; u++) s[k][u] = 0.0;
+    for (int t0 = 0; t0 < nparts; t0 += U * NT) {
+        double v[NV][U];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+#pragma unroll
+            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(t0 + u * NT + t, nparts - 1)];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+#pragma unroll
+            for (int u = 0; u < U; u++) s[k][u] += t0 + u * NT + t < nparts ? v[k][u] : 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+        tot[k] = block_sum<NT>(((s[k][0] + s[k][1]) + (s[k][2] + s[k][3])) +
+                               ((s[k][4] + s[k][5]) + (s[k][6] + s[k][7])));
 }
+
+// threads of the one-block kernels that finish a reduction
+constexpr int FIN_NT = 1024;
 
 // ---------------------------------------------------------------------------
 // operator rows
@@ -369,9 +382,17 @@ struct DownTile {
     static constexpr int TOTAL = NB + NX1 + NXS + NR;
 };
 
+// FP32 tiles issue the smoothing loads together with the right-hand side's (one
+// memory round trip per launch); FP64 tiles, which would need twice the registers,
+// issue them after the first phase.
+template <typename T>
+constexpr bool mg_early() {
+    return sizeof(T) == 4;
+}
+
 template <typename T, int NPRE>
 __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
-    if (slot_skip(A.scal + zslot() * SS, A.gs)) return;
+    const double *guard = A.scal + zslot() * SS;
     {
         const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1, op = zslot() * A.lps;
         A.a = fz(A.a, of);
@@ -402,6 +423,21 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         const size_t P = idx(clampi(gi, 1, n0 - 2), clampi(gj, 1, n1 - 2), n0);
         return BI{A.bsrc[P], NPRE == 2 ? A.invd[P] : T(0)};
     });
+    // faces and 1/diag of the smoothed region (halo 2), also used by the residual
+    struct XR {
+        RowFaces<T> f;
+        T id;
+    };
+    constexpr int NXR = NPRE >= 1 ? TL::SX * TL::SY : 1;
+    auto xs_load = [&]() {
+        return batch_load<NXR, PS_NT>(tid, [&](int k) {
+            const int gi = clampi(fi0 - 2 + k % TL::SX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / TL::SX, 1, n1 - 2);
+            return XR{row_faces<T>(A.a, gi, gj), A.invd[idx(gi, gj, n0)]};
+        });
+    };
+    decltype(xs_load()) lx;
+    if (NPRE >= 1 && mg_early<T>()) lx = xs_load();
+    if (slot_skip(guard, A.gs)) return;
     double mean = 0.0;
     if (A.bpart) {
         double tot[1];
@@ -419,21 +455,11 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         s_b[k] = bv;
         if (NPRE == 2) s_x1[k] = in ? w * bv * r.id : T(0);
     });
+    if (NPRE >= 1 && !mg_early<T>()) lx = xs_load();
     __syncthreads();
     if (NPRE >= 1) {
         constexpr int HB = TL::HB, BWX = TL::BWX;
-        struct XR {
-            RowFaces<T> f;
-            T id;
-        };
-        const auto lx = batch_load<TL::SX * TL::SY, PS_NT>(tid, [&](int k) {
-            const int gi = clampi(fi0 - 2 + k % TL::SX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / TL::SX, 1, n1 - 2);
-            XR r;
-            r.id = A.invd[idx(gi, gj, n0)];
-            if (NPRE == 2) r.f = row_faces<T>(A.a, gi, gj);
-            return r;
-        });
-        batch_run<TL::SX * TL::SY, PS_NT>(tid, lx, [&](int k, const XR &r) {
+        batch_run<NXR, PS_NT>(tid, lx, [&](int k, const XR &r) {
             const int li = k % TL::SX, lj = k / TL::SX;
             const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
             const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
@@ -451,23 +477,20 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
             s_xs[k] = v;
         });
         __syncthreads();
-        // residual, halo 1
-        const auto lr = batch_load<TL::RX * TL::RY, PS_NT>(tid, [&](int k) {
-            const int gi = clampi(fi0 - 1 + k % TL::RX, 1, n0 - 2), gj = clampi(fj0 - 1 + k / TL::RX, 1, n1 - 2);
-            return row_faces<T>(A.a, gi, gj);
-        });
-        batch_run<TL::RX * TL::RY, PS_NT>(tid, lr, [&](int k, const RowFaces<T> &f) {
-            const int li = k % TL::RX, lj = k / TL::RX;
-            const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
-            const int kb = (lj + TL::HB - 1) * TL::BWX + (li + TL::HB - 1);
+        // residual (halo 1) on the same elements, the ring of the region left out
+        batch_run<NXR, PS_NT>(tid, lx, [&](int k, const XR &r) {
+            const int li = k % TL::SX, lj = k / TL::SX;
+            if (li < 1 || li > TL::SX - 2 || lj < 1 || lj > TL::SY - 2) return;
+            const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
+            const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
             T v = T(0);
             if (inner_node(gi, gj, n0, n1)) {
-                const T ax = frow_v<T>(f, n0, n1, gi, gj, [&](int ii, int jj) {
+                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_xs[(jj - fj0 + 2) * TL::SX + (ii - fi0 + 2)];
                 });
                 v = s_b[kb] - ax;
             }
-            s_r[k] = v;
+            s_r[(lj - 1) * TL::RX + (li - 1)] = v;
         });
     }
     // owned nodes: smoothed x and the mean-free right-hand side
@@ -520,7 +543,7 @@ struct UpArgs {
 
 template <typename T, int NPOST>
 __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
-    if (slot_skip(A.scal + zslot() * SS, A.gs)) return;
+    const double *guard = A.scal + zslot() * SS;
     {
         const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1;
         A.a = fz(A.a, of);
@@ -531,9 +554,10 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         A.xc += oc;
     }
     constexpr int YX = MG_FX + 2 * NPOST, YY = MG_FY + 2 * NPOST;  // x + e, halo NPOST
-    constexpr int ZX = MG_FX + 2, ZY = MG_FY + 2;                  // first post sweep (NPOST == 2)
+    // first post-sweep region (halo NPOST - 1): its elements also produce the output
+    constexpr int QX = NPOST >= 1 ? MG_FX + 2 * (NPOST - 1) : 1, QY = NPOST >= 1 ? MG_FY + 2 * (NPOST - 1) : 1;
     __shared__ T s_y[YX * YY];
-    __shared__ T s_z[NPOST == 2 ? ZX * ZY : 1];
+    __shared__ T s_z[NPOST == 2 ? QX * QY : 1];
     const int tid = tlin();
     const int n0 = A.a.n0, n1 = A.a.n1, m0 = A.m0, m1 = A.m1;
     const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
@@ -550,6 +574,22 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         return YR{A.x[idx(gi, gj, n0)], XC(I, J), XC(I + 1, J), XC(I, J + 1), XC(I + 1, J + 1)};
     });
 #undef XC
+    // faces, right-hand side and 1/diag of the first post-sweep region
+    struct PR {
+        RowFaces<T> f;
+        T b, id;
+    };
+    auto q_load = [&]() {
+        return batch_load<QX * QY, PS_NT>(tid, [&](int k) {
+            const int gi = clampi(fi0 - (NPOST - 1) + k % QX, 1, n0 - 2);
+            const int gj = clampi(fj0 - (NPOST - 1) + k / QX, 1, n1 - 2);
+            const size_t P = idx(gi, gj, n0);
+            return PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P]};
+        });
+    };
+    decltype(q_load()) lq;
+    if (NPOST >= 1 && mg_early<T>()) lq = q_load();
+    if (slot_skip(guard, A.gs)) return;
     batch_run<YX * YY, PS_NT>(tid, ly, [&](int k, const YR &r) {
         const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
         T v = T(0);
@@ -561,22 +601,19 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
             else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
             v = r.x + e;
         }
-        s_y[k] = v;
+        if (NPOST == 0) {
+            if (gi < n0 && gj < n1) A.out[idx(gi, gj, n0)] = v;
+        } else {
+            s_y[k] = v;
+        }
     });
+    if (NPOST == 0) return;
+    if (!mg_early<T>()) lq = q_load();
     __syncthreads();
-    // faces, right-hand side and 1/diag of a post-smoothed node
-    struct PR {
-        RowFaces<T> f;
-        T b, id;
-    };
+    // first post sweep on the Q region (NPOST == 2), output on its inner part
     if (NPOST == 2) {
-        const auto lz = batch_load<ZX * ZY, PS_NT>(tid, [&](int k) {
-            const int gi = clampi(fi0 - 1 + k % ZX, 1, n0 - 2), gj = clampi(fj0 - 1 + k / ZX, 1, n1 - 2);
-            const size_t P = idx(gi, gj, n0);
-            return PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P]};
-        });
-        batch_run<ZX * ZY, PS_NT>(tid, lz, [&](int k, const PR &r) {
-            const int li = k % ZX, lj = k / ZX;
+        batch_run<QX * QY, PS_NT>(tid, lq, [&](int k, const PR &r) {
+            const int li = k % QX, lj = k / QX;
             const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
             T v = T(0);
             if (inner_node(gi, gj, n0, n1)) {
@@ -589,31 +626,24 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         });
         __syncthreads();
     }
-    const auto lo = batch_load<MG_FX * MG_FY, PS_NT>(tid, [&](int k) {
-        const int gi = clampi(fi0 + k % MG_FX, 1, n0 - 2), gj = clampi(fj0 + k / MG_FX, 1, n1 - 2);
-        const size_t P = idx(gi, gj, n0);
-        PR r{};
-        if (NPOST >= 1) r = PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P]};
-        return r;
-    });
-    batch_run<MG_FX * MG_FY, PS_NT>(tid, lo, [&](int k, const PR &r) {
-        const int li = k % MG_FX, lj = k / MG_FX;
+    batch_run<QX * QY, PS_NT>(tid, lq, [&](int k, const PR &r) {
+        constexpr int H = NPOST - 1;  // ring of the Q region outside the tile
+        const int li = k % QX - H, lj = k / QX - H;
+        if (li < 0 || li >= MG_FX || lj < 0 || lj >= MG_FY) return;
         const int gi = fi0 + li, gj = fj0 + lj;
         if (gi >= n0 || gj >= n1) return;
         T v = T(0);
         if (inner_node(gi, gj, n0, n1)) {
-            if (NPOST == 0) {
-                v = s_y[lj * YX + li];
-            } else if (NPOST == 1) {
+            if (NPOST == 1) {
                 const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
                     return s_y[(jj - fj0 + 1) * YX + (ii - fi0 + 1)];
                 });
                 v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
             } else {
                 const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_z[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
+                    return s_z[(jj - fj0 + 1) * QX + (ii - fi0 + 1)];
                 });
-                v = s_z[(lj + 1) * ZX + (li + 1)] + w * (r.b - ax) * r.id;
+                v = s_z[(lj + 1) * QX + (li + 1)] + w * (r.b - ax) * r.id;
             }
         }
         A.out[idx(gi, gj, n0)] = v;
@@ -1085,13 +1115,13 @@ enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
 // totals of a phase's partials and what follows from them; skipped while *guard
 // is set (the phase's kernel did not run)
 template <int OP>
-__global__ __launch_bounds__(PS_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot) {
+__global__ __launch_bounds__(FIN_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot) {
     ra = rz(ra);
-    if (slot_skip(ra.scal, gs)) return;
+    const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
     constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
     double tot[NV];
-    sum_partials<NV>(ra.part, ra.pstride, nparts, tot);
-    if (tlin() != 0) return;
+    sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
+    if (skip || tlin() != 0) return;
     double *s = ra.scal;
     if (OP == FIN_START1) {
         s[S_SUM] = tot[0];
@@ -1397,9 +1427,9 @@ struct PSolver {
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
         k_bi_start1<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, b, x, acc, r, ra);
-        k_bi_fin<FIN_START1><<<g1, PS_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
+        k_bi_fin<FIN_START1><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
         k_bi_start2<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
-        k_bi_fin<FIN_START2><<<g1, PS_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
+        k_bi_fin<FIN_START2><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
@@ -1408,17 +1438,17 @@ struct PSolver {
             cx.prof_b(cx.owner, PK_APPLY);
             k_bi_v<T><<<gr, bl, 0, cx.s>>>(a0, ph, vv, r0, ra);
             cx.prof_e(cx.owner);
-            k_bi_fin<FIN_V><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_V><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
-            k_bi_fin<FIN_S><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_S><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             CFD_RET(precond(cx, s, sh, S_DONE));
             cx.prof_b(cx.owner, PK_APPLY);
             k_bi_t<T><<<gr, bl, 0, cx.s>>>(a0, sh, t, s, ra);
             cx.prof_e(cx.owner);
-            k_bi_fin<FIN_T><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
-            k_bi_fin<FIN_XR><<<g1, PS_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_XR><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             if (!graph) {
                 CFD_RET(ps_sync(cx));
