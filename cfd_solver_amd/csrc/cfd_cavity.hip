@@ -115,33 +115,77 @@ struct GridPtrs {
     const double *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *p1, *q1;
 };
 
+// Tiled: a 64 x 8 tile of nodes per 64 x 4 block (two nodes per thread).  u, v, T
+// (halo 2) and p (halo 1) are staged in LDS with coalesced loads issued together,
+// as are the metric and operator coefficients of the thread's own nodes, so the
+// 25-point rows read LDS instead of issuing a global load per stencil point.
+// Halo entries beyond the grid hold clamped copies that no interior row reads.
+constexpr int RT_X = 64, RT_Y = 8;
+inline dim3 grid_res(int n0, int n1, int B) { return dim3((n0 + RT_X - 1) / RT_X, (n1 + RT_Y - 1) / RT_Y, B); }
+
 template <typename TR>
-__global__ __launch_bounds__(NT) void k_residual(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_residual(
     CGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ T,
     const double *__restrict__ p, GridPtrs gp, TR *__restrict__ Ru, TR *__restrict__ Rv, TR *__restrict__ RT,
     double *__restrict__ dd, TR *__restrict__ Ua, TR *__restrict__ Va, RedArgs ra) {
     const SlotP ph = g.sp[blockIdx.z];
-    if (ph.active == 0.0) return;
     ra = rz(ra);
     const size_t o = zslot() * g.N;
     u += o; v += o; T += o; p += o; Ru += o; Rv += o; RT += o; dd += o; Ua += o; Va += o;
-    const double *__restrict__ xix = gp.xix + o, *__restrict__ xiy = gp.xiy + o, *__restrict__ etx = gp.etx + o,
-                 *__restrict__ ety = gp.ety + o, *__restrict__ alph = gp.alph + o, *__restrict__ beta = gp.beta + o,
-                 *__restrict__ gamma = gp.gamma + o, *__restrict__ p1 = gp.p1 + o, *__restrict__ q1 = gp.q1 + o;
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    constexpr int FX = RT_X + 4, FY = RT_Y + 4, PX = RT_X + 2, PY = RT_Y + 2;
+    constexpr int NODES = RT_Y / BY;  // nodes per thread
+    __shared__ double s_u[FX * FY], s_v[FX * FY], s_t[FX * FY], s_p[PX * PY];
+    const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
+    const int i0 = blockIdx.x * RT_X, j0 = blockIdx.y * RT_Y;
+    struct F3 {
+        double u, v, t;
+    };
+    const auto lf = batch_load<FX * FY, NT>(tid, [&](int k) {
+        const size_t Q = idx(clampi(i0 - 2 + k % FX, 0, n0 - 1), clampi(j0 - 2 + k / FX, 0, n1 - 1), n0);
+        return F3{u[Q], v[Q], T[Q]};
+    });
+    const auto lp = batch_load<PX * PY, NT>(tid, [&](int k) {
+        return p[idx(clampi(i0 - 1 + k % PX, 0, n0 - 1), clampi(j0 - 1 + k / PX, 0, n1 - 1), n0)];
+    });
+    struct NC {
+        double xix, xiy, etx, ety, al, be, ga, pp, qq;
+    };
+    NC nc[NODES];
+#pragma unroll
+    for (int m = 0; m < NODES; m++) {
+        const size_t Q = o + idx(clampi(i0 + (int)threadIdx.x, 0, n0 - 1),
+                                 clampi(j0 + (int)threadIdx.y + m * BY, 0, n1 - 1), n0);
+        nc[m] = NC{gp.xix[Q], gp.xiy[Q], gp.etx[Q], gp.ety[Q], gp.alph[Q],
+                   gp.beta[Q], gp.gamma[Q], gp.p1[Q], gp.q1[Q]};
+    }
+    if (ph.active == 0.0) return;
+    batch_run<FX * FY, NT>(tid, lf, [&](int k, const F3 &f) {
+        s_u[k] = f.u;
+        s_v[k] = f.v;
+        s_t[k] = f.t;
+    });
+    batch_run<PX * PY, NT>(tid, lp, [&](int k, double pv) { s_p[k] = pv; });
+    __syncthreads();
     const double h1 = g.h1, h2 = g.h2;
     double e_abs = 0.0, m_abs = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
+#pragma unroll
+    for (int m = 0; m < NODES; m++) {
+        const int li = threadIdx.x, lj = threadIdx.y + m * BY;
+        const int i = i0 + li, j = j0 + lj;
+        if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) continue;
+        const NC &c = nc[m];
         const size_t P = idx(i, j, n0);
+        auto at = [&](const double *sf, int di, int dj) { return sf[(lj + 2 + dj) * FX + (li + 2 + di)]; };
+        auto pat = [&](int di, int dj) { return s_p[(lj + 1 + dj) * PX + (li + 1 + di)]; };
         const bool fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
         const bool hyb = fit_i && fit_j;
         const int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
         const int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
-        const double al = alph[P], be = beta[P], ga = gamma[P], pp = p1[P], qq = q1[P];
-        const double UP = xix[P] * u[P] + xiy[P] * v[P];
-        const double VP = etx[P] * u[P] + ety[P] * v[P];
-        const double *fl[3] = {u, v, T};
+        const double al = c.al, be = c.be, ga = c.ga, pp = c.pp, qq = c.qq;
+        const double UP = c.xix * at(s_u, 0, 0) + c.xiy * at(s_v, 0, 0);
+        const double VP = c.etx * at(s_u, 0, 0) + c.ety * at(s_v, 0, 0);
+        const double *fl[3] = {s_u, s_v, s_t};
         double conv[3];
         const double ial = 1.0 / al, iga = 1.0 / ga;
 #pragma unroll
@@ -150,22 +194,25 @@ __global__ __launch_bounds__(NT) void k_residual(
             const double sc = (k <= 1) ? ph.Re : ph.Re * ph.Pr;
             const double pec1 = UP * sc * h1 * ial;
             const double pec2 = VP * sc * h2 * iga;
-            const double fm2i = fit_i ? f[P - 2] : 0.0, fp2i = fit_i ? f[P + 2] : 0.0;
-            const double fm2j = fit_j ? f[idx(i, j - 2, n0)] : 0.0, fp2j = fit_j ? f[idx(i, j + 2, n0)] : 0.0;
-            const double dxi = conv_deriv_r(mode_i, fm2i, f[P - 1], f[P], f[P + 1], fp2i, UP, pec1, g.r.ih1);
-            const double det = conv_deriv_r(mode_j, fm2j, f[idx(i, j - 1, n0)], f[P], f[idx(i, j + 1, n0)],
-                                            fp2j, VP, pec2, g.r.ih2);
+            const double fm2i = fit_i ? at(f, -2, 0) : 0.0, fp2i = fit_i ? at(f, 2, 0) : 0.0;
+            const double fm2j = fit_j ? at(f, 0, -2) : 0.0, fp2j = fit_j ? at(f, 0, 2) : 0.0;
+            const double dxi = conv_deriv_r(mode_i, fm2i, at(f, -1, 0), at(f, 0, 0), at(f, 1, 0), fp2i, UP, pec1,
+                                            g.r.ih1);
+            const double det = conv_deriv_r(mode_j, fm2j, at(f, 0, -1), at(f, 0, 0), at(f, 0, 1), fp2j, VP, pec2,
+                                            g.r.ih2);
             conv[k] = UP * dxi + VP * det;
         }
-        const double dp_dxi = (p[P + 1] - p[P - 1]) * g.r.i2h1;
-        const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) * g.r.i2h2;
-        const double dp_dx = xix[P] * dp_dxi + etx[P] * dp_de;
-        const double dp_dy = xiy[P] * dp_dxi + ety[P] * dp_de;
+        const double dp_dxi = (pat(1, 0) - pat(-1, 0)) * g.r.i2h1;
+        const double dp_de = (pat(0, 1) - pat(0, -1)) * g.r.i2h2;
+        const double dp_dx = c.xix * dp_dxi + c.etx * dp_de;
+        const double dp_dy = c.xiy * dp_dxi + c.ety * dp_de;
         const double dt = ph.dt;
-        const double ru = dt * (-conv[0] - dp_dx) - aref_offdiag_r(u, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnu, hyb);
-        const double rv = dt * (-conv[1] - dp_dy + ph.Ri * T[P]) -
-                          aref_offdiag_r(v, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnu, hyb);
-        const double rt = -dt * conv[2] - aref_offdiag_r(T, n0, i, j, al, be, ga, pp, qq, g.r, ph.dnt, hyb);
+        auto A_ = [&](const double *f, double dn) {
+            return aref_offdiag_a([&](int di, int dj) { return at(f, di, dj); }, al, be, ga, pp, qq, g.r, dn, hyb);
+        };
+        const double ru = dt * (-conv[0] - dp_dx) - A_(s_u, ph.dnu);
+        const double rv = dt * (-conv[1] - dp_dy + ph.Ri * at(s_t, 0, 0)) - A_(s_v, ph.dnu);
+        const double rt = -dt * conv[2] - A_(s_t, ph.dnt);
         Ru[P] = (TR)ru;
         Rv[P] = (TR)rv;
         RT[P] = (TR)rt;
@@ -174,8 +221,8 @@ __global__ __launch_bounds__(NT) void k_residual(
         double cl[9];
         alo_coefs_r<double>(al, be, ga, pp, qq, UP, VP, g.r, dt, ph.dnu, cl);
         dd[P] = dt * ph.alpha_u / cl[4];
-        e_abs = fabs(rt);
-        m_abs = fabs(ru) + fabs(rv);
+        e_abs += fabs(rt);
+        m_abs += fabs(ru) + fabs(rv);
     }
     double sums[2] = {e_abs, m_abs};
     block_partials<2>(sums, ra.part, ra.pstride);
@@ -257,16 +304,26 @@ struct JArgs {
 
 constexpr int JT_X = 64, JT_Y = 16;  // output tile of k_jacobi2
 
-// one sweep at interior node (i, j) = P; `dat(f, i, j)` reads the old correction
-// of field f (0 u, 1 v, 2 T) at a neighbour
+// the node's coefficient inputs and right-hand sides
+template <typename TR>
+struct JNode {
+    TR al, be, ga, pp, qq, U, V, ru, rv, rt;
+};
+template <typename TR>
+__device__ __forceinline__ JNode<TR> jac_load(const JArgs<TR> &a, size_t P) {
+    return JNode<TR>{a.alph[P], a.beta[P], a.gamma[P], a.p1[P], a.q1[P], a.Ua[P], a.Va[P], a.Ru[P], a.Rv[P], a.RT[P]};
+}
+
+// one sweep at interior node (i, j); `dat(f, i, j)` reads the old correction of
+// field f (0 u, 1 v, 2 T) at a neighbour
 template <typename TR, bool FIRST, typename F>
-__device__ __forceinline__ void jac_node(const JArgs<TR> &a, const SlotP &ph, const RecipT<TR> &r, size_t P, int i,
-                                         int j, F dat, TR &ou, TR &ov, TR &ot) {
+__device__ __forceinline__ void jac_node(const JNode<TR> &c, const SlotP &ph, const RecipT<TR> &r, int i, int j,
+                                         F dat, TR &ou, TR &ov, TR &ot) {
     const TR dt = (TR)ph.dt;
     TR cu[9], ct[9];
-    alo_coefs_r<TR>(a.alph[P], a.beta[P], a.gamma[P], a.p1[P], a.q1[P], a.Ua[P], a.Va[P], r, dt, (TR)ph.dnu, cu);
-    alo_coefs_r<TR>(a.alph[P], a.beta[P], a.gamma[P], a.p1[P], a.q1[P], a.Ua[P], a.Va[P], r, dt, (TR)ph.dnt, ct);
-    TR su = a.Ru[P], sv = a.Rv[P], st = a.RT[P];
+    alo_coefs_r<TR>(c.al, c.be, c.ga, c.pp, c.qq, c.U, c.V, r, dt, (TR)ph.dnu, cu);
+    alo_coefs_r<TR>(c.al, c.be, c.ga, c.pp, c.qq, c.U, c.V, r, dt, (TR)ph.dnt, ct);
+    TR su = c.ru, sv = c.rv, st = c.rt;
     if (!FIRST) {
 #pragma unroll
         for (int dj = -1; dj <= 1; dj++)
@@ -285,10 +342,14 @@ __device__ __forceinline__ void jac_node(const JArgs<TR> &a, const SlotP &ph, co
     ot = st * it;
 }
 
+// The old correction (halo 2) is staged in LDS and the coefficients of the
+// first sweep's region (halo 1) are loaded into registers, all loads issued
+// together; the second sweep runs on the inner part of the same elements, so a
+// node's coefficients are read once.  Halo entries beyond the grid hold clamped
+// copies that no interior node reads.
 template <typename TR, bool FIRST>
 __global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
     const SlotP ph = g.sp[blockIdx.z];
-    if (ph.active == 0.0) return;
     {
         const size_t o = zslot() * g.N;
         a.Ru += o; a.Rv += o; a.RT += o; a.Ua += o; a.Va += o; a.alph += o; a.beta += o; a.gamma += o;
@@ -296,44 +357,56 @@ __global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
     }
     constexpr int AX = JT_X + 4, AY = JT_Y + 4;  // old correction, halo 2
     constexpr int MX = JT_X + 2, MY = JT_Y + 2;  // first sweep, halo 1
-    constexpr int NA = AX * AY, NM = MX * MY;
-    __shared__ TR s_old[FIRST ? 1 : 3 * NA];
+    constexpr int NA = FIRST ? 1 : AX * AY, NM = MX * MY;
+    __shared__ TR s_old[3 * NA];
     __shared__ TR s_mid[3 * NM];
     const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
     const int fi0 = blockIdx.x * JT_X, fj0 = blockIdx.y * JT_Y;
     const RecipT<TR> r = recip_cast<TR>(g.r);
+    struct O3 {
+        TR u, v, t;
+    };
+    const auto lo = batch_load<NA, NT>(tid, [&](int k) {
+        if (FIRST) return O3{TR(0), TR(0), TR(0)};
+        const size_t Q = idx(clampi(fi0 - 2 + k % AX, 0, n0 - 1), clampi(fj0 - 2 + k / AX, 0, n1 - 1), n0);
+        return O3{a.du[Q], a.dv[Q], a.dT[Q]};
+    });
+    const auto lc = batch_load<NM, NT>(tid, [&](int k) {
+        return jac_load<TR>(a, idx(clampi(fi0 - 1 + k % MX, 0, n0 - 1), clampi(fj0 - 1 + k / MX, 0, n1 - 1), n0));
+    });
+    if (ph.active == 0.0) return;
     if (!FIRST) {
-        const TR *src[3] = {a.du, a.dv, a.dT};
-        for (int k = tid; k < NA; k += NT) {
-            const int gi = fi0 - 2 + k % AX, gj = fj0 - 2 + k / AX;
-            const bool in = gi >= 0 && gi < n0 && gj >= 0 && gj < n1;
-#pragma unroll
-            for (int f = 0; f < 3; f++) s_old[f * NA + k] = in ? src[f][idx(gi, gj, n0)] : TR(0);
-        }
+        batch_run<NA, NT>(tid, lo, [&](int k, const O3 &v) {
+            s_old[k] = v.u;
+            s_old[NA + k] = v.v;
+            s_old[2 * NA + k] = v.t;
+        });
         __syncthreads();
     }
     auto old_at = [&](int f, int ii, int jj) { return s_old[f * NA + (jj - fj0 + 2) * AX + (ii - fi0 + 2)]; };
-    for (int k = tid; k < NM; k += NT) {
+    batch_run<NM, NT>(tid, lc, [&](int k, const JNode<TR> &c) {
         const int gi = fi0 - 1 + k % MX, gj = fj0 - 1 + k / MX;
         TR ou = TR(0), ov = TR(0), ot = TR(0);
-        if (inner_node(gi, gj, n0, n1)) jac_node<TR, FIRST>(a, ph, r, idx(gi, gj, n0), gi, gj, old_at, ou, ov, ot);
+        if (inner_node(gi, gj, n0, n1)) jac_node<TR, FIRST>(c, ph, r, gi, gj, old_at, ou, ov, ot);
         s_mid[k] = ou;
         s_mid[NM + k] = ov;
         s_mid[2 * NM + k] = ot;
-    }
+    });
     __syncthreads();
     auto mid_at = [&](int f, int ii, int jj) { return s_mid[f * NM + (jj - fj0 + 1) * MX + (ii - fi0 + 1)]; };
-    for (int k = tid; k < JT_X * JT_Y; k += NT) {
-        const int gi = fi0 + k % JT_X, gj = fj0 + k / JT_X;
-        if (!inner_node(gi, gj, n0, n1)) continue;
+    batch_run<NM, NT>(tid, lc, [&](int k, const JNode<TR> &c) {
+        const int li = k % MX, lj = k / MX;
+        if (li < 1 || li > JT_X || lj < 1 || lj > JT_Y) return;
+        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
+        if (!inner_node(gi, gj, n0, n1)) return;
         const size_t P = idx(gi, gj, n0);
         TR ou, ov, ot;
-        jac_node<TR, false>(a, ph, r, P, gi, gj, mid_at, ou, ov, ot);
+        jac_node<TR, false>(c, ph, r, gi, gj, mid_at, ou, ov, ot);
         a.nu[P] = ou;
         a.nv[P] = ov;
         a.nT[P] = ot;
-    }
+    });
 }
 
 // u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u.
@@ -373,59 +446,76 @@ __global__ __launch_bounds__(NT) void k_star(
 }
 
 // q = J-weighted divergence of the face velocities (oracle: face_div)
+// Tiled: the nine input arrays of a 64 x 4 tile and its one-node halo are staged
+// in LDS with coalesced loads issued together (halo entries beyond the grid hold
+// clamped copies that no interior node reads); each face is then evaluated from
+// LDS with the same arithmetic as before, a wall face replaced by zero.
 __global__ __launch_bounds__(NT) void k_face_div(
     CGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
     const double *__restrict__ d, const double *__restrict__ xix, const double *__restrict__ xiy,
     const double *__restrict__ etx, const double *__restrict__ ety, const double *__restrict__ jac,
     double *__restrict__ q, RedArgs ra) {
-    if (g.sp[blockIdx.z].active == 0.0) return;
+    const bool active = g.sp[blockIdx.z].active != 0.0;
     ra = rz(ra);
-    {
-        const size_t o = zslot() * g.N;
-        uh += o; vh += o; p += o; d += o; xix += o; xiy += o; etx += o; ety += o; jac += o; q += o;
-    }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const size_t o = zslot() * g.N;
+    q += o;
+    constexpr int HX = BX + 2, HY = BY + 2, NA = 9;
+    enum { UH, VH, PP, DD, XX, XY, EX, EY, JJ };
+    __shared__ double s[NA][HX * HY];
+    const double *src[NA] = {uh + o, vh + o, p + o, d + o, xix + o, xiy + o, etx + o, ety + o, jac + o};
     const int n0 = g.n0, n1 = g.n1;
+    const int tid = tlin();
+    const int i0 = blockIdx.x * BX, j0 = blockIdx.y * BY;
+    struct V9 {
+        double a[NA];
+    };
+    const auto lv = batch_load<HX * HY, NT>(tid, [&](int k) {
+        const size_t Q = idx(clampi(i0 - 1 + k % HX, 0, n0 - 1), clampi(j0 - 1 + k / HX, 0, n1 - 1), n0);
+        V9 r;
+#pragma unroll
+        for (int a = 0; a < NA; a++) r.a[a] = src[a][Q];
+        return r;
+    });
+    if (!active) return;
+    batch_run<HX * HY, NT>(tid, lv, [&](int k, const V9 &r) {
+#pragma unroll
+        for (int a = 0; a < NA; a++) s[a][k] = r.a[a];
+    });
+    __syncthreads();
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = i0 + li, j = j0 + lj;
     double aq = 0.0;
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-        const int ipp = i + 1, inn = i - 1, jpp = j + 1, jnn = j - 1;
-        const size_t P = idx(i, j, n0), E = idx(ipp, j, n0), W = idx(inn, j, n0), Nn = idx(i, jpp, n0),
-                     S = idx(i, jnn, n0);
-        double fe = 0.0, fw = 0.0, gn = 0.0, gs = 0.0;
-        if (ipp != n0 - 1) {
-            const double de = 0.5 * (d[P] + d[E]);
-            const double dpdxi = (p[E] - p[P]) * g.r.ih1;
-            const double dpde = (p[idx(ipp, jpp, n0)] + p[Nn] - p[S] - p[idx(ipp, jnn, n0)]) * (0.25 * g.r.ih2);
-            const double uf = 0.5 * (uh[P] + uh[E]) - 0.5 * de * ((xix[P] + xix[E]) * dpdxi + (etx[P] + etx[E]) * dpde);
-            const double vf = 0.5 * (vh[P] + vh[E]) - 0.5 * de * ((xiy[P] + xiy[E]) * dpdxi + (ety[P] + ety[E]) * dpde);
-            fe = 0.5 * (jac[P] * xix[P] + jac[E] * xix[E]) * uf + 0.5 * (jac[P] * xiy[P] + jac[E] * xiy[E]) * vf;
-        }
-        if (inn != 0) {
-            const double dw = 0.5 * (d[P] + d[W]);
-            const double dpdxi = (p[P] - p[W]) * g.r.ih1;
-            const double dpde = (p[Nn] + p[idx(inn, jpp, n0)] - p[S] - p[idx(inn, jnn, n0)]) * (0.25 * g.r.ih2);
-            const double uf = 0.5 * (uh[P] + uh[W]) - 0.5 * dw * ((xix[P] + xix[W]) * dpdxi + (etx[P] + etx[W]) * dpde);
-            const double vf = 0.5 * (vh[P] + vh[W]) - 0.5 * dw * ((xiy[P] + xiy[W]) * dpdxi + (ety[P] + ety[W]) * dpde);
-            fw = 0.5 * (jac[P] * xix[P] + jac[W] * xix[W]) * uf + 0.5 * (jac[P] * xiy[P] + jac[W] * xiy[W]) * vf;
-        }
-        if (jpp != n1 - 1) {
-            const double dn = 0.5 * (d[P] + d[Nn]);
-            const double dpdxi = (p[idx(ipp, jpp, n0)] - p[idx(inn, jpp, n0)] + p[E] - p[W]) * (0.25 * g.r.ih1);
-            const double dpde = (p[Nn] - p[P]) * g.r.ih2;
-            const double uf = 0.5 * (uh[P] + uh[Nn]) - 0.5 * dn * ((xix[P] + xix[Nn]) * dpdxi + (etx[P] + etx[Nn]) * dpde);
-            const double vf = 0.5 * (vh[P] + vh[Nn]) - 0.5 * dn * ((xiy[P] + xiy[Nn]) * dpdxi + (ety[P] + ety[Nn]) * dpde);
-            gn = 0.5 * (jac[P] * etx[P] + jac[Nn] * etx[Nn]) * uf + 0.5 * (jac[P] * ety[P] + jac[Nn] * ety[Nn]) * vf;
-        }
-        if (jnn != 0) {
-            const double ds = 0.5 * (d[P] + d[S]);
-            const double dpdxi = (p[E] - p[W] + p[idx(ipp, jnn, n0)] - p[idx(inn, jnn, n0)]) * (0.25 * g.r.ih1);
-            const double dpde = (p[P] - p[S]) * g.r.ih2;
-            const double uf = 0.5 * (uh[P] + uh[S]) - 0.5 * ds * ((xix[P] + xix[S]) * dpdxi + (etx[P] + etx[S]) * dpde);
-            const double vf = 0.5 * (vh[P] + vh[S]) - 0.5 * ds * ((xiy[P] + xiy[S]) * dpdxi + (ety[P] + ety[S]) * dpde);
-            gs = 0.5 * (jac[P] * etx[P] + jac[S] * etx[S]) * uf + 0.5 * (jac[P] * ety[P] + jac[S] * ety[S]) * vf;
-        }
+        // array a at the node offset (di, dj) from (i, j)
+        auto L = [&](int a, int di, int dj) { return s[a][(lj + 1 + dj) * HX + (li + 1 + di)]; };
+        // face between (i, j) and its neighbour (di, dj) (one of them zero): the
+        // face-normal velocity times the J-weighted metric of direction xi (di != 0)
+        // or eta (dj != 0)
+        auto face = [&](int di, int dj, double dpdxi, double dpde) {
+            const double dm = 0.5 * (L(DD, 0, 0) + L(DD, di, dj));
+            const double uf = 0.5 * (L(UH, 0, 0) + L(UH, di, dj)) -
+                              0.5 * dm * ((L(XX, 0, 0) + L(XX, di, dj)) * dpdxi + (L(EX, 0, 0) + L(EX, di, dj)) * dpde);
+            const double vf = 0.5 * (L(VH, 0, 0) + L(VH, di, dj)) -
+                              0.5 * dm * ((L(XY, 0, 0) + L(XY, di, dj)) * dpdxi + (L(EY, 0, 0) + L(EY, di, dj)) * dpde);
+            const int mx = di != 0 ? XX : EX, my = di != 0 ? XY : EY;
+            return 0.5 * (L(JJ, 0, 0) * L(mx, 0, 0) + L(JJ, di, dj) * L(mx, di, dj)) * uf +
+                   0.5 * (L(JJ, 0, 0) * L(my, 0, 0) + L(JJ, di, dj) * L(my, di, dj)) * vf;
+        };
+        auto P_ = [&](int di, int dj) { return L(PP, di, dj); };
+        double fe = face(1, 0, (P_(1, 0) - P_(0, 0)) * g.r.ih1,
+                         (P_(1, 1) + P_(0, 1) - P_(0, -1) - P_(1, -1)) * (0.25 * g.r.ih2));
+        double fw = face(-1, 0, (P_(0, 0) - P_(-1, 0)) * g.r.ih1,
+                         (P_(0, 1) + P_(-1, 1) - P_(0, -1) - P_(-1, -1)) * (0.25 * g.r.ih2));
+        double gn = face(0, 1, (P_(1, 1) - P_(-1, 1) + P_(1, 0) - P_(-1, 0)) * (0.25 * g.r.ih1),
+                         (P_(0, 1) - P_(0, 0)) * g.r.ih2);
+        double gs = face(0, -1, (P_(1, 0) - P_(-1, 0) + P_(1, -1) - P_(-1, -1)) * (0.25 * g.r.ih1),
+                         (P_(0, 0) - P_(0, -1)) * g.r.ih2);
+        fe = i + 1 != n0 - 1 ? fe : 0.0;
+        fw = i - 1 != 0 ? fw : 0.0;
+        gn = j + 1 != n1 - 1 ? gn : 0.0;
+        gs = j - 1 != 0 ? gs : 0.0;
         const double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
-        q[P] = qq;
+        q[idx(i, j, n0)] = qq;
         aq = fabs(qq);
     }
     double v[1] = {aq};
@@ -947,8 +1037,9 @@ dim3 grid_b(const Case *c) {
 
 // one-block-per-slot finish of a case-level reduction over the grid's block partials
 template <int OP>
-int cav_fin(Case *c, int gs = -1, double rtol = 0.0, double margin = 0.5) {
-    k_cav_fin<OP><<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(gs, red(c), nblk(c->n0, c->n1), c->n0, c->n1, rtol, margin);
+int cav_fin(Case *c, int gs = -1, double rtol = 0.0, double margin = 0.5, int nparts = -1) {
+    if (nparts < 0) nparts = nblk(c->n0, c->n1);
+    k_cav_fin<OP><<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(gs, red(c), nparts, c->n0, c->n1, rtol, margin);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1076,12 +1167,12 @@ WallIn wall_in(const Case *c) {
 template <typename TR>
 int launch_residual_t(Case *c) {
     prof_begin(c, 2);
-    k_residual<TR><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), (TR *)c->Ru,
-                                                          (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V,
-                                                          red(c));
+    const dim3 gr = grid_res(c->n0, c->n1, c->B);
+    k_residual<TR><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), (TR *)c->Ru,
+                                                   (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c));
     prof_end(c);
     CFD_LAUNCH_CHECK();
-    return cav_fin<CF_RES>(c);
+    return cav_fin<CF_RES>(c, -1, 0.0, 0.5, (int)(gr.x * gr.y));
 }
 int launch_residual(Case *c) { return c->mixed ? launch_residual_t<float>(c) : launch_residual_t<double>(c); }
 

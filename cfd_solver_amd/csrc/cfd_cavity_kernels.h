@@ -58,11 +58,11 @@ __device__ __forceinline__ double conv_deriv_r(int mode, double fm2, double fm1,
     return vel > 0.0 ? d4 + d4p : d4 - d4p;
 }
 
-// (A_ref - I) phi at interior node (i, j) with dn = dt / nd (nd = Re or Re Pr)
-__device__ __forceinline__ double aref_offdiag_r(const double *__restrict__ f, int n0, int i, int j, double al,
-                                                 double be, double ga, double pp, double qq,
+// (A_ref - I) phi at an interior node with dn = dt / nd (nd = Re or Re Pr); F(di, dj)
+// reads phi at the neighbour (i + di, j + dj)
+template <typename FA>
+__device__ __forceinline__ double aref_offdiag_a(FA F, double al, double be, double ga, double pp, double qq,
                                                  const RecipT<double> &r, double dn, bool wide) {
-#define F(di, dj) f[idx(i + (di), j + (dj), n0)]
     const double A = al * r.ih1sq, G = ga * r.ih2sq;
     if (!wide) {
         const double Pp = pp * r.i2h1, Qq = qq * r.i2h2;
@@ -95,7 +95,13 @@ __device__ __forceinline__ double aref_offdiag_r(const double *__restrict__ f, i
            c22 * (F(2, 2) + F(-2, -2)) + c2m * (F(-2, 2) + F(2, -2)) +
            c12 * (F(1, 2) + F(-1, -2) + F(2, 1) + F(-2, -1)) +
            c1m * (F(-1, 2) + F(1, -2) + F(-2, 1) + F(2, -1));
-#undef F
+}
+
+__device__ __forceinline__ double aref_offdiag_r(const double *__restrict__ f, int n0, int i, int j, double al,
+                                                 double be, double ga, double pp, double qq,
+                                                 const RecipT<double> &r, double dn, bool wide) {
+    return aref_offdiag_a([&](int di, int dj) { return f[idx(i + di, j + dj, n0)]; }, al, be, ga, pp, qq, r, dn,
+                          wide);
 }
 
 // low-order predictor row with reciprocals: dn = dt / nd

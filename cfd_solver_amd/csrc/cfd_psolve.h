@@ -125,12 +125,20 @@ template <int NV, int NT = PS_NT>
 __device__ __forceinline__ void sum_partials(const double *part, int pstride, int nparts, double (&tot)[NV]) {
     constexpr int U = 8;
     const int t = tlin();
+    const int last = max(nparts - 1, 0);
     double s[NV][U];
+    {  // the first pass, outside the loop: straight-line loads the compiler issues early
+        double v[NV][U];
 #pragma unroll
-    for (int k = 0; k < NV; k++)
+        for (int k = 0; k < NV; k++)
 #pragma unroll
-        for (int u = 0; u < U; u++) s[k][u] = 0.0;
-    for (int t0 = 0; t0 < nparts; t0 += U * NT) {
+            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(u * NT + t, last)];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+#pragma unroll
+            for (int u = 0; u < U; u++) s[k][u] = u * NT + t < nparts ? v[k][u] : 0.0;
+    }
+    for (int t0 = U * NT; t0 < nparts; t0 += U * NT) {
         double v[NV][U];
 #pragma unroll
         for (int k = 0; k < NV; k++)
@@ -228,48 +236,6 @@ __global__ __launch_bounds__(PS_NT) void k_fapply(FArgs<T> a, const T *__restric
 // ---------------------------------------------------------------------------
 // per-outer-iteration set-up of every level in one launch each
 
-// Face (normal kn, tangential kt) between node (i, j) of level `lv` and its east
-// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face; d is level 0's
-// d read at (i << lv, j << lv) (injection).  Zero on faces touching a wall node.
-__device__ __forceinline__ void p9_face_lv(int n0, int n1, const double *__restrict__ xix,
-                                           const double *__restrict__ xiy, const double *__restrict__ etx,
-                                           const double *__restrict__ ety, const double *__restrict__ jac,
-                                           const double *__restrict__ d0, int dn0, int lv, double h1, double h2,
-                                           int i, int j, int dir, double &kn, double &kt) {
-    kn = 0.0;
-    kt = 0.0;
-    const double half = 0.5, four = 4.0;
-    const size_t P = idx(i, j, n0);
-    if (dir == 0) {
-        if (i < 1 || i > n0 - 3 || j < 1 || j > n1 - 2) return;
-    } else {
-        if (j < 1 || j > n1 - 3 || i < 1 || i > n0 - 2) return;
-    }
-    const double xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P];
-    const double dP = d0[idx(i << lv, j << lv, dn0)];
-    if (dir == 0) {
-        const size_t E = P + 1;
-        const double xE = xix[E], yE = xiy[E], eE = etx[E], fE = ety[E], jE = jac[E];
-        const double mx = half * (jP * xP + jE * xE), my = half * (jP * yP + jE * yE);
-        const double a = mx * (xP + xE) + my * (yP + yE);
-        const double b = mx * (eP + eE) + my * (fP + fE);
-        const double de = half * (dP + d0[idx((i + 1) << lv, j << lv, dn0)]);
-        kn = half * de * a / (h1 * h1);
-        kt = half * de * b / (four * h1 * h2);
-    } else {
-        const size_t Nn = P + n0;
-        const double xN = xix[Nn], yN = xiy[Nn], eN = etx[Nn], fN = ety[Nn], jN = jac[Nn];
-        const double mx = half * (jP * eP + jN * eN), my = half * (jP * fP + jN * fN);
-        const double a = mx * (eP + eN) + my * (fP + fN);
-        const double b = mx * (xP + xN) + my * (yP + yN);
-        const double dn = half * (dP + d0[idx(i << lv, (j + 1) << lv, dn0)]);
-        kn = half * dn * a / (h2 * h2);
-        kt = half * dn * b / (four * h1 * h2);
-    }
-}
-
-// all levels of one solver, 1-D block numbering (block b belongs to the last
-// level whose first block is <= b)
 template <typename T>
 struct LevTab {
     int nlev;
@@ -295,6 +261,48 @@ __device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i,
     return i < A.n0[l] && j < A.n1[l];
 }
 
+// Face (normal kn, tangential kt) between node (i, j) of level `lv` and its east
+// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face; d is level 0's
+// d read at (i << lv, j << lv) (injection).  Zero on faces touching a wall node.
+// No branch precedes the loads: the indices are clamped into the grid, every load
+// is issued, and a wall face is zeroed after the arithmetic.
+__device__ __forceinline__ void p9_face_lv(int n0, int n1, const double *__restrict__ xix,
+                                              const double *__restrict__ xiy, const double *__restrict__ etx,
+                                              const double *__restrict__ ety, const double *__restrict__ jac,
+                                              const double *__restrict__ d0, int dn0, int lv, double h1, double h2,
+                                              int i, int j, int dir, double &kn, double &kt) {
+    const bool valid = dir == 0 ? (i >= 1 && i <= n0 - 3 && j >= 1 && j <= n1 - 2)
+                                : (j >= 1 && j <= n1 - 3 && i >= 1 && i <= n0 - 2);
+    const int ic = clampi(i, 0, dir == 0 ? n0 - 2 : n0 - 1), jc = clampi(j, 0, dir == 0 ? n1 - 1 : n1 - 2);
+    const int ie = dir == 0 ? ic + 1 : ic, je = dir == 0 ? jc : jc + 1;
+    const double half = 0.5, four = 4.0;
+    const size_t P = idx(ic, jc, n0), Q = idx(ie, je, n0);
+    const double xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P];
+    const double xQ = xix[Q], yQ = xiy[Q], eQ = etx[Q], fQ = ety[Q], jQ = jac[Q];
+    const double dP = d0[idx(ic << lv, jc << lv, dn0)], dQ = d0[idx(ie << lv, je << lv, dn0)];
+    if (dir == 0) {
+        const double mx = half * (jP * xP + jQ * xQ), my = half * (jP * yP + jQ * yQ);
+        const double a = mx * (xP + xQ) + my * (yP + yQ);
+        const double b = mx * (eP + eQ) + my * (fP + fQ);
+        const double de = half * (dP + dQ);
+        kn = half * de * a / (h1 * h1);
+        kt = half * de * b / (four * h1 * h2);
+    } else {
+        const double mx = half * (jP * eP + jQ * eQ), my = half * (jP * fP + jQ * fQ);
+        const double a = mx * (eP + eQ) + my * (fP + fQ);
+        const double b = mx * (xP + xQ) + my * (yP + yQ);
+        const double dn = half * (dP + dQ);
+        kn = half * dn * a / (h2 * h2);
+        kt = half * dn * b / (four * h1 * h2);
+    }
+    kn = valid ? kn : 0.0;
+    kt = valid ? kt : 0.0;
+}
+
+// Face coefficients of every node of every level, and the inverse diagonal of
+// the clamped operator (every coefficient whose neighbour clamps onto the node)
+// at interior nodes: the thread also evaluates its west neighbour's east face and
+// its south neighbour's north face, so the row needs no second launch.
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<T> A) {
     if (slot_skip(scal + zslot() * SS, -1)) return;
@@ -304,11 +312,14 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
     const size_t o = zslot() * n0 * n1, o0 = zslot() * A.dn0 * A.n1[0];
     const size_t P = o + idx(i, j, n0);
     const double *d0 = A.d0 + o0;
-    double en, et, nn, nt;
-    p9_face_lv(n0, n1, A.xix[l] + o, A.xiy[l] + o, A.etx[l] + o, A.ety[l] + o, A.jac[l] + o, d0, A.dn0, l, A.h1[l],
-               A.h2[l], i, j, 0, en, et);
-    p9_face_lv(n0, n1, A.xix[l] + o, A.xiy[l] + o, A.etx[l] + o, A.ety[l] + o, A.jac[l] + o, d0, A.dn0, l, A.h1[l],
-               A.h2[l], i, j, 1, nn, nt);
+    const double *xix = A.xix[l] + o, *xiy = A.xiy[l] + o, *etx = A.etx[l] + o, *ety = A.ety[l] + o,
+                 *jac = A.jac[l] + o;
+    const double h1 = A.h1[l], h2 = A.h2[l];
+    double en, et, nn, nt, wn, wt, sn, st;
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j, 0, en, et);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j, 1, nn, nt);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i - 1, j, 0, wn, wt);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j - 1, 1, sn, st);
     A.fen[l][P] = (T)en;
     A.fet[l][P] = (T)et;
     A.fnn[l][P] = (T)nn;
@@ -319,25 +330,14 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
         A.f64[2][P] = nn;
         A.f64[3][P] = nt;
     }
-}
-
-// inverse of the clamped operator's diagonal (every coefficient whose neighbour
-// clamps onto P), all levels
-template <typename T>
-__global__ __launch_bounds__(PS_NT) void k_invd_all(const double *scal, LevTab<T> A) {
-    if (slot_skip(scal + zslot() * SS, -1)) return;
-    int l, i, j;
-    if (!lev_of_block(A, l, i, j)) return;
-    const int n0 = A.n0[l], n1 = A.n1[l];
     if (!inner_node(i, j, n0, n1)) return;
-    const size_t o = zslot() * n0 * n1;
     T c[9];
-    p9_row_faces<T>(n0, A.fen[l] + o, A.fet[l] + o, A.fnn[l] + o, A.fnt[l] + o, i, j, c);
+    p9_row_from<T>(RowFaces<T>{(T)en, (T)et, (T)wn, (T)wt, (T)nn, (T)nt, (T)sn, (T)st}, c);
     T s = T(0);
     for (int dj = -1; dj <= 1; dj++)
         for (int di = -1; di <= 1; di++)
             if (clampi(i + di, 1, n0 - 2) == i && clampi(j + dj, 1, n1 - 2) == j) s += c[(dj + 1) * 3 + (di + 1)];
-    A.invd[l][o + idx(i, j, n0)] = T(1) / s;
+    A.invd[l][P] = T(1) / s;
 }
 
 // ---------------------------------------------------------------------------
@@ -1354,7 +1354,6 @@ struct PSolver {
         for (int k = 0; k < 4; k++) t.f64[k] = f64 ? f64[k] : nullptr;
         const dim3 gr(tab_blocks, 1, B);
         k_faces_all<T><<<gr, PS_NT, 0, cx.s>>>(cx.scal, t);
-        k_invd_all<T><<<gr, PS_NT, 0, cx.s>>>(cx.scal, t);
         CFD_LAUNCH_CHECK();
         return CFD_OK;
     }
