@@ -762,11 +762,19 @@ __global__ __launch_bounds__(NT) void k_dmax(CGrid g, const double *__restrict__
     if (tlin() == 0) part[blin()] = dm;
 }
 
-// one block: dmax from the partials, floored at 0 (solver.cpp:1806), and both
-// wall Nusselt numbers
-__global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, const double *__restrict__ T,
-                                                    const double *__restrict__ ajac, const double *__restrict__ etx,
-                                                    const double *__restrict__ ety) {
+// Partial-sum slots (pstride each) per case slot: 0-2 for reductions finished
+// right away, 3-4 for the energy/momentum residual norms and 5 for the mass
+// residual, which stay there until the end-of-iteration finish below.
+enum { PS_RES = 3, PS_MASS = 5, PS_SLOTS = 6 };
+
+// one block: dmax from the partials, floored at 0 (solver.cpp:1806), both wall
+// Nusselt numbers, and the energy, momentum (nres partials) and mass (nparts
+// partials) residual norms of the iteration
+__global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, int nres,
+                                                        const double *__restrict__ T,
+                                                        const double *__restrict__ ajac,
+                                                        const double *__restrict__ etx,
+                                                        const double *__restrict__ ety) {
     const SlotP &sp = cg.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     ra = rz(ra);
@@ -778,6 +786,9 @@ __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, in
         const size_t o = zslot() * cg.N;
         T += o; ajac += o; etx += o; ety += o;
     }
+    double res[2], mass[1];
+    sum_partials<2, FIN_NT>(part + (size_t)PS_RES * ra.pstride, ra.pstride, nres, res);
+    sum_partials<1, FIN_NT>(part + (size_t)PS_MASS * ra.pstride, ra.pstride, nparts, mass);
     double m = -1.0e300;
     for (int t = tlin(); t < nparts; t += FIN_NT) m = fmax(m, part[t]);
     m = block_max<FIN_NT>(m);
@@ -785,6 +796,9 @@ __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, in
     const double nc = nusselt_wall<FIN_NT>(g, T, ajac, etx, ety, 1);
     if (tlin() == 0) {
         const double den = dT != 0.0 ? dT : 1.0;
+        scal[S_ENERGY] = res[0];
+        scal[S_MOM] = res[1];
+        scal[S_MASS] = mass[0];
         scal[S_DMAX] = fmax(m, 0.0);
         scal[S_NUH] = -nh / (len_hot * den);
         scal[S_NUC] = nc / (len_cold * den);
@@ -913,8 +927,38 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
     block_partials<1>(v, ra.part, ra.pstride);
 }
 
+// End of a mixed p' solve with one refinement step: p' was zeroed before it, so e
+// is the whole correction.  One pass computes the FP64 defect q - L e (partials
+// of its sum and square, the defect norm with the mean removed), sets p' = e and
+// adds up the interior p' (third partial).  e is left as it is (the next solve
+// starts from zero).
+__global__ __launch_bounds__(NT) void k_defect_final(FArgs<double> a, const double *__restrict__ q,
+                                                     double *__restrict__ pc, const float *__restrict__ ef,
+                                                     RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, -1)) return;
+    {
+        const size_t o = zslot() * a.n0 * a.n1;
+        a = fz(a, o); q += o; pc += o; ef += o;
+    }
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < a.n0 && j < a.n1) {
+        const size_t P = idx(i, j, a.n0);
+        const double x = (double)ef[P];
+        if (inner_node(i, j, a.n0, a.n1)) {
+            const double y = q[P] - frow_f<double>(a, i, j, [&](int ii, int jj) { return (double)ef[idx(ii, jj, a.n0)]; });
+            v[0] = y;
+            v[1] = y * y;
+            v[2] = x;
+        }
+        pc[P] = x;
+    }
+    block_partials<3>(v, ra.part, ra.pstride);
+}
+
 // totals of the case-level reductions and what follows from them (one block)
-enum { CF_RES, CF_MASS, CF_Q, CF_DEFECT, CF_PSUM };
+enum { CF_Q, CF_DEFECT, CF_PSUM, CF_DEFECT_PSUM };
 // margin: fraction of the remaining FP64 tolerance the first FP32 solve aims at
 // (1 when no further refinement step follows it)
 template <int OP>
@@ -922,18 +966,14 @@ __global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int npar
                                                     double margin) {
     ra = rz(ra);
     const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
-    constexpr int NV = (OP == CF_MASS || OP == CF_PSUM) ? 1 : 2;
+    constexpr int NV = OP == CF_PSUM ? 1 : (OP == CF_DEFECT_PSUM ? 3 : 2);
     double tot[NV];
     sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
     if (skip || tlin() != 0) return;
     double *s = ra.scal;
     const double nin = (double)(n0 - 2) * (double)(n1 - 2);
-    if (OP == CF_RES) {
-        s[S_ENERGY] = tot[0];
-        s[S_MOM] = tot[NV - 1];
-    } else if (OP == CF_MASS) {
-        s[S_MASS] = tot[0];
-    } else if (OP == CF_PSUM) {
+    if (OP == CF_DEFECT_PSUM) s[S_PSUM] = tot[2];
+    if (OP == CF_PSUM) {
         s[S_PSUM] = tot[0];
     } else if (OP == CF_Q) {
         // ||q|| and ||q - mean(q)|| = sqrt(sum q^2 - sum(q) mean); refinement start
@@ -950,6 +990,8 @@ __global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int npar
         s[S_RTOL] = rn > 0.0 ? fmax(margin * rtol * bn / rn, 1e-5) : 1e-5;
     } else {
         // refinement test on the FP64 defect (mean removed) and the next inner tolerance
+        // (after a solve that the start test skipped there is nothing to add)
+        if (OP == CF_DEFECT_PSUM && s[S_RDONE] != 0.0) return;
         s[S_ITTOT] += fmax(s[S_IT], 1.0);
         const double rr = fmax(tot[NV - 1] - tot[0] * (tot[0] / nin), 0.0);
         const double rn = sqrt(rr);
@@ -1022,12 +1064,14 @@ void cb_begin(void *o, int kind) { prof_begin(static_cast<Case *>(o), kind); }
 void cb_end(void *o) { prof_end(static_cast<Case *>(o)); }
 void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 
-size_t psz(const Case *c) { return (size_t)4 * c->pstride; }
+size_t psz(const Case *c) { return (size_t)PS_SLOTS * c->pstride; }
 PsCtx ctx(Case *c) {
     return PsCtx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
                  &c->allocs, &c->bytes, c->dry};
 }
-RedArgs red(const Case *c) { return RedArgs{c->part, c->pstride, psz(c), c->scal}; }
+RedArgs red(const Case *c, int slot = 0) {
+    return RedArgs{c->part + (size_t)slot * c->pstride, c->pstride, psz(c), c->scal};
+}
 CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd, make_recip(c->h1, c->h2)}; }
 dim3 grid_b(const Case *c) {
     dim3 g = grid2d(c->n0, c->n1);
@@ -1141,6 +1185,11 @@ int psolve(Case *c, double rtol, int maxit, bool graph) {
         }
         RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, S_RDONE,
                            graph));
+        if (R == 1) {  // defect, p' = e and its sum in one pass
+            k_defect_final<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, red(c));
+            CFD_LAUNCH_CHECK();
+            return cav_fin<CF_DEFECT_PSUM>(c);
+        }
         k_defect<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, c->r64, red(c));
         CFD_LAUNCH_CHECK();
         RET(cav_fin<CF_DEFECT>(c, S_RDONE));
@@ -1169,10 +1218,11 @@ int launch_residual_t(Case *c) {
     prof_begin(c, 2);
     const dim3 gr = grid_res(c->n0, c->n1, c->B);
     k_residual<TR><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), (TR *)c->Ru,
-                                                   (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c));
+                                                   (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V,
+                                                   red(c, PS_RES));
     prof_end(c);
     CFD_LAUNCH_CHECK();
-    return cav_fin<CF_RES>(c, -1, 0.0, 0.5, (int)(gr.x * gr.y));
+    return CFD_OK;
 }
 int launch_residual(Case *c) { return c->mixed ? launch_residual_t<float>(c) : launch_residual_t<double>(c); }
 
@@ -1272,10 +1322,9 @@ int outer_body(Case *c, bool graph) {
     else RET(star_t<double>(c, du, dv, dT));
     prof_begin(c, 3);
     k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
-                                    c->q, red(c));
+                                    c->q, red(c, PS_MASS));
     prof_end(c);
     CFD_LAUNCH_CHECK();
-    RET(cav_fin<CF_MASS>(c));
     // 5 p' solve
     RET(mg_setup(c));
     RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph));
@@ -1286,8 +1335,11 @@ int outer_body(Case *c, bool graph) {
     RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax and the wall Nusselt numbers (FP64)
     k_dmax<<<gr, bl, 0, c->s>>>(cg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, red(c));
-    k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), c->T, c->ajac, c->etx,
-                                                     c->ety);
+    {
+        const dim3 grr = grid_res(c->n0, c->n1, 1);
+        k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), (int)(grr.x * grr.y),
+                                                             c->T, c->ajac, c->etx, c->ety);
+    }
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1854,7 +1906,7 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
             break;
         case 3:
             k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
-                                            c->ajac, c->q, red(c));
+                                            c->ajac, c->q, red(c, PS_MASS));
             break;
         }
     }
