@@ -55,6 +55,19 @@ def kernel_bytes(name, mixed):
     }[name]
 
 
+def hbm_traffic(mixed, name, n0, n1):
+    """HBM bytes per launch of kernel `name` at 1025 x 1025 from the committed counter
+    summary profiles/hbm_traffic_cavity1025.json (two rocprofv3 runs, FETCH_SIZE then
+    WRITE_SIZE, each scaled by its reading for a known-size copy), or None."""
+    if (n0, n1) != (1025, 1025):
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_cavity1025.json")) as f:
+            return json.load(f)["mixed" if mixed else "fp64"].get(name)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +142,7 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
     bytes_launch = int(kernel_bytes(name, mixed) * n0 * n1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(mixed, name, n0, n1), "kernel": name,
             "avg_launch_ms": round(avg_ms, 5), "launches": launches, "bytes_per_launch": bytes_launch,
             "timed_in": f"{prof_steps} outer iterations right after the timed steps",
             "profiled_ms_per_step": round(eager_ms, 4),

@@ -243,6 +243,7 @@ struct LevTab {
     int bx[PS_MAXLEV];
     int n0[PS_MAXLEV], n1[PS_MAXLEV];
     double h1[PS_MAXLEV], h2[PS_MAXLEV];
+    double ih1sq[PS_MAXLEV], ih2sq[PS_MAXLEV], i4h1h2[PS_MAXLEV];  // 1/h1^2, 1/h2^2, 1/(4 h1 h2)
     const double *xix[PS_MAXLEV], *xiy[PS_MAXLEV], *etx[PS_MAXLEV], *ety[PS_MAXLEV], *jac[PS_MAXLEV];
     T *fen[PS_MAXLEV], *fet[PS_MAXLEV], *fnn[PS_MAXLEV], *fnt[PS_MAXLEV], *invd[PS_MAXLEV];
     const double *d0;
@@ -262,38 +263,41 @@ __device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i,
 }
 
 // Face (normal kn, tangential kt) between node (i, j) of level `lv` and its east
-// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face; d is level 0's
-// d read at (i << lv, j << lv) (injection).  Zero on faces touching a wall node.
-// No branch precedes the loads: the indices are clamped into the grid, every load
-// is issued, and a wall face is zeroed after the arithmetic.
+// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face with the grid
+// divisions taken as host-computed reciprocals (ih1sq = 1/h1^2, ih2sq = 1/h2^2,
+// i4 = 1/(4 h1 h2)); d is level 0's d read at (i << lv, j << lv) (injection).
+// Zero on faces touching a wall node.  No branch precedes the loads: the indices
+// are clamped into the grid, every load is issued, and a wall face is zeroed
+// after the arithmetic.  Offsets are 32-bit within the slot's arrays.
 __device__ __forceinline__ void p9_face_lv(int n0, int n1, const double *__restrict__ xix,
-                                              const double *__restrict__ xiy, const double *__restrict__ etx,
-                                              const double *__restrict__ ety, const double *__restrict__ jac,
-                                              const double *__restrict__ d0, int dn0, int lv, double h1, double h2,
-                                              int i, int j, int dir, double &kn, double &kt) {
+                                           const double *__restrict__ xiy, const double *__restrict__ etx,
+                                           const double *__restrict__ ety, const double *__restrict__ jac,
+                                           const double *__restrict__ d0, int dn0, int lv, double ih1sq,
+                                           double ih2sq, double i4, int i, int j, int dir, double &kn, double &kt) {
     const bool valid = dir == 0 ? (i >= 1 && i <= n0 - 3 && j >= 1 && j <= n1 - 2)
                                 : (j >= 1 && j <= n1 - 3 && i >= 1 && i <= n0 - 2);
     const int ic = clampi(i, 0, dir == 0 ? n0 - 2 : n0 - 1), jc = clampi(j, 0, dir == 0 ? n1 - 1 : n1 - 2);
     const int ie = dir == 0 ? ic + 1 : ic, je = dir == 0 ? jc : jc + 1;
-    const double half = 0.5, four = 4.0;
-    const size_t P = idx(ic, jc, n0), Q = idx(ie, je, n0);
+    const double half = 0.5;
+    const unsigned P = (unsigned)jc * (unsigned)n0 + (unsigned)ic, Q = (unsigned)je * (unsigned)n0 + (unsigned)ie;
     const double xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P];
     const double xQ = xix[Q], yQ = xiy[Q], eQ = etx[Q], fQ = ety[Q], jQ = jac[Q];
-    const double dP = d0[idx(ic << lv, jc << lv, dn0)], dQ = d0[idx(ie << lv, je << lv, dn0)];
+    const double dP = d0[((unsigned)jc << lv) * (unsigned)dn0 + ((unsigned)ic << lv)];
+    const double dQ = d0[((unsigned)je << lv) * (unsigned)dn0 + ((unsigned)ie << lv)];
     if (dir == 0) {
         const double mx = half * (jP * xP + jQ * xQ), my = half * (jP * yP + jQ * yQ);
         const double a = mx * (xP + xQ) + my * (yP + yQ);
         const double b = mx * (eP + eQ) + my * (fP + fQ);
         const double de = half * (dP + dQ);
-        kn = half * de * a / (h1 * h1);
-        kt = half * de * b / (four * h1 * h2);
+        kn = half * de * a * ih1sq;
+        kt = half * de * b * i4;
     } else {
         const double mx = half * (jP * eP + jQ * eQ), my = half * (jP * fP + jQ * fQ);
         const double a = mx * (eP + eQ) + my * (fP + fQ);
         const double b = mx * (xP + xQ) + my * (yP + yQ);
         const double dn = half * (dP + dQ);
-        kn = half * dn * a / (h2 * h2);
-        kt = half * dn * b / (four * h1 * h2);
+        kn = half * dn * a * ih2sq;
+        kt = half * dn * b * i4;
     }
     kn = valid ? kn : 0.0;
     kt = valid ? kt : 0.0;
@@ -314,12 +318,12 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
     const double *d0 = A.d0 + o0;
     const double *xix = A.xix[l] + o, *xiy = A.xiy[l] + o, *etx = A.etx[l] + o, *ety = A.ety[l] + o,
                  *jac = A.jac[l] + o;
-    const double h1 = A.h1[l], h2 = A.h2[l];
+    const double r1 = A.ih1sq[l], r2 = A.ih2sq[l], r12 = A.i4h1h2[l];
     double en, et, nn, nt, wn, wt, sn, st;
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j, 0, en, et);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j, 1, nn, nt);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i - 1, j, 0, wn, wt);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, h1, h2, i, j - 1, 1, sn, st);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j, 0, en, et);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j, 1, nn, nt);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i - 1, j, 0, wn, wt);
+    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j - 1, 1, sn, st);
     A.fen[l][P] = (T)en;
     A.fet[l][P] = (T)et;
     A.fnn[l][P] = (T)nn;
@@ -1329,6 +1333,9 @@ struct PSolver {
             tab.n1[l] = g[l].n1;
             tab.h1[l] = g[l].h1;
             tab.h2[l] = g[l].h2;
+            tab.ih1sq[l] = 1.0 / (g[l].h1 * g[l].h1);
+            tab.ih2sq[l] = 1.0 / (g[l].h2 * g[l].h2);
+            tab.i4h1h2[l] = 1.0 / (4.0 * g[l].h1 * g[l].h2);
             tab.xix[l] = g[l].xix;
             tab.xiy[l] = g[l].xiy;
             tab.etx[l] = g[l].etx;
