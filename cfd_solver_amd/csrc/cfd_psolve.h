@@ -107,10 +107,10 @@ __device__ __forceinline__ bool inner_node(int i, int j, int n0, int n1) {
 // deterministic result) and evaluates what depends on the totals.  (A last-block
 // finish inside the first kernel needs a device-scope fence per block, which
 // costs more on gfx950 than the extra launch.)
-template <int NV>
+template <int NV, int NT = PS_NT>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double *part, int pstride) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = block_sum<PS_NT>(v[k]);
+    for (int k = 0; k < NV; k++) v[k] = block_sum<NT>(v[k]);
     if (tlin() == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) part[(size_t)k * pstride + blin()] = v[k];
@@ -353,6 +353,8 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
 // x = w b / diag (no neighbours).
 
 constexpr int MG_FX = 64, MG_FY = 16;  // fine tile
+constexpr int MG_NT_WIDE = 1024;       // threads per tile on levels with few tiles
+constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
 
 template <typename T>
@@ -394,8 +396,8 @@ constexpr bool mg_early() {
     return sizeof(T) == 4;
 }
 
-template <typename T, int NPRE>
-__global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
+template <typename T, int NPRE, int NTB>
+__global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     const double *guard = A.scal + zslot() * SS;
     {
         const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1, op = zslot() * A.lps;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
     struct BI {
         T b, id;
     };
-    const auto lb = batch_load<TL::NB, PS_NT>(tid, [&](int k) {
+    const auto lb = batch_load<TL::NB, NTB>(tid, [&](int k) {
         const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
         const size_t P = idx(clampi(gi, 1, n0 - 2), clampi(gj, 1, n1 - 2), n0);
         return BI{A.bsrc[P], NPRE == 2 ? A.invd[P] : T(0)};
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
     };
     constexpr int NXR = NPRE >= 1 ? TL::SX * TL::SY : 1;
     auto xs_load = [&]() {
-        return batch_load<NXR, PS_NT>(tid, [&](int k) {
+        return batch_load<NXR, NTB>(tid, [&](int k) {
             const int gi = clampi(fi0 - 2 + k % TL::SX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / TL::SX, 1, n1 - 2);
             return XR{row_faces<T>(A.a, gi, gj), A.invd[idx(gi, gj, n0)]};
         });
@@ -445,14 +447,14 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
     double mean = 0.0;
     if (A.bpart) {
         double tot[1];
-        sum_partials<1>(A.bpart, 0, A.nbp, tot);
+        sum_partials<1, NTB>(A.bpart, 0, A.nbp, tot);
         __shared__ double s_mean;
         if (tid == 0) s_mean = tot[0] / ((double)(n0 - 2) * (double)(n1 - 2));
         __syncthreads();
         mean = s_mean;
     }
     // mean-free right-hand side; first sweep from zero (no neighbours) in place
-    batch_run<TL::NB, PS_NT>(tid, lb, [&](int k, const BI &r) {
+    batch_run<TL::NB, NTB>(tid, lb, [&](int k, const BI &r) {
         const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
         const bool in = inner_node(gi, gj, n0, n1);
         const T bv = in ? (A.bpart ? (T)((double)r.b - mean) : r.b) : T(0);
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
     __syncthreads();
     if (NPRE >= 1) {
         constexpr int HB = TL::HB, BWX = TL::BWX;
-        batch_run<NXR, PS_NT>(tid, lx, [&](int k, const XR &r) {
+        batch_run<NXR, NTB>(tid, lx, [&](int k, const XR &r) {
             const int li = k % TL::SX, lj = k / TL::SX;
             const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
             const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         });
         __syncthreads();
         // residual (halo 1) on the same elements, the ring of the region left out
-        batch_run<NXR, PS_NT>(tid, lx, [&](int k, const XR &r) {
+        batch_run<NXR, NTB>(tid, lx, [&](int k, const XR &r) {
             const int li = k % TL::SX, lj = k / TL::SX;
             if (li < 1 || li > TL::SX - 2 || lj < 1 || lj > TL::SY - 2) return;
             const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         });
     }
     // owned nodes: smoothed x and the mean-free right-hand side
-    for (int k = tid; k < MG_FX * MG_FY; k += PS_NT) {
+    for (int k = tid; k < MG_FX * MG_FY; k += NTB) {
         const int li = k % MG_FX, lj = k / MG_FX;
         const int gi = fi0 + li, gj = fj0 + lj;
         if (gi >= n0 || gj >= n1) continue;
@@ -507,9 +509,10 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
         if (A.bout) A.bout[P] = s_b[(lj + TL::HB) * TL::BWX + (li + TL::HB)];
     }
     __syncthreads();
-    // full weighting onto the coarse tile, one coarse node per thread
+    // full weighting onto the coarse tile, one coarse node per thread (threads
+    // beyond the tile's MG_CX * MG_CY coarse nodes have none)
     double cs[1] = {0.0};
-    {
+    if (NTB <= MG_CX * MG_CY || tid < MG_CX * MG_CY) {
         const int ci = tid % MG_CX, cj = tid / MG_CX;
         const int I = blockIdx.x * MG_CX + ci, J = blockIdx.y * MG_CY + cj;
         if (I < A.m0 && J < A.m1) {
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_down(DownArgs<T> A) {
             A.craw[idx(I, J, A.m0)] = v;
         }
     }
-    block_partials<1>(cs, A.cpart, 0);
+    block_partials<1, NTB>(cs, A.cpart, 0);
 }
 
 template <typename T>
@@ -545,8 +548,8 @@ struct UpArgs {
     int gs;              // guard slot (-1: none)
 };
 
-template <typename T, int NPOST>
-__global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
+template <typename T, int NPOST, int NTB>
+__global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
     const double *guard = A.scal + zslot() * SS;
     {
         const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1;
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         T x, c00, c10, c01, c11;
     };
 #define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
-    const auto ly = batch_load<YX * YY, PS_NT>(tid, [&](int k) {
+    const auto ly = batch_load<YX * YY, NTB>(tid, [&](int k) {
         const int gi = clampi(fi0 - NPOST + k % YX, 1, n0 - 2), gj = clampi(fj0 - NPOST + k / YX, 1, n1 - 2);
         const int I = gi >> 1, J = gj >> 1;
         return YR{A.x[idx(gi, gj, n0)], XC(I, J), XC(I + 1, J), XC(I, J + 1), XC(I + 1, J + 1)};
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         T b, id;
     };
     auto q_load = [&]() {
-        return batch_load<QX * QY, PS_NT>(tid, [&](int k) {
+        return batch_load<QX * QY, NTB>(tid, [&](int k) {
             const int gi = clampi(fi0 - (NPOST - 1) + k % QX, 1, n0 - 2);
             const int gj = clampi(fj0 - (NPOST - 1) + k / QX, 1, n1 - 2);
             const size_t P = idx(gi, gj, n0);
@@ -594,7 +597,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
     decltype(q_load()) lq;
     if (NPOST >= 1 && mg_early<T>()) lq = q_load();
     if (slot_skip(guard, A.gs)) return;
-    batch_run<YX * YY, PS_NT>(tid, ly, [&](int k, const YR &r) {
+    batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
         const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
         T v = T(0);
         if (inner_node(gi, gj, n0, n1)) {
@@ -616,7 +619,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
     __syncthreads();
     // first post sweep on the Q region (NPOST == 2), output on its inner part
     if (NPOST == 2) {
-        batch_run<QX * QY, PS_NT>(tid, lq, [&](int k, const PR &r) {
+        batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
             const int li = k % QX, lj = k / QX;
             const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
             T v = T(0);
@@ -630,7 +633,7 @@ __global__ __launch_bounds__(PS_NT) void k_mg_up(UpArgs<T> A) {
         });
         __syncthreads();
     }
-    batch_run<QX * QY, PS_NT>(tid, lq, [&](int k, const PR &r) {
+    batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
         constexpr int H = NPOST - 1;  // ring of the Q region outside the tile
         const int li = k % QX - H, lj = k / QX - H;
         if (li < 0 || li >= MG_FX || lj < 0 || lj >= MG_FY) return;
@@ -1365,6 +1368,11 @@ struct PSolver {
         return CFD_OK;
     }
 
+    // A level with fewer tiles than CUs runs them with 1024 threads each: with one
+    // wavefront per SIMD a tile's phases are bound by that wavefront's serial
+    // instruction latency, so four times the threads cut it about fourfold.
+    static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
+
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {
         return dim3((lev[l + 1].n0 + MG_CX - 1) / MG_CX, (lev[l + 1].n1 + MG_CY - 1) / MG_CY);
@@ -1382,10 +1390,18 @@ struct PSolver {
             dim3 gr = down_grid(l);
             gr.z = B;
             if (l == 0) cx.prof_b(cx.owner, PK_MG_DOWN);
-            switch (pre) {
-            case 0: k_mg_down<T, 0><<<gr, PS_NT, 0, cx.s>>>(a); break;
-            case 1: k_mg_down<T, 1><<<gr, PS_NT, 0, cx.s>>>(a); break;
-            default: k_mg_down<T, 2><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            if (wide_blocks(gr)) {
+                switch (pre) {
+                case 0: k_mg_down<T, 0, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                case 1: k_mg_down<T, 1, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                default: k_mg_down<T, 2, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                }
+            } else {
+                switch (pre) {
+                case 0: k_mg_down<T, 0, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                case 1: k_mg_down<T, 1, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                default: k_mg_down<T, 2, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                }
             }
             if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
@@ -1401,10 +1417,18 @@ struct PSolver {
                         l == 0 ? z : L.t, w, cx.scal, gs};
             const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
             if (l == 0) cx.prof_b(cx.owner, PK_MG_UP);
-            switch (post) {
-            case 0: k_mg_up<T, 0><<<gr, PS_NT, 0, cx.s>>>(a); break;
-            case 1: k_mg_up<T, 1><<<gr, PS_NT, 0, cx.s>>>(a); break;
-            default: k_mg_up<T, 2><<<gr, PS_NT, 0, cx.s>>>(a); break;
+            if (wide_blocks(gr)) {
+                switch (post) {
+                case 0: k_mg_up<T, 0, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                case 1: k_mg_up<T, 1, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                default: k_mg_up<T, 2, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
+                }
+            } else {
+                switch (post) {
+                case 0: k_mg_up<T, 0, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                case 1: k_mg_up<T, 1, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                default: k_mg_up<T, 2, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
+                }
             }
             if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
