@@ -302,7 +302,7 @@ struct JArgs {
     TR *nu, *nv, *nT;
 };
 
-constexpr int JT_X = 64, JT_Y = 16;  // output tile of k_jacobi2
+constexpr int JT_X = 64, JT_Y = 8;   // output tile of k_jacobi2
 
 // the node's coefficient inputs and right-hand sides
 template <typename TR>

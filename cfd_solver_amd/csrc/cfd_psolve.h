@@ -352,7 +352,7 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
 // Jacobi sweep: x <- x + w (b - L x) / diag; the first sweep from zero is
 // x = w b / diag (no neighbours).
 
-constexpr int MG_FX = 64, MG_FY = 16;  // fine tile
+constexpr int MG_FX = 64, MG_FY = 8;   // fine tile
 constexpr int MG_NT_WIDE = 1024;       // threads per tile on levels with few tiles
 constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
