@@ -12,11 +12,11 @@
 //   k_residual      reference RHS (hybrid convection, grad p, buoyancy) minus the
 //                   reference implicit operator applied to the current fields
 //                   -> R_u, R_v, R_T, the d factors, U, V      (solver.cpp:1062-1236, 751-848)
-//   k_jacobi x m    (A_lo/alpha) delta = R, Jacobi sweeps for u, v, T
-//   k_star          u* = u + du, T += dT, u-hat = u* + d grad p
-//   k_face_div      q = J-weighted divergence of the face velocities (solver.cpp:1389-1448)
+//   k_jacobi2 x m/2 (A_lo/alpha) delta = R, Jacobi sweeps for u, v, T (two per launch)
+//   k_face_div      u* = u + du, u-hat = u* + d grad p, and q = J-weighted divergence
+//                   of the face velocities (solver.cpp:1389-1448)
 //   p' solve        BiCGStab, right-preconditioned by a multigrid V-cycle
-//   k_correct       u = u* - d grad p', p += alpha_p p'         (solver.cpp:1504-1541)
+//   k_correct       u = u* - d grad p', p += alpha_p p', T += dT (solver.cpp:1504-1541)
 //   BC kernels      adiabatic sides, wall pressure from normal momentum (solver.cpp:1667-1728)
 //   reductions      sum|q|, sum|R_T|, dmax (solver.cpp:1806-1852), wall Nusselt (1951-1982)
 #include <cmath>
@@ -71,7 +71,7 @@ struct Case {
     // state and FP64 work
     double *u, *v, *T, *p;
     double *d;
-    double *us, *vs, *uh, *vh, *q, *pc, *scratch;
+    double *uh, *vh, *q, *pc, *scratch;
     // predictor arrays in the predictor precision (double, or float in the mixed path)
     void *Ru, *Rv, *RT, *U, *V;
     void *jb[6];                           // Jacobi ping-pong: du dv dT | tu tv tT
@@ -409,82 +409,69 @@ __global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
     });
 }
 
-// u* = u + du, T += dT (interior), u-hat = u* + d grad p (interior); borders keep u.
-// Also completes d on the border (written there by no other kernel of the iteration).
+// Face velocities and their divergence in one pass.  On a 64 x 4 tile plus a
+// one-node halo, û = u* + d grad p with u* = u + du (the pressure-free velocity
+// the face interpolation starts from) is evaluated into LDS from the tile's
+// inputs (p with a two-node halo); then q, the J-weighted flux divergence of the
+// face velocities, with block partials of Σ|q| (mass residual, rmass) and of Σq,
+// Σq² (the FP64 norms the mixed p' solve starts from, rq).  All loads of the tile
+// are issued together.  Halo entries beyond the grid hold clamped copies, and û
+// on a wall node is evaluated like an interior one: both only enter faces that
+// touch a wall, which are replaced by zero.
 template <typename TR>
-__global__ __launch_bounds__(NT) void k_star(
-    CGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
-    const TR *__restrict__ dv, const TR *__restrict__ dT, double *__restrict__ T,
-    const double *__restrict__ p, double *__restrict__ d, const double *__restrict__ xix,
-    const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
-    double *__restrict__ us, double *__restrict__ vs, double *__restrict__ uh, double *__restrict__ vh) {
-    if (g.sp[blockIdx.z].active == 0.0) return;
-    {
-        const size_t o = zslot() * g.N;
-        u += o; v += o; du += o; dv += o; dT += o; T += o; p += o; d += o; xix += o; xiy += o; etx += o;
-        ety += o; us += o; vs += o; uh += o; vh += o;
-    }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const int n0 = g.n0, n1 = g.n1;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = idx(i, j, n0);
-    if (i == 0 || j == 0 || i == n0 - 1 || j == n1 - 1) {
-        us[P] = u[P];
-        vs[P] = v[P];
-        // d on the border = nearest interior value (corners from the diagonal)
-        d[P] = d[idx(clampi(i, 1, n0 - 2), clampi(j, 1, n1 - 2), n0)];
-        return;
-    }
-    const double a = u[P] + (double)du[P], b = v[P] + (double)dv[P];
-    us[P] = a;
-    vs[P] = b;
-    T[P] = T[P] + (double)dT[P];
-    const double dp_dxi = (p[P + 1] - p[P - 1]) * g.r.i2h1;
-    const double dp_de = (p[idx(i, j + 1, n0)] - p[idx(i, j - 1, n0)]) * g.r.i2h2;
-    uh[P] = a + d[P] * (xix[P] * dp_dxi + etx[P] * dp_de);
-    vh[P] = b + d[P] * (xiy[P] * dp_dxi + ety[P] * dp_de);
-}
-
-// q = J-weighted divergence of the face velocities (oracle: face_div)
-// Tiled: the nine input arrays of a 64 x 4 tile and its one-node halo are staged
-// in LDS with coalesced loads issued together (halo entries beyond the grid hold
-// clamped copies that no interior node reads); each face is then evaluated from
-// LDS with the same arithmetic as before, a wall face replaced by zero.
 __global__ __launch_bounds__(NT) void k_face_div(
-    CGrid g, const double *__restrict__ uh, const double *__restrict__ vh, const double *__restrict__ p,
-    const double *__restrict__ d, const double *__restrict__ xix, const double *__restrict__ xiy,
-    const double *__restrict__ etx, const double *__restrict__ ety, const double *__restrict__ jac,
-    double *__restrict__ q, RedArgs ra) {
+    CGrid g, const double *__restrict__ u, const double *__restrict__ v, const TR *__restrict__ du,
+    const TR *__restrict__ dv, const double *__restrict__ p, const double *__restrict__ d,
+    const double *__restrict__ xix, const double *__restrict__ xiy, const double *__restrict__ etx,
+    const double *__restrict__ ety, const double *__restrict__ jac, double *__restrict__ q, RedArgs rmass,
+    RedArgs rq) {
     const bool active = g.sp[blockIdx.z].active != 0.0;
-    ra = rz(ra);
+    rmass = rz(rmass);
+    rq = rz(rq);
     const size_t o = zslot() * g.N;
     q += o;
-    constexpr int HX = BX + 2, HY = BY + 2, NA = 9;
-    enum { UH, VH, PP, DD, XX, XY, EX, EY, JJ };
+    constexpr int HX = BX + 2, HY = BY + 2, PX = BX + 4, PY = BY + 4;
+    enum { UH, VH, DD, XX, XY, EX, EY, JJ, NA };
     __shared__ double s[NA][HX * HY];
-    const double *src[NA] = {uh + o, vh + o, p + o, d + o, xix + o, xiy + o, etx + o, ety + o, jac + o};
+    __shared__ double s_p[PX * PY];
     const int n0 = g.n0, n1 = g.n1;
     const int tid = tlin();
     const int i0 = blockIdx.x * BX, j0 = blockIdx.y * BY;
-    struct V9 {
-        double a[NA];
+    struct H {
+        double u, v, du, dv, d, x, y, e, f, j;
     };
-    const auto lv = batch_load<HX * HY, NT>(tid, [&](int k) {
-        const size_t Q = idx(clampi(i0 - 1 + k % HX, 0, n0 - 1), clampi(j0 - 1 + k / HX, 0, n1 - 1), n0);
-        V9 r;
-#pragma unroll
-        for (int a = 0; a < NA; a++) r.a[a] = src[a][Q];
-        return r;
+    const auto lh = batch_load<HX * HY, NT>(tid, [&](int k) {
+        const size_t Q = o + idx(clampi(i0 - 1 + k % HX, 0, n0 - 1), clampi(j0 - 1 + k / HX, 0, n1 - 1), n0);
+        return H{u[Q], v[Q], (double)du[Q], (double)dv[Q], d[Q], xix[Q], xiy[Q], etx[Q], ety[Q], jac[Q]};
+    });
+    const auto lp = batch_load<PX * PY, NT>(tid, [&](int k) {
+        return p[o + idx(clampi(i0 - 2 + k % PX, 0, n0 - 1), clampi(j0 - 2 + k / PX, 0, n1 - 1), n0)];
     });
     if (!active) return;
-    batch_run<HX * HY, NT>(tid, lv, [&](int k, const V9 &r) {
-#pragma unroll
-        for (int a = 0; a < NA; a++) s[a][k] = r.a[a];
+    batch_run<PX * PY, NT>(tid, lp, [&](int k, double pv) { s_p[k] = pv; });
+    batch_run<HX * HY, NT>(tid, lh, [&](int k, const H &h) {
+        s[DD][k] = h.d;
+        s[XX][k] = h.x;
+        s[XY][k] = h.y;
+        s[EX][k] = h.e;
+        s[EY][k] = h.f;
+        s[JJ][k] = h.j;
+    });
+    __syncthreads();
+    // û, v̂ at every node of the halo-1 region (the arithmetic of u* and û)
+    batch_run<HX * HY, NT>(tid, lh, [&](int k, const H &h) {
+        const int ki = k % HX, kj = k / HX;
+        auto P_ = [&](int di, int dj) { return s_p[(kj + 1 + dj) * PX + (ki + 1 + di)]; };
+        const double a = h.u + h.du, b = h.v + h.dv;
+        const double dp_dxi = (P_(1, 0) - P_(-1, 0)) * g.r.i2h1;
+        const double dp_de = (P_(0, 1) - P_(0, -1)) * g.r.i2h2;
+        s[UH][k] = a + h.d * (h.x * dp_dxi + h.e * dp_de);
+        s[VH][k] = b + h.d * (h.y * dp_dxi + h.f * dp_de);
     });
     __syncthreads();
     const int li = threadIdx.x, lj = threadIdx.y;
     const int i = i0 + li, j = j0 + lj;
-    double aq = 0.0;
+    double aq[1] = {0.0}, nq[2] = {0.0, 0.0};
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
         // array a at the node offset (di, dj) from (i, j)
         auto L = [&](int a, int di, int dj) { return s[a][(lj + 1 + dj) * HX + (li + 1 + di)]; };
@@ -501,7 +488,7 @@ __global__ __launch_bounds__(NT) void k_face_div(
             return 0.5 * (L(JJ, 0, 0) * L(mx, 0, 0) + L(JJ, di, dj) * L(mx, di, dj)) * uf +
                    0.5 * (L(JJ, 0, 0) * L(my, 0, 0) + L(JJ, di, dj) * L(my, di, dj)) * vf;
         };
-        auto P_ = [&](int di, int dj) { return L(PP, di, dj); };
+        auto P_ = [&](int di, int dj) { return s_p[(lj + 2 + dj) * PX + (li + 2 + di)]; };
         double fe = face(1, 0, (P_(1, 0) - P_(0, 0)) * g.r.ih1,
                          (P_(1, 1) + P_(0, 1) - P_(0, -1) - P_(1, -1)) * (0.25 * g.r.ih2));
         double fw = face(-1, 0, (P_(0, 0) - P_(-1, 0)) * g.r.ih1,
@@ -516,26 +503,32 @@ __global__ __launch_bounds__(NT) void k_face_div(
         gs = j - 1 != 0 ? gs : 0.0;
         const double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
         q[idx(i, j, n0)] = qq;
-        aq = fabs(qq);
+        aq[0] = fabs(qq);
+        nq[0] = qq;
+        nq[1] = qq * qq;
     }
-    double v[1] = {aq};
-    block_partials<1>(v, ra.part, ra.pstride);
+    block_partials<1>(aq, rmass.part, rmass.pstride);
+    block_partials<2>(nq, rq.part, rq.pstride);
 }
 
-// u = u* - d grad p' (p' read clamped = wall copy), p += alpha_p p', with p' taken
-// with its interior mean (scal[S_PSUM] / interior count) removed
+// u = u* - d grad p' with u* = u + du (p' read clamped = wall copy), p += alpha_p p'
+// with p' taken with its interior mean (scal[S_PSUM] / interior count) removed,
+// and T += dT (interior)
+template <typename TR>
 __global__ __launch_bounds__(NT) void k_correct(
-    CGrid g, const double *__restrict__ us, const double *__restrict__ vs,
+    CGrid g, const TR *__restrict__ du, const TR *__restrict__ dv, const TR *__restrict__ dT,
     const double *__restrict__ pc, const double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
-    const double *__restrict__ scal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ p) {
+    const double *__restrict__ scal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ p,
+    double *__restrict__ T) {
     const SlotP &sp = g.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     const double alpha_p = sp.alpha_p;
     scal += zslot() * SS;
     {
         const size_t o = zslot() * g.N;
-        us += o; vs += o; pc += o; d += o; xix += o; xiy += o; etx += o; ety += o; u += o; v += o; p += o;
+        du += o; dv += o; dT += o; pc += o; d += o; xix += o; xiy += o; etx += o; ety += o; u += o; v += o;
+        p += o; T += o;
     }
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1;
@@ -546,13 +539,13 @@ __global__ __launch_bounds__(NT) void k_correct(
     const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)] - m, ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)] - m;
     const double dpx = 0.5 * (pe - pw) * g.r.ih1;
     const double dpe = 0.5 * (pn - ps) * g.r.ih2;
-    u[P] = us[P] - d[P] * (xix[P] * dpx + etx[P] * dpe);
-    v[P] = vs[P] - d[P] * (xiy[P] * dpx + ety[P] * dpe);
+    const double us = u[P] + (double)du[P], vs = v[P] + (double)dv[P];
+    u[P] = us - d[P] * (xix[P] * dpx + etx[P] * dpe);
+    v[P] = vs - d[P] * (xiy[P] * dpx + ety[P] * dpe);
     p[P] = p[P] + alpha_p * (pc[P] - m);
+    T[P] = T[P] + (double)dT[P];
 }
 
-// adiabatic side walls (oracle: temperature_sides), one workgroup: every new value
-// is computed from the old wall column (into scratch) before any is written
 // Adiabatic side walls in two launches spread over the chip (the column-strided
 // wall accesses of one block would be bound by that CU's outstanding misses): the
 // new wall values, computed from the old ones, go to scratch, then into T.
@@ -1158,7 +1151,7 @@ int mg_setup(Case *c) {
 // aiming at half the remaining tolerance).  graph: capturable (no host
 // synchronisation).
 // Iterations and relative residual: S_IT / S_RELRES (FP64), S_ITTOT / S_RELRES64.
-int psolve(Case *c, double rtol, int maxit, bool graph) {
+int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = false) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid_b(c), bl(BX, BY);
@@ -1170,8 +1163,13 @@ int psolve(Case *c, double rtol, int maxit, bool graph) {
     }
     const int R = rtol >= 1e-4 ? 1 : 6;
     const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
-    k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
-    CFD_LAUNCH_CHECK();
+    if (!q_norms_ready) {
+        k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
+        CFD_LAUNCH_CHECK();
+    } else if (R > 1) {  // the refinement accumulates into p' from zero
+        k_fill<double><<<(unsigned)((c->B * c->N + 255) / 256), 256, 0, c->s>>>(c->B * c->N, c->pc, 0.0);
+        CFD_LAUNCH_CHECK();
+    }
     RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5));
     for (int k = 0; k < R; k++) {
         if (!graph && k > 0) {
@@ -1297,11 +1295,22 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     return CFD_OK;
 }
 
-template <typename TR>
-int star_t(Case *c, void *du, void *dv, void *dT) {
-    k_star<TR><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (TR *)du, (TR *)dv, (TR *)dT, c->T, c->p,
-                                                      c->d, c->xix, c->xiy, c->etx, c->ety, c->us, c->vs, c->uh,
-                                                      c->vh);
+// û and the face divergence q of one outer iteration with its norms (mass
+// partials in PS_MASS, Σq and Σq² for the p' solve in slots 0-1); du, dv: the
+// predictor's final corrections
+int launch_face_div(Case *c, const void *du, const void *dv) {
+    prof_begin(c, 3);
+    if (c->mixed)
+        k_face_div<float><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const float *)du,
+                                                                 (const float *)dv, c->p, c->d, c->xix, c->xiy,
+                                                                 c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS),
+                                                                 red(c));
+    else
+        k_face_div<double><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const double *)du,
+                                                                  (const double *)dv, c->p, c->d, c->xix, c->xiy,
+                                                                  c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS),
+                                                                  red(c));
+    prof_end(c);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1317,20 +1326,20 @@ int outer_body(Case *c, bool graph) {
     void *du, *dv, *dT;
     if (c->mixed) RET(predictor_t<float>(c, &du, &dv, &dT));
     else RET(predictor_t<double>(c, &du, &dv, &dT));
-    // 4 u*, T, u-hat, face divergence (FP64)
-    if (c->mixed) RET(star_t<float>(c, du, dv, dT));
-    else RET(star_t<double>(c, du, dv, dT));
-    prof_begin(c, 3);
-    k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
-                                    c->q, red(c, PS_MASS));
-    prof_end(c);
-    CFD_LAUNCH_CHECK();
-    // 5 p' solve
+    // 4 u* = u + du, u-hat, face divergence and its norms (FP64)
+    RET(launch_face_div(c, du, dv));
+    // 5 p' solve (its right-hand side's norms come from step 4)
     RET(mg_setup(c));
-    RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph));
-    // 6-7 correction (p' mean removed) and BCs (FP64 accumulation)
-    k_correct<<<gr, bl, 0, c->s>>>(cg(c), c->us, c->vs, c->pc, c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u,
-                                   c->v, c->p);
+    RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph, true));
+    // 6-7 correction (u*, T += dT, p' mean removed) and BCs (FP64 accumulation)
+    if (c->mixed)
+        k_correct<float><<<gr, bl, 0, c->s>>>(cg(c), (const float *)du, (const float *)dv, (const float *)dT, c->pc,
+                                              c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v, c->p,
+                                              c->T);
+    else
+        k_correct<double><<<gr, bl, 0, c->s>>>(cg(c), (const double *)du, (const double *)dv, (const double *)dT,
+                                               c->pc, c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v,
+                                               c->p, c->T);
     CFD_LAUNCH_CHECK();
     RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax and the wall Nusselt numbers (FP64)
@@ -1499,7 +1508,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     const size_t BN = (size_t)slots * c->N;
     int rc = CFD_OK;
     double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
-                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->d, &c->us, &c->vs,
+                         &c->ajac, &c->p1, &c->q1, &c->u, &c->v, &c->T, &c->p, &c->d,
                          &c->uh, &c->vh, &c->q, &c->pc};
     for (double **f : fields)
         if (rc == CFD_OK) rc = dalloc(c, f, BN);
@@ -1907,8 +1916,7 @@ int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch
             (void)launch_residual(c);
             break;
         case 3:
-            k_face_div<<<gr, bl, 0, c->s>>>(cg(c), c->uh, c->vh, c->p, c->d, c->xix, c->xiy, c->etx, c->ety,
-                                            c->ajac, c->q, red(c, PS_MASS));
+            (void)launch_face_div(c, c->jb[0], c->jb[1]);
             break;
         }
     }

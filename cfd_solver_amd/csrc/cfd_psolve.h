@@ -262,51 +262,51 @@ __device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i,
     return i < A.n0[l] && j < A.n1[l];
 }
 
-// Face (normal kn, tangential kt) between node (i, j) of level `lv` and its east
-// (dir 0) or north (dir 1) neighbour, the arithmetic of p9_face with the grid
-// divisions taken as host-computed reciprocals (ih1sq = 1/h1^2, ih2sq = 1/h2^2,
-// i4 = 1/(4 h1 h2)); d is level 0's d read at (i << lv, j << lv) (injection).
-// Zero on faces touching a wall node.  No branch precedes the loads: the indices
-// are clamped into the grid, every load is issued, and a wall face is zeroed
-// after the arithmetic.  Offsets are 32-bit within the slot's arrays.
-__device__ __forceinline__ void p9_face_lv(int n0, int n1, const double *__restrict__ xix,
-                                           const double *__restrict__ xiy, const double *__restrict__ etx,
-                                           const double *__restrict__ ety, const double *__restrict__ jac,
-                                           const double *__restrict__ d0, int dn0, int lv, double ih1sq,
-                                           double ih2sq, double i4, int i, int j, int dir, double &kn, double &kt) {
-    const bool valid = dir == 0 ? (i >= 1 && i <= n0 - 3 && j >= 1 && j <= n1 - 2)
-                                : (j >= 1 && j <= n1 - 3 && i >= 1 && i <= n0 - 2);
-    const int ic = clampi(i, 0, dir == 0 ? n0 - 2 : n0 - 1), jc = clampi(j, 0, dir == 0 ? n1 - 1 : n1 - 2);
-    const int ie = dir == 0 ? ic + 1 : ic, je = dir == 0 ? jc : jc + 1;
+// Metric terms and d at one node, loaded once and shared by the faces that touch it.
+struct FNode {
+    double x, y, e, f, j, d;  // xi_x, xi_y, eta_x, eta_y, J, d
+};
+__device__ __forceinline__ FNode fnode_load(const double *__restrict__ xix, const double *__restrict__ xiy,
+                                            const double *__restrict__ etx, const double *__restrict__ ety,
+                                            const double *__restrict__ jac, const double *__restrict__ d0,
+                                            int n0, int n1, int dn0, int lv, int i, int j) {
+    const int ic = clampi(i, 0, n0 - 1), jc = clampi(j, 0, n1 - 1);
+    const unsigned P = (unsigned)jc * (unsigned)n0 + (unsigned)ic;
+    return FNode{xix[P], xiy[P], etx[P], ety[P], jac[P],
+                 d0[((unsigned)jc << lv) * (unsigned)dn0 + ((unsigned)ic << lv)]};
+}
+
+// Face (normal kn, tangential kt) between node P and its east (dir 0) or north
+// (dir 1) neighbour Q: the arithmetic of p9_face with the grid divisions taken as
+// host-computed reciprocals (ih1sq = 1/h1^2, ih2sq = 1/h2^2, i4 = 1/(4 h1 h2)); d
+// is level 0's d injected at (i << lv, j << lv).  The caller zeroes faces that
+// touch a wall node.
+__device__ __forceinline__ void p9_face_of(const FNode &P, const FNode &Q, int dir, double ih1sq, double ih2sq,
+                                           double i4, double &kn, double &kt) {
     const double half = 0.5;
-    const unsigned P = (unsigned)jc * (unsigned)n0 + (unsigned)ic, Q = (unsigned)je * (unsigned)n0 + (unsigned)ie;
-    const double xP = xix[P], yP = xiy[P], eP = etx[P], fP = ety[P], jP = jac[P];
-    const double xQ = xix[Q], yQ = xiy[Q], eQ = etx[Q], fQ = ety[Q], jQ = jac[Q];
-    const double dP = d0[((unsigned)jc << lv) * (unsigned)dn0 + ((unsigned)ic << lv)];
-    const double dQ = d0[((unsigned)je << lv) * (unsigned)dn0 + ((unsigned)ie << lv)];
     if (dir == 0) {
-        const double mx = half * (jP * xP + jQ * xQ), my = half * (jP * yP + jQ * yQ);
-        const double a = mx * (xP + xQ) + my * (yP + yQ);
-        const double b = mx * (eP + eQ) + my * (fP + fQ);
-        const double de = half * (dP + dQ);
+        const double mx = half * (P.j * P.x + Q.j * Q.x), my = half * (P.j * P.y + Q.j * Q.y);
+        const double a = mx * (P.x + Q.x) + my * (P.y + Q.y);
+        const double b = mx * (P.e + Q.e) + my * (P.f + Q.f);
+        const double de = half * (P.d + Q.d);
         kn = half * de * a * ih1sq;
         kt = half * de * b * i4;
     } else {
-        const double mx = half * (jP * eP + jQ * eQ), my = half * (jP * fP + jQ * fQ);
-        const double a = mx * (eP + eQ) + my * (fP + fQ);
-        const double b = mx * (xP + xQ) + my * (yP + yQ);
-        const double dn = half * (dP + dQ);
+        const double mx = half * (P.j * P.e + Q.j * Q.e), my = half * (P.j * P.f + Q.j * Q.f);
+        const double a = mx * (P.e + Q.e) + my * (P.f + Q.f);
+        const double b = mx * (P.x + Q.x) + my * (P.y + Q.y);
+        const double dn = half * (P.d + Q.d);
         kn = half * dn * a * ih2sq;
         kt = half * dn * b * i4;
     }
-    kn = valid ? kn : 0.0;
-    kt = valid ? kt : 0.0;
 }
 
 // Face coefficients of every node of every level, and the inverse diagonal of
 // the clamped operator (every coefficient whose neighbour clamps onto the node)
 // at interior nodes: the thread also evaluates its west neighbour's east face and
-// its south neighbour's north face, so the row needs no second launch.
+// its south neighbour's north face, so the row needs no second launch.  The five
+// nodes' metric terms and d are loaded once (clamped into the grid, no branch
+// before the loads); a face touching a wall node is zeroed after its arithmetic.
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<T> A) {
     if (slot_skip(scal + zslot() * SS, -1)) return;
@@ -319,11 +319,25 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
     const double *xix = A.xix[l] + o, *xiy = A.xiy[l] + o, *etx = A.etx[l] + o, *ety = A.ety[l] + o,
                  *jac = A.jac[l] + o;
     const double r1 = A.ih1sq[l], r2 = A.ih2sq[l], r12 = A.i4h1h2[l];
+    const int dn0 = A.dn0;
+    const FNode nP = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j);
+    const FNode nE = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i + 1, j);
+    const FNode nW = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i - 1, j);
+    const FNode nN = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j + 1);
+    const FNode nS = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j - 1);
     double en, et, nn, nt, wn, wt, sn, st;
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j, 0, en, et);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j, 1, nn, nt);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i - 1, j, 0, wn, wt);
-    p9_face_lv(n0, n1, xix, xiy, etx, ety, jac, d0, A.dn0, l, r1, r2, r12, i, j - 1, 1, sn, st);
+    p9_face_of(nP, nE, 0, r1, r2, r12, en, et);
+    p9_face_of(nP, nN, 1, r1, r2, r12, nn, nt);
+    p9_face_of(nW, nP, 0, r1, r2, r12, wn, wt);
+    p9_face_of(nS, nP, 1, r1, r2, r12, sn, st);
+    // east face of (i, j) valid for 1 <= i <= n0 - 3, 1 <= j <= n1 - 2; north face
+    // for 1 <= j <= n1 - 3, 1 <= i <= n0 - 2 (zero on faces touching a wall node)
+    auto ev = [&](int ii, int jj) { return ii >= 1 && ii <= n0 - 3 && jj >= 1 && jj <= n1 - 2; };
+    auto nv = [&](int ii, int jj) { return jj >= 1 && jj <= n1 - 3 && ii >= 1 && ii <= n0 - 2; };
+    if (!ev(i, j)) en = et = 0.0;
+    if (!nv(i, j)) nn = nt = 0.0;
+    if (!ev(i - 1, j)) wn = wt = 0.0;
+    if (!nv(i, j - 1)) sn = st = 0.0;
     A.fen[l][P] = (T)en;
     A.fet[l][P] = (T)et;
     A.fnn[l][P] = (T)nn;
