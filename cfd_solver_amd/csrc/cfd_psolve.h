@@ -34,6 +34,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -721,10 +722,12 @@ __device__ __forceinline__ double cw_sum(double v) {
 template <typename T>
 struct CNode {
     int k;     // node index j * n0 + i; -1: none
-    int i, j;
+    int ij;    // i | j << 16
     T c[9];    // operator row, ghosts folded in
     T id;      // 1 / diag
     T b;       // right-hand side
+    __device__ int i() const { return ij & 0xffff; }
+    __device__ int j() const { return ij >> 16; }
 };
 
 // row (i, j) from its faces with every clamped neighbour's coefficient moved onto
@@ -797,14 +800,13 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
         const int q = min(tid, m0 * m1 - 1);
         const int i = 1 + q % m0, j = 1 + q / m0;
         const size_t o = zslot() * cl.n0 * cl.n1;
-        nd[L].i = i;
-        nd[L].j = j;
+        nd[L].ij = i | j << 16;
         nd[L].k = (l0 + L <= last && tid < m0 * m1) ? j * cl.n0 + i : -1;
         rf[L] = load_row_faces<T>(cl.n0, cl.fen + o, cl.fet + o, cl.fnn + o, cl.fnt + o, i, j);
         nd[L].id = cl.invd[o + (size_t)j * cl.n0 + i];
         nd[L].b = T(0);
     }
-    const T braw = b0[nd[0].j * A.L[l0].n0 + nd[0].i];
+    const T braw = b0[nd[0].j() * A.L[l0].n0 + nd[0].i()];
     double part = 0.0;
     if (b0part)
         for (int t = tid; t < nb0p; t += CW) part += b0part[t];
@@ -816,98 +818,146 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
 #pragma unroll
     for (int L = 0; L < CW_LEVS; L++) {
         const int l = min(l0 + L, last);
-        fold_row<T>(rf[L], A.L[l].n0, A.L[l].n1, nd[L].i, nd[L].j, nd[L].c);
+        fold_row<T>(rf[L], A.L[l].n0, A.L[l].n1, nd[L].i(), nd[L].j(), nd[L].c);
     }
     {
         const double mean = cw_sum(part) / (double)INNER(0);  // also orders the zero fill
         nd[0].b = b0part ? (T)((double)braw - mean) : braw;
     }
 
-    // `count` Jacobi sweeps of level L (from zero when from_zero); the result is
-    // left in the level's iterate
-    auto sweeps = [&](const CNode<T> &c, int L, int count, bool from_zero) __attribute__((always_inline)) {
+    // Levels of at most CW_WAVE interior nodes ("wave levels", from LW on) are owned
+    // by wavefront 0 alone: their down leg, the coarsest solve and their up leg run
+    // in that wavefront, ordered by wavefront barriers (its LDS writes precede its
+    // next reads), while the other wavefronts wait at one block barrier.
+    int LW = nl;
+    for (int L = nl - 1; L >= 0 && INNER(L) <= CW_WAVE; L--) LW = L;
+    auto wsync = [&]() __attribute__((always_inline)) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // `count` Jacobi sweeps of level L (from zero when from_zero), the result left in
+    // the level's iterate; WAVE: run by wavefront 0 with wavefront barriers, else by
+    // the block (ending with a block barrier)
+    auto sweeps = [&](auto wave_tag, const CNode<T> &c, int L, int count, bool from_zero)
+                      __attribute__((always_inline)) {
+        constexpr bool WAVE = decltype(wave_tag)::value;
         if (count <= 0) return;
         const int n0 = A.L[l0 + L].n0;
         T *x = XA(L), *t = TA(L);
         // output of sweep s: the buffers alternate so that the last one is x
         // (from zero); otherwise sweep 0 reads x and writes t
-        auto out_of = [&](int s) __attribute__((always_inline)) { return from_zero ? (((count - 1 - s) & 1) ? t : x) : ((s & 1) ? x : t); };
-        auto one = [&](int s) __attribute__((always_inline)) {
-            if (c.k < 0) return;
-            T *out = out_of(s);
-            if (from_zero && s == 0) {
-                out[c.k] = w * c.b * c.id;
-            } else {
-                const T *in = (out == x) ? t : x;
-                out[c.k] = in[c.k] + w * (c.b - cw_row<T>(c, in, n0)) * c.id;
-            }
+        auto out_of = [&](int s) __attribute__((always_inline)) {
+            return from_zero ? (((count - 1 - s) & 1) ? t : x) : ((s & 1) ? x : t);
         };
-        if (INNER(L) <= CW_WAVE) {
-            if (tid < 64) {
-                for (int s = 0; s < count; s++) {
-                    one(s);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int s = 0; s < count; s++) {
+            if (c.k >= 0) {
+                T *out = out_of(s);
+                if (from_zero && s == 0) {
+                    out[c.k] = w * c.b * c.id;
+                } else {
+                    const T *in = (out == x) ? t : x;
+                    out[c.k] = in[c.k] + w * (c.b - cw_row<T>(c, in, n0)) * c.id;
                 }
             }
-        } else {
-            for (int s = 0; s < count; s++) {
-                one(s);
-                if (s + 1 < count) __syncthreads();
-            }
+            if (WAVE) wsync();
+            else if (s + 1 < count) __syncthreads();
         }
         if (!from_zero && (count & 1) && c.k >= 0) x[c.k] = t[c.k];  // the last sweep wrote t
-        __syncthreads();
+        if (WAVE) wsync();
+        else __syncthreads();
+    };
+    using Block = std::integral_constant<bool, false>;
+    using Wave = std::integral_constant<bool, true>;
+    // full weighting of level L's residual r onto the owned node of level L + 1
+    auto restrict_to = [&](const T *r, int n0, const CNode<T> &cc) __attribute__((always_inline)) {
+        T v = T(0);
+        if (cc.k >= 0) {
+            const int k = 2 * cc.j() * n0 + 2 * cc.i();
+            v = (T(4) * r[k] + T(2) * (r[k - 1] + r[k + 1] + r[k - n0] + r[k + n0]) +
+                 (r[k - n0 - 1] + r[k - n0 + 1] + r[k + n0 - 1] + r[k + n0 + 1])) /
+                T(16);
+        }
+        return v;
+    };
+    // bilinear prolongation of level L + 1's correction added to level L's iterate
+    auto prolong = [&](const CNode<T> &c, int L) __attribute__((always_inline)) {
+        if (c.k < 0) return;
+        const int m0 = A.L[l0 + L + 1].n0, m1 = A.L[l0 + L + 1].n1;
+        const T *xcc = XA(L + 1);
+        T *x = XA(L);
+        const int i = c.i(), j = c.j(), I = i >> 1, J = j >> 1;
+#define XC(a_, b_) xcc[clampi((b_), 1, m1 - 2) * m0 + clampi((a_), 1, m0 - 2)]
+        T e;
+        if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
+        else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
+        else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
+        else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
+#undef XC
+        x[c.k] += e;
     };
 
-    // down: smoothing, residual, full weighting onto the next level (mean removed)
+    // down, block levels: smoothing, residual, full weighting onto the next level
+    // (its interior mean removed)
 #pragma unroll
     for (int L = 0; L < CW_LEVS - 1; L++) {
-        if (L < nl - 1) {
+        if (L < nl - 1 && L < LW) {
             const int n0 = A.L[l0 + L].n0;
             const CNode<T> &c = nd[L];
             T *x = XA(L), *r = TA(L);
-            if (A.pre > 0) sweeps(c, L, A.pre, true);  // with pre == 0 x stays zero
+            if (A.pre > 0) sweeps(Block{}, c, L, A.pre, true);  // with pre == 0 x stays zero
             if (c.k >= 0) r[c.k] = c.b - cw_row<T>(c, x, n0);
             __syncthreads();
             CNode<T> &cc = nd[L + 1];
-            T v = T(0);
-            if (cc.k >= 0) {
-                const int k = 2 * cc.j * n0 + 2 * cc.i;
-                v = (T(4) * r[k] + T(2) * (r[k - 1] + r[k + 1] + r[k - n0] + r[k + n0]) +
-                     (r[k - n0 - 1] + r[k - n0 + 1] + r[k + n0 - 1] + r[k + n0 + 1])) /
-                    T(16);
-            }
+            const T v = restrict_to(r, n0, cc);
             const double mean = cw_sum(cc.k >= 0 ? (double)v : 0.0) / (double)INNER(L + 1);
             cc.b = (T)((double)v - mean);
         }
     }
-    // coarsest level
+    // wave levels, wavefront 0
+    if (LW < nl) {
+        if (tid < 64) {
 #pragma unroll
-    for (int L = 0; L < CW_LEVS; L++)
-        if (L == nl - 1) sweeps(nd[L], L, A.coarse, true);  // with coarse == 0 x stays zero
-    // up: bilinear prolongation of the correction, post-smoothing
+            for (int L = 0; L < CW_LEVS - 1; L++) {
+                if (L >= LW && L < nl - 1) {
+                    const int n0 = A.L[l0 + L].n0;
+                    const CNode<T> &c = nd[L];
+                    T *x = XA(L), *r = TA(L);
+                    if (A.pre > 0) sweeps(Wave{}, c, L, A.pre, true);
+                    if (c.k >= 0) r[c.k] = c.b - cw_row<T>(c, x, n0);
+                    wsync();
+                    CNode<T> &cc = nd[L + 1];
+                    const T v = restrict_to(r, n0, cc);
+                    const double tot = __shfl(wave_sum(cc.k >= 0 ? (double)v : 0.0), 0, 64);
+                    cc.b = (T)((double)v - tot / (double)INNER(L + 1));
+                }
+            }
+#pragma unroll
+            for (int L = 0; L < CW_LEVS; L++)
+                if (L == nl - 1) sweeps(Wave{}, nd[L], L, A.coarse, true);  // with coarse == 0 x stays zero
+#pragma unroll
+            for (int L = CW_LEVS - 2; L >= 0; L--) {
+                if (L >= LW && L < nl - 1) {
+                    prolong(nd[L], L);
+                    wsync();
+                    sweeps(Wave{}, nd[L], L, A.post, false);
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        // the coarsest level is a block level
+#pragma unroll
+        for (int L = 0; L < CW_LEVS; L++)
+            if (L == nl - 1) sweeps(Block{}, nd[L], L, A.coarse, true);
+    }
+    // up, block levels: bilinear prolongation of the correction, post-smoothing
 #pragma unroll
     for (int L = CW_LEVS - 2; L >= 0; L--) {
-        if (L < nl - 1) {
-            const CNode<T> &c = nd[L];
-            const int m0 = A.L[l0 + L + 1].n0, m1 = A.L[l0 + L + 1].n1;
-            const T *xcc = XA(L + 1);
-            T *x = XA(L);
-            if (c.k >= 0) {
-                const int i = c.i, j = c.j, I = i >> 1, J = j >> 1;
-#define XC(a_, b_) xcc[clampi((b_), 1, m1 - 2) * m0 + clampi((a_), 1, m0 - 2)]
-                T e;
-                if ((i & 1) == 0 && (j & 1) == 0) e = XC(I, J);
-                else if ((j & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I + 1, J));
-                else if ((i & 1) == 0) e = T(0.5) * (XC(I, J) + XC(I, J + 1));
-                else e = T(0.25) * (XC(I, J) + XC(I + 1, J) + XC(I, J + 1) + XC(I + 1, J + 1));
-#undef XC
-                x[c.k] += e;
-            }
+        if (L < nl - 1 && L < LW) {
+            prolong(nd[L], L);
             __syncthreads();
-            sweeps(c, L, A.post, false);
+            sweeps(Block{}, nd[L], L, A.post, false);
         }
     }
     {
@@ -1277,7 +1327,7 @@ struct PSolver {
     OpLevel<T> lev[PS_MAXLEV];
     int nlev = 0;
     T *bi[8] = {nullptr};  // r r0 pv vv s t ph sh (level-0 size)
-    int pre = 2, post = 2, coarse = 30;
+    int pre = 2, post = 2, coarse = 10;
     double omega = 0.8;
     int lc = 0;                  // first level handled by the single-workgroup kernel
     CoarseArgs<T> cargs{};
