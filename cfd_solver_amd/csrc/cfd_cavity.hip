@@ -88,6 +88,9 @@ struct Case {
     int pstride = 0;
     double *scal = nullptr;  // device scalars, SS per slot
     double *hscal = nullptr; // pinned host mirror
+    // per-slot end-of-iteration flags of the eager p' solve: pinned, coherent host
+    // memory (host and device views) and their sequence counter
+    unsigned long long *hflag = nullptr, *dflag = nullptr, flag_seq = 0;
     size_t bytes = 0;
     bool dry = false;    // size estimate only: no device work
     std::vector<void *> allocs;
@@ -1041,6 +1044,9 @@ void prof_end(Case *c) {
 }
 void prof_drain(Case *c) {
     if (!c->prof) return;
+    // the caller may have seen the device's flags before the runtime marks the
+    // events complete
+    if (c->ev_used > 0) (void)hipEventSynchronize(c->ev[c->ev_used - 1]);
     for (int k = 0; k + 1 < c->ev_used; k += 2) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
@@ -1059,8 +1065,14 @@ void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 
 size_t psz(const Case *c) { return (size_t)PS_SLOTS * c->pstride; }
 PsCtx ctx(Case *c) {
-    return PsCtx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
-                 &c->allocs, &c->bytes, c->dry};
+    PsCtx cx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
+             &c->allocs, &c->bytes, c->dry};
+    if (c->dflag) {
+        cx.flag_d = c->dflag;
+        cx.flag_h = c->hflag;
+        cx.seq = &c->flag_seq;
+    }
+    return cx;
 }
 RedArgs red(const Case *c, int slot = 0) {
     return RedArgs{c->part + (size_t)slot * c->pstride, c->pstride, psz(c), c->scal};
@@ -1481,6 +1493,7 @@ void batch_destroy(Case *c) {
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hflag) (void)hipHostFree(c->hflag);
     if (c->s) (void)hipStreamDestroy(c->s);
     delete c->ps64;
     delete c->ps32;
@@ -1539,6 +1552,12 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     if (rc == CFD_OK && hipHostMalloc((void **)&c->hscal, (size_t)slots * SS * sizeof(double)) != hipSuccess)
         rc = CFD_E_OOM;
     if (rc == CFD_OK) std::memset(c->hscal, 0, (size_t)slots * SS * sizeof(double));
+    if (rc == CFD_OK &&
+        (hipHostMalloc((void **)&c->hflag, (size_t)slots * sizeof(unsigned long long),
+                       hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+         hipHostGetDevicePointer((void **)&c->dflag, c->hflag, 0) != hipSuccess))
+        rc = CFD_E_OOM;
+    if (rc == CFD_OK) std::memset(c->hflag, 0, (size_t)slots * sizeof(unsigned long long));
     if (rc == CFD_OK) rc = build_levels(c);
     if (rc == CFD_OK && hipStreamSynchronize(c->s) != hipSuccess) rc = CFD_E_HIP;
     if (rc != CFD_OK) {

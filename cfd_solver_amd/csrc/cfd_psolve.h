@@ -1183,17 +1183,10 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
 
 enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
 
-// totals of a phase's partials and what follows from them; skipped while *guard
-// is set (the phase's kernel did not run)
-template <int OP>
-__global__ __launch_bounds__(FIN_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot) {
-    ra = rz(ra);
-    const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
-    constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
-    double tot[NV];
-    sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
-    if (skip || tlin() != 0) return;
-    double *s = ra.scal;
+// the scalar recurrence / convergence test of phase OP from the totals
+template <int OP, int NV>
+__device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV], int it, double rtol,
+                                                    int rtol_slot) {
     if (OP == FIN_START1) {
         s[S_SUM] = tot[0];
         s[S_BB] = tot[NV - 1];
@@ -1246,6 +1239,27 @@ __global__ __launch_bounds__(FIN_NT) void k_bi_fin(int gs, RedArgs ra, int npart
     }
 }
 
+// totals of a phase's partials and what follows from them; skipped while *guard
+// is set (the phase's kernel did not run)
+// FIN_XR with flag != null also writes the slot's end-of-iteration flag into host
+// memory: (seq << 1) | done, done when the slot is skipped or has converged.
+template <int OP>
+__global__ __launch_bounds__(FIN_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot,
+                                                   unsigned long long seq = 0, unsigned long long *flag = nullptr) {
+    ra = rz(ra);
+    const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
+    constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
+    double tot[NV];
+    sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
+    if (tlin() != 0) return;
+    if (!skip) bi_fin_scalars<OP>(ra.scal, tot, it, rtol, rtol_slot);
+    if (OP == FIN_XR && flag) {
+        const unsigned long long done = (skip || ra.scal[S_DONE] != 0.0) ? 1ull : 0ull;
+        __threadfence_system();
+        __hip_atomic_store(flag + zslot(), (seq << 1) | done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 template <typename A, typename B>
 __global__ void k_cvt(size_t n, const A *__restrict__ a, B *__restrict__ b) {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1282,6 +1296,11 @@ struct PsCtx {
     std::vector<void *> *allocs;
     size_t *bytes;
     bool dry;       // count bytes only (no device allocation)
+    // per-slot end-of-iteration flags in pinned, coherent host memory (device and
+    // host views) and the owner's flag sequence counter; null: not used
+    unsigned long long *flag_d = nullptr;
+    unsigned long long *flag_h = nullptr;
+    unsigned long long *seq = nullptr;
     RedArgs ra() const { return RedArgs{part, pstride, psz, scal}; }
     // every active slot has its S_DONE flag set (host mirror)
     bool all_done() const {
@@ -1312,6 +1331,38 @@ inline int ps_alloc(PsCtx &cx, void **p, size_t bytes) {
 inline int nblk(int n0, int n1) {
     const dim3 g = grid2d(n0, n1);
     return (int)(g.x * g.y);
+}
+
+// Wait until every slot's end-of-iteration flag carries sequence number `sq`
+// (written by k_bi_fin<FIN_XR> straight into host memory), then report whether
+// every slot is done.  Cheaper than copying the scalars back and synchronising
+// the stream.  If the stream drains and a flag is still missing the protocol is
+// broken: CFD_E_HIP, never an endless wait.
+inline int ps_wait(PsCtx &cx, unsigned long long sq, bool &all_done) {
+    for (long spin = 0;; spin++) {
+        bool seen = true, done = true;
+        for (int z = 0; z < cx.B && seen; z++) {
+            const unsigned long long v = __atomic_load_n(cx.flag_h + z, __ATOMIC_ACQUIRE);
+            if ((v >> 1) != sq) seen = false;
+            else if ((v & 1) == 0) done = false;
+        }
+        if (seen) {
+            all_done = done;
+            break;
+        }
+        if ((spin & 255) == 255) {
+            const hipError_t e = hipStreamQuery(cx.s);
+            if (e == hipSuccess) {  // drained: the flags are final now
+                for (int z = 0; z < cx.B; z++)
+                    if ((__atomic_load_n(cx.flag_h + z, __ATOMIC_ACQUIRE) >> 1) != sq) return CFD_E_HIP;
+            } else if (e != hipErrorNotReady) {
+                return CFD_E_HIP;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    if (cx.drain) cx.drain(cx.owner);
+    return CFD_OK;
 }
 
 inline int ps_sync(PsCtx &cx) {
@@ -1542,9 +1593,17 @@ struct PSolver {
             cx.prof_e(cx.owner);
             k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
-            k_bi_fin<FIN_XR><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            // eager: the iteration's end flags go straight to host memory and the host
+            // waits for them (no copy, no stream synchronisation)
+            const bool flags = !graph && cx.flag_d && cx.seq;
+            const unsigned long long sq = flags ? ++*cx.seq : 0;
+            k_bi_fin<FIN_XR><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1, sq, flags ? cx.flag_d : nullptr);
             CFD_LAUNCH_CHECK();
-            if (!graph) {
+            if (flags) {
+                bool done = false;
+                CFD_RET(ps_wait(cx, sq, done));
+                if (done) return CFD_OK;
+            } else if (!graph) {
                 CFD_RET(ps_sync(cx));
                 if (cx.all_done()) return CFD_OK;
             }
