@@ -1463,8 +1463,7 @@ int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
                       ps->lps, c->scal, -1};
         dim3 g2 = ps->down_grid(0);
         g2.z = c->B;
-        if (PSolver<T>::wide_blocks(g2)) k_mg_down<T, 2, MG_NT_WIDE><<<g2, MG_NT_WIDE, 0, c->s>>>(a);
-        else k_mg_down<T, 2, PS_NT><<<g2, PS_NT, 0, c->s>>>(a);
+        PSolver<T>::launch_down(g2, c->s, a, 2, PSolver<T>::wide_blocks(g2), false);
     } else {
         OpLevel<T> &C = ps->lev[1];
         UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega, c->scal, -1};

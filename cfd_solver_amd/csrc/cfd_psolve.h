@@ -411,7 +411,9 @@ constexpr bool mg_early() {
     return sizeof(T) == 4;
 }
 
-template <typename T, int NPRE, int NTB>
+// MEAN: the right-hand side's interior mean is removed first (A.bpart holds the
+// partial sums); false on level 0, where it is not, so no registers are held for it
+template <typename T, int NPRE, int NTB, bool MEAN>
 __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     const double *guard = A.scal + zslot() * SS;
     {
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
         if (A.bout) A.bout += of;
         A.x += of;
         A.craw += oc;
-        if (A.bpart) A.bpart += op;
+        if (MEAN) A.bpart += op;
         A.cpart += op;
     }
     using TL = DownTile<NPRE>;
@@ -460,7 +462,7 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     if (NPRE >= 1 && mg_early<T>()) lx = xs_load();
     if (slot_skip(guard, A.gs)) return;
     double mean = 0.0;
-    if (A.bpart) {
+    if (MEAN) {
         double tot[1];
         sum_partials<1, NTB>(A.bpart, 0, A.nbp, tot);
         __shared__ double s_mean;
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     batch_run<TL::NB, NTB>(tid, lb, [&](int k, const BI &r) {
         const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
         const bool in = inner_node(gi, gj, n0, n1);
-        const T bv = in ? (A.bpart ? (T)((double)r.b - mean) : r.b) : T(0);
+        const T bv = in ? (MEAN ? (T)((double)r.b - mean) : r.b) : T(0);
         s_b[k] = bv;
         if (NPRE == 2) s_x1[k] = in ? w * bv * r.id : T(0);
     });
@@ -1483,6 +1485,25 @@ struct PSolver {
         return CFD_OK;
     }
 
+    // k_mg_down<T, pre, threads, mean> on grid gr
+    template <int NPRE, int NTB>
+    static void launch_down_n(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool mean) {
+        if (mean) k_mg_down<T, NPRE, NTB, true><<<gr, NTB, 0, st>>>(a);
+        else k_mg_down<T, NPRE, NTB, false><<<gr, NTB, 0, st>>>(a);
+    }
+    template <int NPRE>
+    static void launch_down_p(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool wide, bool mean) {
+        if (wide) launch_down_n<NPRE, MG_NT_WIDE>(gr, st, a, mean);
+        else launch_down_n<NPRE, PS_NT>(gr, st, a, mean);
+    }
+    static void launch_down(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int pre, bool wide, bool mean) {
+        switch (pre) {
+        case 0: launch_down_p<0>(gr, st, a, wide, mean); break;
+        case 1: launch_down_p<1>(gr, st, a, wide, mean); break;
+        default: launch_down_p<2>(gr, st, a, wide, mean); break;
+        }
+    }
+
     // A level with fewer tiles than CUs runs them with 1024 threads each: with one
     // wavefront per SIMD a tile's phases are bound by that wavefront's serial
     // instruction latency, so four times the threads cut it about fourfold.
@@ -1505,19 +1526,7 @@ struct PSolver {
             dim3 gr = down_grid(l);
             gr.z = B;
             if (l == 0) cx.prof_b(cx.owner, PK_MG_DOWN);
-            if (wide_blocks(gr)) {
-                switch (pre) {
-                case 0: k_mg_down<T, 0, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                case 1: k_mg_down<T, 1, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                default: k_mg_down<T, 2, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                }
-            } else {
-                switch (pre) {
-                case 0: k_mg_down<T, 0, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                case 1: k_mg_down<T, 1, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                default: k_mg_down<T, 2, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                }
-            }
+            launch_down(gr, cx.s, a, pre, wide_blocks(gr), a.bpart != nullptr);
             if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
         }
