@@ -516,42 +516,87 @@ __global__ __launch_bounds__(NT) void k_face_div(
 
 // u = u* - d grad p' with u* = u + du (p' read clamped = wall copy), p += alpha_p p'
 // with p' taken with its interior mean (scal[S_PSUM] / interior count) removed,
-// and T += dT (interior)
+// T += dT (interior); then the signed max of the nodal dilation of the corrected
+// velocities (solver.cpp:1821-1850) as block partials (rd).  A 64 x 4 tile computes
+// the corrected u, v on itself plus a one-node halo into LDS (p' staged with a
+// two-node halo; wall nodes keep their stored velocities, which no correction
+// changes), writes its own nodes and takes the dilation from LDS.  All loads of
+// the tile are issued together; halo entries beyond the grid are clamped copies
+// that no interior node uses.
 template <typename TR>
 __global__ __launch_bounds__(NT) void k_correct(
     CGrid g, const TR *__restrict__ du, const TR *__restrict__ dv, const TR *__restrict__ dT,
     const double *__restrict__ pc, const double *__restrict__ d, const double *__restrict__ xix,
     const double *__restrict__ xiy, const double *__restrict__ etx, const double *__restrict__ ety,
     const double *__restrict__ scal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ p,
-    double *__restrict__ T) {
+    double *__restrict__ T, RedArgs rd) {
     const SlotP &sp = g.sp[blockIdx.z];
+    const size_t o = zslot() * g.N;
+    rd = rz(rd);
+    scal += zslot() * SS;
+    constexpr int HX = BX + 2, HY = BY + 2, PX = BX + 4, PY = BY + 4;
+    __shared__ double s_pc[PX * PY];
+    __shared__ double s_u[HX * HY], s_v[HX * HY];
+    const int n0 = g.n0, n1 = g.n1;
+    const int tid = tlin();
+    const int i0 = blockIdx.x * BX, j0 = blockIdx.y * BY;
+    struct H {
+        double u, v, du, dv, d, x, y, e, f;
+    };
+    const auto lh = batch_load<HX * HY, NT>(tid, [&](int k) {
+        const size_t Q = o + idx(clampi(i0 - 1 + k % HX, 0, n0 - 1), clampi(j0 - 1 + k / HX, 0, n1 - 1), n0);
+        return H{u[Q], v[Q], (double)du[Q], (double)dv[Q], d[Q], xix[Q], xiy[Q], etx[Q], ety[Q]};
+    });
+    const auto lp = batch_load<PX * PY, NT>(tid, [&](int k) {
+        return pc[o + idx(clampi(i0 - 2 + k % PX, 0, n0 - 1), clampi(j0 - 2 + k / PX, 0, n1 - 1), n0)];
+    });
+    const int li = threadIdx.x, lj = threadIdx.y;
+    const int i = i0 + li, j = j0 + lj;
+    const bool own = i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2;
+    const size_t P = o + idx(clampi(i, 0, n0 - 1), clampi(j, 0, n1 - 1), n0);
+    const double p_own = p[P], T_own = T[P], dT_own = (double)dT[P];
+    const double m_x = xix[P], m_y = xiy[P], m_e = etx[P], m_f = ety[P];
     if (sp.active == 0.0) return;
     const double alpha_p = sp.alpha_p;
-    scal += zslot() * SS;
-    {
-        const size_t o = zslot() * g.N;
-        du += o; dv += o; dT += o; pc += o; d += o; xix += o; xiy += o; etx += o; ety += o; u += o; v += o;
-        p += o; T += o;
-    }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const int n0 = g.n0, n1 = g.n1;
-    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
     const double m = scal[S_PSUM] / ((double)(n0 - 2) * (double)(n1 - 2));
-    const size_t P = idx(i, j, n0);
-    const double pe = pc[idx(clampi(i + 1, 1, n0 - 2), j, n0)] - m, pw = pc[idx(clampi(i - 1, 1, n0 - 2), j, n0)] - m;
-    const double pn = pc[idx(i, clampi(j + 1, 1, n1 - 2), n0)] - m, ps = pc[idx(i, clampi(j - 1, 1, n1 - 2), n0)] - m;
-    const double dpx = 0.5 * (pe - pw) * g.r.ih1;
-    const double dpe = 0.5 * (pn - ps) * g.r.ih2;
-    const double us = u[P] + (double)du[P], vs = v[P] + (double)dv[P];
-    u[P] = us - d[P] * (xix[P] * dpx + etx[P] * dpe);
-    v[P] = vs - d[P] * (xiy[P] * dpx + ety[P] * dpe);
-    p[P] = p[P] + alpha_p * (pc[P] - m);
-    T[P] = T[P] + (double)dT[P];
+    batch_run<PX * PY, NT>(tid, lp, [&](int k, double pv) { s_pc[k] = pv; });
+    __syncthreads();
+    // corrected velocities of the halo-1 region
+    batch_run<HX * HY, NT>(tid, lh, [&](int k, const H &h) {
+        const int gi = i0 - 1 + k % HX, gj = j0 - 1 + k / HX;
+        double un = h.u, vn = h.v;
+        if (gi >= 1 && gi <= n0 - 2 && gj >= 1 && gj <= n1 - 2) {
+            auto PC = [&](int ii, int jj) { return s_pc[(jj - j0 + 2) * PX + (ii - i0 + 2)]; };
+            const double pe = PC(clampi(gi + 1, 1, n0 - 2), gj) - m, pw = PC(clampi(gi - 1, 1, n0 - 2), gj) - m;
+            const double pn = PC(gi, clampi(gj + 1, 1, n1 - 2)) - m, ps = PC(gi, clampi(gj - 1, 1, n1 - 2)) - m;
+            const double dpx = 0.5 * (pe - pw) * g.r.ih1;
+            const double dpe = 0.5 * (pn - ps) * g.r.ih2;
+            const double us = h.u + h.du, vs = h.v + h.dv;
+            un = us - h.d * (h.x * dpx + h.e * dpe);
+            vn = vs - h.d * (h.y * dpx + h.f * dpe);
+        }
+        s_u[k] = un;
+        s_v[k] = vn;
+    });
+    __syncthreads();
+    double dm = 0.0;
+    if (own) {
+        auto U = [&](int di, int dj) { return s_u[(lj + 1 + dj) * HX + (li + 1 + di)]; };
+        auto V = [&](int di, int dj) { return s_v[(lj + 1 + dj) * HX + (li + 1 + di)]; };
+        u[P] = U(0, 0);
+        v[P] = V(0, 0);
+        p[P] = p_own + alpha_p * (s_pc[(lj + 2) * PX + (li + 2)] - m);
+        T[P] = T_own + dT_own;
+        dm = m_x * (U(1, 0) - U(-1, 0)) / (2 * g.h1) + m_e * (U(0, 1) - U(0, -1)) / (2 * g.h2) +
+             m_f * (V(0, 1) - V(0, -1)) / (2 * g.h2) + m_y * (V(1, 0) - V(-1, 0)) / (2 * g.h1);
+    }
+    dm = block_max<NT>(dm);
+    if (tid == 0) rd.part[blin()] = dm;
 }
 
-// Adiabatic side walls in two launches spread over the chip (the column-strided
-// wall accesses of one block would be bound by that CU's outstanding misses): the
-// new wall values, computed from the old ones, go to scratch, then into T.
+// Adiabatic side walls spread over the chip (the column-strided wall accesses of
+// one block would be bound by that CU's outstanding misses): the new wall values,
+// computed from the old ones, go to scratch; k_p_walls uses and writes them.
 constexpr int WB = 64;  // threads per block of the wall kernels
 __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restrict__ T,
                                                 const double *__restrict__ alph, const double *__restrict__ beta,
@@ -574,18 +619,6 @@ __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restric
     txi = beta[P] / alph[P] * te;
     scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
 }
-__global__ __launch_bounds__(WB) void k_t_sides_put(CGrid g, double *__restrict__ T,
-                                                    const double *__restrict__ scratch, size_t scr) {
-    if (g.sp[blockIdx.z].active == 0.0) return;
-    T += zslot() * g.N;
-    scratch += zslot() * scr;
-    const int n0 = g.n0, n1 = g.n1;
-    const int j = 1 + (int)(blockIdx.x * WB + threadIdx.x);
-    if (j > n1 - 2) return;
-    T[idx(0, j, n0)] = scratch[j];
-    T[idx(n0 - 1, j, n0)] = scratch[n1 + j];
-}
-
 // wall pressure from the normal momentum equation (oracle: wall_p_value)
 struct WallIn {
     const double *u, *v, *T, *p, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *p1, *q1, *ajac;
@@ -596,7 +629,9 @@ __device__ __forceinline__ WallIn wall_at(WallIn w, size_t o) {
     return w;
 }
 
-__device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j) {
+// Tw: the wall node's temperature (the side walls' new values are passed in before
+// they are written back)
+__device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j, double Tw) {
     const int n0 = g.n0;
     const double h1 = g.h1, h2 = g.h2;
     const size_t P = idx(i, j, n0);
@@ -636,7 +671,7 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
         }
 #undef F
         if (k == 0) dpx = d2u / Re - conv;
-        else dpy = d2u / Re - conv + Ri * w.T[P];
+        else dpy = d2u / Re - conv + Ri * Tw;
     }
     const double J = w.ajac[P];
     if (side <= 1) {
@@ -652,23 +687,35 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
 // Wall pressure written in place: wall_p_value reads p only at interior nodes, so
 // no thread reads a value another one writes.  Block row y < 4 holds the nodes of
 // wall y (one per thread); row 4 the corners, each the mean of the two adjacent
-// wall values, computed by a lane pair.
-__global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p) {
+// wall values, computed by a lane pair.  The side walls' new temperatures come from
+// scratch (k_t_sides): each side-wall thread uses its node's value and writes it
+// into T; no thread here reads side-wall T from T.
+__global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p,
+                                                double *__restrict__ T, const double *__restrict__ scratch,
+                                                size_t scr) {
     const SlotP &sp = cg.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     const double Re = sp.Re, Ri = sp.Ri;
     const KGrid g = cg.kg();
     w = wall_at(w, zslot() * cg.N);
     p += zslot() * cg.N;
+    T += zslot() * cg.N;
+    scratch += zslot() * scr;
     const int side = blockIdx.y;
     const int t = blockIdx.x * WB + threadIdx.x;
     const int n0 = g.n0, n1 = g.n1;
+    // temperature of wall node (i, j) of wall `s`: new side-wall value or T
+    auto Tw = [&](int s, int i, int j) {
+        return s == 2 ? scratch[j] : (s == 3 ? scratch[n1 + j] : T[idx(i, j, n0)]);
+    };
     if (side < 4) {
         const int len = side <= 1 ? n0 : n1;
         if (t < 1 || t > len - 2) return;
         const int i = side == 2 ? 0 : (side == 3 ? n0 - 1 : t);
         const int j = side == 0 ? 0 : (side == 1 ? n1 - 1 : t);
-        p[idx(i, j, n0)] = wall_p_value(w, g, Re, Ri, side, i, j);
+        const double tw = Tw(side, i, j);
+        p[idx(i, j, n0)] = wall_p_value(w, g, Re, Ri, side, i, j, tw);
+        if (side >= 2) T[idx(i, j, n0)] = tw;
         return;
     }
     if (blockIdx.x != 0 || threadIdx.x >= 8) return;
@@ -680,7 +727,7 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
     const int ws = h == 0 ? (north ? 1 : 0) : (east ? 3 : 2);
     const int wi = h == 0 ? (east ? n0 - 2 : 1) : ci;
     const int wj = h == 0 ? cj : (north ? n1 - 2 : 1);
-    const double v = wall_p_value(w, g, Re, Ri, ws, wi, wj);
+    const double v = wall_p_value(w, g, Re, Ri, ws, wi, wj, Tw(ws, wi, wj));
     const double o = __shfl_xor(v, 1, 64);
     if (h == 0) p[idx(ci, cj, n0)] = 0.5 * (v + o);
 }
@@ -730,32 +777,6 @@ __device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const doub
         acc += 0.5 * (fh1 + fh2) * g.h1;
     }
     return block_sum<NTB>(acc);
-}
-
-// signed max of the nodal dilation (solver.cpp:1821-1850) -> block partials
-__global__ __launch_bounds__(NT) void k_dmax(CGrid g, const double *__restrict__ u, const double *__restrict__ v,
-                                             const double *__restrict__ xix, const double *__restrict__ xiy,
-                                             const double *__restrict__ etx, const double *__restrict__ ety,
-                                             RedArgs ra) {
-    if (g.sp[blockIdx.z].active == 0.0) return;
-    ra = rz(ra);
-    double *part = ra.part;
-    {
-        const size_t o = zslot() * g.N;
-        u += o; v += o; xix += o; xiy += o; etx += o; ety += o;
-    }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    const int n0 = g.n0, n1 = g.n1;
-    double dm = 0.0;
-    if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
-        const size_t P = idx(i, j, n0);
-        dm = xix[P] * (u[P + 1] - u[P - 1]) / (2 * g.h1) +
-             etx[P] * (u[idx(i, j + 1, n0)] - u[idx(i, j - 1, n0)]) / (2 * g.h2) +
-             ety[P] * (v[idx(i, j + 1, n0)] - v[idx(i, j - 1, n0)]) / (2 * g.h2) +
-             xiy[P] * (v[P + 1] - v[P - 1]) / (2 * g.h1);
-    }
-    dm = block_max<NT>(dm);
-    if (tlin() == 0) part[blin()] = dm;
 }
 
 // Partial-sum slots (pstride each) per case slot: 0-2 for reductions finished
@@ -1301,8 +1322,8 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     if (uvt) k_uvt_walls<<<g1, 256, 0, c->s>>>(g, c->u + o, c->v + o, c->T + o);
     const dim3 gs((c->n1 + WB - 1) / WB, 1, nz);
     k_t_sides<<<gs, WB, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr, c->scr);
-    k_t_sides_put<<<gs, WB, 0, c->s>>>(g, c->T + o, c->scratch + z0 * c->scr, c->scr);
-    k_p_walls<<<dim3((m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o);
+    k_p_walls<<<dim3((m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o, c->T + o, c->scratch + z0 * c->scr,
+                                                               c->scr);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1347,15 +1368,14 @@ int outer_body(Case *c, bool graph) {
     if (c->mixed)
         k_correct<float><<<gr, bl, 0, c->s>>>(cg(c), (const float *)du, (const float *)dv, (const float *)dT, c->pc,
                                               c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v, c->p,
-                                              c->T);
+                                              c->T, red(c));
     else
         k_correct<double><<<gr, bl, 0, c->s>>>(cg(c), (const double *)du, (const double *)dv, (const double *)dT,
                                                c->pc, c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v,
-                                               c->p, c->T);
+                                               c->p, c->T, red(c));
     CFD_LAUNCH_CHECK();
     RET(wall_bcs(c, 0, c->B, false));
-    // 8 dmax and the wall Nusselt numbers (FP64)
-    k_dmax<<<gr, bl, 0, c->s>>>(cg(c), c->u, c->v, c->xix, c->xiy, c->etx, c->ety, red(c));
+    // 8 dmax (partials from the correction) and the wall Nusselt numbers (FP64)
     {
         const dim3 grr = grid_res(c->n0, c->n1, 1);
         k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), (int)(grr.x * grr.y),
