@@ -12,7 +12,7 @@
 //   k_residual      reference RHS (hybrid convection, grad p, buoyancy) minus the
 //                   reference implicit operator applied to the current fields
 //                   -> R_u, R_v, R_T, the d factors, U, V      (solver.cpp:1062-1236, 751-848)
-//   k_jacobi2 x m/2 (A_lo/alpha) delta = R, Jacobi sweeps for u, v, T (two per launch)
+//   k_jacobin       (A_lo/alpha) delta = R, Jacobi sweeps for u, v, T (four per launch)
 //   k_face_div      u* = u + du, u-hat = u* + d grad p, and q = J-weighted divergence
 //                   of the face velocities (solver.cpp:1389-1448)
 //   p' solve        BiCGStab, right-preconditioned by a multigrid V-cycle
@@ -295,17 +295,14 @@ __global__ __launch_bounds__(NT) void k_jacobi(
     nT[P] = st * it;
 }
 
-// Two Jacobi sweeps per launch: the first sweep is computed on the block's tile
-// plus a one-node halo into LDS, the second on the tile from there, so a pair of
-// sweeps reads the coefficient arrays and the old correction once.  Node for node
-// the arithmetic of k_jacobi (same order), so a pair equals two single sweeps.
+// Several Jacobi sweeps per launch (k_jacobin): arguments, and the tile.
 template <typename TR>
 struct JArgs {
     const TR *Ru, *Rv, *RT, *Ua, *Va, *alph, *beta, *gamma, *p1, *q1, *du, *dv, *dT;
     TR *nu, *nv, *nT;
 };
 
-constexpr int JT_X = 64, JT_Y = 8;   // output tile of k_jacobi2
+constexpr int JT_X = 64, JT_Y = 8;   // output tile of k_jacobin
 
 // the node's coefficient inputs and right-hand sides
 template <typename TR>
@@ -345,24 +342,28 @@ __device__ __forceinline__ void jac_node(const JNode<TR> &c, const SlotP &ph, co
     ot = st * it;
 }
 
-// The old correction (halo 2) is staged in LDS and the coefficients of the
-// first sweep's region (halo 1) are loaded into registers, all loads issued
-// together; the second sweep runs on the inner part of the same elements, so a
-// node's coefficients are read once.  Halo entries beyond the grid hold clamped
-// copies that no interior node reads.
-template <typename TR, bool FIRST>
-__global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
+// NS sweeps per launch on a JT_X x JT_Y tile (temporal blocking): the old
+// correction (halo NS) is staged in LDS, and the coefficients of the first
+// sweep's region (halo NS - 1) are loaded into registers, all loads issued
+// together.  Sweep s covers the region with halo NS - 1 - s, on the inner part of
+// the same elements, so a node's coefficients are read once per launch; the
+// intermediate sweeps ping-pong between two LDS buffers (zero on wall nodes, like
+// the arrays) and the last writes the tile.  Node for node the arithmetic of
+// k_jacobi.  Halo entries beyond the grid hold clamped copies that no interior
+// node reads.
+template <typename TR, bool FIRST, int NS>
+__global__ __launch_bounds__(NT) void k_jacobin(CGrid g, JArgs<TR> a) {
     const SlotP ph = g.sp[blockIdx.z];
     {
         const size_t o = zslot() * g.N;
         a.Ru += o; a.Rv += o; a.RT += o; a.Ua += o; a.Va += o; a.alph += o; a.beta += o; a.gamma += o;
         a.p1 += o; a.q1 += o; a.du += o; a.dv += o; a.dT += o; a.nu += o; a.nv += o; a.nT += o;
     }
-    constexpr int AX = JT_X + 4, AY = JT_Y + 4;  // old correction, halo 2
-    constexpr int MX = JT_X + 2, MY = JT_Y + 2;  // first sweep, halo 1
+    constexpr int AX = JT_X + 2 * NS, AY = JT_Y + 2 * NS;              // old correction, halo NS
+    constexpr int MX = JT_X + 2 * (NS - 1), MY = JT_Y + 2 * (NS - 1);  // first sweep, halo NS - 1
     constexpr int NA = FIRST ? 1 : AX * AY, NM = MX * MY;
     __shared__ TR s_old[3 * NA];
-    __shared__ TR s_mid[3 * NM];
+    __shared__ TR s_buf[2][3 * NM];
     const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
     const int fi0 = blockIdx.x * JT_X, fj0 = blockIdx.y * JT_Y;
@@ -372,11 +373,12 @@ __global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
     };
     const auto lo = batch_load<NA, NT>(tid, [&](int k) {
         if (FIRST) return O3{TR(0), TR(0), TR(0)};
-        const size_t Q = idx(clampi(fi0 - 2 + k % AX, 0, n0 - 1), clampi(fj0 - 2 + k / AX, 0, n1 - 1), n0);
+        const size_t Q = idx(clampi(fi0 - NS + k % AX, 0, n0 - 1), clampi(fj0 - NS + k / AX, 0, n1 - 1), n0);
         return O3{a.du[Q], a.dv[Q], a.dT[Q]};
     });
     const auto lc = batch_load<NM, NT>(tid, [&](int k) {
-        return jac_load<TR>(a, idx(clampi(fi0 - 1 + k % MX, 0, n0 - 1), clampi(fj0 - 1 + k / MX, 0, n1 - 1), n0));
+        return jac_load<TR>(a, idx(clampi(fi0 - (NS - 1) + k % MX, 0, n0 - 1),
+                                   clampi(fj0 - (NS - 1) + k / MX, 0, n1 - 1), n0));
     });
     if (ph.active == 0.0) return;
     if (!FIRST) {
@@ -387,29 +389,41 @@ __global__ __launch_bounds__(NT) void k_jacobi2(CGrid g, JArgs<TR> a) {
         });
         __syncthreads();
     }
-    auto old_at = [&](int f, int ii, int jj) { return s_old[f * NA + (jj - fj0 + 2) * AX + (ii - fi0 + 2)]; };
-    batch_run<NM, NT>(tid, lc, [&](int k, const JNode<TR> &c) {
-        const int gi = fi0 - 1 + k % MX, gj = fj0 - 1 + k / MX;
-        TR ou = TR(0), ov = TR(0), ot = TR(0);
-        if (inner_node(gi, gj, n0, n1)) jac_node<TR, FIRST>(c, ph, r, gi, gj, old_at, ou, ov, ot);
-        s_mid[k] = ou;
-        s_mid[NM + k] = ov;
-        s_mid[2 * NM + k] = ot;
-    });
-    __syncthreads();
-    auto mid_at = [&](int f, int ii, int jj) { return s_mid[f * NM + (jj - fj0 + 1) * MX + (ii - fi0 + 1)]; };
-    batch_run<NM, NT>(tid, lc, [&](int k, const JNode<TR> &c) {
-        const int li = k % MX, lj = k / MX;
-        if (li < 1 || li > JT_X || lj < 1 || lj > JT_Y) return;
-        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
-        if (!inner_node(gi, gj, n0, n1)) return;
-        const size_t P = idx(gi, gj, n0);
-        TR ou, ov, ot;
-        jac_node<TR, false>(c, ph, r, gi, gj, mid_at, ou, ov, ot);
-        a.nu[P] = ou;
-        a.nv[P] = ov;
-        a.nT[P] = ot;
-    });
+    auto old_at = [&](int f, int ii, int jj) { return s_old[f * NA + (jj - fj0 + NS) * AX + (ii - fi0 + NS)]; };
+#pragma unroll
+    for (int sw = 0; sw < NS; sw++) {
+        const TR *in = s_buf[(sw + 1) & 1];  // the previous sweep's output (sw >= 1)
+        TR *out = s_buf[sw & 1];
+        auto mid_at = [&](int f, int ii, int jj) {
+            return in[f * NM + (jj - fj0 + NS - 1) * MX + (ii - fi0 + NS - 1)];
+        };
+        batch_run<NM, NT>(tid, lc, [&](int k, const JNode<TR> &c) {
+            const int li = k % MX, lj = k / MX;
+            if (li < sw || li >= MX - sw || lj < sw || lj >= MY - sw) return;  // outside this sweep's region
+            const int gi = fi0 - (NS - 1) + li, gj = fj0 - (NS - 1) + lj;
+            const bool in_ = inner_node(gi, gj, n0, n1);
+            if (sw == NS - 1) {  // the tile itself: to global memory
+                if (!in_) return;
+                const size_t P = idx(gi, gj, n0);
+                TR ou, ov, ot;
+                if (sw == 0) jac_node<TR, FIRST>(c, ph, r, gi, gj, old_at, ou, ov, ot);
+                else jac_node<TR, false>(c, ph, r, gi, gj, mid_at, ou, ov, ot);
+                a.nu[P] = ou;
+                a.nv[P] = ov;
+                a.nT[P] = ot;
+                return;
+            }
+            TR ou = TR(0), ov = TR(0), ot = TR(0);
+            if (in_) {
+                if (sw == 0) jac_node<TR, FIRST>(c, ph, r, gi, gj, old_at, ou, ov, ot);
+                else jac_node<TR, false>(c, ph, r, gi, gj, mid_at, ou, ov, ot);
+            }
+            out[k] = ou;
+            out[NM + k] = ov;
+            out[2 * NM + k] = ot;
+        });
+        if (sw < NS - 1) __syncthreads();
+    }
 }
 
 // Face velocities and their divergence in one pass.  On a 64 x 4 tile plus a
@@ -1275,16 +1289,19 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
     const dim3 g2((c->n0 + JT_X - 1) / JT_X, (c->n1 + JT_Y - 1) / JT_Y, c->B);
     const int m = c->prm.mom_sweeps;
     for (int s = 0; s < m;) {
-        if (m - s >= 2) {  // two sweeps in one launch
+        if (m - s >= 2) {  // four or two sweeps in one launch
             const JArgs<TR> ja{Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0], a[1], a[2], b[0], b[1], b[2]};
-            if (s == 0) {
-                k_jacobi2<TR, true><<<g2, NT, 0, c->s>>>(cg(c), ja);
+            const int ns = m - s >= 4 ? 4 : 2;
+            if (s > 0) prof_begin(c, 1);
+            if (ns == 4) {
+                if (s == 0) k_jacobin<TR, true, 4><<<g2, NT, 0, c->s>>>(cg(c), ja);
+                else k_jacobin<TR, false, 4><<<g2, NT, 0, c->s>>>(cg(c), ja);
             } else {
-                prof_begin(c, 1);
-                k_jacobi2<TR, false><<<g2, NT, 0, c->s>>>(cg(c), ja);
-                prof_end(c);
+                if (s == 0) k_jacobin<TR, true, 2><<<g2, NT, 0, c->s>>>(cg(c), ja);
+                else k_jacobin<TR, false, 2><<<g2, NT, 0, c->s>>>(cg(c), ja);
             }
-            s += 2;
+            if (s > 0) prof_end(c);
+            s += ns;
         } else if (s == 0) {
             k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
                                                     a[1], a[2], b[0], b[1], b[2]);
