@@ -49,7 +49,7 @@ def kernel_bytes(name, mixed):
         "p_apply": 7 * w,          # x, 4 face coefficients, r0 (or s) -> y; dot product in the kernel
         "mg_down": 7.25 * w,       # b, 1/diag, 4 face coefficients -> x, restricted residual (1/4 node)
         "mg_up": 8.25 * w,         # x, b, 1/diag, 4 faces, coarse x (1/4 node) -> x
-        "jacobi": 16 * w,          # two sweeps: R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
+        "jacobi": 16 * w,          # four sweeps: R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
         "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
         "face_div": 10 * 8 + 2 * w,  # u,v,p,d, 4 metrics, J (FP64), du,dv -> q (FP64)
     }[name]
