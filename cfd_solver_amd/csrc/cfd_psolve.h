@@ -369,7 +369,7 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
 
 constexpr int MG_FX = 64, MG_FY = 8;   // fine tile
 constexpr int MG_NT_WIDE = 1024;       // threads per tile on levels with few tiles
-constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
+constexpr long MG_WIDE_BELOW = 256;    // tiles per slot below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
 
 template <typename T>
@@ -1506,8 +1506,10 @@ struct PSolver {
 
     // A level with fewer tiles than CUs runs them with 1024 threads each: with one
     // wavefront per SIMD a tile's phases are bound by that wavefront's serial
-    // instruction latency, so four times the threads cut it about fourfold.
-    static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
+    // instruction latency, so four times the threads cut it about fourfold.  The
+    // choice counts one slot's tiles only, so a case computes the same bits alone
+    // and in a batch (the block size fixes the order of the partial sums).
+    static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y < MG_WIDE_BELOW; }
 
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {

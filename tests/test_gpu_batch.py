@@ -95,3 +95,26 @@ def test_batch_eager_equals_graph():
     for z in range(3):
         for k in ("u", "v", "T", "p"):
             assert np.array_equal(fe[z][k], fg[z][k]), (z, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_slot_equals_alone_at_257(precision):
+    """At 257x257 the finest multigrid level has 165 tiles per slot: the block size
+    chosen for it (which fixes the order of its partial sums) must not depend on
+    how many slots share the launch, or a slot would differ from the case alone."""
+    print(f"\n[gpu] 257x257: 2-slot batch vs alone, precision {precision}", flush=True)
+    from cfd_solver_amd.cavity import CavityBatch, default_params
+    n, steps = 257, 4
+    skew, beta, phys = CASES[1]
+    g0 = cavity_grid(n)
+    b = CavityBatch(n, n, g0.dxi, g0.deta, 2, default_params(precision=precision))
+    b.set_case(0, cavity_grid(n, skew=skew, beta=beta), **phys)
+    b.set_case(1, cavity_grid(n, skew=CASES[0][0], beta=CASES[0][1]), **CASES[0][2])
+    for _ in range(steps):
+        st = b.step()
+    fa, sa = _alone(n, skew, beta, phys, precision, steps)
+    fb = b.fields(0)
+    for k in ("u", "v", "T", "p"):
+        assert np.array_equal(fa[k], fb[k]), (k, np.abs(fa[k] - fb[k]).max())
+    assert st[0].p_iters == sa.p_iters and st[0].mass_l1 == sa.mass_l1
+    b.close()
