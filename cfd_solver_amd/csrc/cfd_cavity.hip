@@ -1064,18 +1064,14 @@ int dalloc(Case *c, T **p, size_t n) {
 // ---- live kernel timing
 // kinds: 0 p' operator apply (fine level), 1 predictor Jacobi sweep,
 // 2 residual/RHS, 3 face divergence, 4 multigrid down (fine level), 5 multigrid up
-void prof_begin(Case *c, int kind) {
-    if (!c->prof || c->capturing) return;
-    if (c->ev_kind.size() * 2 > (size_t)c->ev_used) c->ev_kind.pop_back();  // drop a dangling begin
-    if (c->ev_used + 2 > (int)c->ev.size()) return;
-    (void)hipEventRecord(c->ev[c->ev_used], c->s);
-    c->ev_kind.push_back(kind);
-}
-void prof_end(Case *c) {
-    // a begin is pending when there is one more kind than recorded pairs
-    if (!c->prof || c->ev_kind.size() * 2 != (size_t)c->ev_used + 2) return;
-    (void)hipEventRecord(c->ev[c->ev_used + 1], c->s);
+// The pair of events that the next launch of `kind` is to be bound to (both null
+// when profiling is off, the stream is being captured or the events are used up).
+Timed prof_timed(Case *c, int kind) {
+    if (!c->prof || c->capturing || c->ev_used + 2 > (int)c->ev.size()) return Timed{};
+    const Timed t{c->ev[c->ev_used], c->ev[c->ev_used + 1]};
     c->ev_used += 2;
+    c->ev_kind.push_back(kind);
+    return t;
 }
 void prof_drain(Case *c) {
     if (!c->prof) return;
@@ -1094,13 +1090,12 @@ void prof_drain(Case *c) {
     c->ev_used = 0;
     c->ev_kind.clear();
 }
-void cb_begin(void *o, int kind) { prof_begin(static_cast<Case *>(o), kind); }
-void cb_end(void *o) { prof_end(static_cast<Case *>(o)); }
+Timed cb_timed(void *o, int kind) { return prof_timed(static_cast<Case *>(o), kind); }
 void cb_drain(void *o) { prof_drain(static_cast<Case *>(o)); }
 
 size_t psz(const Case *c) { return (size_t)PS_SLOTS * c->pstride; }
 PsCtx ctx(Case *c) {
-    PsCtx cx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_begin, cb_end, cb_drain, c,
+    PsCtx cx{c->s, c->B, c->part, c->pstride, psz(c), c->scal, c->hscal, cb_timed, cb_drain, c,
              &c->allocs, &c->bytes, c->dry};
     if (c->dflag) {
         cx.flag_d = c->dflag;
@@ -1260,12 +1255,10 @@ WallIn wall_in(const Case *c) {
 
 template <typename TR>
 int launch_residual_t(Case *c) {
-    prof_begin(c, 2);
     const dim3 gr = grid_res(c->n0, c->n1, c->B);
-    k_residual<TR><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), (TR *)c->Ru,
-                                                   (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V,
-                                                   red(c, PS_RES));
-    prof_end(c);
+    launch_timed(prof_timed(c, 2), k_residual<TR>, gr, dim3(BX, BY), c->s, cg(c), (const double *)c->u,
+                 (const double *)c->v, (const double *)c->T, (const double *)c->p, gptrs(c), (TR *)c->Ru,
+                 (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c, PS_RES));
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1292,15 +1285,14 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
         if (m - s >= 2) {  // four or two sweeps in one launch
             const JArgs<TR> ja{Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0], a[1], a[2], b[0], b[1], b[2]};
             const int ns = m - s >= 4 ? 4 : 2;
-            if (s > 0) prof_begin(c, 1);
+            const Timed tm = s > 0 ? prof_timed(c, 1) : Timed{};
             if (ns == 4) {
-                if (s == 0) k_jacobin<TR, true, 4><<<g2, NT, 0, c->s>>>(cg(c), ja);
-                else k_jacobin<TR, false, 4><<<g2, NT, 0, c->s>>>(cg(c), ja);
+                if (s == 0) launch_timed(tm, k_jacobin<TR, true, 4>, g2, dim3(NT), c->s, cg(c), ja);
+                else launch_timed(tm, k_jacobin<TR, false, 4>, g2, dim3(NT), c->s, cg(c), ja);
             } else {
-                if (s == 0) k_jacobin<TR, true, 2><<<g2, NT, 0, c->s>>>(cg(c), ja);
-                else k_jacobin<TR, false, 2><<<g2, NT, 0, c->s>>>(cg(c), ja);
+                if (s == 0) launch_timed(tm, k_jacobin<TR, true, 2>, g2, dim3(NT), c->s, cg(c), ja);
+                else launch_timed(tm, k_jacobin<TR, false, 2>, g2, dim3(NT), c->s, cg(c), ja);
             }
-            if (s > 0) prof_end(c);
             s += ns;
         } else if (s == 0) {
             k_jacobi<TR, true><<<gr, bl, 0, c->s>>>(cg(c), Ru, Rv, RT, U, V, co[0], co[1], co[2], co[3], co[4], a[0],
@@ -1349,18 +1341,15 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
 // partials in PS_MASS, Σq and Σq² for the p' solve in slots 0-1); du, dv: the
 // predictor's final corrections
 int launch_face_div(Case *c, const void *du, const void *dv) {
-    prof_begin(c, 3);
+    const Timed tm = prof_timed(c, 3);
     if (c->mixed)
-        k_face_div<float><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const float *)du,
-                                                                 (const float *)dv, c->p, c->d, c->xix, c->xiy,
-                                                                 c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS),
-                                                                 red(c));
+        launch_timed(tm, k_face_div<float>, grid_b(c), dim3(BX, BY), c->s, cg(c), c->u, c->v, (const float *)du,
+                     (const float *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->q,
+                     red(c, PS_MASS), red(c));
     else
-        k_face_div<double><<<grid_b(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const double *)du,
-                                                                  (const double *)dv, c->p, c->d, c->xix, c->xiy,
-                                                                  c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS),
-                                                                  red(c));
-    prof_end(c);
+        launch_timed(tm, k_face_div<double>, grid_b(c), dim3(BX, BY), c->s, cg(c), c->u, c->v,
+                     (const double *)du, (const double *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
+                     c->q, red(c, PS_MASS), red(c));
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }

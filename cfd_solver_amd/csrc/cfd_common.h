@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdio>
@@ -35,6 +36,19 @@
     } while (0)
 
 namespace cfd {
+
+// A launch that a pair of HIP events may time.  With events, hipExtLaunchKernelGGL
+// binds both to the kernel's own dispatch, so their elapsed time is the kernel's
+// duration without the gap to the launches around it; without events it is a plain
+// launch (the form a graph capture records).
+struct Timed {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+template <typename... P, typename... A>
+inline void launch_timed(const Timed &t, void (*k)(P...), dim3 g, dim3 b, hipStream_t s, A... a) {
+    if (t.start) hipExtLaunchKernelGGL(k, g, b, 0, s, t.start, t.stop, 0, a...);
+    else k<<<g, b, 0, s>>>(a...);
+}
 
 // 2-D launch shape: 64 lanes along xi (one wavefront per row segment), 4 rows per block.
 constexpr int BX = 64;

@@ -1291,8 +1291,7 @@ struct PsCtx {
     size_t psz;
     double *scal;   // B x SS
     double *hscal;  // pinned host mirror of scal
-    void (*prof_b)(void *, int);
-    void (*prof_e)(void *);
+    Timed (*prof)(void *, int);  // events timing the next launch of a kind (none: untimed)
     void (*drain)(void *);
     void *owner;
     std::vector<void *> *allocs;
@@ -1487,21 +1486,28 @@ struct PSolver {
 
     // k_mg_down<T, pre, threads, mean> on grid gr
     template <int NPRE, int NTB>
-    static void launch_down_n(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool mean) {
-        if (mean) k_mg_down<T, NPRE, NTB, true><<<gr, NTB, 0, st>>>(a);
-        else k_mg_down<T, NPRE, NTB, false><<<gr, NTB, 0, st>>>(a);
+    static void launch_down_n(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool mean, const Timed &tm) {
+        if (mean) launch_timed(tm, k_mg_down<T, NPRE, NTB, true>, gr, NTB, st, a);
+        else launch_timed(tm, k_mg_down<T, NPRE, NTB, false>, gr, NTB, st, a);
     }
     template <int NPRE>
-    static void launch_down_p(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool wide, bool mean) {
-        if (wide) launch_down_n<NPRE, MG_NT_WIDE>(gr, st, a, mean);
-        else launch_down_n<NPRE, PS_NT>(gr, st, a, mean);
+    static void launch_down_p(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool wide, bool mean,
+                              const Timed &tm) {
+        if (wide) launch_down_n<NPRE, MG_NT_WIDE>(gr, st, a, mean, tm);
+        else launch_down_n<NPRE, PS_NT>(gr, st, a, mean, tm);
     }
-    static void launch_down(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int pre, bool wide, bool mean) {
+    static void launch_down(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int pre, bool wide, bool mean,
+                            const Timed &tm = Timed{}) {
         switch (pre) {
-        case 0: launch_down_p<0>(gr, st, a, wide, mean); break;
-        case 1: launch_down_p<1>(gr, st, a, wide, mean); break;
-        default: launch_down_p<2>(gr, st, a, wide, mean); break;
+        case 0: launch_down_p<0>(gr, st, a, wide, mean, tm); break;
+        case 1: launch_down_p<1>(gr, st, a, wide, mean, tm); break;
+        default: launch_down_p<2>(gr, st, a, wide, mean, tm); break;
         }
+    }
+    template <int NPOST>
+    static void launch_up_p(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, bool wide, const Timed &tm) {
+        if (wide) launch_timed(tm, k_mg_up<T, NPOST, MG_NT_WIDE>, gr, MG_NT_WIDE, st, a);
+        else launch_timed(tm, k_mg_up<T, NPOST, PS_NT>, gr, PS_NT, st, a);
     }
 
     // A level with fewer tiles than CUs runs them with 1024 threads each: with one
@@ -1527,9 +1533,8 @@ struct PSolver {
                           l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w, lps, cx.scal, gs};
             dim3 gr = down_grid(l);
             gr.z = B;
-            if (l == 0) cx.prof_b(cx.owner, PK_MG_DOWN);
-            launch_down(gr, cx.s, a, pre, wide_blocks(gr), a.bpart != nullptr);
-            if (l == 0) cx.prof_e(cx.owner);
+            launch_down(gr, cx.s, a, pre, wide_blocks(gr), a.bpart != nullptr,
+                        l == 0 ? cx.prof(cx.owner, PK_MG_DOWN) : Timed{});
             CFD_LAUNCH_CHECK();
         }
         T *cout = lc == 0 ? z : lev[lc].x;
@@ -1542,21 +1547,12 @@ struct PSolver {
             UpArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.b, L.x, C.n0, C.n1, l + 1 == lc ? C.x : C.t,
                         l == 0 ? z : L.t, w, cx.scal, gs};
             const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
-            if (l == 0) cx.prof_b(cx.owner, PK_MG_UP);
-            if (wide_blocks(gr)) {
-                switch (post) {
-                case 0: k_mg_up<T, 0, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                case 1: k_mg_up<T, 1, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                default: k_mg_up<T, 2, MG_NT_WIDE><<<gr, MG_NT_WIDE, 0, cx.s>>>(a); break;
-                }
-            } else {
-                switch (post) {
-                case 0: k_mg_up<T, 0, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                case 1: k_mg_up<T, 1, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                default: k_mg_up<T, 2, PS_NT><<<gr, PS_NT, 0, cx.s>>>(a); break;
-                }
+            const Timed tm = l == 0 ? cx.prof(cx.owner, PK_MG_UP) : Timed{};
+            switch (post) {
+            case 0: launch_up_p<0>(gr, cx.s, a, wide_blocks(gr), tm); break;
+            case 1: launch_up_p<1>(gr, cx.s, a, wide_blocks(gr), tm); break;
+            default: launch_up_p<2>(gr, cx.s, a, wide_blocks(gr), tm); break;
             }
-            if (l == 0) cx.prof_e(cx.owner);
             CFD_LAUNCH_CHECK();
         }
         return CFD_OK;
@@ -1591,17 +1587,15 @@ struct PSolver {
             k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
             CFD_RET(precond(cx, pv, ph, S_DONE));
-            cx.prof_b(cx.owner, PK_APPLY);
-            k_bi_v<T><<<gr, bl, 0, cx.s>>>(a0, ph, vv, r0, ra);
-            cx.prof_e(cx.owner);
+            launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
+                         (const T *)r0, ra);
             k_bi_fin<FIN_V><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
             k_bi_fin<FIN_S><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             CFD_RET(precond(cx, s, sh, S_DONE));
-            cx.prof_b(cx.owner, PK_APPLY);
-            k_bi_t<T><<<gr, bl, 0, cx.s>>>(a0, sh, t, s, ra);
-            cx.prof_e(cx.owner);
+            launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
+                         (const T *)s, ra);
             k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
             // eager: the iteration's end flags go straight to host memory and the host
