@@ -47,6 +47,16 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RefParams(C.Structure):
+    _fields_ = [("Re", _d), ("Pr", _d), ("Ri", _d), ("dt", _d), ("uinf", _d), ("vinf", _d),
+                ("speed_amp", _d), ("accn_amp", _d), ("F", _d), ("time", _d), ("mom_rtol", _d),
+                ("p_rtol", _d), ("mom_maxit", _i), ("p_maxit", _i)]
+
+
+class RefStats(C.Structure):
+    _fields_ = [("dmax", _d), ("circ", _d), ("iters", _i * 6), ("relres", _d * 6)]
+
+
 # every symbol include/cfd_gpu.h declares, with its ctypes signature
 SIGNATURES = {
     "cfd_version": ([], C.c_char_p),
@@ -59,9 +69,18 @@ SIGNATURES = {
     "cfd_sip9p": ([_i, _i] + [dp] * 9 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
     "cfd_ref_pcor_bc": ([_i, _i, dp, dp, dp, _d, _d], _i),
     "cfd_ref_correct": ([_i, _i, _d, _d] + [dp] * 7 + [_d, dp, dp, dp], _i),
+    "cfd_ref_outflow_up": ([_i, _i, dp, dp, dp, dp, dp, dp, _d, _d], _i),
+    "cfd_ref_farfield": ([_i, _i, _d, dp, dp] + [dp] * 5 + [dp] * 7 + [_d] * 6 + [_p], _i),
+    "cfd_ref_exit_pressure": ([_i, _i, _d, _d] + [dp] * 6 + [dp] * 9 + [dp, dp] + [_d] * 5, _i),
     "cfd_ref_wall_pressure": ([_i, _i, _d, _d] + [dp] * 4 + [dp] * 9 + [_p, _p, _d, _d], _i),
     "cfd_ref_dilation": ([_i, _i, _d, _d] + [dp] * 6 + [dp, C.POINTER(_d)], _i),
     "cfd_ref_nusselt": ([_i, _i, _d, _d, dp, dp, dp, dp, _d, C.POINTER(_d), _p], _i),
+    "cfd_ref_case_create": ([_i, _i, _d, _d] + [dp] * 10 + [_p, _p, dp, dp, C.POINTER(RefParams),
+                            C.POINTER(_p)], _i),
+    "cfd_ref_case_set_state": ([_p] + [dp] * 6, _i),
+    "cfd_ref_case_get_state": ([_p] + [dp] * 6, _i),
+    "cfd_ref_case_step": ([_p, _i, C.POINTER(RefStats)], _i),
+    "cfd_ref_case_destroy": ([_p], None),
     "cfd_params_default": ([C.POINTER(Params)], None),
     "cfd_case_create": ([_i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), C.POINTER(_p)], _i),
     "cfd_case_destroy": ([_p], _i),

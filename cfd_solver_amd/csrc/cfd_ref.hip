@@ -353,6 +353,155 @@ __global__ void k_wall_p_ref(int n0, int n1, double h1, double h2, WallRef w, do
     if (i == 0) p[ix(n0 - 1, j, n0)] = val;
 }
 
+// ---------------------------------------------------------------- outer-boundary rows
+// Predictor velocities on the outer row j = n1-1 (solver.cpp:1351-1381): the
+// current velocity where the free stream enters (vnn >= 0), else the quadratic
+// extrapolation (5 f_{j-1} - 4 f_{j-2} + f_{j-3}) / 2.  One thread per column.
+__global__ void k_outflow_up_ref(int n0, int n1, double *__restrict__ up, double *__restrict__ vp,
+                                 const double *__restrict__ u, const double *__restrict__ v,
+                                 const double *__restrict__ xnox, const double *__restrict__ xnoy, double uinf,
+                                 double vinf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n0 - 2) return;
+    const int j = n1 - 1;
+    const size_t P = ix(i, j, n0);
+    const double vnn = uinf * xnox[i] + vinf * xnoy[i];
+    double a, b;
+    if (vnn >= 0) {
+        a = u[P];
+        b = v[P];
+    } else {
+        a = (5.0 * up[ix(i, j - 1, n0)] - 4.0 * up[ix(i, j - 2, n0)] + up[ix(i, j - 3, n0)]) / 2.0;
+        b = (5.0 * vp[ix(i, j - 1, n0)] - 4.0 * vp[ix(i, j - 2, n0)] + vp[ix(i, j - 3, n0)]) / 2.0;
+    }
+    up[P] = a;
+    vp[P] = b;
+    if (i == 0) {
+        up[ix(n0 - 1, j, n0)] = a;
+        vp[ix(n0 - 1, j, n0)] = b;
+    }
+}
+
+// Boundary velocities after the correction (solver.cpp:1543-1655), one block:
+// the circulation along row n1-2 (a block sum of the trapezoid terms), then per
+// outer column the far-field prediction from the radial/tangential velocity of
+// row n1-2 -- vr0 cr^2 + vr_inf (1 - cr^2) and vth0 cr^k + vth_inf (1 - cr^k),
+// k = 1 when |circulation| > 1e-2 else 2, cr the radius ratio of the two rows
+// (cr^2 evaluated as cr * cr) -- where the flow leaves, the free stream (T = 0)
+// where it enters, and T convected from the step's start values Told, uet; the
+// wall row gets the cylinder velocity (-a y, a x), a = amp * cwt, also in up/vp.
+// Rows n1-2 and n1-1 are distinct (n1 >= 5), so the circulation reads no value
+// this kernel writes.
+constexpr int FF_NT = 256;
+struct FarRef {
+    const double *x, *y, *Told, *uet, *etx, *ety, *ajac, *xnox, *xnoy;
+};
+__global__ __launch_bounds__(FF_NT) void k_farfield_ref(int n0, int n1, double h2, FarRef a, double *__restrict__ u,
+                                                        double *__restrict__ v, double *__restrict__ T,
+                                                        double *__restrict__ up, double *__restrict__ vp,
+                                                        double uinf, double vinf, double dt, double amp, double cwt,
+                                                        double *__restrict__ circ_out) {
+    const int t = threadIdx.x;
+    const int jm = n1 - 2, j = n1 - 1;
+    const double de = 1.0 / (n0 - 2);
+    double part = 0.0;
+    for (int i = t; i < n0 - 1; i += FF_NT) {
+        const size_t A = ix(i, jm, n0), B = ix(i + 1, jm, n0);
+        const double f1 = (u[A] * a.ety[A] - v[A] * a.etx[A]) * fabs(a.ajac[A]);
+        const double f2 = (u[B] * a.ety[B] - v[B] * a.etx[B]) * fabs(a.ajac[B]);
+        part += de * 0.5 * (f1 + f2);
+    }
+    __shared__ double s_circ;
+    const double circ = block_sum<FF_NT>(part);
+    if (t == 0) {
+        s_circ = circ;
+        if (circ_out) *circ_out = circ;
+    }
+    __syncthreads();
+    const bool k1 = fabs(s_circ) > 1e-2;
+    for (int i = t; i < n0 - 1; i += FF_NT) {
+        const size_t Q = ix(i, jm, n0), P = ix(i, j, n0);
+        const double rq = sqrt(a.x[Q] * a.x[Q] + a.y[Q] * a.y[Q]);
+        const double cq = a.x[Q] / rq, sq = a.y[Q] / rq;
+        const double vr0 = u[Q] * cq + v[Q] * sq, vt0 = -u[Q] * sq + v[Q] * cq;
+        const double rp = sqrt(a.x[P] * a.x[P] + a.y[P] * a.y[P]);
+        const double cr = rq / rp, c = a.x[P] / rp, s = a.y[P] / rp;
+        const double vrinf = uinf * c + vinf * s, vtinf = -uinf * s + vinf * c;
+        const double cr2 = cr * cr, crk = k1 ? cr : cr2;
+        const double vr1 = vr0 * cr2 + vrinf * (1 - cr2), vt1 = vt0 * crk + vtinf * (1 - crk);
+        const double vnn = uinf * a.xnox[i] + vinf * a.xnoy[i];
+        double uu = uinf, vv = vinf, tt = 0.0;
+        if (vnn < 0) {
+            uu = c * vr1 - s * vt1;
+            vv = s * vr1 + c * vt1;
+            tt = a.Told[P] - (a.uet[P] * dt / h2) * (a.Told[P] - a.Told[ix(i, j - 1, n0)]);
+        }
+        u[P] = uu;
+        v[P] = vv;
+        T[P] = tt;
+        if (i == 0) {
+            const size_t D = ix(n0 - 1, j, n0);
+            u[D] = uu;
+            v[D] = vv;
+            T[D] = tt;
+        }
+    }
+    for (int i = t; i < n0; i += FF_NT) {
+        const size_t P = ix(i, 0, n0);
+        const double uu = -amp * cwt * a.y[P], vv = amp * cwt * a.x[P];
+        u[P] = uu;
+        up[P] = uu;
+        v[P] = vv;
+        vp[P] = vv;
+    }
+}
+
+// Outer-row pressure (solver.cpp:1731-1782): the normal momentum equation with
+// one-sided eta differences and local acceleration (f - f_old)/dt where the free
+// stream enters, Gresho's condition (3 uet_j - 4 uet_{j-1} + uet_{j-2}) / (2 Re h2)
+// where it leaves (uxi, uet of the current velocity).  One thread per column.
+struct ExitRef {
+    const double *u, *v, *T, *uold, *vold, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *q1, *ajac, *xnox, *xnoy;
+};
+__global__ void k_exit_p_ref(int n0, int n1, double h1, double h2, ExitRef w, double *__restrict__ p, double uinf,
+                             double vinf, double dt, double Re, double Ri) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n0 - 2) return;
+    const int j = n1 - 1;
+    const size_t P = ix(i, j, n0), Q = ix(i, j - 1, n0), R = ix(i, j - 2, n0);
+    const double vnn = uinf * w.xnox[i] + vinf * w.xnoy[i];
+    double val;
+    if (vnn >= 0) {
+        const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+        const double uxi = w.xix[P] * w.u[P] + w.xiy[P] * w.v[P];
+        const double uet = w.etx[P] * w.u[P] + w.ety[P] * w.v[P];
+        double dp_dx = 0.0, dp_dy = 0.0;
+        for (int k = 0; k < 2; k++) {
+            const double *f = k == 0 ? w.u : w.v;
+            const double *fo = k == 0 ? w.uold : w.vold;
+            const double aa = w.alph[P] * (f[ix(ipp, j, n0)] + f[ix(inn, j, n0)] - 2 * f[P]) / (h1 * h1);
+            const double gg = w.gamma[P] * (f[P] + f[R] - 2 * f[Q]) / (h2 * h2);
+            const double bb = w.beta[P] * (f[ix(ipp, j, n0)] + f[ix(inn, j - 1, n0)] - f[ix(ipp, j - 1, n0)] -
+                                           f[ix(inn, j, n0)]) / (2 * h1 * h2);
+            const double qqq = w.q1[P] * (3 * f[P] - 4 * f[Q] + f[R]) / (2 * h2);
+            const double d2u = aa + gg - 2 * bb + qqq;
+            double conv = uxi * 0.5 * (f[ix(ipp, j, n0)] - f[ix(inn, j, n0)]) / h1;
+            conv = conv + uet * (3.0 * f[P] - 4 * f[Q] + f[R]) / (2 * h2);
+            const double alc = (f[P] - fo[P]) / dt;
+            if (k == 0) dp_dx = 1.0 * d2u / Re - conv - alc;
+            else dp_dy = 1.0 * d2u / Re - conv - alc + Ri * w.T[P];
+        }
+        val = p[Q] + (dp_dx * (-w.xiy[P] * w.ajac[P]) + dp_dy * (w.xix[P] * w.ajac[P])) * h2;
+    } else {
+        const double e0 = w.etx[P] * w.u[P] + w.ety[P] * w.v[P];
+        const double e1 = w.etx[Q] * w.u[Q] + w.ety[Q] * w.v[Q];
+        const double e2 = w.etx[R] * w.u[R] + w.ety[R] * w.v[R];
+        val = 0.5 * (1.0 / Re) * ((3 * e0 - 4 * e1 + e2) / h2);
+    }
+    p[P] = val;
+    if (i == 0) p[ix(n0 - 1, j, n0)] = val;
+}
+
 // ---------------------------------------------------------------- a12, a13
 __global__ void k_dil_ref(int n0, int n1, double h1, double h2, const double *__restrict__ u,
                           const double *__restrict__ v, const double *__restrict__ xix,
@@ -518,42 +667,57 @@ int down(double *h, const double *d, size_t n) {
 
 bool bad_grid(int n0, int n1) { return n0 < 5 || n1 < 5; }
 
-// GPU BiCGStab (Jacobi-preconditioned) on a stored 9/25-point operator
+// Work vectors of the stored-operator BiCGStab on n0 x n1 nodes (device).
+struct SopWork {
+    double *r = nullptr, *r0 = nullptr, *pv = nullptr, *vv = nullptr, *s = nullptr, *t = nullptr, *ph = nullptr,
+           *sh = nullptr, *dx = nullptr, *invd = nullptr, *part = nullptr, *dsc = nullptr;
+    // the scratch that one solve needs: 10 node vectors, block partials, 8 scalars
+    static size_t doubles(int n0, int n1) {
+        const dim3 gr = grid2d(n0, n1);
+        return 10 * (size_t)n0 * n1 + (size_t)gr.x * gr.y + 16;
+    }
+    static SopWork carve(double *base, int n0, int n1) {
+        const size_t N = (size_t)n0 * n1;
+        SopWork w;
+        double **v[10] = {&w.r, &w.r0, &w.pv, &w.vv, &w.s, &w.t, &w.ph, &w.sh, &w.dx, &w.invd};
+        for (int k = 0; k < 10; k++) *v[k] = base + (size_t)k * N;
+        w.part = base + 10 * N;
+        const dim3 gr = grid2d(n0, n1);
+        w.dsc = w.part + (size_t)gr.x * gr.y + 8;
+        return w;
+    }
+};
+
+// GPU BiCGStab (Jacobi-preconditioned) on a stored 9/25-point operator A (device
+// planes), x (device, boundary rows fixed) updated in place, right-hand side qd
+// (device), on stream st; the dot products are read back on the host.
 template <int R>
-int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double *phi, const double *q,
-              double rtol, int maxit, int *iters, double *relres) {
-    if (bad_grid(n0, n1) || !phi || !q) return CFD_E_ARG;
-    for (int k = 0; k < npts; k++)
-        if (!planes_host[k]) return CFD_E_ARG;
+int sop_core(int n0, int n1, const double *A, double *x, const double *qd, const SopWork &w, double rtol, int maxit,
+             int *iters, double *relres, hipStream_t st) {
     const size_t N = (size_t)n0 * n1;
-    DevBufs B;
-    double *A = B.get((size_t)npts * N);
-    NEED(A);
-    for (int k = 0; k < npts; k++)
-        CFD_HIP(hipMemcpy(A + (size_t)k * N, planes_host[k], N * sizeof(double), hipMemcpyHostToDevice));
-    double *x = B.up(phi, N), *qd = B.up(q, N);
-    double *r = B.get(N), *r0 = B.get(N), *pv = B.up(nullptr, N), *vv = B.up(nullptr, N), *s = B.get(N);
-    double *t = B.get(N), *ph = B.get(N), *sh = B.get(N), *dx = B.up(nullptr, N), *invd = B.get(N);
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
     const int nb = (int)(gr.x * gr.y);
-    double *part = B.get((size_t)nb + 8), *dsc = B.get(8);
-    NEED(x && qd && r && r0 && pv && vv && s && t && ph && sh && dx && invd && part && dsc);
     const int center = (R == 2) ? 12 : 4;
-    k_invdiag<<<gr, bl>>>(n0, n1, center, A, invd);
+    CFD_HIP(hipMemsetAsync(w.pv, 0, N * sizeof(double), st));
+    CFD_HIP(hipMemsetAsync(w.vv, 0, N * sizeof(double), st));
+    CFD_HIP(hipMemsetAsync(w.dx, 0, N * sizeof(double), st));
+    k_invdiag<<<gr, bl, 0, st>>>(n0, n1, center, A, w.invd);
     CFD_LAUNCH_CHECK();
     auto dot = [&](const double *a, const double *b, double *out) -> int {
-        k_dot_part<<<gr, bl>>>(n0, n1, a, b, part);
-        k_sum_final<<<1, 256>>>(part, nb, dsc);
+        k_dot_part<<<gr, bl, 0, st>>>(n0, n1, a, b, w.part);
+        k_sum_final<<<1, 256, 0, st>>>(w.part, nb, w.dsc);
         CFD_LAUNCH_CHECK();
-        CFD_HIP(hipMemcpy(out, dsc, sizeof(double), hipMemcpyDeviceToHost));
+        CFD_HIP(hipMemcpyAsync(out, w.dsc, sizeof(double), hipMemcpyDeviceToHost, st));
+        CFD_HIP(hipStreamSynchronize(st));
         return CFD_OK;
     };
     const int lb = 256;
     const int lg = (int)((N + lb - 1) / lb);
+    double *r = w.r, *r0 = w.r0, *pv = w.pv, *vv = w.vv, *s = w.s, *t = w.t, *ph = w.ph, *sh = w.sh, *dx = w.dx;
     // r = q - A phi (boundary rows of phi enter here), solve A dx = r
-    k_sop_apply<R><<<gr, bl>>>(n0, n1, A, x, qd, r);
+    k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, x, qd, r);
     CFD_LAUNCH_CHECK();
-    CFD_HIP(hipMemcpy(r0, r, N * sizeof(double), hipMemcpyDeviceToDevice));
+    CFD_HIP(hipMemcpyAsync(r0, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
     double rr0 = 0.0;
     int rc = dot(r, r, &rr0);
     if (rc) return rc;
@@ -566,36 +730,36 @@ int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double
         for (it = 1; it <= maxit; it++) {
             if (rho == 0.0) break;
             if (it == 1) {
-                CFD_HIP(hipMemcpy(pv, r, N * sizeof(double), hipMemcpyDeviceToDevice));
+                CFD_HIP(hipMemcpyAsync(pv, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
             } else {
                 const double beta = (rho / rho_old) * (alpha / omega);
                 // pv = r + beta (pv - omega vv)
-                k_lin3<<<lg, lb>>>(N, pv, pv, -omega, vv, 0.0, nullptr);
-                k_lin3<<<lg, lb>>>(N, pv, r, beta, pv, 0.0, nullptr);
+                k_lin3<<<lg, lb, 0, st>>>(N, pv, pv, -omega, vv, 0.0, nullptr);
+                k_lin3<<<lg, lb, 0, st>>>(N, pv, r, beta, pv, 0.0, nullptr);
             }
-            k_jac<<<lg, lb>>>(N, pv, invd, ph);
-            k_sop_apply<R><<<gr, bl>>>(n0, n1, A, ph, nullptr, vv);
+            k_jac<<<lg, lb, 0, st>>>(N, pv, w.invd, ph);
+            k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, ph, nullptr, vv);
             CFD_LAUNCH_CHECK();
             double rv = 0.0;
             if ((rc = dot(r0, vv, &rv))) return rc;
             alpha = rho / rv;
-            k_lin3<<<lg, lb>>>(N, s, r, -alpha, vv, 0.0, nullptr);
+            k_lin3<<<lg, lb, 0, st>>>(N, s, r, -alpha, vv, 0.0, nullptr);
             double ss = 0.0;
             if ((rc = dot(s, s, &ss))) return rc;
             if (std::sqrt(ss) <= tol) {
-                k_lin3<<<lg, lb>>>(N, dx, dx, alpha, ph, 0.0, nullptr);
+                k_lin3<<<lg, lb, 0, st>>>(N, dx, dx, alpha, ph, 0.0, nullptr);
                 rn = std::sqrt(ss);
                 break;
             }
-            k_jac<<<lg, lb>>>(N, s, invd, sh);
-            k_sop_apply<R><<<gr, bl>>>(n0, n1, A, sh, nullptr, t);
+            k_jac<<<lg, lb, 0, st>>>(N, s, w.invd, sh);
+            k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, sh, nullptr, t);
             CFD_LAUNCH_CHECK();
             double ts = 0.0, tt = 0.0;
             if ((rc = dot(t, s, &ts))) return rc;
             if ((rc = dot(t, t, &tt))) return rc;
             omega = tt > 0.0 ? ts / tt : 0.0;
-            k_lin3<<<lg, lb>>>(N, dx, dx, alpha, ph, omega, sh);
-            k_lin3<<<lg, lb>>>(N, r, s, -omega, t, 0.0, nullptr);
+            k_lin3<<<lg, lb, 0, st>>>(N, dx, dx, alpha, ph, omega, sh);
+            k_lin3<<<lg, lb, 0, st>>>(N, r, s, -omega, t, 0.0, nullptr);
             double rr = 0.0;
             if ((rc = dot(r, r, &rr))) return rc;
             rn = std::sqrt(rr);
@@ -604,13 +768,188 @@ int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double
             if ((rc = dot(r0, r, &rho))) return rc;
         }
         if (it > maxit) it = maxit;
-        k_lin3<<<lg, lb>>>(N, x, x, 1.0, dx, 0.0, nullptr);
-        k_dupcol<<<(n1 + 255) / 256, 256>>>(n0, n1, x);
+        k_lin3<<<lg, lb, 0, st>>>(N, x, x, 1.0, dx, 0.0, nullptr);
+        k_dupcol<<<(n1 + 255) / 256, 256, 0, st>>>(n0, n1, x);
         CFD_LAUNCH_CHECK();
     }
     if (iters) *iters = it;
     if (relres) *relres = r0n > 0.0 ? rn / r0n : 0.0;
+    return CFD_OK;
+}
+
+// The host-array form: uploads the planes, phi and q, solves, downloads phi.
+template <int R>
+int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double *phi, const double *q,
+              double rtol, int maxit, int *iters, double *relres) {
+    if (bad_grid(n0, n1) || !phi || !q) return CFD_E_ARG;
+    for (int k = 0; k < npts; k++)
+        if (!planes_host[k]) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *A = B.get((size_t)npts * N);
+    NEED(A);
+    for (int k = 0; k < npts; k++)
+        CFD_HIP(hipMemcpy(A + (size_t)k * N, planes_host[k], N * sizeof(double), hipMemcpyHostToDevice));
+    double *x = B.up(phi, N), *qd = B.up(q, N), *ws = B.get(SopWork::doubles(n0, n1));
+    NEED(x && qd && ws);
+    CFD_RET(sop_core<R>(n0, n1, A, x, qd, SopWork::carve(ws, n0, n1), rtol, maxit, iters, relres, nullptr));
+    CFD_HIP(hipDeviceSynchronize());
     return down(phi, x, N);
+}
+
+// ---------------------------------------------------------------- device-resident time steps
+// Local accelerations of the wall row (solver.cpp:1715-1718): a_s = accn_amp sin(2 pi F t).
+__global__ void k_alc(int n0, const double *__restrict__ x, const double *__restrict__ y, double a_s,
+                      double *__restrict__ alc_u, double *__restrict__ alc_v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n0) return;
+    alc_u[i] = a_s * y[i];
+    alc_v[i] = -a_s * x[i];
+}
+
+// One reference O-grid case resident in HBM: grid, assembled operators, state and
+// every stage array of solver.cpp's time step, in one allocation.
+struct RefCase {
+    int n0 = 0, n1 = 0;
+    double dxi = 0, deta = 0;
+    cfd_ref_params prm{};
+    hipStream_t s = nullptr;
+    double *mem = nullptr;
+    double *x, *y, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *ajac, *p1, *q1, *xnox, *xnoy;
+    double *AU, *AT, *BU, *BT, *A9;
+    double *u, *v, *T, *p, *up, *vp, *us, *vs, *uold, *vold, *Told, *uxi, *uet, *qu, *qv, *qt, *qup, *qvp, *q,
+        *pcor, *dil, *sol, *alc, *part, *scal, *ws;
+    double *hscal = nullptr;  // pinned: circulation, dmax
+    bool have_state = false;
+};
+
+size_t ref_case_doubles(int n0, int n1) {
+    const size_t N = (size_t)n0 * n1;
+    const dim3 gr = grid2d(n0, n1);
+    return 12 * N + 2 * (size_t)n0 + 59 * N + 12 * (size_t)n0 + 22 * N + 2 * (size_t)n0 + (size_t)gr.x * gr.y + 8 +
+           SopWork::doubles(n0, n1) + 64;
+}
+
+void ref_case_carve(RefCase *c) {
+    const size_t N = (size_t)c->n0 * c->n1, m = (size_t)c->n0;
+    double *b = c->mem;
+    auto take = [&](size_t n) {
+        double *r = b;
+        b += n;
+        return r;
+    };
+    double **grid[12] = {&c->x, &c->y, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma, &c->ajac,
+                         &c->p1, &c->q1};
+    for (double **g : grid) *g = take(N);
+    c->xnox = take(m);
+    c->xnoy = take(m);
+    c->AU = take(25 * N);
+    c->AT = take(25 * N);
+    c->A9 = take(9 * N);
+    c->BU = take(6 * m);
+    c->BT = take(6 * m);
+    double **st[22] = {&c->u,  &c->v,  &c->T,   &c->p,   &c->up,  &c->vp,  &c->us, &c->vs,   &c->uold, &c->vold, &c->Told,
+                       &c->uxi, &c->uet, &c->qu, &c->qv, &c->qt, &c->qup, &c->qvp, &c->q, &c->pcor, &c->dil, &c->sol};
+    for (double **f : st) *f = take(N);
+    c->alc = take(2 * m);
+    const dim3 gr = grid2d(c->n0, c->n1);
+    c->part = take((size_t)gr.x * gr.y);
+    c->scal = take(8);
+    c->ws = take(SopWork::doubles(c->n0, c->n1));
+}
+
+// sol <- its initial value, solve A sol = rhs, then rows 1..n1-2 of sol into dst
+// (column n0-1 already duplicates column 0)
+template <int R>
+int ref_solve_rows(RefCase *c, const double *A, const double *rhs, double *dst, double rtol, int maxit, int *it,
+                   double *rr) {
+    const size_t N = (size_t)c->n0 * c->n1;
+    CFD_RET(sop_core<R>(c->n0, c->n1, A, c->sol, rhs, SopWork::carve(c->ws, c->n0, c->n1), rtol, maxit, it, rr,
+                        c->s));
+    CFD_HIP(hipMemcpyAsync(dst + c->n0, c->sol + c->n0, (N - 2 * (size_t)c->n0) * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->s));
+    return CFD_OK;
+}
+
+// One time step: solver.cpp:1057-1852 in order, the Gauss-Seidel and SIP solves
+// replaced by the BiCGStab above at the case's tolerances.
+int ref_case_step(RefCase *c, cfd_ref_stats *st) {
+    const int n0 = c->n0, n1 = c->n1;
+    const size_t N = (size_t)n0 * n1, B = N * sizeof(double);
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY), g1((n0 + 255) / 256);
+    const int nb = (int)(gr.x * gr.y);
+    const cfd_ref_params &P = c->prm;
+    hipStream_t s = c->s;
+    cfd_ref_stats o{};
+    // 1062-1068: contravariant velocities and old T; 1074-1236: right-hand sides
+    k_contra<<<gr, bl, 0, s>>>(n0, n1, c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->uxi, c->uet);
+    CFD_HIP(hipMemcpyAsync(c->Told, c->T, B, hipMemcpyDeviceToDevice, s));
+    const RhsIn ri{c->u, c->v, c->T, c->up, c->vp, c->p, c->xix, c->xiy, c->etx, c->ety,
+                   c->alph, c->gamma, c->BU, c->BT, c->uxi, c->uet};
+    const RhsOut ro{c->qu, c->qv, c->qt, c->qup, c->qvp};
+    k_rhs<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, ri, ro, P.Re, P.Pr, P.Ri, P.dt);
+    CFD_LAUNCH_CHECK();
+    // 1239-1350: u, v and T from their current values, the predictors from zero
+    // with their wall row
+    const double mt = P.mom_rtol;
+    const int mi = P.mom_maxit;
+    CFD_HIP(hipMemcpyAsync(c->sol, c->u, B, hipMemcpyDeviceToDevice, s));
+    CFD_RET(ref_solve_rows<2>(c, c->AU, c->qu, c->us, mt, mi, &o.iters[0], &o.relres[0]));
+    CFD_HIP(hipMemcpyAsync(c->sol, c->v, B, hipMemcpyDeviceToDevice, s));
+    CFD_RET(ref_solve_rows<2>(c, c->AU, c->qv, c->vs, mt, mi, &o.iters[1], &o.relres[1]));
+    CFD_HIP(hipMemcpyAsync(c->sol, c->T, B, hipMemcpyDeviceToDevice, s));
+    CFD_RET(ref_solve_rows<2>(c, c->AT, c->qt, c->T, mt, mi, &o.iters[2], &o.relres[2]));
+    double *pf[2] = {c->up, c->vp}, *pq[2] = {c->qup, c->qvp};
+    for (int k = 0; k < 2; k++) {
+        CFD_HIP(hipMemsetAsync(c->sol, 0, B, s));
+        CFD_HIP(hipMemcpyAsync(c->sol, pf[k], (size_t)n0 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        CFD_RET(ref_solve_rows<2>(c, c->AU, pq[k], pf[k], mt, mi, &o.iters[3 + k], &o.relres[3 + k]));
+    }
+    // 1351-1381, 1389-1448
+    k_outflow_up_ref<<<g1, 256, 0, s>>>(n0, n1, c->up, c->vp, c->u, c->v, c->xnox, c->xnoy, P.uinf, P.vinf);
+    k_face_div_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->up, c->vp, c->p, c->xix, c->xiy, c->etx, c->ety,
+                                     P.dt, c->q);
+    CFD_LAUNCH_CHECK();
+    // 1451-1498: p' from zero, its boundary rows
+    CFD_HIP(hipMemsetAsync(c->pcor, 0, B, s));
+    CFD_HIP(hipMemcpyAsync(c->uold, c->u, B, hipMemcpyDeviceToDevice, s));
+    CFD_HIP(hipMemcpyAsync(c->vold, c->v, B, hipMemcpyDeviceToDevice, s));
+    CFD_RET(sop_core<1>(n0, n1, c->A9, c->pcor, c->q, SopWork::carve(c->ws, n0, n1), P.p_rtol, P.p_maxit,
+                        &o.iters[5], &o.relres[5], s));
+    k_pcor_bc<<<g1, 256, 0, s>>>(n0, n1, c->pcor, c->xnox, c->xnoy, P.uinf, P.vinf);
+    // 1504-1655: correction, far-field and wall velocities
+    k_correct_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->us, c->vs, c->pcor, c->xix, c->xiy, c->etx, c->ety,
+                                    P.dt, c->u, c->v, c->p);
+    const double wt = 2.0 * acos(-1.0) * P.F * P.time;
+    const FarRef fr{c->x, c->y, c->Told, c->uet, c->etx, c->ety, c->ajac, c->xnox, c->xnoy};
+    k_farfield_ref<<<1, FF_NT, 0, s>>>(n0, n1, c->deta, fr, c->u, c->v, c->T, c->up, c->vp, P.uinf, P.vinf, P.dt,
+                                       P.speed_amp, cos(wt), c->scal);
+    CFD_LAUNCH_CHECK();
+    // 1666-1782: wall and outer-row pressure
+    const double a_s = P.accn_amp * sin(wt);
+    const double *alc_u = nullptr, *alc_v = nullptr;
+    if (a_s != 0.0) {
+        k_alc<<<g1, 256, 0, s>>>(n0, c->x, c->y, a_s, c->alc, c->alc + n0);
+        alc_u = c->alc;
+        alc_v = c->alc + n0;
+    }
+    const WallRef w{c->u, c->v, c->T, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->q1, c->ajac,
+                    alc_u, alc_v};
+    k_wall_p_ref<<<g1, 256, 0, s>>>(n0, n1, c->dxi, c->deta, w, c->p, P.Re, P.Ri);
+    const ExitRef e{c->u, c->v, c->T, c->uold, c->vold, c->xix, c->xiy, c->etx,
+                    c->ety, c->alph, c->beta, c->gamma, c->q1, c->ajac, c->xnox, c->xnoy};
+    k_exit_p_ref<<<g1, 256, 0, s>>>(n0, n1, c->dxi, c->deta, e, c->p, P.uinf, P.vinf, P.dt, P.Re, P.Ri);
+    // 1806-1852: dilation and its maximum
+    k_dil_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->dil, c->part);
+    k_max_final<<<1, 256, 0, s>>>(c->part, nb, c->scal + 1);
+    CFD_LAUNCH_CHECK();
+    CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CFD_HIP(hipStreamSynchronize(s));
+    o.circ = c->hscal[0];
+    o.dmax = c->hscal[1];
+    if (st) *st = o;
+    if (!std::isfinite(o.dmax)) return CFD_E_NAN;
+    return CFD_OK;
 }
 
 }  // namespace
@@ -762,6 +1101,68 @@ int cfd_ref_wall_pressure(int n0, int n1, double dxi, double deta, const double 
     return down(p, pp, N);
 }
 
+int cfd_ref_outflow_up(int n0, int n1, double *up, double *vp, const double *u, const double *v,
+                       const double *xnox, const double *xnoy, double uinf, double vinf) {
+    if (bad_grid(n0, n1) || !up || !vp || !u || !v || !xnox || !xnoy) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(up, N), *b = B.up(vp, N), *c = B.up(u, N), *d = B.up(v, N), *e = B.up(xnox, n0),
+           *f = B.up(xnoy, n0);
+    NEED(a && b && c && d && e && f);
+    k_outflow_up_ref<<<(n0 + 255) / 256, 256>>>(n0, n1, a, b, c, d, e, f, uinf, vinf);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(up, a, N));
+    return down(vp, b, N);
+}
+
+int cfd_ref_farfield(int n0, int n1, double deta, const double *x, const double *y, double *u, double *v, double *T,
+                     double *up, double *vp, const double *Told, const double *uet, const double *etx,
+                     const double *ety, const double *ajac, const double *xnox, const double *xnoy, double uinf,
+                     double vinf, double dt, double speed_amp, double F, double time, double *circ) {
+    if (bad_grid(n0, n1) || !x || !y || !u || !v || !T || !up || !vp || !Told || !uet || !etx || !ety || !ajac ||
+        !xnox || !xnoy)
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    const FarRef a{B.up(x, N), B.up(y, N), B.up(Told, N), B.up(uet, N), B.up(etx, N),
+                   B.up(ety, N), B.up(ajac, N), B.up(xnox, n0), B.up(xnoy, n0)};
+    double *uu = B.up(u, N), *vv = B.up(v, N), *tt = B.up(T, N), *pu = B.up(up, N), *pv = B.up(vp, N),
+           *ci = B.get(1);
+    NEED(a.x && a.y && a.Told && a.uet && a.etx && a.ety && a.ajac && a.xnox && a.xnoy && uu && vv && tt && pu &&
+         pv && ci);
+    const double cwt = cos(2.0 * acos(-1.0) * F * time);
+    k_farfield_ref<<<1, FF_NT>>>(n0, n1, deta, a, uu, vv, tt, pu, pv, uinf, vinf, dt, speed_amp, cwt, ci);
+    CFD_LAUNCH_CHECK();
+    CFD_RET(down(u, uu, N));
+    CFD_RET(down(v, vv, N));
+    CFD_RET(down(T, tt, N));
+    CFD_RET(down(up, pu, N));
+    CFD_RET(down(vp, pv, N));
+    return down(circ, ci, 1);
+}
+
+int cfd_ref_exit_pressure(int n0, int n1, double dxi, double deta, const double *u, const double *v,
+                          const double *T, const double *uold, const double *vold, double *p, const double *xix,
+                          const double *xiy, const double *etx, const double *ety, const double *alph,
+                          const double *beta, const double *gamma, const double *q1, const double *ajac,
+                          const double *xnox, const double *xnoy, double uinf, double vinf, double dt, double Re,
+                          double Ri) {
+    if (bad_grid(n0, n1) || !u || !v || !T || !uold || !vold || !p || !xix || !xiy || !etx || !ety || !alph ||
+        !beta || !gamma || !ajac || !xnox || !xnoy)
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    const ExitRef w{B.up(u, N),     B.up(v, N),     B.up(T, N),     B.up(uold, N),  B.up(vold, N),  B.up(xix, N),
+                    B.up(xiy, N),   B.up(etx, N),   B.up(ety, N),   B.up(alph, N),  B.up(beta, N),  B.up(gamma, N),
+                    B.up(q1, N),    B.up(ajac, N),  B.up(xnox, n0), B.up(xnoy, n0)};
+    double *pp = B.up(p, N);
+    NEED(w.u && w.v && w.T && w.uold && w.vold && w.xix && w.xiy && w.etx && w.ety && w.alph && w.beta && w.gamma &&
+         w.q1 && w.ajac && w.xnox && w.xnoy && pp);
+    k_exit_p_ref<<<(n0 + 255) / 256, 256>>>(n0, n1, dxi, deta, w, pp, uinf, vinf, dt, Re, Ri);
+    CFD_LAUNCH_CHECK();
+    return down(p, pp, N);
+}
+
 int cfd_ref_dilation(int n0, int n1, double dxi, double deta, const double *u, const double *v, const double *xix,
                      const double *xiy, const double *etx, const double *ety, double *dil, double *dmax) {
     if (bad_grid(n0, n1) || !u || !v || !xix || !xiy || !etx || !ety) return CFD_E_ARG;
@@ -791,6 +1192,99 @@ int cfd_ref_nusselt(int n0, int n1, double dxi, double deta, const double *T, co
     CFD_LAUNCH_CHECK();
     CFD_RET(down(nu, o, 1));
     return down(dthdn, prof, (size_t)n0);
+}
+
+int cfd_ref_case_create(int n0, int n1, double dxi, double deta, const double *x, const double *y, const double *xix,
+                        const double *xiy, const double *etx, const double *ety, const double *alph,
+                        const double *beta, const double *gamma, const double *ajac, const double *p1,
+                        const double *q1, const double *xnox, const double *xnoy, const cfd_ref_params *prm,
+                        cfd_ref_case *out) {
+    if (!out) return CFD_E_ARG;
+    *out = nullptr;
+    if (bad_grid(n0, n1) || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac ||
+        !xnox || !xnoy || !prm || !(prm->dt > 0) || !(prm->Re > 0) || !(prm->Pr > 0) || !(prm->mom_rtol > 0) ||
+        !(prm->p_rtol > 0) || prm->mom_maxit < 1 || prm->p_maxit < 1)
+        return CFD_E_ARG;
+    RefCase *c = new RefCase;
+    c->n0 = n0;
+    c->n1 = n1;
+    c->dxi = dxi;
+    c->deta = deta;
+    c->prm = *prm;
+    const size_t nd = ref_case_doubles(n0, n1), N = (size_t)n0 * n1;
+    auto fail = [&](int rc) {
+        cfd_ref_case_destroy(reinterpret_cast<cfd_ref_case>(c));
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(CFD_E_HIP);
+    if (hipMalloc(&c->mem, nd * sizeof(double)) != hipSuccess) return fail(CFD_E_OOM);
+    if (hipHostMalloc(&c->hscal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(CFD_E_OOM);
+    ref_case_carve(c);
+    if (hipMemsetAsync(c->mem, 0, nd * sizeof(double), c->s) != hipSuccess) return fail(CFD_E_HIP);
+    const double *src[12] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1};
+    double *dst[12] = {c->x, c->y, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->ajac, c->p1, c->q1};
+    for (int k = 0; k < 12; k++)
+        if (src[k] && hipMemcpyAsync(dst[k], src[k], N * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
+            return fail(CFD_E_HIP);
+    if (hipMemcpyAsync(c->xnox, xnox, n0 * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess ||
+        hipMemcpyAsync(c->xnoy, xnoy, n0 * sizeof(double), hipMemcpyHostToDevice, c->s) != hipSuccess)
+        return fail(CFD_E_HIP);
+    // operators of the reference's constructor (solver.cpp:751-1039); the temperature
+    // solve passes the (-1,+1) plane in the (-1,+2) slot (solver.cpp:1289)
+    const dim3 gr = grid2d(n0, n1), bl(BX, BY);
+    k_asm_mom<<<gr, bl, 0, c->s>>>(n0, n1, dxi, deta, c->alph, c->beta, c->gamma, c->p1, c->q1, prm->dt, prm->Re,
+                                   prm->Pr, c->AU, c->AT, c->BU, c->BT);
+    k_asm_p9<<<gr, bl, 0, c->s>>>(n0, n1, dxi, deta, c->xix, c->xiy, c->etx, c->ety, c->A9);
+    if (hipGetLastError() != hipSuccess) return fail(CFD_E_HIP);
+    if (hipMemcpyAsync(c->AT + 21 * N, c->AT + 16 * N, N * sizeof(double), hipMemcpyDeviceToDevice, c->s) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->s) != hipSuccess)
+        return fail(CFD_E_HIP);
+    *out = reinterpret_cast<cfd_ref_case>(c);
+    return CFD_OK;
+}
+
+int cfd_ref_case_set_state(cfd_ref_case h, const double *u, const double *v, const double *T, const double *p,
+                           const double *up, const double *vp) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c || !u || !v || !T || !p || !up || !vp) return CFD_E_ARG;
+    const size_t B = (size_t)c->n0 * c->n1 * sizeof(double);
+    const double *src[6] = {u, v, T, p, up, vp};
+    double *dst[6] = {c->u, c->v, c->T, c->p, c->up, c->vp};
+    for (int k = 0; k < 6; k++) CFD_HIP(hipMemcpyAsync(dst[k], src[k], B, hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    c->have_state = true;
+    return CFD_OK;
+}
+
+int cfd_ref_case_get_state(cfd_ref_case h, double *u, double *v, double *T, double *p, double *up, double *vp) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c) return CFD_E_ARG;
+    const size_t B = (size_t)c->n0 * c->n1 * sizeof(double);
+    double *dst[6] = {u, v, T, p, up, vp};
+    const double *src[6] = {c->u, c->v, c->T, c->p, c->up, c->vp};
+    for (int k = 0; k < 6; k++)
+        if (dst[k]) CFD_HIP(hipMemcpyAsync(dst[k], src[k], B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c || nsteps < 0) return CFD_E_ARG;
+    if (!c->have_state) return CFD_E_STATE;
+    for (int k = 0; k < nsteps; k++) CFD_RET(ref_case_step(c, last));
+    return CFD_OK;
+}
+
+void cfd_ref_case_destroy(cfd_ref_case h) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c) return;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    if (c->mem) (void)hipFree(c->mem);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
 }
 
 const char *cfd_version(void) { return "cfd_gpu 0.1 (gfx950)"; }

@@ -1,0 +1,71 @@
+"""The reference solver's O-grid time step, resident on the GPU (cfd_ref_case_*).
+
+`RefCase` holds one O-grid case in HBM: the grid and metric arrays of INP.DAT
+(solver.cpp:516-553), the operators the reference's constructor assembles
+(solver.cpp:751-1039) and the state u, v, T, p, up, vp.  `step()` runs
+solver.cpp:1057-1852 -- every stage of the reference's time step in its order --
+with the Gauss-Seidel and SIP solves replaced by BiCGStab to the tolerances in
+the parameters (same linear systems, same fixed points).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import RefParams, RefStats, call, f64
+
+
+def ref_params(**kw) -> RefParams:
+    """The reference's compiled-in constants (solver.cpp:159-173) plus solver tolerances."""
+    a = 82.0 * math.pi / 180.0
+    base = dict(Re=1000.0, Pr=0.71, Ri=0.0, dt=1e-4, uinf=math.sin(a), vinf=math.cos(a), speed_amp=0.0,
+                accn_amp=0.0, F=0.0, time=0.0, mom_rtol=1e-12, p_rtol=1e-12, mom_maxit=500, p_maxit=2000)
+    base.update(kw)
+    return RefParams(**base)
+
+
+class RefCase:
+    def __init__(self, grid, params: RefParams | None = None, **kw):
+        """grid: an O-grid with n0, n1, dxi, deta, x, y, xix, xiy, etx, ety, alph, beta,
+        gamma, ajac, xnox, xnoy (and optionally p1, q1) -- e.g. grid.ogrid_ellipse()."""
+        _lib.load()
+        self.n0, self.n1 = int(grid.n0), int(grid.n1)
+        self.params = params if params is not None else ref_params(**kw)
+        g = grid
+        self._keep = [f64(a) for a in (g.x, g.y, g.xix, g.xiy, g.etx, g.ety, g.alph, g.beta, g.gamma, g.ajac)]
+        opt = [f64(getattr(g, k)) if getattr(g, k, None) is not None else None for k in ("p1", "q1")]
+        self._keep += [a for a in opt if a is not None]
+        nox, noy = f64(g.xnox), f64(g.xnoy)
+        self._keep += [nox, noy]
+        h = C.c_void_p()
+        call("cfd_ref_case_create", self.n0, self.n1, float(g.dxi), float(g.deta), *self._keep[:10],
+             opt[0].ctypes.data if opt[0] is not None else None, opt[1].ctypes.data if opt[1] is not None else None,
+             nox, noy, C.byref(self.params), C.byref(h))
+        self.h = h
+
+    def set_state(self, u, v, T, p, up, vp):
+        call("cfd_ref_case_set_state", self.h, *[f64(a) for a in (u, v, T, p, up, vp)])
+
+    def state(self):
+        out = [np.zeros((self.n1, self.n0)) for _ in range(6)]
+        call("cfd_ref_case_get_state", self.h, *out)
+        return dict(zip(("u", "v", "T", "p", "up", "vp"), out))
+
+    def step(self, nsteps=1):
+        st = RefStats()
+        call("cfd_ref_case_step", self.h, int(nsteps), C.byref(st))
+        return dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().cfd_ref_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

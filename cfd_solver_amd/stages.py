@@ -10,6 +10,8 @@ the C ABI (include/cfd_gpu.h).  Method names follow the reference:
   pcor_bc        solver.cpp:1474-1497    correct      solver.cpp:1504-1541
   wall_pressure  solver.cpp:1667-1728    dilation     solver.cpp:1806-1852
   nusselt        solver.cpp:1951-1982, 2031-2038
+  outflow_up     solver.cpp:1351-1381    farfield     solver.cpp:1543-1655
+  exit_pressure  solver.cpp:1731-1782
 
 Like the reference, `sip9p` and `gauss` update `phi` in place and take the
 right-hand side `q`; unlike it they return (iterations, relative residual)
@@ -119,6 +121,28 @@ class RefStages:
              p_, f64(xix), f64(xiy), f64(etx), f64(ety), f64(alph), f64(beta), f64(gamma), f64(q1),
              f64(ajac), au.ctypes.data if au is not None else None,
              av.ctypes.data if av is not None else None, Re, Ri)
+        return p_
+
+    def outflow_up(self, up, vp, u, v, xnox, xnoy, uinf, vinf):
+        up_, vp_ = f64(up).copy(), f64(vp).copy()
+        call("cfd_ref_outflow_up", self.n0, self.n1, up_, vp_, f64(u), f64(v), f64(xnox), f64(xnoy), uinf, vinf)
+        return up_, vp_
+
+    def farfield(self, x, y, u, v, T, up, vp, Told, uet, etx, ety, ajac, xnox, xnoy, uinf, vinf, dt,
+                 speed_amp=0.0, F=0.0, time=0.0):
+        """Returns (u, v, T, up, vp, circulation)."""
+        o = [f64(a).copy() for a in (u, v, T, up, vp)]
+        circ = C.c_double()
+        call("cfd_ref_farfield", self.n0, self.n1, self.deta, f64(x), f64(y), *o, f64(Told), f64(uet), f64(etx),
+             f64(ety), f64(ajac), f64(xnox), f64(xnoy), uinf, vinf, dt, speed_amp, F, time, C.byref(circ))
+        return (*o, circ.value)
+
+    def exit_pressure(self, u, v, T, uold, vold, p, xix, xiy, etx, ety, alph, beta, gamma, q1, ajac, xnox, xnoy,
+                      uinf, vinf, dt, Re, Ri):
+        p_ = f64(p).copy()
+        call("cfd_ref_exit_pressure", self.n0, self.n1, self.dxi, self.deta, f64(u), f64(v), f64(T), f64(uold),
+             f64(vold), p_, f64(xix), f64(xiy), f64(etx), f64(ety), f64(alph), f64(beta), f64(gamma), f64(q1),
+             f64(ajac), f64(xnox), f64(xnoy), uinf, vinf, dt, Re, Ri)
         return p_
 
     def dilation(self, u, v, xix, xiy, etx, ety):

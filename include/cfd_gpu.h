@@ -106,6 +106,12 @@ int cfd_sip9p(int n0, int n1,
               const double *anw, double *phi, const double *q,
               double rtol, int maxit, int *iters, double *relres);
 
+/* Predictor velocities up/vp on the outer row j = n1-1: u/v where the free
+ * stream enters (uinf*xnox + vinf*xnoy >= 0), else quadratic extrapolation from
+ * the three rows inside.  Replaces solver.cpp:1351-1381. */
+int cfd_ref_outflow_up(int n0, int n1, double *up, double *vp, const double *u, const double *v,
+                       const double *xnox, const double *xnoy, double uinf, double vinf);
+
 /* p' boundary rows after the solve: wall copy and far-field 0/copy by the
  * sign of the free-stream normal velocity.  Replaces solver.cpp:1474-1497. */
 int cfd_ref_pcor_bc(int n0, int n1, double *pcor, const double *xnox, const double *xnoy,
@@ -127,6 +133,32 @@ int cfd_ref_wall_pressure(int n0, int n1, double dxi, double deta,
                           const double *q1, const double *ajac,
                           const double *alc_u, const double *alc_v, double Re, double Ri);
 
+/* Boundary velocities after the correction: on the outer row the free stream
+ * (T = 0) where it enters, where it leaves the far-field prediction from row
+ * n1-2's radial/tangential velocity (tangential decay exponent 1 when the
+ * circulation along row n1-2 exceeds 1e-2 in magnitude, else 2) with T convected
+ * from Told by uet (both the values at the start of the step); on the wall row
+ * the oscillating cylinder's velocity (-a y, a x), a = speed_amp cos(2 pi F time),
+ * copied into up/vp.  circ (may be NULL) receives the circulation.
+ * Replaces solver.cpp:1543-1655. */
+int cfd_ref_farfield(int n0, int n1, double deta, const double *x, const double *y,
+                     double *u, double *v, double *T, double *up, double *vp,
+                     const double *Told, const double *uet, const double *etx, const double *ety,
+                     const double *ajac, const double *xnox, const double *xnoy,
+                     double uinf, double vinf, double dt, double speed_amp, double F, double time,
+                     double *circ);
+
+/* Outer-row (j = n1-1) pressure: normal momentum equation with local
+ * acceleration (u - uold)/dt where the free stream enters, Gresho's condition
+ * where it leaves.  q1 may be NULL (zero).  Replaces solver.cpp:1731-1782. */
+int cfd_ref_exit_pressure(int n0, int n1, double dxi, double deta,
+                          const double *u, const double *v, const double *T,
+                          const double *uold, const double *vold, double *p,
+                          const double *xix, const double *xiy, const double *etx, const double *ety,
+                          const double *alph, const double *beta, const double *gamma,
+                          const double *q1, const double *ajac, const double *xnox, const double *xnoy,
+                          double uinf, double vinf, double dt, double Re, double Ri);
+
 /* Dilation field and its signed maximum floored at 0 (the mass-imbalance
  * monitor).  Replaces solver.cpp:1806-1852. */
 int cfd_ref_dilation(int n0, int n1, double dxi, double deta,
@@ -140,6 +172,50 @@ int cfd_ref_dilation(int n0, int n1, double dxi, double deta,
 int cfd_ref_nusselt(int n0, int n1, double dxi, double deta, const double *T,
                     const double *ajac, const double *etx, const double *ety,
                     double ar, double *nu, double *dthdn);
+
+/* Reference-mode time steps resident on the device: the reference's O-grid
+ * case (its constructor's operators, solver.cpp:751-1039, assembled once) and
+ * its whole time step, solver.cpp:1057-1852, stage for stage in the reference's
+ * order -- right-hand sides, the u/v/T and predictor solves, outer-row
+ * predictors, face divergence, p' solve and boundary rows, correction, far-field
+ * and wall velocities, wall and outer-row pressure, dilation.  The reference's
+ * Gauss-Seidel (solver.cpp:2279) and SIP (solver.cpp:2066) solves, which stop at
+ * 0.75e-2 of the first L1 residual, are replaced by BiCGStab to the relative
+ * 2-norm tolerances below: same systems, same fixed points.  time is held fixed
+ * (the reference declares it int, so time += dt leaves it at 0). */
+typedef struct {
+    double Re, Pr, Ri, dt;           /* the reference's defaults: 1000, 0.71, 0, 1e-4      */
+    double uinf, vinf;               /* free stream (sin 82 deg, cos 82 deg)              */
+    double speed_amp, accn_amp, F, time;  /* cylinder oscillation (0 when F = 0)          */
+    double mom_rtol, p_rtol;         /* BiCGStab tolerances of the u/v/T/predictor and p' solves */
+    int mom_maxit, p_maxit;
+} cfd_ref_params;
+
+typedef struct {
+    double dmax;       /* the dilation maximum the reference prints per step   */
+    double circ;       /* circulation along row n1-2 (far-field switch)        */
+    int iters[6];      /* BiCGStab iterations: u, v, T, up, vp, p'             */
+    double relres[6];  /* their final relative residuals                      */
+} cfd_ref_stats;
+
+typedef struct cfd_ref_case_s *cfd_ref_case;
+
+/* The O-grid of INP.DAT (solver.cpp:516-553): n0 x n1 nodes, x/y and metric
+ * arrays, p1/q1 (NULL: zero), outer-boundary normals xnox/xnoy (n0). */
+int cfd_ref_case_create(int n0, int n1, double dxi, double deta,
+                        const double *x, const double *y,
+                        const double *xix, const double *xiy, const double *etx, const double *ety,
+                        const double *alph, const double *beta, const double *gamma, const double *ajac,
+                        const double *p1, const double *q1, const double *xnox, const double *xnoy,
+                        const cfd_ref_params *prm, cfd_ref_case *out);
+/* State at the end of a step: u, v, T, p and the predictor velocities up, vp. */
+int cfd_ref_case_set_state(cfd_ref_case h, const double *u, const double *v, const double *T,
+                           const double *p, const double *up, const double *vp);
+int cfd_ref_case_get_state(cfd_ref_case h, double *u, double *v, double *T, double *p,
+                           double *up, double *vp);
+/* nsteps time steps; last (may be NULL) receives the last step's statistics. */
+int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last);
+void cfd_ref_case_destroy(cfd_ref_case h);
 
 /* ------------------------------------------------------------------ */
 /* 2. Cavity cases                                                     */

@@ -77,6 +77,11 @@ void or_face_div_ref(int n0, int n1, double h1, double h2,
 int or_sip9p_ref(int n0, int n1, const double *A9, double *phi, const double *q,
                  double tol, int maxit, double alp, double *last_ratio);
 
+/* Predictor velocities on the outer row: current velocity where the free
+ * stream enters, quadratic extrapolation where it leaves (solver.cpp:1351-1381). */
+void or_outflow_up_ref(int n0, int n1, double *up, double *vp, const double *u, const double *v,
+                       const double *xnox, const double *xnoy, double uinf, double vinf);
+
 /* p' boundary rows after the solve (solver.cpp:1474-1497). */
 void or_pcor_bc_ref(int n0, int n1, double *pcor, const double *xnox, const double *xnoy,
                     double uinf, double vinf);
@@ -87,6 +92,16 @@ void or_correct_ref(int n0, int n1, double h1, double h2,
                     const double *xix, const double *xiy, const double *etx, const double *ety,
                     double dt, double *u, double *v, double *p);
 
+/* Boundary velocities after the correction: far-field prediction from the
+ * radial/tangential velocity of row n1-2 and the circulation switch, the
+ * oscillating-cylinder wall (cos_ft = cos(2 pi F t)), convected outflow T
+ * (solver.cpp:1543-1655).  Returns the circulation along row n1-2. */
+double or_farfield_ref(int n0, int n1, double h2, const double *x, const double *y,
+                       double *u, double *v, double *T, double *up, double *vp,
+                       const double *Told, const double *uet, const double *etx, const double *ety,
+                       const double *ajac, const double *xnox, const double *xnoy,
+                       double uinf, double vinf, double dt, double speed_amp, double cos_ft);
+
 /* Wall pressure from the normal momentum equation at j = 0 (solver.cpp:1667-1728).
  * alc_u/alc_v: the local-acceleration terms per wall node (zero when F = 0). */
 void or_wall_p_ref(int n0, int n1, double h1, double h2,
@@ -95,6 +110,16 @@ void or_wall_p_ref(int n0, int n1, double h1, double h2,
                    const double *alph, const double *beta, const double *gamma,
                    const double *q1, const double *ajac,
                    const double *alc_u, const double *alc_v, double Re, double Ri);
+
+/* Outer-row pressure: normal momentum equation where the free stream enters,
+ * Gresho's condition where it leaves (solver.cpp:1731-1782). */
+void or_exit_p_ref(int n0, int n1, double h1, double h2,
+                   const double *u, const double *v, const double *T, const double *uold,
+                   const double *vold, double *p,
+                   const double *xix, const double *xiy, const double *etx, const double *ety,
+                   const double *alph, const double *beta, const double *gamma,
+                   const double *q1, const double *ajac, const double *xnox, const double *xnoy,
+                   double uinf, double vinf, double dt, double Re, double Ri);
 
 /* Dilation and its signed maximum floored at 0 (solver.cpp:1806-1852). */
 double or_dil_ref(int n0, int n1, double h1, double h2,

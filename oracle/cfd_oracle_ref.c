@@ -523,6 +523,31 @@ int or_sip9p_ref(int n0, int n1, const double *A9, double *phi, const double *q,
 }
 
 /* ------------------------------------------------------------------------ */
+/* solver.cpp:1351-1381: predictor velocities on the outer boundary j = n1-1 --
+ * the current velocity where the free stream enters (vnn >= 0), else a
+ * quadratic extrapolation from the three rows inside. */
+void or_outflow_up_ref(int n0, int n1, double *up, double *vp, const double *u, const double *v,
+                       const double *xnox, const double *xnoy, double uinf, double vinf)
+{
+    const int j = n1 - 1;
+    for (int i = 0; i < n0 - 1; i++) {
+        const double vnn = uinf * xnox[i] + vinf * xnoy[i];
+        const size_t P = ID(i, j);
+        if (vnn >= 0) {
+            up[P] = u[P];
+            vp[P] = v[P];
+        } else {
+            up[P] = (5.0 * up[ID(i, j - 1)] - 4.0 * up[ID(i, j - 2)] + up[ID(i, j - 3)]) / 2.0;
+            vp[P] = (5.0 * vp[ID(i, j - 1)] - 4.0 * vp[ID(i, j - 2)] + vp[ID(i, j - 3)]) / 2.0;
+        }
+        if (i == 0) {
+            up[ID(n0 - 1, j)] = up[P];
+            vp[ID(n0 - 1, j)] = vp[P];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* solver.cpp:1474-1497 */
 void or_pcor_bc_ref(int n0, int n1, double *pcor, const double *xnox, const double *xnoy,
                     double uinf, double vinf)
@@ -569,6 +594,88 @@ void or_correct_ref(int n0, int n1, double h1, double h2,
     }
 }
 
+/* solver.cpp:1543-1655: boundary velocities after the correction.  Radial and
+ * tangential velocity on row n1-2, the circulation along that row, the far-field
+ * prediction vr = vr0 cr^2 + vr_inf (1 - cr^2), vth = vth0 cr^k + vth_inf (1 - cr^k)
+ * with k = 1 when |circulation| > 1e-2 else 2 (cr = radius ratio of rows n1-2 and
+ * n1-1); the wall row j = 0 gets the oscillating-cylinder velocity
+ * (-a y, a x) with a = speed_amp cos_ft (cos_ft = cos(2 pi F t), evaluated by the
+ * caller), copied into up/vp; the outer row gets the free stream (T = 0) where it
+ * enters and the predicted velocity with convected T (Told, uet: the values at the
+ * start of the step) where it leaves.  Returns the circulation. */
+double or_farfield_ref(int n0, int n1, double h2, const double *x, const double *y,
+                       double *u, double *v, double *T, double *up, double *vp,
+                       const double *Told, const double *uet, const double *etx, const double *ety,
+                       const double *ajac, const double *xnox, const double *xnoy,
+                       double uinf, double vinf, double dt, double speed_amp, double cos_ft)
+{
+    double *vr0 = (double *)malloc(sizeof(double) * (size_t)n0 * 4);
+    double *vt0 = vr0 + n0, *vr1 = vr0 + 2 * n0, *vt1 = vr0 + 3 * n0;
+    int j = n1 - 2;
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t P = ID(i, j);
+        const double r = sqrt(x[P] * x[P] + y[P] * y[P]);
+        const double c = x[P] / r, s = y[P] / r;
+        vr0[i] = u[P] * c + v[P] * s;
+        vt0[i] = -u[P] * s + v[P] * c;
+    }
+    vr0[n0 - 1] = vr0[0];
+    vt0[n0 - 1] = vt0[0];
+    double circ = 0.0;
+    for (int i = 0; i < n0 - 1; i++) {
+        const double de = 1.0 / (n0 - 2);
+        const size_t A = ID(i, j), B = ID(i + 1, j);
+        const double f1 = (u[A] * ety[A] - v[A] * etx[A]) * fabs(ajac[A]);
+        const double f2 = (u[B] * ety[B] - v[B] * etx[B]) * fabs(ajac[B]);
+        circ = circ + de * 0.5 * (f1 + f2);
+    }
+    j = n1 - 1;
+    const int kk = fabs(circ) > 1e-2 ? 1 : 2;
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t P = ID(i, j), Q = ID(i, j - 1);
+        const double r = sqrt(x[P] * x[P] + y[P] * y[P]);
+        const double cr = sqrt(x[Q] * x[Q] + y[Q] * y[Q]) / r;
+        const double c = x[P] / r, s = y[P] / r;
+        const double vrinf = uinf * c + vinf * s, vtinf = -uinf * s + vinf * c;
+        vr1[i] = vr0[i] * pow(cr, 2.0) + vrinf * (1 - pow(cr, 2.0));
+        vt1[i] = vt0[i] * pow(cr, (double)kk) + vtinf * (1 - pow(cr, (double)kk));
+    }
+    vr1[n0 - 1] = vr1[0];
+    vt1[n0 - 1] = vt1[0];
+    for (int i = 0; i < n0; i++) {
+        const size_t P = ID(i, 0);
+        u[P] = -speed_amp * cos_ft * y[P];
+        up[P] = u[P];
+    }
+    for (int i = 0; i < n0; i++) {
+        const size_t P = ID(i, 0);
+        v[P] = speed_amp * cos_ft * x[P];
+        vp[P] = v[P];
+    }
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t P = ID(i, j);
+        const double vnn = uinf * xnox[i] + vinf * xnoy[i];
+        if (vnn >= 0) {
+            u[P] = uinf;
+            v[P] = vinf;
+            T[P] = 0.0;
+        } else {
+            const double r = sqrt(x[P] * x[P] + y[P] * y[P]);
+            const double c = x[P] / r, s = y[P] / r;
+            u[P] = c * vr1[i] - s * vt1[i];
+            v[P] = s * vr1[i] + c * vt1[i];
+            T[P] = Told[P] - (uet[P] * dt / h2) * (Told[P] - Told[ID(i, j - 1)]);
+        }
+        if (i == 0) {
+            u[ID(n0 - 1, j)] = u[ID(0, j)];
+            v[ID(n0 - 1, j)] = v[ID(0, j)];
+            T[ID(n0 - 1, j)] = T[ID(0, j)];
+        }
+    }
+    free(vr0);
+    return circ;
+}
+
 /* solver.cpp:1667-1728 */
 void or_wall_p_ref(int n0, int n1, double h1, double h2,
                    const double *u, const double *v, const double *T, double *p,
@@ -600,6 +707,55 @@ void or_wall_p_ref(int n0, int n1, double h1, double h2,
             else dp_dy = 1.0 * d2u / Re - conv - alc_v[i] + Ri * T[P];
         }
         p[P] = p[ID(i, j + 1)] - (dp_dx * (-xiy[P] * ajac[P]) + dp_dy * (xix[P] * ajac[P])) * h2;
+        if (i == 0) p[ID(n0 - 1, j)] = p[P];
+    }
+}
+
+/* solver.cpp:1731-1782: pressure on the outer row j = n1-1 -- the normal
+ * momentum equation with one-sided eta differences where the free stream enters
+ * (local acceleration (u - uold)/dt), Gresho's condition
+ * p = (3 uet_j - 4 uet_{j-1} + uet_{j-2}) / (2 Re h2) where it leaves; uxi and uet
+ * are those of the current (boundary-updated) velocity. */
+void or_exit_p_ref(int n0, int n1, double h1, double h2,
+                   const double *u, const double *v, const double *T, const double *uold,
+                   const double *vold, double *p,
+                   const double *xix, const double *xiy, const double *etx, const double *ety,
+                   const double *alph, const double *beta, const double *gamma,
+                   const double *q1, const double *ajac, const double *xnox, const double *xnoy,
+                   double uinf, double vinf, double dt, double Re, double Ri)
+{
+    const double *fld[2] = {u, v}, *old[2] = {uold, vold};
+    const int j = n1 - 1;
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t P = ID(i, j);
+        const double vnn = uinf * xnox[i] + vinf * xnoy[i];
+        if (vnn >= 0) {
+            const int ipp = i + 1, inn = (i == 0) ? n0 - 2 : i - 1;
+            const double uxi = xix[P] * u[P] + xiy[P] * v[P];
+            const double uet = etx[P] * u[P] + ety[P] * v[P];
+            double dp_dx = 0.0, dp_dy = 0.0;
+            for (int k = 0; k < 2; k++) {
+                const double *f = fld[k];
+                const double aa = alph[P] * (f[ID(ipp, j)] + f[ID(inn, j)] - 2 * f[P]) / (h1 * h1);
+                const double gg = gamma[P] * (f[P] + f[ID(i, j - 2)] - 2 * f[ID(i, j - 1)]) / (h2 * h2);
+                const double bb = beta[P] * (f[ID(ipp, j)] + f[ID(inn, j - 1)] - f[ID(ipp, j - 1)] - f[ID(inn, j)]) /
+                                  (2 * h1 * h2);
+                const double qqq = q1[P] * (3 * f[P] - 4 * f[ID(i, j - 1)] + f[ID(i, j - 2)]) / (2 * h2);
+                const double d2u = aa + gg - 2 * bb + qqq;
+                double conv = uxi * 0.5 * (f[ID(ipp, j)] - f[ID(inn, j)]) / h1;
+                conv = conv + uet * (3.0 * f[P] - 4 * f[ID(i, j - 1)] + f[ID(i, j - 2)]) / (2 * h2);
+                const double alc = (f[P] - old[k][P]) / dt;
+                if (k == 0) dp_dx = 1.0 * d2u / Re - conv - alc;
+                else dp_dy = 1.0 * d2u / Re - conv - alc + Ri * T[P];
+            }
+            p[P] = p[ID(i, j - 1)] + (dp_dx * (-xiy[P] * ajac[P]) + dp_dy * (xix[P] * ajac[P])) * h2;
+        } else {
+            const size_t Q = ID(i, j - 1), R = ID(i, j - 2);
+            const double e0 = etx[P] * u[P] + ety[P] * v[P];
+            const double e1 = etx[Q] * u[Q] + ety[Q] * v[Q];
+            const double e2 = etx[R] * u[R] + ety[R] * v[R];
+            p[P] = 0.5 * (1.0 / Re) * ((3 * e0 - 4 * e1 + e2) / h2);
+        }
         if (i == 0) p[ID(n0 - 1, j)] = p[P];
     }
 }

@@ -43,6 +43,10 @@ def lib():
         L.or_sip9p_ref.argtypes = [_i, _i, dp, dp, dp, _d, _i, _d, C.POINTER(_d)]
         L.or_sip9p_ref.restype = _i
         L.or_pcor_bc_ref.argtypes = [_i, _i, dp, dp, dp, _d, _d]
+        L.or_outflow_up_ref.argtypes = [_i, _i, dp, dp, dp, dp, dp, dp, _d, _d]
+        L.or_farfield_ref.argtypes = [_i, _i, _d] + [dp] * 2 + [dp] * 5 + [dp] * 7 + [_d] * 5
+        L.or_farfield_ref.restype = _d
+        L.or_exit_p_ref.argtypes = [_i, _i, _d, _d] + [dp] * 6 + [dp] * 9 + [dp, dp] + [_d] * 5
         L.or_correct_ref.argtypes = [_i, _i, _d, _d] + [dp] * 7 + [_d, dp, dp, dp]
         L.or_wall_p_ref.argtypes = [_i, _i, _d, _d] + [dp] * 4 + [dp] * 9 + [dp, dp, _d, _d]
         L.or_dil_ref.argtypes = [_i, _i, _d, _d] + [dp] * 6 + [dp]
@@ -143,3 +147,92 @@ class CavityOracle:
                 self.h = None
         except Exception:
             pass
+
+
+def ref_step(g, ops, st, prm, inner_calls=1):
+    """One time step of the reference solver on its O-grid (solver.cpp:1057-1852),
+    composed from the oracle's reference-mode stages in the reference's order.
+
+    g: grid arrays x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac (n1 x n0),
+       xnox, xnoy (n0) and optionally q1 (zero when absent);
+    ops: AU, AT (25 planes), BU, BT (6 x n0), A9 (9 planes);
+    st: u, v, T, p, up, vp at the end of the previous step (not modified);
+    prm: dxi, deta, dt, Re, Pr, Ri, uinf, vinf and optionally speed_amp, accn_amp,
+         F, time (all zero by default: the reference's F = 0, and its `int time`
+         never leaves 0);
+    inner_calls: calls of the Gauss-Seidel and SIP solvers per system (1: the
+         reference's own loose solves; more: repeated calls converge them).
+    Returns the new state and the stage outputs (us, vs, q, pcor, dil, dmax, circ, ...).
+    """
+    import math
+    L = lib()
+    n1, n0 = st["u"].shape
+    h1, h2, dt = prm["dxi"], prm["deta"], prm["dt"]
+    Re, Pr, Ri, uinf, vinf = prm["Re"], prm["Pr"], prm["Ri"], prm["uinf"], prm["vinf"]
+    amp, accn = prm.get("speed_amp", 0.0), prm.get("accn_amp", 0.0)
+    wt = 2.0 * math.acos(-1.0) * prm.get("F", 0.0) * prm.get("time", 0.0)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    u, v, T, p, up, vp = (c(st[k]).copy() for k in ("u", "v", "T", "p", "up", "vp"))
+    x, y, xix, xiy, etx, ety = (c(g[k]) for k in ("x", "y", "xix", "xiy", "etx", "ety"))
+    alph, beta, gamma, ajac = (c(g[k]) for k in ("alph", "beta", "gamma", "ajac"))
+    xnox, xnoy = c(g["xnox"]), c(g["xnoy"])
+    q1 = c(g["q1"]) if "q1" in g else np.zeros((n1, n0))
+    AU, AT, BU, BT, A9 = (c(ops[k]) for k in ("AU", "AT", "BU", "BT", "A9"))
+    r = _d()
+    # 1062-1236: contravariant velocities, old T, right-hand sides
+    Told = T.copy()
+    uxi, uet, qu, qv, qt, qup, qvp = (np.zeros((n1, n0)) for _ in range(7))
+    L.or_rhs_ref(n0, n1, h1, h2, u, v, T, up, vp, p, xix, xiy, etx, ety, alph, gamma, BU, BT,
+                 Re, Pr, Ri, dt, uxi, uet, qu, qv, qt, qup, qvp)
+
+    # Repeated calls restart each solver from its last iterate; once the residual is at
+    # rounding level a call can no longer reach 0.75e-2 of its first residual (its
+    # ratio stays near 1), so the repetition stops there and later calls are capped.
+    def gauss(A, quirk, phi, q):
+        for k in range(inner_calls):
+            L.or_gauss_ref(n0, n1, A, quirk, phi, q, 0.75e-2, 100000 if k == 0 else 500, C.byref(r))
+            if k > 0 and r.value > 0.5:
+                break
+        return phi
+
+    def interior(dst, sol):  # rows 1..n1-2, column n0-1 duplicating column 0
+        dst[1:-1, :n0 - 1] = sol[1:-1, :n0 - 1]
+        dst[1:-1, n0 - 1] = sol[1:-1, 0]
+
+    # 1239-1350: u, v, T and predictor solves
+    us, vs = np.zeros((n1, n0)), np.zeros((n1, n0))
+    interior(us, gauss(AU, 0, u.copy(), qu))
+    interior(vs, gauss(AU, 0, v.copy(), qv))
+    interior(T, gauss(AT, 1, T.copy(), qt))
+    for f, q in ((up, qup), (vp, qvp)):
+        sol = np.zeros((n1, n0))
+        sol[0] = f[0]
+        interior(f, gauss(AU, 0, sol, q))
+    # 1351-1381, 1389-1448: outer-row predictor velocities, face divergence
+    L.or_outflow_up_ref(n0, n1, up, vp, u, v, xnox, xnoy, uinf, vinf)
+    q = np.zeros((n1, n0))
+    L.or_face_div_ref(n0, n1, h1, h2, up, vp, p, xix, xiy, etx, ety, dt, q)
+    # 1451-1498: p' solve and its boundary rows
+    pcor = np.zeros((n1, n0))
+    uold, vold = u.copy(), v.copy()
+    for k in range(inner_calls):
+        L.or_sip9p_ref(n0, n1, A9, pcor, q, 0.75e-2, 100000 if k == 0 else 500, 0.92, C.byref(r))
+        if k > 0 and r.value > 0.5:
+            break
+    L.or_pcor_bc_ref(n0, n1, pcor, xnox, xnoy, uinf, vinf)
+    # 1504-1655: correction, far-field and wall velocities
+    L.or_correct_ref(n0, n1, h1, h2, us, vs, pcor, xix, xiy, etx, ety, dt, u, v, p)
+    circ = L.or_farfield_ref(n0, n1, h2, x, y, u, v, T, up, vp, Told, uet, etx, ety, ajac, xnox, xnoy,
+                             uinf, vinf, dt, amp, math.cos(wt))
+    # 1666-1782: wall and outer-row pressure
+    s = math.sin(wt)
+    alc_u, alc_v = c(accn * s * y[0]), c(-accn * s * x[0])
+    L.or_wall_p_ref(n0, n1, h1, h2, u, v, T, p, xix, xiy, etx, ety, alph, beta, gamma, q1, ajac,
+                    alc_u, alc_v, Re, Ri)
+    L.or_exit_p_ref(n0, n1, h1, h2, u, v, T, uold, vold, p, xix, xiy, etx, ety, alph, beta, gamma, q1, ajac,
+                    xnox, xnoy, uinf, vinf, dt, Re, Ri)
+    # 1806-1852: dilation
+    dil = np.zeros((n1, n0))
+    dmax = L.or_dil_ref(n0, n1, h1, h2, u, v, xix, xiy, etx, ety, dil)
+    return dict(u=u, v=v, T=T, p=p, up=up, vp=vp, us=us, vs=vs, q=q, pcor=pcor, qu=qu, qv=qv, qt=qt,
+                qup=qup, qvp=qvp, uold=uold, vold=vold, Told=Told, dil=dil, dmax=dmax, circ=circ)

@@ -37,6 +37,18 @@ def test_argument_errors_need_no_gpu():
     z = np.zeros((3, 3))
     # grids below 5 x 5 are rejected before any device work
     assert lib.cfd_ref_assemble_p9(3, 3, 0.5, 0.5, z, z, z, z, np.zeros((9, 3, 3))) == -1
+    # the reference-mode case checks its grid and parameters before touching the device
+    import ctypes as C
+    from cfd_solver_amd.refcase import ref_params
+    h = C.c_void_p()
+    n = np.zeros(3)
+    assert lib.cfd_ref_case_create(3, 3, 0.5, 0.5, *[z] * 10, None, None, n, n, C.byref(ref_params()),
+                                   C.byref(h)) == -1 and not h.value
+    z7 = np.zeros((7, 7))
+    n7 = np.zeros(7)
+    assert lib.cfd_ref_case_create(7, 7, 0.5, 0.5, *[z7] * 10, None, None, n7, n7,
+                                   C.byref(ref_params(p_maxit=0)), C.byref(h)) == -1
+    assert lib.cfd_ref_case_step(None, 1, None) == -1
 
 
 def test_batch_argument_errors_need_no_gpu():

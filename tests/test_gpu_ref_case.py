@@ -1,0 +1,71 @@
+"""The reference's whole O-grid time step on the GPU (cfd_ref_case_*) against the oracle.
+
+The oracle composes the reference's stages in its order (pinned bit for bit to
+the compiled reference's step 2 by test_oracle_golden.py); here its Gauss-Seidel
+and SIP solves are repeated until converged, and the GPU solves the same systems
+with BiCGStab to 1e-13, so the two agree to FP64 rounding of converged solves
+(1e-9 relative L2 per field).
+"""
+import types
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("u", "v", "T", "p", "up", "vp")
+
+
+def _grid(d):
+    g = types.SimpleNamespace(n0=d["n0"], n1=d["n1"], dxi=d["dxi"], deta=d["deta"])
+    for k in ["x", "y", "xix", "xiy", "etx", "ety", "alph", "beta", "gamma", "ajac", "xnox", "xnoy"]:
+        setattr(g, k, d["grid_" + k])
+    return g
+
+
+def _gpu_case(d, **kw):
+    from cfd_solver_amd.refcase import RefCase, ref_params
+    prm = ref_params(Re=d["Re"], Pr=d["Pr"], dt=d["dt"], Ri=float(d["s2_Ri"][0]), uinf=d["uinf"], vinf=d["vinf"],
+                     mom_rtol=1e-13, p_rtol=1e-13, **kw)
+    c = RefCase(_grid(d), prm)
+    c.set_state(d["s2_uold"], d["s2_vold"], d["s2_Told"], d["s1_p"], d["s1_up"], d["s1_vp"])
+    return c
+
+
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_steps_match_oracle_with_converged_solves(golden, nsteps):
+    from oracle.oracle import ref_step
+    from tests.ref_bc_cases import golden_step_inputs
+    print(f"\n[gpu] reference O-grid {golden['n0']}x{golden['n1']}, {nsteps} device time steps", flush=True)
+    c = _gpu_case(golden)
+    st = c.step(nsteps)
+    g, ops, s, prm = golden_step_inputs(golden)
+    for _ in range(nsteps):
+        o = ref_step(g, ops, s, prm, inner_calls=12)
+        s = {k: o[k] for k in FIELDS}
+    f = c.state()
+    for k in FIELDS:
+        assert rel_l2(f[k], o[k]) < 1e-9, (k, rel_l2(f[k], o[k]))
+    assert abs(st["dmax"] - o["dmax"]) <= 1e-8 * abs(o["dmax"])
+    assert abs(st["circ"] - o["circ"]) <= 1e-8 * abs(o["circ"]) + 1e-15
+    assert max(st["relres"]) <= 1e-13 and min(st["iters"]) >= 1
+    c.close()
+
+
+def test_step_with_oscillating_cylinder(golden):
+    """speed_amp != 0: the wall row moves as (-a y, a x) and the step stays equal to the oracle's."""
+    from oracle.oracle import ref_step
+    from tests.ref_bc_cases import golden_step_inputs
+    print("\n[gpu] reference O-grid, cylinder speed amplitude 0.05", flush=True)
+    c = _gpu_case(golden, speed_amp=0.05)
+    c.step(1)
+    g, ops, s, prm = golden_step_inputs(golden)
+    o = ref_step(g, ops, s, dict(prm, speed_amp=0.05), inner_calls=12)
+    f = c.state()
+    for k in FIELDS:
+        assert rel_l2(f[k], o[k]) < 1e-9, (k, rel_l2(f[k], o[k]))
+    np.testing.assert_allclose(f["u"][0], -0.05 * golden["grid_y"][0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f["v"][0], 0.05 * golden["grid_x"][0], rtol=0, atol=1e-15)
+    c.close()

@@ -106,3 +106,55 @@ def test_nusselt(golden, st):
     nu, prof = st.nusselt(d["s0_T"], g["ajac"], g["etx"], g["ety"], d["ar"])
     assert abs(nu - d["s0_nu_8dp"][0]) <= 5e-9
     np.testing.assert_allclose(prof, d["s0_dthdn_6sig"], rtol=1e-5, atol=1e-12)
+
+
+def _farfield(st, a):
+    return st.farfield(a["x"], a["y"], a["u"], a["v"], a["T"], a["up"], a["vp"], a["Told"], a["uet"], a["etx"],
+                       a["ety"], a["ajac"], a["xnox"], a["xnoy"], a["uinf"], a["vinf"], a["dt"],
+                       speed_amp=a["speed_amp"])
+
+
+def test_outflow_up(golden, st):
+    from tests.ref_bc_cases import outflow_golden
+    a, (wu, wv) = outflow_golden(golden)
+    up, vp = st.outflow_up(a["up"], a["vp"], a["u"], a["v"], a["xnox"], a["xnoy"], a["uinf"], a["vinf"])
+    assert rel_l2(up, wu) < TOL and rel_l2(vp, wv) < TOL
+
+
+def test_farfield(golden, st):
+    from tests.ref_bc_cases import farfield_golden
+    a, want = farfield_golden(golden)
+    u, v, T, up, vp, circ = _farfield(st, a)
+    for f, k in ((u, "u"), (v, "v"), (T, "T"), (up, "up"), (vp, "vp")):
+        assert rel_l2(f, want[k]) < TOL, k
+    assert abs(circ) < 1e-2
+
+
+@pytest.mark.parametrize("G", [1.0, 1e-3])
+def test_farfield_closed_form(G):
+    """Vortex decay G/r (k = 1) and G r0/r1^2 (k = 2), rigid wall rotation, CFL-1 shift of T."""
+    from cfd_solver_amd.stages import RefStages
+    from tests.ref_bc_cases import polar_case
+    print(f"\n[gpu] far-field closed form, G = {G}", flush=True)
+    a, (ou, ov, oT), (wu, wv), leave, k, circ_est = polar_case(G)
+    n1, n0 = a["u"].shape
+    s = RefStages(n0, n1, 1.0 / (n0 - 1), a["deta"])
+    u, v, T, up, vp, circ = _farfield(s, a)
+    assert abs(circ - circ_est) <= 1e-12 * abs(circ_est)
+    np.testing.assert_allclose(u[-1], ou, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(v[-1], ov, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(T[-1], oT, rtol=0, atol=1e-13)
+    for f, w in ((u[0], wu), (v[0], wv), (up[0], wu), (vp[0], wv)):
+        np.testing.assert_allclose(f, w, rtol=0, atol=1e-15)
+
+
+def test_exit_pressure(golden, st):
+    from tests.ref_bc_cases import exit_p_golden
+    d, g = golden, _g(golden)
+    a, want = exit_p_golden(d)
+    z = np.zeros((d["n1"], d["n0"]))
+    p = st.exit_pressure(a["u"], a["v"], a["T"], a["uold"], a["vold"], a["p"], g["xix"], g["xiy"], g["etx"],
+                         g["ety"], g["alph"], g["beta"], g["gamma"], z, g["ajac"], a["xnox"], a["xnoy"], a["uinf"],
+                         a["vinf"], a["dt"], a["Re"], a["Ri"])
+    assert rel_l2(p[-1], want[-1]) < 1e-12
+    assert np.array_equal(p[:-1], want[:-1])
