@@ -12,7 +12,7 @@ unroll the iteration cap); the batched and served runs replay captured graphs
 so the size classes overlap on their streams.
 
     python bench.py [--steps K] [--warmup W] [--n 1025] [--cpu-iters 6] [--no-cpu]
-                    [--sections main,batch,serve,piso]
+                    [--sections main,batch,serve,piso,ogrid]
 
 Prints one JSON line (value = cell-iterations/s, a cell = an interior node).
 The roofline object is for the kernel with the largest device time per outer
@@ -24,7 +24,9 @@ this host for a bounded number of outer iterations.  Further objects in config:
             size-class batches, all slots per launch, vs one case at a time;
   served    continuous batching of Poisson-arriving solves to 1e-8: solves/s,
             latency p50/p99;
-  p_isolate the p' solve alone at 2049 x 2049 to 1e-10, FP64 vs mixed.
+  p_isolate the p' solve alone at 2049 x 2049 to 1e-10, FP64 vs mixed;
+  ref_ogrid the reference's own O-grid time step resident on the GPU (257 x 513),
+            with the CPU oracle's reference-faithful step beside it.
 """
 from __future__ import annotations
 
@@ -77,7 +79,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
-    ap.add_argument("--sections", default="main,batch,serve,piso")
+    ap.add_argument("--sections", default="main,batch,serve,piso,ogrid")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="served arrivals per second")
     ap.add_argument("--serve-count", type=int, default=200)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
@@ -300,6 +302,58 @@ def run_p_isolate(n=2049, reps=3):
     return out
 
 
+def run_ref_ogrid(n0=257, n1=513, steps=10, warmup=2, cpu_steps=2, no_cpu=False):
+    """The reference solver's own O-grid time step (solver.cpp:1057-1852), resident on the
+    GPU, from the reference's initial state on an n0 x n1 O-grid around an ellipse of
+    aspect ratio 2 (outer radius 20), the reference's constants (Re 1000, dt 1e-4).
+    Every inner solve stops at 0.75e-2 of its first residual (2-norm; the reference's
+    Gauss-Seidel and SIP use the L1 norm); the CPU oracle's reference-faithful step
+    (SIP + Gauss-Seidel, serial) is timed beside it on the same grid."""
+    from cfd_solver_amd.grid import ogrid_ellipse
+    from cfd_solver_amd.refcase import RefCase, initial_state, ref_params
+    g = ogrid_ellipse(n0, n1, ar=2.0, r_out=20.0)
+    prm = ref_params(mom_rtol=0.75e-2, p_rtol=0.75e-2, mom_maxit=200, p_maxit=2000)
+    st0 = initial_state(g, prm)
+    c = RefCase(g, prm)
+    c.set_state(**st0)
+    c.step(warmup)
+    t0 = time.perf_counter()
+    s = c.step(steps)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    c.close()
+    out = {"grid": [n0, n1], "ms_per_step": ms, "steps": steps, "last_iters": s["iters"], "dmax": s["dmax"],
+           "workload": "reference O-grid time step, ellipse ar 2, r_out 20, Re 1000, dt 1e-4, "
+                       "inner solves to 0.75e-2 relative"}
+    if not no_cpu:
+        import ctypes as C
+        from oracle.oracle import build, lib, ref_step
+        build()
+        L = lib()
+        AU, AT = np.zeros((25, n1, n0)), np.zeros((25, n1, n0))
+        BU, BT, A9 = np.zeros((6, n0)), np.zeros((6, n0)), np.zeros((9, n1, n0))
+        z = np.zeros((n1, n0))
+        L.or_assemble_mom_ref(n0, n1, g.dxi, g.deta, g.alph, g.beta, g.gamma, z, z, prm.dt, prm.Re, prm.Pr,
+                              AU, AT, BU, BT)
+        L.or_assemble_p9_ref(n0, n1, g.dxi, g.deta, g.xix, g.xiy, g.etx, g.ety, A9)
+        gd = {k: getattr(g, k) for k in ("x", "y", "xix", "xiy", "etx", "ety", "alph", "beta", "gamma", "ajac",
+                                         "xnox", "xnoy")}
+        ops = dict(AU=AU, AT=AT, BU=BU, BT=BT, A9=A9)
+        pd = dict(dxi=g.dxi, deta=g.deta, dt=prm.dt, Re=prm.Re, Pr=prm.Pr, Ri=prm.Ri, uinf=prm.uinf, vinf=prm.vinf)
+        st = dict(st0)
+        for _ in range(warmup):
+            o = ref_step(gd, ops, st, pd)
+            st = {k: o[k] for k in ("u", "v", "T", "p", "up", "vp")}
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            o = ref_step(gd, ops, st, pd)
+            st = {k: o[k] for k in ("u", "v", "T", "p", "up", "vp")}
+        cms = (time.perf_counter() - t0) / cpu_steps * 1e3
+        out["cpu_oracle_ms_per_step"] = cms
+        out["cpu_sample"] = f"{cpu_steps} steps after {warmup}, SIP + Gauss-Seidel as in the reference, 1 thread"
+        out["gpu_over_cpu"] = cms / ms
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -341,6 +395,10 @@ def main():
     if "piso" in sections:
         cfg["p_isolate"] = run_p_isolate()
         print("[bench] p' isolate done", file=sys.stderr, flush=True)
+    if "ogrid" in sections:
+        cfg["ref_ogrid"] = run_ref_ogrid(no_cpu=a.no_cpu)
+        print(f"[bench] reference O-grid step: {cfg['ref_ogrid']['ms_per_step']:.2f} ms", file=sys.stderr,
+              flush=True)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))

@@ -69,3 +69,18 @@ class RefCase:
             self.close()
         except Exception:
             pass
+
+
+def initial_state(grid, params: RefParams):
+    """The reference's fields for a fresh start (restart == 0, solver.cpp:612-720): the
+    free stream everywhere, T = 0 and p = 0; the wall row moving with the cylinder
+    (-a y, a x) at T = 1 (a = speed_amp); the outer row keeps the free stream where
+    uinf*xnox + vinf*xnoy >= 0 and copies row n1-2 elsewhere (the same values here)."""
+    n1, n0 = np.shape(grid.x)
+    u = np.full((n1, n0), params.uinf)
+    v = np.full((n1, n0), params.vinf)
+    T = np.zeros((n1, n0))
+    u[0] = -params.speed_amp * np.asarray(grid.y)[0]
+    v[0] = params.speed_amp * np.asarray(grid.x)[0]
+    T[0] = 1.0
+    return dict(u=u, v=v, T=T, p=np.zeros((n1, n0)), up=u.copy(), vp=v.copy())
