@@ -1019,6 +1019,12 @@ __global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int npar
         s[S_ITTOT] = 0.0;
         s[S_RDONE] = (bn == 0.0 || rn <= rtol * bn) ? 1.0 : 0.0;
         s[S_RTOL] = rn > 0.0 ? fmax(margin * rtol * bn / rn, 1e-5) : 1e-5;
+        // the start of the first inner solve (its right-hand side is q)
+        s[S_SUM] = tot[0];
+        s[S_BB] = tot[NV - 1];
+        s[S_DONE] = 0.0;
+        s[S_IT] = 0.0;
+        s[S_SCONV] = 0.0;
     } else {
         // refinement test on the FP64 defect (mean removed) and the next inner tolerance
         // (after a solve that the start test skipped there is nothing to add)
@@ -1198,7 +1204,9 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
     const int n0 = c->n0, n1 = c->n1;
     const dim3 gr = grid_b(c), bl(BX, BY);
     if (!c->mixed) {
-        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph));
+        // with the face divergence's sums of q finished, the solve starts in one pass
+        if (q_norms_ready) RET(cav_fin<CF_Q>(c, -1, rtol, 1.0));
+        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph, q_norms_ready));
         k_pc_final<false><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
         return cav_fin<CF_PSUM>(c);
@@ -1224,7 +1232,7 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
             if (all) break;
         }
         RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, S_RDONE,
-                           graph));
+                           graph, k == 0));
         if (R == 1) {  // defect, p' = e and its sum in one pass
             k_defect_final<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, red(c));
             CFD_LAUNCH_CHECK();

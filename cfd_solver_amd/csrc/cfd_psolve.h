@@ -1050,6 +1050,42 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start2(int og, int n0, int n1, T *
     block_partials<1>(v, ra.part, ra.pstride);
 }
 
+// The start when the right-hand side's interior sum and sum of squares are already
+// in S_SUM / S_BB (and S_DONE, S_IT, S_SCONV cleared), e.g. from the pass that
+// produced b: x = 0, r = b - interior mean, r0 = r, partials of (r, r).  With og
+// set the solve is skipped (S_DONE = 1) after clearing x, as in k_bi_start1.
+template <typename T>
+__global__ __launch_bounds__(PS_NT) void k_bi_start_m(int og, int n0, int n1, const double *__restrict__ b,
+                                                      T *__restrict__ x, T *__restrict__ r, T *__restrict__ r0,
+                                                      RedArgs ra) {
+    ra = rz(ra);
+    if (slot_skip(ra.scal, -1)) return;
+    {
+        const size_t o = zslot() * n0 * n1;
+        b += o; x += o; r += o; r0 += o;
+    }
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const bool in = i < n0 && j < n1;
+    const size_t P = in ? idx(i, j, n0) : 0;
+    if (in) x[P] = T(0);
+    if (og >= 0 && ra.scal[og] != 0.0) {
+        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
+        return;
+    }
+    const double mean = ra.scal[S_SUM] / ((double)(n0 - 2) * (double)(n1 - 2));
+    double v[1] = {0.0};
+    if (in) {
+        T rv = T(0);
+        if (inner_node(i, j, n0, n1)) {
+            rv = (T)(b[P] - mean);
+            v[0] = (double)rv * (double)rv;
+        }
+        r[P] = rv;
+        r0[P] = rv;
+    }
+    block_partials<1>(v, ra.part, ra.pstride);
+}
+
 // p = r (first iteration) or r + beta (p - omega v); stops the solve when rho = 0
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *scal, const T *__restrict__ r,
@@ -1567,8 +1603,10 @@ struct PSolver {
     // at the start or at the ||s|| test turns the rest of its iteration's
     // launches into no-ops).  The iteration count and relative residual are
     // left in S_IT / S_RELRES of each slot.
+    // sums_ready: S_SUM / S_BB already hold b's interior sum and sum of squares and
+    // acc is null -- one start pass instead of two.
     int solve(PsCtx &cx, const double *b, T *x, double *acc, double rtol, int rtol_slot, int maxit, int og,
-              bool graph) {
+              bool graph, bool sums_ready = false) {
         OpLevel<T> &L0 = lev[0];
         const int n0 = L0.n0, n1 = L0.n1;
         dim3 gr = grid2d(n0, n1);
@@ -1578,9 +1616,13 @@ struct PSolver {
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
-        k_bi_start1<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, b, x, acc, r, ra);
-        k_bi_fin<FIN_START1><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
-        k_bi_start2<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
+        if (sums_ready && !acc) {
+            k_bi_start_m<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, b, x, r, r0, ra);
+        } else {
+            k_bi_start1<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, b, x, acc, r, ra);
+            k_bi_fin<FIN_START1><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
+            k_bi_start2<T><<<gr, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
+        }
         k_bi_fin<FIN_START2><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
         for (int it = 1; it <= maxit; it++) {
