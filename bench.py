@@ -8,8 +8,9 @@ and p' inner solve, FP64 residuals, refinement and accumulation), with the FP64
 path beside it.  One step = one outer iteration over the whole case (fields
 resident in HBM), launched eagerly with the p' solver's convergence flags read
 once per BiCGStab iteration (faster than replaying a captured graph, which must
-unroll the iteration cap); the batched and served runs replay captured graphs
-so the size classes overlap on their streams.
+unroll the iteration cap); the K timed steps are one call of the library's outer
+loop, which reads the statistics back after the last.  The batched and served
+runs replay captured graphs so the size classes overlap on their streams.
 
     python bench.py [--steps K] [--warmup W] [--n 1025] [--cpu-iters 6] [--no-cpu]
                     [--sections main,batch,serve,piso,ogrid]
@@ -122,9 +123,7 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
     for _ in range(max(warmup, 1)):
         case.step()
     t0 = time.perf_counter()
-    st = None
-    for _ in range(steps):
-        st = case.step()
+    st = case.step(steps)  # the K outer iterations in one call of the C driver (cfd_case_outer)
     elapsed = time.perf_counter() - t0
     case.profile(True)
     case.step()

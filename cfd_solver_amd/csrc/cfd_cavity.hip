@@ -1821,11 +1821,26 @@ int cfd_case_apply_bcs(cfd_case h) {
     return CFD_OK;
 }
 
+// n outer iterations.  The scalars are read back after the last one and after a
+// case's first (which sets the residual normalisers); the iterations in between
+// are enqueued back to back (the p' solve's flag waits still pace the host), so
+// the next iteration's launches queue behind the previous one's tail.  A residual
+// that turns non-finite stays so, and is reported after the last iteration.
 int cfd_case_outer(cfd_case h, int n, cfd_stats *st) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c || n < 0) return CFD_E_ARG;
     cfd_stats tmp{};
-    for (int k = 0; k < n; k++) RET(outer_once(c, st ? st : &tmp));
+    for (int k = 0; k < n; k++) {
+        bool first = false;
+        for (int z = 0; z < c->B; z++) first |= c->sph[z].active && c->iter[z] == 0;
+        if (k == n - 1 || first || c->prof) {
+            RET(outer_once(c, st ? st : &tmp));
+            continue;
+        }
+        RET(outer_launch(c));
+        for (int z = 0; z < c->B; z++)
+            if (c->sph[z].active) c->iter[z]++;
+    }
     return CFD_OK;
 }
 

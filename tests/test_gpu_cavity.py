@@ -174,3 +174,22 @@ def test_mixed_multi_step_refinement_in_outer_iterations():
         assert rel_l2(fg[k], fc[k]) < 1e-5, k
     pg, pc = fg["p"] - fg["p"][1:-1, 1:-1].mean(), fc["p"] - fc["p"][1:-1, 1:-1].mean()
     assert rel_l2(pg, pc) < 1e-5
+
+
+def test_multi_step_call_equals_single_steps():
+    """cfd_case_outer(n) enqueues the iterations between its first and last without
+    reading the scalars back; the result equals n single calls bit for bit."""
+    print("\n[gpu] 10 outer iterations in one call vs ten calls, 65x65, both precisions", flush=True)
+    from cfd_solver_amd.cavity import CavityCase, default_params
+    g = cavity_grid(65, skew=0.1, beta=1.5)
+    for prec in (0, 1):
+        a = CavityCase(g, default_params(Re=400.0, Ri=1.0, precision=prec))
+        b = CavityCase(g, default_params(Re=400.0, Ri=1.0, precision=prec))
+        sa = a.step(10)
+        for _ in range(10):
+            sb = b.step()
+        fa, fb = a.fields(), b.fields()
+        for k in ("u", "v", "T", "p"):
+            assert np.array_equal(fa[k], fb[k]), (prec, k)
+        assert sa.iter == sb.iter == 10
+        assert sa.mass_res == sb.mass_res and sa.p_iters == sb.p_iters
