@@ -563,56 +563,6 @@ __global__ void k_nusselt_ref(int n0, double h1, double h2, const double *__rest
 
 __device__ __forceinline__ bool is_unk(int i, int j, int n0, int n1) { return i <= n0 - 2 && j >= 1 && j <= n1 - 2; }
 
-template <int R>
-__global__ void k_sop_apply(int n0, int n1, const double *__restrict__ A, const double *__restrict__ x,
-                            const double *__restrict__ q, double *__restrict__ y) {
-    // y = q - A x on the unknowns when q != nullptr, else y = A x; 0 elsewhere
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    if (i >= n0 || j >= n1) return;
-    const size_t P = ix(i, j, n0);
-    if (!is_unk(i, j, n0, n1)) { y[P] = 0.0; return; }
-    const size_t N = (size_t)n0 * n1;
-    const int r = (R == 2 && (j == 1 || j == n1 - 2)) ? 1 : R;
-    double s = 0.0;
-    for (int dj = -r; dj <= r; dj++)
-        for (int di = -r; di <= r; di++) {
-            const int k = (R == 2) ? (dj + 2) * 5 + (di + 2) : (dj + 1) * 3 + (di + 1);
-            // wrap with period n0-1 (column n0-1 is read as column 0)
-            int ii = i + di;
-            if (ii < 0) ii += n0 - 1;
-            if (ii > n0 - 2) ii -= n0 - 1;
-            s += A[(size_t)k * N + P] * x[ix(ii, j + dj, n0)];
-        }
-    y[P] = q ? q[P] - s : s;
-}
-
-__global__ void k_dot_part(int n0, int n1, const double *__restrict__ a, const double *__restrict__ b,
-                           double *__restrict__ part) {
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-    double s = 0.0;
-    if (i < n0 && j < n1 && is_unk(i, j, n0, n1)) s = a[ix(i, j, n0)] * b[ix(i, j, n0)];
-    s = block_sum<BX * BY>(s);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = s;
-}
-
-__global__ void k_sum_final(const double *__restrict__ part, int n, double *__restrict__ out) {
-    double s = 0.0;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) s += part[t];
-    s = block_sum<256>(s);
-    if (threadIdx.x == 0) *out = s;
-}
-
-// z = invdiag * y
-__global__ void k_jac(size_t n, const double *__restrict__ y, const double *__restrict__ invd, double *__restrict__ z) {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) z[k] = y[k] * invd[k];
-}
-// y = a + s1*b + s2*c
-__global__ void k_lin3(size_t n, double *__restrict__ y, const double *__restrict__ a, double s1,
-                       const double *__restrict__ b, double s2, const double *__restrict__ c) {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) y[k] = a[k] + s1 * b[k] + (c ? s2 * c[k] : 0.0);
-}
 __global__ void k_invdiag(int n0, int n1, int center, const double *__restrict__ A, double *__restrict__ invd) {
     const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     if (i >= n0 || j >= n1) return;
@@ -667,113 +617,250 @@ int down(double *h, const double *d, size_t n) {
 
 bool bad_grid(int n0, int n1) { return n0 < 5 || n1 < 5; }
 
-// Work vectors of the stored-operator BiCGStab on n0 x n1 nodes (device).
+// Work vectors of the stored-operator BiCGStab on n0 x n1 nodes (device): nine
+// node vectors, two rows of block partials and the solver's scalars.
 struct SopWork {
     double *r = nullptr, *r0 = nullptr, *pv = nullptr, *vv = nullptr, *s = nullptr, *t = nullptr, *ph = nullptr,
-           *sh = nullptr, *dx = nullptr, *invd = nullptr, *part = nullptr, *dsc = nullptr;
-    // the scratch that one solve needs: 10 node vectors, block partials, 8 scalars
-    static size_t doubles(int n0, int n1) {
+           *sh = nullptr, *invd = nullptr, *part = nullptr, *sc = nullptr;
+    int np = 0;  // partials per row
+    static int parts(int n0, int n1) {
         const dim3 gr = grid2d(n0, n1);
-        return 10 * (size_t)n0 * n1 + (size_t)gr.x * gr.y + 16;
+        const long n1d = ((long)n0 * n1 + 255) / 256;
+        return (int)std::max<long>((long)gr.x * gr.y, n1d);
     }
+    static size_t doubles(int n0, int n1) { return 9 * (size_t)n0 * n1 + 2 * (size_t)parts(n0, n1) + 16; }
     static SopWork carve(double *base, int n0, int n1) {
         const size_t N = (size_t)n0 * n1;
         SopWork w;
-        double **v[10] = {&w.r, &w.r0, &w.pv, &w.vv, &w.s, &w.t, &w.ph, &w.sh, &w.dx, &w.invd};
-        for (int k = 0; k < 10; k++) *v[k] = base + (size_t)k * N;
-        w.part = base + 10 * N;
-        const dim3 gr = grid2d(n0, n1);
-        w.dsc = w.part + (size_t)gr.x * gr.y + 8;
+        double **v[9] = {&w.r, &w.r0, &w.pv, &w.vv, &w.s, &w.t, &w.ph, &w.sh, &w.invd};
+        for (int k = 0; k < 9; k++) *v[k] = base + (size_t)k * N;
+        w.np = parts(n0, n1);
+        w.part = base + 9 * N;
+        w.sc = w.part + 2 * (size_t)w.np;
         return w;
     }
 };
 
+// the solver's device scalars
+enum { Q_RHO, Q_RHO_OLD, Q_ALPHA, Q_OMEGA, Q_BETA, Q_TOL, Q_R0N, Q_RN, Q_SCONV, Q_DONE, Q_NSC };
+
+// block partials of up to two dot products, row k at part[k * np]
+template <int NV, int NT>
+__device__ __forceinline__ void sop_partials(const double (&v)[NV], double *part, int np, int b) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double t = block_sum<NT>(v[k]);
+        if (threadIdx.x + threadIdx.y * blockDim.x == 0) part[(size_t)k * np + b] = t;
+    }
+}
+
+// y = A x on the unknowns (0 elsewhere) for a 9-point (R = 1) or 25-point (R = 2,
+// 9-point on rows 1 and n1-2) operator, columns wrapping with period n0-1
+template <int R>
+__device__ __forceinline__ double sop_row(int n0, int n1, const double *__restrict__ A, const double *__restrict__ x,
+                                          int i, int j) {
+    const size_t N = (size_t)n0 * n1, P = ix(i, j, n0);
+    const int r = (R == 2 && (j == 1 || j == n1 - 2)) ? 1 : R;
+    double s = 0.0;
+    for (int dj = -r; dj <= r; dj++)
+        for (int di = -r; di <= r; di++) {
+            const int k = (R == 2) ? (dj + 2) * 5 + (di + 2) : (dj + 1) * 3 + (di + 1);
+            int ii = i + di;
+            if (ii < 0) ii += n0 - 1;
+            if (ii > n0 - 2) ii -= n0 - 1;
+            s += A[(size_t)k * N + P] * x[ix(ii, j + dj, n0)];
+        }
+    return s;
+}
+
+// start: r = q - A x (unknowns), r0 = r, partials of (r, r)
+template <int R>
+__global__ void k_q_start(int n0, int n1, const double *__restrict__ A, const double *__restrict__ x,
+                          const double *__restrict__ q, double *__restrict__ r, double *__restrict__ r0,
+                          double *__restrict__ part, int np) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[1] = {0.0};
+    if (i < n0 && j < n1) {
+        const size_t P = ix(i, j, n0);
+        const double rv = is_unk(i, j, n0, n1) ? q[P] - sop_row<R>(n0, n1, A, x, i, j) : 0.0;
+        r[P] = rv;
+        r0[P] = rv;
+        v[0] = rv * rv;
+    }
+    sop_partials<1, BX * BY>(v, part, np, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+// p = r (first iteration) or r + beta (p - omega v); ph = p / diag
+__global__ void k_q_p(size_t n, int first, const double *__restrict__ sc, const double *__restrict__ r,
+                      const double *__restrict__ vv, const double *__restrict__ invd, double *__restrict__ pv,
+                      double *__restrict__ ph) {
+    if (sc[Q_DONE] != 0.0) return;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double p = first ? r[k] : r[k] + sc[Q_BETA] * (pv[k] - sc[Q_OMEGA] * vv[k]);
+    pv[k] = p;
+    ph[k] = p * invd[k];
+}
+
+// y = A x and partials of (w, y); x is ph (v) or sh (t, with (t, s) and (t, t))
+template <int R, int NV>
+__global__ void k_q_apply(int n0, int n1, const double *__restrict__ sc, const double *__restrict__ A,
+                          const double *__restrict__ x, const double *__restrict__ w, double *__restrict__ y,
+                          double *__restrict__ part, int np) {
+    if (sc[Q_DONE] != 0.0 || (NV == 2 && sc[Q_SCONV] != 0.0)) return;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    if (i < n0 && j < n1) {
+        const size_t P = ix(i, j, n0);
+        const double yv = is_unk(i, j, n0, n1) ? sop_row<R>(n0, n1, A, x, i, j) : 0.0;
+        y[P] = yv;
+        v[0] = w[P] * yv;
+        if (NV == 2) v[NV - 1] = yv * yv;
+    }
+    sop_partials<NV, BX * BY>(v, part, np, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+// s = r - alpha v, sh = s / diag, partials of (s, s)
+__global__ void k_q_s(size_t n, const double *__restrict__ sc, const double *__restrict__ r,
+                      const double *__restrict__ vv, const double *__restrict__ invd, double *__restrict__ s,
+                      double *__restrict__ sh, double *__restrict__ part, int np) {
+    if (sc[Q_DONE] != 0.0) return;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (k < n) {
+        const double sv = r[k] - sc[Q_ALPHA] * vv[k];
+        s[k] = sv;
+        sh[k] = sv * invd[k];
+        v[0] = sv * sv;
+    }
+    sop_partials<1, 256>(v, part, np, blockIdx.x);
+}
+
+// x += alpha ph + omega sh (omega taken as 0 after the ||s|| test stopped the
+// iteration); r = s - omega t with partials of (r, r) and (r0, r)
+__global__ void k_q_xr(size_t n, const double *__restrict__ sc, const double *__restrict__ ph,
+                       const double *__restrict__ sh, const double *__restrict__ s, const double *__restrict__ t,
+                       const double *__restrict__ r0, double *__restrict__ x, double *__restrict__ r,
+                       double *__restrict__ part, int np) {
+    if (sc[Q_DONE] != 0.0) return;
+    const bool sconv = sc[Q_SCONV] != 0.0;
+    const double a = sc[Q_ALPHA], om = sconv ? 0.0 : sc[Q_OMEGA];
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (k < n) {
+        x[k] += a * ph[k] + om * sh[k];
+        if (!sconv) {
+            const double rv = s[k] - om * t[k];
+            r[k] = rv;
+            v[0] = rv * rv;
+            v[1] = r0[k] * rv;
+        }
+    }
+    sop_partials<2, 256>(v, part, np, blockIdx.x);
+}
+
+enum { QF_START, QF_V, QF_S, QF_T, QF_XR };
+// totals of a phase's partials (one block) and the BiCGStab scalars that follow
+template <int OP>
+__global__ void k_q_fin(const double *__restrict__ part, int np, int nparts, double *__restrict__ sc, double rtol,
+                        double *__restrict__ flag) {
+    constexpr int NV = (OP == QF_T || OP == QF_XR) ? 2 : 1;
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += 256) a += part[(size_t)k * np + b];
+        tot[k] = block_sum<256>(a);
+    }
+    if (threadIdx.x != 0) return;
+    if (OP == QF_START) {
+        const double rn = sqrt(tot[0]);
+        sc[Q_RHO] = tot[0];
+        sc[Q_RHO_OLD] = 1.0;
+        sc[Q_ALPHA] = 1.0;
+        sc[Q_OMEGA] = 1.0;
+        sc[Q_BETA] = 0.0;
+        sc[Q_R0N] = rn;
+        sc[Q_RN] = rn;
+        sc[Q_TOL] = rtol * rn;
+        sc[Q_SCONV] = 0.0;
+        sc[Q_DONE] = rn > 0.0 ? 0.0 : 1.0;
+    } else if (sc[Q_DONE] != 0.0) {
+        // nothing ran in this phase; the flag below still reports the state
+    } else if (OP == QF_V) {
+        sc[Q_ALPHA] = sc[Q_RHO] / tot[0];
+    } else if (OP == QF_S) {
+        const double sn = sqrt(tot[0]);
+        if (sn <= sc[Q_TOL]) {
+            sc[Q_SCONV] = 1.0;
+            sc[Q_RN] = sn;
+        }
+    } else if (OP == QF_T) {
+        if (sc[Q_SCONV] == 0.0) sc[Q_OMEGA] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+    } else {
+        if (sc[Q_SCONV] != 0.0) {
+            sc[Q_DONE] = 1.0;
+        } else {
+            const double rn = sqrt(tot[0]);
+            sc[Q_RN] = rn;
+            if (rn <= sc[Q_TOL] || sc[Q_OMEGA] == 0.0 || tot[1] == 0.0) {
+                sc[Q_DONE] = 1.0;
+            } else {
+                sc[Q_RHO_OLD] = sc[Q_RHO];
+                sc[Q_RHO] = tot[1];
+                sc[Q_BETA] = (tot[1] / sc[Q_RHO_OLD]) * (sc[Q_ALPHA] / sc[Q_OMEGA]);
+            }
+        }
+    }
+    if (flag) {
+        flag[0] = sc[Q_DONE];
+        flag[1] = sc[Q_RN];
+        flag[2] = sc[Q_R0N];
+    }
+}
+
 // GPU BiCGStab (Jacobi-preconditioned) on a stored 9/25-point operator A (device
 // planes), x (device, boundary rows fixed) updated in place, right-hand side qd
-// (device), on stream st; the dot products are read back on the host.
+// (device), on stream st.  The recurrence's scalars stay on the device; the host
+// reads one done flag per iteration (into hflag: 3 doubles, pinned when it can be).
 template <int R>
 int sop_core(int n0, int n1, const double *A, double *x, const double *qd, const SopWork &w, double rtol, int maxit,
-             int *iters, double *relres, hipStream_t st) {
+             int *iters, double *relres, hipStream_t st, double *hflag) {
     const size_t N = (size_t)n0 * n1;
     const dim3 gr = grid2d(n0, n1), bl(BX, BY);
-    const int nb = (int)(gr.x * gr.y);
+    const int nb2 = (int)(gr.x * gr.y), lb = 256;
+    const int nb1 = (int)((N + lb - 1) / lb);
     const int center = (R == 2) ? 12 : 4;
-    CFD_HIP(hipMemsetAsync(w.pv, 0, N * sizeof(double), st));
-    CFD_HIP(hipMemsetAsync(w.vv, 0, N * sizeof(double), st));
-    CFD_HIP(hipMemsetAsync(w.dx, 0, N * sizeof(double), st));
     k_invdiag<<<gr, bl, 0, st>>>(n0, n1, center, A, w.invd);
+    k_q_start<R><<<gr, bl, 0, st>>>(n0, n1, A, x, qd, w.r, w.r0, w.part, w.np);
+    k_q_fin<QF_START><<<1, 256, 0, st>>>(w.part, w.np, nb2, w.sc, rtol, w.sc + Q_NSC);
     CFD_LAUNCH_CHECK();
-    auto dot = [&](const double *a, const double *b, double *out) -> int {
-        k_dot_part<<<gr, bl, 0, st>>>(n0, n1, a, b, w.part);
-        k_sum_final<<<1, 256, 0, st>>>(w.part, nb, w.dsc);
-        CFD_LAUNCH_CHECK();
-        CFD_HIP(hipMemcpyAsync(out, w.dsc, sizeof(double), hipMemcpyDeviceToHost, st));
-        CFD_HIP(hipStreamSynchronize(st));
-        return CFD_OK;
-    };
-    const int lb = 256;
-    const int lg = (int)((N + lb - 1) / lb);
-    double *r = w.r, *r0 = w.r0, *pv = w.pv, *vv = w.vv, *s = w.s, *t = w.t, *ph = w.ph, *sh = w.sh, *dx = w.dx;
-    // r = q - A phi (boundary rows of phi enter here), solve A dx = r
-    k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, x, qd, r);
-    CFD_LAUNCH_CHECK();
-    CFD_HIP(hipMemcpyAsync(r0, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-    double rr0 = 0.0;
-    int rc = dot(r, r, &rr0);
-    if (rc) return rc;
-    const double r0n = std::sqrt(rr0);
-    double rn = r0n;
+    CFD_HIP(hipMemcpyAsync(hflag, w.sc + Q_NSC, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    CFD_HIP(hipStreamSynchronize(st));
     int it = 0;
-    if (r0n > 0.0) {
-        const double tol = rtol * r0n;
-        double rho_old = 1.0, alpha = 1.0, omega = 1.0, rho = rr0;
-        for (it = 1; it <= maxit; it++) {
-            if (rho == 0.0) break;
-            if (it == 1) {
-                CFD_HIP(hipMemcpyAsync(pv, r, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-            } else {
-                const double beta = (rho / rho_old) * (alpha / omega);
-                // pv = r + beta (pv - omega vv)
-                k_lin3<<<lg, lb, 0, st>>>(N, pv, pv, -omega, vv, 0.0, nullptr);
-                k_lin3<<<lg, lb, 0, st>>>(N, pv, r, beta, pv, 0.0, nullptr);
-            }
-            k_jac<<<lg, lb, 0, st>>>(N, pv, w.invd, ph);
-            k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, ph, nullptr, vv);
-            CFD_LAUNCH_CHECK();
-            double rv = 0.0;
-            if ((rc = dot(r0, vv, &rv))) return rc;
-            alpha = rho / rv;
-            k_lin3<<<lg, lb, 0, st>>>(N, s, r, -alpha, vv, 0.0, nullptr);
-            double ss = 0.0;
-            if ((rc = dot(s, s, &ss))) return rc;
-            if (std::sqrt(ss) <= tol) {
-                k_lin3<<<lg, lb, 0, st>>>(N, dx, dx, alpha, ph, 0.0, nullptr);
-                rn = std::sqrt(ss);
-                break;
-            }
-            k_jac<<<lg, lb, 0, st>>>(N, s, w.invd, sh);
-            k_sop_apply<R><<<gr, bl, 0, st>>>(n0, n1, A, sh, nullptr, t);
-            CFD_LAUNCH_CHECK();
-            double ts = 0.0, tt = 0.0;
-            if ((rc = dot(t, s, &ts))) return rc;
-            if ((rc = dot(t, t, &tt))) return rc;
-            omega = tt > 0.0 ? ts / tt : 0.0;
-            k_lin3<<<lg, lb, 0, st>>>(N, dx, dx, alpha, ph, omega, sh);
-            k_lin3<<<lg, lb, 0, st>>>(N, r, s, -omega, t, 0.0, nullptr);
-            double rr = 0.0;
-            if ((rc = dot(r, r, &rr))) return rc;
-            rn = std::sqrt(rr);
-            if (rn <= tol || omega == 0.0) break;
-            rho_old = rho;
-            if ((rc = dot(r0, r, &rho))) return rc;
-        }
-        if (it > maxit) it = maxit;
-        k_lin3<<<lg, lb, 0, st>>>(N, x, x, 1.0, dx, 0.0, nullptr);
-        k_dupcol<<<(n1 + 255) / 256, 256, 0, st>>>(n0, n1, x);
+    for (it = 1; it <= maxit && hflag[0] == 0.0; it++) {  // none when r0 = 0
+        k_q_p<<<nb1, lb, 0, st>>>(N, it == 1, w.sc, w.r, w.vv, w.invd, w.pv, w.ph);
+        k_q_apply<R, 1><<<gr, bl, 0, st>>>(n0, n1, w.sc, A, w.ph, w.r0, w.vv, w.part, w.np);
+        k_q_fin<QF_V><<<1, 256, 0, st>>>(w.part, w.np, nb2, w.sc, rtol, nullptr);
+        k_q_s<<<nb1, lb, 0, st>>>(N, w.sc, w.r, w.vv, w.invd, w.s, w.sh, w.part, w.np);
+        k_q_fin<QF_S><<<1, 256, 0, st>>>(w.part, w.np, nb1, w.sc, rtol, nullptr);
+        k_q_apply<R, 2><<<gr, bl, 0, st>>>(n0, n1, w.sc, A, w.sh, w.s, w.t, w.part, w.np);
+        k_q_fin<QF_T><<<1, 256, 0, st>>>(w.part, w.np, nb2, w.sc, rtol, nullptr);
+        k_q_xr<<<nb1, lb, 0, st>>>(N, w.sc, w.ph, w.sh, w.s, w.t, w.r0, x, w.r, w.part, w.np);
+        k_q_fin<QF_XR><<<1, 256, 0, st>>>(w.part, w.np, nb1, w.sc, rtol, w.sc + Q_NSC);
         CFD_LAUNCH_CHECK();
+        CFD_HIP(hipMemcpyAsync(hflag, w.sc + Q_NSC, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+        CFD_HIP(hipStreamSynchronize(st));
+        if (hflag[0] != 0.0) break;
     }
+    if (hflag[2] == 0.0) it = 0;
+    if (it > maxit) it = maxit;
+    k_dupcol<<<(n1 + 255) / 256, 256, 0, st>>>(n0, n1, x);
+    CFD_LAUNCH_CHECK();
     if (iters) *iters = it;
-    if (relres) *relres = r0n > 0.0 ? rn / r0n : 0.0;
+    if (relres) *relres = hflag[2] > 0.0 ? hflag[1] / hflag[2] : 0.0;
     return CFD_OK;
 }
 
@@ -792,7 +879,8 @@ int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double
         CFD_HIP(hipMemcpy(A + (size_t)k * N, planes_host[k], N * sizeof(double), hipMemcpyHostToDevice));
     double *x = B.up(phi, N), *qd = B.up(q, N), *ws = B.get(SopWork::doubles(n0, n1));
     NEED(x && qd && ws);
-    CFD_RET(sop_core<R>(n0, n1, A, x, qd, SopWork::carve(ws, n0, n1), rtol, maxit, iters, relres, nullptr));
+    double hflag[3] = {0.0, 0.0, 0.0};
+    CFD_RET(sop_core<R>(n0, n1, A, x, qd, SopWork::carve(ws, n0, n1), rtol, maxit, iters, relres, nullptr, hflag));
     CFD_HIP(hipDeviceSynchronize());
     return down(phi, x, N);
 }
@@ -865,7 +953,7 @@ int ref_solve_rows(RefCase *c, const double *A, const double *rhs, double *dst, 
                    double *rr) {
     const size_t N = (size_t)c->n0 * c->n1;
     CFD_RET(sop_core<R>(c->n0, c->n1, A, c->sol, rhs, SopWork::carve(c->ws, c->n0, c->n1), rtol, maxit, it, rr,
-                        c->s));
+                        c->s, c->hscal + 4));
     CFD_HIP(hipMemcpyAsync(dst + c->n0, c->sol + c->n0, (N - 2 * (size_t)c->n0) * sizeof(double),
                            hipMemcpyDeviceToDevice, c->s));
     return CFD_OK;
@@ -915,7 +1003,7 @@ int ref_case_step(RefCase *c, cfd_ref_stats *st) {
     CFD_HIP(hipMemcpyAsync(c->uold, c->u, B, hipMemcpyDeviceToDevice, s));
     CFD_HIP(hipMemcpyAsync(c->vold, c->v, B, hipMemcpyDeviceToDevice, s));
     CFD_RET(sop_core<1>(n0, n1, c->A9, c->pcor, c->q, SopWork::carve(c->ws, n0, n1), P.p_rtol, P.p_maxit,
-                        &o.iters[5], &o.relres[5], s));
+                        &o.iters[5], &o.relres[5], s, c->hscal + 4));
     k_pcor_bc<<<g1, 256, 0, s>>>(n0, n1, c->pcor, c->xnox, c->xnoy, P.uinf, P.vinf);
     // 1504-1655: correction, far-field and wall velocities
     k_correct_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->us, c->vs, c->pcor, c->xix, c->xiy, c->etx, c->ety,
