@@ -1503,7 +1503,7 @@ int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
         UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega, c->scal, -1};
         const dim3 g2((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, c->B);
         if (PSolver<T>::wide_blocks(g2)) k_mg_up<T, 2, MG_NT_WIDE><<<g2, MG_NT_WIDE, 0, c->s>>>(a);
-        else k_mg_up<T, 2, PS_NT><<<g2, PS_NT, 0, c->s>>>(a);
+        else k_mg_up<T, 2, MG_NT_NARROW><<<g2, MG_NT_NARROW, 0, c->s>>>(a);
     }
     CFD_LAUNCH_CHECK();
     return CFD_OK;
