@@ -140,7 +140,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ double s_u[FX * FY], s_v[FX * FY], s_t[FX * FY], s_p[PX * PY];
     const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
-    const int i0 = blockIdx.x * RT_X, j0 = blockIdx.y * RT_Y;
+    const int2 tl = xcd_tile();
+    const int i0 = tl.x * RT_X, j0 = tl.y * RT_Y;
     struct F3 {
         double u, v, t;
     };
@@ -366,7 +367,8 @@ __global__ __launch_bounds__(NT) void k_jacobin(CGrid g, JArgs<TR> a) {
     __shared__ TR s_buf[2][3 * NM];
     const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
-    const int fi0 = blockIdx.x * JT_X, fj0 = blockIdx.y * JT_Y;
+    const int2 tl = xcd_tile();
+    const int fi0 = tl.x * JT_X, fj0 = tl.y * JT_Y;
     const RecipT<TR> r = recip_cast<TR>(g.r);
     struct O3 {
         TR u, v, t;
@@ -453,7 +455,8 @@ __global__ __launch_bounds__(NT) void k_face_div(
     __shared__ double s_p[PX * PY];
     const int n0 = g.n0, n1 = g.n1;
     const int tid = tlin();
-    const int i0 = blockIdx.x * BX, j0 = blockIdx.y * BY;
+    const int2 tl = xcd_tile();
+    const int i0 = tl.x * BX, j0 = tl.y * BY;
     struct H {
         double u, v, du, dv, d, x, y, e, f, j;
     };
@@ -553,7 +556,8 @@ __global__ __launch_bounds__(NT) void k_correct(
     __shared__ double s_u[HX * HY], s_v[HX * HY];
     const int n0 = g.n0, n1 = g.n1;
     const int tid = tlin();
-    const int i0 = blockIdx.x * BX, j0 = blockIdx.y * BY;
+    const int2 tl = xcd_tile();
+    const int i0 = tl.x * BX, j0 = tl.y * BY;
     struct H {
         double u, v, du, dv, d, x, y, e, f;
     };

@@ -50,6 +50,20 @@ inline void launch_timed(const Timed &t, void (*k)(P...), dim3 g, dim3 b, hipStr
     else k<<<g, b, 0, s>>>(a...);
 }
 
+// Tile (x, y) of this block, numbered so that each XCD works on one contiguous
+// band of tiles: the dispatcher hands the blocks of a launch to the 8 XCDs in
+// turn (block b to XCD b % 8), so renumbering b to xcd * (its share) + b / 8
+// keeps neighbouring tiles, which read each other's halo rows, in one L2.  A
+// bijection on the tiles of one slot (blockIdx.z).
+__device__ __forceinline__ int2 xcd_tile() {
+    constexpr int NX = 8;
+    const int gx = (int)gridDim.x, nb = gx * (int)gridDim.y;
+    const int b = (int)blockIdx.x + gx * (int)blockIdx.y;
+    const int xcd = b % NX, q = nb / NX, rm = nb % NX;
+    const int t = xcd * q + min(xcd, rm) + b / NX;
+    return make_int2(t % gx, t / gx);
+}
+
 // 2-D launch shape: 64 lanes along xi (one wavefront per row segment), 4 rows per block.
 constexpr int BX = 64;
 constexpr int BY = 4;
