@@ -918,7 +918,8 @@ __global__ __launch_bounds__(NT) void k_defect(FArgs<double> a, const double *__
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o; r += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int2 tl = xcd_tile();
+    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < a.n0 && j < a.n1) {
         const size_t P = idx(i, j, a.n0);
@@ -976,7 +977,8 @@ __global__ __launch_bounds__(NT) void k_defect_final(FArgs<double> a, const doub
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int2 tl = xcd_tile();
+    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
     double v[3] = {0.0, 0.0, 0.0};
     if (i < a.n0 && j < a.n1) {
         const size_t P = idx(i, j, a.n0);

@@ -55,12 +55,14 @@ inline void launch_timed(const Timed &t, void (*k)(P...), dim3 g, dim3 b, hipStr
 // turn (block b to XCD b % 8), so renumbering b to xcd * (its share) + b / 8
 // keeps neighbouring tiles, which read each other's halo rows, in one L2.  A
 // bijection on the tiles of one slot (blockIdx.z).
-__device__ __forceinline__ int2 xcd_tile() {
+__device__ __forceinline__ int xcd_order(int b, int nb) {
     constexpr int NX = 8;
-    const int gx = (int)gridDim.x, nb = gx * (int)gridDim.y;
-    const int b = (int)blockIdx.x + gx * (int)blockIdx.y;
     const int xcd = b % NX, q = nb / NX, rm = nb % NX;
-    const int t = xcd * q + min(xcd, rm) + b / NX;
+    return xcd * q + min(xcd, rm) + b / NX;
+}
+__device__ __forceinline__ int2 xcd_tile() {
+    const int gx = (int)gridDim.x;
+    const int t = xcd_order((int)blockIdx.x + gx * (int)blockIdx.y, gx * (int)gridDim.y);
     return make_int2(t % gx, t / gx);
 }
 

@@ -257,7 +257,7 @@ __device__ __forceinline__ bool lev_of_block(const LevTab<T> &A, int &l, int &i,
     const int b = blockIdx.x;
     l = 0;
     while (l + 1 < A.nlev && A.bofs[l + 1] <= b) l++;
-    const int lb = b - A.bofs[l];
+    const int lb = xcd_order(b - A.bofs[l], A.bofs[l + 1] - A.bofs[l]);  // neighbours on one XCD
     i = (lb % A.bx[l]) * BX + (threadIdx.x % BX);
     j = (lb / A.bx[l]) * BY + (threadIdx.x / BX);
     return i < A.n0[l] && j < A.n1[l];
@@ -436,7 +436,8 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     T *s_r = NPRE == 2 ? s_x1 : (NPRE == 1 ? s_xs + TL::NXS : s_b);
     const int tid = tlin();
     const int n0 = A.a.n0, n1 = A.a.n1;
-    const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
+    const int2 tl = xcd_tile();
+    const int fi0 = tl.x * MG_FX, fj0 = tl.y * MG_FY;
     const T w = A.w;
     // right-hand side (and 1/diag for a first sweep staged here), halo HB
     struct BI {
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     double cs[1] = {0.0};
     if (NTB <= MG_CX * MG_CY || tid < MG_CX * MG_CY) {
         const int ci = tid % MG_CX, cj = tid / MG_CX;
-        const int I = blockIdx.x * MG_CX + ci, J = blockIdx.y * MG_CY + cj;
+        const int I = tl.x * MG_CX + ci, J = tl.y * MG_CY + cj;
         if (I < A.m0 && J < A.m1) {
             T v = T(0);
             if (inner_node(I, J, A.m0, A.m1)) {
@@ -585,7 +586,8 @@ __global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
     __shared__ T s_z[NPOST == 2 ? QX * QY : 1];
     const int tid = tlin();
     const int n0 = A.a.n0, n1 = A.a.n1, m0 = A.m0, m1 = A.m1;
-    const int fi0 = blockIdx.x * MG_FX, fj0 = blockIdx.y * MG_FY;
+    const int2 tl = xcd_tile();
+    const int fi0 = tl.x * MG_FX, fj0 = tl.y * MG_FY;
     const T w = A.w;
     const T *xc = A.xc;
     // x and the coarse values its bilinear interpolation uses, halo NPOST
@@ -1121,7 +1123,8 @@ __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict_
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); ph += o; vv += o; r0 += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int2 tl = xcd_tile();
+    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
     double v[1] = {0.0};
     if (i < a.n0 && j < a.n1) {
         const size_t P = idx(i, j, a.n0);
@@ -1167,7 +1170,8 @@ __global__ __launch_bounds__(PS_NT) void k_bi_t(FArgs<T> a, const T *__restrict_
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); sh += o; t += o; s += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const int2 tl = xcd_tile();
+    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
     double v[2] = {0.0, 0.0};
     if (i < a.n0 && j < a.n1) {
         const size_t P = idx(i, j, a.n0);
