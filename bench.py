@@ -51,7 +51,9 @@ def kernel_bytes(name, mixed):
     return {
         "p_apply": 7 * w,          # x, 4 face coefficients, r0 (or s) -> y; dot product in the kernel
         "mg_down": 7.25 * w,       # b, 1/diag, 4 face coefficients -> x, restricted residual (1/4 node)
-        "mg_up": 8.25 * w,         # x, b, 1/diag, 4 faces, coarse x (1/4 node) -> x
+        # finest up leg fused with BiCGStab's operator application: x, b, 1/diag, 4 faces,
+        # coarse x (1/4 node), r0 (every other launch) -> z, y = L z
+        "mg_up": 9.75 * w,
         "jacobi": 16 * w,          # four sweeps: R_u,R_v,R_T, U,V, alph,beta,gamma,P,Q, du,dv,dT -> du,dv,dT
         "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
         "face_div": 10 * 8 + 2 * w,  # u,v,p,d, 4 metrics, J (FP64), du,dv -> q (FP64)

@@ -1113,6 +1113,140 @@ __global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *
     pv[P] = (T)((double)r[P] + beta * ((double)pv[P] - scal[S_OMEGA] * (double)vv[P]));
 }
 
+// The finest level's up leg (k_mg_up with two post-sweeps) fused with the
+// operator application that follows the preconditioner in BiCGStab: the
+// prolongation and the two post-sweeps run on a region one node wider than
+// k_mg_up's (halos 3, 2 and 1), so that the output z (ph or sh) is known on the
+// tile and the ring around it; then y = L z on the tile, with the partials of
+// k_bi_v (MGA_V: (r0, y)) or k_bi_t (MGA_T: (y, s), (y, y); s is the level's
+// right-hand side).  Writes z and y.
+enum { MGA_V, MGA_T };
+template <typename T, int NTB, int MODE>
+__global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict__ yout, const T *__restrict__ r0,
+                                                     RedArgs ra) {
+    ra = rz(ra);
+    const double *guard = A.scal + zslot() * SS;
+    {
+        const size_t of = zslot() * A.a.n0 * A.a.n1, oc = zslot() * A.m0 * A.m1;
+        A.a = fz(A.a, of);
+        A.invd += of;
+        A.b += of;
+        A.x += of;
+        A.out += of;
+        A.xc += oc;
+        yout += of;
+        if (MODE == MGA_V) r0 += of;
+    }
+    constexpr int YX = MG_FX + 6, YY = MG_FY + 6;  // x + e, halo 3
+    constexpr int QX = MG_FX + 4, QY = MG_FY + 4;  // first post-sweep, halo 2
+    constexpr int ZX = MG_FX + 2, ZY = MG_FY + 2;  // second post-sweep (z), halo 1
+    constexpr int NV = MODE == MGA_V ? 1 : 2;
+    __shared__ T s_y[YX * YY];
+    __shared__ T s_q[QX * QY];
+    __shared__ T s_o[ZX * ZY];
+    const int tid = tlin();
+    const int n0 = A.a.n0, n1 = A.a.n1, m0 = A.m0, m1 = A.m1;
+    const int2 tl = xcd_tile();
+    const int fi0 = tl.x * MG_FX, fj0 = tl.y * MG_FY;
+    const T w = A.w;
+    const T *xc = A.xc;
+    struct YR {
+        T x, c00, c10, c01, c11;
+    };
+#define XC(a_, b_) xc[idx(clampi((a_), 1, m0 - 2), clampi((b_), 1, m1 - 2), m0)]
+    const auto ly = batch_load<YX * YY, NTB>(tid, [&](int k) {
+        const int gi = clampi(fi0 - 3 + k % YX, 1, n0 - 2), gj = clampi(fj0 - 3 + k / YX, 1, n1 - 2);
+        const int I = gi >> 1, J = gj >> 1;
+        return YR{A.x[idx(gi, gj, n0)], XC(I, J), XC(I + 1, J), XC(I, J + 1), XC(I + 1, J + 1)};
+    });
+#undef XC
+    // faces, right-hand side, 1/diag (and r0) of the first post-sweep region
+    struct PR {
+        RowFaces<T> f;
+        T b, id, r0;
+    };
+    auto q_load = [&]() {
+        return batch_load<QX * QY, NTB>(tid, [&](int k) {
+            const int gi = clampi(fi0 - 2 + k % QX, 1, n0 - 2);
+            const int gj = clampi(fj0 - 2 + k / QX, 1, n1 - 2);
+            const size_t P = idx(gi, gj, n0);
+            return PR{row_faces<T>(A.a, gi, gj), A.b[P], A.invd[P], MODE == MGA_V ? r0[P] : T(0)};
+        });
+    };
+    decltype(q_load()) lq;
+    if (mg_early<T>()) lq = q_load();
+    if (slot_skip(guard, A.gs)) return;
+    batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
+        const int gi = fi0 - 3 + k % YX, gj = fj0 - 3 + k / YX;
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            T e;
+            if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
+            else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
+            else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
+            else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
+            v = r.x + e;
+        }
+        s_y[k] = v;
+    });
+    if (!mg_early<T>()) lq = q_load();
+    __syncthreads();
+    // first post-sweep on the halo-2 region
+    batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
+        const int li = k % QX, lj = k / QX;
+        const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
+                return s_y[(jj - fj0 + 3) * YX + (ii - fi0 + 3)];
+            });
+            v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
+        }
+        s_q[k] = v;
+    });
+    __syncthreads();
+    // second post-sweep on the halo-1 region: z, written on the tile
+    batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
+        const int li = k % QX - 1, lj = k / QX - 1;
+        if (li < 0 || li >= ZX || lj < 0 || lj >= ZY) return;
+        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
+        T v = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
+                return s_q[(jj - fj0 + 2) * QX + (ii - fi0 + 2)];
+            });
+            v = s_q[(lj + 1) * QX + (li + 1)] + w * (r.b - ax) * r.id;
+        }
+        s_o[lj * ZX + li] = v;
+        if (li >= 1 && li <= MG_FX && lj >= 1 && lj <= MG_FY && gi < n0 && gj < n1) A.out[idx(gi, gj, n0)] = v;
+    });
+    __syncthreads();
+    // y = L z on the tile and the dot products
+    double dv[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) dv[q] = 0.0;
+    batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
+        const int li = k % QX - 2, lj = k / QX - 2;
+        if (li < 0 || li >= MG_FX || lj < 0 || lj >= MG_FY) return;
+        const int gi = fi0 + li, gj = fj0 + lj;
+        if (gi >= n0 || gj >= n1) return;
+        T y = T(0);
+        if (inner_node(gi, gj, n0, n1)) {
+            y = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
+                return s_o[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
+            });
+            if (MODE == MGA_V) {
+                dv[0] += (double)r.r0 * (double)y;
+            } else {
+                dv[0] += (double)y * (double)r.b;
+                dv[NV - 1] += (double)y * (double)y;
+            }
+        }
+        yout[idx(gi, gj, n0)] = y;
+    });
+    block_partials<NV, NTB>(dv, ra.part, ra.pstride);
+}
+
 // v = L ph; partials of (r0, v)
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict__ ph, T *__restrict__ vv,
@@ -1567,7 +1701,25 @@ struct PSolver {
 
     // z = M^-1 y: one V-cycle from zero with right-hand side y (slots whose
     // scalar `gs` is set are skipped)
-    int precond(PsCtx &cx, const T *y, T *z, int gs) {
+    // The operator application that follows a preconditioner call in BiCGStab,
+    // fused into the finest up leg when there is a fine level and two post-sweeps:
+    // y = L z with the partials of k_bi_v (MGA_V, with r0) or k_bi_t (MGA_T).
+    struct Tail {
+        int mode;
+        T *y;
+        const T *r0;
+        RedArgs ra;
+    };
+    bool fused_tail() const { return lc > 0 && post == 2; }
+    // blocks (partials) of the fused launch: level 0's 64 x 8 tiles
+    int tail_parts() const { return ((lev[0].n0 + MG_FX - 1) / MG_FX) * ((lev[0].n1 + MG_FY - 1) / MG_FY); }
+    template <int NTB>
+    static void launch_up_apply(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, const Tail &t, const Timed &tm) {
+        if (t.mode == MGA_V) launch_timed(tm, k_mg_up_apply<T, NTB, MGA_V>, gr, NTB, st, a, t.y, t.r0, t.ra);
+        else launch_timed(tm, k_mg_up_apply<T, NTB, MGA_T>, gr, NTB, st, a, t.y, t.r0, t.ra);
+    }
+
+    int precond(PsCtx &cx, const T *y, T *z, int gs, const Tail *tail = nullptr) {
         const T w = (T)omega;
         for (int l = 0; l < lc; l++) {
             OpLevel<T> &L = lev[l];
@@ -1591,6 +1743,12 @@ struct PSolver {
                         l == 0 ? z : L.t, w, cx.scal, gs};
             const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
             const Timed tm = l == 0 ? cx.prof(cx.owner, PK_MG_UP) : Timed{};
+            if (l == 0 && tail) {
+                if (wide_blocks(gr)) launch_up_apply<MG_NT_WIDE>(gr, cx.s, a, *tail, tm);
+                else launch_up_apply<MG_NT_NARROW>(gr, cx.s, a, *tail, tm);
+                CFD_LAUNCH_CHECK();
+                continue;
+            }
             switch (post) {
             case 0: launch_up_p<0>(gr, cx.s, a, wide_blocks(gr), tm); break;
             case 1: launch_up_p<1>(gr, cx.s, a, wide_blocks(gr), tm); break;
@@ -1620,6 +1778,8 @@ struct PSolver {
         gr.z = B;
         const dim3 bl(BX, BY), g1(1, 1, B);
         const int nb = nblk(n0, n1);
+        const bool fz = fused_tail();
+        const int nbt = fz ? tail_parts() : nb;  // partials of the v and t phases
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
@@ -1635,17 +1795,21 @@ struct PSolver {
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
-            CFD_RET(precond(cx, pv, ph, S_DONE));
-            launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
-                         (const T *)r0, ra);
-            k_bi_fin<FIN_V><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            const Tail tv{MGA_V, vv, r0, ra};
+            CFD_RET(precond(cx, pv, ph, S_DONE, fz ? &tv : nullptr));
+            if (!fz)
+                launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
+                             (const T *)r0, ra);
+            k_bi_fin<FIN_V><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
             k_bi_s<T><<<gr, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
             k_bi_fin<FIN_S><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
-            CFD_RET(precond(cx, s, sh, S_DONE));
-            launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
-                         (const T *)s, ra);
-            k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            const Tail tt{MGA_T, t, nullptr, ra};
+            CFD_RET(precond(cx, s, sh, S_DONE, fz ? &tt : nullptr));
+            if (!fz)
+                launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
+                             (const T *)s, ra);
+            k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
             k_bi_xr<T><<<gr, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
             // eager: the iteration's end flags go straight to host memory and the host
             // waits for them (no copy, no stream synchronisation)
