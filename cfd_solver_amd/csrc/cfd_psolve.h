@@ -118,14 +118,17 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *part, in
     }
 }
 
-// totals of NV partial arrays, summed by the whole NT-thread block in a fixed
-// order: each thread keeps eight running sums per value, and every load of a pass
-// over 8 * NT partials is issued before any is added (one memory round trip per
-// pass instead of one per partial); valid in thread 0
-template <int NV, int NT = PS_NT>
+// totals of NV partial arrays, summed by the first VS threads of the NT-thread
+// block in a fixed order: each of them keeps eight running sums per value, and
+// every load of a pass over 8 * VS partials is issued before any is added (one
+// memory round trip per pass instead of one per partial); valid in thread 0.
+// The other threads add zeros, so for a given VS the bits do not depend on NT.
+template <int NV, int NT = PS_NT, int VS = NT>
 __device__ __forceinline__ void sum_partials(const double *part, int pstride, int nparts, double (&tot)[NV]) {
+    static_assert(VS <= NT && VS % 64 == 0, "summing lanes");
     constexpr int U = 8;
     const int t = tlin();
+    const bool lane = t < VS;
     const int last = max(nparts - 1, 0);
     double s[NV][U];
     {  // the first pass, outside the loop: straight-line loads the compiler issues early
@@ -133,22 +136,22 @@ __device__ __forceinline__ void sum_partials(const double *part, int pstride, in
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(u * NT + t, last)];
+            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(u * VS + t, last)];
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int u = 0; u < U; u++) s[k][u] = u * NT + t < nparts ? v[k][u] : 0.0;
+            for (int u = 0; u < U; u++) s[k][u] = lane && u * VS + t < nparts ? v[k][u] : 0.0;
     }
-    for (int t0 = U * NT; t0 < nparts; t0 += U * NT) {
+    for (int t0 = U * VS; t0 < nparts; t0 += U * VS) {
         double v[NV][U];
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(t0 + u * NT + t, nparts - 1)];
+            for (int u = 0; u < U; u++) v[k][u] = part[(size_t)k * pstride + min(t0 + u * VS + t, nparts - 1)];
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int u = 0; u < U; u++) s[k][u] += t0 + u * NT + t < nparts ? v[k][u] : 0.0;
+            for (int u = 0; u < U; u++) s[k][u] += lane && t0 + u * VS + t < nparts ? v[k][u] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < NV; k++)
@@ -370,7 +373,11 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
 constexpr int MG_FX = 64, MG_FY = 8;   // fine tile
 constexpr int MG_NT_WIDE = 1024;       // threads per tile on levels with few tiles
 constexpr int MG_NT_NARROW = 512;      // threads per tile elsewhere
-constexpr long MG_WIDE_BELOW = 256;    // tiles per slot below which tiles get MG_NT_WIDE threads
+// Lanes that sum a tile's reductions: every tile runs at least this many threads,
+// and the sums use exactly these lanes in a fixed order, so a tile's partials have
+// the same bits whichever block size runs it.
+constexpr int MG_SUM_LANES = 256;
+constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
 
 template <typename T>
@@ -466,7 +473,7 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
     double mean = 0.0;
     if (MEAN) {
         double tot[1];
-        sum_partials<1, NTB>(A.bpart, 0, A.nbp, tot);
+        sum_partials<1, NTB, MG_SUM_LANES>(A.bpart, 0, A.nbp, tot);
         __shared__ double s_mean;
         if (tid == 0) s_mean = tot[0] / ((double)(n0 - 2) * (double)(n1 - 2));
         __syncthreads();
@@ -1144,6 +1151,7 @@ __global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict_
     __shared__ T s_y[YX * YY];
     __shared__ T s_q[QX * QY];
     __shared__ T s_o[ZX * ZY];
+    __shared__ double s_dp[NV][MG_FX * MG_FY];  // each tile node's dot-product terms
     const int tid = tlin();
     const int n0 = A.a.n0, n1 = A.a.n1, m0 = A.m0, m1 = A.m1;
     const int2 tl = xcd_tile();
@@ -1221,29 +1229,32 @@ __global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict_
         if (li >= 1 && li <= MG_FX && lj >= 1 && lj <= MG_FY && gi < n0 && gj < n1) A.out[idx(gi, gj, n0)] = v;
     });
     __syncthreads();
-    // y = L z on the tile and the dot products
-    double dv[NV];
-#pragma unroll
-    for (int q = 0; q < NV; q++) dv[q] = 0.0;
+    // y = L z on the tile and the dot products: each node's terms go to LDS and the
+    // first MG_SUM_LANES lanes add them in a fixed order (the same bits for any NTB)
     batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
         const int li = k % QX - 2, lj = k / QX - 2;
         if (li < 0 || li >= MG_FX || lj < 0 || lj >= MG_FY) return;
         const int gi = fi0 + li, gj = fj0 + lj;
-        if (gi >= n0 || gj >= n1) return;
-        T y = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            y = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                return s_o[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
-            });
-            if (MODE == MGA_V) {
-                dv[0] += (double)r.r0 * (double)y;
-            } else {
-                dv[0] += (double)y * (double)r.b;
-                dv[NV - 1] += (double)y * (double)y;
+        double d0 = 0.0, d1 = 0.0;
+        if (gi < n0 && gj < n1) {
+            T y = T(0);
+            if (inner_node(gi, gj, n0, n1)) {
+                y = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
+                    return s_o[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
+                });
+                d0 = MODE == MGA_V ? (double)r.r0 * (double)y : (double)y * (double)r.b;
+                d1 = (double)y * (double)y;
             }
+            yout[idx(gi, gj, n0)] = y;
         }
-        yout[idx(gi, gj, n0)] = y;
+        s_dp[0][lj * MG_FX + li] = d0;
+        if (NV == 2) s_dp[NV - 1][lj * MG_FX + li] = d1;
     });
+    __syncthreads();
+    static_assert(MG_FX * MG_FY == 2 * MG_SUM_LANES && NTB >= MG_SUM_LANES, "tile sum shape");
+    double dv[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) dv[q] = tid < MG_SUM_LANES ? s_dp[q][tid] + s_dp[q][tid + MG_SUM_LANES] : 0.0;
     block_partials<NV, NTB>(dv, ra.part, ra.pstride);
 }
 
@@ -1685,14 +1696,14 @@ struct PSolver {
         else launch_timed(tm, k_mg_up<T, NPOST, MG_NT_NARROW>, gr, MG_NT_NARROW, st, a);
     }
 
-    // A level with fewer tiles than CUs runs them with 1024 threads each: with one
-    // wavefront per SIMD a tile's phases are bound by that wavefront's serial
-    // instruction latency, so four times the threads cut it about fourfold.  Larger
-    // levels use 512 threads per tile (measured ~1% faster per outer iteration than
-    // 256 at 1025^2; 128 is ~4% slower).  The choice counts one slot's tiles only,
-    // so a case computes the same bits alone and in a batch (the block size fixes
-    // the order of the partial sums).
-    static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y < MG_WIDE_BELOW; }
+    // A launch with fewer tiles (over all slots) than CUs runs them with 1024
+    // threads each: with one wavefront per SIMD a tile's phases are bound by that
+    // wavefront's serial instruction latency, so four times the threads cut it
+    // about fourfold.  Larger launches use 512 threads per tile (measured ~1% faster
+    // per outer iteration than 256 at 1025^2; 128 is ~4% slower).  A tile's sums
+    // use MG_SUM_LANES lanes whatever the block size, so a case still computes the
+    // same bits alone and in a batch.
+    static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
 
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {
