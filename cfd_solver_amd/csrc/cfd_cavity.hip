@@ -777,34 +777,37 @@ __global__ void k_uvt_walls(CGrid g, double *__restrict__ u, double *__restrict_
     }
 }
 
-// wall Nusselt number of the bottom (wall 0) or top (wall 1) wall (oracle: nusselt),
-// summed by the calling NTB-thread block
-template <int NTB>
-__device__ double nusselt_wall(KGrid g, const double *__restrict__ T, const double *__restrict__ ajac,
-                               const double *__restrict__ etx, const double *__restrict__ ety, int wall) {
-    const int n0 = g.n0, n1 = g.n1;
-    const int j = wall ? n1 - 1 : 0, s = wall ? -1 : 1;
-    double acc = 0.0;
-    for (int i = tlin(); i < n0 - 1; i += NTB) {
-        const int ip = i + 1;
-        const size_t A = idx(i, j, n0), B = idx(ip, j, n0);
-        const double tj1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
-        const double tj2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
-        const double fh1 = tj1 * (4 * T[idx(i, j + s, n0)] - 3 * T[A] - T[idx(i, j + 2 * s, n0)]) / (2 * g.h2);
-        const double fh2 = tj2 * (4 * T[idx(ip, j + s, n0)] - 3 * T[B] - T[idx(ip, j + 2 * s, n0)]) / (2 * g.h2);
-        acc += 0.5 * (fh1 + fh2) * g.h1;
-    }
-    return block_sum<NTB>(acc);
-}
-
 // Partial-sum slots (pstride each) per case slot: 0-2 for reductions finished
 // right away, 3-4 for the energy/momentum residual norms and 5 for the mass
 // residual, which stay there until the end-of-iteration finish below.
 enum { PS_RES = 3, PS_MASS = 5, PS_SLOTS = 6 };
 
+// Wall-Nusselt integrand of segment [i, i + 1] of the bottom (s = 1, j = 0) or top
+// (s = -1, j = n1 - 1) wall from its twelve loaded values (oracle: nusselt).
+struct NuSeg {
+    double ja, xa, ya, jb, xb, yb, t0a, t1a, t2a, t0b, t1b, t2b;
+};
+__device__ __forceinline__ NuSeg nu_load(const KGrid &g, const double *__restrict__ T,
+                                         const double *__restrict__ ajac, const double *__restrict__ etx,
+                                         const double *__restrict__ ety, int i, int j, int s) {
+    const int n0 = g.n0;
+    const size_t A = idx(i, j, n0), B = idx(i + 1, j, n0);
+    return NuSeg{ajac[A], etx[A], ety[A], ajac[B], etx[B], ety[B], T[A], T[idx(i, j + s, n0)],
+                 T[idx(i, j + 2 * s, n0)], T[B], T[idx(i + 1, j + s, n0)], T[idx(i + 1, j + 2 * s, n0)]};
+}
+__device__ __forceinline__ double nu_term(const KGrid &g, const NuSeg &q) {
+    const double tj1 = q.ja * (q.xa * q.xa + q.ya * q.ya);
+    const double tj2 = q.jb * (q.xb * q.xb + q.yb * q.yb);
+    const double fh1 = tj1 * (4 * q.t1a - 3 * q.t0a - q.t2a) / (2 * g.h2);
+    const double fh2 = tj2 * (4 * q.t1b - 3 * q.t0b - q.t2b) / (2 * g.h2);
+    return 0.5 * (fh1 + fh2) * g.h1;
+}
+
 // one block: dmax from the partials, floored at 0 (solver.cpp:1806), both wall
 // Nusselt numbers, and the energy, momentum (nres partials) and mass (nparts
-// partials) residual norms of the iteration
+// partials) residual norms of the iteration.  Every thread first issues the loads
+// of its share of all five quantities (one memory round trip for grids up to
+// 5 * FIN_NT partials and FIN_NT wall segments), then one block reduction.
 __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, int nparts, int nres,
                                                         const double *__restrict__ T,
                                                         const double *__restrict__ ajac,
@@ -821,15 +824,71 @@ __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, in
         const size_t o = zslot() * cg.N;
         T += o; ajac += o; etx += o; ety += o;
     }
-    double res[2], mass[1];
-    sum_partials<2, FIN_NT>(part + (size_t)PS_RES * ra.pstride, ra.pstride, nres, res);
-    sum_partials<1, FIN_NT>(part + (size_t)PS_MASS * ra.pstride, ra.pstride, nparts, mass);
+    constexpr int U = 5;
+    const int t = tlin();
+    const size_t ps = ra.pstride;
+    const double *pr0 = part + (size_t)PS_RES * ps, *pr1 = pr0 + ps, *pm = part + (size_t)PS_MASS * ps;
+    const int lr = max(nres - 1, 0), lp = max(nparts - 1, 0), ns = g.n0 - 1;
+    // first pass: every load before any use
+    double r0[U], r1[U], mm[U], dx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        r0[u] = pr0[min(u * FIN_NT + t, lr)];
+        r1[u] = pr1[min(u * FIN_NT + t, lr)];
+        mm[u] = pm[min(u * FIN_NT + t, lp)];
+        dx[u] = part[min(u * FIN_NT + t, lp)];
+    }
+    const int i0 = min(t, max(ns - 1, 0));
+    const NuSeg qh = nu_load(g, T, ajac, etx, ety, i0, 0, 1);
+    const NuSeg qc = nu_load(g, T, ajac, etx, ety, i0, g.n1 - 1, -1);
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // energy, momentum, mass, Nu hot, Nu cold
     double m = -1.0e300;
-    for (int t = tlin(); t < nparts; t += FIN_NT) m = fmax(m, part[t]);
-    m = block_max<FIN_NT>(m);
-    const double nh = nusselt_wall<FIN_NT>(g, T, ajac, etx, ety, 0);
-    const double nc = nusselt_wall<FIN_NT>(g, T, ajac, etx, ety, 1);
-    if (tlin() == 0) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (u * FIN_NT + t < nres) {
+            v[0] += r0[u];
+            v[1] += r1[u];
+        }
+        if (u * FIN_NT + t < nparts) {
+            v[2] += mm[u];
+            m = fmax(m, dx[u]);
+        }
+    }
+    if (t < ns) {
+        v[3] += nu_term(g, qh);
+        v[4] += nu_term(g, qc);
+    }
+    // larger grids: the rest, pass by pass
+    for (int k = U * FIN_NT + t; k < nres; k += FIN_NT) {
+        v[0] += pr0[k];
+        v[1] += pr1[k];
+    }
+    for (int k = U * FIN_NT + t; k < nparts; k += FIN_NT) {
+        v[2] += pm[k];
+        m = fmax(m, part[k]);
+    }
+    for (int i = FIN_NT + t; i < ns; i += FIN_NT) {
+        v[3] += nu_term(g, nu_load(g, T, ajac, etx, ety, i, 0, 1));
+        v[4] += nu_term(g, nu_load(g, T, ajac, etx, ety, i, g.n1 - 1, -1));
+    }
+    // one block reduction of the five sums and the maximum
+    __shared__ double s_v[6][FIN_NT / 64];
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] = wave_sum(v[q]);
+    m = wave_max(m);
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) s_v[q][t >> 6] = v[q];
+        s_v[5][t >> 6] = m;
+    }
+    __syncthreads();
+    if (t >= 64) return;
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] = wave_sum(t < FIN_NT / 64 ? s_v[q][t] : 0.0);
+    m = wave_max(t < FIN_NT / 64 ? s_v[5][t] : -1.0e300);
+    const double res[2] = {v[0], v[1]}, mass[1] = {v[2]};
+    const double nh = v[3], nc = v[4];
+    if (t == 0) {
         const double den = dT != 0.0 ? dT : 1.0;
         scal[S_ENERGY] = res[0];
         scal[S_MOM] = res[1];
