@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int npar
     if (skip || tlin() != 0) return;
     double *s = ra.scal;
     const double nin = (double)(n0 - 2) * (double)(n1 - 2);
-    if (OP == CF_DEFECT_PSUM) s[S_PSUM] = tot[2];
+    if constexpr (OP == CF_DEFECT_PSUM) s[S_PSUM] = tot[NV - 1];
     if (OP == CF_PSUM) {
         s[S_PSUM] = tot[0];
     } else if (OP == CF_Q) {
@@ -2022,7 +2022,6 @@ int cfd_case_profile_read(cfd_case h, double *ms_total, long *launches, int nkin
 int cfd_case_time_kernel(cfd_case h, int kernel, int reps, double *ms_per_launch) {
     Case *c = reinterpret_cast<Case *>(h);
     if (!c || reps < 1 || !ms_per_launch || kernel < 0 || kernel > 5) return CFD_E_ARG;
-    const dim3 gr = grid_b(c), bl(BX, BY);
     hipEvent_t e0, e1;
     CFD_HIP(hipEventCreate(&e0));
     CFD_HIP(hipEventCreate(&e1));

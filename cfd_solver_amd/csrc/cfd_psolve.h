@@ -308,27 +308,37 @@ __device__ __forceinline__ void p9_face_of(const FNode &P, const FNode &Q, int d
 // Face coefficients of every node of every level, and the inverse diagonal of
 // the clamped operator (every coefficient whose neighbour clamps onto the node)
 // at interior nodes: the thread also evaluates its west neighbour's east face and
-// its south neighbour's north face, so the row needs no second launch.  The five
-// nodes' metric terms and d are loaded once (clamped into the grid, no branch
-// before the loads); a face touching a wall node is zeroed after its arithmetic.
+// its south neighbour's north face, so the row needs no second launch.  A block's
+// BX x BY tile and a one-node ring have their metric terms and d staged in LDS
+// (loads clamped into the grid, all issued before any is used); a face touching a
+// wall node is zeroed after its arithmetic.
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<T> A) {
     if (slot_skip(scal + zslot() * SS, -1)) return;
     int l, i, j;
-    if (!lev_of_block(A, l, i, j)) return;
+    const bool own = lev_of_block(A, l, i, j);
     const int n0 = A.n0[l], n1 = A.n1[l];
     const size_t o = zslot() * n0 * n1, o0 = zslot() * A.dn0 * A.n1[0];
-    const size_t P = o + idx(i, j, n0);
+    const size_t P = o + idx(min(i, n0 - 1), min(j, n1 - 1), n0);
     const double *d0 = A.d0 + o0;
     const double *xix = A.xix[l] + o, *xiy = A.xiy[l] + o, *etx = A.etx[l] + o, *ety = A.ety[l] + o,
                  *jac = A.jac[l] + o;
     const double r1 = A.ih1sq[l], r2 = A.ih2sq[l], r12 = A.i4h1h2[l];
     const int dn0 = A.dn0;
-    const FNode nP = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j);
-    const FNode nE = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i + 1, j);
-    const FNode nW = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i - 1, j);
-    const FNode nN = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j + 1);
-    const FNode nS = fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i, j - 1);
+    constexpr int SX = BX + 2, SY = BY + 2;
+    __shared__ FNode s_n[SX * SY];
+    {
+        const int tid = threadIdx.x;
+        const int i0 = i - tid % BX, j0 = j - tid / BX;  // the tile's origin
+        const auto ln = batch_load<SX * SY, PS_NT>(tid, [&](int k) {
+            return fnode_load(xix, xiy, etx, ety, jac, d0, n0, n1, dn0, l, i0 - 1 + k % SX, j0 - 1 + k / SX);
+        });
+        batch_run<SX * SY, PS_NT>(tid, ln, [&](int k, const FNode &v) { s_n[k] = v; });
+    }
+    __syncthreads();
+    if (!own) return;
+    const int sc = (threadIdx.x / BX + 1) * SX + threadIdx.x % BX + 1;
+    const FNode nP = s_n[sc], nE = s_n[sc + 1], nW = s_n[sc - 1], nN = s_n[sc + SX], nS = s_n[sc - SX];
     double en, et, nn, nt, wn, wt, sn, st;
     p9_face_of(nP, nE, 0, r1, r2, r12, en, et);
     p9_face_of(nP, nN, 1, r1, r2, r12, nn, nt);
