@@ -50,7 +50,7 @@ class Stats(C.Structure):
 class RefParams(C.Structure):
     _fields_ = [("Re", _d), ("Pr", _d), ("Ri", _d), ("dt", _d), ("uinf", _d), ("vinf", _d),
                 ("speed_amp", _d), ("accn_amp", _d), ("F", _d), ("time", _d), ("mom_rtol", _d),
-                ("p_rtol", _d), ("mom_maxit", _i), ("p_maxit", _i)]
+                ("p_rtol", _d), ("mom_maxit", _i), ("p_maxit", _i), ("time_accurate", _i)]
 
 
 class RefStats(C.Structure):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "cfd_ref_case_get_state": ([_p] + [dp] * 6, _i),
     "cfd_ref_case_step": ([_p, _i, C.POINTER(RefStats)], _i),
     "cfd_ref_case_destroy": ([_p], None),
+    "cfd_ref_case_time": ([_p, C.POINTER(_d)], _i),
     "cfd_params_default": ([C.POINTER(Params)], None),
     "cfd_case_create": ([_i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), C.POINTER(_p)], _i),
     "cfd_case_destroy": ([_p], _i),

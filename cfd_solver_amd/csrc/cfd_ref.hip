@@ -969,6 +969,7 @@ int ref_case_step(RefCase *c, cfd_ref_stats *st) {
     const cfd_ref_params &P = c->prm;
     hipStream_t s = c->s;
     cfd_ref_stats o{};
+    if (P.time_accurate) c->prm.time += P.dt;  // 1058, with time a double
     // 1062-1068: contravariant velocities and old T; 1074-1236: right-hand sides
     k_contra<<<gr, bl, 0, s>>>(n0, n1, c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->uxi, c->uet);
     CFD_HIP(hipMemcpyAsync(c->Told, c->T, B, hipMemcpyDeviceToDevice, s));
@@ -1362,6 +1363,13 @@ int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last) {
     if (!c || nsteps < 0) return CFD_E_ARG;
     if (!c->have_state) return CFD_E_STATE;
     for (int k = 0; k < nsteps; k++) CFD_RET(ref_case_step(c, last));
+    return CFD_OK;
+}
+
+int cfd_ref_case_time(cfd_ref_case h, double *time) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c || !time) return CFD_E_ARG;
+    *time = c->prm.time;
     return CFD_OK;
 }
 

@@ -19,10 +19,18 @@ from ._lib import RefParams, RefStats, call, f64
 
 
 def ref_params(**kw) -> RefParams:
-    """The reference's compiled-in constants (solver.cpp:159-173) plus solver tolerances."""
+    """The reference's compiled-in constants (solver.cpp:159-173) plus solver tolerances.
+    Given F alone, the oscillation amplitudes follow the reference's definitions
+    (solver.cpp:163-164): speed_amp = (pi/12) 2 pi F, accn_amp = 2 pi F speed_amp."""
     a = 82.0 * math.pi / 180.0
     base = dict(Re=1000.0, Pr=0.71, Ri=0.0, dt=1e-4, uinf=math.sin(a), vinf=math.cos(a), speed_amp=0.0,
-                accn_amp=0.0, F=0.0, time=0.0, mom_rtol=1e-12, p_rtol=1e-12, mom_maxit=500, p_maxit=2000)
+                accn_amp=0.0, F=0.0, time=0.0, mom_rtol=1e-12, p_rtol=1e-12, mom_maxit=500, p_maxit=2000,
+                time_accurate=0)
+    F = kw.get("F", 0.0)
+    if F and "speed_amp" not in kw:
+        pi = math.acos(-1.0)
+        base["speed_amp"] = pi / 12.0 * 2.0 * pi * F
+        base["accn_amp"] = 2.0 * pi * F * base["speed_amp"]
     base.update(kw)
     return RefParams(**base)
 
@@ -58,6 +66,11 @@ class RefCase:
         st = RefStats()
         call("cfd_ref_case_step", self.h, int(nsteps), C.byref(st))
         return dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres))
+
+    def time(self):
+        t = C.c_double()
+        call("cfd_ref_case_time", self.h, C.byref(t))
+        return t.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -182,13 +182,17 @@ int cfd_ref_nusselt(int n0, int n1, double dxi, double deta, const double *T,
  * Gauss-Seidel (solver.cpp:2279) and SIP (solver.cpp:2066) solves, which stop at
  * 0.75e-2 of the first L1 residual, are replaced by BiCGStab to the relative
  * 2-norm tolerances below: same systems, same fixed points.  time is held fixed
- * (the reference declares it int, so time += dt leaves it at 0). */
+ * as in the reference (it declares time int, so time += dt leaves it at 0) unless
+ * time_accurate is set: then each step first advances time by dt
+ * (solver.cpp:1058 with a double time), and the oscillating cylinder's wall
+ * velocity and acceleration (solver.cpp:1616-1629, 1715-1718) follow it. */
 typedef struct {
     double Re, Pr, Ri, dt;           /* the reference's defaults: 1000, 0.71, 0, 1e-4      */
     double uinf, vinf;               /* free stream (sin 82 deg, cos 82 deg)              */
     double speed_amp, accn_amp, F, time;  /* cylinder oscillation (0 when F = 0)          */
     double mom_rtol, p_rtol;         /* BiCGStab tolerances of the u/v/T/predictor and p' solves */
     int mom_maxit, p_maxit;
+    int time_accurate;               /* 1: time += dt at the start of every step               */
 } cfd_ref_params;
 
 typedef struct {
@@ -215,6 +219,8 @@ int cfd_ref_case_get_state(cfd_ref_case h, double *u, double *v, double *T, doub
                            double *up, double *vp);
 /* nsteps time steps; last (may be NULL) receives the last step's statistics. */
 int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last);
+/* The case's current time (advanced by the steps when time_accurate is set). */
+int cfd_ref_case_time(cfd_ref_case h, double *time);
 void cfd_ref_case_destroy(cfd_ref_case h);
 
 /* ------------------------------------------------------------------ */

@@ -69,3 +69,31 @@ def test_step_with_oscillating_cylinder(golden):
     np.testing.assert_allclose(f["u"][0], -0.05 * golden["grid_y"][0], rtol=0, atol=1e-15)
     np.testing.assert_allclose(f["v"][0], 0.05 * golden["grid_x"][0], rtol=0, atol=1e-15)
     c.close()
+
+
+def test_time_accurate_oscillating_cylinder(golden):
+    """time_accurate: every step advances time by dt first, so the cylinder's wall velocity
+    (-a cos(2 pi F t) y, a cos(2 pi F t) x) and the wall pressure's local acceleration
+    (accn_amp sin(2 pi F t)) change from step to step; three steps against the oracle's
+    step taking the same times."""
+    import math
+    from oracle.oracle import ref_step
+    from tests.ref_bc_cases import golden_step_inputs
+    print("\n[gpu] reference O-grid, time-accurate, F 200, 3 steps", flush=True)
+    F, amp = 200.0, 0.05
+    acc = 2.0 * math.pi * F * amp
+    c = _gpu_case(golden, speed_amp=amp, accn_amp=acc, F=F, time_accurate=1)
+    c.step(3)
+    g, ops, s, prm = golden_step_inputs(golden)
+    for k in range(1, 4):
+        o = ref_step(g, ops, s, dict(prm, speed_amp=amp, accn_amp=acc, F=F, time=k * prm["dt"]), inner_calls=12)
+        s = {q: o[q] for q in FIELDS}
+    f = c.state()
+    for q in FIELDS:
+        assert rel_l2(f[q], o[q]) < 1e-9, (q, rel_l2(f[q], o[q]))
+    t = c.time()
+    assert abs(t - 3 * golden["dt"]) < 1e-18
+    cw = math.cos(2.0 * math.acos(-1.0) * F * t)
+    np.testing.assert_allclose(f["u"][0], -amp * cw * golden["grid_y"][0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f["v"][0], amp * cw * golden["grid_x"][0], rtol=0, atol=1e-15)
+    c.close()
