@@ -50,11 +50,13 @@ class Stats(C.Structure):
 class RefParams(C.Structure):
     _fields_ = [("Re", _d), ("Pr", _d), ("Ri", _d), ("dt", _d), ("uinf", _d), ("vinf", _d),
                 ("speed_amp", _d), ("accn_amp", _d), ("F", _d), ("time", _d), ("mom_rtol", _d),
-                ("p_rtol", _d), ("mom_maxit", _i), ("p_maxit", _i), ("time_accurate", _i)]
+                ("p_rtol", _d), ("mom_maxit", _i), ("p_maxit", _i), ("time_accurate", _i),
+                ("alpha_deg", _d), ("ar", _d)]
 
 
 class RefStats(C.Structure):
-    _fields_ = [("dmax", _d), ("circ", _d), ("iters", _i * 6), ("relres", _d * 6)]
+    _fields_ = [("dmax", _d), ("circ", _d), ("iters", _i * 6), ("relres", _d * 6), ("cl", _d), ("cd", _d),
+                ("cm", _d), ("nu", _d)]
 
 
 # every symbol include/cfd_gpu.h declares, with its ctypes signature
@@ -82,6 +84,10 @@ SIGNATURES = {
     "cfd_ref_case_step": ([_p, _i, C.POINTER(RefStats)], _i),
     "cfd_ref_case_destroy": ([_p], None),
     "cfd_ref_case_time": ([_p, C.POINTER(_d)], _i),
+    "cfd_ref_case_get_post": ([_p, dp, dp], _i),
+    "cfd_ref_stream_function": ([_i, _i, _d] + [dp] * 5 + [dp], _i),
+    "cfd_ref_vorticity": ([_i, _i, _d, _d] + [dp] * 6 + [dp], _i),
+    "cfd_ref_forces": ([_i, _i, _d, _d] + [dp] * 8 + [_d, _d, _d, dp], _i),
     "cfd_params_default": ([C.POINTER(Params)], None),
     "cfd_case_create": ([_i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), C.POINTER(_p)], _i),
     "cfd_case_destroy": ([_p], _i),

@@ -502,6 +502,97 @@ __global__ void k_exit_p_ref(int n0, int n1, double h1, double h2, ExitRef w, do
     if (i == 0) p[ix(n0 - 1, j, n0)] = val;
 }
 
+// ---------------------------------------------------------------- post-processing
+// Stream function (solver.cpp:1784-1800): one thread per eta line, the trapezoid
+// sum of the J-weighted contravariant flux along it in the reference's order.
+__global__ void k_si_ref(int n0, int n1, double h2, const double *__restrict__ u, const double *__restrict__ v,
+                         const double *__restrict__ xix, const double *__restrict__ xiy,
+                         const double *__restrict__ ajac, double *__restrict__ si) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n0) return;
+    double acc = 0.0;
+    si[ix(i, 0, n0)] = 0.0;
+    for (int j = 1; j < n1; j++) {
+        const size_t P = ix(i, j, n0), Q = ix(i, j - 1, n0);
+        const double ca = (xix[P] * u[P] * fabs(ajac[P]) + xix[Q] * u[Q] * fabs(ajac[Q]));
+        const double cb = (xiy[P] * v[P] * fabs(ajac[P]) + xiy[Q] * v[Q] * fabs(ajac[Q]));
+        acc = acc + (ca + cb) * 0.5 * h2;
+        si[P] = acc;
+    }
+}
+
+// Vorticity dv/dx - du/dy (solver.cpp:1802-1887): central differences inside,
+// one-sided eta differences on the wall and outer rows; column n0-1 duplicates 0.
+__global__ void k_vort_ref(int n0, int n1, double h1, double h2, const double *__restrict__ u,
+                           const double *__restrict__ v, const double *__restrict__ xix,
+                           const double *__restrict__ xiy, const double *__restrict__ etx,
+                           const double *__restrict__ ety, double *__restrict__ vort) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (i > n0 - 2 || j >= n1) return;
+    const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+    const size_t P = ix(i, j, n0);
+    double dv_det, du_det;
+    if (j == 0) {
+        dv_det = 1.0 / h2 * (v[ix(i, j + 1, n0)] - v[P]);
+        du_det = 1.0 / h2 * (u[ix(i, j + 1, n0)] - u[P]);
+    } else if (j == n1 - 1) {
+        dv_det = 1.0 / h2 * (v[P] - v[ix(i, j - 1, n0)]);
+        du_det = 1.0 / h2 * (u[P] - u[ix(i, j - 1, n0)]);
+    } else {
+        dv_det = 0.5 / h2 * (v[ix(i, j + 1, n0)] - v[ix(i, j - 1, n0)]);
+        du_det = 0.5 / h2 * (u[ix(i, j + 1, n0)] - u[ix(i, j - 1, n0)]);
+    }
+    const double dv_dx = xix[P] * (0.5 / h1 * (v[ix(ipp, j, n0)] - v[ix(inn, j, n0)])) + etx[P] * dv_det;
+    const double du_dy = xiy[P] * (0.5 / h1 * (u[ix(ipp, j, n0)] - u[ix(inn, j, n0)])) + ety[P] * du_det;
+    const double w = dv_dx - du_dy;
+    vort[P] = w;
+    if (i == 0) vort[ix(n0 - 1, j, n0)] = w;
+}
+
+// Force and moment coefficients and the mean wall Nusselt number from the wall
+// row (solver.cpp:1891-1982), one block: the seven trapezoid integrals, then
+// out = cl, cd, cm, CL_pr, CD_pr, CL_vor, CD_vor, Nu.  sa, ca: sin and cos of the
+// flow angle; nu_den: the ellipse-perimeter factor of Nu (both from the host).
+struct ForceRef {
+    const double *x, *y, *p, *vort, *T, *ajac, *etx, *ety;
+};
+__global__ void k_forces_ref(int n0, double h1, double h2, ForceRef a, double Re, double sa, double ca,
+                             double nu_den, double *__restrict__ out) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};  // pr_x, pr_y, vor_x, vor_y, press_i, vor_i, temp_i
+    for (int i = threadIdx.x; i < n0 - 1; i += 256) {
+        const size_t A = i, B = i + 1;  // row j = 0
+        const double PJ1 = a.p[A] * a.ajac[A], pj2 = a.p[B] * a.ajac[B];
+        const double VJ1 = a.vort[A] * a.ajac[A], VJ2 = a.vort[B] * a.ajac[B];
+        v[0] += 0.5 * h1 * (PJ1 * a.etx[A] + pj2 * a.etx[B]);
+        v[1] += 0.5 * h1 * (PJ1 * a.ety[A] + pj2 * a.ety[B]);
+        v[2] += 0.5 * h1 * (VJ1 * a.ety[A] + VJ2 * a.ety[B]);
+        v[3] += 0.5 * h1 * (VJ1 * a.etx[A] + VJ2 * a.etx[B]);
+        const double TJ1 = a.ajac[A] * (a.etx[A] * a.etx[A] + a.ety[A] * a.ety[A]);
+        const double TJ2 = a.ajac[B] * (a.etx[B] * a.etx[B] + a.ety[B] * a.ety[B]);
+        const double fp1 = PJ1 * (a.x[A] * a.ety[A] - a.y[A] * a.etx[A]);
+        const double fp2 = pj2 * (a.x[B] * a.ety[B] - a.y[B] * a.etx[B]);
+        const double fv1 = VJ1 * (a.x[A] * a.etx[A] + a.y[A] * a.ety[A]);
+        const double fv2 = VJ2 * (a.x[B] * a.etx[B] + a.y[B] * a.ety[B]);
+        const double fh1 = TJ1 * (4 * a.T[n0 + A] - 3 * a.T[A] - a.T[2 * (size_t)n0 + A]) / (2 * h2);
+        const double fh2 = TJ2 * (4 * a.T[n0 + B] - 3 * a.T[B] - a.T[2 * (size_t)n0 + B]) / (2 * h2);
+        v[4] += 0.5 * h1 * (fp1 + fp2);
+        v[5] += 0.5 * h1 * (fv1 + fv2);
+        v[6] += 0.5 * (fh1 + fh2) * h1;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[k] = block_sum<256>(v[k]);
+    if (threadIdx.x != 0) return;
+    const double cx = 2 * v[0] + (2.0 / Re) * v[2], cy = 2 * v[1] - (2.0 / Re) * v[3];
+    out[0] = cy * sa - cx * ca;
+    out[1] = cy * ca + cx * sa;
+    out[2] = 2 * v[4] - (2.0 / Re) * v[5];
+    out[3] = 2 * v[1] * sa - 2 * v[0] * ca;
+    out[4] = 2 * v[1] * ca + 2 * v[0] * sa;
+    out[5] = -(2.0 / Re) * v[3] * sa - (2.0 / Re) * v[2] * ca;
+    out[6] = -(2.0 / Re) * v[3] * ca + (2.0 / Re) * v[2] * sa;
+    out[7] = (2 * v[6]) / nu_den;
+}
+
 // ---------------------------------------------------------------- a12, a13
 __global__ void k_dil_ref(int n0, int n1, double h1, double h2, const double *__restrict__ u,
                           const double *__restrict__ v, const double *__restrict__ xix,
@@ -885,6 +976,17 @@ int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double
     return down(phi, x, N);
 }
 
+// sin and cos of the flow angle and the ellipse-perimeter factor of the Nusselt
+// number, evaluated on the host as the reference does (solver.cpp:1927-1982)
+struct ForceConst {
+    double sa, ca, nu_den;
+};
+ForceConst force_const(double alpha_deg, double ar) {
+    const double Pi = acos(-1.0);
+    return ForceConst{sin(alpha_deg * Pi / 180.0), cos(alpha_deg * Pi / 180.0),
+                      Pi * (3 * (1 + (1.0 / ar)) - sqrt((3 + (1.0 / ar)) * ((3.0 / ar) + 1)))};
+}
+
 // ---------------------------------------------------------------- device-resident time steps
 // Local accelerations of the wall row (solver.cpp:1715-1718): a_s = accn_amp sin(2 pi F t).
 __global__ void k_alc(int n0, const double *__restrict__ x, const double *__restrict__ y, double a_s,
@@ -906,15 +1008,17 @@ struct RefCase {
     double *x, *y, *xix, *xiy, *etx, *ety, *alph, *beta, *gamma, *ajac, *p1, *q1, *xnox, *xnoy;
     double *AU, *AT, *BU, *BT, *A9;
     double *u, *v, *T, *p, *up, *vp, *us, *vs, *uold, *vold, *Told, *uxi, *uet, *qu, *qv, *qt, *qup, *qvp, *q,
-        *pcor, *dil, *sol, *alc, *part, *scal, *ws;
-    double *hscal = nullptr;  // pinned: circulation, dmax
+        *pcor, *dil, *sol, *si, *vort, *alc, *part, *scal, *ws;
+    // pinned: circulation, dmax, the eight force coefficients (0-9); the inner
+    // solver's flags (12-14)
+    double *hscal = nullptr;
     bool have_state = false;
 };
 
 size_t ref_case_doubles(int n0, int n1) {
     const size_t N = (size_t)n0 * n1;
     const dim3 gr = grid2d(n0, n1);
-    return 12 * N + 2 * (size_t)n0 + 59 * N + 12 * (size_t)n0 + 22 * N + 2 * (size_t)n0 + (size_t)gr.x * gr.y + 8 +
+    return 12 * N + 2 * (size_t)n0 + 59 * N + 12 * (size_t)n0 + 24 * N + 2 * (size_t)n0 + (size_t)gr.x * gr.y + 16 +
            SopWork::doubles(n0, n1) + 64;
 }
 
@@ -936,13 +1040,14 @@ void ref_case_carve(RefCase *c) {
     c->A9 = take(9 * N);
     c->BU = take(6 * m);
     c->BT = take(6 * m);
-    double **st[22] = {&c->u,  &c->v,  &c->T,   &c->p,   &c->up,  &c->vp,  &c->us, &c->vs,   &c->uold, &c->vold, &c->Told,
-                       &c->uxi, &c->uet, &c->qu, &c->qv, &c->qt, &c->qup, &c->qvp, &c->q, &c->pcor, &c->dil, &c->sol};
+    double **st[24] = {&c->u,   &c->v,   &c->T,  &c->p,  &c->up, &c->vp,  &c->us,  &c->vs,
+                       &c->uold, &c->vold, &c->Told, &c->uxi, &c->uet, &c->qu, &c->qv, &c->qt,
+                       &c->qup, &c->qvp, &c->q,  &c->pcor, &c->dil, &c->sol, &c->si, &c->vort};
     for (double **f : st) *f = take(N);
     c->alc = take(2 * m);
     const dim3 gr = grid2d(c->n0, c->n1);
     c->part = take((size_t)gr.x * gr.y);
-    c->scal = take(8);
+    c->scal = take(16);
     c->ws = take(SopWork::doubles(c->n0, c->n1));
 }
 
@@ -953,7 +1058,7 @@ int ref_solve_rows(RefCase *c, const double *A, const double *rhs, double *dst, 
                    double *rr) {
     const size_t N = (size_t)c->n0 * c->n1;
     CFD_RET(sop_core<R>(c->n0, c->n1, A, c->sol, rhs, SopWork::carve(c->ws, c->n0, c->n1), rtol, maxit, it, rr,
-                        c->s, c->hscal + 4));
+                        c->s, c->hscal + 12));
     CFD_HIP(hipMemcpyAsync(dst + c->n0, c->sol + c->n0, (N - 2 * (size_t)c->n0) * sizeof(double),
                            hipMemcpyDeviceToDevice, c->s));
     return CFD_OK;
@@ -1004,7 +1109,7 @@ int ref_case_step(RefCase *c, cfd_ref_stats *st) {
     CFD_HIP(hipMemcpyAsync(c->uold, c->u, B, hipMemcpyDeviceToDevice, s));
     CFD_HIP(hipMemcpyAsync(c->vold, c->v, B, hipMemcpyDeviceToDevice, s));
     CFD_RET(sop_core<1>(n0, n1, c->A9, c->pcor, c->q, SopWork::carve(c->ws, n0, n1), P.p_rtol, P.p_maxit,
-                        &o.iters[5], &o.relres[5], s, c->hscal + 4));
+                        &o.iters[5], &o.relres[5], s, c->hscal + 12));
     k_pcor_bc<<<g1, 256, 0, s>>>(n0, n1, c->pcor, c->xnox, c->xnoy, P.uinf, P.vinf);
     // 1504-1655: correction, far-field and wall velocities
     k_correct_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->us, c->vs, c->pcor, c->xix, c->xiy, c->etx, c->ety,
@@ -1031,11 +1136,22 @@ int ref_case_step(RefCase *c, cfd_ref_stats *st) {
     // 1806-1852: dilation and its maximum
     k_dil_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->dil, c->part);
     k_max_final<<<1, 256, 0, s>>>(c->part, nb, c->scal + 1);
+    // 1784-1982: stream function, vorticity, force coefficients and Nusselt number
+    k_si_ref<<<g1, 256, 0, s>>>(n0, n1, c->deta, c->u, c->v, c->xix, c->xiy, c->ajac, c->si);
+    k_vort_ref<<<gr, bl, 0, s>>>(n0, n1, c->dxi, c->deta, c->u, c->v, c->xix, c->xiy, c->etx, c->ety, c->vort);
+    const ForceConst fc = force_const(P.alpha_deg, P.ar);
+    k_forces_ref<<<1, 256, 0, s>>>(n0, c->dxi, c->deta,
+                                   ForceRef{c->x, c->y, c->p, c->vort, c->T, c->ajac, c->etx, c->ety}, P.Re, fc.sa,
+                                   fc.ca, fc.nu_den, c->scal + 2);
     CFD_LAUNCH_CHECK();
-    CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CFD_HIP(hipMemcpyAsync(c->hscal, c->scal, 10 * sizeof(double), hipMemcpyDeviceToHost, s));
     CFD_HIP(hipStreamSynchronize(s));
     o.circ = c->hscal[0];
     o.dmax = c->hscal[1];
+    o.cl = c->hscal[2];
+    o.cd = c->hscal[3];
+    o.cm = c->hscal[4];
+    o.nu = c->hscal[9];
     if (st) *st = o;
     if (!std::isfinite(o.dmax)) return CFD_E_NAN;
     return CFD_OK;
@@ -1292,7 +1408,7 @@ int cfd_ref_case_create(int n0, int n1, double dxi, double deta, const double *x
     *out = nullptr;
     if (bad_grid(n0, n1) || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac ||
         !xnox || !xnoy || !prm || !(prm->dt > 0) || !(prm->Re > 0) || !(prm->Pr > 0) || !(prm->mom_rtol > 0) ||
-        !(prm->p_rtol > 0) || prm->mom_maxit < 1 || prm->p_maxit < 1)
+        !(prm->p_rtol > 0) || prm->mom_maxit < 1 || prm->p_maxit < 1 || !(prm->ar > 0))
         return CFD_E_ARG;
     RefCase *c = new RefCase;
     c->n0 = n0;
@@ -1307,7 +1423,7 @@ int cfd_ref_case_create(int n0, int n1, double dxi, double deta, const double *x
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) return fail(CFD_E_HIP);
     if (hipMalloc(&c->mem, nd * sizeof(double)) != hipSuccess) return fail(CFD_E_OOM);
-    if (hipHostMalloc(&c->hscal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(CFD_E_OOM);
+    if (hipHostMalloc(&c->hscal, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(CFD_E_OOM);
     ref_case_carve(c);
     if (hipMemsetAsync(c->mem, 0, nd * sizeof(double), c->s) != hipSuccess) return fail(CFD_E_HIP);
     const double *src[12] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1};
@@ -1364,6 +1480,58 @@ int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last) {
     if (!c->have_state) return CFD_E_STATE;
     for (int k = 0; k < nsteps; k++) CFD_RET(ref_case_step(c, last));
     return CFD_OK;
+}
+
+int cfd_ref_case_get_post(cfd_ref_case h, double *si, double *vort) {
+    RefCase *c = reinterpret_cast<RefCase *>(h);
+    if (!c) return CFD_E_ARG;
+    const size_t B = (size_t)c->n0 * c->n1 * sizeof(double);
+    if (si) CFD_HIP(hipMemcpyAsync(si, c->si, B, hipMemcpyDeviceToHost, c->s));
+    if (vort) CFD_HIP(hipMemcpyAsync(vort, c->vort, B, hipMemcpyDeviceToHost, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+int cfd_ref_stream_function(int n0, int n1, double deta, const double *u, const double *v, const double *xix,
+                            const double *xiy, const double *ajac, double *si) {
+    if (bad_grid(n0, n1) || !u || !v || !xix || !xiy || !ajac || !si) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(u, N), *b = B.up(v, N), *c = B.up(xix, N), *d = B.up(xiy, N), *e = B.up(ajac, N), *o = B.get(N);
+    NEED(a && b && c && d && e && o);
+    k_si_ref<<<(n0 + 255) / 256, 256>>>(n0, n1, deta, a, b, c, d, e, o);
+    CFD_LAUNCH_CHECK();
+    return down(si, o, N);
+}
+
+int cfd_ref_vorticity(int n0, int n1, double dxi, double deta, const double *u, const double *v, const double *xix,
+                      const double *xiy, const double *etx, const double *ety, double *vort) {
+    if (bad_grid(n0, n1) || !u || !v || !xix || !xiy || !etx || !ety || !vort) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    double *a = B.up(u, N), *b = B.up(v, N), *c = B.up(xix, N), *d = B.up(xiy, N), *e = B.up(etx, N),
+           *f = B.up(ety, N), *o = B.up(vort, N);
+    NEED(a && b && c && d && e && f && o);
+    k_vort_ref<<<grid2d(n0, n1), dim3(BX, BY)>>>(n0, n1, dxi, deta, a, b, c, d, e, f, o);
+    CFD_LAUNCH_CHECK();
+    return down(vort, o, N);
+}
+
+int cfd_ref_forces(int n0, int n1, double dxi, double deta, const double *x, const double *y, const double *p,
+                   const double *vort, const double *T, const double *ajac, const double *etx, const double *ety,
+                   double Re, double alpha_deg, double ar, double *coef) {
+    if (bad_grid(n0, n1) || !x || !y || !p || !vort || !T || !ajac || !etx || !ety || !coef || !(ar > 0))
+        return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    DevBufs B;
+    const ForceRef a{B.up(x, N), B.up(y, N), B.up(p, N), B.up(vort, N), B.up(T, N), B.up(ajac, N), B.up(etx, N),
+                     B.up(ety, N)};
+    double *o = B.get(8);
+    NEED(a.x && a.y && a.p && a.vort && a.T && a.ajac && a.etx && a.ety && o);
+    const ForceConst fc = force_const(alpha_deg, ar);
+    k_forces_ref<<<1, 256>>>(n0, dxi, deta, a, Re, fc.sa, fc.ca, fc.nu_den, o);
+    CFD_LAUNCH_CHECK();
+    return down(coef, o, 8);
 }
 
 int cfd_ref_case_time(cfd_ref_case h, double *time) {

@@ -25,7 +25,7 @@ def ref_params(**kw) -> RefParams:
     a = 82.0 * math.pi / 180.0
     base = dict(Re=1000.0, Pr=0.71, Ri=0.0, dt=1e-4, uinf=math.sin(a), vinf=math.cos(a), speed_amp=0.0,
                 accn_amp=0.0, F=0.0, time=0.0, mom_rtol=1e-12, p_rtol=1e-12, mom_maxit=500, p_maxit=2000,
-                time_accurate=0)
+                time_accurate=0, alpha_deg=82.0, ar=1.0)
     F = kw.get("F", 0.0)
     if F and "speed_amp" not in kw:
         pi = math.acos(-1.0)
@@ -65,7 +65,14 @@ class RefCase:
     def step(self, nsteps=1):
         st = RefStats()
         call("cfd_ref_case_step", self.h, int(nsteps), C.byref(st))
-        return dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres))
+        return dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres), cl=st.cl, cd=st.cd,
+                    cm=st.cm, nu=st.nu)
+
+    def post(self):
+        """Stream function and vorticity of the last step."""
+        si, vo = np.zeros((self.n1, self.n0)), np.zeros((self.n1, self.n0))
+        call("cfd_ref_case_get_post", self.h, si, vo)
+        return si, vo
 
     def time(self):
         t = C.c_double()

@@ -11,7 +11,8 @@ the C ABI (include/cfd_gpu.h).  Method names follow the reference:
   wall_pressure  solver.cpp:1667-1728    dilation     solver.cpp:1806-1852
   nusselt        solver.cpp:1951-1982, 2031-2038
   outflow_up     solver.cpp:1351-1381    farfield     solver.cpp:1543-1655
-  exit_pressure  solver.cpp:1731-1782
+  exit_pressure  solver.cpp:1731-1782    stream_function solver.cpp:1784-1800
+  vorticity      solver.cpp:1802-1887    forces       solver.cpp:1891-1982
 
 Like the reference, `sip9p` and `gauss` update `phi` in place and take the
 right-hand side `q`; unlike it they return (iterations, relative residual)
@@ -144,6 +145,25 @@ class RefStages:
              f64(vold), p_, f64(xix), f64(xiy), f64(etx), f64(ety), f64(alph), f64(beta), f64(gamma), f64(q1),
              f64(ajac), f64(xnox), f64(xnoy), uinf, vinf, dt, Re, Ri)
         return p_
+
+    def stream_function(self, u, v, xix, xiy, ajac):
+        si = self._z()
+        call("cfd_ref_stream_function", self.n0, self.n1, self.deta, f64(u), f64(v), f64(xix), f64(xiy), f64(ajac),
+             si)
+        return si
+
+    def vorticity(self, u, v, xix, xiy, etx, ety):
+        vo = self._z()
+        call("cfd_ref_vorticity", self.n0, self.n1, self.dxi, self.deta, f64(u), f64(v), f64(xix), f64(xiy),
+             f64(etx), f64(ety), vo)
+        return vo
+
+    def forces(self, x, y, p, vort, T, ajac, etx, ety, Re, alpha_deg=82.0, ar=1.0):
+        """cl, cd, cm, CL_pr, CD_pr, CL_vor, CD_vor, Nu."""
+        out = np.zeros(8)
+        call("cfd_ref_forces", self.n0, self.n1, self.dxi, self.deta, f64(x), f64(y), f64(p), f64(vort), f64(T),
+             f64(ajac), f64(etx), f64(ety), Re, alpha_deg, ar, out)
+        return out
 
     def dilation(self, u, v, xix, xiy, etx, ety):
         dil = self._z()

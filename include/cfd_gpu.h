@@ -159,6 +159,25 @@ int cfd_ref_exit_pressure(int n0, int n1, double dxi, double deta,
                           const double *q1, const double *ajac, const double *xnox, const double *xnoy,
                           double uinf, double vinf, double dt, double Re, double Ri);
 
+/* Stream function si, integrated along each eta line from 0 on the wall.
+ * Replaces solver.cpp:1784-1800. */
+int cfd_ref_stream_function(int n0, int n1, double deta, const double *u, const double *v,
+                            const double *xix, const double *xiy, const double *ajac, double *si);
+
+/* Vorticity: central differences inside, one-sided in eta on the wall and outer
+ * rows.  Replaces solver.cpp:1802-1887 (its vorticity part). */
+int cfd_ref_vorticity(int n0, int n1, double dxi, double deta, const double *u, const double *v,
+                      const double *xix, const double *xiy, const double *etx, const double *ety,
+                      double *vort);
+
+/* coef[8] = cl, cd, cm, CL_pr, CD_pr, CL_vor, CD_vor, Nu from the wall row (free
+ * stream at alpha_deg degrees, ellipse aspect ratio ar).  Replaces
+ * solver.cpp:1891-1982. */
+int cfd_ref_forces(int n0, int n1, double dxi, double deta, const double *x, const double *y,
+                   const double *p, const double *vort, const double *T, const double *ajac,
+                   const double *etx, const double *ety, double Re, double alpha_deg, double ar,
+                   double *coef);
+
 /* Dilation field and its signed maximum floored at 0 (the mass-imbalance
  * monitor).  Replaces solver.cpp:1806-1852. */
 int cfd_ref_dilation(int n0, int n1, double dxi, double deta,
@@ -193,6 +212,8 @@ typedef struct {
     double mom_rtol, p_rtol;         /* BiCGStab tolerances of the u/v/T/predictor and p' solves */
     int mom_maxit, p_maxit;
     int time_accurate;               /* 1: time += dt at the start of every step               */
+    double alpha_deg;                /* free-stream angle for the force coefficients (82)     */
+    double ar;                       /* ellipse aspect ratio (Nusselt normalisation)          */
 } cfd_ref_params;
 
 typedef struct {
@@ -200,6 +221,8 @@ typedef struct {
     double circ;       /* circulation along row n1-2 (far-field switch)        */
     int iters[6];      /* BiCGStab iterations: u, v, T, up, vp, p'             */
     double relres[6];  /* their final relative residuals                      */
+    double cl, cd, cm; /* lift, drag, moment coefficients (solver.cpp:1891-1949) */
+    double nu;         /* mean wall Nusselt number (solver.cpp:1951-1982)       */
 } cfd_ref_stats;
 
 typedef struct cfd_ref_case_s *cfd_ref_case;
@@ -219,6 +242,9 @@ int cfd_ref_case_get_state(cfd_ref_case h, double *u, double *v, double *T, doub
                            double *up, double *vp);
 /* nsteps time steps; last (may be NULL) receives the last step's statistics. */
 int cfd_ref_case_step(cfd_ref_case h, int nsteps, cfd_ref_stats *last);
+/* Stream function and vorticity of the last step (solver.cpp:1784-1887); either
+ * pointer may be NULL. */
+int cfd_ref_case_get_post(cfd_ref_case h, double *si, double *vort);
 /* The case's current time (advanced by the steps when time_accurate is set). */
 int cfd_ref_case_time(cfd_ref_case h, double *time);
 void cfd_ref_case_destroy(cfd_ref_case h);

@@ -121,6 +121,19 @@ void or_exit_p_ref(int n0, int n1, double h1, double h2,
                    const double *q1, const double *ajac, const double *xnox, const double *xnoy,
                    double uinf, double vinf, double dt, double Re, double Ri);
 
+/* Stream function (solver.cpp:1784-1800). */
+void or_si_ref(int n0, int n1, double h2, const double *u, const double *v, const double *xix,
+               const double *xiy, const double *ajac, double *si);
+
+/* Vorticity, interior and wall/outer rows (solver.cpp:1802-1887). */
+void or_vort_ref(int n0, int n1, double h1, double h2, const double *u, const double *v,
+                 const double *xix, const double *xiy, const double *etx, const double *ety, double *vort);
+
+/* cl, cd, cm, CL_pr, CD_pr, CL_vor, CD_vor, Nu into out[8] (solver.cpp:1891-1982). */
+void or_forces_ref(int n0, int n1, double h1, double h2, const double *x, const double *y,
+                   const double *p, const double *vort, const double *T, const double *ajac,
+                   const double *etx, const double *ety, double Re, double alpha, double ar, double *out);
+
 /* Dilation and its signed maximum floored at 0 (solver.cpp:1806-1852). */
 double or_dil_ref(int n0, int n1, double h1, double h2,
                   const double *u, const double *v,

@@ -609,7 +609,7 @@ double or_farfield_ref(int n0, int n1, double h2, const double *x, const double 
                        const double *ajac, const double *xnox, const double *xnoy,
                        double uinf, double vinf, double dt, double speed_amp, double cos_ft)
 {
-    double *vr0 = (double *)malloc(sizeof(double) * (size_t)n0 * 4);
+    double *vr0 = (double *)calloc((size_t)n0 * 4, sizeof(double));
     double *vt0 = vr0 + n0, *vr1 = vr0 + 2 * n0, *vt1 = vr0 + 3 * n0;
     int j = n1 - 2;
     for (int i = 0; i < n0 - 1; i++) {
@@ -758,6 +758,108 @@ void or_exit_p_ref(int n0, int n1, double h1, double h2,
         }
         if (i == 0) p[ID(n0 - 1, j)] = p[P];
     }
+}
+
+/* solver.cpp:1784-1800: stream function, the trapezoid integral of the
+ * J-weighted contravariant flux along each eta line from si = 0 on the wall. */
+void or_si_ref(int n0, int n1, double h2, const double *u, const double *v, const double *xix,
+               const double *xiy, const double *ajac, double *si)
+{
+    for (int i = 0; i < n0; i++) si[ID(i, 0)] = 0;
+    for (int i = 0; i < n0; i++) {
+        for (int j = 1; j < n1; j++) {
+            const size_t P = ID(i, j), Q = ID(i, j - 1);
+            const double ca = (xix[P] * u[P] * fabs(ajac[P]) + xix[Q] * u[Q] * fabs(ajac[Q]));
+            const double cb = (xiy[P] * v[P] * fabs(ajac[P]) + xiy[Q] * v[Q] * fabs(ajac[Q]));
+            si[P] = si[Q] + (ca + cb) * 0.5 * h2;
+        }
+    }
+}
+
+/* solver.cpp:1802-1887 (vorticity part): dv/dx - du/dy by the chain rule, central
+ * differences inside, one-sided eta differences on the wall and outer rows. */
+void or_vort_ref(int n0, int n1, double h1, double h2, const double *u, const double *v,
+                 const double *xix, const double *xiy, const double *etx, const double *ety, double *vort)
+{
+    for (int i = 0; i < n0 - 1; i++) {
+        for (int j = 1; j < n1 - 1; j++) {
+            const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+            const size_t P = ID(i, j);
+            const double dv_dxi = 0.5 / h1 * (v[ID(ipp, j)] - v[ID(inn, j)]);
+            const double dv_det = 0.5 / h2 * (v[ID(i, j + 1)] - v[ID(i, j - 1)]);
+            const double dv_dx = xix[P] * dv_dxi + etx[P] * dv_det;
+            const double du_dxi = 0.5 / h1 * (u[ID(ipp, j)] - u[ID(inn, j)]);
+            const double du_det = 0.5 / h2 * (u[ID(i, j + 1)] - u[ID(i, j - 1)]);
+            const double du_dy = xiy[P] * du_dxi + ety[P] * du_det;
+            vort[P] = dv_dx - du_dy;
+            if (i == 0) vort[ID(n0 - 1, j)] = vort[P];
+        }
+    }
+    for (int j = 0; j < n1; j += n1 - 1) {
+        for (int i = 0; i < n0 - 1; i++) {
+            const int inn = (i == 0) ? n0 - 2 : i - 1, ipp = i + 1;
+            const size_t P = ID(i, j);
+            const double dv_dxi = 0.5 / h1 * (v[ID(ipp, j)] - v[ID(inn, j)]);
+            const double dv_det = (j == 0) ? 1.0 / h2 * (v[ID(i, j + 1)] - v[P]) : 1.0 / h2 * (v[P] - v[ID(i, j - 1)]);
+            const double dv_dx = xix[P] * dv_dxi + etx[P] * dv_det;
+            const double du_dxi = 0.5 / h1 * (u[ID(ipp, j)] - u[ID(inn, j)]);
+            const double du_det = (j == 0) ? 1.0 / h2 * (u[ID(i, j + 1)] - u[P]) : 1.0 / h2 * (u[P] - u[ID(i, j - 1)]);
+            const double du_dy = xiy[P] * du_dxi + ety[P] * du_det;
+            vort[P] = dv_dx - du_dy;
+            if (i == 0) vort[ID(n0 - 1, j)] = vort[P];
+        }
+    }
+}
+
+/* solver.cpp:1891-1982: force and moment coefficients and the mean wall Nusselt
+ * number from the wall row's pressure, vorticity and temperature gradient.
+ * out: cl, cd, cm, CL_pr, CD_pr, CL_vor, CD_vor, Nu (alpha in degrees). */
+void or_forces_ref(int n0, int n1, double h1, double h2, const double *x, const double *y,
+                   const double *p, const double *vort, const double *T, const double *ajac,
+                   const double *etx, const double *ety, double Re, double alpha, double ar, double *out)
+{
+    (void)n1;
+    const double Pi = acos(-1.0);
+    const int j = 0;
+    double pr_x = 0.0, pr_y = 0.0, vor_x = 0.0, vor_y = 0.0;
+    for (int i = 0; i < n0 - 1; i++) {
+        const size_t A = ID(i, j), B = ID(i + 1, j);
+        const double PJ1 = p[A] * ajac[A], pj2 = p[B] * ajac[B];
+        const double VJ1 = vort[A] * ajac[A], VJ2 = vort[B] * ajac[B];
+        pr_x = pr_x + 0.5 * h1 * (PJ1 * etx[A] + pj2 * etx[B]);
+        pr_y = pr_y + 0.5 * h1 * (PJ1 * ety[A] + pj2 * ety[B]);
+        vor_x = vor_x + 0.5 * h1 * (VJ1 * ety[A] + VJ2 * ety[B]);
+        vor_y = vor_y + 0.5 * h1 * (VJ1 * etx[A] + VJ2 * etx[B]);
+    }
+    const double sa = sin(alpha * Pi / 180.0), ca = cos(alpha * Pi / 180.0);
+    const double cx = 2 * pr_x + (2.0 / Re) * vor_x;
+    const double cy = 2 * pr_y - (2.0 / Re) * vor_y;
+    out[3] = 2 * pr_y * sa - 2 * pr_x * ca;
+    out[4] = 2 * pr_y * ca + 2 * pr_x * sa;
+    out[5] = -(2.0 / Re) * vor_y * sa - (2.0 / Re) * vor_x * ca;
+    out[6] = -(2.0 / Re) * vor_y * ca + (2.0 / Re) * vor_x * sa;
+    out[0] = cy * sa - cx * ca;
+    out[1] = cy * ca + cx * sa;
+    double press_i = 0.0, vor_i = 0.0, temp_i = 0.0;
+    for (int i = 0; i < n0 - 1; i++) {
+        const int ip = i + 1;
+        const size_t A = ID(i, j), B = ID(ip, j);
+        const double PJ1 = p[A] * ajac[A], pj2 = p[B] * ajac[B];
+        const double VJ1 = vort[A] * ajac[A], VJ2 = vort[B] * ajac[B];
+        const double TJ1 = ajac[A] * (etx[A] * etx[A] + ety[A] * ety[A]);
+        const double TJ2 = ajac[B] * (etx[B] * etx[B] + ety[B] * ety[B]);
+        const double fp1 = PJ1 * (x[A] * ety[A] - y[A] * etx[A]);
+        const double fp2 = pj2 * (x[B] * ety[B] - y[B] * etx[B]);
+        const double fv1 = VJ1 * (x[A] * etx[A] + y[A] * ety[A]);
+        const double fv2 = VJ2 * (x[B] * etx[B] + y[B] * ety[B]);
+        const double fh1 = TJ1 * (4 * T[ID(i, j + 1)] - 3 * T[A] - T[ID(i, j + 2)]) / (2 * h2);
+        const double fh2 = TJ2 * (4 * T[ID(ip, j + 1)] - 3 * T[B] - T[ID(ip, j + 2)]) / (2 * h2);
+        press_i = press_i + 0.5 * h1 * (fp1 + fp2);
+        vor_i = vor_i + 0.5 * h1 * (fv1 + fv2);
+        temp_i = temp_i + 0.5 * (fh1 + fh2) * h1;
+    }
+    out[2] = 2 * press_i - (2.0 / Re) * vor_i;
+    out[7] = (2 * temp_i) / (Pi * (3 * (1 + (1.0 / ar)) - sqrt((3 + (1.0 / ar)) * ((3.0 / ar) + 1))));
 }
 
 /* solver.cpp:1806-1852 */

@@ -49,6 +49,9 @@ def lib():
         L.or_exit_p_ref.argtypes = [_i, _i, _d, _d] + [dp] * 6 + [dp] * 9 + [dp, dp] + [_d] * 5
         L.or_correct_ref.argtypes = [_i, _i, _d, _d] + [dp] * 7 + [_d, dp, dp, dp]
         L.or_wall_p_ref.argtypes = [_i, _i, _d, _d] + [dp] * 4 + [dp] * 9 + [dp, dp, _d, _d]
+        L.or_si_ref.argtypes = [_i, _i, _d] + [dp] * 5 + [dp]
+        L.or_vort_ref.argtypes = [_i, _i, _d, _d] + [dp] * 6 + [dp]
+        L.or_forces_ref.argtypes = [_i, _i, _d, _d] + [dp] * 8 + [_d, _d, _d, dp]
         L.or_dil_ref.argtypes = [_i, _i, _d, _d] + [dp] * 6 + [dp]
         L.or_dil_ref.restype = _d
         L.or_nusselt_ref.argtypes = [_i, _i, _d, _d, dp, dp, dp, dp, _d, C.c_void_p]

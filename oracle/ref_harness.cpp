@@ -122,6 +122,7 @@ void dump_state(int L) {
     dump2(t, "qup", s.qup); dump2(t, "qvp", s.qvp);
     dump2(t, "dil", s.dil); dump2(t, "uxi", s.uxi); dump2(t, "uet", s.uet);
     dump2(t, "sol", s.sol);
+    dump2(t, "si", s.si); dump2(t, "vort", s.vort);
     dump_scalar(t, "dmax", s.dmax);
     dump_scalar(t, "Ri", s.Ri);
 }
@@ -186,7 +187,9 @@ void on_step_line(int L) {
     if (L == 1) {
         // step 0 wrote its coefficient history (Nu) and local dT/dn profile
         copy_text("COEFF_HIS.dat", "s0_COEFF_HIS.dat");
+        copy_text("COEFF_HIS_pr_vor.dat", "s0_COEFF_HIS_pr_vor.dat");
         copy_text("SURF_DIST.dat", "s0_SURF_DIST.dat");
+        copy_text("spt100.dat", "s0_spt100.dat");
     }
     if (L >= g_last) {
         fflush(nullptr);

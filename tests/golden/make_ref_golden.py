@@ -39,7 +39,7 @@ PLANES25 = {"p": (0, 0), "e": (1, 0), "w": (-1, 0), "n": (0, 1), "s": (0, -1), "
 PLANES9 = {"ap": (0, 0), "ae": (1, 0), "aw": (-1, 0), "an": (0, 1), "as": (0, -1), "ane": (1, 1),
            "anw": (-1, 1), "ase": (1, -1), "asw": (-1, -1)}
 STATE = ["u", "v", "T", "up", "vp", "us", "vs", "uold", "vold", "Told", "p", "pcor", "q", "qu",
-         "qv", "qt", "qup", "qvp", "dil", "uxi", "uet"]
+         "qv", "qt", "qup", "qvp", "dil", "uxi", "uet", "si", "vort"]
 
 
 def main():
@@ -82,11 +82,21 @@ def main():
         d["s2_Ri"] = rd("s2_Ri", None)
         for k in ["pcor_conv", "pcor_one", "u_conv", "T_conv"]:
             d["s2_" + k] = rd("s2_" + k)
-        d["s0_T"] = rd("s0_T")
+        for k in ["u", "v", "T", "p", "si", "vort"]:
+            d["s0_" + k] = rd("s0_" + k)
         his = open(os.path.join(out, "s0_COEFF_HIS.dat")).read().split()
         d["s0_nu_8dp"] = np.array([float(his[4])])
+        # COEFF_HIS.dat: time, CL, CD, CM, Nu; COEFF_HIS_pr_vor.dat: time, CL_pr, CD_pr,
+        # CL_vor, CD_vor (8 decimals each); SURF_DIST.dat: xi, p, vorticity, dT/dn at
+        # the wall (6 significant digits); spt100.dat, the Tecplot field file, as bytes
+        d["s0_coeff_his_8dp"] = np.array([float(v) for v in his[:5]])
+        prv = open(os.path.join(out, "s0_COEFF_HIS_pr_vor.dat")).read().split()
+        d["s0_coeff_pr_vor_8dp"] = np.array([float(v) for v in prv[:5]])
         surf = np.loadtxt(os.path.join(out, "s0_SURF_DIST.dat"))
         d["s0_dthdn_6sig"] = surf[:, 3].copy()
+        d["s0_surf_dist_6sig"] = surf.copy()
+        with open(os.path.join(out, "s0_spt100.dat"), "rb") as f:
+            d["s0_spt100_bytes"] = np.frombuffer(f.read(), dtype=np.uint8).copy()
     np.savez_compressed(OUT, **d)
     print(OUT, os.path.getsize(OUT), "bytes")
 

@@ -28,7 +28,7 @@ def _grid(d):
 def _gpu_case(d, **kw):
     from cfd_solver_amd.refcase import RefCase, ref_params
     prm = ref_params(Re=d["Re"], Pr=d["Pr"], dt=d["dt"], Ri=float(d["s2_Ri"][0]), uinf=d["uinf"], vinf=d["vinf"],
-                     mom_rtol=1e-13, p_rtol=1e-13, **kw)
+                     mom_rtol=1e-13, p_rtol=1e-13, ar=d["ar"], **kw)
     c = RefCase(_grid(d), prm)
     c.set_state(d["s2_uold"], d["s2_vold"], d["s2_Told"], d["s1_p"], d["s1_up"], d["s1_vp"])
     return c
@@ -51,6 +51,20 @@ def test_steps_match_oracle_with_converged_solves(golden, nsteps):
     assert abs(st["dmax"] - o["dmax"]) <= 1e-8 * abs(o["dmax"])
     assert abs(st["circ"] - o["circ"]) <= 1e-8 * abs(o["circ"]) + 1e-15
     assert max(st["relres"]) <= 1e-13 and min(st["iters"]) >= 1
+    # the step's post-processing (solver.cpp:1784-1982) against the oracle's on its own step
+    import ctypes as C
+    from oracle.oracle import lib
+    d, n0, n1 = golden, golden["n0"], golden["n1"]
+    gg = {k: np.ascontiguousarray(d["grid_" + k]) for k in ("x", "y", "xix", "xiy", "etx", "ety", "ajac")}
+    si, vo, coef = np.zeros((n1, n0)), np.zeros((n1, n0)), np.zeros(8)
+    lib().or_si_ref(n0, n1, d["deta"], o["u"], o["v"], gg["xix"], gg["xiy"], gg["ajac"], si)
+    lib().or_vort_ref(n0, n1, d["dxi"], d["deta"], o["u"], o["v"], gg["xix"], gg["xiy"], gg["etx"], gg["ety"], vo)
+    lib().or_forces_ref(n0, n1, d["dxi"], d["deta"], gg["x"], gg["y"], o["p"], vo, o["T"], gg["ajac"], gg["etx"],
+                        gg["ety"], d["Re"], 82.0, d["ar"], coef)
+    gsi, gvo = c.post()
+    assert rel_l2(gsi, si) < 1e-9 and rel_l2(gvo, vo) < 1e-9
+    for k, q in (("cl", 0), ("cd", 1), ("cm", 2), ("nu", 7)):
+        assert abs(st[k] - coef[q]) <= 1e-8 * abs(coef[q]) + 1e-12, (k, st[k], coef[q])
     c.close()
 
 

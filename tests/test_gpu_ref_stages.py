@@ -158,3 +158,18 @@ def test_exit_pressure(golden, st):
                          a["vinf"], a["dt"], a["Re"], a["Ri"])
     assert rel_l2(p[-1], want[-1]) < 1e-12
     assert np.array_equal(p[:-1], want[:-1])
+
+
+def test_stream_function_vorticity_forces(golden, st):
+    """solver.cpp:1784-1982 on the reference's own step-0/step-2 fields: stream function and
+    vorticity (all rows) against its arrays, the coefficients against its COEFF_HIS lines."""
+    d, g = golden, _g(golden)
+    for s_ in ("s0", "s2"):
+        si = st.stream_function(d[s_ + "_u"], d[s_ + "_v"], g["xix"], g["xiy"], g["ajac"])
+        vo = st.vorticity(d[s_ + "_u"], d[s_ + "_v"], g["xix"], g["xiy"], g["etx"], g["ety"])
+        assert rel_l2(si, d[s_ + "_si"]) < TOL, s_
+        assert rel_l2(vo, d[s_ + "_vort"]) < TOL, s_
+    out = st.forces(d["grid_x"], d["grid_y"], d["s0_p"], d["s0_vort"], d["s0_T"], g["ajac"], g["etx"], g["ety"],
+                    d["Re"], 82.0, d["ar"])
+    np.testing.assert_allclose(out[[0, 1, 2, 7]], d["s0_coeff_his_8dp"][1:], rtol=0, atol=5e-9)
+    np.testing.assert_allclose(out[3:7], d["s0_coeff_pr_vor_8dp"][1:], rtol=0, atol=5e-9)

@@ -253,3 +253,35 @@ def test_farfield_closed_form(G):
     np.testing.assert_allclose(o["T"][-1], oT, rtol=0, atol=1e-13)
     for f, w in ((o["u"][0], wu), (o["v"][0], wv), (o["up"][0], wu), (o["vp"][0], wv)):
         np.testing.assert_allclose(f, w, rtol=0, atol=1e-15)
+
+
+# ---- post-processing: stream function, vorticity, force coefficients ---------------------------
+
+def test_stream_function_and_vorticity(golden):
+    """solver.cpp:1784-1887 on the reference's step-0 and step-2 velocities, bit for bit
+    (vorticity including the one-sided wall and outer rows)."""
+    d, g = golden, _g(golden)
+    n0, n1 = d["n0"], d["n1"]
+    for st in ("s0", "s2"):
+        si, vo = np.zeros((n1, n0)), np.zeros((n1, n0))
+        lib().or_si_ref(n0, n1, d["deta"], d[st + "_u"], d[st + "_v"], g["xix"], g["xiy"], g["ajac"], si)
+        lib().or_vort_ref(n0, n1, d["dxi"], d["deta"], d[st + "_u"], d[st + "_v"], g["xix"], g["xiy"], g["etx"],
+                          g["ety"], vo)
+        assert np.array_equal(si, d[st + "_si"]), st
+        assert np.array_equal(vo, d[st + "_vort"]), st
+
+
+def test_force_coefficients(golden):
+    """CL, CD, CM, their pressure and vorticity parts and Nu from step 0's wall row, against
+    the reference's COEFF_HIS.dat and COEFF_HIS_pr_vor.dat lines (printed with 8 decimals)."""
+    d, g = golden, _g(golden)
+    out = np.zeros(8)
+    lib().or_forces_ref(d["n0"], d["n1"], d["dxi"], d["deta"], g["x"], g["y"], d["s0_p"], d["s0_vort"], d["s0_T"],
+                        g["ajac"], g["etx"], g["ety"], d["Re"], 82.0, d["ar"], out)
+    his, prv = d["s0_coeff_his_8dp"], d["s0_coeff_pr_vor_8dp"]
+    np.testing.assert_allclose(out[[0, 1, 2, 7]], his[1:], rtol=0, atol=5e-9)
+    np.testing.assert_allclose(out[3:7], prv[1:], rtol=0, atol=5e-9)
+    # the wall columns of SURF_DIST.dat (6 significant digits): p and vorticity
+    surf = d["s0_surf_dist_6sig"]
+    np.testing.assert_allclose(surf[:, 1], d["s0_p"][0], rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(surf[:, 2], d["s0_vort"][0], rtol=1e-5, atol=1e-12)
