@@ -56,7 +56,7 @@ class RefParams(C.Structure):
 
 class RefStats(C.Structure):
     _fields_ = [("dmax", _d), ("circ", _d), ("iters", _i * 6), ("relres", _d * 6), ("cl", _d), ("cd", _d),
-                ("cm", _d), ("nu", _d)]
+                ("cm", _d), ("nu", _d), ("cl_pr", _d), ("cd_pr", _d), ("cl_vor", _d), ("cd_vor", _d)]
 
 
 # every symbol include/cfd_gpu.h declares, with its ctypes signature

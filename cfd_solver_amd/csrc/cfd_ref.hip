@@ -1151,6 +1151,10 @@ int ref_case_step(RefCase *c, cfd_ref_stats *st) {
     o.cl = c->hscal[2];
     o.cd = c->hscal[3];
     o.cm = c->hscal[4];
+    o.cl_pr = c->hscal[5];
+    o.cd_pr = c->hscal[6];
+    o.cl_vor = c->hscal[7];
+    o.cd_vor = c->hscal[8];
     o.nu = c->hscal[9];
     if (st) *st = o;
     if (!std::isfinite(o.dmax)) return CFD_E_NAN;

@@ -53,6 +53,7 @@ class RefCase:
              opt[0].ctypes.data if opt[0] is not None else None, opt[1].ctypes.data if opt[1] is not None else None,
              nox, noy, C.byref(self.params), C.byref(h))
         self.h = h
+        self.grid = grid
 
     def set_state(self, u, v, T, p, up, vp):
         call("cfd_ref_case_set_state", self.h, *[f64(a) for a in (u, v, T, p, up, vp)])
@@ -65,14 +66,52 @@ class RefCase:
     def step(self, nsteps=1):
         st = RefStats()
         call("cfd_ref_case_step", self.h, int(nsteps), C.byref(st))
-        return dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres), cl=st.cl, cd=st.cd,
-                    cm=st.cm, nu=st.nu)
+        self.last = dict(dmax=st.dmax, circ=st.circ, iters=list(st.iters), relres=list(st.relres), cl=st.cl,
+                         cd=st.cd, cm=st.cm, nu=st.nu, cl_pr=st.cl_pr, cd_pr=st.cd_pr, cl_vor=st.cl_vor,
+                         cd_vor=st.cd_vor)
+        return self.last
 
     def post(self):
         """Stream function and vorticity of the last step."""
         si, vo = np.zeros((self.n1, self.n0)), np.zeros((self.n1, self.n0))
         call("cfd_ref_case_get_post", self.h, si, vo)
         return si, vo
+
+    def write_outputs(self, directory, loop, time=None):
+        """The reference's output files of the last step (solver.cpp:1984-2039) in
+        `directory`: spt100.dat and SURF_DIST.dat rewritten, the COEFF_HIS lines
+        appended.  time: printed in COEFF_HIS (default: the case's time, an int 0
+        when it is held fixed, as the reference prints it)."""
+        import os
+        from . import refio
+        s, (si, vo) = self.state(), self.post()
+        g, st = self.grid, self.last
+        if time is None:
+            time = self.time() if self.params.time_accurate else int(self.params.time)
+        refio.write_tecplot(os.path.join(directory, "spt100.dat"), g.x, g.y, s["u"], s["v"], s["T"], s["p"], si, vo)
+        refio.append_coeff_his(os.path.join(directory, "COEFF_HIS.dat"), time, st["cl"], st["cd"], st["cm"], st["nu"])
+        refio.append_coeff_pr_vor(os.path.join(directory, "COEFF_HIS_pr_vor.dat"), time, st["cl_pr"], st["cd_pr"],
+                                  st["cl_vor"], st["cd_vor"])
+        refio.write_surf_dist(os.path.join(directory, "SURF_DIST.dat"), g.dxi, g.deta, s["p"], vo, s["T"], g.etx,
+                              g.ety)
+
+    def save_restart(self, path, loop):
+        from . import refio
+        s, (si, _) = self.state(), self.post()
+        refio.write_restart(path, loop, self.time(), self.last["dmax"] if getattr(self, "last", None) else 0.0,
+                            dict(x=self.grid.x, y=self.grid.y, si=si, **s))
+
+    @classmethod
+    def from_restart(cls, grid, path, **kw):
+        """A case continuing from a restart file: its state, and its time as the
+        parameters' time."""
+        from . import refio
+        r = refio.read_restart(path)
+        if (r["n0"], r["n1"]) != (int(grid.n0), int(grid.n1)):
+            raise ValueError(f"restart holds a {r['n0']}x{r['n1']} grid")
+        c = cls(grid, ref_params(**dict(kw, time=r["time"])))
+        c.set_state(r["u"], r["v"], r["T"], r["p"], r["up"], r["vp"])
+        return c
 
     def time(self):
         t = C.c_double()

@@ -223,6 +223,7 @@ typedef struct {
     double relres[6];  /* their final relative residuals                      */
     double cl, cd, cm; /* lift, drag, moment coefficients (solver.cpp:1891-1949) */
     double nu;         /* mean wall Nusselt number (solver.cpp:1951-1982)       */
+    double cl_pr, cd_pr, cl_vor, cd_vor;  /* pressure and vorticity parts of cl, cd */
 } cfd_ref_stats;
 
 typedef struct cfd_ref_case_s *cfd_ref_case;

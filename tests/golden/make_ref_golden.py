@@ -95,8 +95,9 @@ def main():
         surf = np.loadtxt(os.path.join(out, "s0_SURF_DIST.dat"))
         d["s0_dthdn_6sig"] = surf[:, 3].copy()
         d["s0_surf_dist_6sig"] = surf.copy()
-        with open(os.path.join(out, "s0_spt100.dat"), "rb") as f:
-            d["s0_spt100_bytes"] = np.frombuffer(f.read(), dtype=np.uint8).copy()
+        for name in ("spt100", "COEFF_HIS", "COEFF_HIS_pr_vor", "SURF_DIST"):
+            with open(os.path.join(out, "s0_" + name + ".dat"), "rb") as f:
+                d["s0_" + name.lower() + "_bytes"] = np.frombuffer(f.read(), dtype=np.uint8).copy()
     np.savez_compressed(OUT, **d)
     print(OUT, os.path.getsize(OUT), "bytes")
 

@@ -25,11 +25,14 @@ def _grid(d):
     return g
 
 
+def _prm_kw(d, **kw):
+    return dict(Re=d["Re"], Pr=d["Pr"], dt=d["dt"], Ri=float(d["s2_Ri"][0]), uinf=d["uinf"], vinf=d["vinf"],
+                mom_rtol=1e-13, p_rtol=1e-13, ar=d["ar"], **kw)
+
+
 def _gpu_case(d, **kw):
     from cfd_solver_amd.refcase import RefCase, ref_params
-    prm = ref_params(Re=d["Re"], Pr=d["Pr"], dt=d["dt"], Ri=float(d["s2_Ri"][0]), uinf=d["uinf"], vinf=d["vinf"],
-                     mom_rtol=1e-13, p_rtol=1e-13, ar=d["ar"], **kw)
-    c = RefCase(_grid(d), prm)
+    c = RefCase(_grid(d), ref_params(**_prm_kw(d, **kw)))
     c.set_state(d["s2_uold"], d["s2_vold"], d["s2_Told"], d["s1_p"], d["s1_up"], d["s1_vp"])
     return c
 
@@ -111,3 +114,28 @@ def test_time_accurate_oscillating_cylinder(golden):
     np.testing.assert_allclose(f["u"][0], -amp * cw * golden["grid_y"][0], rtol=0, atol=1e-15)
     np.testing.assert_allclose(f["v"][0], amp * cw * golden["grid_x"][0], rtol=0, atol=1e-15)
     c.close()
+
+
+def test_restart_continues_bit_for_bit(golden, tmp_path):
+    """Two steps, a restart file, a new case from it and one more step equal three steps
+    of one case bit for bit; the output files hold the step's coefficients."""
+    from cfd_solver_amd.refcase import RefCase
+    print("\n[gpu] reference O-grid, restart after 2 of 3 steps", flush=True)
+    a = _gpu_case(golden)
+    a.step(3)
+    b = _gpu_case(golden)
+    b.step(2)
+    path = tmp_path / "spa100.dat"
+    b.save_restart(path, 2)
+    c = RefCase.from_restart(_grid(golden), path, **_prm_kw(golden))
+    st = c.step(1)
+    fa, fc = a.state(), c.state()
+    for q in FIELDS:
+        assert np.array_equal(fa[q], fc[q]), q
+    c.write_outputs(tmp_path, 3)
+    his = (tmp_path / "COEFF_HIS.dat").read_text().split()
+    assert his[0] == "0" and abs(float(his[1]) - st["cl"]) <= 5e-9 and abs(float(his[4]) - st["nu"]) <= 5e-9
+    assert (tmp_path / "spt100.dat").read_text().startswith(f"zone\nI={golden['n0']}\nJ={golden['n1']}\n")
+    assert len((tmp_path / "SURF_DIST.dat").read_text().splitlines()) == golden["n0"]
+    for x in (a, b, c):
+        x.close()
