@@ -1264,6 +1264,18 @@ int mg_setup(Case *c) {
 // aiming at half the remaining tolerance).  graph: capturable (no host
 // synchronisation).
 // Iterations and relative residual: S_IT / S_RELRES (FP64), S_ITTOT / S_RELRES64.
+// Times the multigrid block sizes once (PSolver::tune) unless switched off or the
+// stream is being captured.  The timing launches overwrite the solver's scratch
+// (level arrays, BiCGStab t, partial rows 0-1), so this runs only where none of
+// them holds a value still needed: before an outer iteration or a p' solve starts.
+int tune_mg(Case *c) {
+    if (!c->prm.mg_tune || c->capturing) return CFD_OK;
+    PsCtx cx = ctx(c);
+    if (c->ps32) RET(c->ps32->tune(cx));
+    if (c->ps64) RET(c->ps64->tune(cx));
+    return CFD_OK;
+}
+
 int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = false) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
@@ -1498,6 +1510,7 @@ int capture(Case *c) {
 
 // launch one outer iteration of every active slot (no synchronisation)
 int outer_launch(Case *c) {
+    RET(tune_mg(c));  // once, eagerly, before any capture
     // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
     if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
@@ -1562,12 +1575,12 @@ int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
                       ps->lps, c->scal, -1};
         dim3 g2 = ps->down_grid(0);
         g2.z = c->B;
-        PSolver<T>::launch_down(g2, c->s, a, 2, PSolver<T>::wide_blocks(g2), false);
+        PSolver<T>::launch_down(g2, c->s, a, 2, ps->wdn[0], false);
     } else {
         OpLevel<T> &C = ps->lev[1];
         UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega, c->scal, -1};
         const dim3 g2((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, c->B);
-        if (PSolver<T>::wide_blocks(g2)) k_mg_up<T, 2, MG_NT_WIDE><<<g2, MG_NT_WIDE, 0, c->s>>>(a);
+        if (ps->wup[0]) k_mg_up<T, 2, MG_NT_WIDE><<<g2, MG_NT_WIDE, 0, c->s>>>(a);
         else k_mg_up<T, 2, MG_NT_NARROW><<<g2, MG_NT_NARROW, 0, c->s>>>(a);
     }
     CFD_LAUNCH_CHECK();
@@ -1738,6 +1751,7 @@ void cfd_params_default(cfd_params *p) {
     p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 10; p->mg_omega = 0.8;
     p->precision = CFD_FP64;
     p->graph = 0;
+    p->mg_tune = 1;
 }
 
 // ---- batches
@@ -1977,6 +1991,7 @@ int cfd_case_p_solve(cfd_case h, const double *d, const double *q, double *x, do
     if (d) CFD_HIP(hipMemcpyAsync(c->d, d, B, hipMemcpyHostToDevice, c->s));
     if (q) CFD_HIP(hipMemcpyAsync(c->q, q, B, hipMemcpyHostToDevice, c->s));
     RET(mg_setup(c));
+    RET(tune_mg(c));
     RET(psolve(c, rtol, maxit, false));
     RET(sync_scal(c));
     int its = 0;

@@ -1586,6 +1586,10 @@ struct PSolver {
     int lnp[PS_MAXLEV] = {0};      // and their number
     size_t lps = 0;                // per-slot stride of lpart
     int B = 1;                     // slots
+    // per fine level: whether its down (up) kernel runs MG_NT_WIDE threads per
+    // tile (else MG_NT_NARROW); from the launch-size rule, or timed by tune()
+    bool wdn[PS_MAXLEV] = {}, wup[PS_MAXLEV] = {};
+    bool tuned = false;
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
         nlev = nlev_;
@@ -1665,6 +1669,7 @@ struct PSolver {
         tab.bofs[nlev] = b;
         tab_blocks = b;
         tab.dn0 = g[0].n0;
+        size_rule();
         return CFD_OK;
     }
 
@@ -1715,6 +1720,82 @@ struct PSolver {
     // same bits alone and in a batch.
     static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
 
+    dim3 up_grid(int l) const {
+        return dim3((lev[l].n0 + MG_FX - 1) / MG_FX, (lev[l].n1 + MG_FY - 1) / MG_FY, B);
+    }
+    void size_rule() {
+        for (int l = 0; l < lc; l++) {
+            dim3 gd = down_grid(l);
+            gd.z = B;
+            wdn[l] = wide_blocks(gd);
+            wup[l] = wide_blocks(up_grid(l));
+        }
+    }
+
+    // Autotuning of the multigrid block sizes: each fine level's down and up
+    // kernels are timed with MG_NT_NARROW and MG_NT_WIDE threads per tile on the
+    // solver's own buffers (three launches of each, alternating, HIP events on the
+    // solver's stream) and the faster is kept.  A tile's sums use the same lanes
+    // whatever the block size, so the choice changes timing only, never a result.
+    // Runs once; not while the stream is being captured.
+    int tune(PsCtx &cx) {
+        if (tuned) return CFD_OK;
+        tuned = true;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        CFD_HIP(hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return CFD_E_HIP;
+        }
+        int rc = CFD_OK;
+        auto time_pair = [&](auto launch, bool &pick) {
+            float tot[2] = {0.f, 0.f};
+            for (int rep = 0; rep < 3 && rc == CFD_OK; rep++)
+                for (int k = 0; k < 2 && rc == CFD_OK; k++) {
+                    float ms = 0.f;
+                    if (hipEventRecord(e0, cx.s) != hipSuccess) rc = CFD_E_HIP;
+                    launch(k == 1);
+                    if (hipGetLastError() != hipSuccess || hipEventRecord(e1, cx.s) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                        rc = CFD_E_HIP;
+                    tot[k] += ms;
+                }
+            if (rc == CFD_OK) pick = tot[1] < tot[0];
+        };
+        const T w = (T)omega;
+        const RedArgs ra = cx.ra();
+        for (int l = 0; l < lc && rc == CFD_OK; l++) {
+            OpLevel<T> &L = lev[l];
+            OpLevel<T> &C = lev[l + 1];
+            const DownArgs<T> a{fargs(L), L.invd, L.t, l == 0 ? nullptr : lpart + lofs[l], lnp[l],
+                                l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w, lps, cx.scal,
+                                -1};
+            dim3 gd = down_grid(l);
+            gd.z = B;
+            time_pair([&](bool wide) { launch_down(gd, cx.s, a, pre, wide, a.bpart != nullptr); }, wdn[l]);
+            const UpArgs<T> u{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.t, L.t, w, cx.scal, -1};
+            const dim3 gu = up_grid(l);
+            if (l == 0 && fused_tail()) {
+                const Tail tl{MGA_T, bi[5], nullptr, ra};
+                time_pair([&](bool wide) {
+                    if (wide) launch_up_apply<MG_NT_WIDE>(gu, cx.s, u, tl, Timed{});
+                    else launch_up_apply<MG_NT_NARROW>(gu, cx.s, u, tl, Timed{});
+                }, wup[l]);
+            } else {
+                time_pair([&](bool wide) {
+                    switch (post) {
+                    case 0: launch_up_p<0>(gu, cx.s, u, wide, Timed{}); break;
+                    case 1: launch_up_p<1>(gu, cx.s, u, wide, Timed{}); break;
+                    default: launch_up_p<2>(gu, cx.s, u, wide, Timed{}); break;
+                    }
+                }, wup[l]);
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {
         return dim3((lev[l + 1].n0 + MG_CX - 1) / MG_CX, (lev[l + 1].n1 + MG_CY - 1) / MG_CY);
@@ -1749,7 +1830,7 @@ struct PSolver {
                           l == 0 ? nullptr : L.b, L.x, C.n0, C.n1, C.t, lpart + lofs[l + 1], w, lps, cx.scal, gs};
             dim3 gr = down_grid(l);
             gr.z = B;
-            launch_down(gr, cx.s, a, pre, wide_blocks(gr), a.bpart != nullptr,
+            launch_down(gr, cx.s, a, pre, wdn[l], a.bpart != nullptr,
                         l == 0 ? cx.prof(cx.owner, PK_MG_DOWN) : Timed{});
             CFD_LAUNCH_CHECK();
         }
@@ -1765,15 +1846,15 @@ struct PSolver {
             const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
             const Timed tm = l == 0 ? cx.prof(cx.owner, PK_MG_UP) : Timed{};
             if (l == 0 && tail) {
-                if (wide_blocks(gr)) launch_up_apply<MG_NT_WIDE>(gr, cx.s, a, *tail, tm);
+                if (wup[l]) launch_up_apply<MG_NT_WIDE>(gr, cx.s, a, *tail, tm);
                 else launch_up_apply<MG_NT_NARROW>(gr, cx.s, a, *tail, tm);
                 CFD_LAUNCH_CHECK();
                 continue;
             }
             switch (post) {
-            case 0: launch_up_p<0>(gr, cx.s, a, wide_blocks(gr), tm); break;
-            case 1: launch_up_p<1>(gr, cx.s, a, wide_blocks(gr), tm); break;
-            default: launch_up_p<2>(gr, cx.s, a, wide_blocks(gr), tm); break;
+            case 0: launch_up_p<0>(gr, cx.s, a, wup[l], tm); break;
+            case 1: launch_up_p<1>(gr, cx.s, a, wup[l], tm); break;
+            default: launch_up_p<2>(gr, cx.s, a, wup[l], tm); break;
             }
             CFD_LAUNCH_CHECK();
         }

@@ -279,6 +279,9 @@ typedef struct {
                            the p' BiCGStab then runs p_maxit unrolled iterations with
                            device-side convergence tests (no host synchronisation,
                            so batches on several streams overlap)                   */
+    int mg_tune;        /* 1 (default): each multigrid level's block size is timed
+                           once, before the first solve, and the faster kept (results
+                           do not depend on it); 0: the launch-size rule            */
 } cfd_params;
 
 typedef struct {

@@ -197,3 +197,20 @@ def test_multi_step_call_equals_single_steps():
             assert np.array_equal(fa[k], fb[k]), (prec, k)
         assert sa.iter == sb.iter == 10
         assert sa.mass_res == sb.mass_res and sa.p_iters == sb.p_iters
+
+
+def test_autotuned_block_sizes_change_no_bits():
+    """mg_tune times each multigrid level's block size once and keeps the faster; a tile's
+    sums use the same lanes at either size, so the fields equal those of the fixed
+    launch-size rule bit for bit (257x257: fine levels of both sizes)."""
+    print("\n[gpu] autotuned vs rule block sizes, 257x257, 10 outer iterations", flush=True)
+    from cfd_solver_amd.cavity import CavityCase, default_params
+    g = cavity_grid(257, skew=0.1, beta=1.5)
+    for prec in (0, 1):
+        a = CavityCase(g, default_params(Re=400.0, Ri=1.0, precision=prec, mg_tune=1))
+        b = CavityCase(g, default_params(Re=400.0, Ri=1.0, precision=prec, mg_tune=0))
+        sa, sb = a.step(10), b.step(10)
+        fa, fb = a.fields(), b.fields()
+        for k in ("u", "v", "T", "p"):
+            assert np.array_equal(fa[k], fb[k]), (prec, k)
+        assert sa.p_iters == sb.p_iters and sa.mass_res == sb.mass_res
