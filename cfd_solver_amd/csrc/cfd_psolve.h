@@ -390,6 +390,40 @@ constexpr int MG_SUM_LANES = 256;
 constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
 
+// Tiles of a multigrid level cover nodes 0 .. n - 2 in each direction: ceil((n - 1) /
+// width) of them.  A 2^k + 1 level then has exactly (n - 1) / width tile columns --
+// 16 x 128 tiles at 1025^2 instead of 17 x 129, whose last column and row would hold
+// one wall node each and add a partial round of blocks to every launch.  The wall
+// nodes that fall outside every tile (column n0 - 1 when width divides n0 - 1, row
+// n1 - 1 likewise) are set by the tiles next to them (edge_walls).
+inline int mg_tiles(int n, int width) { return (n - 1 + width - 1) / width; }
+
+// Calls f(node index) for the east-wall nodes of a TX x TY tile at (i0, j0) when the
+// tile's east edge is the column before the wall, for the north-wall nodes likewise,
+// and for the corner when both hold.
+template <int TX, int TY, typename F>
+__device__ __forceinline__ void edge_walls(int tid, int nt, int i0, int j0, int n0, int n1, F f) {
+    const bool e = i0 + TX == n0 - 1, n = j0 + TY == n1 - 1;
+    if (!e && !n) return;
+    for (int k = tid; k < TY + TX + 1; k += nt) {
+        int i, j;
+        if (k < TY) {
+            if (!e) continue;
+            i = n0 - 1;
+            j = j0 + k;
+        } else if (k < TY + TX) {
+            if (!n) continue;
+            i = i0 + k - TY;
+            j = n1 - 1;
+        } else {
+            if (!(e && n)) continue;
+            i = n0 - 1;
+            j = n1 - 1;
+        }
+        if (i < n0 && j < n1) f(idx(i, j, n0));
+    }
+}
+
 template <typename T>
 struct DownArgs {
     FArgs<T> a;
@@ -544,6 +578,11 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
         A.x[P] = NPRE >= 1 ? s_xs[(lj + 2) * TL::SX + (li + 2)] : T(0);
         if (A.bout) A.bout[P] = s_b[(lj + TL::HB) * TL::BWX + (li + TL::HB)];
     }
+    edge_walls<MG_FX, MG_FY>(tid, NTB, fi0, fj0, n0, n1, [&](size_t P) {
+        A.x[P] = T(0);
+        if (A.bout) A.bout[P] = T(0);
+    });
+    edge_walls<MG_CX, MG_CY>(tid, NTB, tl.x * MG_CX, tl.y * MG_CY, A.m0, A.m1, [&](size_t P) { A.craw[P] = T(0); });
     __syncthreads();
     // full weighting onto the coarse tile, one coarse node per thread (threads
     // beyond the tile's MG_CX * MG_CY coarse nodes have none)
@@ -634,6 +673,7 @@ __global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
     decltype(q_load()) lq;
     if (NPOST >= 1 && mg_early<T>()) lq = q_load();
     if (slot_skip(guard, A.gs)) return;
+    edge_walls<MG_FX, MG_FY>(tid, NTB, fi0, fj0, n0, n1, [&](size_t P) { A.out[P] = T(0); });
     batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
         const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
         T v = T(0);
@@ -1194,6 +1234,10 @@ __global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict_
     decltype(q_load()) lq;
     if (mg_early<T>()) lq = q_load();
     if (slot_skip(guard, A.gs)) return;
+    edge_walls<MG_FX, MG_FY>(tid, NTB, fi0, fj0, n0, n1, [&](size_t P) {
+        A.out[P] = T(0);
+        yout[P] = T(0);
+    });
     batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
         const int gi = fi0 - 3 + k % YX, gj = fj0 - 3 + k / YX;
         T v = T(0);
@@ -1721,7 +1765,7 @@ struct PSolver {
     static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
 
     dim3 up_grid(int l) const {
-        return dim3((lev[l].n0 + MG_FX - 1) / MG_FX, (lev[l].n1 + MG_FY - 1) / MG_FY, B);
+        return dim3(mg_tiles(lev[l].n0, MG_FX), mg_tiles(lev[l].n1, MG_FY), B);
     }
     void size_rule() {
         for (int l = 0; l < lc; l++) {
@@ -1798,7 +1842,7 @@ struct PSolver {
 
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {
-        return dim3((lev[l + 1].n0 + MG_CX - 1) / MG_CX, (lev[l + 1].n1 + MG_CY - 1) / MG_CY);
+        return dim3(mg_tiles(lev[l + 1].n0, MG_CX), mg_tiles(lev[l + 1].n1, MG_CY));
     }
 
     // z = M^-1 y: one V-cycle from zero with right-hand side y (slots whose
@@ -1814,7 +1858,7 @@ struct PSolver {
     };
     bool fused_tail() const { return lc > 0 && post == 2; }
     // blocks (partials) of the fused launch: level 0's 64 x 8 tiles
-    int tail_parts() const { return ((lev[0].n0 + MG_FX - 1) / MG_FX) * ((lev[0].n1 + MG_FY - 1) / MG_FY); }
+    int tail_parts() const { return mg_tiles(lev[0].n0, MG_FX) * mg_tiles(lev[0].n1, MG_FY); }
     template <int NTB>
     static void launch_up_apply(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, const Tail &t, const Timed &tm) {
         if (t.mode == MGA_V) launch_timed(tm, k_mg_up_apply<T, NTB, MGA_V>, gr, NTB, st, a, t.y, t.r0, t.ra);
@@ -1843,7 +1887,7 @@ struct PSolver {
             OpLevel<T> &C = lev[l + 1];
             UpArgs<T> a{fargs(L), L.invd, l == 0 ? y : L.b, L.x, C.n0, C.n1, l + 1 == lc ? C.x : C.t,
                         l == 0 ? z : L.t, w, cx.scal, gs};
-            const dim3 gr((L.n0 + MG_FX - 1) / MG_FX, (L.n1 + MG_FY - 1) / MG_FY, B);
+            const dim3 gr = up_grid(l);
             const Timed tm = l == 0 ? cx.prof(cx.owner, PK_MG_UP) : Timed{};
             if (l == 0 && tail) {
                 if (wup[l]) launch_up_apply<MG_NT_WIDE>(gr, cx.s, a, *tail, tm);
