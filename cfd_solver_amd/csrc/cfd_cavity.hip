@@ -124,7 +124,7 @@ struct GridPtrs {
 // 25-point rows read LDS instead of issuing a global load per stencil point.
 // Halo entries beyond the grid hold clamped copies that no interior row reads.
 constexpr int RT_X = 64, RT_Y = 8;
-inline dim3 grid_res(int n0, int n1, int B) { return dim3((n0 + RT_X - 1) / RT_X, (n1 + RT_Y - 1) / RT_Y, B); }
+inline dim3 grid_res(int n0, int n1, int B) { return dim3(tiles_inner(n0, RT_X), tiles_inner(n1, RT_Y), B); }
 
 template <typename TR>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_residual(
@@ -1184,6 +1184,10 @@ dim3 grid_b(const Case *c) {
     g.z = c->B;
     return g;
 }
+// 64 x 4 tiles of the face divergence and the correction, which write interior
+// nodes only, and the number of their block partials
+dim3 grid_in(const Case *c) { return dim3(tiles_inner(c->n0, BX), tiles_inner(c->n1, BY), c->B); }
+int nblk_in(int n0, int n1) { return tiles_inner(n0, BX) * tiles_inner(n1, BY); }
 
 // one-block-per-slot finish of a case-level reduction over the grid's block partials
 template <int OP>
@@ -1282,7 +1286,7 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
     const dim3 gr = grid_b(c), bl(BX, BY);
     if (!c->mixed) {
         // with the face divergence's sums of q finished, the solve starts in one pass
-        if (q_norms_ready) RET(cav_fin<CF_Q>(c, -1, rtol, 1.0));
+        if (q_norms_ready) RET(cav_fin<CF_Q>(c, -1, rtol, 1.0, nblk_in(n0, n1)));
         RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph, q_norms_ready));
         k_pc_final<false><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
@@ -1297,7 +1301,7 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
         k_fill<double><<<(unsigned)((c->B * c->N + 255) / 256), 256, 0, c->s>>>(c->B * c->N, c->pc, 0.0);
         CFD_LAUNCH_CHECK();
     }
-    RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5));
+    RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5, q_norms_ready ? nblk_in(n0, n1) : nblk(n0, n1)));
     for (int k = 0; k < R; k++) {
         if (!graph && k > 0) {
             RET(ps_sync(cx));
@@ -1364,7 +1368,7 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
     const TR *Ru = (const TR *)c->Ru, *Rv = (const TR *)c->Rv, *RT = (const TR *)c->RT;
     const TR *U = (const TR *)c->U, *V = (const TR *)c->V;
     const dim3 gr = grid_b(c), bl(BX, BY);
-    const dim3 g2((c->n0 + JT_X - 1) / JT_X, (c->n1 + JT_Y - 1) / JT_Y, c->B);
+    const dim3 g2(tiles_inner(c->n0, JT_X), tiles_inner(c->n1, JT_Y), c->B);
     const int m = c->prm.mom_sweeps;
     for (int s = 0; s < m;) {
         if (m - s >= 2) {  // four or two sweeps in one launch
@@ -1428,11 +1432,11 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
 int launch_face_div(Case *c, const void *du, const void *dv) {
     const Timed tm = prof_timed(c, 3);
     if (c->mixed)
-        launch_timed(tm, k_face_div<float>, grid_b(c), dim3(BX, BY), c->s, cg(c), c->u, c->v, (const float *)du,
+        launch_timed(tm, k_face_div<float>, grid_in(c), dim3(BX, BY), c->s, cg(c), c->u, c->v, (const float *)du,
                      (const float *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->q,
                      red(c, PS_MASS), red(c));
     else
-        launch_timed(tm, k_face_div<double>, grid_b(c), dim3(BX, BY), c->s, cg(c), c->u, c->v,
+        launch_timed(tm, k_face_div<double>, grid_in(c), dim3(BX, BY), c->s, cg(c), c->u, c->v,
                      (const double *)du, (const double *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
                      c->q, red(c, PS_MASS), red(c));
     CFD_LAUNCH_CHECK();
@@ -1443,7 +1447,7 @@ int launch_face_div(Case *c, const void *du, const void *dv) {
 // (the p' solve is controlled on the device); otherwise the p' solve reads its
 // convergence flags on the host between half-iterations.
 int outer_body(Case *c, bool graph) {
-    const dim3 gr = grid_b(c), bl(BX, BY);
+    const dim3 bl(BX, BY);
     // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
     // 3 predictor
@@ -1457,11 +1461,11 @@ int outer_body(Case *c, bool graph) {
     RET(psolve(c, c->prm.p_rtol, c->prm.p_maxit, graph, true));
     // 6-7 correction (u*, T += dT, p' mean removed) and BCs (FP64 accumulation)
     if (c->mixed)
-        k_correct<float><<<gr, bl, 0, c->s>>>(cg(c), (const float *)du, (const float *)dv, (const float *)dT, c->pc,
+        k_correct<float><<<grid_in(c), bl, 0, c->s>>>(cg(c), (const float *)du, (const float *)dv, (const float *)dT, c->pc,
                                               c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v, c->p,
                                               c->T, red(c));
     else
-        k_correct<double><<<gr, bl, 0, c->s>>>(cg(c), (const double *)du, (const double *)dv, (const double *)dT,
+        k_correct<double><<<grid_in(c), bl, 0, c->s>>>(cg(c), (const double *)du, (const double *)dv, (const double *)dT,
                                                c->pc, c->d, c->xix, c->xiy, c->etx, c->ety, c->scal, c->u, c->v,
                                                c->p, c->T, red(c));
     CFD_LAUNCH_CHECK();
@@ -1469,7 +1473,7 @@ int outer_body(Case *c, bool graph) {
     // 8 dmax (partials from the correction) and the wall Nusselt numbers (FP64)
     {
         const dim3 grr = grid_res(c->n0, c->n1, 1);
-        k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk(c->n0, c->n1), (int)(grr.x * grr.y),
+        k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk_in(c->n0, c->n1), (int)(grr.x * grr.y),
                                                              c->T, c->ajac, c->etx, c->ety);
     }
     CFD_LAUNCH_CHECK();

@@ -72,6 +72,13 @@ constexpr int BY = 4;
 
 inline dim3 grid2d(int n0, int n1) { return dim3((n0 + BX - 1) / BX, (n1 + BY - 1) / BY); }
 
+// Tiles of width w covering nodes 0 .. n - 2 of one direction, for kernels that
+// write interior nodes only (or whose wall nodes the tiles next to them set).  On
+// a 2^k + 1 grid this is (n - 1) / w: the tile column (row) that ceil(n / w) adds
+// would hold only the east (north) wall, and would add a partial round of blocks
+// to the launch.
+inline int tiles_inner(int n, int w) { return (n - 1 + w - 1) / w; }
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Wavefront (64 lanes) sum of a double.

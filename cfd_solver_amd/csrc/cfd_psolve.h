@@ -411,13 +411,10 @@ constexpr int MG_SUM_LANES = 256;
 constexpr long MG_WIDE_BELOW = 256;    // tiles per launch below which tiles get MG_NT_WIDE threads
 constexpr int MG_CX = MG_FX / 2, MG_CY = MG_FY / 2;
 
-// Tiles of a multigrid level cover nodes 0 .. n - 2 in each direction: ceil((n - 1) /
-// width) of them.  A 2^k + 1 level then has exactly (n - 1) / width tile columns --
-// 16 x 128 tiles at 1025^2 instead of 17 x 129, whose last column and row would hold
-// one wall node each and add a partial round of blocks to every launch.  The wall
-// nodes that fall outside every tile (column n0 - 1 when width divides n0 - 1, row
+// Tiles of a multigrid level cover nodes 0 .. n - 2 in each direction
+// (tiles_inner): 16 x 128 tiles at 1025^2 instead of 17 x 129.  The wall nodes that
+// fall outside every tile (column n0 - 1 when the tile width divides n0 - 1, row
 // n1 - 1 likewise) are set by the tiles next to them (edge_walls).
-inline int mg_tiles(int n, int width) { return (n - 1 + width - 1) / width; }
 
 // Calls f(node index) for the east-wall nodes of a TX x TY tile at (i0, j0) when the
 // tile's east edge is the column before the wall, for the north-wall nodes likewise,
@@ -1791,7 +1788,7 @@ struct PSolver {
     static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
 
     dim3 up_grid(int l) const {
-        return dim3(mg_tiles(lev[l].n0, MG_FX), mg_tiles(lev[l].n1, MG_FY), B);
+        return dim3(tiles_inner(lev[l].n0, MG_FX), tiles_inner(lev[l].n1, MG_FY), B);
     }
     void size_rule() {
         for (int l = 0; l < lc; l++) {
@@ -1868,7 +1865,7 @@ struct PSolver {
 
     // launch shape of k_mg_down on fine level l (one block per coarse 32 x 8 tile)
     dim3 down_grid(int l) const {
-        return dim3(mg_tiles(lev[l + 1].n0, MG_CX), mg_tiles(lev[l + 1].n1, MG_CY));
+        return dim3(tiles_inner(lev[l + 1].n0, MG_CX), tiles_inner(lev[l + 1].n1, MG_CY));
     }
 
     // z = M^-1 y: one V-cycle from zero with right-hand side y (slots whose
@@ -1884,7 +1881,7 @@ struct PSolver {
     };
     bool fused_tail() const { return lc > 0 && post == 2; }
     // blocks (partials) of the fused launch: level 0's 64 x 8 tiles
-    int tail_parts() const { return mg_tiles(lev[0].n0, MG_FX) * mg_tiles(lev[0].n1, MG_FY); }
+    int tail_parts() const { return tiles_inner(lev[0].n0, MG_FX) * tiles_inner(lev[0].n1, MG_FY); }
     template <int NTB>
     static void launch_up_apply(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, const Tail &t, const Timed &tm) {
         if (t.mode == MGA_V) launch_timed(tm, k_mg_up_apply<T, NTB, MGA_V>, gr, NTB, st, a, t.y, t.r0, t.ra);
