@@ -953,10 +953,11 @@ __global__ __launch_bounds__(NT) void k_q_norms(int n0, int n1, const double *__
         const size_t o = zslot() * n0 * n1;
         q += o; pc += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const FlatNode fn = flat_node(n0, n1);
+    const int i = fn.i, j = fn.j;
     double v[2] = {0.0, 0.0};
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
+    if (fn.in) {
+        const size_t P = fn.P;
         pc[P] = 0.0;
         if (inner_node(i, j, n0, n1)) {
             v[0] = q[P];
@@ -977,11 +978,11 @@ __global__ __launch_bounds__(NT) void k_defect(FArgs<double> a, const double *__
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o; r += o;
     }
-    const int2 tl = xcd_tile();
-    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
+    const FlatNode fn = flat_node(a.n0, a.n1);
+    const int i = fn.i, j = fn.j;
     double v[2] = {0.0, 0.0};
-    if (i < a.n0 && j < a.n1) {
-        const size_t P = idx(i, j, a.n0);
+    if (fn.in) {
+        const size_t P = fn.P;
         double y = 0.0;
         if (inner_node(i, j, a.n0, a.n1)) {
             y = q[P] - frow_f<double>(a, i, j, [&](int ii, int jj) {
@@ -1007,10 +1008,11 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
         pc += o;
         if (ACC) ef += o;
     }
-    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    const FlatNode fn = flat_node(n0, n1);
+    const int i = fn.i, j = fn.j;
     double v[1] = {0.0};
-    if (i < n0 && j < n1) {
-        const size_t P = idx(i, j, n0);
+    if (fn.in) {
+        const size_t P = fn.P;
         double x = pc[P];
         if (ACC) {
             x += (double)ef[P];
@@ -1036,11 +1038,11 @@ __global__ __launch_bounds__(NT) void k_defect_final(FArgs<double> a, const doub
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o;
     }
-    const int2 tl = xcd_tile();
-    const int i = tl.x * BX + threadIdx.x, j = tl.y * BY + threadIdx.y;
+    const FlatNode fn = flat_node(a.n0, a.n1);
+    const int i = fn.i, j = fn.j;
     double v[3] = {0.0, 0.0, 0.0};
-    if (i < a.n0 && j < a.n1) {
-        const size_t P = idx(i, j, a.n0);
+    if (fn.in) {
+        const size_t P = fn.P;
         const double x = (double)ef[P];
         if (inner_node(i, j, a.n0, a.n1)) {
             const double y = q[P] - frow_f<double>(a, i, j, [&](int ii, int jj) { return (double)ef[idx(ii, jj, a.n0)]; });
@@ -1283,25 +1285,27 @@ int tune_mg(Case *c) {
 int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = false) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
-    const dim3 gr = grid_b(c), bl(BX, BY);
+    // the p' kernels here run one thread per node in memory order
+    const dim3 gf(nflat(n0, n1), 1, c->B), bl(BX, BY);
+    const int nf = nflat(n0, n1);
     if (!c->mixed) {
         // with the face divergence's sums of q finished, the solve starts in one pass
         if (q_norms_ready) RET(cav_fin<CF_Q>(c, -1, rtol, 1.0, nblk_in(n0, n1)));
         RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph, q_norms_ready));
-        k_pc_final<false><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
+        k_pc_final<false><<<gf, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
-        return cav_fin<CF_PSUM>(c);
+        return cav_fin<CF_PSUM>(c, -1, 0.0, 0.5, nf);
     }
     const int R = rtol >= 1e-4 ? 1 : 6;
     const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
     if (!q_norms_ready) {
-        k_q_norms<<<gr, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
+        k_q_norms<<<gf, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
         CFD_LAUNCH_CHECK();
     } else if (R > 1) {  // the refinement accumulates into p' from zero
         k_fill<double><<<(unsigned)((c->B * c->N + 255) / 256), 256, 0, c->s>>>(c->B * c->N, c->pc, 0.0);
         CFD_LAUNCH_CHECK();
     }
-    RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5, q_norms_ready ? nblk_in(n0, n1) : nblk(n0, n1)));
+    RET(cav_fin<CF_Q>(c, -1, rtol, R == 1 ? 1.0 : 0.5, q_norms_ready ? nblk_in(n0, n1) : nf));
     for (int k = 0; k < R; k++) {
         if (!graph && k > 0) {
             RET(ps_sync(cx));
@@ -1315,17 +1319,17 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
         RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, S_RDONE,
                            graph, k == 0));
         if (R == 1) {  // defect, p' = e and its sum in one pass
-            k_defect_final<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, red(c));
+            k_defect_final<<<gf, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, red(c));
             CFD_LAUNCH_CHECK();
-            return cav_fin<CF_DEFECT_PSUM>(c);
+            return cav_fin<CF_DEFECT_PSUM>(c, -1, 0.0, 0.5, nf);
         }
-        k_defect<<<gr, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, c->r64, red(c));
+        k_defect<<<gf, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, c->r64, red(c));
         CFD_LAUNCH_CHECK();
-        RET(cav_fin<CF_DEFECT>(c, S_RDONE));
+        RET(cav_fin<CF_DEFECT>(c, S_RDONE, 0.0, 0.5, nf));
     }
-    k_pc_final<true><<<gr, bl, 0, c->s>>>(n0, n1, c->pc, c->ef, red(c));
+    k_pc_final<true><<<gf, bl, 0, c->s>>>(n0, n1, c->pc, c->ef, red(c));
     CFD_LAUNCH_CHECK();
-    return cav_fin<CF_PSUM>(c);
+    return cav_fin<CF_PSUM>(c, -1, 0.0, 0.5, nf);
 }
 
 void psolve_result(const Case *c, int z, int *its, double *relres) {
