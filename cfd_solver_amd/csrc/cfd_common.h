@@ -81,14 +81,54 @@ inline int tiles_inner(int n, int w) { return (n - 1 + w - 1) / w; }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Wavefront (64 lanes) sum of a double.
+// Wavefront (64 lanes) reductions of a double with DPP lane moves instead of LDS
+// permutes (a dependent ds_bpermute costs an LDS round trip; a DPP move is a VALU
+// operand modifier).  The double moves as two 32-bit halves.  A lane whose source
+// is outside its row keeps `old`: zero for the sum, its own value for the maximum.
+// Steps: swap neighbours and pairs within each quad (every lane then holds its
+// quad's total), row_shr 4 and 8 (lane 15 of each 16-lane row holds the row's
+// total), then the gfx9 row broadcasts of lane 15 into the next row and of lane 31
+// into rows 2 and 3, which leave the wavefront's total in lane 63.  The order of the
+// additions is fixed, so results are deterministic; the total is returned in every
+// lane (read from lane 63).  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v, double old) {
+    const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane63(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+enum : int {
+    DPP_QUAD_1032 = 0xB1,  // quad_perm [1, 0, 3, 2]
+    DPP_QUAD_2301 = 0x4E,  // quad_perm [2, 3, 0, 1]
+    DPP_ROW_SHR4 = 0x114,
+    DPP_ROW_SHR8 = 0x118,
+    DPP_ROW_BCAST15 = 0x142,
+    DPP_ROW_BCAST31 = 0x143,
+};
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
+    v += dpp_move<DPP_QUAD_1032>(v, 0.0);
+    v += dpp_move<DPP_QUAD_2301>(v, 0.0);
+    v += dpp_move<DPP_ROW_SHR4>(v, 0.0);
+    v += dpp_move<DPP_ROW_SHR8>(v, 0.0);
+    v += dpp_move<DPP_ROW_BCAST15>(v, 0.0);
+    v += dpp_move<DPP_ROW_BCAST31>(v, 0.0);
+    return lane63(v);
 }
 __device__ __forceinline__ double wave_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-    return v;
+    v = fmax(v, dpp_move<DPP_QUAD_1032>(v, v));
+    v = fmax(v, dpp_move<DPP_QUAD_2301>(v, v));
+    v = fmax(v, dpp_move<DPP_ROW_SHR4>(v, v));
+    v = fmax(v, dpp_move<DPP_ROW_SHR8>(v, v));
+    v = fmax(v, dpp_move<DPP_ROW_BCAST15>(v, v));
+    v = fmax(v, dpp_move<DPP_ROW_BCAST31>(v, v));
+    return lane63(v);
 }
 
 // Block reduction (sum) for up to 1024 threads; result valid in thread 0.

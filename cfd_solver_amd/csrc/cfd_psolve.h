@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
                     wsync();
                     CNode<T> &cc = nd[L + 1];
                     const T v = restrict_to(r, n0, cc);
-                    const double tot = __shfl(wave_sum(cc.k >= 0 ? (double)v : 0.0), 0, 64);
+                    const double tot = wave_sum(cc.k >= 0 ? (double)v : 0.0);
                     cc.b = (T)((double)v - tot / (double)INNER(L + 1));
                 }
             }
