@@ -527,8 +527,13 @@ __global__ __launch_bounds__(NT) void k_face_div(
         nq[0] = qq;
         nq[1] = qq * qq;
     }
-    block_partials<1>(aq, rmass.part, rmass.pstride);
-    block_partials<2>(nq, rq.part, rq.pstride);
+    double all[3] = {aq[0], nq[0], nq[1]};
+    block_sums<3, NT>(all);
+    if (tid == 0) {
+        rmass.part[blin()] = all[0];
+        rq.part[blin()] = all[1];
+        rq.part[(size_t)rq.pstride + blin()] = all[2];
+    }
 }
 
 // u = u* - d grad p' with u* = u + du (p' read clamped = wall copy), p += alpha_p p'

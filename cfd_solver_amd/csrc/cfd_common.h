@@ -147,6 +147,26 @@ __device__ __forceinline__ double block_sum(double v) {
     __syncthreads();
     return r;
 }
+// Block sums of NV values at once for up to 1024 threads, valid in thread 0: per
+// value the additions of block_sum in the same order (the same bits), with one
+// barrier and one LDS round trip for all NV values instead of one per value.
+template <int NV, int NT>
+__device__ __forceinline__ void block_sums(double (&v)[NV]) {
+    __shared__ double s_part[NV][NT / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    if ((lane & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) s_part[k][lane >> 6] = v[k];
+    }
+    __syncthreads();
+    if (lane < 64) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = wave_sum((lane < NT / 64) ? s_part[k][lane] : 0.0);
+    }
+    __syncthreads();
+}
 template <int NT>
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double s_part[NT / 64];

@@ -131,8 +131,7 @@ inline int nflat(int n0, int n1) { return (int)(((long)n0 * n1 + PS_NT - 1) / PS
 // costs more on gfx950 than the extra launch.)
 template <int NV, int NT = PS_NT>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double *part, int pstride) {
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = block_sum<NT>(v[k]);
+    block_sums<NV, NT>(v);
     if (tlin() == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) part[(size_t)k * pstride + blin()] = v[k];
@@ -176,8 +175,8 @@ __device__ __forceinline__ void sum_partials(const double *part, int pstride, in
     }
 #pragma unroll
     for (int k = 0; k < NV; k++)
-        tot[k] = block_sum<NT>(((s[k][0] + s[k][1]) + (s[k][2] + s[k][3])) +
-                               ((s[k][4] + s[k][5]) + (s[k][6] + s[k][7])));
+        tot[k] = ((s[k][0] + s[k][1]) + (s[k][2] + s[k][3])) + ((s[k][4] + s[k][5]) + (s[k][6] + s[k][7]));
+    block_sums<NV, NT>(tot);
 }
 
 // threads of the one-block kernels that finish a reduction

@@ -579,8 +579,7 @@ __global__ void k_forces_ref(int n0, double h1, double h2, ForceRef a, double Re
         v[5] += 0.5 * h1 * (fv1 + fv2);
         v[6] += 0.5 * (fh1 + fh2) * h1;
     }
-#pragma unroll
-    for (int k = 0; k < 7; k++) v[k] = block_sum<256>(v[k]);
+    block_sums<7, 256>(v);
     if (threadIdx.x != 0) return;
     const double cx = 2 * v[0] + (2.0 / Re) * v[2], cy = 2 * v[1] - (2.0 / Re) * v[3];
     out[0] = cy * sa - cx * ca;
@@ -738,10 +737,13 @@ enum { Q_RHO, Q_RHO_OLD, Q_ALPHA, Q_OMEGA, Q_BETA, Q_TOL, Q_R0N, Q_RN, Q_SCONV, 
 // block partials of up to two dot products, row k at part[k * np]
 template <int NV, int NT>
 __device__ __forceinline__ void sop_partials(const double (&v)[NV], double *part, int np, int b) {
+    double t[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const double t = block_sum<NT>(v[k]);
-        if (threadIdx.x + threadIdx.y * blockDim.x == 0) part[(size_t)k * np + b] = t;
+    for (int k = 0; k < NV; k++) t[k] = v[k];
+    block_sums<NV, NT>(t);
+    if (threadIdx.x + threadIdx.y * blockDim.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) part[(size_t)k * np + b] = t[k];
     }
 }
 
@@ -863,8 +865,9 @@ __global__ void k_q_fin(const double *__restrict__ part, int np, int nparts, dou
     for (int k = 0; k < NV; k++) {
         double a = 0.0;
         for (int b = threadIdx.x; b < nparts; b += 256) a += part[(size_t)k * np + b];
-        tot[k] = block_sum<256>(a);
+        tot[k] = a;
     }
+    block_sums<NV, 256>(tot);
     if (threadIdx.x != 0) return;
     if (OP == QF_START) {
         const double rn = sqrt(tot[0]);
