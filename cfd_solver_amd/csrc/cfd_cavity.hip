@@ -1596,8 +1596,7 @@ int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
         OpLevel<T> &C = ps->lev[1];
         UpArgs<T> a{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.x, L.t, (T)ps->omega, c->scal, -1};
         const dim3 g2 = ps->up_grid(0);
-        if (ps->wup[0]) k_mg_up<T, 2, MG_NT_WIDE><<<g2, MG_NT_WIDE, 0, c->s>>>(a);
-        else k_mg_up<T, 2, MG_NT_NARROW><<<g2, MG_NT_NARROW, 0, c->s>>>(a);
+        PSolver<T>::template launch_up_p<2>(g2, c->s, a, ps->wup[0], Timed{});
     }
     CFD_LAUNCH_CHECK();
     return CFD_OK;

@@ -403,6 +403,18 @@ __global__ __launch_bounds__(PS_NT) void k_faces_all(const double *scal, LevTab<
 constexpr int MG_FX = 64, MG_FY = 8;   // fine tile
 constexpr int MG_NT_WIDE = 1024;       // threads per tile on levels with few tiles
 constexpr int MG_NT_NARROW = 512;      // threads per tile elsewhere
+constexpr int MG_NT_SLIM = 256;        // a third size the autotuner may choose
+// block-size classes of a multigrid launch (index into the sizes above)
+enum { MGB_SLIM = 0, MGB_NARROW = 1, MGB_WIDE = 2, MGB_COUNT = 3 };
+// calls f(std::integral_constant<int, threads>) for block-size class nb
+template <typename F>
+inline void by_block_size(int nb, F f) {
+    switch (nb) {
+    case MGB_SLIM: f(std::integral_constant<int, MG_NT_SLIM>{}); break;
+    case MGB_WIDE: f(std::integral_constant<int, MG_NT_WIDE>{}); break;
+    default: f(std::integral_constant<int, MG_NT_NARROW>{}); break;
+    }
+}
 // Lanes that sum a tile's reductions: every tile runs at least this many threads,
 // and the sums use exactly these lanes in a fixed order, so a tile's partials have
 // the same bits whichever block size runs it.
@@ -1652,9 +1664,9 @@ struct PSolver {
     int lnp[PS_MAXLEV] = {0};      // and their number
     size_t lps = 0;                // per-slot stride of lpart
     int B = 1;                     // slots
-    // per fine level: whether its down (up) kernel runs MG_NT_WIDE threads per
-    // tile (else MG_NT_NARROW); from the launch-size rule, or timed by tune()
-    bool wdn[PS_MAXLEV] = {}, wup[PS_MAXLEV] = {};
+    // per fine level: the block-size class (MGB_*) of its down (up) kernel, from
+    // the launch-size rule or timed by tune()
+    int wdn[PS_MAXLEV] = {}, wup[PS_MAXLEV] = {};
     bool tuned = false;
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
@@ -1758,32 +1770,35 @@ struct PSolver {
         else launch_timed(tm, k_mg_down<T, NPRE, NTB, false>, gr, NTB, st, a);
     }
     template <int NPRE>
-    static void launch_down_p(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, bool wide, bool mean,
+    static void launch_down_p(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int nb, bool mean,
                               const Timed &tm) {
-        if (wide) launch_down_n<NPRE, MG_NT_WIDE>(gr, st, a, mean, tm);
-        else launch_down_n<NPRE, MG_NT_NARROW>(gr, st, a, mean, tm);
+        by_block_size(nb, [&](auto n) { launch_down_n<NPRE, decltype(n)::value>(gr, st, a, mean, tm); });
     }
-    static void launch_down(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int pre, bool wide, bool mean,
+    static void launch_down(const dim3 &gr, hipStream_t st, const DownArgs<T> &a, int pre, int nb, bool mean,
                             const Timed &tm = Timed{}) {
         switch (pre) {
-        case 0: launch_down_p<0>(gr, st, a, wide, mean, tm); break;
-        case 1: launch_down_p<1>(gr, st, a, wide, mean, tm); break;
-        default: launch_down_p<2>(gr, st, a, wide, mean, tm); break;
+        case 0: launch_down_p<0>(gr, st, a, nb, mean, tm); break;
+        case 1: launch_down_p<1>(gr, st, a, nb, mean, tm); break;
+        default: launch_down_p<2>(gr, st, a, nb, mean, tm); break;
         }
     }
     template <int NPOST>
-    static void launch_up_p(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, bool wide, const Timed &tm) {
-        if (wide) launch_timed(tm, k_mg_up<T, NPOST, MG_NT_WIDE>, gr, MG_NT_WIDE, st, a);
-        else launch_timed(tm, k_mg_up<T, NPOST, MG_NT_NARROW>, gr, MG_NT_NARROW, st, a);
+    static void launch_up_p(const dim3 &gr, hipStream_t st, const UpArgs<T> &a, int nb, const Timed &tm) {
+        by_block_size(nb, [&](auto n) {
+            constexpr int NTB = decltype(n)::value;
+            launch_timed(tm, k_mg_up<T, NPOST, NTB>, gr, NTB, st, a);
+        });
     }
 
-    // A launch with fewer tiles (over all slots) than CUs runs them with 1024
-    // threads each: with one wavefront per SIMD a tile's phases are bound by that
-    // wavefront's serial instruction latency, so four times the threads cut it
-    // about fourfold.  Larger launches use 512 threads per tile (measured ~1% faster
-    // per outer iteration than 256 at 1025^2; 128 is ~4% slower).  A tile's sums
-    // use MG_SUM_LANES lanes whatever the block size, so a case still computes the
-    // same bits alone and in a batch.
+    // The rule without tuning: a launch with fewer tiles (over all slots) than CUs
+    // runs them with 1024 threads each (with one wavefront per SIMD a tile's phases
+    // are bound by that wavefront's serial instruction latency, so four times the
+    // threads cut it about fourfold); larger launches use 512.  tune() times 256,
+    // 512 and 1024 instead: 256 threads fit all 2048 level-0 tiles of a 1025^2 down
+    // leg in one round of resident blocks, and win for the many small tiles of a
+    // batch, while the levels below favour 512 or 1024.  A tile's sums use
+    // MG_SUM_LANES lanes whatever the block size, so a case computes the same bits
+    // alone and in a batch, tuned or not.
     static bool wide_blocks(const dim3 &gr) { return (long)gr.x * gr.y * gr.z < MG_WIDE_BELOW; }
 
     dim3 up_grid(int l) const {
@@ -1793,8 +1808,8 @@ struct PSolver {
         for (int l = 0; l < lc; l++) {
             dim3 gd = down_grid(l);
             gd.z = B;
-            wdn[l] = wide_blocks(gd);
-            wup[l] = wide_blocks(up_grid(l));
+            wdn[l] = wide_blocks(gd) ? MGB_WIDE : MGB_NARROW;
+            wup[l] = wide_blocks(up_grid(l)) ? MGB_WIDE : MGB_NARROW;
         }
     }
 
@@ -1815,25 +1830,31 @@ struct PSolver {
         }
         int rc = CFD_OK;
         // one untimed launch of each block size (a kernel's first launch carries
-        // one-off costs), then the shorter of the two minima over five alternating
-        // timed launches (a minimum ignores launches slowed by other work)
-        auto time_pair = [&](auto launch, bool &pick) {
-            for (int k = 0; k < 2 && rc == CFD_OK; k++) {
-                launch(k == 1);
+        // one-off costs), then the size with the smallest minimum over five rounds of
+        // alternating timed launches (a minimum ignores launches slowed by other work)
+        auto time_sizes = [&](auto launch, int &pick) {
+            for (int k = 0; k < MGB_COUNT && rc == CFD_OK; k++) {
+                launch(k);
                 if (hipGetLastError() != hipSuccess) rc = CFD_E_HIP;
             }
-            float best[2] = {3.0e38f, 3.0e38f};
+            float best[MGB_COUNT];
+            for (int k = 0; k < MGB_COUNT; k++) best[k] = 3.0e38f;
             for (int rep = 0; rep < 5 && rc == CFD_OK; rep++)
-                for (int k = 0; k < 2 && rc == CFD_OK; k++) {
+                for (int k = 0; k < MGB_COUNT && rc == CFD_OK; k++) {
                     float ms = 0.f;
                     if (hipEventRecord(e0, cx.s) != hipSuccess) rc = CFD_E_HIP;
-                    launch(k == 1);
+                    launch(k);
                     if (hipGetLastError() != hipSuccess || hipEventRecord(e1, cx.s) != hipSuccess ||
                         hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
                         rc = CFD_E_HIP;
                     best[k] = std::min(best[k], ms);
                 }
-            if (rc == CFD_OK) pick = best[1] < best[0];
+            if (rc == CFD_OK) {
+                int k0 = 0;
+                for (int k = 1; k < MGB_COUNT; k++)
+                    if (best[k] < best[k0]) k0 = k;
+                pick = k0;
+            }
         };
         const T w = (T)omega;
         const RedArgs ra = cx.ra();
@@ -1845,21 +1866,20 @@ struct PSolver {
                                 -1};
             dim3 gd = down_grid(l);
             gd.z = B;
-            time_pair([&](bool wide) { launch_down(gd, cx.s, a, pre, wide, a.bpart != nullptr); }, wdn[l]);
+            time_sizes([&](int nb) { launch_down(gd, cx.s, a, pre, nb, a.bpart != nullptr); }, wdn[l]);
             const UpArgs<T> u{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.t, L.t, w, cx.scal, -1};
             const dim3 gu = up_grid(l);
             if (l == 0 && fused_tail()) {
                 const Tail tl{MGA_T, bi[5], nullptr, ra};
-                time_pair([&](bool wide) {
-                    if (wide) launch_up_apply<MG_NT_WIDE>(gu, cx.s, u, tl, Timed{});
-                    else launch_up_apply<MG_NT_NARROW>(gu, cx.s, u, tl, Timed{});
+                time_sizes([&](int nb) {
+                    by_block_size(nb, [&](auto n) { launch_up_apply<decltype(n)::value>(gu, cx.s, u, tl, Timed{}); });
                 }, wup[l]);
             } else {
-                time_pair([&](bool wide) {
+                time_sizes([&](int nb) {
                     switch (post) {
-                    case 0: launch_up_p<0>(gu, cx.s, u, wide, Timed{}); break;
-                    case 1: launch_up_p<1>(gu, cx.s, u, wide, Timed{}); break;
-                    default: launch_up_p<2>(gu, cx.s, u, wide, Timed{}); break;
+                    case 0: launch_up_p<0>(gu, cx.s, u, nb, Timed{}); break;
+                    case 1: launch_up_p<1>(gu, cx.s, u, nb, Timed{}); break;
+                    default: launch_up_p<2>(gu, cx.s, u, nb, Timed{}); break;
                     }
                 }, wup[l]);
             }
@@ -1919,8 +1939,7 @@ struct PSolver {
             const dim3 gr = up_grid(l);
             const Timed tm = l == 0 ? cx.prof(cx.owner, PK_MG_UP) : Timed{};
             if (l == 0 && tail) {
-                if (wup[l]) launch_up_apply<MG_NT_WIDE>(gr, cx.s, a, *tail, tm);
-                else launch_up_apply<MG_NT_NARROW>(gr, cx.s, a, *tail, tm);
+                by_block_size(wup[l], [&](auto n) { launch_up_apply<decltype(n)::value>(gr, cx.s, a, *tail, tm); });
                 CFD_LAUNCH_CHECK();
                 continue;
             }
