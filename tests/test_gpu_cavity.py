@@ -8,10 +8,10 @@ from tests.conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def pair(n, skew=0.05, beta=None, **kw):
+def pair(n, skew=0.05, beta=None, n1=None, **kw):
     from cfd_solver_amd.cavity import CavityCase, default_params
     from oracle.oracle import CavityOracle, cav_params
-    g = cavity_grid(n, skew=skew, beta=beta)
+    g = cavity_grid(n, n1, skew=skew, beta=beta)
     gp = default_params(**kw)
     op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields()})  # precision has no oracle field
     return CavityCase(g, gp), CavityOracle(g, op), g
@@ -41,6 +41,44 @@ def test_p_apply_and_solve_match_oracle(n, post):
     xg = xg[1:-1, 1:-1] - xg[1:-1, 1:-1].mean()
     xc = xc[1:-1, 1:-1] - xc[1:-1, 1:-1].mean()
     assert rel_l2(xg, xc) < 1e-9
+
+
+def test_rectangular_grid_p_solve_matches_oracle():
+    """97 x 41 nodes, neither 2^k + 1: tile edges fall inside the grid along xi (96 is
+    not a multiple of 64) and exactly on the north wall along eta (40 is a multiple of
+    8, so that wall row lies outside every tile and is set by the edge tiles); four
+    multigrid levels of unequal shape (97x41, 49x21, 25x11, 13x6)."""
+    print("\n[gpu] p' operator and solve, 97x41 skew 0.1 tanh 1.5", flush=True)
+    n0, n1 = 97, 41
+    gpu, cpu, g = pair(n0, n1=n1, skew=0.1, beta=1.5)
+    rng = np.random.default_rng(11)
+    d = 0.5 + rng.random((n1, n0))
+    x = rng.standard_normal((n1, n0))
+    assert rel_l2(gpu.p_apply(d, x), cpu.p_apply(d, x)) < 1e-13
+    q = np.zeros((n1, n0))
+    q[1:-1, 1:-1] = rng.standard_normal((n1 - 2, n0 - 2))
+    q[1:-1, 1:-1] -= q[1:-1, 1:-1].mean()
+    xg, itg, rg = gpu.p_solve(d, q, rtol=1e-10, maxit=200)
+    xc, itc, rc = cpu.p_solve(d, q, rtol=1e-10, maxit=200)
+    assert rg <= 1e-10 and itg == itc, (itg, itc, rg)
+    xg = xg[1:-1, 1:-1] - xg[1:-1, 1:-1].mean()
+    xc = xc[1:-1, 1:-1] - xc[1:-1, 1:-1].mean()
+    assert rel_l2(xg, xc) < 1e-9
+
+
+def test_rectangular_grid_outer_iterations_match_oracle():
+    """30 FP64 outer iterations on the 97 x 41 grid: equal p' iteration counts every
+    iteration and fields within 1e-9 of the oracle."""
+    print("\n[gpu] cavity 97x41, 30 outer iterations", flush=True)
+    gpu, cpu, g = pair(97, n1=41, skew=0.1, beta=1.5, Re=200.0, Ri=1.0)
+    for k in range(30):
+        sg = gpu.step()
+        sc = cpu.step()
+        assert sg.p_iters == sc.p_iters, k
+    fg, fc = gpu.fields(), cpu.fields()
+    for k in ("u", "v", "T", "p"):
+        assert rel_l2(fg[k], fc[k]) < 1e-9, k
+    assert abs(sg.nu_hot - sc.nu_hot) < 1e-9 * abs(sc.nu_hot)
 
 
 def test_residuals_match_oracle():
