@@ -961,15 +961,17 @@ __global__ __launch_bounds__(NT) void k_q_norms(int n0, int n1, const double *__
         const size_t o = zslot() * n0 * n1;
         q += o; pc += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
     double v[2] = {0.0, 0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const int i = fn.i, j = fn.j;
         const size_t P = fn.P;
         pc[P] = 0.0;
         if (inner_node(i, j, n0, n1)) {
-            v[0] = q[P];
-            v[1] = q[P] * q[P];
+            v[0] += q[P];
+            v[1] += q[P] * q[P];
         }
     }
     block_partials<2>(v, ra.part, ra.pstride);
@@ -986,10 +988,12 @@ __global__ __launch_bounds__(NT) void k_defect(FArgs<double> a, const double *__
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o; r += o;
     }
-    const FlatNode fn = flat_node(a.n0, a.n1);
-    const int i = fn.i, j = fn.j;
     double v[2] = {0.0, 0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(a.n0, a.n1, m);
+        if (!fn.in) continue;
+        const int i = fn.i, j = fn.j;
         const size_t P = fn.P;
         double y = 0.0;
         if (inner_node(i, j, a.n0, a.n1)) {
@@ -997,8 +1001,8 @@ __global__ __launch_bounds__(NT) void k_defect(FArgs<double> a, const double *__
                     const size_t Q = idx(ii, jj, a.n0);
                     return pc[Q] + (double)ef[Q];
                 });
-            v[0] = y;
-            v[1] = y * y;
+            v[0] += y;
+            v[1] += y * y;
         }
         r[P] = y;
     }
@@ -1016,10 +1020,12 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
         pc += o;
         if (ACC) ef += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
     double v[1] = {0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const int i = fn.i, j = fn.j;
         const size_t P = fn.P;
         double x = pc[P];
         if (ACC) {
@@ -1027,7 +1033,7 @@ __global__ __launch_bounds__(NT) void k_pc_final(int n0, int n1, double *__restr
             pc[P] = x;
             ef[P] = 0.0f;
         }
-        if (inner_node(i, j, n0, n1)) v[0] = x;
+        if (inner_node(i, j, n0, n1)) v[0] += x;
     }
     block_partials<1>(v, ra.part, ra.pstride);
 }
@@ -1046,17 +1052,19 @@ __global__ __launch_bounds__(NT) void k_defect_final(FArgs<double> a, const doub
         const size_t o = zslot() * a.n0 * a.n1;
         a = fz(a, o); q += o; pc += o; ef += o;
     }
-    const FlatNode fn = flat_node(a.n0, a.n1);
-    const int i = fn.i, j = fn.j;
     double v[3] = {0.0, 0.0, 0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(a.n0, a.n1, m);
+        if (!fn.in) continue;
+        const int i = fn.i, j = fn.j;
         const size_t P = fn.P;
         const double x = (double)ef[P];
         if (inner_node(i, j, a.n0, a.n1)) {
             const double y = q[P] - frow_f<double>(a, i, j, [&](int ii, int jj) { return (double)ef[idx(ii, jj, a.n0)]; });
-            v[0] = y;
-            v[1] = y * y;
-            v[2] = x;
+            v[0] += y;
+            v[1] += y * y;
+            v[2] += x;
         }
         pc[P] = x;
     }

@@ -102,17 +102,20 @@ __device__ __forceinline__ bool inner_node(int i, int j, int n0, int n1) {
     return i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2;
 }
 
-// The elementwise BiCGStab kernels run one thread per node of a slot in memory
-// order, PS_NT nodes per block, so a launch has ceil(n0 n1 / PS_NT) blocks along x
-// (4105 at 1025^2 against 4369 64 x 4 tiles, 17 against 34 at 65^2) and every
-// block reads and writes one contiguous range.
+// The elementwise kernels take the nodes of a slot in memory order: FLAT_NPT
+// nodes per thread, each pass over a contiguous PS_NT-node chunk, so a launch has
+// ceil(n0 n1 / (FLAT_NPT PS_NT)) blocks along x (2053 at 1025^2, one round of
+// resident blocks; 9 at 65^2) and every block reads and writes one contiguous
+// range, with the loads of all its nodes issued together.
+constexpr int FLAT_NPT = 2;
 struct FlatNode {
     int i, j;
     size_t P;
-    bool in;  // the thread has a node
+    bool in;  // the node exists
 };
-__device__ __forceinline__ FlatNode flat_node(int n0, int n1) {
-    const unsigned q = blockIdx.x * (unsigned)PS_NT + (unsigned)tlin();
+// the m-th node (0 <= m < FLAT_NPT) of this thread
+__device__ __forceinline__ FlatNode flat_node(int n0, int n1, int m) {
+    const unsigned q = (blockIdx.x * (unsigned)FLAT_NPT + (unsigned)m) * (unsigned)PS_NT + (unsigned)tlin();
     FlatNode f;
     f.in = q < (unsigned)n0 * (unsigned)n1;
     const unsigned j = f.in ? q / (unsigned)n0 : 0u;
@@ -121,7 +124,7 @@ __device__ __forceinline__ FlatNode flat_node(int n0, int n1) {
     f.P = f.in ? (size_t)q : 0;
     return f;
 }
-inline int nflat(int n0, int n1) { return (int)(((long)n0 * n1 + PS_NT - 1) / PS_NT); }
+inline int nflat(int n0, int n1) { return (int)(((long)n0 * n1 + FLAT_NPT * PS_NT - 1) / (FLAT_NPT * PS_NT)); }
 
 // ---------------------------------------------------------------------------
 // Reductions in two launches: a kernel stores one partial per block and value
@@ -1089,27 +1092,27 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start1(int og, int n0, int n1, con
         b += o; x += o; r += o;
         if (acc) acc += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
-    const bool in = fn.in;
-    const size_t P = fn.P;
-    if (in) {
+    const bool skip = og >= 0 && ra.scal[og] != 0.0;
+    double v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const size_t P = fn.P;
         if (acc) acc[P] += (double)x[P];
         x[P] = T(0);
-    }
-    if (og >= 0 && ra.scal[og] != 0.0) {
-        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
-        return;
-    }
-    double v[2] = {0.0, 0.0};
-    if (in) {
+        if (skip) continue;
         T rv = T(0);
-        if (inner_node(i, j, n0, n1)) {
+        if (inner_node(fn.i, fn.j, n0, n1)) {
             rv = (T)b[P];
-            v[0] = (double)rv;
-            v[1] = (double)rv * (double)rv;
+            v[0] += (double)rv;
+            v[1] += (double)rv * (double)rv;
         }
         r[P] = rv;
+    }
+    if (skip) {
+        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
+        return;
     }
     block_partials<2>(v, ra.part, ra.pstride);
 }
@@ -1124,17 +1127,18 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start2(int og, int n0, int n1, T *
         const size_t o = zslot() * n0 * n1;
         r += o; r0 += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
     const double mean = ra.scal[S_SUM] / ((double)(n0 - 2) * (double)(n1 - 2));
     double v[1] = {0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
         const size_t P = fn.P;
         T rv = T(0);
-        if (inner_node(i, j, n0, n1)) {
+        if (inner_node(fn.i, fn.j, n0, n1)) {
             rv = (T)((double)r[P] - mean);
             r[P] = rv;
-            v[0] = (double)rv * (double)rv;
+            v[0] += (double)rv * (double)rv;
         }
         r0[P] = rv;
     }
@@ -1155,25 +1159,27 @@ __global__ __launch_bounds__(PS_NT) void k_bi_start_m(int og, int n0, int n1, co
         const size_t o = zslot() * n0 * n1;
         b += o; x += o; r += o; r0 += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
-    const bool in = fn.in;
-    const size_t P = fn.P;
-    if (in) x[P] = T(0);
-    if (og >= 0 && ra.scal[og] != 0.0) {
-        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
-        return;
-    }
+    const bool skip = og >= 0 && ra.scal[og] != 0.0;
     const double mean = ra.scal[S_SUM] / ((double)(n0 - 2) * (double)(n1 - 2));
     double v[1] = {0.0};
-    if (in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const size_t P = fn.P;
+        x[P] = T(0);
+        if (skip) continue;
         T rv = T(0);
-        if (inner_node(i, j, n0, n1)) {
+        if (inner_node(fn.i, fn.j, n0, n1)) {
             rv = (T)(b[P] - mean);
-            v[0] = (double)rv * (double)rv;
+            v[0] += (double)rv * (double)rv;
         }
         r[P] = rv;
         r0[P] = rv;
+    }
+    if (skip) {
+        if (blin() == 0 && tlin() == 0) ra.scal[S_DONE] = 1.0;
+        return;
     }
     block_partials<1>(v, ra.part, ra.pstride);
 }
@@ -1194,12 +1200,19 @@ __global__ __launch_bounds__(PS_NT) void k_bi_p(int n0, int n1, int it, double *
         if (rho == 0.0) scal[S_DONE] = 1.0;
     }
     if (rho == 0.0) return;
-    const FlatNode fn = flat_node(n0, n1);
-    if (!fn.in) return;
-    const size_t P = fn.P;
-    if (it == 1) { pv[P] = r[P]; return; }
-    const double beta = (rho / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
-    pv[P] = (T)((double)r[P] + beta * ((double)pv[P] - scal[S_OMEGA] * (double)vv[P]));
+    double beta = 0.0, omega = 0.0;  // read only after the first iteration
+    if (it != 1) {
+        beta = (rho / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_OMEGA]);
+        omega = scal[S_OMEGA];
+    }
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const size_t P = fn.P;
+        if (it == 1) pv[P] = r[P];
+        else pv[P] = (T)((double)r[P] + beta * ((double)pv[P] - omega * (double)vv[P]));
+    }
 }
 
 // The finest level's up leg (k_mg_up with two post-sweeps) fused with the
@@ -1379,15 +1392,16 @@ __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restr
         const size_t o = zslot() * n0 * n1;
         r += o; vv += o; s += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
     const double alpha = ra.scal[S_ALPHA];
     double v[1] = {0.0};
-    if (fn.in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
         const size_t P = fn.P;
         const T sv = (T)((double)r[P] - alpha * (double)vv[P]);
         s[P] = sv;
-        if (inner_node(i, j, n0, n1)) v[0] = (double)sv * (double)sv;
+        if (inner_node(fn.i, fn.j, n0, n1)) v[0] += (double)sv * (double)sv;
     }
     block_partials<1>(v, ra.part, ra.pstride);
 }
@@ -1434,24 +1448,28 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
         const size_t o = zslot() * n0 * n1;
         ph += o; sh += o; s += o; t += o; r0 += o; x += o; r += o;
     }
-    const FlatNode fn = flat_node(n0, n1);
-    const int i = fn.i, j = fn.j;
-    const bool in = fn.in;
-    const size_t P = fn.P;
     const double alpha = sc[S_ALPHA];
     if (sconv) {
-        if (in) x[P] = (T)((double)x[P] + alpha * (double)ph[P]);
+#pragma unroll
+        for (int m = 0; m < FLAT_NPT; m++) {
+            const FlatNode fn = flat_node(n0, n1, m);
+            if (fn.in) x[fn.P] = (T)((double)x[fn.P] + alpha * (double)ph[fn.P]);
+        }
         return;
     }
     const double omega = sc[S_OMEGA];
     double v[2] = {0.0, 0.0};
-    if (in) {
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        const FlatNode fn = flat_node(n0, n1, m);
+        if (!fn.in) continue;
+        const size_t P = fn.P;
         x[P] = (T)((double)x[P] + alpha * (double)ph[P] + omega * (double)sh[P]);
         const T rv = (T)((double)s[P] - omega * (double)t[P]);
         r[P] = rv;
-        if (inner_node(i, j, n0, n1)) {
-            v[0] = (double)rv * (double)rv;
-            v[1] = (double)r0[P] * (double)rv;
+        if (inner_node(fn.i, fn.j, n0, n1)) {
+            v[0] += (double)rv * (double)rv;
+            v[1] += (double)r0[P] * (double)rv;
         }
     }
     block_partials<2>(v, ra.part, ra.pstride);
