@@ -279,9 +279,10 @@ typedef struct {
                            the p' BiCGStab then runs p_maxit unrolled iterations with
                            device-side convergence tests (no host synchronisation,
                            so batches on several streams overlap)                   */
-    int mg_tune;        /* 1 (default): each multigrid level's block size is timed
-                           once, before the first solve, and the faster kept (results
-                           do not depend on it); 0: the launch-size rule            */
+    int mg_tune;        /* 1 (default): each multigrid level's block size (256, 512
+                           or 1024 threads per tile) is timed once, before the first
+                           solve, and the fastest kept (results do not depend on
+                           it); 0: the launch-size rule                             */
 } cfd_params;
 
 typedef struct {
