@@ -104,10 +104,12 @@ __device__ __forceinline__ bool inner_node(int i, int j, int n0, int n1) {
 
 // The elementwise kernels take the nodes of a slot in memory order: FLAT_NPT
 // nodes per thread, each pass over a contiguous PS_NT-node chunk, so a launch has
-// ceil(n0 n1 / (FLAT_NPT PS_NT)) blocks along x (2053 at 1025^2, one round of
-// resident blocks; 9 at 65^2) and every block reads and writes one contiguous
-// range, with the loads of all its nodes issued together.
-constexpr int FLAT_NPT = 2;
+// ceil(n0 n1 / (FLAT_NPT PS_NT)) blocks along x (1027 at 1025^2, fewer than the
+// resident-block capacity; 5 at 65^2) and every block reads and writes one
+// contiguous range, with the loads of all its nodes issued together and one block
+// reduction for all of them (measured: one node per thread 9.1 us per k_bi_xr at
+// 1025^2, two 8.0-8.4 us, four 7.3 us).
+constexpr int FLAT_NPT = 4;
 struct FlatNode {
     int i, j;
     size_t P;
