@@ -186,6 +186,10 @@ __device__ __forceinline__ void sum_partials(const double *part, int pstride, in
 
 // threads of the one-block kernels that finish a reduction
 constexpr int FIN_NT = 1024;
+// the BiCGStab finishes: at 1025^2 their phases have at most 2048 partials per
+// value (one pass of eight loads per thread at 256 threads; larger grids take
+// more passes), and four wavefronts reduce and start faster than sixteen
+constexpr int BI_FIN_NT = 256;
 
 // ---------------------------------------------------------------------------
 // operator rows
@@ -1540,13 +1544,13 @@ __device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV
 // FIN_XR with flag != null also writes the slot's end-of-iteration flag into host
 // memory: (seq << 1) | done, done when the slot is skipped or has converged.
 template <int OP>
-__global__ __launch_bounds__(FIN_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot,
+__global__ __launch_bounds__(BI_FIN_NT) void k_bi_fin(int gs, RedArgs ra, int nparts, int it, double rtol, int rtol_slot,
                                                    unsigned long long seq = 0, unsigned long long *flag = nullptr) {
     ra = rz(ra);
     const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
     constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
     double tot[NV];
-    sum_partials<NV, FIN_NT>(ra.part, ra.pstride, nparts, tot);
+    sum_partials<NV, BI_FIN_NT>(ra.part, ra.pstride, nparts, tot);
     if (tlin() != 0) return;
     if (!skip) bi_fin_scalars<OP>(ra.scal, tot, it, rtol, rtol_slot);
     if (OP == FIN_XR && flag) {
@@ -2002,10 +2006,10 @@ struct PSolver {
             k_bi_start_m<T><<<gf, bl, 0, cx.s>>>(og, n0, n1, b, x, r, r0, ra);
         } else {
             k_bi_start1<T><<<gf, bl, 0, cx.s>>>(og, n0, n1, b, x, acc, r, ra);
-            k_bi_fin<FIN_START1><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
+            k_bi_fin<FIN_START1><<<g1, BI_FIN_NT, 0, cx.s>>>(og, ra, nb, 0, 0.0, -1);
             k_bi_start2<T><<<gf, bl, 0, cx.s>>>(og, n0, n1, r, r0, ra);
         }
-        k_bi_fin<FIN_START2><<<g1, FIN_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
+        k_bi_fin<FIN_START2><<<g1, BI_FIN_NT, 0, cx.s>>>(og, ra, nb, 0, rtol, rtol_slot);
         CFD_LAUNCH_CHECK();
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
@@ -2015,22 +2019,22 @@ struct PSolver {
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
                              (const T *)r0, ra);
-            k_bi_fin<FIN_V><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
+            k_bi_fin<FIN_V><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
             k_bi_s<T><<<gf, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
-            k_bi_fin<FIN_S><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
+            k_bi_fin<FIN_S><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             const Tail tt{MGA_T, t, nullptr, ra};
             CFD_RET(precond(cx, s, sh, S_DONE, fz ? &tt : nullptr));
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
                              (const T *)s, ra);
-            k_bi_fin<FIN_T><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
+            k_bi_fin<FIN_T><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
             k_bi_xr<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
             // eager: the iteration's end flags go straight to host memory and the host
             // waits for them (no copy, no stream synchronisation)
             const bool flags = !graph && cx.flag_d && cx.seq;
             const unsigned long long sq = flags ? ++*cx.seq : 0;
-            k_bi_fin<FIN_XR><<<g1, FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1, sq, flags ? cx.flag_d : nullptr);
+            k_bi_fin<FIN_XR><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1, sq, flags ? cx.flag_d : nullptr);
             CFD_LAUNCH_CHECK();
             if (flags) {
                 bool done = false;
