@@ -788,7 +788,8 @@ __global__ void k_uvt_walls(CGrid g, double *__restrict__ u, double *__restrict_
 // Partial-sum slots (pstride each) per case slot: 0-2 for reductions finished
 // right away, 3-4 for the energy/momentum residual norms and 5 for the mass
 // residual, which stay there until the end-of-iteration finish below.
-enum { PS_RES = 3, PS_MASS = 5, PS_SLOTS = 6 };
+enum { PS_RES = 3, PS_MASS = 5, PS_SLOTS = 8 };  // rows 6-7: the p' solver's operator-application phases
+static_assert(PS_SLOTS >= PS_APPLY_ROW + 2 && PS_MASS < PS_APPLY_ROW, "partial rows");
 
 // Wall-Nusselt integrand of segment [i, i + 1] of the bottom (s = 1, j = 0) or top
 // (s = -1, j = n1 - 1) wall from its twelve loaded values (oracle: nusselt).

@@ -11,16 +11,18 @@
 // the faces of P, W and S, so an operator application reads 4 coefficient
 // arrays and the vector instead of 5 metric arrays, d and the vector.
 //
-// Launch structure (one BiCGStab iteration: 9 + 2 x (2 x fine levels + 1) launches):
+// Launch structure (one BiCGStab iteration: 5 + 2 x (2 x fine levels + 1) launches,
+// two more where the operator application is not fused into the finest up leg):
 //  * a fine multigrid level costs two launches: k_mg_down (pre-smoothing from
 //    zero, residual, full-weighting restriction and the coarse interior sum, on
 //    an LDS tile) and k_mg_up (bilinear prolongation and post-smoothing on an
 //    LDS tile); levels of at most 33 x 33 nodes run in one single-workgroup
 //    kernel resident in LDS (k_mg_coarse);
-//  * every BiCGStab reduction is finished by a one-block kernel (k_bi_fin) that
-//    also evaluates the solver's scalar recurrences and convergence tests; the
-//    coarse-level means of the V-cycle are summed from the block partials by the
-//    kernel that consumes them;
+//  * the ||s|| and end-of-iteration reductions are finished by one-block kernels
+//    (k_bi_fin) that also evaluate the solver's scalar recurrences and
+//    convergence tests; alpha and omega, and the coarse-level means of the
+//    V-cycle, are totalled from the block partials by the kernels that consume
+//    them;
 //  * the convergence state lives in device scalars: the same kernel sequence
 //    runs eagerly (with host reads of the flags between half-iterations) or
 //    captured in a graph (all iterations unrolled; kernels after convergence
@@ -1077,6 +1079,11 @@ struct RedArgs {
     size_t psz;
     double *scal;   // device scalars, slot stride SS
 };
+// Partial rows PS_APPLY_ROW and PS_APPLY_ROW + 1 hold the operator-application
+// phases of BiCGStab ((r0, v); (t, s) and (t, t)), which the next vector kernel
+// totals itself while it writes its own partials to rows 0-1; the owner of the
+// buffers allocates at least PS_APPLY_ROW + 2 rows.
+constexpr int PS_APPLY_ROW = 6;
 // the block's slot view of the reduction buffers
 __device__ __forceinline__ RedArgs rz(RedArgs ra) {
     ra.part += zslot() * ra.psz;
@@ -1389,25 +1396,49 @@ __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict_
 }
 
 // s = r - alpha v; partials of (s, s)
+// alpha = rho / (r0, v) from the nvp partials of the v phase (vpart, row
+// PS_APPLY_ROW), totalled by every block in the fixed order of a one-block finish
+// (the same bits) while its own nodes load; block 0 keeps (r0, v) and alpha in the
+// device scalars for the kernels that follow
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restrict__ r, const T *__restrict__ vv,
-                                                T *__restrict__ s, RedArgs ra) {
+                                                T *__restrict__ s, RedArgs ra, const double *__restrict__ vpart,
+                                                int nvp) {
     ra = rz(ra);
     if (slot_skip(ra.scal, S_DONE)) return;
+    vpart += zslot() * ra.psz;
     {
         const size_t o = zslot() * n0 * n1;
         r += o; vv += o; s += o;
     }
-    const double alpha = ra.scal[S_ALPHA];
+    FlatNode fn[FLAT_NPT];
+    T rl[FLAT_NPT], vl[FLAT_NPT];
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        fn[m] = flat_node(n0, n1, m);
+        rl[m] = r[fn[m].P];  // P = 0 for a missing node: in bounds, unused
+        vl[m] = vv[fn[m].P];
+    }
+    double tv[1];
+    sum_partials<1, PS_NT>(vpart, ra.pstride, nvp, tv);
+    __shared__ double s_alpha;
+    if (tlin() == 0) {
+        const double a = ra.scal[S_RHO] / tv[0];
+        s_alpha = a;
+        if (blin() == 0) {
+            ra.scal[S_RV] = tv[0];
+            ra.scal[S_ALPHA] = a;
+        }
+    }
+    __syncthreads();
+    const double alpha = s_alpha;
     double v[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < FLAT_NPT; m++) {
-        const FlatNode fn = flat_node(n0, n1, m);
-        if (!fn.in) continue;
-        const size_t P = fn.P;
-        const T sv = (T)((double)r[P] - alpha * (double)vv[P]);
-        s[P] = sv;
-        if (inner_node(fn.i, fn.j, n0, n1)) v[0] += (double)sv * (double)sv;
+        if (!fn[m].in) continue;
+        const T sv = (T)((double)rl[m] - alpha * (double)vl[m]);
+        s[fn[m].P] = sv;
+        if (inner_node(fn[m].i, fn[m].j, n0, n1)) v[0] += (double)sv * (double)sv;
     }
     block_partials<1>(v, ra.part, ra.pstride);
 }
@@ -1439,17 +1470,22 @@ __global__ __launch_bounds__(PS_NT) void k_bi_t(FArgs<T> a, const T *__restrict_
 }
 
 // x += alpha ph + omega sh, r = s - omega t, partials of (r, r), (r0, r); in the
-// iteration that stopped at the ||s|| test only x += alpha ph
+// iteration that stopped at the ||s|| test only x += alpha ph.  omega = (t, s) /
+// (t, t) from the ntp partials of the t phase (tpart, rows PS_APPLY_ROW and
+// PS_APPLY_ROW + 1), totalled by every block as k_bi_s totals alpha's; block 0
+// keeps the totals and omega in the device scalars
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T *__restrict__ ph,
                                                  const T *__restrict__ sh, const T *__restrict__ s,
                                                  const T *__restrict__ t, const T *__restrict__ r0, T *__restrict__ x,
-                                                 T *__restrict__ r, RedArgs ra) {
+                                                 T *__restrict__ r, RedArgs ra, const double *__restrict__ tpart,
+                                                 int ntp) {
     ra = rz(ra);
-    const double *sc = ra.scal;
+    double *sc = ra.scal;
     if (sc[S_ACTIVE] == 0.0) return;
     const bool sconv = sc[S_SCONV] == (double)it;
     if (!sconv && sc[S_DONE] != 0.0) return;
+    tpart += zslot() * ra.psz;
     {
         const size_t o = zslot() * n0 * n1;
         ph += o; sh += o; s += o; t += o; r0 += o; x += o; r += o;
@@ -1463,24 +1499,50 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
         }
         return;
     }
-    const double omega = sc[S_OMEGA];
+    FlatNode fn[FLAT_NPT];
+    T xl[FLAT_NPT], pl[FLAT_NPT], hl[FLAT_NPT], sl[FLAT_NPT], tl[FLAT_NPT], ol[FLAT_NPT];
+#pragma unroll
+    for (int m = 0; m < FLAT_NPT; m++) {
+        fn[m] = flat_node(n0, n1, m);
+        const size_t P = fn[m].P;  // 0 for a missing node: in bounds, unused
+        xl[m] = x[P];
+        pl[m] = ph[P];
+        hl[m] = sh[P];
+        sl[m] = s[P];
+        tl[m] = t[P];
+        ol[m] = r0[P];
+    }
+    double tt[2];
+    sum_partials<2, PS_NT>(tpart, ra.pstride, ntp, tt);
+    __shared__ double s_omega;
+    if (tlin() == 0) {
+        const double om = tt[1] > 0.0 ? tt[0] / tt[1] : 0.0;
+        s_omega = om;
+        if (blin() == 0) {
+            sc[S_TS] = tt[0];
+            sc[S_TT] = tt[1];
+            sc[S_OMEGA] = om;
+        }
+    }
+    __syncthreads();
+    const double omega = s_omega;
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < FLAT_NPT; m++) {
-        const FlatNode fn = flat_node(n0, n1, m);
-        if (!fn.in) continue;
-        const size_t P = fn.P;
-        x[P] = (T)((double)x[P] + alpha * (double)ph[P] + omega * (double)sh[P]);
-        const T rv = (T)((double)s[P] - omega * (double)t[P]);
+        if (!fn[m].in) continue;
+        const size_t P = fn[m].P;
+        x[P] = (T)((double)xl[m] + alpha * (double)pl[m] + omega * (double)hl[m]);
+        const T rv = (T)((double)sl[m] - omega * (double)tl[m]);
         r[P] = rv;
-        if (inner_node(fn.i, fn.j, n0, n1)) {
+        if (inner_node(fn[m].i, fn[m].j, n0, n1)) {
             v[0] += (double)rv * (double)rv;
-            v[1] += (double)r0[P] * (double)rv;
+            v[1] += (double)ol[m] * (double)rv;
         }
     }
     block_partials<2>(v, ra.part, ra.pstride);
 }
 
+// (the v and t phases, FIN_V and FIN_T, are totalled inside k_bi_s and k_bi_xr)
 enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
 
 // the scalar recurrence / convergence test of phase OP from the totals
@@ -1508,9 +1570,6 @@ __device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV
         s[S_ALPHA] = 1.0;
         s[S_OMEGA] = 1.0;
         s[S_DONE] = (bn == 0.0 || rn <= tol) ? 1.0 : 0.0;
-    } else if (OP == FIN_V) {
-        s[S_RV] = tot[0];
-        s[S_ALPHA] = s[S_RHO] / tot[0];
     } else if (OP == FIN_S) {
         s[S_SS] = tot[0];
         const double sn = sqrt(tot[0]);
@@ -1520,10 +1579,6 @@ __device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV
             s[S_RN] = sn;
             s[S_RELRES] = s[S_BN] > 0.0 ? sn / s[S_BN] : 0.0;
         }
-    } else if (OP == FIN_T) {
-        s[S_TS] = tot[0];
-        s[S_TT] = tot[NV - 1];
-        s[S_OMEGA] = tot[NV - 1] > 0.0 ? tot[0] / tot[NV - 1] : 0.0;
     } else {
         s[S_RR] = tot[0];
         s[S_R0R] = tot[NV - 1];
@@ -1548,7 +1603,8 @@ __global__ __launch_bounds__(BI_FIN_NT) void k_bi_fin(int gs, RedArgs ra, int np
                                                    unsigned long long seq = 0, unsigned long long *flag = nullptr) {
     ra = rz(ra);
     const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
-    constexpr int NV = (OP == FIN_START1 || OP == FIN_T || OP == FIN_XR) ? 2 : 1;
+    static_assert(OP != FIN_V && OP != FIN_T, "totalled by the next vector kernel");
+    constexpr int NV = (OP == FIN_START1 || OP == FIN_XR) ? 2 : 1;
     double tot[NV];
     sum_partials<NV, BI_FIN_NT>(ra.part, ra.pstride, nparts, tot);
     if (tlin() != 0) return;
@@ -1894,7 +1950,9 @@ struct PSolver {
             const UpArgs<T> u{fargs(L), L.invd, L.b, L.x, C.n0, C.n1, C.t, L.t, w, cx.scal, -1};
             const dim3 gu = up_grid(l);
             if (l == 0 && fused_tail()) {
-                const Tail tl{MGA_T, bi[5], nullptr, ra};
+                RedArgs rap = ra;
+                rap.part += (size_t)PS_APPLY_ROW * ra.pstride;
+                const Tail tl{MGA_T, bi[5], nullptr, rap};
                 time_sizes([&](int nb) {
                     by_block_size(nb, [&](auto n) { launch_up_apply<decltype(n)::value>(gu, cx.s, u, tl, Timed{}); });
                 }, wup[l]);
@@ -2002,6 +2060,8 @@ struct PSolver {
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
+        RedArgs rap = ra;  // rows of the operator-application phases
+        rap.part += (size_t)PS_APPLY_ROW * ra.pstride;
         if (sums_ready && !acc) {
             k_bi_start_m<T><<<gf, bl, 0, cx.s>>>(og, n0, n1, b, x, r, r0, ra);
         } else {
@@ -2014,22 +2074,20 @@ struct PSolver {
         for (int it = 1; it <= maxit; it++) {
             k_bi_p<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, cx.scal, r, vv, pv);
             CFD_LAUNCH_CHECK();
-            const Tail tv{MGA_V, vv, r0, ra};
+            const Tail tv{MGA_V, vv, r0, rap};
             CFD_RET(precond(cx, pv, ph, S_DONE, fz ? &tv : nullptr));
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
-                             (const T *)r0, ra);
-            k_bi_fin<FIN_V><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
-            k_bi_s<T><<<gf, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra);
+                             (const T *)r0, rap);
+            k_bi_s<T><<<gf, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra, (const double *)rap.part, nbt);
             k_bi_fin<FIN_S><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
-            const Tail tt{MGA_T, t, nullptr, ra};
+            const Tail tt{MGA_T, t, nullptr, rap};
             CFD_RET(precond(cx, s, sh, S_DONE, fz ? &tt : nullptr));
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
-                             (const T *)s, ra);
-            k_bi_fin<FIN_T><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nbt, it, 0.0, -1);
-            k_bi_xr<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra);
+                             (const T *)s, rap);
+            k_bi_xr<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra, (const double *)rap.part, nbt);
             // eager: the iteration's end flags go straight to host memory and the host
             // waits for them (no copy, no stream synchronisation)
             const bool flags = !graph && cx.flag_d && cx.seq;
