@@ -19,9 +19,9 @@ CASES = [  # (skew, beta, physics)
 ]
 
 
-def _alone(n, skew, beta, phys, precision, steps):
+def _alone(n, skew, beta, phys, precision, steps, n1=None):
     from cfd_solver_amd.cavity import CavityCase, default_params
-    c = CavityCase(cavity_grid(n, skew=skew, beta=beta), default_params(precision=precision, **phys))
+    c = CavityCase(cavity_grid(n, n1, skew=skew, beta=beta), default_params(precision=precision, **phys))
     st = None
     for _ in range(steps):
         st = c.step()
@@ -117,4 +117,27 @@ def test_batch_slot_equals_alone_at_257(precision):
     for k in ("u", "v", "T", "p"):
         assert np.array_equal(fa[k], fb[k]), (k, np.abs(fa[k] - fb[k]).max())
     assert st[0].p_iters == sa.p_iters and st[0].mass_l1 == sa.mass_l1
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_slots_equal_alone_on_rectangular_grid(precision):
+    """97 x 41 nodes: the north wall row lies outside every tile (40 is a multiple
+    of the tile height), so the edge tiles of each slot set it; slot 1 stays empty.
+    Every filled slot is bitwise equal to its case alone."""
+    print(f"\n[gpu] 97x41: 3-slot batch (slot 1 empty) vs alone, precision {precision}", flush=True)
+    from cfd_solver_amd.cavity import CavityBatch, default_params
+    n0, n1, steps = 97, 41, 4
+    g0 = cavity_grid(n0, n1)
+    b = CavityBatch(n0, n1, g0.dxi, g0.deta, 3, default_params(precision=precision))
+    for z, (skew, beta, phys) in ((0, CASES[1]), (2, CASES[2])):
+        b.set_case(z, cavity_grid(n0, n1, skew=skew, beta=beta), **phys)
+    for _ in range(steps):
+        st = b.step()
+    for z, (skew, beta, phys) in ((0, CASES[1]), (2, CASES[2])):
+        fa, sa = _alone(n0, skew, beta, phys, precision, steps, n1=n1)
+        fb = b.fields(z)
+        for k in ("u", "v", "T", "p"):
+            assert np.array_equal(fa[k], fb[k]), (z, k, np.abs(fa[k] - fb[k]).max())
+        assert st[z].p_iters == sa.p_iters and st[z].mass_l1 == sa.mass_l1
     b.close()
