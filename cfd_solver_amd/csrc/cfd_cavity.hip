@@ -48,8 +48,23 @@ struct CGrid {
     size_t N;          // nodes per slot
     const SlotP *sp;   // sp[blockIdx.z]
     RecipT<double> r;  // reciprocals of the grid constants
+    int scheme;        // convection next to walls: 0 reference, 1 upwinded, 2 upwinded everywhere
+    double blend;      // width of the blended hybrid switch (0: the reference's step)
     __host__ __device__ KGrid kg() const { return KGrid{n0, n1, h1, h2}; }
 };
+
+// Source terms of a coarse FAS level (null on a level without them): u, v, T of
+// the momentum and energy residuals, m of the mass residual; on the border, T
+// holds the side-wall and m the wall-pressure sources.
+struct Src {
+    const double *u, *v, *T, *m;
+};
+__device__ __forceinline__ Src src_at(Src s, size_t o) {
+    if (s.u) {
+        s.u += o; s.v += o; s.T += o; s.m += o;
+    }
+    return s;
+}
 
 struct Case {
     int n0 = 0, n1 = 0;
@@ -105,6 +120,15 @@ struct Case {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool capturing = false;
+    // FAS hierarchy: the next coarser level (same slots, same stream), this level's
+    // residuals at its current state (FP64: u, v, T, and the wall pressure the wall
+    // update would set), and on a coarse level its sources and restricted state
+    Case *coarse = nullptr;
+    bool own_stream = true;
+    double *fr[4] = {nullptr};
+    double *src[4] = {nullptr};
+    double *x0[4] = {nullptr};
+    bool use_src = false;
 };
 
 // ------------------------------------------------------------------------
@@ -130,11 +154,12 @@ template <typename TR>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_residual(
     CGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ T,
     const double *__restrict__ p, GridPtrs gp, TR *__restrict__ Ru, TR *__restrict__ Rv, TR *__restrict__ RT,
-    double *__restrict__ dd, TR *__restrict__ Ua, TR *__restrict__ Va, RedArgs ra) {
+    double *__restrict__ dd, TR *__restrict__ Ua, TR *__restrict__ Va, RedArgs ra, Src src) {
     const SlotP ph = g.sp[blockIdx.z];
     ra = rz(ra);
     const size_t o = zslot() * g.N;
     u += o; v += o; T += o; p += o; Ru += o; Rv += o; RT += o; dd += o; Ua += o; Va += o;
+    src = src_at(src, o);
     constexpr int FX = RT_X + 4, FY = RT_Y + 4, PX = RT_X + 2, PY = RT_Y + 2;
     constexpr int NODES = RT_Y / BY;  // nodes per thread
     __shared__ double s_u[FX * FY], s_v[FX * FY], s_t[FX * FY], s_p[PX * PY];
@@ -184,8 +209,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         auto pat = [&](int di, int dj) { return s_p[(lj + 1 + dj) * PX + (li + 1 + di)]; };
         const bool fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
         const bool hyb = fit_i && fit_j;
-        const int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
-        const int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+        int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
+        int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+        if (g.scheme == 1) {
+            mode_i = fit_i ? 2 : 3;
+            mode_j = fit_j ? 2 : 3;
+        } else if (g.scheme == 2) {
+            mode_i = mode_j = 3;
+        }
         const double al = c.al, be = c.be, ga = c.ga, pp = c.pp, qq = c.qq;
         const double UP = c.xix * at(s_u, 0, 0) + c.xiy * at(s_v, 0, 0);
         const double VP = c.etx * at(s_u, 0, 0) + c.ety * at(s_v, 0, 0);
@@ -201,9 +232,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const double fm2i = fit_i ? at(f, -2, 0) : 0.0, fp2i = fit_i ? at(f, 2, 0) : 0.0;
             const double fm2j = fit_j ? at(f, 0, -2) : 0.0, fp2j = fit_j ? at(f, 0, 2) : 0.0;
             const double dxi = conv_deriv_r(mode_i, fm2i, at(f, -1, 0), at(f, 0, 0), at(f, 1, 0), fp2i, UP, pec1,
-                                            g.r.ih1);
+                                            g.r.ih1, g.blend);
             const double det = conv_deriv_r(mode_j, fm2j, at(f, 0, -1), at(f, 0, 0), at(f, 0, 1), fp2j, VP, pec2,
-                                            g.r.ih2);
+                                            g.r.ih2, g.blend);
             conv[k] = UP * dxi + VP * det;
         }
         const double dp_dxi = (pat(1, 0) - pat(-1, 0)) * g.r.i2h1;
@@ -214,9 +245,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         auto A_ = [&](const double *f, double dn) {
             return aref_offdiag_a([&](int di, int dj) { return at(f, di, dj); }, al, be, ga, pp, qq, g.r, dn, hyb);
         };
-        const double ru = dt * (-conv[0] - dp_dx) - A_(s_u, ph.dnu);
-        const double rv = dt * (-conv[1] - dp_dy + ph.Ri * at(s_t, 0, 0)) - A_(s_v, ph.dnu);
-        const double rt = -dt * conv[2] - A_(s_t, ph.dnt);
+        double ru = dt * (-conv[0] - dp_dx) - A_(s_u, ph.dnu);
+        double rv = dt * (-conv[1] - dp_dy + ph.Ri * at(s_t, 0, 0)) - A_(s_v, ph.dnu);
+        double rt = -dt * conv[2] - A_(s_t, ph.dnt);
+        if (src.u) {  // FAS sources of a coarse level
+            ru += dt * src.u[P];
+            rv += dt * src.v[P];
+            rt += dt * src.T[P];
+        }
         Ru[P] = (TR)ru;
         Rv[P] = (TR)rv;
         RT[P] = (TR)rt;
@@ -446,7 +482,7 @@ __global__ __launch_bounds__(NT) void k_face_div(
     const TR *__restrict__ dv, const double *__restrict__ p, const double *__restrict__ d,
     const double *__restrict__ xix, const double *__restrict__ xiy, const double *__restrict__ etx,
     const double *__restrict__ ety, const double *__restrict__ jac, double *__restrict__ q, RedArgs rmass,
-    RedArgs rq) {
+    RedArgs rq, const double *__restrict__ sm) {
     const bool active = g.sp[blockIdx.z].active != 0.0;
     rmass = rz(rmass);
     rq = rz(rq);
@@ -465,7 +501,8 @@ __global__ __launch_bounds__(NT) void k_face_div(
     };
     const auto lh = batch_load<HX * HY, NT>(tid, [&](int k) {
         const size_t Q = o + idx(clampi(i0 - 1 + k % HX, 0, n0 - 1), clampi(j0 - 1 + k / HX, 0, n1 - 1), n0);
-        return H{u[Q], v[Q], (double)du[Q], (double)dv[Q], d[Q], xix[Q], xiy[Q], etx[Q], ety[Q], jac[Q]};
+        return H{u[Q], v[Q], du ? (double)du[Q] : 0.0, dv ? (double)dv[Q] : 0.0, d[Q], xix[Q], xiy[Q], etx[Q], ety[Q],
+                 jac[Q]};
     });
     const auto lp = batch_load<PX * PY, NT>(tid, [&](int k) {
         return p[o + idx(clampi(i0 - 2 + k % PX, 0, n0 - 1), clampi(j0 - 2 + k / PX, 0, n1 - 1), n0)];
@@ -524,7 +561,8 @@ __global__ __launch_bounds__(NT) void k_face_div(
         fw = i - 1 != 0 ? fw : 0.0;
         gn = j + 1 != n1 - 1 ? gn : 0.0;
         gs = j - 1 != 0 ? gs : 0.0;
-        const double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
+        double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
+        if (sm) qq -= sm[o + idx(i, j, n0)];
         q[idx(i, j, n0)] = qq;
         aq[0] = fabs(qq);
         nq[0] = qq;
@@ -626,11 +664,13 @@ __global__ __launch_bounds__(NT) void k_correct(
 constexpr int WB = 64;  // threads per block of the wall kernels
 __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restrict__ T,
                                                 const double *__restrict__ alph, const double *__restrict__ beta,
-                                                double *__restrict__ scratch, size_t scr) {
+                                                double *__restrict__ scratch, size_t scr,
+                                                const double *__restrict__ sT) {
     if (g.sp[blockIdx.z].active == 0.0) return;
     {
         const size_t o = zslot() * g.N;
         T += o; alph += o; beta += o;
+        if (sT) sT += o;
         scratch += zslot() * scr;
     }
     const int n0 = g.n0, n1 = g.n1;
@@ -644,6 +684,10 @@ __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restric
     te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
     txi = beta[P] / alph[P] * te;
     scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
+    if (sT) {  // FAS boundary sources of a coarse level
+        scratch[j] += sT[idx(0, j, n0)];
+        scratch[n1 + j] += sT[idx(n0 - 1, j, n0)];
+    }
 }
 // wall pressure from the normal momentum equation (oracle: wall_p_value)
 struct WallIn {
@@ -716,9 +760,12 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
 // wall values, computed by a lane pair.  The side walls' new temperatures come from
 // scratch (k_t_sides): each side-wall thread uses its node's value and writes it
 // into T; no thread here reads side-wall T from T.
+// sm (may be null): FAS boundary sources added to every wall value after the
+// corners are formed; pout (may be null): the new values go there instead of into p
+// and T is left alone (the wall residual of a FAS restriction).
 __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__restrict__ p,
                                                 double *__restrict__ T, const double *__restrict__ scratch,
-                                                size_t scr) {
+                                                size_t scr, const double *__restrict__ sm, double *__restrict__ pout) {
     const SlotP &sp = cg.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     const double Re = sp.Re, Ri = sp.Ri;
@@ -726,6 +773,8 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
     w = wall_at(w, zslot() * cg.N);
     p += zslot() * cg.N;
     T += zslot() * cg.N;
+    if (sm) sm += zslot() * cg.N;
+    double *const pw = pout ? pout + zslot() * cg.N : p;
     scratch += zslot() * scr;
     const int side = blockIdx.y;
     const int t = blockIdx.x * WB + threadIdx.x;
@@ -740,8 +789,9 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
         const int i = side == 2 ? 0 : (side == 3 ? n0 - 1 : t);
         const int j = side == 0 ? 0 : (side == 1 ? n1 - 1 : t);
         const double tw = Tw(side, i, j);
-        p[idx(i, j, n0)] = wall_p_value(w, g, Re, Ri, side, i, j, tw);
-        if (side >= 2) T[idx(i, j, n0)] = tw;
+        const size_t P = idx(i, j, n0);
+        pw[P] = wall_p_value(w, g, Re, Ri, side, i, j, tw) + (sm ? sm[P] : 0.0);
+        if (side >= 2 && !pout) T[P] = tw;
         return;
     }
     if (blockIdx.x != 0 || threadIdx.x >= 8) return;
@@ -755,7 +805,8 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
     const int wj = h == 0 ? cj : (north ? n1 - 2 : 1);
     const double v = wall_p_value(w, g, Re, Ri, ws, wi, wj, Tw(ws, wi, wj));
     const double o = __shfl_xor(v, 1, 64);
-    if (h == 0) p[idx(ci, cj, n0)] = 0.5 * (v + o);
+    const size_t C = idx(ci, cj, n0);
+    if (h == 0) pw[C] = 0.5 * (v + o) + (sm ? sm[C] : 0.0);
 }
 
 // velocity and temperature wall values (oracle: velocity_temperature_bcs)
@@ -905,6 +956,138 @@ __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, in
         scal[S_DMAX] = fmax(m, 0.0);
         scal[S_NUH] = -nh / (len_hot * den);
         scal[S_NUC] = nc / (len_cold * den);
+        if (scal[S_FRESH] != 0.0) {  // the case's first iteration: the normalisers of the residual ratios
+            scal[S_MASS0] = mass[0];
+            scal[S_ENERGY0] = res[0];
+            scal[S_FRESH] = 0.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// FAS multigrid outer cycle (oracle: fas_cycle).  Levels are batches of the same
+// slots on the same stream; level l+1 takes every other node of level l.
+
+// coarse node (I, J) <- fine node (2I, 2J) for u, v, T, p, kept in x0 as well
+struct FasInj {
+    int n0, m0, m1;
+    size_t Nf, Nc;
+    const SlotP *sp;
+    const double *f[4];
+    double *c[4], *c0[4];
+};
+__global__ void k_fas_inject(FasInj a) {
+    if (a.sp[blockIdx.z].active == 0.0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.m0 * a.m1) return;
+    const int I = k % a.m0, J = k / a.m0;
+    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = zslot() * a.Nc + (size_t)k;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double v = a.f[q][F];
+        a.c[q][C] = v;
+        a.c0[q][C] = v;
+    }
+}
+
+// Sources of the coarse level.  Interior: s_u = FW(R_f / dt) - R_c / dt (v, T
+// likewise; R_f with the fine level's own sources, R_c of the restricted state
+// without), s_m = q_c - FW(q_f); side walls: s_T = the fine side-wall update's
+// change injected minus the coarse one's.  Every other entry is zeroed (the wall
+// pressure sources follow in k_fas_src_p).
+struct FasSrc {
+    int n0, n1, m0, m1;
+    size_t Nf, Nc, scrf, scrc;
+    const SlotP *sp;
+    const double *fr[3], *fq, *fT, *fscr;
+    const double *cr[3], *cq, *cT, *cscr;
+    double *s[4];
+};
+__global__ void k_fas_src(FasSrc a) {
+    const SlotP &ph = a.sp[blockIdx.z];
+    if (ph.active == 0.0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m0 = a.m0, m1 = a.m1, n0 = a.n0;
+    if (k >= m0 * m1) return;
+    const int I = k % m0, J = k / m0;
+    const size_t zf = zslot() * a.Nf, zc = zslot() * a.Nc, C = zc + (size_t)k;
+    const double idt = 1.0 / ph.dt;
+    double sv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (I >= 1 && I <= m0 - 2 && J >= 1 && J <= m1 - 2) {
+        const int i = 2 * I, j = 2 * J;
+        auto fw = [&](const double *r, double sc) {
+            auto R = [&](int di, int dj) { return r[zf + idx(i + di, j + dj, n0)] * sc; };
+            return (4.0 * R(0, 0) + 2.0 * (R(-1, 0) + R(1, 0) + R(0, -1) + R(0, 1)) +
+                    (R(-1, -1) + R(1, -1) + R(-1, 1) + R(1, 1))) / 16.0;
+        };
+#pragma unroll
+        for (int q = 0; q < 3; q++) sv[q] = fw(a.fr[q], idt) - a.cr[q][C] * idt;
+        sv[3] = a.cq[C] - fw(a.fq, 1.0);
+    } else if ((I == 0 || I == m0 - 1) && J >= 1 && J <= m1 - 2) {
+        const int i = I == 0 ? 0 : a.n0 - 1, j = 2 * J;
+        const double *fs = a.fscr + zslot() * a.scrf, *cs = a.cscr + zslot() * a.scrc;
+        const double wf = (I == 0 ? fs[j] : fs[a.n1 + j]) - a.fT[zf + idx(i, j, n0)];
+        const double wc = (I == 0 ? cs[J] : cs[m1 + J]) - a.cT[C];
+        sv[2] = wf - wc;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) a.s[q][C] = sv[q];
+}
+
+// wall-pressure sources of the coarse level's border: the fine wall update's
+// change injected minus the coarse one's (after the coarse side-wall update with
+// its source)
+struct FasSrcP {
+    int n0, m0, m1;
+    size_t Nf, Nc;
+    const SlotP *sp;
+    const double *fpw, *fp, *cpw, *cp;
+    double *sm;
+};
+__global__ void k_fas_src_p(FasSrcP a) {
+    if (a.sp[blockIdx.z].active == 0.0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m0 = a.m0, m1 = a.m1;
+    const int nb = 2 * m0 + 2 * (m1 - 2);  // border nodes
+    if (k >= nb) return;
+    int I, J;
+    if (k < m0) { I = k; J = 0; }
+    else if (k < 2 * m0) { I = k - m0; J = m1 - 1; }
+    else if (k < 2 * m0 + m1 - 2) { I = 0; J = 1 + k - 2 * m0; }
+    else { I = m0 - 1; J = 1 + k - 2 * m0 - (m1 - 2); }
+    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = zslot() * a.Nc + idx(I, J, m0);
+    a.sm[C] = (a.fpw[F] - a.fp[F]) - (a.cpw[C] - a.cp[C]);
+}
+
+// fine interior += bilinear interpolation of the coarse change x - x0 (u, v, T, p)
+struct FasPro {
+    int n0, n1, m0;
+    size_t Nf, Nc;
+    const SlotP *sp;
+    const double *c[4], *c0[4];
+    double *f[4];
+};
+__global__ void k_fas_prolong(FasPro a) {
+    if (a.sp[blockIdx.z].active == 0.0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = a.n0, n1 = a.n1, m0 = a.m0;
+    if (k >= n0 * n1) return;
+    const int i = k % n0, j = k / n0;
+    if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
+    const size_t zc = zslot() * a.Nc;
+    const int I = i >> 1, J = j >> 1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        auto E = [&](int ii, int jj) {
+            const size_t C = zc + idx(ii, jj, m0);
+            return a.c[q][C] - a.c0[q][C];
+        };
+        double e;
+        if ((i & 1) == 0 && (j & 1) == 0) e = E(I, J);
+        else if ((j & 1) == 0) e = 0.5 * (E(I, J) + E(I + 1, J));
+        else if ((i & 1) == 0) e = 0.5 * (E(I, J) + E(I, J + 1));
+        else e = 0.25 * (E(I, J) + E(I + 1, J) + E(I, J + 1) + E(I + 1, J + 1));
+        a.f[q][zslot() * a.Nf + (size_t)k] += e;
     }
 }
 
@@ -1197,7 +1380,15 @@ PsCtx ctx(Case *c) {
 RedArgs red(const Case *c, int slot = 0) {
     return RedArgs{c->part + (size_t)slot * c->pstride, c->pstride, psz(c), c->scal};
 }
-CGrid cg(const Case *c) { return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd, make_recip(c->h1, c->h2)}; }
+CGrid cg(const Case *c) {
+    return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd, make_recip(c->h1, c->h2), c->prm.wall_upwind,
+                 c->prm.hyb_blend};
+}
+// the level's FAS sources when it uses them
+Src srcs(const Case *c) {
+    if (!c->use_src) return Src{nullptr, nullptr, nullptr, nullptr};
+    return Src{c->src[0], c->src[1], c->src[2], c->src[3]};
+}
 dim3 grid_b(const Case *c) {
     dim3 g = grid2d(c->n0, c->n1);
     g.z = c->B;
@@ -1368,7 +1559,7 @@ int launch_residual_t(Case *c) {
     const dim3 gr = grid_res(c->n0, c->n1, c->B);
     launch_timed(prof_timed(c, 2), k_residual<TR>, gr, dim3(BX, BY), c->s, cg(c), (const double *)c->u,
                  (const double *)c->v, (const double *)c->T, (const double *)c->p, gptrs(c), (TR *)c->Ru,
-                 (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c, PS_RES));
+                 (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c, PS_RES), srcs(c));
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1432,6 +1623,8 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
 int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     const int m = std::max(c->n0, c->n1);
     const size_t o = (size_t)z0 * c->N;
+    const Src sr = srcs(c);
+    const double *sT = sr.T ? sr.T + o : nullptr, *sm = sr.m ? sr.m + o : nullptr;
     CGrid g = cg(c);
     g.sp += z0;
     WallIn w = wall_in(c);
@@ -1440,9 +1633,9 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
     const dim3 g1((m + 255) / 256, 1, nz);
     if (uvt) k_uvt_walls<<<g1, 256, 0, c->s>>>(g, c->u + o, c->v + o, c->T + o);
     const dim3 gs((c->n1 + WB - 1) / WB, 1, nz);
-    k_t_sides<<<gs, WB, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr, c->scr);
+    k_t_sides<<<gs, WB, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr, c->scr, sT);
     k_p_walls<<<dim3((m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o, c->T + o, c->scratch + z0 * c->scr,
-                                                               c->scr);
+                                                               c->scr, sm, nullptr);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -1452,22 +1645,25 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt) {
 // predictor's final corrections
 int launch_face_div(Case *c, const void *du, const void *dv) {
     const Timed tm = prof_timed(c, 3);
+    const double *sm = srcs(c).m;
     if (c->mixed)
         launch_timed(tm, k_face_div<float>, grid_in(c), dim3(BX, BY), c->s, cg(c), c->u, c->v, (const float *)du,
                      (const float *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac, c->q,
-                     red(c, PS_MASS), red(c));
+                     red(c, PS_MASS), red(c), sm);
     else
         launch_timed(tm, k_face_div<double>, grid_in(c), dim3(BX, BY), c->s, cg(c), c->u, c->v,
                      (const double *)du, (const double *)dv, c->p, c->d, c->xix, c->xiy, c->etx, c->ety, c->ajac,
-                     c->q, red(c, PS_MASS), red(c));
+                     c->q, red(c, PS_MASS), red(c), sm);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 
-// Everything of one outer iteration that runs on the stream.  graph: capturable
+// Everything of one SIMPLE iteration that runs on the stream.  graph: capturable
 // (the p' solve is controlled on the device); otherwise the p' solve reads its
-// convergence flags on the host between half-iterations.
-int outer_body(Case *c, bool graph) {
+// convergence flags on the host between half-iterations.  stats: finish the
+// iteration's statistics (the top level of a FAS hierarchy; a slot's first
+// statistics also become its residual normalisers).
+int simple_body(Case *c, bool graph, bool stats = true) {
     const dim3 bl(BX, BY);
     // 1-2 residuals and d (FP64 arithmetic; stored in the predictor precision)
     RET(launch_residual(c));
@@ -1492,13 +1688,113 @@ int outer_body(Case *c, bool graph) {
     CFD_LAUNCH_CHECK();
     RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax (partials from the correction) and the wall Nusselt numbers (FP64)
-    {
+    if (stats) {
         const dim3 grr = grid_res(c->n0, c->n1, 1);
         k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk_in(c->n0, c->n1), (int)(grr.x * grr.y),
                                                              c->T, c->ajac, c->etx, c->ety);
+        CFD_LAUNCH_CHECK();
     }
+    return CFD_OK;
+}
+
+// ---- FAS multigrid outer cycle (oracle: fas_cycle)
+
+unsigned nblk256(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// R/dt's numerators (FP64) of u, v, T at the current state into fr[0..2], d, and the
+// mass residual q of the state's own face velocities (no predictor correction),
+// with the level's sources when with_src
+int state_residual(Case *c, bool with_src) {
+    const bool keep = c->use_src;
+    c->use_src = with_src && keep;
+    const dim3 gr = grid_res(c->n0, c->n1, c->B);
+    k_residual<double><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0], c->fr[1],
+                                                     c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES), srcs(c));
+    CFD_LAUNCH_CHECK();
+    k_face_div<double><<<grid_in(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const double *)nullptr,
+                                                             (const double *)nullptr, c->p, c->d, c->xix, c->xiy,
+                                                             c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS), red(c),
+                                                             srcs(c).m);
+    CFD_LAUNCH_CHECK();
+    c->use_src = keep;
+    return CFD_OK;
+}
+
+// the side-wall temperature update into scratch (with_src: the level's sources)
+int side_walls(Case *c, bool with_src) {
+    const dim3 gs((c->n1 + WB - 1) / WB, 1, c->B);
+    k_t_sides<<<gs, WB, 0, c->s>>>(cg(c), c->T, c->alph, c->beta, c->scratch, c->scr,
+                                   with_src ? srcs(c).T : nullptr);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
+}
+// the wall pressure the wall update would set (after the side-wall update in
+// scratch) into fr[3]
+int wall_p_out(Case *c, bool with_src) {
+    const int m = std::max(c->n0, c->n1);
+    k_p_walls<<<dim3((m + WB - 1) / WB, 5, c->B), WB, 0, c->s>>>(cg(c), wall_in(c), c->p, c->T, c->scratch, c->scr,
+                                                                 with_src ? srcs(c).m : nullptr, c->fr[3]);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+int fas_cycle(Case *c, bool graph, bool top) {
+    const cfd_params &pr = c->prm;
+    Case *cc = c->coarse;
+    // coarse levels control their p' solves on the device (no host round trips)
+    const bool g = graph || !top;
+    if (!cc) {
+        for (int k = 0; k < pr.fas_coarse; k++) RET(simple_body(c, g, top));
+        return CFD_OK;
+    }
+    for (int k = 0; k < pr.fas_pre; k++) RET(simple_body(c, g, top));
+    // this level's residuals at its state, with its own sources
+    RET(state_residual(c, true));
+    RET(side_walls(c, true));
+    RET(wall_p_out(c, true));
+    // restricted state and the coarse residuals of it without sources
+    const int m0 = cc->n0, m1 = cc->n1;
+    const size_t Mc = (size_t)m0 * m1;
+    {
+        FasInj a{c->n0, m0, m1, c->N, cc->N, c->spd, {c->u, c->v, c->T, c->p}, {cc->u, cc->v, cc->T, cc->p},
+                 {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}};
+        k_fas_inject<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    RET(state_residual(cc, false));
+    RET(side_walls(cc, false));
+    {
+        FasSrc a{c->n0, c->n1, m0, m1, c->N, cc->N, c->scr, cc->scr, c->spd,
+                 {c->fr[0], c->fr[1], c->fr[2]}, c->q, c->T, c->scratch,
+                 {cc->fr[0], cc->fr[1], cc->fr[2]}, cc->q, cc->T, cc->scratch,
+                 {cc->src[0], cc->src[1], cc->src[2], cc->src[3]}};
+        k_fas_src<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    cc->use_src = true;
+    RET(side_walls(cc, true));
+    RET(wall_p_out(cc, false));
+    {
+        FasSrcP a{c->n0, m0, m1, c->N, cc->N, c->spd, c->fr[3], c->p, cc->fr[3], cc->p, cc->src[3]};
+        k_fas_src_p<<<dim3(nblk256((size_t)2 * (m0 + m1)), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    RET(fas_cycle(cc, graph, false));
+    {
+        FasPro a{c->n0, c->n1, m0, c->N, cc->N, c->spd, {cc->u, cc->v, cc->T, cc->p},
+                 {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}, {c->u, c->v, c->T, c->p}};
+        k_fas_prolong<<<dim3(nblk256(c->N), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    RET(wall_bcs(c, 0, c->B, true));
+    for (int k = 0; k < pr.fas_post; k++) RET(simple_body(c, g, top));
+    return CFD_OK;
+}
+
+// one outer iteration: a FAS cycle when the case has coarse levels, else one SIMPLE iteration
+int cycle_body(Case *c, bool graph) {
+    if (c->coarse) return fas_cycle(c, graph, true);
+    return simple_body(c, graph, true);
 }
 
 void drop_graph(Case *c) {
@@ -1517,7 +1813,7 @@ int capture(Case *c) {
     c->ev_kind.clear();
     CFD_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    const int rc = outer_body(c, true);
+    const int rc = cycle_body(c, true);
     c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->s, &g);
@@ -1535,14 +1831,14 @@ int capture(Case *c) {
 
 // launch one outer iteration of every active slot (no synchronisation)
 int outer_launch(Case *c) {
-    RET(tune_mg(c));  // once, eagerly, before any capture
+    for (Case *l = c; l; l = l->coarse) RET(tune_mg(l));  // once, eagerly, before any capture
     // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
     if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
         CFD_HIP(hipGraphLaunch(c->gexec, c->s));
         return CFD_OK;
     }
-    return outer_body(c, false);
+    return cycle_body(c, false);
 }
 
 // after an outer iteration: per-slot statistics from the scalars (host mirror current)
@@ -1556,8 +1852,8 @@ int outer_stats(Case *c, cfd_stats *st) {
         }
         c->iter[z]++;
         if (c->iter[z] == 1) {
-            c->mass0[z] = h[S_MASS];
-            c->energy0[z] = h[S_ENERGY];
+            c->mass0[z] = h[S_MASS0];
+            c->energy0[z] = h[S_ENERGY0];
         }
         if (st) {
             cfd_stats &s = st[z];
@@ -1612,11 +1908,21 @@ int time_solver_kernel(Case *c, PSolver<T> *ps, int kernel) {
 }
 
 // device-side slot state: SlotP and the S_ACTIVE scalar
-int put_slot(Case *c, int z) {
-    CFD_HIP(hipMemcpyAsync(c->spd + z, &c->sph[z], sizeof(SlotP), hipMemcpyHostToDevice, c->s));
-    c->hscal[(size_t)z * SS + S_ACTIVE] = c->sph[z].active;
-    CFD_HIP(hipMemcpyAsync(c->scal + (size_t)z * SS + S_ACTIVE, c->hscal + (size_t)z * SS + S_ACTIVE, sizeof(double),
-                           hipMemcpyHostToDevice, c->s));
+// fresh: the slot's next statistics become its residual normalisers.  Every level
+// of a FAS hierarchy holds the slot alike.
+int put_slot(Case *c, int z, bool fresh = false) {
+    for (Case *l = c; l; l = l->coarse) {
+        if (l != c) l->sph[z].active = c->sph[z].active;
+        CFD_HIP(hipMemcpyAsync(l->spd + z, &l->sph[z], sizeof(SlotP), hipMemcpyHostToDevice, l->s));
+        double *h = l->hscal + (size_t)z * SS;
+        h[S_ACTIVE] = l->sph[z].active;
+        h[S_FRESH] = fresh ? 1.0 : 0.0;
+        CFD_HIP(hipMemcpyAsync(l->scal + (size_t)z * SS + S_ACTIVE, h + S_ACTIVE, sizeof(double),
+                               hipMemcpyHostToDevice, l->s));
+        if (fresh)
+            CFD_HIP(hipMemcpyAsync(l->scal + (size_t)z * SS + S_FRESH, h + S_FRESH, sizeof(double),
+                                   hipMemcpyHostToDevice, l->s));
+    }
     CFD_HIP(hipStreamSynchronize(c->s));
     return CFD_OK;
 }
@@ -1624,21 +1930,29 @@ int put_slot(Case *c, int z) {
 void batch_destroy(Case *c) {
     if (!c) return;
     if (c->s) (void)hipStreamSynchronize(c->s);
+    batch_destroy(c->coarse);
     drop_graph(c);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hflag) (void)hipHostFree(c->hflag);
-    if (c->s) (void)hipStreamDestroy(c->s);
+    if (c->s && c->own_stream) (void)hipStreamDestroy(c->s);
     delete c->ps64;
     delete c->ps32;
     delete c;
     (void)hipGetLastError();
 }
 
-// dry: only count the bytes the batch would hold (c->bytes), touching no device
+// the coarse level of a FAS hierarchy below an n0 x n1 level, if there is one
+bool fas_coarsens(int n0, int n1, const cfd_params &p) {
+    const int lo = p.fas_min_n < 5 ? 5 : p.fas_min_n;
+    return p.fas_levels > 1 && (n0 & 1) && (n1 & 1) && (n0 + 1) / 2 >= lo && (n1 + 1) / 2 >= lo;
+}
+
+// dry: only count the bytes the batch would hold (c->bytes), touching no device.
+// coarse_level: a coarse level of a FAS hierarchy, on the finer level's stream share.
 int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_params *params, Case **out,
-                 bool dry = false) {
+                 bool dry = false, hipStream_t share = nullptr, bool coarse_level = false) {
     if (!out || !params) return CFD_E_ARG;
     if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0) || slots < 1 || slots > 65535) return CFD_E_ARG;
     if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
@@ -1652,7 +1966,14 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     c->energy0.assign(slots, 0.0);
     c->iter.assign(slots, 0);
     if (!dry && hipGetDevice(&c->device) != hipSuccess) { delete c; return CFD_E_HIP; }
-    if (!dry && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) { delete c; return CFD_E_HIP; }
+    if (share) {
+        c->s = share;
+        c->own_stream = false;
+    } else if (!dry && hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CFD_E_HIP;
+    }
+    const bool is_coarse = coarse_level;
     const size_t BN = (size_t)slots * c->N;
     int rc = CFD_OK;
     double **fields[] = {&c->xg, &c->yg, &c->xix, &c->xiy, &c->etx, &c->ety, &c->alph, &c->beta, &c->gamma,
@@ -1675,12 +1996,30 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     }
     c->scr = (size_t)2 * (n0 + n1) + 8;
     if (rc == CFD_OK) rc = dalloc(c, &c->scratch, (size_t)slots * c->scr);
+    // FAS: residual arrays on every level of a hierarchy, sources and the
+    // restricted state on the coarse ones
+    const bool coarsens = fas_coarsens(n0, n1, *params);
+    if (coarsens || is_coarse)
+        for (int k = 0; k < 4 && rc == CFD_OK; k++) rc = dalloc(c, &c->fr[k], BN);
+    if (is_coarse)
+        for (int k = 0; k < 4 && rc == CFD_OK; k++) {
+            rc = dalloc(c, &c->src[k], BN);
+            if (rc == CFD_OK) rc = dalloc(c, &c->x0[k], BN);
+        }
     c->pstride = (nblk(n0, n1) + 255) / 256 * 256;
     if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)slots * psz(c));
     if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)slots * SS);
     if (rc == CFD_OK) rc = dalloc(c, &c->spd, (size_t)slots);
+    cfd_params cp = *params;
+    cp.fas_levels = params->fas_levels - 1;
+    if (cp.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = cp.fas_coarse_scheme;
     if (dry) {
         if (rc == CFD_OK) rc = build_levels(c);
+        if (rc == CFD_OK && coarsens) {
+            rc = batch_create((n0 + 1) / 2, (n1 + 1) / 2, 2 * dxi, 2 * deta, slots, &cp, &c->coarse, true, nullptr,
+                              true);
+            if (c->coarse) c->bytes += c->coarse->bytes;
+        }
         *out = c;
         return rc;
     }
@@ -1694,12 +2033,54 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
         rc = CFD_E_OOM;
     if (rc == CFD_OK) std::memset(c->hflag, 0, (size_t)slots * sizeof(unsigned long long));
     if (rc == CFD_OK) rc = build_levels(c);
+    if (rc == CFD_OK && coarsens) {
+        rc = batch_create((n0 + 1) / 2, (n1 + 1) / 2, 2 * dxi, 2 * deta, slots, &cp, &c->coarse, false, c->s, true);
+        if (c->coarse) c->bytes += c->coarse->bytes;
+    }
     if (rc == CFD_OK && hipStreamSynchronize(c->s) != hipSuccess) rc = CFD_E_HIP;
     if (rc != CFD_OK) {
         batch_destroy(c);
         return rc;
     }
     *out = c;
+    return CFD_OK;
+}
+
+// free a dry-run hierarchy
+void dry_free(Case *c) {
+    if (!c) return;
+    dry_free(c->coarse);
+    delete c->ps64;
+    delete c->ps32;
+    delete c;
+}
+
+// slot z of the coarse levels below c: grid arrays injected from the finer level,
+// the FP32 copies of the mixed path, the multigrid geometry, the slot's physics
+int set_coarse_slots(Case *c, int z) {
+    for (Case *f = c, *l = c->coarse; l; f = l, l = l->coarse) {
+        const size_t of = (size_t)z * f->N, ol = (size_t)z * l->N;
+        const double *src[12] = {f->xg, f->yg, f->xix, f->xiy, f->etx, f->ety, f->alph, f->beta, f->gamma,
+                                 f->ajac, f->p1, f->q1};
+        double *dst[12] = {l->xg, l->yg, l->xix, l->xiy, l->etx, l->ety, l->alph, l->beta, l->gamma, l->ajac,
+                           l->p1, l->q1};
+        const dim3 gr = grid2d(l->n0, l->n1);
+        for (int k = 0; k < 12; k++) k_inject<<<gr, dim3(BX, BY), 0, l->s>>>(f->n0, f->n1, src[k] + of, l->n0, l->n1,
+                                                                          dst[k] + ol);
+        CFD_LAUNCH_CHECK();
+        double *state[] = {l->u, l->v, l->T, l->p, l->pc};
+        for (double *a : state) CFD_HIP(hipMemsetAsync(a + ol, 0, l->N * sizeof(double), l->s));
+        if (l->mixed) {
+            const double *m[5] = {l->alph, l->beta, l->gamma, l->p1, l->q1};
+            const int lb = 256, lg = (int)((l->N + lb - 1) / lb);
+            for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, l->s>>>(l->N, m[k] + ol, l->a32[k] + ol);
+            CFD_HIP(hipMemsetAsync(l->ef + ol, 0, l->N * sizeof(float), l->s));
+            CFD_LAUNCH_CHECK();
+        }
+        RET(inject_slot(l, z));
+        l->sph[z] = f->sph[z];
+        l->iter[z] = 0;
+    }
     return CFD_OK;
 }
 
@@ -1748,8 +2129,9 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
         sp.len_hot += std::hypot(x[b] - x[a], y[b] - y[a]);
         sp.len_cold += std::hypot(x[b1] - x[a1], y[b1] - y[a1]);
     }
+    RET(set_coarse_slots(c, z));
     sp.active = 1.0;
-    RET(put_slot(c, z));
+    RET(put_slot(c, z, true));
     c->iter[z] = 0;
     c->mass0[z] = c->energy0[z] = 0.0;
     RET(wall_bcs(c, z, 1, true));
@@ -1769,13 +2151,16 @@ extern "C" {
 void cfd_params_default(cfd_params *p) {
     if (!p) return;
     p->Re = 100.0; p->Ri = 1.0; p->Pr = 0.71; p->dt = 1.0e4;
-    p->alpha_u = 0.7; p->alpha_p = 0.3; p->alpha_t = 1.0;
+    p->alpha_u = 0.7; p->alpha_p = 0.3; p->alpha_t = 0.85;
     p->lid_u = 1.0; p->t_hot = 1.0; p->t_cold = 0.0;
     p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 4;
     p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 10; p->mg_omega = 0.8;
     p->precision = CFD_FP64;
     p->graph = 0;
     p->mg_tune = 1;
+    p->wall_upwind = 1; p->hyb_blend = 1.0;
+    p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 17;
+    p->fas_coarse_scheme = 2;
 }
 
 // ---- batches
@@ -1859,7 +2244,7 @@ int cfd_batch_set_fields(cfd_batch h, int slot, const double *u, const double *v
     CFD_HIP(hipStreamSynchronize(c->s));
     c->iter[slot] = 0;
     c->mass0[slot] = c->energy0[slot] = 0.0;
-    return CFD_OK;
+    return put_slot(c, slot, true);
 }
 
 int cfd_batch_bytes(cfd_batch h, size_t *bytes) {
@@ -1876,9 +2261,7 @@ int cfd_batch_bytes_needed(int n0, int n1, double dxi, double deta, int slots, c
     const int rc = batch_create(n0, n1, dxi, deta, slots, params, &c, true);
     if (c) {
         *bytes = c->bytes;
-        delete c->ps64;
-        delete c->ps32;
-        delete c;
+        dry_free(c);
     }
     return rc;
 }

@@ -6,7 +6,8 @@
 // host (Recip, and dt/nd per slot).  The results differ from the oracle's in the
 // last bits only; a division by a per-node value stays a division.
 //  * conv_deriv   -- hybrid 4th-order central / Kawamura-Kuwahara upwind switch
-//                    on the cell Peclet number (solver.cpp:1120-1173);
+//                    on the cell Peclet number (solver.cpp:1120-1173), with the
+//                    cavity mode's upwinding next to walls and continuous blend;
 //  * aref_offdiag -- A_ref phi - phi of the implicit diffusion operator
 //                    I - dt/Re L (coefficients of solver.cpp:764-847);
 //  * alo_coefs    -- low-order predictor operator (the solver.cpp:764-786
@@ -44,17 +45,27 @@ inline RecipT<double> make_recip(double h1, double h2) {
 }
 
 // conv_deriv with the reciprocal ih = 1/h; the upwind branch divides by vel
-// only symbolically: (vel D4 + |vel| D4') / vel = D4 + sign(vel) D4'
+// only symbolically: (vel D4 + |vel| D4') / vel = D4 + sign(vel) D4'.
+// mode 3 (cavity rows next to a wall): second-order central, first-order upwind
+// when |Pe| > 2.  blend > 0 replaces the switch at |Pe| = 2 by a linear blend from
+// the central to the upwind form over 2 < |Pe| < 2 + blend (cavity mode).
 __device__ __forceinline__ double conv_deriv_r(int mode, double fm2, double fm1, double f0, double fp1,
-                                               double fp2, double vel, double pec, double ih) {
-    if (mode == 0) return 0.5 * (fp1 - fm1) * ih;
-    if (mode == 1 || (pec <= 2 && pec > -2)) {
-        const double xpp = 8.0 * (fp1 - fm1);
-        const double xnn = fp2 - fm2;
-        return (1.0 / 12.0) * (xpp - xnn) * ih;
+                                               double fp2, double vel, double pec, double ih, double blend) {
+    const bool central = pec <= 2 && pec > -2;
+    const double w = blend > 0.0 ? fmin(1.0, (fabs(pec) - 2.0) / blend) : 1.0;
+    const double sg = vel > 0.0 ? 1.0 : -1.0;
+    if (mode == 3) {
+        const double c2 = 0.5 * (fp1 - fm1) * ih;
+        if (central) return c2;
+        if (blend > 0.0) return c2 + w * sg * (2.0 * f0 - fm1 - fp1) * (0.5 * ih);
+        return vel > 0.0 ? (f0 - fm1) * ih : (fp1 - f0) * ih;
     }
-    const double d4 = (-fp2 + 8 * fp1 - 8 * fm1 + fm2) * ((1.0 / 12.0) * ih);
+    if (mode == 0) return 0.5 * (fp1 - fm1) * ih;
+    const double c4 = (1.0 / 12.0) * (8.0 * (fp1 - fm1) - (fp2 - fm2)) * ih;
+    if (mode == 1 || central) return c4;
     const double d4p = (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) * (0.25 * ih);
+    if (blend > 0.0) return c4 + w * sg * d4p;
+    const double d4 = (-fp2 + 8 * fp1 - 8 * fm1 + fm2) * ((1.0 / 12.0) * ih);
     return vel > 0.0 ? d4 + d4p : d4 - d4p;
 }
 

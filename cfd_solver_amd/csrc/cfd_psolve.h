@@ -58,6 +58,8 @@ enum {
     S_BB64, S_RR64, S_TOL64, S_RN64, S_BN64, S_RDONE, S_ITTOT, S_RELRES64,
     S_PSUM,   // interior sum of p' (its mean is removed by the correction)
     S_ACTIVE, // slot holds a case (set by the host)
+    S_MASS0, S_ENERGY0,  // residual norms of the case's first SIMPLE iteration
+    S_FRESH,  // 1 until the statistics of the case's first SIMPLE iteration are finished (set by the host)
     S_NSCAL
 };
 constexpr int SS = 64;  // device scalars per slot

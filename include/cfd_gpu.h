@@ -283,6 +283,26 @@ typedef struct {
                            or 1024 threads per tile) is timed once, before the first
                            solve, and the fastest kept (results do not depend on
                            it); 0: the launch-size rule                             */
+    int wall_upwind;    /* convection in the rows next to a wall: 0 the reference's
+                           central differences at any Peclet number (solver.cpp:
+                           1120-1175 rows j = 1, n1-2); 1 (default) the hybrid switch
+                           there too (Kawamura-Kuwahara where its five points fit,
+                           first-order upwind where they do not); 2 the switch to
+                           first-order upwind everywhere                            */
+    double hyb_blend;   /* 0: the reference's switch at |Pe| = 2; > 0 (default 1):
+                           a linear blend from central to upwind over
+                           2 < |Pe| < 2 + hyb_blend, so the discrete equations are
+                           continuous in the state and have an exact fixed point   */
+    int fas_levels;     /* grid levels of the nonlinear multigrid (FAS) outer cycle
+                           with SIMPLE iterations as its smoother (default 8, as
+                           many as the grid allows); 1: plain SIMPLE iterations.
+                           An outer iteration (cfd_case_outer) is one cycle        */
+    int fas_pre, fas_post; /* SIMPLE iterations per level before / after the
+                           coarse-grid correction (default 2, 2)                  */
+    int fas_coarse;     /* SIMPLE iterations on the coarsest level (default 40)   */
+    int fas_min_n;      /* no level coarser than this many nodes per side (17)   */
+    int fas_coarse_scheme; /* wall_upwind value of the coarse levels (default 2;
+                           the more upwinded of it and wall_upwind is used)        */
 } cfd_params;
 
 typedef struct {

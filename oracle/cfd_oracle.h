@@ -162,6 +162,18 @@ typedef struct {
     int mg_levels;      /* multigrid levels (1 = Jacobi preconditioner only)      */
     int mg_pre, mg_post, mg_coarse; /* smoothing sweeps                            */
     double mg_omega;    /* Jacobi damping in the multigrid smoother               */
+    int wall_upwind;    /* 1: rows next to a wall use the hybrid switch too (KK where
+                           the five-point stencil fits, first-order upwind where it
+                           does not, when |Pe| > 2); 0: central there at any Pe
+                           (the reference's rows j = 1, n1-2)                     */
+    int fas_levels;     /* grid levels of the nonlinear (FAS) multigrid outer cycle;
+                           1 = plain SIMPLE iterations                            */
+    int fas_pre, fas_post, fas_coarse; /* SIMPLE iterations per level and cycle   */
+    int fas_min_n;      /* no FAS level coarser than this many nodes per side     */
+    int fas_coarse_scheme; /* convection scheme of the coarse FAS levels (wall_upwind
+                           values; the finer of it and wall_upwind is used)         */
+    double hyb_blend;   /* 0: the reference's switch at |Pe| = 2; > 0: blend from the
+                           central to the upwind form over 2 < |Pe| < 2 + hyb_blend  */
 } or_cav_params;
 
 typedef struct {
@@ -172,6 +184,8 @@ typedef struct {
     double nu_hot, nu_cold;
     int p_iters;
     double p_relres;
+    double mass_res;    /* mass_l1 / mass_l1 of the level's first SIMPLE iteration */
+    double energy_res;  /* energy_l1 likewise                                      */
 } or_cav_stats;
 
 typedef struct or_cav_case or_cav_case;
@@ -189,6 +203,15 @@ void or_cav_set_fields(or_cav_case *c, const double *u, const double *v,
 void or_cav_get_fields(const or_cav_case *c, double *u, double *v, double *T, double *p);
 void or_cav_apply_bcs(or_cav_case *c);
 int or_cav_outer(or_cav_case *c, or_cav_stats *st);
+/* One outer iteration: a FAS V-cycle (fas_pre SIMPLE iterations, the
+ * coarse-grid correction, fas_post SIMPLE iterations; fas_coarse on the
+ * coarsest level) when the case has coarse levels, else one SIMPLE iteration;
+ * *st gets the last fine SIMPLE iteration's statistics. */
+int or_cav_fas_cycle(or_cav_case *c, or_cav_stats *st);
+int or_cav_fas_depth(const or_cav_case *c);
+/* Residuals of the steady equations at the current state, sources included:
+ * R/dt of u, v, T and the mass residual q - s_m (interior; 0 on walls). */
+void or_cav_state_residual(or_cav_case *c, double *ru, double *rv, double *rT, double *rm);
 
 /* stage-level entry points of the cavity path (tests compare them one by one) */
 void or_cav_lo_diag(const or_cav_case *c, double *d_out);

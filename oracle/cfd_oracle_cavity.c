@@ -58,6 +58,11 @@ struct or_cav_case {
     double mass0, energy0;
     int iter;
     double wall_len_hot, wall_len_cold;
+    /* FAS: source terms of this level's equations (NULL on a level without them),
+     * the next coarser level, and (on a coarse level) the state restricted to it */
+    double *su, *sv, *sT, *sm;
+    or_cav_case *coarse;
+    double *u0, *v0, *T0, *p0;
 };
 
 static double *dup_arr(const double *a, size_t n) {
@@ -388,6 +393,43 @@ static void inject_d(or_cav_case *c)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* FAS hierarchy: level l+1 takes every other node of level l (grids of      */
+/* 2^k + 1 nodes coarsen vertex-wise), its grid and metric arrays injected,  */
+/* h doubled; it carries source arrays for the FAS right-hand sides and a    */
+/* copy of the state restricted to it.                                       */
+static double *inject_new(const double *f, int n0, int m0, int m1)
+{
+    double *r = (double *)malloc((size_t)m0 * m1 * sizeof(double));
+    for (int J = 0; J < m1; J++)
+        for (int I = 0; I < m0; I++) r[(size_t)J * m0 + I] = f[(size_t)(2 * J) * n0 + 2 * I];
+    return r;
+}
+
+static void fas_build(or_cav_case *c, const double *x, const double *y, const double *xix,
+                      const double *xiy, const double *etx, const double *ety, const double *alph,
+                      const double *beta, const double *gamma, const double *ajac, const double *p1,
+                      const double *q1)
+{
+    const int n0 = c->n0, n1 = c->n1;
+    const int m0 = (n0 + 1) / 2, m1 = (n1 + 1) / 2;
+    const int lo = c->prm.fas_min_n < 5 ? 5 : c->prm.fas_min_n;
+    if (c->prm.fas_levels <= 1 || (n0 & 1) == 0 || (n1 & 1) == 0 || m0 < lo || m1 < lo) return;
+    or_cav_params cp = c->prm;
+    cp.fas_levels = c->prm.fas_levels - 1;
+    if (c->prm.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = c->prm.fas_coarse_scheme;
+    const double *src[12] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1};
+    double *inj[12];
+    for (int k = 0; k < 12; k++) inj[k] = inject_new(src[k], n0, m0, m1);
+    or_cav_case *cc = or_cav_create(m0, m1, 2.0 * c->h1, 2.0 * c->h2, inj[0], inj[1], inj[2], inj[3],
+                                    inj[4], inj[5], inj[6], inj[7], inj[8], inj[9], inj[10], inj[11], &cp);
+    for (int k = 0; k < 12; k++) free(inj[k]);
+    const size_t M = (size_t)m0 * m1;
+    cc->su = zalloc(M); cc->sv = zalloc(M); cc->sT = zalloc(M); cc->sm = zalloc(M);
+    cc->u0 = zalloc(M); cc->v0 = zalloc(M); cc->T0 = zalloc(M); cc->p0 = zalloc(M);
+    c->coarse = cc;
+}
+
 or_cav_case *or_cav_create(int n0, int n1, double h1, double h2,
                            const double *x, const double *y,
                            const double *xix, const double *xiy,
@@ -415,6 +457,7 @@ or_cav_case *or_cav_create(int n0, int n1, double h1, double h2,
         c->wall_len_hot += hypot(x[b] - x[a], y[b] - y[a]);
         c->wall_len_cold += hypot(x[b1] - x[a1], y[b1] - y[a1]);
     }
+    fas_build(c, x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1);
     return c;
 }
 
@@ -426,6 +469,11 @@ void or_cav_destroy(or_cav_case *c)
                      c->RT, c->du, c->dv, c->dT, c->tu, c->tv, c->tT, c->d, c->us, c->vs, c->q, c->pc};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(all[k]);
     for (int k = 0; k < 8; k++) free(c->bi[k]);
+    or_cav_destroy(c->coarse);
+    {
+        double *fas[] = {c->su, c->sv, c->sT, c->sm, c->u0, c->v0, c->T0, c->p0};
+        for (size_t k = 0; k < sizeof(fas) / sizeof(fas[0]); k++) free(fas[k]);
+    }
     for (int l = 0; l < c->nlev; l++) {
         free(c->lev[l].x); free(c->lev[l].b); free(c->lev[l].r);
         if (l > 0) {
@@ -498,6 +546,10 @@ static void temperature_sides(or_cav_case *c)
     for (int j = 1; j < n1 - 1; j++) {
         T[ID(0, j)] = tl[j];
         T[ID(n0 - 1, j)] = tr[j];
+        if (c->sT) { /* FAS: boundary sources of a coarse level */
+            T[ID(0, j)] += c->sT[ID(0, j)];
+            T[ID(n0 - 1, j)] += c->sT[ID(n0 - 1, j)];
+        }
     }
 }
 
@@ -584,6 +636,16 @@ static void pressure_walls(or_cav_case *c)
     c->p[ID(n0 - 1, 0)] = 0.5 * (c->p[ID(n0 - 2, 0)] + c->p[ID(n0 - 1, 1)]);
     c->p[ID(0, n1 - 1)] = 0.5 * (c->p[ID(1, n1 - 1)] + c->p[ID(0, n1 - 2)]);
     c->p[ID(n0 - 1, n1 - 1)] = 0.5 * (c->p[ID(n0 - 2, n1 - 1)] + c->p[ID(n0 - 1, n1 - 2)]);
+    if (c->sm) { /* FAS: boundary sources of a coarse level (border entries of sm) */
+        for (int i = 0; i < n0; i++) {
+            c->p[ID(i, 0)] += c->sm[ID(i, 0)];
+            c->p[ID(i, n1 - 1)] += c->sm[ID(i, n1 - 1)];
+        }
+        for (int j = 1; j < n1 - 1; j++) {
+            c->p[ID(0, j)] += c->sm[ID(0, j)];
+            c->p[ID(n0 - 1, j)] += c->sm[ID(n0 - 1, j)];
+        }
+    }
 }
 
 void or_cav_apply_bcs(or_cav_case *c)
@@ -596,16 +658,28 @@ void or_cav_apply_bcs(or_cav_case *c)
 /* ------------------------------------------------------------------------ */
 /* Derivative along one grid direction at node P of the hybrid scheme.
  * mode 2: hybrid (KK upwind when |Pe| > 2), 1: 4th-order central,
- * 0: 2nd-order central.  Same arithmetic as solver.cpp:1124-1172. */
+ * 0: 2nd-order central.  Same arithmetic as solver.cpp:1124-1172.
+ * mode 3 (cavity rows next to a wall with wall_upwind): the switch of
+ * solver.cpp:1124 with first-order upwinding in place of KK where the
+ * five-point stencil does not fit. */
 static inline double conv_deriv(int mode, double fm2, double fm1, double f0, double fp1, double fp2,
-                                 double vel, double pec, double h)
+                                 double vel, double pec, double h, double blend)
 {
-    if (mode == 0) return 0.5 * (fp1 - fm1) / h;
-    if (mode == 1 || (pec <= 2 && pec > -2)) {
-        const double xpp = 8.0 * (fp1 - fm1);
-        const double xnn = fp2 - fm2;
-        return (1.0 / 12.0) * (xpp - xnn) / h;
+    const int central = (pec <= 2 && pec > -2);
+    const double w = blend > 0.0 ? fmin(1.0, (fabs(pec) - 2.0) / blend) : 1.0;
+    const double sg = vel > 0.0 ? 1.0 : -1.0;
+    if (mode == 3) { /* next to a wall: 2nd-order central, first-order upwind when |Pe| > 2 */
+        const double c2 = 0.5 * (fp1 - fm1) / h;
+        if (central) return c2;
+        if (blend > 0.0) return c2 + w * sg * (2.0 * f0 - fm1 - fp1) / (2.0 * h);
+        return vel > 0.0 ? (f0 - fm1) / h : (fp1 - f0) / h;
     }
+    if (mode == 0) return 0.5 * (fp1 - fm1) / h;
+    const double xpp = 8.0 * (fp1 - fm1);
+    const double xnn = fp2 - fm2;
+    const double c4 = (1.0 / 12.0) * (xpp - xnn) / h;
+    if (mode == 1 || central) return c4;
+    if (blend > 0.0) return c4 + w * sg * (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) / (4.0 * h);
     double ak1 = vel * (-fp2 + 8 * fp1 - 8 * fm1 + fm2) / (12.0 * h);
     const double ak2 = fabs(vel) * (fp2 - 4 * fp1 + 6 * f0 - 4 * fm1 + fm2) / (4.0 * h);
     ak1 = ak1 + ak2;
@@ -686,8 +760,14 @@ void or_cav_residuals(or_cav_case *c, double *Ru, double *Rv, double *RT, double
             const size_t P = ID(i, j);
             const int fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
             const int hyb = fit_i && fit_j;
-            const int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
-            const int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+            int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
+            int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+            if (pr->wall_upwind == 1) {
+                mode_i = fit_i ? 2 : 3;
+                mode_j = fit_j ? 2 : 3;
+            } else if (pr->wall_upwind == 2) {
+                mode_i = mode_j = 3;
+            }
             const double al = c->alph[P], be = c->beta[P], ga = c->gamma[P];
             const double pp = c->p1[P], qq = c->q1[P];
             double conv[3];
@@ -698,8 +778,8 @@ void or_cav_residuals(or_cav_case *c, double *Ru, double *Rv, double *RT, double
                 const double pec2 = V[P] * sc * h2 / ga;
                 const double fm2i = fit_i ? f[ID(i - 2, j)] : 0.0, fp2i = fit_i ? f[ID(i + 2, j)] : 0.0;
                 const double fm2j = fit_j ? f[ID(i, j - 2)] : 0.0, fp2j = fit_j ? f[ID(i, j + 2)] : 0.0;
-                const double dxi = conv_deriv(mode_i, fm2i, f[ID(i - 1, j)], f[P], f[ID(i + 1, j)], fp2i, U[P], pec1, h1);
-                const double det = conv_deriv(mode_j, fm2j, f[ID(i, j - 1)], f[P], f[ID(i, j + 1)], fp2j, V[P], pec2, h2);
+                const double dxi = conv_deriv(mode_i, fm2i, f[ID(i - 1, j)], f[P], f[ID(i + 1, j)], fp2i, U[P], pec1, h1, pr->hyb_blend);
+                const double det = conv_deriv(mode_j, fm2j, f[ID(i, j - 1)], f[P], f[ID(i, j + 1)], fp2j, V[P], pec2, h2, pr->hyb_blend);
                 conv[k] = U[P] * dxi + V[P] * det;
             }
             const double *p = c->p;
@@ -711,6 +791,11 @@ void or_cav_residuals(or_cav_case *c, double *Ru, double *Rv, double *RT, double
             Ru[P] = dt * (-conv[0] - dp_dx) - aref_offdiag(n0, c->u, i, j, P, al, be, ga, pp, qq, h1, h2, dt, Re, wide);
             Rv[P] = dt * (-conv[1] - dp_dy + Ri * c->T[P]) - aref_offdiag(n0, c->v, i, j, P, al, be, ga, pp, qq, h1, h2, dt, Re, wide);
             RT[P] = -dt * conv[2] - aref_offdiag(n0, c->T, i, j, P, al, be, ga, pp, qq, h1, h2, dt, Re * Pr, wide);
+            if (c->su) { /* FAS source terms of a coarse level */
+                Ru[P] += dt * c->su[P];
+                Rv[P] += dt * c->sv[P];
+                RT[P] += dt * c->sT[P];
+            }
             double cl[9];
             alo_coefs(al, be, ga, pp, qq, U[P], V[P], h1, h2, dt, Re, cl);
             dout[P] = dt * pr->alpha_u / cl[4];
@@ -840,7 +925,8 @@ static double face_div(or_cav_case *c)
                 const double vf = 0.5 * (vh[P] + vh[S]) - 0.5 * ds * ((xiy[P] + xiy[S]) * dpdxi + (ety[P] + ety[S]) * dpde);
                 gs = 0.5 * (jac[P] * etx[P] + jac[S] * etx[S]) * uf + 0.5 * (jac[P] * ety[P] + jac[S] * ety[S]) * vf;
             }
-            const double qq = (fe - fw) / h1 + (gn - gs) / h2;
+            double qq = (fe - fw) / h1 + (gn - gs) / h2;
+            if (c->sm) qq -= c->sm[P];
             c->q[P] = qq;
             sum += fabs(qq);
         }
@@ -966,8 +1052,179 @@ int or_cav_outer(or_cav_case *c, or_cav_stats *st)
         st->nu_cold = -nusselt(c, 1);
         st->p_iters = its;
         st->p_relres = relres;
+        st->mass_res = c->mass0 > 0.0 ? mass / c->mass0 : 0.0;
+        st->energy_res = c->energy0 > 0.0 ? el / c->energy0 : 0.0;
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FAS multigrid outer cycle with SIMPLE iterations as the smoother.          */
+/*                                                                            */
+/* The steady equations of a level are rho(x) = 0 with                        */
+/*   rho_u = R_u/dt + s_u, rho_v = R_v/dt + s_v, rho_T = R_T/dt + s_T          */
+/*   (R: the residual of or_cav_residuals, dt the pseudo-time step) and         */
+/*   rho_m = q(x) - s_m (q: the J-weighted face divergence of the state's face */
+/*   velocities with its own d factors).  A coarse level's sources are set so  */
+/* that its rho at the restricted state equals the full-weighting restriction */
+/* of the fine rho (the FAS tau correction); a SIMPLE iteration of a level    */
+/* drives its rho to zero, and the coarse change of the state, interpolated   */
+/* bilinearly, corrects the fine interior.                                    */
+
+/* rho of the current state on the interior (0 on walls) */
+static void state_residual(or_cav_case *c, double *ru, double *rv, double *rT, double *rm)
+{
+    const int n0 = c->n0, n1 = c->n1;
+    const size_t N = (size_t)n0 * n1;
+    const double idt = 1.0 / c->prm.dt;
+    or_cav_residuals(c, c->Ru, c->Rv, c->RT, c->d);
+    memcpy(c->us, c->u, N * sizeof(double));
+    memcpy(c->vs, c->v, N * sizeof(double));
+    face_div(c);
+    memset(ru, 0, N * sizeof(double)); memset(rv, 0, N * sizeof(double));
+    memset(rT, 0, N * sizeof(double)); memset(rm, 0, N * sizeof(double));
+    for (int j = 1; j < n1 - 1; j++)
+        for (int i = 1; i < n0 - 1; i++) {
+            const size_t P = ID(i, j);
+            ru[P] = c->Ru[P] * idt;
+            rv[P] = c->Rv[P] * idt;
+            rT[P] = c->RT[P] * idt;
+            rm[P] = c->q[P];
+        }
+}
+
+/* Residuals of the wall relations at the current state: the change that the
+ * adiabatic side-wall update of a SIMPLE iteration would make (wT, side walls),
+ * and the change that its wall-pressure update would make after that side-wall
+ * update (wp, the whole border), each with the level's sources as they are set;
+ * the state is left as it was. */
+static void wall_residual(or_cav_case *c, double *wT, double *wp)
+{
+    const int n0 = c->n0, n1 = c->n1;
+    const size_t N = (size_t)n0 * n1;
+    double *Ts = c->bi[6], *ps = c->bi[7];
+    memcpy(Ts, c->T, N * sizeof(double));
+    memcpy(ps, c->p, N * sizeof(double));
+    temperature_sides(c);
+    if (wT)
+        for (size_t k = 0; k < N; k++) wT[k] = c->T[k] - Ts[k];
+    if (wp) {
+        pressure_walls(c);
+        for (size_t k = 0; k < N; k++) wp[k] = c->p[k] - ps[k];
+    }
+    memcpy(c->T, Ts, N * sizeof(double));
+    memcpy(c->p, ps, N * sizeof(double));
+}
+
+/* full weighting of fine interior values onto coarse interior node (I, J) */
+static double fw(const double *r, int n0, int I, int J)
+{
+    const int i = 2 * I, j = 2 * J;
+    return (4.0 * r[ID(i, j)] + 2.0 * (r[ID(i - 1, j)] + r[ID(i + 1, j)] + r[ID(i, j - 1)] + r[ID(i, j + 1)]) +
+            (r[ID(i - 1, j - 1)] + r[ID(i + 1, j - 1)] + r[ID(i - 1, j + 1)] + r[ID(i + 1, j + 1)])) / 16.0;
+}
+
+/* bilinear interpolation of the coarse change e = xc - x0 at fine node (i, j) */
+static double prolong_at(const double *xc, const double *x0, int m0, int i, int j)
+{
+#define E(I, J) (xc[(size_t)(J) * m0 + (I)] - x0[(size_t)(J) * m0 + (I)])
+    const int I = i >> 1, J = j >> 1;
+    if ((i & 1) == 0 && (j & 1) == 0) return E(I, J);
+    if ((j & 1) == 0) return 0.5 * (E(I, J) + E(I + 1, J));
+    if ((i & 1) == 0) return 0.5 * (E(I, J) + E(I, J + 1));
+    return 0.25 * (E(I, J) + E(I + 1, J) + E(I, J + 1) + E(I + 1, J + 1));
+#undef E
+}
+
+static void fas_cycle(or_cav_case *c, or_cav_stats *st)
+{
+    const or_cav_params *pr = &c->prm;
+    or_cav_case *cc = c->coarse;
+    if (!cc) {
+        for (int k = 0; k < pr->fas_coarse; k++) or_cav_outer(c, st);
+        return;
+    }
+    for (int k = 0; k < pr->fas_pre; k++) or_cav_outer(c, st);
+    const int n0 = c->n0, n1 = c->n1, m0 = cc->n0, m1 = cc->n1;
+    const size_t M = (size_t)m0 * m1;
+    double *ru = c->bi[0], *rv = c->bi[1], *rT = c->bi[2], *rm = c->bi[3];
+    state_residual(c, ru, rv, rT, rm);
+    double *wT = c->bi[4], *wp = c->bi[5];
+    wall_residual(c, wT, wp);
+    /* restricted state (injection) */
+    for (int J = 0; J < m1; J++)
+        for (int I = 0; I < m0; I++) {
+            const size_t Q = (size_t)J * m0 + I, P = ID(2 * I, 2 * J);
+            cc->u[Q] = c->u[P]; cc->v[Q] = c->v[P]; cc->T[Q] = c->T[P]; cc->p[Q] = c->p[P];
+        }
+    memcpy(cc->u0, cc->u, M * sizeof(double)); memcpy(cc->v0, cc->v, M * sizeof(double));
+    memcpy(cc->T0, cc->T, M * sizeof(double)); memcpy(cc->p0, cc->p, M * sizeof(double));
+    /* coarse residuals of the restricted state without sources */
+    double *cu = cc->bi[0], *cv = cc->bi[1], *cT = cc->bi[2], *cm = cc->bi[3];
+    double *cwT = cc->bi[4], *cwp = cc->bi[5];
+    double *keep[4] = {cc->su, cc->sv, cc->sT, cc->sm};
+    cc->su = cc->sv = cc->sT = cc->sm = NULL;
+    state_residual(cc, cu, cv, cT, cm);
+    wall_residual(cc, cwT, NULL);
+    cc->su = keep[0]; cc->sv = keep[1]; cc->sT = keep[2]; cc->sm = keep[3];
+    memset(cc->su, 0, M * sizeof(double)); memset(cc->sv, 0, M * sizeof(double));
+    memset(cc->sT, 0, M * sizeof(double)); memset(cc->sm, 0, M * sizeof(double));
+    for (int J = 1; J < m1 - 1; J++)
+        for (int I = 1; I < m0 - 1; I++) {
+            const size_t Q = (size_t)J * m0 + I;
+            cc->su[Q] = fw(ru, n0, I, J) - cu[Q];
+            cc->sv[Q] = fw(rv, n0, I, J) - cv[Q];
+            cc->sT[Q] = fw(rT, n0, I, J) - cT[Q];
+            cc->sm[Q] = cm[Q] - fw(rm, n0, I, J);
+        }
+    /* boundary sources: the injected fine wall residual minus the coarse one; the
+     * pressure relation is evaluated after the side-wall update with its source */
+    for (int J = 1; J < m1 - 1; J++) {
+        const size_t P0 = ID(0, 2 * J), P1 = ID(n0 - 1, 2 * J);
+        cc->sT[(size_t)J * m0] = wT[P0] - cwT[(size_t)J * m0];
+        cc->sT[(size_t)J * m0 + m0 - 1] = wT[P1] - cwT[(size_t)J * m0 + m0 - 1];
+    }
+    cc->sm = NULL;
+    wall_residual(cc, NULL, cwp);
+    cc->sm = keep[3];
+    for (int J = 0; J < m1; J++)
+        for (int I = 0; I < m0; I++) {
+            if (I != 0 && J != 0 && I != m0 - 1 && J != m1 - 1) continue;
+            const size_t Q = (size_t)J * m0 + I;
+            cc->sm[Q] = wp[ID(2 * I, 2 * J)] - cwp[Q];
+        }
+    fas_cycle(cc, NULL);
+    /* coarse-grid correction of the fine interior, then the fine wall conditions */
+    for (int j = 1; j < n1 - 1; j++)
+        for (int i = 1; i < n0 - 1; i++) {
+            const size_t P = ID(i, j);
+            c->u[P] += prolong_at(cc->u, cc->u0, m0, i, j);
+            c->v[P] += prolong_at(cc->v, cc->v0, m0, i, j);
+            c->T[P] += prolong_at(cc->T, cc->T0, m0, i, j);
+            c->p[P] += prolong_at(cc->p, cc->p0, m0, i, j);
+        }
+    or_cav_apply_bcs(c);
+    for (int k = 0; k < pr->fas_post; k++) or_cav_outer(c, st);
+}
+
+int or_cav_fas_cycle(or_cav_case *c, or_cav_stats *st)
+{
+    or_cav_stats tmp;
+    if (!c->coarse) return or_cav_outer(c, st ? st : &tmp);
+    fas_cycle(c, st ? st : &tmp);
+    return 0;
+}
+
+int or_cav_fas_depth(const or_cav_case *c)
+{
+    int d = 0;
+    for (; c; c = c->coarse) d++;
+    return d;
+}
+
+void or_cav_state_residual(or_cav_case *c, double *ru, double *rv, double *rT, double *rm)
+{
+    state_residual(c, ru, rv, rT, rm);
 }
 
 void or_cav_p_apply(const or_cav_case *c, const double *d, const double *x, double *y)

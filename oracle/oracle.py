@@ -69,6 +69,11 @@ def lib():
         L.or_cav_p_solve.argtypes = [C.c_void_p, dp, dp, dp, _d, _i, C.POINTER(_d)]
         L.or_cav_p_solve.restype = _i
         L.or_cav_residuals.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_cav_fas_cycle.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_cav_fas_cycle.restype = _i
+        L.or_cav_fas_depth.argtypes = [C.c_void_p]
+        L.or_cav_fas_depth.restype = _i
+        L.or_cav_state_residual.argtypes = [C.c_void_p, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -78,18 +83,23 @@ class CavParams(C.Structure):
                 ("alpha_p", _d), ("alpha_t", _d), ("lid_u", _d), ("t_hot", _d),
                 ("t_cold", _d), ("mom_sweeps", _i), ("p_rtol", _d), ("p_maxit", _i),
                 ("mg_levels", _i), ("mg_pre", _i), ("mg_post", _i), ("mg_coarse", _i),
-                ("mg_omega", _d)]
+                ("mg_omega", _d), ("wall_upwind", _i), ("fas_levels", _i), ("fas_pre", _i),
+                ("fas_post", _i), ("fas_coarse", _i), ("fas_min_n", _i),
+                ("fas_coarse_scheme", _i), ("hyb_blend", _d)]
 
 
 class CavStats(C.Structure):
     _fields_ = [("mass_l1", _d), ("energy_l1", _d), ("mom_l1", _d), ("dmax", _d),
-                ("nu_hot", _d), ("nu_cold", _d), ("p_iters", _i), ("p_relres", _d)]
+                ("nu_hot", _d), ("nu_cold", _d), ("p_iters", _i), ("p_relres", _d),
+                ("mass_res", _d), ("energy_res", _d)]
 
 
 def cav_params(**kw):
-    base = dict(Re=100.0, Ri=1.0, Pr=0.71, dt=1.0e4, alpha_u=0.7, alpha_p=0.3, alpha_t=1.0,
+    base = dict(Re=100.0, Ri=1.0, Pr=0.71, dt=1.0e4, alpha_u=0.7, alpha_p=0.3, alpha_t=0.85,
                 lid_u=1.0, t_hot=1.0, t_cold=0.0, mom_sweeps=8, p_rtol=1e-2, p_maxit=4,
-                mg_levels=12, mg_pre=2, mg_post=2, mg_coarse=10, mg_omega=0.8)
+                mg_levels=12, mg_pre=2, mg_post=2, mg_coarse=10, mg_omega=0.8,
+                wall_upwind=1, fas_levels=8, fas_pre=2, fas_post=2, fas_coarse=40, fas_min_n=17,
+                fas_coarse_scheme=2, hyb_blend=1.0)
     base.update(kw)
     return CavParams(**base)
 
@@ -122,6 +132,13 @@ class CavityOracle:
         return dict(u=out[0], v=out[1], T=out[2], p=out[3])
 
     def step(self):
+        """One outer iteration: a FAS cycle, or one SIMPLE iteration without coarse levels."""
+        st = CavStats()
+        lib().or_cav_fas_cycle(self.h, C.byref(st))
+        return st
+
+    def simple_step(self):
+        """One SIMPLE iteration on this (finest) level alone."""
         st = CavStats()
         lib().or_cav_outer(self.h, C.byref(st))
         return st
@@ -129,6 +146,14 @@ class CavityOracle:
     def residuals(self):
         o = [np.zeros((self.n1, self.n0)) for _ in range(4)]
         lib().or_cav_residuals(self.h, *o)
+        return o
+
+    def fas_depth(self):
+        return lib().or_cav_fas_depth(self.h)
+
+    def state_residual(self):
+        o = [np.zeros((self.n1, self.n0)) for _ in range(4)]
+        lib().or_cav_state_residual(self.h, *o)
         return o
 
     def p_apply(self, d, x):

@@ -74,9 +74,12 @@ def test_p_solve_matches_dense():
 
 
 def test_residual_exact_on_quadratic():
+    """The reference's scheme (central next to walls, wall_upwind 0) and the reference's
+    step switch (hyb_blend 0): exact on quadratics wherever the hybrid branch is central
+    or KK (both exact to cubic order), zero on constants."""
     n = 33
     g = cavity_grid(n, skew=0.0, beta=None, control=False)
-    prm = cav_params(Re=50.0, dt=0.3, Ri=0.0)
+    prm = cav_params(Re=50.0, dt=0.3, Ri=0.0, wall_upwind=0, hyb_blend=0.0, fas_levels=1)
     c = CavityOracle(g, prm)
     X, Y = g.x, g.y
     u = X ** 2 + Y ** 2
@@ -93,11 +96,38 @@ def test_residual_exact_on_quadratic():
     assert np.abs(RT[1:-1, 1:-1]).max() < 1e-13
 
 
+@pytest.mark.parametrize("pe,w", [(2.5, 0.5), (8.0, 1.0), (1.5, 0.0)])
+def test_wall_rows_upwind_and_blend_closed_form(pe, w):
+    """Cavity convection next to walls (wall_upwind 1, hyb_blend 1): T = x^2 carried by a
+    uniform u = U0 on a Cartesian grid.  Interior rows use the hybrid scheme (central
+    or KK, exact on quadratics); the columns next to the side walls, where the
+    five-point stencil does not fit, use 2nd-order central blended towards first-order
+    upwind by w = min(1, |Pe| - 2): dT/dx = 2x - w h for U0 > 0 (2x + w h for U0 < 0).
+    A wrong upwind side, sign or blend weight changes these values."""
+    n = 17
+    g = cavity_grid(n, skew=0.0, beta=None, control=False)
+    h = 1.0 / (n - 1)
+    Re, Pr, dt = 1000.0, 0.71, 0.3
+    for sgn in (1.0, -1.0):
+        U0 = sgn * pe / (Re * Pr * h)  # cell Peclet of T (alph = 1): U0 Re Pr h
+        prm = cav_params(Re=Re, Pr=Pr, dt=dt, Ri=0.0, wall_upwind=1, hyb_blend=1.0, fas_levels=1)
+        c = CavityOracle(g, prm)
+        X = g.x
+        T = X ** 2
+        c.set_fields(np.full_like(X, U0), np.zeros_like(X), T, np.zeros_like(X))
+        RT = c.residuals()[2]
+        dTdx = 2 * X
+        dTdx[:, 1] -= sgn * w * h
+        dTdx[:, n - 2] -= sgn * w * h
+        expect = -dt * U0 * dTdx + dt * 2.0 / (Re * Pr)
+        np.testing.assert_allclose(RT[1:-1, 1:-1], expect[1:-1, 1:-1], rtol=1e-9, atol=1e-11)
+
+
 @pytest.mark.parametrize("skew,beta,tol", [(0.0, None, 1e-9), (0.1, 1.5, 1e-2)])
 def test_pure_conduction_nusselt(skew, beta, tol):
     g = cavity_grid(17, skew=skew, beta=beta)
     c = CavityOracle(g, cav_params(Ri=0.0, lid_u=0.0, mom_sweeps=8))
-    for _ in range(600):
+    for _ in range(1500):
         st = c.step()
     f = c.fields()
     assert np.abs(f["u"]).max() < 1e-12 and np.abs(f["v"]).max() < 1e-12
