@@ -174,6 +174,11 @@ typedef struct {
                            values; the finer of it and wall_upwind is used)         */
     double hyb_blend;   /* 0: the reference's switch at |Pe| = 2; > 0: blend from the
                            central to the upwind form over 2 < |Pe| < 2 + hyb_blend  */
+    int plant;          /* tests only, a deliberate mistake (0: none): 1-4 flips the
+                           sign of the normal-momentum term of the wall pressure on
+                           wall 0-3 (j=0, j=n1-1, i=0, i=n0-1), 5 the sign of the
+                           tangential term of the adiabatic side walls, 6 the sign
+                           of the velocity correction                              */
 } or_cav_params;
 
 typedef struct {
@@ -220,6 +225,14 @@ int or_cav_p_solve(or_cav_case *c, const double *d, const double *q, double *x,
                    double rtol, int maxit, double *relres);
 void or_cav_residuals(or_cav_case *c, double *Ru, double *Rv, double *RT, double *d);
 void or_cav_last_q(const or_cav_case *c, double *q, double *pc);
+/* q = J-weighted face divergence of the face velocities built from the nodal
+ * velocities us, vs, pressure p and factors d (minus the level's mass source). */
+void or_cav_face_div(or_cav_case *c, const double *us, const double *vs, const double *p,
+                     const double *d, double *q);
+/* The correction of an outer iteration: p' copied to the walls (pc_out), then
+ * u = us - d grad p', v likewise, p += alpha_p p' on the interior (p in/out). */
+void or_cav_correct(or_cav_case *c, const double *us, const double *vs, const double *pc,
+                    const double *d, double *u, double *v, double *p, double *pc_out);
 
 #ifdef __cplusplus
 }

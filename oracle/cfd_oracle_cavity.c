@@ -536,10 +536,12 @@ static void temperature_sides(or_cav_case *c)
     for (int j = 1; j < n1 - 1; j++) {
         size_t P = ID(0, j);
         double te = (T[ID(0, j + 1)] - T[ID(0, j - 1)]) / (2.0 * h2);
+        if (c->prm.plant == 5) te = -te;
         double txi = c->beta[P] / c->alph[P] * te;
         tl[j] = (4.0 * T[ID(1, j)] - T[ID(2, j)] - 2.0 * h1 * txi) / 3.0;
         P = ID(n0 - 1, j);
         te = (T[ID(n0 - 1, j + 1)] - T[ID(n0 - 1, j - 1)]) / (2.0 * h2);
+        if (c->prm.plant == 5) te = -te;
         txi = c->beta[P] / c->alph[P] * te;
         tr[j] = (4.0 * T[ID(n0 - 2, j)] - T[ID(n0 - 3, j)] + 2.0 * h1 * txi) / 3.0;
     }
@@ -602,6 +604,7 @@ static double wall_p_value(const or_cav_case *c, int side, int i, int j)
         else dpy = d2u / Re - conv + Ri * c->T[P];
     }
     const double J = c->ajac[P];
+    if (c->prm.plant == side + 1) { dpx = -dpx; dpy = -dpy; }
     if (side <= 1) {
         const double tx = -c->xiy[P] * J, ty = c->xix[P] * J; /* (x_eta, y_eta) */
         const double dpn = (dpx * tx + dpy * ty) * h2;
@@ -954,8 +957,9 @@ static void correct(or_cav_case *c)
     for (int j = 1; j < n1 - 1; j++)
         for (int i = 1; i < n0 - 1; i++) {
             const size_t P = ID(i, j);
-            const double dpx = 0.5 * (pc[ID(i + 1, j)] - pc[ID(i - 1, j)]) / h1;
-            const double dpe = 0.5 * (pc[ID(i, j + 1)] - pc[ID(i, j - 1)]) / h2;
+            double dpx = 0.5 * (pc[ID(i + 1, j)] - pc[ID(i - 1, j)]) / h1;
+            double dpe = 0.5 * (pc[ID(i, j + 1)] - pc[ID(i, j - 1)]) / h2;
+            if (c->prm.plant == 6) { dpx = -dpx; dpe = -dpe; }
             c->u[P] = c->us[P] - c->d[P] * (c->xix[P] * dpx + c->etx[P] * dpe);
             c->v[P] = c->vs[P] - c->d[P] * (c->xiy[P] * dpx + c->ety[P] * dpe);
             c->p[P] = c->p[P] + c->prm.alpha_p * pc[P];
@@ -1241,6 +1245,37 @@ int or_cav_p_solve(or_cav_case *c, const double *d, const double *q, double *x,
     memcpy(c->d, d, N * sizeof(double));
     inject_d(c);
     return bicgstab(c, q, x, rtol, maxit, relres);
+}
+
+void or_cav_face_div(or_cav_case *c, const double *us, const double *vs, const double *p,
+                     const double *d, double *q)
+{
+    const size_t N = (size_t)c->n0 * c->n1;
+    memcpy(c->us, us, N * sizeof(double));
+    memcpy(c->vs, vs, N * sizeof(double));
+    memcpy(c->p, p, N * sizeof(double));
+    memcpy(c->d, d, N * sizeof(double));
+    memset(c->q, 0, N * sizeof(double));
+    face_div(c);
+    memcpy(q, c->q, N * sizeof(double));
+}
+
+void or_cav_correct(or_cav_case *c, const double *us, const double *vs, const double *pc,
+                    const double *d, double *u, double *v, double *p, double *pc_out)
+{
+    const size_t N = (size_t)c->n0 * c->n1;
+    memcpy(c->us, us, N * sizeof(double));
+    memcpy(c->vs, vs, N * sizeof(double));
+    memcpy(c->pc, pc, N * sizeof(double));
+    memcpy(c->d, d, N * sizeof(double));
+    memcpy(c->u, u, N * sizeof(double));
+    memcpy(c->v, v, N * sizeof(double));
+    memcpy(c->p, p, N * sizeof(double));
+    correct(c);
+    memcpy(u, c->u, N * sizeof(double));
+    memcpy(v, c->v, N * sizeof(double));
+    memcpy(p, c->p, N * sizeof(double));
+    if (pc_out) memcpy(pc_out, c->pc, N * sizeof(double));
 }
 
 /* last p' right-hand side and solution (diagnostics for tests) */

@@ -74,6 +74,8 @@ def lib():
         L.or_cav_fas_depth.argtypes = [C.c_void_p]
         L.or_cav_fas_depth.restype = _i
         L.or_cav_state_residual.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_cav_face_div.argtypes = [C.c_void_p, dp, dp, dp, dp, dp]
+        L.or_cav_correct.argtypes = [C.c_void_p] + [dp] * 8
         _LIB = L
     return _LIB
 
@@ -85,7 +87,7 @@ class CavParams(C.Structure):
                 ("mg_levels", _i), ("mg_pre", _i), ("mg_post", _i), ("mg_coarse", _i),
                 ("mg_omega", _d), ("wall_upwind", _i), ("fas_levels", _i), ("fas_pre", _i),
                 ("fas_post", _i), ("fas_coarse", _i), ("fas_min_n", _i),
-                ("fas_coarse_scheme", _i), ("hyb_blend", _d)]
+                ("fas_coarse_scheme", _i), ("hyb_blend", _d), ("plant", _i)]
 
 
 class CavStats(C.Structure):
@@ -99,7 +101,7 @@ def cav_params(**kw):
                 lid_u=1.0, t_hot=1.0, t_cold=0.0, mom_sweeps=8, p_rtol=1e-2, p_maxit=4,
                 mg_levels=12, mg_pre=2, mg_post=2, mg_coarse=10, mg_omega=0.8,
                 wall_upwind=1, fas_levels=8, fas_pre=2, fas_post=2, fas_coarse=40, fas_min_n=17,
-                fas_coarse_scheme=2, hyb_blend=1.0)
+                fas_coarse_scheme=2, hyb_blend=1.0, plant=0)
     base.update(kw)
     return CavParams(**base)
 
@@ -155,6 +157,20 @@ class CavityOracle:
         o = [np.zeros((self.n1, self.n0)) for _ in range(4)]
         lib().or_cav_state_residual(self.h, *o)
         return o
+
+    def face_div(self, us, vs, p, d):
+        q = np.zeros((self.n1, self.n0))
+        c = np.ascontiguousarray
+        lib().or_cav_face_div(self.h, c(us, dtype=np.float64), c(vs, dtype=np.float64), c(p, dtype=np.float64),
+                              c(d, dtype=np.float64), q)
+        return q
+
+    def correct(self, us, vs, pc, d, u, v, p):
+        """(u, v, p, p' with its wall copies) after the correction of us, vs, p by p'."""
+        f = [np.array(a, dtype=np.float64) for a in (us, vs, pc, d, u, v, p)]
+        pco = np.zeros((self.n1, self.n0))
+        lib().or_cav_correct(self.h, *f, pco)
+        return f[4], f[5], f[6], pco
 
     def p_apply(self, d, x):
         y = np.zeros((self.n1, self.n0))

@@ -144,3 +144,124 @@ def test_cavity_converges_config1_like():
         st = c.step()
     assert st.mass_l1 / first.mass_l1 < 1e-4
     assert np.isfinite(st.nu_hot) and 1.5 < st.nu_hot < 4.0
+
+
+# ---- face divergence, correction and wall relations, pinned without the GPU ----------------
+
+def _pin_case(n=33, **kw):
+    g = cavity_grid(n, skew=0.15, beta=1.5)
+    return g, CavityOracle(g, cav_params(**kw))
+
+
+def test_face_div_constant_velocity_is_discretely_divergence_free():
+    """A constant velocity with p = 0 on the skewed, stretched grid: the J-weighted face
+    fluxes (J xi_x)_f u + (J xi_y)_f v satisfy the discrete metric identity, so q = 0 at
+    every node whose faces all lie inside (faces touching a wall carry no flux); on a
+    Cartesian grid a linear divergence-free field gives q = 0 there too."""
+    g, c = _pin_case()
+    n = g.n0
+    u = np.full((n, n), 0.3)
+    v = np.full((n, n), -0.7)
+    d = 0.5 + np.random.default_rng(5).random((n, n))
+    z = np.zeros((n, n))
+    q = c.face_div(u, v, z, d)
+    scale = np.abs(g.ajac * g.xix).max() * 0.7 / g.dxi
+    assert np.abs(q[2:-2, 2:-2]).max() < 1e-12 * scale
+    assert np.abs(q[1, 2:-2]).max() > 1e-3 * scale   # the wall faces do carry the difference
+    gc = cavity_grid(17, skew=0.0, beta=None)
+    cc = CavityOracle(gc, cav_params())
+    X, Y = gc.x, gc.y
+    qc = cc.face_div(1.0 + 2.0 * X - 3.0 * Y, 4.0 + 5.0 * X - 2.0 * Y, np.zeros_like(X), np.ones_like(X))
+    assert np.abs(qc[2:-2, 2:-2]).max() < 1e-12
+
+
+@pytest.mark.parametrize("plant", [0, 6])
+def test_correction_removes_exactly_L_d_pprime_from_q(plant):
+    """The correction u = u* - d grad p' (p' copied to the walls), p += p' leaves u-hat =
+    u + d grad p unchanged, so the face divergence drops by exactly L_d p', the flux-form
+    operator that the p' solve inverts: q(u_new, p + p') = q(u*, p) - L_d p'.  This ties
+    face_div's pressure terms and correct() to p_apply (pinned by the 5-point Cartesian
+    and conservation tests); a sign error in the velocity correction (plant 6) breaks it."""
+    g, c = _pin_case(alpha_p=1.0, plant=plant)
+    n = g.n0
+    rng = np.random.default_rng(9)
+    us, vs = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    for a in (us, vs):
+        a[0, :] = a[-1, :] = a[:, 0] = a[:, -1] = 0.0
+    p = rng.standard_normal((n, n))
+    pc = np.zeros((n, n))
+    pc[1:-1, 1:-1] = rng.standard_normal((n - 2, n - 2))
+    d = 0.5 + rng.random((n, n))
+    q1 = c.face_div(us, vs, p, d)
+    u2, v2, p2, pcw = c.correct(us, vs, pc, d, us.copy(), vs.copy(), p.copy())
+    # the walls' pressure moves with the wall copies of p' (what the wall update then rebuilds)
+    border = np.ones((n, n), bool)
+    border[1:-1, 1:-1] = False
+    p2[border] = p[border] + pcw[border]
+    q2 = c.face_div(u2, v2, p2, d)
+    Lp = c.p_apply(d, pc)
+    err = np.abs((q2 - q1 + Lp)[1:-1, 1:-1]).max()
+    scale = np.abs(Lp).max()
+    if plant == 0:
+        assert err < 1e-12 * scale
+        np.testing.assert_allclose(p2[1:-1, 1:-1], p[1:-1, 1:-1] + pc[1:-1, 1:-1], rtol=0, atol=1e-14)
+    else:
+        assert err > 0.1 * scale
+
+
+def _hydrostatic_errors(n, plant=0):
+    """Stable stratification T = 0.5 + y/2 (bottom 0.5, top 1), lid at rest, Ri = 1 on the
+    skewed stretched grid: the exact state is u = v = 0, T = 0.5 + y/2 and the hydrostatic
+    p = Ri (y/2 + y^2/4) (its walls are x = 0, 1 and y = 0, 1).  Converged with the FAS
+    cycle; max errors of T (all nodes, side walls), p (interior and each wall, mean
+    removed) and |u|, |v|."""
+    g = cavity_grid(n, skew=0.1, beta=1.5)
+    c = CavityOracle(g, cav_params(Re=100.0, Ri=1.0, lid_u=0.0, t_hot=0.5, t_cold=1.0, plant=plant))
+    for k in range(3000):
+        st = c.step()
+        if k > 3 and st.mass_res < 1e-11 and st.energy_res < 1e-11:
+            break
+    f = c.fields()
+    T_ex = 0.5 + 0.5 * g.y
+    dp = f["p"] - (0.5 * g.y + 0.25 * g.y ** 2)
+    dp -= dp[1:-1, 1:-1].mean()
+    return dict(T=np.abs(f["T"] - T_ex).max(),
+                Tside=max(np.abs(f["T"][:, 0] - T_ex[:, 0]).max(), np.abs(f["T"][:, -1] - T_ex[:, -1]).max()),
+                p=np.abs(dp[1:-1, 1:-1]).max(), p0=np.abs(dp[0, 1:-1]).max(), p1=np.abs(dp[-1, 1:-1]).max(),
+                p2=np.abs(dp[1:-1, 0]).max(), p3=np.abs(dp[1:-1, -1]).max(),
+                uv=max(np.abs(f["u"]).max(), np.abs(f["v"]).max()))
+
+
+@pytest.fixture(scope="module")
+def hydro_33():
+    return _hydrostatic_errors(33)
+
+
+def test_hydrostatic_conduction_converges_at_second_order(hydro_33):
+    """Observed orders over 17/33/65 of every error of _hydrostatic_errors are at least
+    1.5 (measured: T 3.2-3.7, p and the four walls' p 1.7-2.9, |u|, |v| 3.3-3.7): the
+    wall pressure of all four walls, the adiabatic side walls and the correction keep
+    the discrete state consistent with the exact one."""
+    e17, e65 = _hydrostatic_errors(17), _hydrostatic_errors(65)
+    for k in e17:
+        o1 = np.log2(e17[k] / hydro_33[k])
+        o2 = np.log2(hydro_33[k] / e65[k])
+        assert o1 > 1.5 and o2 > 1.5, (k, o1, o2)
+    assert e65["p"] < 3e-4 and e65["T"] < 1e-4 and e65["uv"] < 1e-4
+
+
+@pytest.mark.parametrize("plant,key", [(1, "p0"), (2, "p1"), (3, "p2"), (4, "p3"), (5, "Tside")])
+def test_hydrostatic_kat_catches_planted_mistakes(hydro_33, plant, key):
+    """A sign flip planted in the oracle in the normal-momentum term of one wall's pressure
+    (plants 1-4: walls j = 0, j = n1-1, i = 0, i = n0-1) or in the tangential term of the
+    adiabatic side walls (plant 5) drops that wall's order to about 1 or less and raises
+    its error several-fold (the grid lines meet the walls at an angle, so every term is
+    exercised)."""
+    e33, e65 = _hydrostatic_errors(33, plant), _hydrostatic_errors(65, plant)
+    order = np.log2(e33[key] / e65[key])
+    assert order < 1.3 or e65[key] > 4.0 * _HYDRO_65[key], (plant, key, order, e65[key])
+
+
+# unplanted 65 x 65 errors (_hydrostatic_errors(65)), the reference level of the planted test
+_HYDRO_65 = dict(T=6.48e-05, Tside=3.71e-05, p=1.52e-04, p0=2.12e-04, p1=1.81e-04, p2=1.29e-04, p3=1.32e-04,
+                 uv=3.42e-05)
