@@ -2159,7 +2159,7 @@ void cfd_params_default(cfd_params *p) {
     p->graph = 0;
     p->mg_tune = 1;
     p->wall_upwind = 1; p->hyb_blend = 1.0;
-    p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 17;
+    p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 33;
     p->fas_coarse_scheme = 2;
 }
 

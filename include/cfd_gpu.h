@@ -300,7 +300,8 @@ typedef struct {
     int fas_pre, fas_post; /* SIMPLE iterations per level before / after the
                            coarse-grid correction (default 2, 2)                  */
     int fas_coarse;     /* SIMPLE iterations on the coarsest level (default 40)   */
-    int fas_min_n;      /* no level coarser than this many nodes per side (17)   */
+    int fas_min_n;      /* no level coarser than this many nodes per side (33:
+                           a 17 x 17 level misleads the cycle at Re 1000)           */
     int fas_coarse_scheme; /* wall_upwind value of the coarse levels (default 2;
                            the more upwinded of it and wall_upwind is used)        */
 } cfd_params;

@@ -13,7 +13,8 @@ def pair(n, skew=0.05, beta=None, n1=None, **kw):
     from oracle.oracle import CavityOracle, cav_params
     g = cavity_grid(n, n1, skew=skew, beta=beta)
     gp = default_params(**kw)
-    op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields()})  # precision has no oracle field
+    # precision has no oracle field, the oracle's planted-mistake switch no GPU one
+    op = cav_params(**{k: getattr(gp, k) for k, _ in op_fields() if hasattr(gp, k)})
     return CavityCase(g, gp), CavityOracle(g, op), g
 
 
@@ -145,7 +146,7 @@ def test_mixed_converged_fields_match_fp64_oracle():
     """Refined mixed path vs the FP64 oracle on converged fields (33x33, both to 1e-9)."""
     print("\n[gpu] mixed vs FP64 oracle, 33x33 to 1e-9", flush=True)
     from cfd_solver_amd._lib import CFD_MIXED
-    gpu, cpu, g = pair(33, skew=0.1, beta=1.5, Re=100.0, Ri=1.0, precision=CFD_MIXED)
+    gpu, cpu, g = pair(33, skew=0.1, beta=1.5, Re=100.0, Ri=1.0, precision=CFD_MIXED, fas_min_n=17)
     st, ok = gpu.solve(tol=1e-9, max_outer=400)
     assert ok, (st.mass_res, st.energy_res, st.iter)
     for k in range(400):
