@@ -71,6 +71,7 @@ SIGNATURES = {
     "cfd_gauss": ([_i, _i] + [dp] * 25 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
     "cfd_ref_face_div": ([_i, _i, _d, _d] + [dp] * 7 + [_d, dp], _i),
     "cfd_sip9p": ([_i, _i] + [dp] * 9 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
+    "cfd_sip9p_noalloc": ([_i, _i] + [dp] * 9 + [dp, dp, _d, _i, C.POINTER(_i), C.POINTER(_d)], _i),
     "cfd_ref_pcor_bc": ([_i, _i, dp, dp, dp, _d, _d], _i),
     "cfd_ref_correct": ([_i, _i, _d, _d] + [dp] * 7 + [_d, dp, dp, dp], _i),
     "cfd_ref_outflow_up": ([_i, _i, dp, dp, dp, dp, dp, dp, _d, _d], _i),

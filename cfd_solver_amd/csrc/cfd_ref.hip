@@ -7,6 +7,7 @@
 // column n0-1 is addressed where the reference addresses it.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cfd_common.h"
@@ -979,6 +980,59 @@ int sop_solve(int n0, int n1, const double *const *planes_host, int npts, double
     return down(phi, x, N);
 }
 
+// ---- the p' solve that allocates nothing: every device buffer of the solve (the
+// nine operator planes, phi, q and the BiCGStab work) lives in one file-scope
+// array, sized for the reference's own array bounds, np1 x np2 = 350 x 570 nodes
+// (solver.cpp:13-14).  Each device holds its own copy, at its own address, found
+// once per device with hipGetSymbolAddress.  Calls that overlap on one device
+// share the array and race.
+constexpr size_t NA_MAX_NODES = (size_t)CFD_SIP9P_NOALLOC_MAX_NODES;
+// 9 planes, x, q and SopWork's nine vectors; partials for at most NA_MAX_NODES / 4
+// + 1024 tiles per row (a 5-wide grid has one tile per four rows); scalars, flag
+constexpr size_t NA_DOUBLES = 20 * NA_MAX_NODES + 2 * (NA_MAX_NODES / 4 + 1024) + 64;
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+std::mutex na_mu;
+double *na_ptr[64] = {nullptr};  // scratch's address on each device
+
+int na_scratch(double **out) {
+    int dev = 0;
+    CFD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return CFD_E_ARG;
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        CFD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        na_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = na_ptr[dev];
+    return CFD_OK;
+}
+
+// sop_solve with the buffers carved from scratch
+int sop_solve_noalloc(int n0, int n1, const double *const *planes_host, double *phi, const double *q, double rtol,
+                      int maxit, int *iters, double *relres) {
+    if (bad_grid(n0, n1) || !phi || !q) return CFD_E_ARG;
+    for (int k = 0; k < 9; k++)
+        if (!planes_host[k]) return CFD_E_ARG;
+    const size_t N = (size_t)n0 * n1;
+    if (N > NA_MAX_NODES || 11 * N + SopWork::doubles(n0, n1) > NA_DOUBLES) {
+        fprintf(stderr, "cfd_sip9p_noalloc: %d x %d nodes exceed its %zu-node scratch\n", n0, n1, NA_MAX_NODES);
+        return CFD_E_ARG;
+    }
+    double *base = nullptr;
+    CFD_RET(na_scratch(&base));
+    double *A = base, *x = base + 9 * N, *qd = base + 10 * N, *ws = base + 11 * N;
+    for (int k = 0; k < 9; k++)
+        CFD_HIP(hipMemcpy(A + (size_t)k * N, planes_host[k], N * sizeof(double), hipMemcpyHostToDevice));
+    CFD_HIP(hipMemcpy(x, phi, N * sizeof(double), hipMemcpyHostToDevice));
+    CFD_HIP(hipMemcpy(qd, q, N * sizeof(double), hipMemcpyHostToDevice));
+    double hflag[3] = {0.0, 0.0, 0.0};
+    CFD_RET(sop_core<1>(n0, n1, A, x, qd, SopWork::carve(ws, n0, n1), rtol, maxit, iters, relres, nullptr, hflag));
+    CFD_HIP(hipDeviceSynchronize());
+    return down(phi, x, N);
+}
+
 // sin and cos of the flow angle and the ellipse-perimeter factor of the Nusselt
 // number, evaluated on the host as the reference does (solver.cpp:1927-1982)
 struct ForceConst {
@@ -1248,6 +1302,13 @@ int cfd_sip9p(int n0, int n1, const double *ap, const double *ae, const double *
               double *phi, const double *q, double rtol, int maxit, int *iters, double *relres) {
     const double *pl[9] = {asw, as, ase, aw, ap, ae, anw, an, ane};
     return sop_solve<1>(n0, n1, pl, 9, phi, q, rtol, maxit, iters, relres);
+}
+
+int cfd_sip9p_noalloc(int n0, int n1, const double *ap, const double *ae, const double *as, const double *an,
+                      const double *aw, const double *ase, const double *asw, const double *ane, const double *anw,
+                      double *phi, const double *q, double rtol, int maxit, int *iters, double *relres) {
+    const double *pl[9] = {asw, as, ase, aw, ap, ae, anw, an, ane};
+    return sop_solve_noalloc(n0, n1, pl, phi, q, rtol, maxit, iters, relres);
 }
 
 int cfd_ref_face_div(int n0, int n1, double dxi, double deta, const double *up, const double *vp, const double *p,

@@ -106,6 +106,21 @@ int cfd_sip9p(int n0, int n1,
               const double *anw, double *phi, const double *q,
               double rtol, int maxit, int *iters, double *relres);
 
+/* cfd_sip9p that allocates no memory (no hipMalloc, no library allocation): the
+ * device buffers of the solve live in one static array of the library, sized for
+ * n0 * n1 <= CFD_SIP9P_NOALLOC_MAX_NODES (the reference's array bounds, 350 x 570;
+ * about 33 MB, held on every device the library runs on until the process ends).
+ * Larger systems return CFD_E_ARG.  Same arguments, results and iteration counts
+ * as cfd_sip9p.  Calls that overlap on one device share the array and race.
+ * No counterpart in the reference: this entry point is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). */
+#define CFD_SIP9P_NOALLOC_MAX_NODES (350 * 570)
+int cfd_sip9p_noalloc(int n0, int n1,
+                      const double *ap, const double *ae, const double *as, const double *an,
+                      const double *aw, const double *ase, const double *asw, const double *ane,
+                      const double *anw, double *phi, const double *q,
+                      double rtol, int maxit, int *iters, double *relres);
 /* Predictor velocities up/vp on the outer row j = n1-1: u/v where the free
  * stream enters (uinf*xnox + vinf*xnoy >= 0), else quadratic extrapolation from
  * the three rows inside.  Replaces solver.cpp:1351-1381. */
