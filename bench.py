@@ -1,33 +1,42 @@
 #!/usr/bin/env python3
-"""Benchmark: SIMPLE outer iterations of the steady mixed-convection cavity on one MI355X.
+"""Benchmark: the steady mixed-convection cavity's outer iterations on one MI355X.
 
 Headline (value): 1025 x 1025 nodes, skew amplitude 0.1 with tanh wall clustering
-beta = 1.5, Re = 400, Ri = 1, Pr = 0.71, relaxation 0.7/0.3, 8 predictor sweeps,
-p' BiCGStab + multigrid to 1e-2 per outer iteration; mixed path (FP32 predictor
-and p' inner solve, FP64 residuals, refinement and accumulation), with the FP64
-path beside it.  One step = one outer iteration over the whole case (fields
-resident in HBM), launched eagerly with the p' solver's convergence flags read
-once per BiCGStab iteration (faster than replaying a captured graph, which must
-unroll the iteration cap); the K timed steps are one call of the library's outer
-loop, which reads the statistics back after the last.  The batched and served
-runs replay captured graphs so the size classes overlap on their streams.
+beta = 1.5, Re = 400, Ri = 1, Pr = 0.71 (configuration 2), mixed path (FP32
+predictor sweeps and p' inner solves, FP64 residuals, refinement, accumulation and
+reductions), the FP64 path beside it.  One step = one outer iteration = one FAS
+multigrid cycle over the whole case (SIMPLE iterations as the smoother on grids
+1025, 513, ..., 17: two before and two after the coarse-grid correction on each
+level, 40 on the coarsest), fields resident in HBM; the K timed steps are one
+call of the library's outer loop.  A cell-iteration is one interior node through
+one SIMPLE iteration on any level, so value = sum over levels of interior nodes x
+SIMPLE iterations per cycle x cycles / s.  Then each precision solves the case
+from rest to mass and energy residual ratios of 1e-8 (time to solution, cycles)
+and the converged mixed fields are compared with the FP64 ones per field.
 
-    python bench.py [--steps K] [--warmup W] [--n 1025] [--cpu-iters 6] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n 1025] [--no-cpu]
                     [--sections main,batch,serve,piso,ogrid]
 
-Prints one JSON line (value = cell-iterations/s, a cell = an interior node).
-The roofline object is for the kernel with the largest device time per outer
-iteration, timed live with HIP events on the case's stream; its algorithmic
-bytes per launch are (arrays read + arrays written) x nodes x element size.
-cpu_baseline times the CPU oracle (a serial FP64 port of the same iteration) on
-this host for a bounded number of outer iterations.  Further objects in config:
-  batched   256 cases (65..513 nodes per side, mixed physics) advanced as four
-            size-class batches, all slots per launch, vs one case at a time;
-  served    continuous batching of Poisson-arriving solves to 1e-8: solves/s,
-            latency p50/p99;
-  p_isolate the p' solve alone at 2049 x 2049 to 1e-10, FP64 vs mixed;
-  ref_ogrid the reference's own O-grid time step resident on the GPU (257 x 513),
-            with the CPU oracle's reference-faithful step beside it.
+Prints one JSON line.  roofline: the fine-level kernel with the largest device
+time per step, timed live with HIP events on the case's stream; its algorithmic
+bytes per launch are (arrays read + arrays written) x nodes x element size; a
+whole-cycle estimate (the per-node byte model summed over every level's SIMPLE
+iterations, divided by the step time) stands beside it.  cpu_baseline times the
+CPU oracle (the same FAS cycles in serial FP64 C) on this host for a bounded
+number of cycles of the same case.  Further objects in config:
+  solve     1025^2 from rest to 1e-8, FP64 and mixed: cycles, seconds, per-field
+            relative L2 (u, v, T, p, Nu) of mixed against FP64;
+  batched   the 256-case mix (65..513 nodes per side, Re 50..1000, Ri 0.1..10, Pr
+            0.71/7) as batches of one size and multigrid depth, every slot to 1e-8
+            or its cycle cap: done / not converged / failed counts, time;
+  served    Poisson-arriving solves of 65..1025 nodes per side with the same
+            physics draws, continuous batching to 1e-8: solves/s, latency p50/p99,
+            beside the same requests solved one at a time on the plain FP64 path;
+  p_isolate the p' solve alone at 2049 x 2049 to 1e-10, FP64 vs mixed, with the
+            algorithmic HBM rate of its iterations;
+  ref_ogrid the reference's own O-grid time step resident on the GPU at the
+            reference's largest array size (350 x 570), with the CPU oracle's
+            reference-faithful step (SIP + Gauss-Seidel, serial) beside it.
 """
 from __future__ import annotations
 
@@ -44,6 +53,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SIMPLE cell-iters/s at 1025² FP64 vs mixed; served solves/s + p99 at 1e-8 res"
 HBM_PEAK_GBS = 8000.0
+PHYS = dict(Re=400.0, Ri=1.0, Pr=0.71)
+
 
 # algorithmic bytes per node per launch: (arrays read + arrays written) x element size
 def kernel_bytes(name, mixed):
@@ -58,6 +69,19 @@ def kernel_bytes(name, mixed):
         "residual": 13 * 8 + 8 + 5 * w,  # u,v,T,p, 4 metrics, alph..Q (FP64) -> d (FP64), R_u,R_v,R_T,U,V
         "face_div": 10 * 8 + 2 * w,  # u,v,p,d, 4 metrics, J (FP64), du,dv -> q (FP64)
     }[name]
+
+
+def simple_bytes_per_node(mixed, p_iters):
+    """Algorithmic bytes per node of one SIMPLE iteration: residual, two four-sweep
+    predictor launches, face divergence, face coefficients of every p' level (4/3 of
+    the fine level's), correction, and per BiCGStab iteration two V-cycles (fine down
+    and fused up-and-apply legs, coarser levels a third more) plus the vector updates
+    (p: 4 words, s: 3, x and r: 8); the mixed path adds one FP64 defect/p' pass."""
+    w = 4 if mixed else 8
+    fixed = (kernel_bytes("residual", mixed) + 2 * kernel_bytes("jacobi", mixed) + kernel_bytes("face_div", mixed)
+             + (10 * 8 + 4 * w) * 4 / 3 + 13 * 8 + 3 * w + (13 * 8 if mixed else 0))
+    per_it = 2 * (kernel_bytes("mg_down", mixed) + kernel_bytes("mg_up", mixed)) * 4 / 3 + 15 * w
+    return fixed + per_it * max(p_iters, 1)
 
 
 def hbm_traffic(mixed, name, n0, n1):
@@ -76,52 +100,56 @@ def hbm_traffic(mixed, name, n0, n1):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1025)
-    ap.add_argument("--cpu-iters", type=int, default=6)
+    ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--sections", default="main,batch,serve,piso,ogrid")
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="served arrivals per second")
-    ap.add_argument("--serve-count", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="served arrivals per second")
+    ap.add_argument("--serve-count", type=int, default=48)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
                          "rocprof's FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(grid, n_iters, kw):
-    """Serial CPU oracle on the same workload for a bounded number of outer iterations."""
-    from oracle.oracle import CavityOracle, build, cav_params
-    build()
-    c = CavityOracle(grid, cav_params(**kw))
-    c.step()  # first iteration sets the normalisers; not timed
-    t0 = time.perf_counter()
-    for _ in range(n_iters):
-        c.step()
-    dt = time.perf_counter() - t0
-    cells = (grid.n0 - 2) * (grid.n1 - 2)
-    model = ""
+def host_model():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": cells * n_iters / dt, "unit": "cell-iter/s", "cores": 1, "kind": "port",
-            "sample": f"{n_iters} outer iterations of the same {grid.n0}x{grid.n1} case, serial FP64 "
-                      f"C oracle (gcc -O3), 1 of {os.cpu_count()} host threads ({model})"}
+    return ""
 
 
-def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
-    """Timed region: `steps` outer iterations.  Then `prof_steps` iterations of the
-    same case with every hot-kernel launch bracketed by HIP events on the case's
-    stream (same kernels, same data)."""
+def cpu_baseline(grid, n_cycles):
+    """The CPU oracle (serial FP64 C, gcc -O3) on the same case: one untimed cycle, then
+    n_cycles timed; cell-iterations counted as on the GPU."""
+    from cfd_solver_amd.cavity import cells_per_outer, default_params
+    from oracle.oracle import CavityOracle, build, cav_params
+    build()
+    c = CavityOracle(grid, cav_params(**PHYS))
+    c.step()
+    t0 = time.perf_counter()
+    for _ in range(n_cycles):
+        c.step()
+    dt = time.perf_counter() - t0
+    cells = cells_per_outer(grid.n0, grid.n1, default_params(**PHYS))
+    return {"value": cells * n_cycles / dt, "unit": "cell-iter/s", "cores": 1, "kind": "port",
+            "sample": f"{n_cycles} FAS cycles (after one untimed) of the same {grid.n0}x{grid.n1} case, serial FP64 "
+                      f"C oracle (gcc -O3), 1 of {os.cpu_count()} host threads ({host_model()})"}
+
+
+def run_gpu(grid, mixed, steps, warmup, prof_steps=3):
+    """Timed region: `steps` outer iterations (FAS cycles) after `warmup`.  Then
+    `prof_steps` cycles with every fine-level hot-kernel launch bracketed by HIP
+    events on the case's stream (same kernels, same data)."""
     from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
-    from cfd_solver_amd.cavity import CavityCase
-    case = CavityCase(grid, Re=400.0, Ri=1.0, Pr=0.71, precision=CFD_MIXED if mixed else CFD_FP64)
+    from cfd_solver_amd.cavity import CavityCase, cells_per_outer, fas_hierarchy
+    case = CavityCase(grid, precision=CFD_MIXED if mixed else CFD_FP64, **PHYS)
     for _ in range(max(warmup, 1)):
         case.step()
     t0 = time.perf_counter()
@@ -133,110 +161,162 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=10):
     t1 = time.perf_counter()
     for _ in range(prof_steps):
         case.step()
-    eager_ms = (time.perf_counter() - t1) / prof_steps * 1e3
+    prof_ms = (time.perf_counter() - t1) / prof_steps * 1e3
     prof = case.profile_read()
     case.profile(False)
     n0, n1 = grid.n0, grid.n1
-    value = (n0 - 2) * (n1 - 2) * steps / elapsed
+    cells = cells_per_outer(n0, n1, case.params)
+    value = cells * steps / elapsed
+    ms_step = elapsed / steps * 1e3
     name, (ms_tot, launches) = max(prof.items(), key=lambda kv: kv[1][0])
     if launches <= 0 or ms_tot <= 0.0:
         raise RuntimeError(f"live kernel timing recorded nothing: {prof}")
     avg_ms = ms_tot / launches
     bytes_launch = int(kernel_bytes(name, mixed) * n0 * n1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    levels = fas_hierarchy(n0, n1, case.params)
+    cycle_bytes = sum(a * b * k for a, b, k in levels) * simple_bytes_per_node(mixed, max(st.p_iters, 1))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(mixed, name, n0, n1), "kernel": name,
             "avg_launch_ms": round(avg_ms, 5), "launches": launches, "bytes_per_launch": bytes_launch,
-            "timed_in": f"{prof_steps} outer iterations right after the timed steps",
-            "profiled_ms_per_step": round(eager_ms, 4),
+            "timed_in": f"{prof_steps} cycles right after the timed steps (fine level only)",
+            "profiled_ms_per_step": round(prof_ms, 4),
             "per_kernel_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
             "per_kernel_GBs": {k: round(kernel_bytes(k, mixed) * n0 * n1 / (v[0] / v[1] * 1e-3) / 1e9, 1)
-                               for k, v in prof.items() if v[1] > 0 and v[0] > 0}}
-    info = {"mass_res_last": st.mass_res, "p_iters_last": st.p_iters, "nu_hot_last": st.nu_hot}
+                               for k, v in prof.items() if v[1] > 0 and v[0] > 0},
+            "whole_cycle": {"bytes_model": round(cycle_bytes), "GBs": round(cycle_bytes / (ms_step * 1e-3) / 1e9, 1),
+                            "frac": round(cycle_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "fine_level_share_of_cells": round((n0 - 2) * (n1 - 2) * levels[0][2] / cells, 4)}}
+    info = {"levels": [list(l) for l in levels], "cells_per_step": cells, "mass_res_last": st.mass_res,
+            "p_iters_last": st.p_iters, "timed_cycles": f"{warmup + 1}..{warmup + steps} of a solve from rest"}
     case.close()
-    return value, elapsed / steps * 1e3, roof, info
+    return value, ms_step, roof, info
 
 
-def _single_ms(n, mixed, steps=15, warmup=3):
-    """ms per outer iteration of one n x n case alone."""
+def solve_to_tolerance(grid, mixed, tol=1e-8, cap=300):
+    """Fresh case from rest to mass and energy residual ratios <= tol: cycles, seconds
+    (the first cycle, which times the multigrid block sizes, included), fields."""
     from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
     from cfd_solver_amd.cavity import CavityCase
-    from cfd_solver_amd.grid import cavity_grid
-    c = CavityCase(cavity_grid(n, skew=0.075), precision=CFD_MIXED if mixed else CFD_FP64)
-    c.step(warmup)
+    case = CavityCase(grid, precision=CFD_MIXED if mixed else CFD_FP64, **PHYS)
     t0 = time.perf_counter()
-    c.step(steps)
-    ms = (time.perf_counter() - t0) / steps * 1e3
-    c.close()
-    return ms
+    for k in range(1, cap + 1):
+        st = case.step()
+        if st.mass_res <= tol and st.energy_res <= tol:
+            break
+    el = time.perf_counter() - t0
+    f = case.fields()
+    case.close()
+    return {"cycles": k, "seconds": el, "converged": bool(st.mass_res <= tol and st.energy_res <= tol),
+            "mass_res": st.mass_res, "energy_res": st.energy_res, "nu_hot": st.nu_hot, "nu_cold": st.nu_cold}, f
 
 
-def run_batched(mixed, steps=10, warmup=2):
-    """The 256-case mix as one batch per grid size; every step advances all cases."""
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_solve(grid):
+    out = {}
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        out[prec], fields[prec] = solve_to_tolerance(grid, prec == "mixed")
+        print(f"[bench] solve {prec}: {out[prec]['cycles']} cycles {out[prec]['seconds']:.3f} s", file=sys.stderr,
+              flush=True)
+    a, b = fields["mixed"], fields["fp64"]
+    err = {k: rel_l2(a[k], b[k]) for k in ("u", "v", "T")}
+    pa, pb = a["p"] - a["p"][1:-1, 1:-1].mean(), b["p"] - b["p"][1:-1, 1:-1].mean()
+    err["p"] = rel_l2(pa, pb)
+    err["Nu"] = abs(out["mixed"]["nu_hot"] - out["fp64"]["nu_hot"]) / abs(out["fp64"]["nu_hot"])
+    out["mixed_vs_fp64_rel_l2"] = err
+    out["mixed_speedup_time_to_solution"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["tolerance"] = 1e-8
+    return out
+
+
+def run_batched(mixed, cap=150, tol=1e-8):
+    """The 256-case mix as batches of one grid size and multigrid depth, on their own
+    streams (graph replay), every case to tol or its cap."""
     from collections import defaultdict
-    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
-    from cfd_solver_amd.cavity import CavityBatch, default_params
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED, load
+    import ctypes as C
+    from cfd_solver_amd.cavity import CavityBatch, cells_per_outer, default_params
     from cfd_solver_amd.grid import cavity_grid
     from cfd_solver_amd.workloads import batched_cases
     cases = batched_cases()
-    by_n = defaultdict(list)
-    for n, skew, phys in cases:
-        by_n[n].append((skew, phys))
-    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1)
+    L = load()
+    groups = defaultdict(list)
+    for idx, (n, skew, phys) in enumerate(cases):
+        p = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1, Re=phys["Re"])
+        groups[(n, L.cfd_fas_coarsest(n, n, C.byref(p)))].append((idx, skew, phys))
     batches = []
-    for n in sorted(by_n):
+    for (n, depth), members in sorted(groups.items()):
+        prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1,
+                             Re=max(ph["Re"] for _, _, ph in members))
         g0 = cavity_grid(n)
-        b = CavityBatch(n, n, g0.dxi, g0.deta, len(by_n[n]), prm)
-        for z, (skew, phys) in enumerate(by_n[n]):
+        b = CavityBatch(n, n, g0.dxi, g0.deta, len(members), prm)
+        for z, (_, skew, phys) in enumerate(members):
             b.set_case(z, cavity_grid(n, skew=skew), **phys)
-        batches.append(b)
-    for _ in range(warmup):
-        for b in batches:
-            b.launch()
-        for b in batches:
-            b.finish()
+        batches.append(dict(b=b, n=n, depth=depth, live=set(range(len(members))), members=members,
+                            cells=[cells_per_outer(n, n, default_params(Re=ph["Re"])) for _, _, ph in members]))
+    states = {}
+    cycles = {}
+    cell_its = 0.0
     t0 = time.perf_counter()
-    diverged = 0
-    for k in range(steps):
-        for b in batches:
-            b.launch()
-        for b in batches:
-            st = b.finish()
-            if any(s.iter == 0 for s in st):
-                raise RuntimeError("a batched case did not advance")
-            if k == steps - 1:
-                diverged += sum(1 for s in st if not np.isfinite(s.mass_l1) or not np.isfinite(s.energy_l1))
+    step = 0
+    while any(B["live"] for B in batches):
+        step += 1
+        live = [B for B in batches if B["live"]]
+        for B in live:
+            B["b"].launch()
+        for B in live:
+            st = B["b"].finish()
+            for z in list(B["live"]):
+                s = st[z]
+                cell_its += B["cells"][z]
+                idx = B["members"][z][0]
+                fin = None
+                if not np.isfinite(s.mass_l1) or not np.isfinite(s.energy_l1):
+                    fin = "failed"
+                elif s.mass_res <= tol and s.energy_res <= tol:
+                    fin = "done"
+                elif s.iter >= cap:
+                    fin = "notconv"
+                if fin:
+                    states[idx] = fin
+                    cycles[idx] = s.iter
+                    B["live"].discard(z)
+                    B["b"].release(z)
     el = time.perf_counter() - t0
-    cells = sum((n - 2) ** 2 for n, _, _ in cases)
-    nbytes = sum(b.nbytes() for b in batches)
-    for b in batches:
-        b.close()
-    single = {n: _single_ms(n, mixed) for n in sorted(by_n)}
-    alone_ms = sum(single[n] * len(by_n[n]) for n in by_n)
-    ms = el / steps * 1e3
-    return {"value": cells * steps / el, "unit": "cell-iter/s", "ms_per_step": ms, "cases": len(cases),
-            "nodes": sum(n * n for n, _, _ in cases), "size_classes": {str(n): len(v) for n, v in sorted(by_n.items())},
-            "hbm_bytes": nbytes, "non_finite_cases": diverged, "one_at_a_time_ms_per_step": alone_ms, "speedup_vs_one_at_a_time": alone_ms / ms,
-            "single_case_ms_per_step": {str(n): round(v, 4) for n, v in single.items()},
+    nbytes = sum(B["b"].nbytes() for B in batches)
+    for B in batches:
+        B["b"].close()
+    counts = {k: sum(1 for v in states.values() if v == k) for k in ("done", "notconv", "failed")}
+    done_cycles = [cycles[i] for i, v in states.items() if v == "done"]
+    return {"value": cell_its / el, "unit": "cell-iter/s", "seconds": el, "cases": len(cases), "states": counts,
+            "cycles_done_median": float(np.median(done_cycles)) if done_cycles else None,
+            "cycle_cap": cap, "tolerance": tol, "launch_rounds": step, "hbm_bytes": nbytes,
+            "batches": {f"{n}/{d}": len(v) for (n, d), v in sorted(groups.items())},
             "precision": "mixed" if mixed else "fp64",
             "workload": "256 cases, n log-uniform in [65, 513] (2^k+1), A ~ U[0, 0.15], Re ~ U[50, 1000], "
-                        "Ri log-uniform [0.1, 10], Pr in {0.71, 7}, seed 4; one step = one outer iteration of "
-                        "every case"}
+                        "Ri log-uniform [0.1, 10], Pr in {0.71, 7}, seed 4; batches keyed by size / coarsest "
+                        "multigrid level (\"n/depth\"), each case to mass and energy residual 1e-8 or 150 cycles; "
+                        "cases heated from below at these Rayleigh numbers often have no steady state"}
 
 
-def run_served(mixed, lam, count, kmin=5, kmax=6, tol=1e-8, slots=32, max_outer=10000):
-    """Poisson-arriving solves to tol, served by continuous batching."""
+def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=150):
+    """Poisson-arriving solves to tol, served by continuous batching, then the same
+    requests solved one at a time on the plain FP64 path."""
     from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
-    from cfd_solver_amd.cavity import default_params
+    from cfd_solver_amd.cavity import CavityCase, default_params
     from cfd_solver_amd.grid import cavity_grid
     from cfd_solver_amd.serve import CavityServer
     from cfd_solver_amd.workloads import served_cases
-    arrivals = served_cases(count, lam, kmin=kmin, kmax=kmax)
+    arrivals = served_cases(count, lam, kmin=kmin, kmax=kmax, re=(50.0, 1000.0), ri=(0.1, 10.0), pr=(0.71, 7.0))
     grids = [cavity_grid(n, skew=skew) for _, n, skew, _ in arrivals]
-    srv = CavityServer(default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1), hbm_budget=64 << 30,
+    srv = CavityServer(default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1), hbm_budget=96 << 30,
                        slots_per_batch=slots)
-    # warm the graphs of both size classes outside the timed run
-    warm = {srv.submit(cavity_grid(2 ** k + 1), tol=tol, max_outer=3) for k in range(kmin, kmax + 1)}
+    # warm a graph of every size class outside the timed run
+    warm = {srv.submit(cavity_grid(2 ** k + 1), tol=tol, max_outer=2) for k in range(kmin, kmax + 1)}
     while warm:
         srv.step()
         warm = {t for t in warm if srv.poll(t) is None}
@@ -268,14 +348,36 @@ def run_served(mixed, lam, count, kmin=5, kmax=6, tol=1e-8, slots=32, max_outer=
     el = time.perf_counter() - t0
     srv.close()
     lat = np.sort(np.array(lat))
+    # the same requests, one at a time, plain FP64 path (no batching, no graphs), each
+    # to the same tolerance or cap, back to back
+    t1 = time.perf_counter()
+    solo_states = {}
+    for k, (_, n, skew, phys) in enumerate(arrivals):
+        c = CavityCase(grids[k], precision=CFD_FP64, **phys)
+        st, ok = c.solve(tol=tol, max_outer=max_outer)
+        s = "done" if ok else ("failed" if not np.isfinite(st.mass_l1) else "notconv")
+        solo_states[s] = solo_states.get(s, 0) + 1
+        c.close()
+    solo = time.perf_counter() - t1
+    sizes = {}
+    for _, n, _, _ in arrivals:
+        sizes[str(n)] = sizes.get(str(n), 0) + 1
     return {"solves_per_s": count / el, "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "mean_outer_iterations": float(np.mean(its)),
             "requests": count, "arrival_rate_per_s": lam, "busy_fraction": busy / el, "states": states,
-            "elapsed_s": el, "tolerance": tol, "precision": "mixed" if mixed else "fp64",
+            "elapsed_s": el, "tolerance": tol, "precision": "mixed" if mixed else "fp64", "sizes": sizes,
+            "one_at_a_time_fp64": {"solves_per_s": count / solo, "seconds": solo, "states": solo_states},
             "workload": f"Poisson arrivals (seed 5) at {lam}/s, n log-uniform over 2^k+1, k in [{kmin}, {kmax}], "
-                        "A ~ U[0, 0.15], Re ~ U[50, 150], Ri log-uniform [0.1, 1], Pr 0.71; each solve to mass "
-                        f"and energy residual {tol:g} (cap {max_outer} outer iterations); {slots} slots per size "
-                        "class"}
+                        "A ~ U[0, 0.15], Re ~ U[50, 1000], Ri log-uniform [0.1, 10], Pr in {0.71, 7}; each solve "
+                        f"to mass and energy residual {tol:g} or {max_outer} cycles; {slots} slots per batch class"}
+
+
+def p_isolate_bytes_per_node(mixed, refinements):
+    """Algorithmic bytes per node of one BiCGStab iteration of the isolated p' solve
+    (as simple_bytes_per_node's per-iteration part) and of one FP64 refinement step."""
+    w = 4 if mixed else 8
+    per_it = 2 * (kernel_bytes("mg_down", mixed) + kernel_bytes("mg_up", mixed)) * 4 / 3 + 15 * w
+    return per_it, (13 * 8 if mixed else 0) * refinements
 
 
 def run_p_isolate(n=2049, reps=3):
@@ -297,19 +399,27 @@ def run_p_isolate(n=2049, reps=3):
         for _ in range(reps):
             _, its, rr = c.p_solve(None, None, rtol=1e-10, maxit=400, want_x=False)
         ms = (time.perf_counter() - t0) / reps * 1e3
-        out[name] = {"ms_per_solve": ms, "iterations": its, "relres": rr}
+        per_it, refine = p_isolate_bytes_per_node(name == "mixed", 3 if name == "mixed" else 0)
+        nbytes = (per_it * its + refine) * n * n
+        out[name] = {"ms_per_solve": ms, "iterations": its, "relres": rr,
+                     "algorithmic_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_peak": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         c.close()
     out["mixed_speedup"] = out["fp64"]["ms_per_solve"] / out["mixed"]["ms_per_solve"]
+    out["bytes_model"] = ("per BiCGStab iteration and node: two V-cycles (fine down leg 7.25 words, fused up leg "
+                          "and operator 9.75 words, coarser levels a third more) and 15 words of vector updates; "
+                          "mixed: FP32 words plus 13 FP64 words per refinement step (3 at 1e-10)")
     return out
 
 
-def run_ref_ogrid(n0=257, n1=513, steps=10, warmup=2, cpu_steps=2, no_cpu=False):
+def run_ref_ogrid(n0=350, n1=570, steps=10, warmup=2, cpu_steps=3, cpu_runs=3, no_cpu=False):
     """The reference solver's own O-grid time step (solver.cpp:1057-1852), resident on the
-    GPU, from the reference's initial state on an n0 x n1 O-grid around an ellipse of
-    aspect ratio 2 (outer radius 20), the reference's constants (Re 1000, dt 1e-4).
-    Every inner solve stops at 0.75e-2 of its first residual (2-norm; the reference's
-    Gauss-Seidel and SIP use the L1 norm); the CPU oracle's reference-faithful step
-    (SIP + Gauss-Seidel, serial) is timed beside it on the same grid."""
+    GPU, from the reference's initial state on an n0 x n1 O-grid (the reference's array
+    bounds, solver.cpp:13-14) around an ellipse of aspect ratio 2 (outer radius 20), the
+    reference's constants (Re 1000, dt 1e-4).  Every inner solve stops at 0.75e-2 of its
+    first residual (2-norm; the reference's Gauss-Seidel and SIP use the L1 norm); the
+    CPU oracle's reference-faithful step (SIP + Gauss-Seidel as solver.cpp:2066-2445,
+    serial) is timed beside it on the same grid, median of cpu_runs runs."""
     from cfd_solver_amd.grid import ogrid_ellipse
     from cfd_solver_amd.refcase import RefCase, initial_state, ref_params
     g = ogrid_ellipse(n0, n1, ar=2.0, r_out=20.0)
@@ -322,11 +432,12 @@ def run_ref_ogrid(n0=257, n1=513, steps=10, warmup=2, cpu_steps=2, no_cpu=False)
     s = c.step(steps)
     ms = (time.perf_counter() - t0) / steps * 1e3
     c.close()
+    cells = (n0 - 1) * (n1 - 2)
     out = {"grid": [n0, n1], "ms_per_step": ms, "steps": steps, "last_iters": s["iters"], "dmax": s["dmax"],
+           "cell_steps_per_s": cells / (ms * 1e-3),
            "workload": "reference O-grid time step, ellipse ar 2, r_out 20, Re 1000, dt 1e-4, "
                        "inner solves to 0.75e-2 relative"}
     if not no_cpu:
-        import ctypes as C
         from oracle.oracle import build, lib, ref_step
         build()
         L = lib()
@@ -344,13 +455,18 @@ def run_ref_ogrid(n0=257, n1=513, steps=10, warmup=2, cpu_steps=2, no_cpu=False)
         for _ in range(warmup):
             o = ref_step(gd, ops, st, pd)
             st = {k: o[k] for k in ("u", "v", "T", "p", "up", "vp")}
-        t0 = time.perf_counter()
-        for _ in range(cpu_steps):
-            o = ref_step(gd, ops, st, pd)
-            st = {k: o[k] for k in ("u", "v", "T", "p", "up", "vp")}
-        cms = (time.perf_counter() - t0) / cpu_steps * 1e3
+        runs = []
+        for _ in range(cpu_runs):
+            t0 = time.perf_counter()
+            for _ in range(cpu_steps):
+                o = ref_step(gd, ops, st, pd)
+                st = {k: o[k] for k in ("u", "v", "T", "p", "up", "vp")}
+            runs.append((time.perf_counter() - t0) / cpu_steps * 1e3)
+        cms = float(np.median(runs))
         out["cpu_oracle_ms_per_step"] = cms
-        out["cpu_sample"] = f"{cpu_steps} steps after {warmup}, SIP + Gauss-Seidel as in the reference, 1 thread"
+        out["cpu_cell_steps_per_s"] = cells / (cms * 1e-3)
+        out["cpu_sample"] = (f"median of {cpu_runs} runs of {cpu_steps} steps after {warmup}, SIP + Gauss-Seidel "
+                             f"as in the reference, 1 of {os.cpu_count()} host threads ({host_model()})")
         out["gpu_over_cpu"] = cms / ms
     return out
 
@@ -379,17 +495,20 @@ def main():
         print(f"[bench] {prec}: {res[prec][1]:.4f} ms/step", file=sys.stderr, flush=True)
     main_prec = "mixed" if "mixed" in res else "fp64"
     value, ms_step, roof, info = res[main_prec]
-    cfg = {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, Ri 1, Pr 0.71, "
-                       f"SIMPLE (0.7/0.3), one outer iteration per step, {main_prec} path",
+    cfg = {"workload": f"steady mixed-convection cavity {n}x{n}, skew 0.1 + tanh 1.5, Re 400, Ri 1, Pr 0.71; "
+                       f"one FAS cycle (SIMPLE 0.7/0.3/0.85 smoothing on every level) per step, {main_prec} path",
            "grid": [n, n], "precision": main_prec, **info}
     if "fp64" in res and "mixed" in res:
         cfg["fp64_value"] = res["fp64"][0]
         cfg["fp64_ms_per_step"] = res["fp64"][1]
         cfg["fp64_roofline"] = res["fp64"][2]
         cfg["mixed_over_fp64"] = res["mixed"][0] / res["fp64"][0]
+    if "main" in sections:
+        cfg["solve"] = run_solve(grid)
     if "batch" in sections:
         cfg["batched"] = run_batched(main_prec == "mixed")
-        print(f"[bench] batched: {cfg['batched']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        print(f"[bench] batched: {cfg['batched']['seconds']:.2f} s {cfg['batched']['states']}", file=sys.stderr,
+              flush=True)
     if "serve" in sections:
         cfg["served"] = run_served(main_prec == "mixed", a.serve_rate, a.serve_count)
         print(f"[bench] served: {cfg['served']['solves_per_s']:.2f} solves/s", file=sys.stderr, flush=True)
@@ -402,7 +521,8 @@ def main():
               flush=True)
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(grid, a.cpu_iters, dict(Re=400.0, Ri=1.0, Pr=0.71))
+        cpu = cpu_baseline(grid, a.cpu_cycles)
+        print(f"[bench] cpu baseline: {cpu['value']:.3e} cell-iter/s", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-iter/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

@@ -92,6 +92,7 @@ SIGNATURES = {
     "cfd_ref_vorticity": ([_i, _i, _d, _d] + [dp] * 6 + [dp], _i),
     "cfd_ref_forces": ([_i, _i, _d, _d] + [dp] * 8 + [_d, _d, _d, dp], _i),
     "cfd_params_default": ([C.POINTER(Params)], None),
+    "cfd_fas_coarsest": ([_i, _i, C.POINTER(Params)], _i),
     "cfd_case_create": ([_i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), C.POINTER(_p)], _i),
     "cfd_case_destroy": ([_p], _i),
     "cfd_case_set_fields": ([_p, dp, dp, dp, dp], _i),

@@ -10,6 +10,26 @@ from ._lib import Params, Stats, call, f64
 from .grid import Grid, cavity_grid
 
 
+def fas_hierarchy(n0: int, n1: int, params: Params) -> list[tuple[int, int, int]]:
+    """[(n0, n1, SIMPLE iterations per outer iteration)] of every level of the FAS
+    hierarchy a case of n0 x n1 nodes gets (one level without coarsening: one SIMPLE
+    iteration per outer iteration)."""
+    coarsest = _lib.load().cfd_fas_coarsest(n0, n1, C.byref(params))
+    levels = [(n0, n1)]
+    while min(levels[-1]) > coarsest:
+        a, b = levels[-1]
+        levels.append(((a + 1) // 2, (b + 1) // 2))
+    if len(levels) == 1:
+        return [(n0, n1, 1)]
+    its = [params.fas_pre + params.fas_post] * (len(levels) - 1) + [params.fas_coarse]
+    return [(a, b, k) for (a, b), k in zip(levels, its)]
+
+
+def cells_per_outer(n0: int, n1: int, params: Params) -> int:
+    """Interior nodes x SIMPLE iterations of every level in one outer iteration."""
+    return sum((a - 2) * (b - 2) * k for a, b, k in fas_hierarchy(n0, n1, params))
+
+
 def default_params(**kw) -> Params:
     p = Params()
     _lib.load().cfd_params_default(C.byref(p))

@@ -99,7 +99,7 @@ struct Case {
     float *ef = nullptr;                   // mixed: FP32 correction of one refinement step
     double *r64 = nullptr;                 // mixed: FP64 defect
     double *f64[4] = {nullptr};            // mixed: level-0 face coefficients in FP64 (defect)
-    double *part = nullptr;  // reduction partials: per slot 4 x pstride
+    double *part = nullptr;  // reduction partials: per slot PS_SLOTS rows of pstride
     int pstride = 0;
     double *scal = nullptr;  // device scalars, SS per slot
     double *hscal = nullptr; // pinned host mirror
@@ -1480,7 +1480,7 @@ int mg_setup(Case *c) {
 // Iterations and relative residual: S_IT / S_RELRES (FP64), S_ITTOT / S_RELRES64.
 // Times the multigrid block sizes once (PSolver::tune) unless switched off or the
 // stream is being captured.  The timing launches overwrite the solver's scratch
-// (level arrays, BiCGStab t, partial rows 0-1), so this runs only where none of
+// (level arrays, BiCGStab t, partial rows PS_APPLY_ROW and PS_APPLY_ROW + 1), so this runs only where none of
 // them holds a value still needed: before an outer iteration or a p' solve starts.
 int tune_mg(Case *c) {
     if (!c->prm.mg_tune || c->capturing) return CFD_OK;
@@ -1943,9 +1943,20 @@ void batch_destroy(Case *c) {
     (void)hipGetLastError();
 }
 
+// Smallest level of a FAS hierarchy (nodes per side): fas_min_n, or with
+// fas_min_n <= 0 the smallest 2^k + 1 >= 17 whose cell Reynolds number Re h = Re /
+// 2^k is at most 32 (a 17 x 17 level misleads the cycle from Re ~ 600 on, a 33 x 33
+// one at Re 1000 does not).
+int fas_min_eff(const cfd_params &p) {
+    if (p.fas_min_n > 0) return p.fas_min_n < 5 ? 5 : p.fas_min_n;
+    int m = 16;
+    while (32.0 * m < p.Re && m < (1 << 20)) m *= 2;
+    return m + 1;
+}
+
 // the coarse level of a FAS hierarchy below an n0 x n1 level, if there is one
 bool fas_coarsens(int n0, int n1, const cfd_params &p) {
-    const int lo = p.fas_min_n < 5 ? 5 : p.fas_min_n;
+    const int lo = fas_min_eff(p);
     return p.fas_levels > 1 && (n0 & 1) && (n1 & 1) && (n0 + 1) / 2 >= lo && (n1 + 1) / 2 >= lo;
 }
 
@@ -2094,6 +2105,14 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
         return CFD_E_ARG;
     const int n0 = c->n0, n1 = c->n1;
     const size_t N = c->N, B = N * sizeof(double), o = (size_t)z * N;
+    // a hierarchy built for a lower Reynolds number has a level too coarse for this case
+    if (c->coarse && c->prm.fas_min_n <= 0) {
+        cfd_params q = c->prm;
+        q.Re = phys->Re;
+        const Case *l = c;
+        while (l->coarse) l = l->coarse;
+        if (std::min(l->n0, l->n1) < fas_min_eff(q)) return CFD_E_ARG;
+    }
     // the slot's kernels must not run while its arrays are replaced
     c->sph[z].active = 0.0;
     RET(put_slot(c, z));
@@ -2159,8 +2178,18 @@ void cfd_params_default(cfd_params *p) {
     p->graph = 0;
     p->mg_tune = 1;
     p->wall_upwind = 1; p->hyb_blend = 1.0;
-    p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 33;
+    p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 0;
     p->fas_coarse_scheme = 2;
+}
+
+int cfd_fas_coarsest(int n0, int n1, const cfd_params *p) {
+    if (!p || n0 < 5 || n1 < 5) return CFD_E_ARG;
+    int a = n0, b = n1;
+    for (int l = 1; l < p->fas_levels && fas_coarsens(a, b, *p); l++) {
+        a = (a + 1) / 2;
+        b = (b + 1) / 2;
+    }
+    return std::min(a, b);
 }
 
 // ---- batches

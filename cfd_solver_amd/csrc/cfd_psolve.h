@@ -194,6 +194,8 @@ constexpr int FIN_NT = 1024;
 // value (one pass of eight loads per thread at 256 threads; larger grids take
 // more passes), and four wavefronts reduce and start faster than sixteen
 constexpr int BI_FIN_NT = 256;
+// blocks x partials above which alpha and omega get their own finish kernels
+constexpr long BI_FUSE_MAX = 8L << 20;
 
 // ---------------------------------------------------------------------------
 // operator rows
@@ -1401,7 +1403,9 @@ __global__ __launch_bounds__(PS_NT) void k_bi_v(FArgs<T> a, const T *__restrict_
 // alpha = rho / (r0, v) from the nvp partials of the v phase (vpart, row
 // PS_APPLY_ROW), totalled by every block in the fixed order of a one-block finish
 // (the same bits) while its own nodes load; block 0 keeps (r0, v) and alpha in the
-// device scalars for the kernels that follow
+// device scalars for the kernels that follow.  nvp = 0: alpha is already in the
+// scalars (k_bi_fin<FIN_V>, where every block re-reading the partials would cost
+// more than the finish launch)
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restrict__ r, const T *__restrict__ vv,
                                                 T *__restrict__ s, RedArgs ra, const double *__restrict__ vpart,
@@ -1421,19 +1425,24 @@ __global__ __launch_bounds__(PS_NT) void k_bi_s(int n0, int n1, const T *__restr
         rl[m] = r[fn[m].P];  // P = 0 for a missing node: in bounds, unused
         vl[m] = vv[fn[m].P];
     }
-    double tv[1];
-    sum_partials<1, PS_NT>(vpart, ra.pstride, nvp, tv);
-    __shared__ double s_alpha;
-    if (tlin() == 0) {
-        const double a = ra.scal[S_RHO] / tv[0];
-        s_alpha = a;
-        if (blin() == 0) {
-            ra.scal[S_RV] = tv[0];
-            ra.scal[S_ALPHA] = a;
+    double alpha;
+    if (nvp > 0) {
+        double tv[1];
+        sum_partials<1, PS_NT>(vpart, ra.pstride, nvp, tv);
+        __shared__ double s_alpha;
+        if (tlin() == 0) {
+            const double a = ra.scal[S_RHO] / tv[0];
+            s_alpha = a;
+            if (blin() == 0) {
+                ra.scal[S_RV] = tv[0];
+                ra.scal[S_ALPHA] = a;
+            }
         }
+        __syncthreads();
+        alpha = s_alpha;
+    } else {
+        alpha = ra.scal[S_ALPHA];
     }
-    __syncthreads();
-    const double alpha = s_alpha;
     double v[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < FLAT_NPT; m++) {
@@ -1475,7 +1484,8 @@ __global__ __launch_bounds__(PS_NT) void k_bi_t(FArgs<T> a, const T *__restrict_
 // iteration that stopped at the ||s|| test only x += alpha ph.  omega = (t, s) /
 // (t, t) from the ntp partials of the t phase (tpart, rows PS_APPLY_ROW and
 // PS_APPLY_ROW + 1), totalled by every block as k_bi_s totals alpha's; block 0
-// keeps the totals and omega in the device scalars
+// keeps the totals and omega in the device scalars.  ntp = 0: omega is already in
+// the scalars (k_bi_fin<FIN_T>)
 template <typename T>
 __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T *__restrict__ ph,
                                                  const T *__restrict__ sh, const T *__restrict__ s,
@@ -1514,20 +1524,25 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
         tl[m] = t[P];
         ol[m] = r0[P];
     }
-    double tt[2];
-    sum_partials<2, PS_NT>(tpart, ra.pstride, ntp, tt);
-    __shared__ double s_omega;
-    if (tlin() == 0) {
-        const double om = tt[1] > 0.0 ? tt[0] / tt[1] : 0.0;
-        s_omega = om;
-        if (blin() == 0) {
-            sc[S_TS] = tt[0];
-            sc[S_TT] = tt[1];
-            sc[S_OMEGA] = om;
+    double omega;
+    if (ntp > 0) {
+        double tt[2];
+        sum_partials<2, PS_NT>(tpart, ra.pstride, ntp, tt);
+        __shared__ double s_omega;
+        if (tlin() == 0) {
+            const double om = tt[1] > 0.0 ? tt[0] / tt[1] : 0.0;
+            s_omega = om;
+            if (blin() == 0) {
+                sc[S_TS] = tt[0];
+                sc[S_TT] = tt[1];
+                sc[S_OMEGA] = om;
+            }
         }
+        __syncthreads();
+        omega = s_omega;
+    } else {
+        omega = sc[S_OMEGA];
     }
-    __syncthreads();
-    const double omega = s_omega;
     double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int m = 0; m < FLAT_NPT; m++) {
@@ -1544,7 +1559,8 @@ __global__ __launch_bounds__(PS_NT) void k_bi_xr(int n0, int n1, int it, const T
     block_partials<2>(v, ra.part, ra.pstride);
 }
 
-// (the v and t phases, FIN_V and FIN_T, are totalled inside k_bi_s and k_bi_xr)
+// (the v and t phases, FIN_V and FIN_T, are totalled inside k_bi_s and k_bi_xr,
+// or by their own finish on large grids)
 enum { FIN_START1, FIN_START2, FIN_V, FIN_S, FIN_T, FIN_XR };
 
 // the scalar recurrence / convergence test of phase OP from the totals
@@ -1572,6 +1588,13 @@ __device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV
         s[S_ALPHA] = 1.0;
         s[S_OMEGA] = 1.0;
         s[S_DONE] = (bn == 0.0 || rn <= tol) ? 1.0 : 0.0;
+    } else if (OP == FIN_V) {
+        s[S_RV] = tot[0];
+        s[S_ALPHA] = s[S_RHO] / tot[0];
+    } else if (OP == FIN_T) {
+        s[S_TS] = tot[0];
+        s[S_TT] = tot[NV - 1];
+        s[S_OMEGA] = tot[NV - 1] > 0.0 ? tot[0] / tot[NV - 1] : 0.0;
     } else if (OP == FIN_S) {
         s[S_SS] = tot[0];
         const double sn = sqrt(tot[0]);
@@ -1605,8 +1628,7 @@ __global__ __launch_bounds__(BI_FIN_NT) void k_bi_fin(int gs, RedArgs ra, int np
                                                    unsigned long long seq = 0, unsigned long long *flag = nullptr) {
     ra = rz(ra);
     const bool skip = slot_skip(ra.scal, gs);  // read while the partials load
-    static_assert(OP != FIN_V && OP != FIN_T, "totalled by the next vector kernel");
-    constexpr int NV = (OP == FIN_START1 || OP == FIN_XR) ? 2 : 1;
+    constexpr int NV = (OP == FIN_START1 || OP == FIN_XR || OP == FIN_T) ? 2 : 1;
     double tot[NV];
     sum_partials<NV, BI_FIN_NT>(ra.part, ra.pstride, nparts, tot);
     if (tlin() != 0) return;
@@ -2059,6 +2081,10 @@ struct PSolver {
         const int nb = nflat(n0, n1);
         const bool fz = fused_tail();
         const int nbt = fz ? tail_parts() : nblk(n0, n1);  // partials of the v and t phases
+        // alpha and omega totalled by every block of k_bi_s / k_bi_xr while that reads
+        // little (measured: 1027 blocks x 2048 partials at 1025^2 is faster fused), by
+        // a one-block finish on larger grids (the reads grow as blocks x partials)
+        const bool sep = (long)nb * nbt > BI_FUSE_MAX;
         T *r = bi[0], *r0 = bi[1], *pv = bi[2], *vv = bi[3], *s = bi[4], *t = bi[5], *ph = bi[6], *sh = bi[7];
         const FArgs<T> a0 = fargs(L0);
         const RedArgs ra = cx.ra();
@@ -2081,7 +2107,8 @@ struct PSolver {
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_v<T>, gr, bl, cx.s, a0, (const T *)ph, vv,
                              (const T *)r0, rap);
-            k_bi_s<T><<<gf, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra, (const double *)rap.part, nbt);
+            if (sep) k_bi_fin<FIN_V><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, rap, nbt, it, 0.0, -1);
+            k_bi_s<T><<<gf, bl, 0, cx.s>>>(n0, n1, r, vv, s, ra, (const double *)rap.part, sep ? 0 : nbt);
             k_bi_fin<FIN_S><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, ra, nb, it, 0.0, -1);
             CFD_LAUNCH_CHECK();
             const Tail tt{MGA_T, t, nullptr, rap};
@@ -2089,7 +2116,9 @@ struct PSolver {
             if (!fz)
                 launch_timed(cx.prof(cx.owner, PK_APPLY), k_bi_t<T>, gr, bl, cx.s, a0, (const T *)sh, t,
                              (const T *)s, rap);
-            k_bi_xr<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra, (const double *)rap.part, nbt);
+            if (sep) k_bi_fin<FIN_T><<<g1, BI_FIN_NT, 0, cx.s>>>(S_DONE, rap, nbt, it, 0.0, -1);
+            k_bi_xr<T><<<gf, bl, 0, cx.s>>>(n0, n1, it, ph, sh, s, t, r0, x, r, ra, (const double *)rap.part,
+                                          sep ? 0 : nbt);
             // eager: the iteration's end flags go straight to host memory and the host
             // waits for them (no copy, no stream synchronisation)
             const bool flags = !graph && cx.flag_d && cx.seq;

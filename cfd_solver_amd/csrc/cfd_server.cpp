@@ -3,10 +3,11 @@
 // Requests (a grid, its physics, a tolerance and an iteration cap) queue on
 // submission.  Each scheduling round (cfd_server_step):
 //   1. admission, first come first served with skipping: a request takes a free
-//      slot of the batch of its grid shape; a shape without a batch gets one
-//      when its bytes fit the HBM budget (empty batches are released first to
-//      make room); a request that cannot be placed waits without blocking the
-//      requests behind it;
+//      slot of the batch of its grid shape and multigrid depth (the coarsest
+//      level its Reynolds number allows); a class without a batch gets one when
+//      its bytes fit the HBM budget (empty batches are released first when that
+//      makes room); a request that cannot be placed waits without blocking the
+//      requests behind it, and one whose batch alone exceeds the budget fails;
 //   2. one outer iteration of every batch with an active slot: all batches are
 //      launched on their own streams before any is waited for;
 //   3. retirement: a slot whose mass and energy residuals are at the tolerance,
@@ -52,6 +53,7 @@ struct Request {
 struct Class {
     int n0 = 0, n1 = 0;
     double dxi = 0, deta = 0;
+    int coarsest = 0;  // nodes per side of the batch's coarsest multigrid level
     cfd_batch b = nullptr;
     size_t bytes = 0;
     std::vector<long> owner;  // request per slot, 0 = free
@@ -61,7 +63,9 @@ struct Class {
         for (long o : owner) k += o != 0;
         return k;
     }
-    bool fits(const Request &r) const { return r.n0 == n0 && r.n1 == n1 && r.dxi == dxi && r.deta == deta; }
+    bool fits(const Request &r, int rc) const {
+        return r.n0 == n0 && r.n1 == n1 && r.dxi == dxi && r.deta == deta && rc == coarsest;
+    }
 };
 
 struct Server {
@@ -84,16 +88,37 @@ void drop_class(Server &s, size_t k) {
         if (kv.second.cls > (int)k) kv.second.cls--;
 }
 
-// a batch for r's grid shape within the budget (empty batches released first); -1: no room
+// the batch parameters of r's class: the server's, with r's Reynolds number (which
+// sets the multigrid depth when fas_min_n is automatic)
+cfd_params class_params(const Server &s, const Request &r) {
+    cfd_params p = s.prm;
+    p.Re = r.phys.Re;
+    return p;
+}
+int coarsest_of(const Server &s, const Request &r) {
+    const cfd_params p = class_params(s, r);
+    return cfd_fas_coarsest(r.n0, r.n1, &p);
+}
+
+// A batch for r's class within the budget; empty batches are released only when
+// that makes room.  -1: no room now; -2: never (the batch alone exceeds the budget,
+// or it cannot be built).
 int make_class(Server &s, const Request &r) {
+    const cfd_params p = class_params(s, r);
     size_t need = 0;
-    if (cfd_batch_bytes_needed(r.n0, r.n1, r.dxi, r.deta, s.slots, &s.prm, &need) != CFD_OK) return -2;
+    if (cfd_batch_bytes_needed(r.n0, r.n1, r.dxi, r.deta, s.slots, &p, &need) != CFD_OK) return -2;
+    if (need > s.budget) return -2;
+    size_t idle = 0;
+    for (const Class &c : s.classes)
+        if (c.active() == 0) idle += c.bytes;
+    if (s.used + need > s.budget && s.used - idle + need > s.budget) return -1;
     for (size_t k = s.classes.size(); k-- > 0 && s.used + need > s.budget;)
         if (s.classes[k].active() == 0) drop_class(s, k);
     if (s.used + need > s.budget) return -1;
     Class c;
     c.n0 = r.n0; c.n1 = r.n1; c.dxi = r.dxi; c.deta = r.deta;
-    if (cfd_batch_create(r.n0, r.n1, r.dxi, r.deta, s.slots, &s.prm, &c.b) != CFD_OK) return -2;
+    c.coarsest = cfd_fas_coarsest(r.n0, r.n1, &p);
+    if (cfd_batch_create(r.n0, r.n1, r.dxi, r.deta, s.slots, &p, &c.b) != CFD_OK) return -2;
     cfd_batch_bytes(c.b, &c.bytes);
     c.owner.assign(s.slots, 0);
     c.st.assign(s.slots, cfd_stats{});
@@ -121,14 +146,15 @@ void finish(Server &s, Request &r, int state) {
 void admit(Server &s) {
     for (auto it = s.queue.begin(); it != s.queue.end();) {
         Request &r = s.reqs[*it];
+        const int depth = coarsest_of(s, r);
         int k = -1;
         for (size_t q = 0; q < s.classes.size(); q++)
-            if (s.classes[q].fits(r) && s.classes[q].active() < s.slots) { k = (int)q; break; }
+            if (s.classes[q].fits(r, depth) && s.classes[q].active() < s.slots) { k = (int)q; break; }
         if (k < 0) {
-            bool shape_known = false;
-            for (const Class &c : s.classes) shape_known |= c.fits(r);
-            if (!shape_known) k = make_class(s, r);
-            if (k == -2) {  // the shape cannot be served at all
+            bool class_known = false;
+            for (const Class &c : s.classes) class_known |= c.fits(r, depth);
+            if (!class_known) k = make_class(s, r);
+            if (k == -2) {  // the class cannot be served at all
                 r.state = CFD_REQ_FAILED;
                 r.t_done = clk::now();
                 s.finished++;

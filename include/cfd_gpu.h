@@ -315,8 +315,10 @@ typedef struct {
     int fas_pre, fas_post; /* SIMPLE iterations per level before / after the
                            coarse-grid correction (default 2, 2)                  */
     int fas_coarse;     /* SIMPLE iterations on the coarsest level (default 40)   */
-    int fas_min_n;      /* no level coarser than this many nodes per side (33:
-                           a 17 x 17 level misleads the cycle at Re 1000)           */
+    int fas_min_n;      /* no level coarser than this many nodes per side; <= 0
+                           (default): the smallest 2^k + 1 >= 17 whose cell Reynolds
+                           number Re / 2^k is at most 32, from params->Re (a batch
+                           then takes only slots whose Re fits its coarsest level) */
     int fas_coarse_scheme; /* wall_upwind value of the coarse levels (default 2;
                            the more upwinded of it and wall_upwind is used)        */
 } cfd_params;
@@ -339,6 +341,9 @@ typedef struct cfd_batch_s *cfd_batch;
 
 /* Fill *p with the defaults used by the cavity configurations. */
 void cfd_params_default(cfd_params *p);
+/* Nodes per side (the smaller) of the coarsest level of the FAS hierarchy a case or
+ * batch of n0 x n1 nodes with these parameters gets (n0 or n1 when it has none). */
+int cfd_fas_coarsest(int n0, int n1, const cfd_params *p);
 
 /* Create a cavity case on the current device from INP.DAT-style grid arrays
  * (x, y, metrics, alph/beta/gamma, Jacobian) plus the Laplacian control

@@ -413,7 +413,13 @@ static void fas_build(or_cav_case *c, const double *x, const double *y, const do
 {
     const int n0 = c->n0, n1 = c->n1;
     const int m0 = (n0 + 1) / 2, m1 = (n1 + 1) / 2;
-    const int lo = c->prm.fas_min_n < 5 ? 5 : c->prm.fas_min_n;
+    int lo = c->prm.fas_min_n;
+    if (lo <= 0) { /* the smallest 2^k + 1 >= 17 with cell Reynolds number Re / 2^k <= 32 */
+        int m = 16;
+        while (32.0 * m < c->prm.Re && m < (1 << 20)) m *= 2;
+        lo = m + 1;
+    }
+    if (lo < 5) lo = 5;
     if (c->prm.fas_levels <= 1 || (n0 & 1) == 0 || (n1 & 1) == 0 || m0 < lo || m1 < lo) return;
     or_cav_params cp = c->prm;
     cp.fas_levels = c->prm.fas_levels - 1;
