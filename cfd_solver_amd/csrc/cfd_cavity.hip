@@ -1944,12 +1944,13 @@ void batch_destroy(Case *c) {
 }
 
 // Smallest level of a FAS hierarchy (nodes per side): fas_min_n, or with
-// fas_min_n <= 0 the smallest 2^k + 1 >= 17 whose cell Reynolds number Re h = Re /
-// 2^k is at most 32 (a 17 x 17 level misleads the cycle from Re ~ 600 on, a 33 x 33
-// one at Re 1000 does not).
+// fas_min_n <= 0 the smallest 2^k + 1 >= 33 whose cell Reynolds number Re h = Re /
+// 2^k is at most 32.  A 17 x 17 level misleads the cycle (a fixed point away from
+// the solution) for buoyant cases from Re ~ 300 on and at Re 600 without buoyancy;
+// 33 x 33 converges every steady case of the configurations up to Re 1000.
 int fas_min_eff(const cfd_params &p) {
     if (p.fas_min_n > 0) return p.fas_min_n < 5 ? 5 : p.fas_min_n;
-    int m = 16;
+    int m = 32;
     while (32.0 * m < p.Re && m < (1 << 20)) m *= 2;
     return m + 1;
 }

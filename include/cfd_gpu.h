@@ -316,7 +316,7 @@ typedef struct {
                            coarse-grid correction (default 2, 2)                  */
     int fas_coarse;     /* SIMPLE iterations on the coarsest level (default 40)   */
     int fas_min_n;      /* no level coarser than this many nodes per side; <= 0
-                           (default): the smallest 2^k + 1 >= 17 whose cell Reynolds
+                           (default): the smallest 2^k + 1 >= 33 whose cell Reynolds
                            number Re / 2^k is at most 32, from params->Re (a batch
                            then takes only slots whose Re fits its coarsest level) */
     int fas_coarse_scheme; /* wall_upwind value of the coarse levels (default 2;

@@ -223,7 +223,7 @@ def _hydrostatic_errors(n, plant=0):
     cycle; max errors of T (all nodes, side walls), p (interior and each wall, mean
     removed) and |u|, |v|."""
     g = cavity_grid(n, skew=0.1, beta=1.5)
-    c = CavityOracle(g, cav_params(Re=100.0, Ri=1.0, lid_u=0.0, t_hot=0.5, t_cold=1.0, plant=plant))
+    c = CavityOracle(g, cav_params(Re=100.0, Ri=1.0, lid_u=0.0, t_hot=0.5, t_cold=1.0, plant=plant, fas_min_n=17))
     for k in range(3000):
         st = c.step()
         if k > 3 and st.mass_res < 1e-11 and st.energy_res < 1e-11:
