@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("p_rtol", _d), ("p_maxit", _i), ("mg_levels", _i), ("mg_pre", _i), ("mg_post", _i),
                 ("mg_coarse", _i), ("mg_omega", _d), ("precision", _i), ("graph", _i), ("mg_tune", _i),
                 ("wall_upwind", _i), ("hyb_blend", _d), ("fas_levels", _i), ("fas_pre", _i), ("fas_post", _i),
-                ("fas_coarse", _i), ("fas_min_n", _i), ("fas_coarse_scheme", _i)]
+                ("fas_coarse", _i), ("fas_min_n", _i), ("fas_coarse_scheme", _i),
+                ("fas_small", _i)]
 
 
 class Stats(C.Structure):

@@ -129,6 +129,10 @@ struct Case {
     double *src[4] = {nullptr};
     double *x0[4] = {nullptr};
     bool use_src = false;
+    // a coarse level run by k_simple_small: its p' levels' FP64 face coefficients
+    // (east/north, normal/tangential) and inverse diagonals
+    double *sl_tab[CW_LEVS][5] = {};
+    struct SmallArgs *sl_args = nullptr;  // its arguments (device memory)
 };
 
 // ------------------------------------------------------------------------
@@ -149,6 +153,76 @@ struct GridPtrs {
 // Halo entries beyond the grid hold clamped copies that no interior row reads.
 constexpr int RT_X = 64, RT_Y = 8;
 inline dim3 grid_res(int n0, int n1, int B) { return dim3(tiles_inner(n0, RT_X), tiles_inner(n1, RT_Y), B); }
+
+// metric and operator coefficients of one node
+struct NC {
+    double xix, xiy, etx, ety, al, be, ga, pp, qq;
+};
+// residuals, contravariant velocities and d of one interior node
+struct ResOut {
+    double ru, rv, rt, U, V, d;
+};
+
+// Residual at interior node (i, j) of grid g (oracle: or_cav_residuals): at(f, di, dj)
+// reads field f (one of s_u, s_v, s_t, the arrays of u, v, T in the caller's layout)
+// at the neighbour (i + di, j + dj), pat(di, dj) reads p there; P indexes the node's
+// FAS sources.
+template <typename FA, typename PA>
+__device__ __forceinline__ ResOut residual_node(const CGrid &g, const SlotP &ph, const NC &c, int i, int j, size_t P,
+                                                const double *s_u, const double *s_v, const double *s_t, FA at,
+                                                PA pat, const Src &src) {
+    const int n0 = g.n0, n1 = g.n1;
+    const double h1 = g.h1, h2 = g.h2;
+    const bool fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
+    const bool hyb = fit_i && fit_j;
+    int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
+    int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
+    if (g.scheme == 1) {
+        mode_i = fit_i ? 2 : 3;
+        mode_j = fit_j ? 2 : 3;
+    } else if (g.scheme == 2) {
+        mode_i = mode_j = 3;
+    }
+    const double al = c.al, be = c.be, ga = c.ga, pp = c.pp, qq = c.qq;
+    const double UP = c.xix * at(s_u, 0, 0) + c.xiy * at(s_v, 0, 0);
+    const double VP = c.etx * at(s_u, 0, 0) + c.ety * at(s_v, 0, 0);
+    const double *fl[3] = {s_u, s_v, s_t};
+    double conv[3];
+    const double ial = 1.0 / al, iga = 1.0 / ga;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double *f = fl[k];
+        const double sc = (k <= 1) ? ph.Re : ph.Re * ph.Pr;
+        const double pec1 = UP * sc * h1 * ial;
+        const double pec2 = VP * sc * h2 * iga;
+        const double fm2i = fit_i ? at(f, -2, 0) : 0.0, fp2i = fit_i ? at(f, 2, 0) : 0.0;
+        const double fm2j = fit_j ? at(f, 0, -2) : 0.0, fp2j = fit_j ? at(f, 0, 2) : 0.0;
+        const double dxi = conv_deriv_r(mode_i, fm2i, at(f, -1, 0), at(f, 0, 0), at(f, 1, 0), fp2i, UP, pec1,
+                                        g.r.ih1, g.blend);
+        const double det = conv_deriv_r(mode_j, fm2j, at(f, 0, -1), at(f, 0, 0), at(f, 0, 1), fp2j, VP, pec2,
+                                        g.r.ih2, g.blend);
+        conv[k] = UP * dxi + VP * det;
+    }
+    const double dp_dxi = (pat(1, 0) - pat(-1, 0)) * g.r.i2h1;
+    const double dp_de = (pat(0, 1) - pat(0, -1)) * g.r.i2h2;
+    const double dp_dx = c.xix * dp_dxi + c.etx * dp_de;
+    const double dp_dy = c.xiy * dp_dxi + c.ety * dp_de;
+    const double dt = ph.dt;
+    auto A_ = [&](const double *f, double dn) {
+        return aref_offdiag_a([&](int di, int dj) { return at(f, di, dj); }, al, be, ga, pp, qq, g.r, dn, hyb);
+    };
+    double ru = dt * (-conv[0] - dp_dx) - A_(s_u, ph.dnu);
+    double rv = dt * (-conv[1] - dp_dy + ph.Ri * at(s_t, 0, 0)) - A_(s_v, ph.dnu);
+    double rt = -dt * conv[2] - A_(s_t, ph.dnt);
+    if (src.u) {  // FAS sources of a coarse level
+        ru += dt * src.u[P];
+        rv += dt * src.v[P];
+        rt += dt * src.T[P];
+    }
+    double cl[9];
+    alo_coefs_r<double>(al, be, ga, pp, qq, UP, VP, g.r, dt, ph.dnu, cl);
+    return ResOut{ru, rv, rt, UP, VP, dt * ph.alpha_u / cl[4]};
+}
 
 template <typename TR>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_residual(
@@ -177,9 +251,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const auto lp = batch_load<PX * PY, NT>(tid, [&](int k) {
         return p[idx(clampi(i0 - 1 + k % PX, 0, n0 - 1), clampi(j0 - 1 + k / PX, 0, n1 - 1), n0)];
     });
-    struct NC {
-        double xix, xiy, etx, ety, al, be, ga, pp, qq;
-    };
     NC nc[NODES];
 #pragma unroll
     for (int m = 0; m < NODES; m++) {
@@ -196,73 +267,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     });
     batch_run<PX * PY, NT>(tid, lp, [&](int k, double pv) { s_p[k] = pv; });
     __syncthreads();
-    const double h1 = g.h1, h2 = g.h2;
     double e_abs = 0.0, m_abs = 0.0;
 #pragma unroll
     for (int m = 0; m < NODES; m++) {
         const int li = threadIdx.x, lj = threadIdx.y + m * BY;
         const int i = i0 + li, j = j0 + lj;
         if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) continue;
-        const NC &c = nc[m];
         const size_t P = idx(i, j, n0);
         auto at = [&](const double *sf, int di, int dj) { return sf[(lj + 2 + dj) * FX + (li + 2 + di)]; };
         auto pat = [&](int di, int dj) { return s_p[(lj + 1 + dj) * PX + (li + 1 + di)]; };
-        const bool fit_i = (i >= 2 && i <= n0 - 3), fit_j = (j >= 2 && j <= n1 - 3);
-        const bool hyb = fit_i && fit_j;
-        int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
-        int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
-        if (g.scheme == 1) {
-            mode_i = fit_i ? 2 : 3;
-            mode_j = fit_j ? 2 : 3;
-        } else if (g.scheme == 2) {
-            mode_i = mode_j = 3;
-        }
-        const double al = c.al, be = c.be, ga = c.ga, pp = c.pp, qq = c.qq;
-        const double UP = c.xix * at(s_u, 0, 0) + c.xiy * at(s_v, 0, 0);
-        const double VP = c.etx * at(s_u, 0, 0) + c.ety * at(s_v, 0, 0);
-        const double *fl[3] = {s_u, s_v, s_t};
-        double conv[3];
-        const double ial = 1.0 / al, iga = 1.0 / ga;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double *f = fl[k];
-            const double sc = (k <= 1) ? ph.Re : ph.Re * ph.Pr;
-            const double pec1 = UP * sc * h1 * ial;
-            const double pec2 = VP * sc * h2 * iga;
-            const double fm2i = fit_i ? at(f, -2, 0) : 0.0, fp2i = fit_i ? at(f, 2, 0) : 0.0;
-            const double fm2j = fit_j ? at(f, 0, -2) : 0.0, fp2j = fit_j ? at(f, 0, 2) : 0.0;
-            const double dxi = conv_deriv_r(mode_i, fm2i, at(f, -1, 0), at(f, 0, 0), at(f, 1, 0), fp2i, UP, pec1,
-                                            g.r.ih1, g.blend);
-            const double det = conv_deriv_r(mode_j, fm2j, at(f, 0, -1), at(f, 0, 0), at(f, 0, 1), fp2j, VP, pec2,
-                                            g.r.ih2, g.blend);
-            conv[k] = UP * dxi + VP * det;
-        }
-        const double dp_dxi = (pat(1, 0) - pat(-1, 0)) * g.r.i2h1;
-        const double dp_de = (pat(0, 1) - pat(0, -1)) * g.r.i2h2;
-        const double dp_dx = c.xix * dp_dxi + c.etx * dp_de;
-        const double dp_dy = c.xiy * dp_dxi + c.ety * dp_de;
-        const double dt = ph.dt;
-        auto A_ = [&](const double *f, double dn) {
-            return aref_offdiag_a([&](int di, int dj) { return at(f, di, dj); }, al, be, ga, pp, qq, g.r, dn, hyb);
-        };
-        double ru = dt * (-conv[0] - dp_dx) - A_(s_u, ph.dnu);
-        double rv = dt * (-conv[1] - dp_dy + ph.Ri * at(s_t, 0, 0)) - A_(s_v, ph.dnu);
-        double rt = -dt * conv[2] - A_(s_t, ph.dnt);
-        if (src.u) {  // FAS sources of a coarse level
-            ru += dt * src.u[P];
-            rv += dt * src.v[P];
-            rt += dt * src.T[P];
-        }
-        Ru[P] = (TR)ru;
-        Rv[P] = (TR)rv;
-        RT[P] = (TR)rt;
-        Ua[P] = (TR)UP;
-        Va[P] = (TR)VP;
-        double cl[9];
-        alo_coefs_r<double>(al, be, ga, pp, qq, UP, VP, g.r, dt, ph.dnu, cl);
-        dd[P] = dt * ph.alpha_u / cl[4];
-        e_abs += fabs(rt);
-        m_abs += fabs(ru) + fabs(rv);
+        const ResOut ro = residual_node(g, ph, nc[m], i, j, P, s_u, s_v, s_t, at, pat, src);
+        Ru[P] = (TR)ro.ru;
+        Rv[P] = (TR)ro.rv;
+        RT[P] = (TR)ro.rt;
+        Ua[P] = (TR)ro.U;
+        Va[P] = (TR)ro.V;
+        dd[P] = ro.d;
+        e_abs += fabs(ro.rt);
+        m_abs += fabs(ro.ru) + fabs(ro.rv);
     }
     double sums[2] = {e_abs, m_abs};
     block_partials<2>(sums, ra.part, ra.pstride);
@@ -467,6 +489,48 @@ __global__ __launch_bounds__(NT, sizeof(TR) == 4 && !FIRST ? 4 : 1) void k_jacob
     }
 }
 
+// arrays a face-divergence node reads at its neighbours: u-hat, v-hat, d, xi_x, xi_y,
+// eta_x, eta_y, J
+enum { FD_UH, FD_VH, FD_D, FD_X, FD_Y, FD_E, FD_F, FD_J, FD_N };
+
+// J-weighted divergence of the face velocities at interior node (i, j) (oracle:
+// face_div, without its source): L(a, di, dj) reads array a (FD_*) at the neighbour
+// (i + di, j + dj), P_(di, dj) reads p there.  Every face is evaluated, and the
+// faces touching a wall node are then replaced by zero, so wall values read here
+// need only be finite or not at all: they are selected away.
+template <typename LA, typename PA>
+__device__ __forceinline__ double face_div_node(const CGrid &g, int i, int j, LA L, PA P_) {
+    const int n0 = g.n0, n1 = g.n1;
+    // face between (i, j) and its neighbour (di, dj) (one of them zero): the
+    // face-normal velocity times the J-weighted metric of direction xi (di != 0)
+    // or eta (dj != 0)
+    auto face = [&](int di, int dj, double dpdxi, double dpde) {
+        const double dm = 0.5 * (L(FD_D, 0, 0) + L(FD_D, di, dj));
+        const double uf = 0.5 * (L(FD_UH, 0, 0) + L(FD_UH, di, dj)) -
+                          0.5 * dm * ((L(FD_X, 0, 0) + L(FD_X, di, dj)) * dpdxi +
+                                      (L(FD_E, 0, 0) + L(FD_E, di, dj)) * dpde);
+        const double vf = 0.5 * (L(FD_VH, 0, 0) + L(FD_VH, di, dj)) -
+                          0.5 * dm * ((L(FD_Y, 0, 0) + L(FD_Y, di, dj)) * dpdxi +
+                                      (L(FD_F, 0, 0) + L(FD_F, di, dj)) * dpde);
+        const int mx = di != 0 ? FD_X : FD_E, my = di != 0 ? FD_Y : FD_F;
+        return 0.5 * (L(FD_J, 0, 0) * L(mx, 0, 0) + L(FD_J, di, dj) * L(mx, di, dj)) * uf +
+               0.5 * (L(FD_J, 0, 0) * L(my, 0, 0) + L(FD_J, di, dj) * L(my, di, dj)) * vf;
+    };
+    double fe = face(1, 0, (P_(1, 0) - P_(0, 0)) * g.r.ih1,
+                     (P_(1, 1) + P_(0, 1) - P_(0, -1) - P_(1, -1)) * (0.25 * g.r.ih2));
+    double fw = face(-1, 0, (P_(0, 0) - P_(-1, 0)) * g.r.ih1,
+                     (P_(0, 1) + P_(-1, 1) - P_(0, -1) - P_(-1, -1)) * (0.25 * g.r.ih2));
+    double gn = face(0, 1, (P_(1, 1) - P_(-1, 1) + P_(1, 0) - P_(-1, 0)) * (0.25 * g.r.ih1),
+                     (P_(0, 1) - P_(0, 0)) * g.r.ih2);
+    double gs = face(0, -1, (P_(1, 0) - P_(-1, 0) + P_(1, -1) - P_(-1, -1)) * (0.25 * g.r.ih1),
+                     (P_(0, 0) - P_(0, -1)) * g.r.ih2);
+    fe = i + 1 != n0 - 1 ? fe : 0.0;
+    fw = i - 1 != 0 ? fw : 0.0;
+    gn = j + 1 != n1 - 1 ? gn : 0.0;
+    gs = j - 1 != 0 ? gs : 0.0;
+    return (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
+}
+
 // Face velocities and their divergence in one pass.  On a 64 x 4 tile plus a
 // one-node halo, û = u* + d grad p with u* = u + du (the pressure-free velocity
 // the face interpolation starts from) is evaluated into LDS from the tile's
@@ -489,7 +553,7 @@ __global__ __launch_bounds__(NT) void k_face_div(
     const size_t o = zslot() * g.N;
     q += o;
     constexpr int HX = BX + 2, HY = BY + 2, PX = BX + 4, PY = BY + 4;
-    enum { UH, VH, DD, XX, XY, EX, EY, JJ, NA };
+    enum { UH = FD_UH, VH = FD_VH, DD = FD_D, XX = FD_X, XY = FD_Y, EX = FD_E, EY = FD_F, JJ = FD_J, NA = FD_N };
     __shared__ double s[NA][HX * HY];
     __shared__ double s_p[PX * PY];
     const int n0 = g.n0, n1 = g.n1;
@@ -535,33 +599,8 @@ __global__ __launch_bounds__(NT) void k_face_div(
     if (i >= 1 && i <= n0 - 2 && j >= 1 && j <= n1 - 2) {
         // array a at the node offset (di, dj) from (i, j)
         auto L = [&](int a, int di, int dj) { return s[a][(lj + 1 + dj) * HX + (li + 1 + di)]; };
-        // face between (i, j) and its neighbour (di, dj) (one of them zero): the
-        // face-normal velocity times the J-weighted metric of direction xi (di != 0)
-        // or eta (dj != 0)
-        auto face = [&](int di, int dj, double dpdxi, double dpde) {
-            const double dm = 0.5 * (L(DD, 0, 0) + L(DD, di, dj));
-            const double uf = 0.5 * (L(UH, 0, 0) + L(UH, di, dj)) -
-                              0.5 * dm * ((L(XX, 0, 0) + L(XX, di, dj)) * dpdxi + (L(EX, 0, 0) + L(EX, di, dj)) * dpde);
-            const double vf = 0.5 * (L(VH, 0, 0) + L(VH, di, dj)) -
-                              0.5 * dm * ((L(XY, 0, 0) + L(XY, di, dj)) * dpdxi + (L(EY, 0, 0) + L(EY, di, dj)) * dpde);
-            const int mx = di != 0 ? XX : EX, my = di != 0 ? XY : EY;
-            return 0.5 * (L(JJ, 0, 0) * L(mx, 0, 0) + L(JJ, di, dj) * L(mx, di, dj)) * uf +
-                   0.5 * (L(JJ, 0, 0) * L(my, 0, 0) + L(JJ, di, dj) * L(my, di, dj)) * vf;
-        };
         auto P_ = [&](int di, int dj) { return s_p[(lj + 2 + dj) * PX + (li + 2 + di)]; };
-        double fe = face(1, 0, (P_(1, 0) - P_(0, 0)) * g.r.ih1,
-                         (P_(1, 1) + P_(0, 1) - P_(0, -1) - P_(1, -1)) * (0.25 * g.r.ih2));
-        double fw = face(-1, 0, (P_(0, 0) - P_(-1, 0)) * g.r.ih1,
-                         (P_(0, 1) + P_(-1, 1) - P_(0, -1) - P_(-1, -1)) * (0.25 * g.r.ih2));
-        double gn = face(0, 1, (P_(1, 1) - P_(-1, 1) + P_(1, 0) - P_(-1, 0)) * (0.25 * g.r.ih1),
-                         (P_(0, 1) - P_(0, 0)) * g.r.ih2);
-        double gs = face(0, -1, (P_(1, 0) - P_(-1, 0) + P_(1, -1) - P_(-1, -1)) * (0.25 * g.r.ih1),
-                         (P_(0, 0) - P_(0, -1)) * g.r.ih2);
-        fe = i + 1 != n0 - 1 ? fe : 0.0;
-        fw = i - 1 != 0 ? fw : 0.0;
-        gn = j + 1 != n1 - 1 ? gn : 0.0;
-        gs = j - 1 != 0 ? gs : 0.0;
-        double qq = (fe - fw) * g.r.ih1 + (gn - gs) * g.r.ih2;
+        double qq = face_div_node(g, i, j, L, P_);
         if (sm) qq -= sm[o + idx(i, j, n0)];
         q[idx(i, j, n0)] = qq;
         aq[0] = fabs(qq);
@@ -658,6 +697,17 @@ __global__ __launch_bounds__(NT) void k_correct(
     if (tid == 0) rd.part[blin()] = dm;
 }
 
+// New temperature of side-wall node (0, j) (east: (n0 - 1, j)) from the old values
+// T(i, j) and the node's alpha, beta (oracle: temperature_sides, without its source)
+template <typename TA>
+__device__ __forceinline__ double t_side_value(const KGrid &g, TA T, double alph, double beta, bool east, int j) {
+    const int n0 = g.n0, i = east ? n0 - 1 : 0;
+    const double te = (T(i, j + 1) - T(i, j - 1)) / (2.0 * g.h2);
+    const double txi = beta / alph * te;
+    return east ? (4.0 * T(n0 - 2, j) - T(n0 - 3, j) + 2.0 * g.h1 * txi) / 3.0
+                : (4.0 * T(1, j) - T(2, j) - 2.0 * g.h1 * txi) / 3.0;
+}
+
 // Adiabatic side walls spread over the chip (the column-strided wall accesses of
 // one block would be bound by that CU's outstanding misses): the new wall values,
 // computed from the old ones, go to scratch; k_p_walls uses and writes them.
@@ -676,14 +726,9 @@ __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restric
     const int n0 = g.n0, n1 = g.n1;
     const int j = 1 + (int)(blockIdx.x * WB + threadIdx.x);
     if (j > n1 - 2) return;
-    size_t P = idx(0, j, n0);
-    double te = (T[idx(0, j + 1, n0)] - T[idx(0, j - 1, n0)]) / (2.0 * g.h2);
-    double txi = beta[P] / alph[P] * te;
-    scratch[j] = (4.0 * T[idx(1, j, n0)] - T[idx(2, j, n0)] - 2.0 * g.h1 * txi) / 3.0;
-    P = idx(n0 - 1, j, n0);
-    te = (T[idx(n0 - 1, j + 1, n0)] - T[idx(n0 - 1, j - 1, n0)]) / (2.0 * g.h2);
-    txi = beta[P] / alph[P] * te;
-    scratch[n1 + j] = (4.0 * T[idx(n0 - 2, j, n0)] - T[idx(n0 - 3, j, n0)] + 2.0 * g.h1 * txi) / 3.0;
+    auto Tat = [&](int ii, int jj) { return T[idx(ii, jj, n0)]; };
+    scratch[j] = t_side_value(g.kg(), Tat, alph[idx(0, j, n0)], beta[idx(0, j, n0)], false, j);
+    scratch[n1 + j] = t_side_value(g.kg(), Tat, alph[idx(n0 - 1, j, n0)], beta[idx(n0 - 1, j, n0)], true, j);
     if (sT) {  // FAS boundary sources of a coarse level
         scratch[j] += sT[idx(0, j, n0)];
         scratch[n1 + j] += sT[idx(n0 - 1, j, n0)];
@@ -833,6 +878,417 @@ __global__ void k_uvt_walls(CGrid g, double *__restrict__ u, double *__restrict_
         v[idx(0, t, n0)] = 0.0;
         u[idx(n0 - 1, t, n0)] = 0.0;
         v[idx(n0 - 1, t, n0)] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SIMPLE iterations of a small FAS level in one workgroup (oracle: or_cav_outer
+// without its statistics).  A coarse level of at most 33 x 33 nodes runs its
+// iterations (the coarsest level fas_coarse of them per cycle) at a few
+// microseconds of work per stage, which the multi-kernel iteration spreads over
+// ~50 launches.  Here one 1024-thread workgroup per slot runs `iters` iterations
+// with block barriers between the stages: the level's state, d, the predictor
+// corrections and u-hat live in LDS; each thread owns one interior node and keeps
+// its row of every p' level in registers (as k_mg_coarse does) and its p' solve
+// vectors; the BiCGStab p' solve is preconditioned by the same single-workgroup
+// V-cycle (cw_vcycle).  FP64 in both precision modes (a coarse level steers the
+// fine level's iterates and sets no result of its own).  The per-node arithmetic
+// is the stage kernels' (residual_node, jac_node, face_div_node, t_side_value,
+// wall_p_value, p9_face_of).
+constexpr int SL_NT = 1024;
+constexpr int SL_NODES = 1089;  // at most 33 x 33 nodes (and at most SL_NT interior ones)
+enum { SL_U, SL_V, SL_T, SL_P, SL_D, SL_A0, SL_A1, SL_A2, SL_B0, SL_B1, SL_B2, SL_NARR };
+
+struct SmallArgs {
+    CGrid g;
+    GridPtrs gp;
+    const double *jac;
+    double *u, *v, *T, *p;
+    Src src;
+    int iters, sweeps, p_maxit;
+    double p_rtol;
+    CoarseArgs<double> mg;  // the p' levels' shapes, face tables, V-cycle sweeps and LDS offsets
+    Geom lev[CW_LEVS];      // the p' levels' metrics (level 0 is this level)
+};
+
+// totals of NV values over the workgroup, in every thread
+template <int NV>
+__device__ __forceinline__ void sl_sums(double (&v)[NV]) {
+    __shared__ double red[NV][SL_NT / 64];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k][threadIdx.x >> 6] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double t = 0.0;
+        for (int q = 0; q < SL_NT / 64; q++) t += red[k][q];
+        v[k] = t;
+    }
+    __syncthreads();
+}
+
+// border node b (0 <= b < 2 (n0 + n1) - 4) of an n0 x n1 grid, as a node index
+__device__ __forceinline__ int sl_border(int b, int n0, int n1) {
+    if (b < n0) return b;                          // j = 0
+    b -= n0;
+    if (b < n0) return (n1 - 1) * n0 + b;          // j = n1 - 1
+    b -= n0;
+    if (b < n1 - 2) return (b + 1) * n0;           // i = 0
+    return (b - (n1 - 2) + 1) * n0 + n0 - 1;       // i = n0 - 1
+}
+
+// LDS of k_simple_small: the level's arrays, and the V-cycle's iterates
+__shared__ double sl_sa[SL_NARR * SL_NODES];
+__shared__ double sl_mg[CN_MAX * CW_ARR];
+__device__ __forceinline__ double *sl_arr(int k) { return sl_sa + k * SL_NODES; }
+
+// a thread's node: interior node (i, j) = index P when own
+struct SlNode {
+    int i, j, P;
+    bool own;
+};
+
+// Stages 1-4 of an iteration: residuals and d at the own node (d into SL_D), the
+// predictor's Jacobi sweeps (ping-pong between the A and B arrays, wall entries
+// zero), u* = u + du into SL_U / SL_V (no stage before the correction reads the
+// old u at a neighbour), T += dT, and u-hat, v-hat into SL_A0 / SL_A1.  Ends with
+// a block barrier.
+__device__ __forceinline__ void sl_predict(const SmallArgs *__restrict__ a, SlNode t) {
+    const CGrid &g = a->g;
+    const SlotP ph = g.sp[blockIdx.z];
+    const int n0 = g.n0, n1 = g.n1, P = t.P, tid = threadIdx.x;
+    const size_t o = zslot() * g.N;
+    double *const U_ = sl_arr(SL_U), *const V_ = sl_arr(SL_V), *const T_ = sl_arr(SL_T), *const P_ = sl_arr(SL_P);
+    double *const D_ = sl_arr(SL_D);
+    const Src src = src_at(a->src, o);
+    const GridPtrs &gq = a->gp;
+    const size_t G = o + P;
+    const NC c{gq.xix[G], gq.xiy[G], gq.etx[G], gq.ety[G], gq.alph[G], gq.beta[G], gq.gamma[G], gq.p1[G], gq.q1[G]};
+    ResOut ro{};
+    if (t.own) {
+        auto at = [&](const double *f, int di, int dj) { return f[P + dj * n0 + di]; };
+        auto pat = [&](int di, int dj) { return P_[P + dj * n0 + di]; };
+        ro = residual_node(g, ph, c, t.i, t.j, P, U_, V_, T_, at, pat, src);
+        D_[P] = ro.d;
+    }
+    double *da[3] = {sl_arr(SL_A0), sl_arr(SL_A1), sl_arr(SL_A2)};
+    double *db[3] = {sl_arr(SL_B0), sl_arr(SL_B1), sl_arr(SL_B2)};
+    for (int b = tid; b < 2 * (n0 + n1) - 4; b += SL_NT) {
+        const int k = sl_border(b, n0, n1);
+#pragma unroll
+        for (int f = 0; f < 3; f++) da[f][k] = db[f][k] = 0.0;
+    }
+    const int sweeps = a->sweeps;
+    if (sweeps <= 0 && t.own) da[0][P] = da[1][P] = da[2][P] = 0.0;
+    const JNode<double> jn{c.al, c.be, c.ga, c.pp, c.qq, ro.U, ro.V, ro.ru, ro.rv, ro.rt};
+    for (int sw = 0; sw < sweeps; sw++) {
+        if (t.own) {
+            auto dat = [&](int f, int ii, int jj) { return da[f][jj * n0 + ii]; };
+            double ou, ov, ot;
+            if (sw == 0) jac_node<double, true>(jn, ph, g.r, t.i, t.j, dat, ou, ov, ot);
+            else jac_node<double, false>(jn, ph, g.r, t.i, t.j, dat, ou, ov, ot);
+            db[0][P] = ou;
+            db[1][P] = ov;
+            db[2][P] = ot;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            double *s = da[f];
+            da[f] = db[f];
+            db[f] = s;
+        }
+    }
+    if (t.own) {
+        const double us = U_[P] + da[0][P], vs = V_[P] + da[1][P];
+        T_[P] = T_[P] + da[2][P];
+        U_[P] = us;
+        V_[P] = vs;
+        const double dp_dxi = (P_[P + 1] - P_[P - 1]) * g.r.i2h1;
+        const double dp_de = (P_[P + n0] - P_[P - n0]) * g.r.i2h2;
+        sl_arr(SL_A0)[P] = us + ro.d * (c.xix * dp_dxi + c.etx * dp_de);
+        sl_arr(SL_A1)[P] = vs + ro.d * (c.xiy * dp_dxi + c.ety * dp_de);
+    }
+    __syncthreads();
+}
+
+// Stage 5: the face divergence at the own node (with the level's mass source) into
+// SL_B0
+__device__ __forceinline__ void sl_face_div(const SmallArgs *__restrict__ a, SlNode t) {
+    if (!t.own) return;
+    const CGrid &g = a->g;
+    const int n0 = g.n0, P = t.P;
+    const size_t o = zslot() * g.N;
+    const GridPtrs &gq = a->gp;
+    const double *const uh = sl_arr(SL_A0), *const vh = sl_arr(SL_A1), *const D_ = sl_arr(SL_D);
+    const double *const P_ = sl_arr(SL_P);
+    auto L = [&](int k, int di, int dj) -> double {
+        const int Q = P + dj * n0 + di;
+        switch (k) {
+        case FD_UH: return uh[Q];
+        case FD_VH: return vh[Q];
+        case FD_D: return D_[Q];
+        case FD_X: return gq.xix[o + Q];
+        case FD_Y: return gq.xiy[o + Q];
+        case FD_E: return gq.etx[o + Q];
+        case FD_F: return gq.ety[o + Q];
+        default: return a->jac[o + Q];
+        }
+    };
+    auto pq = [&](int di, int dj) { return P_[P + dj * n0 + di]; };
+    double qv = face_div_node(g, t.i, t.j, L, pq);
+    if (a->src.m) qv -= a->src.m[o + P];
+    sl_arr(SL_B0)[P] = qv;
+}
+
+// Stage 6a: the face coefficients and inverse diagonals of every p' level from the
+// level's metrics and d injected from this level (k_faces_all's arithmetic), into
+// the level tables (global memory).  Ends with a block barrier.
+__device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) {
+    const double *const D_ = sl_arr(SL_D);
+    const int n0 = a->g.n0, last = a->mg.nlev - 1;
+#pragma unroll
+    for (int L = 0; L < CW_LEVS; L++) {
+        if (L > last) break;
+        const Geom &gl = a->lev[L];
+        const CLev<double> &cl = a->mg.L[L];
+        const int ln0 = gl.n0, ln1 = gl.n1;
+        const size_t ol = zslot() * (size_t)ln0 * ln1;
+        const double r1 = 1.0 / (gl.h1 * gl.h1), r2 = 1.0 / (gl.h2 * gl.h2), r12 = 1.0 / (4.0 * gl.h1 * gl.h2);
+        for (int k = threadIdx.x; k < ln0 * ln1; k += SL_NT) {
+            const int li = k % ln0, lj = k / ln0;
+            auto fn = [&](int ii, int jj) {
+                const int ic = clampi(ii, 0, ln0 - 1), jc = clampi(jj, 0, ln1 - 1);
+                const size_t Q = ol + (size_t)jc * ln0 + ic;
+                return FNode{gl.xix[Q], gl.xiy[Q], gl.etx[Q], gl.ety[Q], gl.jac[Q], D_[(jc << L) * n0 + (ic << L)]};
+            };
+            const FNode nP = fn(li, lj), nE = fn(li + 1, lj), nW = fn(li - 1, lj), nN = fn(li, lj + 1),
+                        nS = fn(li, lj - 1);
+            double en, et, nn, nt, wn, wt, sn, st;
+            p9_face_of(nP, nE, 0, r1, r2, r12, en, et);
+            p9_face_of(nP, nN, 1, r1, r2, r12, nn, nt);
+            p9_face_of(nW, nP, 0, r1, r2, r12, wn, wt);
+            p9_face_of(nS, nP, 1, r1, r2, r12, sn, st);
+            auto ev = [&](int ii, int jj) { return ii >= 1 && ii <= ln0 - 3 && jj >= 1 && jj <= ln1 - 2; };
+            auto nv = [&](int ii, int jj) { return jj >= 1 && jj <= ln1 - 3 && ii >= 1 && ii <= ln0 - 2; };
+            if (!ev(li, lj)) en = et = 0.0;
+            if (!nv(li, lj)) nn = nt = 0.0;
+            if (!ev(li - 1, lj)) wn = wt = 0.0;
+            if (!nv(li, lj - 1)) sn = st = 0.0;
+            const_cast<double *>(cl.fen)[ol + k] = en;
+            const_cast<double *>(cl.fet)[ol + k] = et;
+            const_cast<double *>(cl.fnn)[ol + k] = nn;
+            const_cast<double *>(cl.fnt)[ol + k] = nt;
+            if (!inner_node(li, lj, ln0, ln1)) continue;
+            double cr[9];
+            p9_row_from<double>(RowFaces<double>{en, et, wn, wt, nn, nt, sn, st}, cr);
+            double dg = 0.0;
+            for (int dj = -1; dj <= 1; dj++)
+                for (int di = -1; di <= 1; di++)
+                    if (clampi(li + di, 1, ln0 - 2) == li && clampi(lj + dj, 1, ln1 - 2) == lj)
+                        dg += cr[(dj + 1) * 3 + (di + 1)];
+            const_cast<double *>(cl.invd)[ol + k] = 1.0 / dg;
+        }
+    }
+    __syncthreads();
+}
+
+// Stage 6b: L_d p' = q by BiCGStab, right-preconditioned by the single-workgroup
+// V-cycle, from p' = 0 (oracle: bicgstab), with the rows of the own node of every
+// level in registers (as k_mg_coarse holds them) and the vectors at the own node in
+// the free work arrays; p' with its interior mean removed goes to SL_A1.  Ends with
+// a block barrier.
+__device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNode t) {
+    const int n0 = a->g.n0, P = t.P;
+    const bool own = t.own;
+    const double inv_m = 1.0 / ((double)(n0 - 2) * (double)(a->g.n1 - 2));
+    CNode<double> nd[CW_LEVS];
+    {
+        RowFaces<double> rf[CW_LEVS];
+        cw_row_loads<double>(a->mg, 0, nd, rf);
+        cw_row_fold<double>(a->mg, 0, nd, rf);
+    }
+    double *const R = sl_arr(SL_B0), *const R0 = sl_arr(SL_B1), *const PV = sl_arr(SL_B2);
+    double *const VV = sl_arr(SL_A2), *const X = sl_arr(SL_A0);
+    const double bq = own ? R[P] : 0.0;  // the face divergence
+    double ms[1] = {bq};
+    sl_sums<1>(ms);
+    const double r = own ? bq - ms[0] * inv_m : 0.0;
+    if (own) {
+        R[P] = r;
+        R0[P] = r;
+        X[P] = 0.0;
+    }
+    double nb[2] = {bq * bq, r * r};
+    sl_sums<2>(nb);
+    const double bnorm = sqrt(nb[0]), tol = a->p_rtol * bnorm;
+    if (bnorm != 0.0 && sqrt(nb[1]) > tol) {
+        const double *z = sl_mg + a->mg.off[0];  // the V-cycle's result on this level
+        const int maxit = a->p_maxit;
+        double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+        for (int it = 1; it <= maxit; it++) {
+            double t1[1] = {own ? R0[P] * R[P] : 0.0};
+            sl_sums<1>(t1);
+            const double rho = t1[0];
+            if (rho == 0.0) break;
+            double pv = 0.0;
+            if (own) {
+                pv = it == 1 ? R[P] : R[P] + ((rho / rho_old) * (alpha / omega)) * (PV[P] - omega * VV[P]);
+                PV[P] = pv;
+            }
+            nd[0].b = pv;
+            cw_vcycle<double>(a->mg, 0, nd, sl_mg);
+            const double phv = own ? z[P] : 0.0;
+            const double vv = own ? cw_row<double>(nd[0], z, n0) : 0.0;
+            if (own) VV[P] = vv;
+            double t2[1] = {own ? R0[P] * vv : 0.0};
+            sl_sums<1>(t2);
+            alpha = rho / t2[0];
+            const double sv = own ? R[P] - alpha * vv : 0.0;
+            double t3[1] = {sv * sv};
+            sl_sums<1>(t3);
+            if (sqrt(t3[0]) <= tol) {
+                if (own) X[P] = X[P] + alpha * phv;
+                break;
+            }
+            nd[0].b = sv;
+            cw_vcycle<double>(a->mg, 0, nd, sl_mg);
+            const double shv = own ? z[P] : 0.0;
+            const double tv = own ? cw_row<double>(nd[0], z, n0) : 0.0;
+            double t4[2] = {tv * tv, tv * sv};
+            sl_sums<2>(t4);
+            omega = t4[0] > 0.0 ? t4[1] / t4[0] : 0.0;
+            const double rn = sv - omega * tv;
+            if (own) {
+                X[P] = X[P] + (alpha * phv + omega * shv);
+                R[P] = rn;
+            }
+            double t5[1] = {rn * rn};
+            sl_sums<1>(t5);
+            if (sqrt(t5[0]) <= tol || omega == 0.0) break;
+            rho_old = rho;
+        }
+    }
+    const double xv = own ? X[P] : 0.0;
+    double xs[1] = {xv};
+    sl_sums<1>(xs);
+    if (own) sl_arr(SL_A1)[P] = xv - xs[0] * inv_m;
+    __syncthreads();
+}
+
+// Stages 7-9: correction u = u* - d grad p' (p' read clamped into the interior:
+// the wall copies), p += alpha_p p'; the adiabatic side walls (new values from the
+// old ones); the wall pressure (reads p at interior nodes only), the corners from
+// the new wall values (kept without sources in SL_B0), then the sources.  Ends
+// with a block barrier.
+__device__ __forceinline__ void sl_correct_walls(const SmallArgs *__restrict__ a, SlNode t) {
+    const CGrid &g = a->g;
+    const SlotP *const ph = g.sp + blockIdx.z;
+    const int n0 = g.n0, n1 = g.n1, m0 = n0 - 2, m1 = n1 - 2, P = t.P, tid = threadIdx.x;
+    const size_t o = zslot() * g.N;
+    double *const U_ = sl_arr(SL_U), *const V_ = sl_arr(SL_V), *const T_ = sl_arr(SL_T), *const P_ = sl_arr(SL_P);
+    const double *const D_ = sl_arr(SL_D), *const PC = sl_arr(SL_A1);
+    const GridPtrs gp{a->gp.xix + o, a->gp.xiy + o, a->gp.etx + o, a->gp.ety + o, a->gp.alph + o,
+                      a->gp.beta + o, a->gp.gamma + o, a->gp.p1 + o, a->gp.q1 + o};
+    const Src src = src_at(a->src, o);
+    if (t.own) {
+        const int i = t.i, j = t.j;
+        const double pe = PC[clampi(i + 1, 1, n0 - 2) + j * n0], pw = PC[clampi(i - 1, 1, n0 - 2) + j * n0];
+        const double pn = PC[i + clampi(j + 1, 1, n1 - 2) * n0], ps = PC[i + clampi(j - 1, 1, n1 - 2) * n0];
+        const double dpx = 0.5 * (pe - pw) * g.r.ih1;
+        const double dpe = 0.5 * (pn - ps) * g.r.ih2;
+        const double dP = D_[P];
+        U_[P] = U_[P] - dP * (gp.xix[P] * dpx + gp.etx[P] * dpe);
+        V_[P] = V_[P] - dP * (gp.xiy[P] * dpx + gp.ety[P] * dpe);
+        P_[P] = P_[P] + ph->alpha_p * PC[P];
+    }
+    __syncthreads();
+    const KGrid kg = g.kg();
+    double tw = 0.0;
+    int Pt = -1;
+    if (tid < 2 * m1) {
+        const bool east = tid >= m1;
+        const int jw = 1 + (east ? tid - m1 : tid);
+        Pt = jw * n0 + (east ? n0 - 1 : 0);
+        auto Tat = [&](int ii, int jj) { return T_[jj * n0 + ii]; };
+        tw = t_side_value(kg, Tat, gp.alph[Pt], gp.beta[Pt], east, jw);
+        if (src.T) tw += src.T[Pt];
+    }
+    __syncthreads();
+    if (Pt >= 0) T_[Pt] = tw;
+    __syncthreads();
+    double *const PW = sl_arr(SL_B0);
+    if (tid < 2 * (m0 + m1)) {
+        int side, iw, jw;
+        if (tid < 2 * m0) {
+            side = tid < m0 ? 0 : 1;
+            iw = 1 + (tid < m0 ? tid : tid - m0);
+            jw = side == 0 ? 0 : n1 - 1;
+        } else {
+            const int q = tid - 2 * m0;
+            side = q < m1 ? 2 : 3;
+            jw = 1 + (q < m1 ? q : q - m1);
+            iw = side == 2 ? 0 : n0 - 1;
+        }
+        const WallIn w{U_, V_, T_, P_, gp.xix, gp.xiy, gp.etx, gp.ety, gp.alph, gp.beta, gp.gamma,
+                       gp.p1, gp.q1, a->jac + o};
+        const int Pw = jw * n0 + iw;
+        const double pw = wall_p_value(w, kg, ph->Re, ph->Ri, side, iw, jw, T_[Pw]);
+        PW[Pw] = pw;
+        P_[Pw] = pw + (src.m ? src.m[Pw] : 0.0);
+    }
+    __syncthreads();
+    if (tid < 4) {
+        const bool east = tid & 1, north = tid & 2;
+        const int ci = east ? n0 - 1 : 0, cj = north ? n1 - 1 : 0;
+        const int C = cj * n0 + ci;
+        const double v1 = PW[cj * n0 + (east ? n0 - 2 : 1)], v2 = PW[(north ? n1 - 2 : 1) * n0 + ci];
+        P_[C] = 0.5 * (v1 + v2) + (src.m ? src.m[C] : 0.0);
+    }
+    __syncthreads();
+}
+
+// a: the level's arguments in device memory (scalar loads, no per-lane copy)
+__global__ __launch_bounds__(SL_NT) void k_simple_small(const SmallArgs *__restrict__ a, int iters) {
+    if (a->g.sp[blockIdx.z].active == 0.0) return;
+    const int n0 = a->g.n0, n1 = a->g.n1, N = n0 * n1;
+    const size_t o = zslot() * a->g.N;
+    const int tid = threadIdx.x;
+    const int m0 = n0 - 2, m1 = n1 - 2;
+    SlNode t;
+    t.own = tid < m0 * m1;
+    const int qn = t.own ? tid : 0;
+    t.i = 1 + qn % m0;
+    t.j = 1 + qn / m0;
+    t.P = t.j * n0 + t.i;
+    for (int k = tid; k < N; k += SL_NT) {
+        sl_arr(SL_U)[k] = a->u[o + k];
+        sl_arr(SL_V)[k] = a->v[o + k];
+        sl_arr(SL_T)[k] = a->T[o + k];
+        sl_arr(SL_P)[k] = a->p[o + k];
+    }
+    for (int k = tid; k < CN_MAX * CW_ARR; k += SL_NT) sl_mg[k] = 0.0;  // wall entries stay zero
+    __syncthreads();
+    // The stages run as the cases of one loop: a stage's values cannot stay live into
+    // the next one, so the register demand is the largest stage's, not their sum.
+    for (int k = 0; k < 5 * iters; k++) {
+        switch (k % 5) {
+        case 0: sl_predict(a, t); break;
+        case 1: sl_face_div(a, t); break;  // the face tables' barrier orders it before the solve
+        case 2: sl_face_tables(a); break;
+        case 3: sl_pprime(a, t); break;
+        default: sl_correct_walls(a, t); break;
+        }
+    }
+    for (int k = tid; k < N; k += SL_NT) {
+        a->u[o + k] = sl_arr(SL_U)[k];
+        a->v[o + k] = sl_arr(SL_V)[k];
+        a->T[o + k] = sl_arr(SL_T)[k];
+        a->p[o + k] = sl_arr(SL_P)[k];
     }
 }
 
@@ -1738,16 +2194,81 @@ int wall_p_out(Case *c, bool with_src) {
     return CFD_OK;
 }
 
+// A coarse FAS level that k_simple_small takes: at most SL_NODES nodes and SL_NT
+// interior ones, at most CW_LEVS p' levels whose V-cycle arrays fit its LDS
+bool small_fits(const Case *c) {
+    if (!c->prm.fas_small || c->N > (size_t)SL_NODES || (c->n0 - 2) * (c->n1 - 2) > SL_NT) return false;
+    if (c->nlev > CW_LEVS || c->prm.mg_pre > 2 || c->prm.mg_post > 2) return false;
+    int tot = 0;
+    for (int l = 0; l < c->nlev; l++) tot += c->geom[l].n0 * c->geom[l].n1;
+    return tot <= CN_MAX;
+}
+
+// The face tables of a level that k_simple_small takes, and its arguments in device
+// memory (written once here: everything in them is fixed when the level is created;
+// the level's FAS sources are always applied, as on every coarse level)
+int alloc_small(Case *c) {
+    if (!small_fits(c)) return CFD_OK;
+    for (int l = 0; l < c->nlev; l++)
+        for (int k = 0; k < 5; k++) RET(dalloc(c, &c->sl_tab[l][k], (size_t)c->B * c->geom[l].n0 * c->geom[l].n1));
+    RET(dalloc(c, &c->sl_args, 1));
+    if (c->dry) return CFD_OK;
+    SmallArgs a{};
+    a.g = cg(c);
+    a.gp = gptrs(c);
+    a.jac = c->ajac;
+    a.u = c->u; a.v = c->v; a.T = c->T; a.p = c->p;
+    a.src = Src{c->src[0], c->src[1], c->src[2], c->src[3]};
+    a.sweeps = c->prm.mom_sweeps;
+    a.p_maxit = c->prm.p_maxit;
+    a.p_rtol = c->prm.p_rtol;
+    a.mg.nlev = c->nlev;
+    a.mg.pre = c->prm.mg_pre;
+    a.mg.post = c->prm.mg_post;
+    a.mg.coarse = c->prm.mg_coarse;
+    a.mg.omega = c->prm.mg_omega;
+    int off = 0;
+    for (int l = 0; l < c->nlev; l++) {
+        double *const *t = c->sl_tab[l];
+        a.mg.L[l] = CLev<double>{c->geom[l].n0, c->geom[l].n1, t[0], t[1], t[2], t[3], t[4]};
+        a.mg.off[l] = off;
+        off += CW_ARR * c->geom[l].n0 * c->geom[l].n1;
+    }
+    for (int l = 0; l < CW_LEVS; l++) a.lev[l] = c->geom[std::min(l, c->nlev - 1)];
+    CFD_HIP(hipMemcpyAsync(c->sl_args, &a, sizeof(a), hipMemcpyHostToDevice, c->s));
+    CFD_HIP(hipStreamSynchronize(c->s));
+    return CFD_OK;
+}
+
+// `iters` SIMPLE iterations of every slot of a small coarse level in one launch
+int simple_small(Case *c, int iters) {
+    if (iters <= 0) return CFD_OK;
+    if (!c->sl_args || !c->use_src) return CFD_E_ARG;
+    k_simple_small<<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters);
+    CFD_LAUNCH_CHECK();
+    return CFD_OK;
+}
+
+// `iters` SIMPLE iterations of a level below the top of a FAS hierarchy
+int coarse_iters(Case *c, bool graph, int iters) {
+    if (small_fits(c)) return simple_small(c, iters);
+    for (int k = 0; k < iters; k++) RET(simple_body(c, graph, false));
+    return CFD_OK;
+}
+
 int fas_cycle(Case *c, bool graph, bool top) {
     const cfd_params &pr = c->prm;
     Case *cc = c->coarse;
     // coarse levels control their p' solves on the device (no host round trips)
     const bool g = graph || !top;
     if (!cc) {
+        if (!top) return coarse_iters(c, g, pr.fas_coarse);
         for (int k = 0; k < pr.fas_coarse; k++) RET(simple_body(c, g, top));
         return CFD_OK;
     }
-    for (int k = 0; k < pr.fas_pre; k++) RET(simple_body(c, g, top));
+    if (!top) RET(coarse_iters(c, g, pr.fas_pre));
+    else
+        for (int k = 0; k < pr.fas_pre; k++) RET(simple_body(c, g, top));
     // this level's residuals at its state, with its own sources
     RET(state_residual(c, true));
     RET(side_walls(c, true));
@@ -1787,6 +2308,7 @@ int fas_cycle(Case *c, bool graph, bool top) {
         CFD_LAUNCH_CHECK();
     }
     RET(wall_bcs(c, 0, c->B, true));
+    if (!top) return coarse_iters(c, g, pr.fas_post);
     for (int k = 0; k < pr.fas_post; k++) RET(simple_body(c, g, top));
     return CFD_OK;
 }
@@ -1944,13 +2466,17 @@ void batch_destroy(Case *c) {
 }
 
 // Smallest level of a FAS hierarchy (nodes per side): fas_min_n, or with
-// fas_min_n <= 0 the smallest 2^k + 1 >= 33 whose cell Reynolds number Re h = Re /
-// 2^k is at most 32.  A 17 x 17 level misleads the cycle (a fixed point away from
-// the solution) for buoyant cases from Re ~ 300 on and at Re 600 without buoyancy;
-// 33 x 33 converges every steady case of the configurations up to Re 1000.
+// fas_min_n <= 0 the smallest 2^k + 1 >= 17 whose cell Reynolds number Re / 2^k is at
+// most 32, and >= 33 when Pr > 1 or the Rayleigh number Ra = |Ri| Re^2 Pr exceeds
+// 2e5.  A 17 x 17 level misleads the cycle (it stalls away from the solution) for
+// thin thermal layers and strong buoyancy: over 40 drawn mixed-convection cases at
+// 129 x 129 (Re <= 512), every one with Pr 7 or Ra > 2e5 that converged with a
+// 33 x 33 coarsest level stalled with 17 x 17, and every other one converged with
+// 17 x 17 in fewer cycles.
 int fas_min_eff(const cfd_params &p) {
     if (p.fas_min_n > 0) return p.fas_min_n < 5 ? 5 : p.fas_min_n;
-    int m = 32;
+    const double ra = std::fabs(p.Ri) * p.Re * p.Re * p.Pr;
+    int m = (p.Pr <= 1.0 && ra <= 2.0e5) ? 16 : 32;
     while (32.0 * m < p.Re && m < (1 << 20)) m *= 2;
     return m + 1;
 }
@@ -2027,6 +2553,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     if (cp.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = cp.fas_coarse_scheme;
     if (dry) {
         if (rc == CFD_OK) rc = build_levels(c);
+        if (rc == CFD_OK && is_coarse) rc = alloc_small(c);
         if (rc == CFD_OK && coarsens) {
             rc = batch_create((n0 + 1) / 2, (n1 + 1) / 2, 2 * dxi, 2 * deta, slots, &cp, &c->coarse, true, nullptr,
                               true);
@@ -2045,6 +2572,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
         rc = CFD_E_OOM;
     if (rc == CFD_OK) std::memset(c->hflag, 0, (size_t)slots * sizeof(unsigned long long));
     if (rc == CFD_OK) rc = build_levels(c);
+    if (rc == CFD_OK && is_coarse) rc = alloc_small(c);
     if (rc == CFD_OK && coarsens) {
         rc = batch_create((n0 + 1) / 2, (n1 + 1) / 2, 2 * dxi, 2 * deta, slots, &cp, &c->coarse, false, c->s, true);
         if (c->coarse) c->bytes += c->coarse->bytes;
@@ -2106,10 +2634,12 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
         return CFD_E_ARG;
     const int n0 = c->n0, n1 = c->n1;
     const size_t N = c->N, B = N * sizeof(double), o = (size_t)z * N;
-    // a hierarchy built for a lower Reynolds number has a level too coarse for this case
+    // a hierarchy built for other physics may have a level too coarse for this case
     if (c->coarse && c->prm.fas_min_n <= 0) {
         cfd_params q = c->prm;
         q.Re = phys->Re;
+        q.Ri = phys->Ri;
+        q.Pr = phys->Pr;
         const Case *l = c;
         while (l->coarse) l = l->coarse;
         if (std::min(l->n0, l->n1) < fas_min_eff(q)) return CFD_E_ARG;
@@ -2181,6 +2711,7 @@ void cfd_params_default(cfd_params *p) {
     p->wall_upwind = 1; p->hyb_blend = 1.0;
     p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 0;
     p->fas_coarse_scheme = 2;
+    p->fas_small = 1;
 }
 
 int cfd_fas_coarsest(int n0, int n1, const cfd_params *p) {

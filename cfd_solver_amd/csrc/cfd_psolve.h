@@ -873,34 +873,16 @@ __device__ __forceinline__ T cw_row(const CNode<T> &c, const T *x, int n0) {
     return s;
 }
 
-// b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
-// whose mean is removed; null: used as is); the level-l0 result is written to x0out.
+// The owned interior node of every level l0 + L of a single-workgroup V-cycle: its
+// index, the loads of its row faces and 1/diag (slot blockIdx.z of the level
+// arrays in A), b = 0; levels past the last reload the last one and own no node.
 template <typename T>
-__global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, CoarseArgs<T> A, int l0, const T *b0,
-                                                  const double *b0part, int nb0p, size_t lps, T *x0out) {
-    if (slot_skip(scal + zslot() * SS, gs)) return;
-    {
-        const size_t o0 = zslot() * A.L[l0].n0 * A.L[l0].n1;
-        b0 += o0;
-        x0out += o0;
-        if (b0part) b0part += zslot() * lps;
-    }
-    __shared__ T sh[CN_MAX * CW_ARR];
-    const int tid = threadIdx.x;
-    const T w = A.omega;
-    const int last = A.nlev - 1;
-    const int nl = last - l0 + 1;
-    auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
-    auto XA = [&](int L) { return sh + A.off[l0 + L]; };             // iterate of level l0 + L
-    auto TA = [&](int L) { return sh + A.off[l0 + L] + NN(l0 + L); }; // its ping-pong buffer
-    auto INNER = [&](int L) { return (A.L[l0 + L].n0 - 2) * (A.L[l0 + L].n1 - 2); };
-
-    // set-up loads: row faces and 1/diag of the owned node of every level
-    CNode<T> nd[CW_LEVS];
-    RowFaces<T> rf[CW_LEVS];
+__device__ __forceinline__ void cw_row_loads(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS],
+                                             RowFaces<T> (&rf)[CW_LEVS]) {
+    const int tid = threadIdx.x, last = A.nlev - 1;
 #pragma unroll
     for (int L = 0; L < CW_LEVS; L++) {
-        const int l = min(l0 + L, last);  // levels past the last reload it, unused
+        const int l = min(l0 + L, last);
         const CLev<T> &cl = A.L[l];
         const int m0 = cl.n0 - 2, m1 = cl.n1 - 2;
         const int q = min(tid, m0 * m1 - 1);
@@ -912,25 +894,33 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
         nd[L].id = cl.invd[o + (size_t)j * cl.n0 + i];
         nd[L].b = T(0);
     }
-    const T braw = b0[nd[0].j() * A.L[l0].n0 + nd[0].i()];
-    double part = 0.0;
-    if (b0part)
-        for (int t = tid; t < nb0p; t += CW) part += b0part[t];
-    {
-        int tot = 0;
-        for (int l = l0; l <= last; l++) tot += CW_ARR * NN(l);
-        for (int k = tid; k < tot; k += CW) sh[k] = T(0);
-    }
+}
+// the rows from the loaded faces, Neumann ghosts folded in
+template <typename T>
+__device__ __forceinline__ void cw_row_fold(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS],
+                                            const RowFaces<T> (&rf)[CW_LEVS]) {
+    const int last = A.nlev - 1;
 #pragma unroll
     for (int L = 0; L < CW_LEVS; L++) {
         const int l = min(l0 + L, last);
         fold_row<T>(rf[L], A.L[l].n0, A.L[l].n1, nd[L].i(), nd[L].j(), nd[L].c);
     }
-    {
-        const double mean = cw_sum(part) / (double)INNER(0);  // also orders the zero fill
-        nd[0].b = b0part ? (T)((double)braw - mean) : braw;
-    }
+}
 
+// The V-cycle of levels l0 .. A.nlev - 1 in one workgroup of CW threads, from the
+// rows, inverse diagonals and level-l0 right-hand side in nd[] (nd[L].b of the
+// coarser levels is set here), iterates in the LDS arrays at sh + A.off[l]; the
+// result is level l0's iterate, complete and visible to every thread on return.
+template <typename T>
+__device__ __forceinline__ void cw_vcycle(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS], T *sh) {
+    const int tid = threadIdx.x;
+    const T w = A.omega;
+    const int last = A.nlev - 1;
+    const int nl = last - l0 + 1;
+    auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
+    auto XA = [&](int L) { return sh + A.off[l0 + L]; };             // iterate of level l0 + L
+    auto TA = [&](int L) { return sh + A.off[l0 + L] + NN(l0 + L); }; // its ping-pong buffer
+    auto INNER = [&](int L) { return (A.L[l0 + L].n0 - 2) * (A.L[l0 + L].n1 - 2); };
     // Levels of at most CW_WAVE interior nodes ("wave levels", from LW on) are owned
     // by wavefront 0 alone: their down leg, the coarsest solve and their up leg run
     // in that wavefront, ordered by wavefront barriers (its LDS writes precede its
@@ -1066,8 +1056,48 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
             sweeps(Block{}, nd[L], L, A.post, false);
         }
     }
+}
+
+// b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
+// whose mean is removed; null: used as is); the level-l0 result is written to x0out.
+template <typename T>
+__global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, CoarseArgs<T> A, int l0, const T *b0,
+                                                  const double *b0part, int nb0p, size_t lps, T *x0out) {
+    if (slot_skip(scal + zslot() * SS, gs)) return;
     {
-        const T *x = XA(0);
+        const size_t o0 = zslot() * A.L[l0].n0 * A.L[l0].n1;
+        b0 += o0;
+        x0out += o0;
+        if (b0part) b0part += zslot() * lps;
+    }
+    __shared__ T sh[CN_MAX * CW_ARR];
+    const int tid = threadIdx.x;
+    const int last = A.nlev - 1;
+    auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
+    auto INNER = [&](int L) { return (A.L[l0 + L].n0 - 2) * (A.L[l0 + L].n1 - 2); };
+
+    // set-up loads: row faces and 1/diag of the owned node of every level
+    CNode<T> nd[CW_LEVS];
+    RowFaces<T> rf[CW_LEVS];
+    cw_row_loads<T>(A, l0, nd, rf);
+    const T braw = b0[nd[0].j() * A.L[l0].n0 + nd[0].i()];
+    double part = 0.0;
+    if (b0part)
+        for (int t = tid; t < nb0p; t += CW) part += b0part[t];
+    {
+        int tot = 0;
+        for (int l = l0; l <= last; l++) tot += CW_ARR * NN(l);
+        for (int k = tid; k < tot; k += CW) sh[k] = T(0);
+    }
+    cw_row_fold<T>(A, l0, nd, rf);
+    {
+        const double mean = cw_sum(part) / (double)INNER(0);  // also orders the zero fill
+        nd[0].b = b0part ? (T)((double)braw - mean) : braw;
+    }
+
+    cw_vcycle<T>(A, l0, nd, sh);
+    {
+        const T *x = sh + A.off[l0];
         for (int k = tid; k < NN(l0); k += CW) x0out[k] = x[k];
     }
 }

@@ -316,11 +316,16 @@ typedef struct {
                            coarse-grid correction (default 2, 2)                  */
     int fas_coarse;     /* SIMPLE iterations on the coarsest level (default 40)   */
     int fas_min_n;      /* no level coarser than this many nodes per side; <= 0
-                           (default): the smallest 2^k + 1 >= 33 whose cell Reynolds
-                           number Re / 2^k is at most 32, from params->Re (a batch
-                           then takes only slots whose Re fits its coarsest level) */
+                           (default): the smallest 2^k + 1 >= 17 whose cell Reynolds
+                           number Re / 2^k is at most 32, and >= 33 when Pr > 1 or
+                           |Ri| Re^2 Pr > 2e5, from params (a batch then takes only
+                           slots whose physics fits its coarsest level)            */
     int fas_coarse_scheme; /* wall_upwind value of the coarse levels (default 2;
                            the more upwinded of it and wall_upwind is used)        */
+    int fas_small;      /* 1 (default): a coarse FAS level of at most 33 x 33 nodes
+                           (and at most 4 p' multigrid levels) runs its SIMPLE
+                           iterations in one workgroup per case, in FP64; 0: in the
+                           stage kernels of the fine levels                        */
 } cfd_params;
 
 typedef struct {

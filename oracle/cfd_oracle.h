@@ -170,7 +170,8 @@ typedef struct {
                            1 = plain SIMPLE iterations                            */
     int fas_pre, fas_post, fas_coarse; /* SIMPLE iterations per level and cycle   */
     int fas_min_n;      /* no FAS level coarser than this many nodes per side; <= 0:
-                           the smallest 2^k + 1 >= 33 with Re / 2^k <= 32           */
+                           the smallest 2^k + 1 >= 17 (>= 33 when Pr > 1 or
+                           |Ri| Re^2 Pr > 2e5) with Re / 2^k <= 32                   */
     int fas_coarse_scheme; /* convection scheme of the coarse FAS levels (wall_upwind
                            values; the finer of it and wall_upwind is used)         */
     double hyb_blend;   /* 0: the reference's switch at |Pe| = 2; > 0: blend from the

@@ -414,8 +414,10 @@ static void fas_build(or_cav_case *c, const double *x, const double *y, const do
     const int n0 = c->n0, n1 = c->n1;
     const int m0 = (n0 + 1) / 2, m1 = (n1 + 1) / 2;
     int lo = c->prm.fas_min_n;
-    if (lo <= 0) { /* the smallest 2^k + 1 >= 33 with cell Reynolds number Re / 2^k <= 32 */
-        int m = 32;
+    if (lo <= 0) { /* the smallest 2^k + 1 >= 17 (>= 33 when Pr > 1 or Ra = |Ri| Re^2 Pr > 2e5)
+                      with cell Reynolds number Re / 2^k <= 32 */
+        const double ra = fabs(c->prm.Ri) * c->prm.Re * c->prm.Re * c->prm.Pr;
+        int m = (c->prm.Pr <= 1.0 && ra <= 2.0e5) ? 16 : 32;
         while (32.0 * m < c->prm.Re && m < (1 << 20)) m *= 2;
         lo = m + 1;
     }

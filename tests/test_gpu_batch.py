@@ -12,6 +12,10 @@ from cfd_solver_amd.grid import cavity_grid
 
 pytestmark = pytest.mark.gpu
 
+# one FAS depth for the mixed physics of a batch and for each case alone (the automatic
+# coarsest level depends on Re, Ri and Pr)
+DEPTH = dict(fas_min_n=33)
+
 CASES = [  # (skew, beta, physics)
     (0.05, None, dict(Re=100.0, Ri=1.0, Pr=0.71)),
     (0.12, 1.5, dict(Re=400.0, Ri=0.3, Pr=7.0)),
@@ -21,7 +25,7 @@ CASES = [  # (skew, beta, physics)
 
 def _alone(n, skew, beta, phys, precision, steps, n1=None):
     from cfd_solver_amd.cavity import CavityCase, default_params
-    c = CavityCase(cavity_grid(n, n1, skew=skew, beta=beta), default_params(precision=precision, **phys))
+    c = CavityCase(cavity_grid(n, n1, skew=skew, beta=beta), default_params(precision=precision, **DEPTH, **phys))
     st = None
     for _ in range(steps):
         st = c.step()
@@ -36,7 +40,7 @@ def test_batch_slots_equal_single_cases(precision):
     from cfd_solver_amd.cavity import CavityBatch, default_params
     n, steps = 65, 40
     g0 = cavity_grid(n)
-    b = CavityBatch(n, n, g0.dxi, g0.deta, 4, default_params(precision=precision))
+    b = CavityBatch(n, n, g0.dxi, g0.deta, 4, default_params(precision=precision, **DEPTH))
     for z, (skew, beta, phys) in enumerate(CASES):
         b.set_case(z if z < 1 else z + 1, cavity_grid(n, skew=skew, beta=beta), **phys)  # slot 1 stays empty
     for _ in range(steps):
@@ -58,7 +62,7 @@ def test_batch_refill_leaves_other_slots_alone():
     from cfd_solver_amd.cavity import CavityBatch, default_params
     n = 33
     g0 = cavity_grid(n)
-    b = CavityBatch(n, n, g0.dxi, g0.deta, 2, default_params())
+    b = CavityBatch(n, n, g0.dxi, g0.deta, 2, default_params(**DEPTH))
     (s0, b0, p0), (s1, b1, p1), (s2, b2, p2) = CASES
     b.set_case(0, cavity_grid(n, skew=s0, beta=b0), **p0)
     b.set_case(1, cavity_grid(n, skew=s1, beta=b1), **p1)
@@ -84,7 +88,7 @@ def test_batch_eager_equals_graph():
     g0 = cavity_grid(n)
     out = []
     for graph in (0, 1):
-        b = CavityBatch(n, n, g0.dxi, g0.deta, 3, default_params(graph=graph, precision=1))
+        b = CavityBatch(n, n, g0.dxi, g0.deta, 3, default_params(graph=graph, precision=1, **DEPTH))
         for z, (skew, beta, phys) in enumerate(CASES):
             b.set_case(z, cavity_grid(n, skew=skew, beta=beta), **phys)
         st = b.step(30)
@@ -107,7 +111,7 @@ def test_batch_slot_equals_alone_at_257(precision):
     n, steps = 257, 4
     skew, beta, phys = CASES[1]
     g0 = cavity_grid(n)
-    b = CavityBatch(n, n, g0.dxi, g0.deta, 2, default_params(precision=precision))
+    b = CavityBatch(n, n, g0.dxi, g0.deta, 2, default_params(precision=precision, **DEPTH))
     b.set_case(0, cavity_grid(n, skew=skew, beta=beta), **phys)
     b.set_case(1, cavity_grid(n, skew=CASES[0][0], beta=CASES[0][1]), **CASES[0][2])
     for _ in range(steps):
@@ -129,7 +133,7 @@ def test_batch_slots_equal_alone_on_rectangular_grid(precision):
     from cfd_solver_amd.cavity import CavityBatch, default_params
     n0, n1, steps = 97, 41, 4
     g0 = cavity_grid(n0, n1)
-    b = CavityBatch(n0, n1, g0.dxi, g0.deta, 3, default_params(precision=precision))
+    b = CavityBatch(n0, n1, g0.dxi, g0.deta, 3, default_params(precision=precision, **DEPTH))
     for z, (skew, beta, phys) in ((0, CASES[1]), (2, CASES[2])):
         b.set_case(z, cavity_grid(n0, n1, skew=skew, beta=beta), **phys)
     for _ in range(steps):

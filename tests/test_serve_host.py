@@ -26,14 +26,16 @@ def test_request_larger_than_the_budget_fails_at_once():
 
 def test_fas_coarsest_follows_the_reynolds_number():
     """Automatic multigrid depth: the coarsest level keeps Re / (n - 1) <= 32 and has at
-    least 33 nodes per side; an explicit fas_min_n overrides it; a grid that does not
-    coarsen keeps its size."""
+    least 17 nodes per side, 33 when Pr > 1 or Ra = |Ri| Re^2 Pr > 2e5; an explicit
+    fas_min_n overrides it; a grid that does not coarsen keeps its size."""
     from cfd_solver_amd import _lib
     from cfd_solver_amd.cavity import default_params
     L = _lib.load()
     co = lambda n, **kw: L.cfd_fas_coarsest(n, n, C.byref(default_params(**kw)))
-    assert co(1025, Re=100.0) == 33 and co(1025, Re=400.0) == 33 and co(1025, Re=1000.0) == 33
-    assert co(1025, Re=1100.0) == 65 and co(1025, Re=2100.0) == 129
+    assert co(1025, Re=100.0) == 17 and co(1025, Re=400.0) == 17  # Ri 1, Pr 0.71: Ra <= 1.2e5
+    assert co(1025, Re=400.0, Pr=7.0) == 33 and co(1025, Re=400.0, Ri=3.0) == 33  # Ra 3.4e5
+    assert co(1025, Re=100.0, Ri=-1.0) == 17 and co(1025, Re=520.0, Ri=0.1) == 33  # Re / 16 > 32
+    assert co(1025, Re=1000.0) == 33 and co(1025, Re=1100.0) == 65 and co(1025, Re=2100.0) == 129
     assert co(1025, Re=400.0, fas_min_n=65) == 65 and co(1025, Re=400.0, fas_min_n=17) == 17
     assert co(65, Re=1000.0) == 33 and co(33, Re=1000.0) == 33 and co(96, Re=100.0) == 96
     assert co(1025, Re=400.0, fas_levels=1) == 1025
