@@ -283,7 +283,7 @@ typedef struct {
     double t_hot, t_cold;
     int mom_sweeps;     /* Jacobi sweeps of the low-order predictor              */
     double p_rtol;      /* p' relative tolerance (2-norm)                        */
-    int p_maxit;        /* p' BiCGStab iteration cap                             */
+    int p_maxit;        /* p' BiCGStab iteration cap (default 2)                 */
     int mg_levels;      /* multigrid levels of the p' preconditioner             */
     int mg_pre, mg_post, mg_coarse;
     double mg_omega;    /* damping of the Jacobi smoother                        */
