@@ -99,7 +99,10 @@ int cfd_ref_face_div(int n0, int n1, double dxi, double deta,
 
 /* Solve the 9-point pressure-correction system, argument order of
  * Solver::sip9p (solver.cpp:2066-2068).  Same conventions as cfd_gauss;
- * replaces Stone's SIP of solver.cpp:2066-2277. */
+ * replaces Stone's SIP of solver.cpp:2066-2277.  BiCGStab preconditioned by exact
+ * tridiagonal solves along the eta lines (columns) when n1 - 2 <= 1024, else by the
+ * diagonal; the iteration is controlled on the device (the host waits on a flag in
+ * mapped memory, one iteration behind the launches). */
 int cfd_sip9p(int n0, int n1,
               const double *ap, const double *ae, const double *as, const double *an,
               const double *aw, const double *ase, const double *asw, const double *ane,
