@@ -246,18 +246,19 @@ def run_batched(mixed, cap=150, tol=1e-8):
     L = load()
     groups = defaultdict(list)
     for idx, (n, skew, phys) in enumerate(cases):
-        p = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1, Re=phys["Re"])
+        # the coarsest multigrid level a case gets depends on Re, Ri and Pr
+        p = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1, **phys)
         groups[(n, L.cfd_fas_coarsest(n, n, C.byref(p)))].append((idx, skew, phys))
     batches = []
     for (n, depth), members in sorted(groups.items()):
-        prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1,
-                             Re=max(ph["Re"] for _, _, ph in members))
+        prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1, fas_min_n=depth)
         g0 = cavity_grid(n)
         b = CavityBatch(n, n, g0.dxi, g0.deta, len(members), prm)
         for z, (_, skew, phys) in enumerate(members):
             b.set_case(z, cavity_grid(n, skew=skew), **phys)
         batches.append(dict(b=b, n=n, depth=depth, live=set(range(len(members))), members=members,
-                            cells=[cells_per_outer(n, n, default_params(Re=ph["Re"])) for _, _, ph in members]))
+                            cells=[cells_per_outer(n, n, default_params(fas_min_n=depth, **ph))
+                                   for _, _, ph in members]))
     states = {}
     cycles = {}
     cell_its = 0.0

@@ -88,11 +88,13 @@ void drop_class(Server &s, size_t k) {
         if (kv.second.cls > (int)k) kv.second.cls--;
 }
 
-// the batch parameters of r's class: the server's, with r's Reynolds number (which
-// sets the multigrid depth when fas_min_n is automatic)
+// the batch parameters of r's class: the server's, with r's Reynolds, Richardson and
+// Prandtl numbers (which set the multigrid depth when fas_min_n is automatic)
 cfd_params class_params(const Server &s, const Request &r) {
     cfd_params p = s.prm;
     p.Re = r.phys.Re;
+    p.Ri = r.phys.Ri;
+    p.Pr = r.phys.Pr;
     return p;
 }
 int coarsest_of(const Server &s, const Request &r) {
