@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libcfd_gpu.so")
 
 CFD_OK = 0
 ERRORS = {-1: "CFD_E_ARG", -2: "CFD_E_OOM", -3: "CFD_E_NAN", -4: "CFD_E_NOTCONV",
-          -5: "CFD_E_HIP", -6: "CFD_E_STATE"}
+          -5: "CFD_E_HIP", -6: "CFD_E_STATE", -7: "CFD_E_FULL"}
 
 CFD_FP64 = 0
 CFD_MIXED = 1
@@ -119,6 +119,17 @@ SIGNATURES = {
     "cfd_batch_set_fields": ([_p, _i, dp, dp, dp, dp], _i),
     "cfd_batch_bytes": ([_p, C.POINTER(C.c_size_t)], _i),
     "cfd_batch_bytes_needed": ([_i, _i, _d, _d, _i, C.POINTER(Params), C.POINTER(C.c_size_t)], _i),
+    "cfd_pack_create": ([_i, _i, _d, _d, _i, C.POINTER(_i), C.POINTER(Params), C.POINTER(_p)], _i),
+    "cfd_pack_destroy": ([_p], _i),
+    "cfd_pack_bytes_needed": ([_i, _i, _d, _d, _i, C.POINTER(_i), C.POINTER(Params), C.POINTER(C.c_size_t)], _i),
+    "cfd_pack_bytes": ([_p, C.POINTER(C.c_size_t)], _i),
+    "cfd_pack_capacity": ([_p, C.POINTER(_i)], _i),
+    "cfd_pack_add_case": ([_p, _i, _i] + [dp] * 10 + [_p, _p, C.POINTER(Params), C.POINTER(_i)], _i),
+    "cfd_pack_release": ([_p, _i], _i),
+    "cfd_pack_outer": ([_p, _i, _p], _i),
+    "cfd_pack_launch": ([_p], _i),
+    "cfd_pack_finish": ([_p, _p], _i),
+    "cfd_pack_get_fields": ([_p, _i, dp, dp, dp, dp], _i),
     "cfd_server_create": ([C.POINTER(Params), C.c_size_t, _i, C.POINTER(_p)], _i),
     "cfd_server_destroy": ([_p], _i),
     "cfd_server_submit": ([_p, _i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), _d, _i,

@@ -212,3 +212,82 @@ class CavityBatch:
         b = C.c_size_t()
         call("cfd_batch_bytes", self._h, C.byref(b))
         return b.value
+
+
+class CavityPack:
+    """Cavity cases of uneven size sharing every launch (cfd_pack_*).
+
+    Slab k holds slots[k] grids of ((n0 - 1) >> k) + 1 x ((n1 - 1) >> k) + 1 nodes; a
+    case of slab k's size takes a slot there and one in each slab down to its
+    coarsest multigrid level.  Pack-wide settings come from `params`; each case's
+    physics from the keywords of `add_case`.
+    """
+
+    def __init__(self, n0: int, n1: int, dxi: float, deta: float, slots, params: Params | None = None, **kw):
+        self.n0, self.n1 = n0, n1
+        self.params = params if params is not None else default_params(**kw)
+        self.slots = list(slots)
+        arr = (C.c_int * len(self.slots))(*self.slots)
+        self._h = C.c_void_p()
+        call("cfd_pack_create", n0, n1, dxi, deta, len(self.slots), arr, C.byref(self.params), C.byref(self._h))
+        cap = C.c_int()
+        call("cfd_pack_capacity", self._h, C.byref(cap))
+        self.capacity = cap.value
+        self._stats = (Stats * self.capacity)()
+        self._shape = {}
+
+    @staticmethod
+    def bytes_needed(n0, n1, dxi, deta, slots, params: Params) -> int:
+        arr = (C.c_int * len(slots))(*slots)
+        b = C.c_size_t()
+        call("cfd_pack_bytes_needed", n0, n1, dxi, deta, len(slots), arr, C.byref(params), C.byref(b))
+        return b.value
+
+    def close(self):
+        if self._h and self._h.value:
+            _lib.load().cfd_pack_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_case(self, grid: Grid, **phys) -> int:
+        """Load a case; returns its id (raises CfdError CFD_E_FULL when its slabs are full)."""
+        g = grid
+        arrs = [f64(a) for a in (g.x, g.y, g.xix, g.xiy, g.etx, g.ety, g.alph, g.beta, g.gamma, g.ajac,
+                                 g.p1, g.q1)]
+        sp = _slot_params(self.params, **phys)
+        cid = C.c_int()
+        call("cfd_pack_add_case", self._h, g.n0, g.n1, *arrs[:10], arrs[10].ctypes.data, arrs[11].ctypes.data,
+             C.byref(sp), C.byref(cid))
+        self._shape[cid.value] = (g.n0, g.n1)
+        return cid.value
+
+    def release(self, cid: int):
+        call("cfd_pack_release", self._h, int(cid))
+
+    def step(self, n: int = 1):
+        """n outer iterations of every case; returns the Stats array indexed by case id."""
+        call("cfd_pack_outer", self._h, int(n), C.cast(self._stats, C.c_void_p))
+        return self._stats
+
+    def launch(self):
+        call("cfd_pack_launch", self._h)
+
+    def finish(self):
+        call("cfd_pack_finish", self._h, C.cast(self._stats, C.c_void_p))
+        return self._stats
+
+    def fields(self, cid: int):
+        n0, n1 = self._shape[cid]
+        out = [np.zeros((n1, n0)) for _ in range(4)]
+        call("cfd_pack_get_fields", self._h, int(cid), *out)
+        return dict(u=out[0], v=out[1], T=out[2], p=out[3])
+
+    def nbytes(self):
+        b = C.c_size_t()
+        call("cfd_pack_bytes", self._h, C.byref(b))
+        return b.value

@@ -39,6 +39,9 @@ struct SlotP {
     double lid, t_hot, t_cold, len_hot, len_cold;
     double active;  // 1: the slot holds a case
     double dnu, dnt;  // dt / Re, dt / (Re Pr)
+    double scheme;   // convection next to walls (cfd_params::wall_upwind) on this slot's level
+    double fas_src;  // 1: the slot is a coarse level of a FAS hierarchy (applies the level's sources)
+    double deep;     // 1: the slot is the coarsest level of its case's hierarchy
 };
 
 // grid shape shared by the slots, and the slots' physics
@@ -48,18 +51,20 @@ struct CGrid {
     size_t N;          // nodes per slot
     const SlotP *sp;   // sp[blockIdx.z]
     RecipT<double> r;  // reciprocals of the grid constants
-    int scheme;        // convection next to walls: 0 reference, 1 upwinded, 2 upwinded everywhere
     double blend;      // width of the blended hybrid switch (0: the reference's step)
     __host__ __device__ KGrid kg() const { return KGrid{n0, n1, h1, h2}; }
 };
 
 // Source terms of a coarse FAS level (null on a level without them): u, v, T of
 // the momentum and energy residuals, m of the mass residual; on the border, T
-// holds the side-wall and m the wall-pressure sources.
+// holds the side-wall and m the wall-pressure sources.  A slot that is not a coarse
+// level (SlotP::fas_src = 0: the top level of a case packed beside coarse levels of
+// larger ones) has none.
 struct Src {
     const double *u, *v, *T, *m;
 };
-__device__ __forceinline__ Src src_at(Src s, size_t o) {
+__device__ __forceinline__ Src src_at(Src s, size_t o, const SlotP &ph) {
+    if (ph.fas_src == 0.0) return Src{nullptr, nullptr, nullptr, nullptr};
     if (s.u) {
         s.u += o; s.v += o; s.T += o; s.m += o;
     }
@@ -77,6 +82,10 @@ struct Case {
     int device = 0;
     // per-slot physics (device copy, host copy) and host-side per-slot state
     SlotP *spd = nullptr;
+    // the same table with only the slots that have a coarser level (spd_nd) or that
+    // are a coarse level (spd_cr) active; both equal spd in a hierarchy of equal
+    // slots, separate arrays in a pack's slab (cfd_pack)
+    SlotP *spd_nd = nullptr, *spd_cr = nullptr;
     std::vector<SlotP> sph;
     std::vector<double> mass0, energy0;
     std::vector<int> iter;
@@ -177,10 +186,11 @@ __device__ __forceinline__ ResOut residual_node(const CGrid &g, const SlotP &ph,
     const bool hyb = fit_i && fit_j;
     int mode_i = hyb ? 2 : (fit_i ? 1 : 0);
     int mode_j = hyb ? 2 : (fit_j ? 1 : 0);
-    if (g.scheme == 1) {
+    const int scheme = (int)ph.scheme;
+    if (scheme == 1) {
         mode_i = fit_i ? 2 : 3;
         mode_j = fit_j ? 2 : 3;
-    } else if (g.scheme == 2) {
+    } else if (scheme == 2) {
         mode_i = mode_j = 3;
     }
     const double al = c.al, be = c.be, ga = c.ga, pp = c.pp, qq = c.qq;
@@ -233,7 +243,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     ra = rz(ra);
     const size_t o = zslot() * g.N;
     u += o; v += o; T += o; p += o; Ru += o; Rv += o; RT += o; dd += o; Ua += o; Va += o;
-    src = src_at(src, o);
+    src = src_at(src, o, ph);
     constexpr int FX = RT_X + 4, FY = RT_Y + 4, PX = RT_X + 2, PY = RT_Y + 2;
     constexpr int NODES = RT_Y / BY;  // nodes per thread
     __shared__ double s_u[FX * FY], s_v[FX * FY], s_t[FX * FY], s_p[PX * PY];
@@ -548,6 +558,7 @@ __global__ __launch_bounds__(NT) void k_face_div(
     const double *__restrict__ ety, const double *__restrict__ jac, double *__restrict__ q, RedArgs rmass,
     RedArgs rq, const double *__restrict__ sm) {
     const bool active = g.sp[blockIdx.z].active != 0.0;
+    if (g.sp[blockIdx.z].fas_src == 0.0) sm = nullptr;
     rmass = rz(rmass);
     rq = rz(rq);
     const size_t o = zslot() * g.N;
@@ -717,6 +728,7 @@ __global__ __launch_bounds__(WB) void k_t_sides(CGrid g, const double *__restric
                                                 double *__restrict__ scratch, size_t scr,
                                                 const double *__restrict__ sT) {
     if (g.sp[blockIdx.z].active == 0.0) return;
+    if (g.sp[blockIdx.z].fas_src == 0.0) sT = nullptr;
     {
         const size_t o = zslot() * g.N;
         T += o; alph += o; beta += o;
@@ -818,6 +830,7 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
     w = wall_at(w, zslot() * cg.N);
     p += zslot() * cg.N;
     T += zslot() * cg.N;
+    if (sp.fas_src == 0.0) sm = nullptr;
     if (sm) sm += zslot() * cg.N;
     double *const pw = pout ? pout + zslot() * cg.N : p;
     scratch += zslot() * scr;
@@ -964,7 +977,7 @@ __device__ __forceinline__ void sl_predict(const SmallArgs *__restrict__ a, SlNo
     const size_t o = zslot() * g.N;
     double *const U_ = sl_arr(SL_U), *const V_ = sl_arr(SL_V), *const T_ = sl_arr(SL_T), *const P_ = sl_arr(SL_P);
     double *const D_ = sl_arr(SL_D);
-    const Src src = src_at(a->src, o);
+    const Src src = src_at(a->src, o, ph);
     const GridPtrs &gq = a->gp;
     const size_t G = o + P;
     const NC c{gq.xix[G], gq.xiy[G], gq.etx[G], gq.ety[G], gq.alph[G], gq.beta[G], gq.gamma[G], gq.p1[G], gq.q1[G]};
@@ -1041,7 +1054,7 @@ __device__ __forceinline__ void sl_face_div(const SmallArgs *__restrict__ a, SlN
     };
     auto pq = [&](int di, int dj) { return P_[P + dj * n0 + di]; };
     double qv = face_div_node(g, t.i, t.j, L, pq);
-    if (a->src.m) qv -= a->src.m[o + P];
+    if (a->src.m && g.sp[blockIdx.z].fas_src != 0.0) qv -= a->src.m[o + P];
     sl_arr(SL_B0)[P] = qv;
 }
 
@@ -1194,7 +1207,7 @@ __device__ __forceinline__ void sl_correct_walls(const SmallArgs *__restrict__ a
     const double *const D_ = sl_arr(SL_D), *const PC = sl_arr(SL_A1);
     const GridPtrs gp{a->gp.xix + o, a->gp.xiy + o, a->gp.etx + o, a->gp.ety + o, a->gp.alph + o,
                       a->gp.beta + o, a->gp.gamma + o, a->gp.p1 + o, a->gp.q1 + o};
-    const Src src = src_at(a->src, o);
+    const Src src = src_at(a->src, o, *ph);
     if (t.own) {
         const int i = t.i, j = t.j;
         const double pe = PC[clampi(i + 1, 1, n0 - 2) + j * n0], pw = PC[clampi(i - 1, 1, n0 - 2) + j * n0];
@@ -1252,9 +1265,13 @@ __device__ __forceinline__ void sl_correct_walls(const SmallArgs *__restrict__ a
     __syncthreads();
 }
 
-// a: the level's arguments in device memory (scalar loads, no per-lane copy)
-__global__ __launch_bounds__(SL_NT) void k_simple_small(const SmallArgs *__restrict__ a, int iters) {
-    if (a->g.sp[blockIdx.z].active == 0.0) return;
+// a: the level's arguments in device memory (scalar loads, no per-lane copy);
+// iters SIMPLE iterations, iters_deep on a slot that is its case's coarsest level
+__global__ __launch_bounds__(SL_NT) void k_simple_small(const SmallArgs *__restrict__ a, int iters, int iters_deep) {
+    const SlotP &sp = a->g.sp[blockIdx.z];
+    if (sp.active == 0.0) return;
+    if (sp.deep != 0.0) iters = iters_deep;
+    if (iters <= 0) return;
     const int n0 = a->g.n0, n1 = a->g.n1, N = n0 * n1;
     const size_t o = zslot() * a->g.N;
     const int tid = threadIdx.x;
@@ -1424,11 +1441,20 @@ __global__ __launch_bounds__(FIN_NT) void k_dmax_nu_fin(CGrid cg, RedArgs ra, in
 // FAS multigrid outer cycle (oracle: fas_cycle).  Levels are batches of the same
 // slots on the same stream; level l+1 takes every other node of level l.
 
+// The transfers run blockIdx.z over the fine level's slots (sp: the fine level's
+// slot table, in which a slot without a coarse level is inactive); fine slot z's
+// coarse level is slot cmap[z] of the coarse level (z itself when cmap is null:
+// a hierarchy of equal slots).
+__device__ __forceinline__ size_t cslot(const int *cmap) {
+    return cmap ? (size_t)cmap[blockIdx.z] : zslot();
+}
+
 // coarse node (I, J) <- fine node (2I, 2J) for u, v, T, p, kept in x0 as well
 struct FasInj {
     int n0, m0, m1;
     size_t Nf, Nc;
     const SlotP *sp;
+    const int *cmap;
     const double *f[4];
     double *c[4], *c0[4];
 };
@@ -1437,7 +1463,7 @@ __global__ void k_fas_inject(FasInj a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.m0 * a.m1) return;
     const int I = k % a.m0, J = k / a.m0;
-    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = zslot() * a.Nc + (size_t)k;
+    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = cslot(a.cmap) * a.Nc + (size_t)k;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const double v = a.f[q][F];
@@ -1455,6 +1481,7 @@ struct FasSrc {
     int n0, n1, m0, m1;
     size_t Nf, Nc, scrf, scrc;
     const SlotP *sp;
+    const int *cmap;
     const double *fr[3], *fq, *fT, *fscr;
     const double *cr[3], *cq, *cT, *cscr;
     double *s[4];
@@ -1466,7 +1493,7 @@ __global__ void k_fas_src(FasSrc a) {
     const int m0 = a.m0, m1 = a.m1, n0 = a.n0;
     if (k >= m0 * m1) return;
     const int I = k % m0, J = k / m0;
-    const size_t zf = zslot() * a.Nf, zc = zslot() * a.Nc, C = zc + (size_t)k;
+    const size_t zf = zslot() * a.Nf, zc = cslot(a.cmap) * a.Nc, C = zc + (size_t)k;
     const double idt = 1.0 / ph.dt;
     double sv[4] = {0.0, 0.0, 0.0, 0.0};
     if (I >= 1 && I <= m0 - 2 && J >= 1 && J <= m1 - 2) {
@@ -1481,7 +1508,7 @@ __global__ void k_fas_src(FasSrc a) {
         sv[3] = a.cq[C] - fw(a.fq, 1.0);
     } else if ((I == 0 || I == m0 - 1) && J >= 1 && J <= m1 - 2) {
         const int i = I == 0 ? 0 : a.n0 - 1, j = 2 * J;
-        const double *fs = a.fscr + zslot() * a.scrf, *cs = a.cscr + zslot() * a.scrc;
+        const double *fs = a.fscr + zslot() * a.scrf, *cs = a.cscr + cslot(a.cmap) * a.scrc;
         const double wf = (I == 0 ? fs[j] : fs[a.n1 + j]) - a.fT[zf + idx(i, j, n0)];
         const double wc = (I == 0 ? cs[J] : cs[m1 + J]) - a.cT[C];
         sv[2] = wf - wc;
@@ -1497,6 +1524,7 @@ struct FasSrcP {
     int n0, m0, m1;
     size_t Nf, Nc;
     const SlotP *sp;
+    const int *cmap;
     const double *fpw, *fp, *cpw, *cp;
     double *sm;
 };
@@ -1511,7 +1539,7 @@ __global__ void k_fas_src_p(FasSrcP a) {
     else if (k < 2 * m0) { I = k - m0; J = m1 - 1; }
     else if (k < 2 * m0 + m1 - 2) { I = 0; J = 1 + k - 2 * m0; }
     else { I = m0 - 1; J = 1 + k - 2 * m0 - (m1 - 2); }
-    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = zslot() * a.Nc + idx(I, J, m0);
+    const size_t F = zslot() * a.Nf + idx(2 * I, 2 * J, a.n0), C = cslot(a.cmap) * a.Nc + idx(I, J, m0);
     a.sm[C] = (a.fpw[F] - a.fp[F]) - (a.cpw[C] - a.cp[C]);
 }
 
@@ -1520,6 +1548,7 @@ struct FasPro {
     int n0, n1, m0;
     size_t Nf, Nc;
     const SlotP *sp;
+    const int *cmap;
     const double *c[4], *c0[4];
     double *f[4];
 };
@@ -1530,7 +1559,7 @@ __global__ void k_fas_prolong(FasPro a) {
     if (k >= n0 * n1) return;
     const int i = k % n0, j = k / n0;
     if (i < 1 || i > n0 - 2 || j < 1 || j > n1 - 2) return;
-    const size_t zc = zslot() * a.Nc;
+    const size_t zc = cslot(a.cmap) * a.Nc;
     const int I = i >> 1, J = j >> 1;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -1836,9 +1865,8 @@ PsCtx ctx(Case *c) {
 RedArgs red(const Case *c, int slot = 0) {
     return RedArgs{c->part + (size_t)slot * c->pstride, c->pstride, psz(c), c->scal};
 }
-CGrid cg(const Case *c) {
-    return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, c->spd, make_recip(c->h1, c->h2), c->prm.wall_upwind,
-                 c->prm.hyb_blend};
+CGrid cg(const Case *c, const SlotP *sp = nullptr) {
+    return CGrid{c->n0, c->n1, c->h1, c->h2, c->N, sp ? sp : c->spd, make_recip(c->h1, c->h2), c->prm.hyb_blend};
 }
 // the level's FAS sources when it uses them
 Src srcs(const Case *c) {
@@ -2076,12 +2104,13 @@ int predictor_t(Case *c, void **du, void **dv, void **dT) {
 // Wall conditions of the slots [z0, z0 + nz): velocity/temperature values
 // (uvt), adiabatic sides, then the wall pressure (which reads the new wall T).
 // Launched on pointers offset to slot z0, so other slots are untouched.
-int wall_bcs(Case *c, int z0, int nz, bool uvt) {
+// sp (may be null; then z0 = 0 and nz = all slots): the slot table to launch with.
+int wall_bcs(Case *c, int z0, int nz, bool uvt, const SlotP *sp = nullptr) {
     const int m = std::max(c->n0, c->n1);
     const size_t o = (size_t)z0 * c->N;
     const Src sr = srcs(c);
     const double *sT = sr.T ? sr.T + o : nullptr, *sm = sr.m ? sr.m + o : nullptr;
-    CGrid g = cg(c);
+    CGrid g = cg(c, sp);
     g.sp += z0;
     WallIn w = wall_in(c);
     w.u += o; w.v += o; w.T += o; w.p += o; w.xix += o; w.xiy += o; w.etx += o; w.ety += o;
@@ -2159,15 +2188,16 @@ unsigned nblk256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 // R/dt's numerators (FP64) of u, v, T at the current state into fr[0..2], d, and the
 // mass residual q of the state's own face velocities (no predictor correction),
-// with the level's sources when with_src
-int state_residual(Case *c, bool with_src) {
+// with the level's sources when with_src; sp: the slot table to launch with
+int state_residual(Case *c, bool with_src, const SlotP *sp) {
     const bool keep = c->use_src;
     c->use_src = with_src && keep;
     const dim3 gr = grid_res(c->n0, c->n1, c->B);
-    k_residual<double><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0], c->fr[1],
-                                                     c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES), srcs(c));
+    k_residual<double><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0],
+                                                     c->fr[1], c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES),
+                                                     srcs(c));
     CFD_LAUNCH_CHECK();
-    k_face_div<double><<<grid_in(c), dim3(BX, BY), 0, c->s>>>(cg(c), c->u, c->v, (const double *)nullptr,
+    k_face_div<double><<<grid_in(c), dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, (const double *)nullptr,
                                                              (const double *)nullptr, c->p, c->d, c->xix, c->xiy,
                                                              c->etx, c->ety, c->ajac, c->q, red(c, PS_MASS), red(c),
                                                              srcs(c).m);
@@ -2177,19 +2207,19 @@ int state_residual(Case *c, bool with_src) {
 }
 
 // the side-wall temperature update into scratch (with_src: the level's sources)
-int side_walls(Case *c, bool with_src) {
+int side_walls(Case *c, bool with_src, const SlotP *sp) {
     const dim3 gs((c->n1 + WB - 1) / WB, 1, c->B);
-    k_t_sides<<<gs, WB, 0, c->s>>>(cg(c), c->T, c->alph, c->beta, c->scratch, c->scr,
+    k_t_sides<<<gs, WB, 0, c->s>>>(cg(c, sp), c->T, c->alph, c->beta, c->scratch, c->scr,
                                    with_src ? srcs(c).T : nullptr);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 // the wall pressure the wall update would set (after the side-wall update in
 // scratch) into fr[3]
-int wall_p_out(Case *c, bool with_src) {
+int wall_p_out(Case *c, bool with_src, const SlotP *sp) {
     const int m = std::max(c->n0, c->n1);
-    k_p_walls<<<dim3((m + WB - 1) / WB, 5, c->B), WB, 0, c->s>>>(cg(c), wall_in(c), c->p, c->T, c->scratch, c->scr,
-                                                                 with_src ? srcs(c).m : nullptr, c->fr[3]);
+    k_p_walls<<<dim3((m + WB - 1) / WB, 5, c->B), WB, 0, c->s>>>(cg(c, sp), wall_in(c), c->p, c->T, c->scratch,
+                                                                 c->scr, with_src ? srcs(c).m : nullptr, c->fr[3]);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -2240,20 +2270,69 @@ int alloc_small(Case *c) {
     return CFD_OK;
 }
 
-// `iters` SIMPLE iterations of every slot of a small coarse level in one launch
-int simple_small(Case *c, int iters) {
-    if (iters <= 0) return CFD_OK;
+// SIMPLE iterations of every slot of a small coarse level in one launch: `iters`,
+// or `iters_deep` on a slot that is its case's coarsest level
+int simple_small(Case *c, int iters, int iters_deep) {
+    if (iters <= 0 && iters_deep <= 0) return CFD_OK;
     if (!c->sl_args || !c->use_src) return CFD_E_ARG;
-    k_simple_small<<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters);
+    k_simple_small<<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters, iters_deep);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
 
 // `iters` SIMPLE iterations of a level below the top of a FAS hierarchy
 int coarse_iters(Case *c, bool graph, int iters) {
-    if (small_fits(c)) return simple_small(c, iters);
+    if (small_fits(c)) return simple_small(c, iters, iters);
     for (int k = 0; k < iters; k++) RET(simple_body(c, graph, false));
     return CFD_OK;
+}
+
+// Restriction of level c's state to the coarse level cc, and cc's sources: spf is
+// c's slot table with only the slots that have a coarse level active, spc cc's with
+// only its coarse-level slots active, cmap (null: the same slot) c's slot -> cc's.
+int fas_restrict(Case *c, Case *cc, const SlotP *spf, const SlotP *spc, const int *cmap) {
+    // this level's residuals at its state, with its own sources
+    RET(state_residual(c, true, spf));
+    RET(side_walls(c, true, spf));
+    RET(wall_p_out(c, true, spf));
+    // restricted state and the coarse residuals of it without sources
+    const int m0 = cc->n0, m1 = cc->n1;
+    const size_t Mc = (size_t)m0 * m1;
+    {
+        FasInj a{c->n0, m0, m1, c->N, cc->N, spf, cmap, {c->u, c->v, c->T, c->p}, {cc->u, cc->v, cc->T, cc->p},
+                 {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}};
+        k_fas_inject<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    RET(state_residual(cc, false, spc));
+    RET(side_walls(cc, false, spc));
+    {
+        FasSrc a{c->n0, c->n1, m0, m1, c->N, cc->N, c->scr, cc->scr, spf, cmap,
+                 {c->fr[0], c->fr[1], c->fr[2]}, c->q, c->T, c->scratch,
+                 {cc->fr[0], cc->fr[1], cc->fr[2]}, cc->q, cc->T, cc->scratch,
+                 {cc->src[0], cc->src[1], cc->src[2], cc->src[3]}};
+        k_fas_src<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    cc->use_src = true;
+    RET(side_walls(cc, true, spc));
+    RET(wall_p_out(cc, false, spc));
+    {
+        FasSrcP a{c->n0, m0, m1, c->N, cc->N, spf, cmap, c->fr[3], c->p, cc->fr[3], cc->p, cc->src[3]};
+        k_fas_src_p<<<dim3(nblk256((size_t)2 * (m0 + m1)), 1, c->B), 256, 0, c->s>>>(a);
+        CFD_LAUNCH_CHECK();
+    }
+    return CFD_OK;
+}
+
+// Coarse-grid correction of level c from cc (the slots of spf, as fas_restrict's),
+// then c's wall conditions on those slots
+int fas_prolong(Case *c, Case *cc, const SlotP *spf, const int *cmap) {
+    FasPro a{c->n0, c->n1, cc->n0, c->N, cc->N, spf, cmap, {cc->u, cc->v, cc->T, cc->p},
+             {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}, {c->u, c->v, c->T, c->p}};
+    k_fas_prolong<<<dim3(nblk256(c->N), 1, c->B), 256, 0, c->s>>>(a);
+    CFD_LAUNCH_CHECK();
+    return wall_bcs(c, 0, c->B, true, spf);
 }
 
 int fas_cycle(Case *c, bool graph, bool top) {
@@ -2269,45 +2348,9 @@ int fas_cycle(Case *c, bool graph, bool top) {
     if (!top) RET(coarse_iters(c, g, pr.fas_pre));
     else
         for (int k = 0; k < pr.fas_pre; k++) RET(simple_body(c, g, top));
-    // this level's residuals at its state, with its own sources
-    RET(state_residual(c, true));
-    RET(side_walls(c, true));
-    RET(wall_p_out(c, true));
-    // restricted state and the coarse residuals of it without sources
-    const int m0 = cc->n0, m1 = cc->n1;
-    const size_t Mc = (size_t)m0 * m1;
-    {
-        FasInj a{c->n0, m0, m1, c->N, cc->N, c->spd, {c->u, c->v, c->T, c->p}, {cc->u, cc->v, cc->T, cc->p},
-                 {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}};
-        k_fas_inject<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
-        CFD_LAUNCH_CHECK();
-    }
-    RET(state_residual(cc, false));
-    RET(side_walls(cc, false));
-    {
-        FasSrc a{c->n0, c->n1, m0, m1, c->N, cc->N, c->scr, cc->scr, c->spd,
-                 {c->fr[0], c->fr[1], c->fr[2]}, c->q, c->T, c->scratch,
-                 {cc->fr[0], cc->fr[1], cc->fr[2]}, cc->q, cc->T, cc->scratch,
-                 {cc->src[0], cc->src[1], cc->src[2], cc->src[3]}};
-        k_fas_src<<<dim3(nblk256(Mc), 1, c->B), 256, 0, c->s>>>(a);
-        CFD_LAUNCH_CHECK();
-    }
-    cc->use_src = true;
-    RET(side_walls(cc, true));
-    RET(wall_p_out(cc, false));
-    {
-        FasSrcP a{c->n0, m0, m1, c->N, cc->N, c->spd, c->fr[3], c->p, cc->fr[3], cc->p, cc->src[3]};
-        k_fas_src_p<<<dim3(nblk256((size_t)2 * (m0 + m1)), 1, c->B), 256, 0, c->s>>>(a);
-        CFD_LAUNCH_CHECK();
-    }
+    RET(fas_restrict(c, cc, c->spd, cc->spd, nullptr));
     RET(fas_cycle(cc, graph, false));
-    {
-        FasPro a{c->n0, c->n1, m0, c->N, cc->N, c->spd, {cc->u, cc->v, cc->T, cc->p},
-                 {cc->x0[0], cc->x0[1], cc->x0[2], cc->x0[3]}, {c->u, c->v, c->T, c->p}};
-        k_fas_prolong<<<dim3(nblk256(c->N), 1, c->B), 256, 0, c->s>>>(a);
-        CFD_LAUNCH_CHECK();
-    }
-    RET(wall_bcs(c, 0, c->B, true));
+    RET(fas_prolong(c, cc, c->spd, nullptr));
     if (!top) return coarse_iters(c, g, pr.fas_post);
     for (int k = 0; k < pr.fas_post; k++) RET(simple_body(c, g, top));
     return CFD_OK;
@@ -2548,6 +2591,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     if (rc == CFD_OK) rc = dalloc(c, &c->part, (size_t)slots * psz(c));
     if (rc == CFD_OK) rc = dalloc(c, &c->scal, (size_t)slots * SS);
     if (rc == CFD_OK) rc = dalloc(c, &c->spd, (size_t)slots);
+    c->spd_nd = c->spd_cr = c->spd;
     cfd_params cp = *params;
     cp.fas_levels = params->fas_levels - 1;
     if (cp.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = cp.fas_coarse_scheme;
@@ -2595,58 +2639,50 @@ void dry_free(Case *c) {
     delete c;
 }
 
+// Slot zl of level l from slot zf of the finer level f: grid arrays injected, the
+// state at rest, the FP32 copies of the mixed path, the multigrid geometry.
+int inject_level(const Case *f, int zf, Case *l, int zl) {
+    const size_t of = (size_t)zf * f->N, ol = (size_t)zl * l->N;
+    const double *src[12] = {f->xg, f->yg, f->xix, f->xiy, f->etx, f->ety, f->alph, f->beta, f->gamma,
+                             f->ajac, f->p1, f->q1};
+    double *dst[12] = {l->xg, l->yg, l->xix, l->xiy, l->etx, l->ety, l->alph, l->beta, l->gamma, l->ajac,
+                       l->p1, l->q1};
+    const dim3 gr = grid2d(l->n0, l->n1);
+    for (int k = 0; k < 12; k++)
+        k_inject<<<gr, dim3(BX, BY), 0, l->s>>>(f->n0, f->n1, src[k] + of, l->n0, l->n1, dst[k] + ol);
+    CFD_LAUNCH_CHECK();
+    double *state[] = {l->u, l->v, l->T, l->p, l->pc};
+    for (double *a : state) CFD_HIP(hipMemsetAsync(a + ol, 0, l->N * sizeof(double), l->s));
+    if (l->mixed) {
+        const double *m[5] = {l->alph, l->beta, l->gamma, l->p1, l->q1};
+        const int lb = 256, lg = (int)((l->N + lb - 1) / lb);
+        for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, l->s>>>(l->N, m[k] + ol, l->a32[k] + ol);
+        CFD_HIP(hipMemsetAsync(l->ef + ol, 0, l->N * sizeof(float), l->s));
+        CFD_LAUNCH_CHECK();
+    }
+    return inject_slot(l, zl);
+}
+
 // slot z of the coarse levels below c: grid arrays injected from the finer level,
 // the FP32 copies of the mixed path, the multigrid geometry, the slot's physics
 int set_coarse_slots(Case *c, int z) {
     for (Case *f = c, *l = c->coarse; l; f = l, l = l->coarse) {
-        const size_t of = (size_t)z * f->N, ol = (size_t)z * l->N;
-        const double *src[12] = {f->xg, f->yg, f->xix, f->xiy, f->etx, f->ety, f->alph, f->beta, f->gamma,
-                                 f->ajac, f->p1, f->q1};
-        double *dst[12] = {l->xg, l->yg, l->xix, l->xiy, l->etx, l->ety, l->alph, l->beta, l->gamma, l->ajac,
-                           l->p1, l->q1};
-        const dim3 gr = grid2d(l->n0, l->n1);
-        for (int k = 0; k < 12; k++) k_inject<<<gr, dim3(BX, BY), 0, l->s>>>(f->n0, f->n1, src[k] + of, l->n0, l->n1,
-                                                                          dst[k] + ol);
-        CFD_LAUNCH_CHECK();
-        double *state[] = {l->u, l->v, l->T, l->p, l->pc};
-        for (double *a : state) CFD_HIP(hipMemsetAsync(a + ol, 0, l->N * sizeof(double), l->s));
-        if (l->mixed) {
-            const double *m[5] = {l->alph, l->beta, l->gamma, l->p1, l->q1};
-            const int lb = 256, lg = (int)((l->N + lb - 1) / lb);
-            for (int k = 0; k < 5; k++) k_cvt<double, float><<<lg, lb, 0, l->s>>>(l->N, m[k] + ol, l->a32[k] + ol);
-            CFD_HIP(hipMemsetAsync(l->ef + ol, 0, l->N * sizeof(float), l->s));
-            CFD_LAUNCH_CHECK();
-        }
-        RET(inject_slot(l, z));
+        RET(inject_level(f, z, l, z));
         l->sph[z] = f->sph[z];
+        l->sph[z].scheme = l->prm.wall_upwind;
+        l->sph[z].fas_src = 1.0;
+        l->sph[z].deep = l->coarse ? 0.0 : 1.0;
         l->iter[z] = 0;
     }
     return CFD_OK;
 }
 
-// Load a case into slot z: grid arrays (host), coarse metrics, physics; fields at
-// rest with the wall conditions; the slot becomes active.
-int batch_set(Case *c, int z, const double *x, const double *y, const double *xix, const double *xiy,
+// Slot z of level c from host grid arrays: copies, the FP32 copies of the mixed
+// path, the multigrid geometry, the state at rest.
+int load_grid(Case *c, int z, const double *x, const double *y, const double *xix, const double *xiy,
               const double *etx, const double *ety, const double *alph, const double *beta, const double *gamma,
-              const double *ajac, const double *p1, const double *q1, const cfd_params *phys) {
-    if (!c || z < 0 || z >= c->B || !x || !y || !xix || !xiy || !etx || !ety || !alph || !beta || !gamma || !ajac ||
-        !phys)
-        return CFD_E_ARG;
-    const int n0 = c->n0, n1 = c->n1;
+              const double *ajac, const double *p1, const double *q1) {
     const size_t N = c->N, B = N * sizeof(double), o = (size_t)z * N;
-    // a hierarchy built for other physics may have a level too coarse for this case
-    if (c->coarse && c->prm.fas_min_n <= 0) {
-        cfd_params q = c->prm;
-        q.Re = phys->Re;
-        q.Ri = phys->Ri;
-        q.Pr = phys->Pr;
-        const Case *l = c;
-        while (l->coarse) l = l->coarse;
-        if (std::min(l->n0, l->n1) < fas_min_eff(q)) return CFD_E_ARG;
-    }
-    // the slot's kernels must not run while its arrays are replaced
-    c->sph[z].active = 0.0;
-    RET(put_slot(c, z));
     const double *src[] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac};
     double *dst[] = {c->xg, c->yg, c->xix, c->xiy, c->etx, c->ety, c->alph, c->beta, c->gamma, c->ajac};
     for (int k = 0; k < 10; k++) CFD_HIP(hipMemcpyAsync(dst[k] + o, src[k], B, hipMemcpyHostToDevice, c->s));
@@ -2665,13 +2701,21 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
         CFD_HIP(hipMemsetAsync(c->ef + o, 0, N * sizeof(float), c->s));
         CFD_LAUNCH_CHECK();
     }
-    RET(inject_slot(c, z));
-    SlotP &sp = c->sph[z];
+    return inject_slot(c, z);
+}
+
+// a case's physics and wall lengths as the slot entry of its top level (inactive)
+SlotP slot_physics(int n0, int n1, const double *x, const double *y, const cfd_params *phys, int scheme) {
+    SlotP sp{};
     sp.Re = phys->Re; sp.Ri = phys->Ri; sp.Pr = phys->Pr; sp.dt = phys->dt;
     sp.alpha_u = phys->alpha_u; sp.alpha_t = phys->alpha_t; sp.alpha_p = phys->alpha_p;
     sp.lid = phys->lid_u; sp.t_hot = phys->t_hot; sp.t_cold = phys->t_cold;
     sp.dnu = phys->dt / phys->Re;
     sp.dnt = phys->dt / (phys->Re * phys->Pr);
+    sp.scheme = scheme;
+    sp.fas_src = 0.0;
+    sp.deep = 0.0;
+    sp.active = 0.0;
     sp.len_hot = sp.len_cold = 0.0;
     for (int i = 0; i < n0 - 1; i++) {
         const size_t a = (size_t)i, b = (size_t)i + 1;
@@ -2679,6 +2723,39 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
         sp.len_hot += std::hypot(x[b] - x[a], y[b] - y[a]);
         sp.len_cold += std::hypot(x[b1] - x[a1], y[b1] - y[a1]);
     }
+    return sp;
+}
+
+bool grid_args_ok(const double *x, const double *y, const double *xix, const double *xiy, const double *etx,
+                  const double *ety, const double *alph, const double *beta, const double *gamma,
+                  const double *ajac) {
+    return x && y && xix && xiy && etx && ety && alph && beta && gamma && ajac;
+}
+
+// Load a case into slot z: grid arrays (host), coarse metrics, physics; fields at
+// rest with the wall conditions; the slot becomes active.
+int batch_set(Case *c, int z, const double *x, const double *y, const double *xix, const double *xiy,
+              const double *etx, const double *ety, const double *alph, const double *beta, const double *gamma,
+              const double *ajac, const double *p1, const double *q1, const cfd_params *phys) {
+    if (!c || z < 0 || z >= c->B || !grid_args_ok(x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac) || !phys)
+        return CFD_E_ARG;
+    // a hierarchy built for other physics may have a level too coarse for this case
+    if (c->coarse && c->prm.fas_min_n <= 0) {
+        cfd_params q = c->prm;
+        q.Re = phys->Re;
+        q.Ri = phys->Ri;
+        q.Pr = phys->Pr;
+        const Case *l = c;
+        while (l->coarse) l = l->coarse;
+        if (std::min(l->n0, l->n1) < fas_min_eff(q)) return CFD_E_ARG;
+    }
+    // the slot's kernels must not run while its arrays are replaced
+    c->sph[z].active = 0.0;
+    RET(put_slot(c, z));
+    RET(load_grid(c, z, x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1));
+    SlotP &sp = c->sph[z];
+    sp = slot_physics(c->n0, c->n1, x, y, phys, c->prm.wall_upwind);
+    sp.deep = c->coarse ? 0.0 : 1.0;
     RET(set_coarse_slots(c, z));
     sp.active = 1.0;
     RET(put_slot(c, z, true));
@@ -2690,6 +2767,324 @@ int batch_set(Case *c, int z, const double *x, const double *y, const double *xi
 }
 
 int batch_check_slot(const Case *c, int z) { return (!c || z < 0 || z >= c->B) ? CFD_E_ARG : CFD_OK; }
+
+// ---- packs: cases of uneven size sharing every launch (cfd_pack_*)
+//
+// A pack is a chain of slabs, slab k a batch of slots of ((n0 - 1) >> k) + 1 x
+// ((n1 - 1) >> k) + 1 nodes with spacing 2^k (dxi, deta).  A case whose finest grid
+// is slab k's size takes one slot of slab k (its top level) and one slot of every
+// slab below it down to its coarsest FAS level, so one slab holds the top levels of
+// the cases of its size beside the coarse levels of all larger ones, and one FAS
+// cycle of the whole pack is one launch sequence: each SIMPLE stage of each slab is
+// one launch over all of its slots, whatever case they belong to.  The slots' roles
+// are per-slot data (SlotP::scheme / fas_src / deep), the coarse slot of a fine one
+// a per-slab map (cmap), and transfers run on the slot tables with only the slots
+// taking part active (spd_nd, spd_cr), so every case computes what it computes
+// alone in a batch of its own.  Requirements: a case's top level is not a small
+// level (k_simple_small), its coarsest level is one.
+struct Pack {
+    int ns = 0;
+    std::vector<Case *> slab;
+    std::vector<int> off;                 // case ids of slab k's slots start at off[k]
+    hipStream_t s = nullptr;
+    cfd_params prm{};                     // the slabs' (single levels)
+    cfd_params uprm{};                    // the caller's (a case's own FAS depth)
+    std::vector<std::vector<int>> owner;  // case id owning each slab slot, or -1
+    std::vector<std::vector<int>> cmap;   // slot of the next slab of the same case, or -1
+    std::vector<int *> cmap_d;
+    std::vector<int> last;                // per case id: its coarsest slab (-1: no case)
+    std::vector<SlotP *> mask_alloc;
+    bool tuned = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    size_t bytes = 0;
+    bool dry = false;
+};
+
+void pack_destroy(Pack *P) {
+    if (!P) return;
+    if (P->s) (void)hipStreamSynchronize(P->s);
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    for (int *m : P->cmap_d)
+        if (m) (void)hipFree(m);
+    for (SlotP *m : P->mask_alloc)
+        if (m) (void)hipFree(m);
+    for (Case *c : P->slab) {
+        if (!c) continue;
+        if (P->dry) dry_free(c);
+        else batch_destroy(c);
+    }
+    if (P->s) (void)hipStreamDestroy(P->s);
+    (void)hipGetLastError();
+    delete P;
+}
+
+int pack_create(int n0, int n1, double dxi, double deta, int nslabs, const int *slots, const cfd_params *params,
+                Pack **out, bool dry) {
+    if (!out || !params || !slots || nslabs < 2 || nslabs > 16 || n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0))
+        return CFD_E_ARG;
+    *out = nullptr;
+    // the chain's shapes first (no device work for a bad one)
+    for (int k = 0, a = n0, b = n1; k < nslabs; k++) {
+        if (k > 0) {
+            if (!(a & 1) || !(b & 1)) return CFD_E_ARG;
+            a = (a + 1) / 2;
+            b = (b + 1) / 2;
+        }
+        if (a < 5 || b < 5 || slots[k] < 0 || slots[k] > 65535) return CFD_E_ARG;
+    }
+    Pack *P = new Pack();
+    P->dry = dry;
+    P->ns = nslabs;
+    P->prm = *params;
+    P->uprm = *params;
+    P->prm.fas_levels = 1;  // slabs are single levels; the pack links them
+    P->slab.assign(nslabs, nullptr);
+    P->off.assign(nslabs + 1, 0);
+    P->owner.resize(nslabs);
+    P->cmap.resize(nslabs);
+    P->cmap_d.assign(nslabs, nullptr);
+    int rc = CFD_OK;
+    if (!dry && hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking) != hipSuccess) rc = CFD_E_HIP;
+    int a = n0, b = n1;
+    double h1 = dxi, h2 = deta;
+    for (int k = 0; k < nslabs && rc == CFD_OK; k++) {
+        if (k > 0) {
+            if (!(a & 1) || !(b & 1)) { rc = CFD_E_ARG; break; }
+            a = (a + 1) / 2; b = (b + 1) / 2; h1 *= 2.0; h2 *= 2.0;
+        }
+        if (a < 5 || b < 5 || slots[k] < 0 || slots[k] > 65535) { rc = CFD_E_ARG; break; }
+        P->off[k + 1] = P->off[k] + slots[k];
+        P->owner[k].assign(slots[k], -1);
+        P->cmap[k].assign(slots[k], -1);
+        if (slots[k] == 0) continue;
+        rc = batch_create(a, b, h1, h2, slots[k], &P->prm, &P->slab[k], dry, dry ? nullptr : P->s, true);
+        if (rc != CFD_OK) break;
+        Case *c = P->slab[k];
+        if (dry) {
+            c->bytes += 2 * (size_t)slots[k] * sizeof(SlotP) + (size_t)slots[k] * sizeof(int);
+        } else {
+            for (int m = 0; m < 2 && rc == CFD_OK; m++) {
+                SlotP *d = nullptr;
+                if (hipMalloc(&d, (size_t)slots[k] * sizeof(SlotP)) != hipSuccess ||
+                    hipMemsetAsync(d, 0, (size_t)slots[k] * sizeof(SlotP), P->s) != hipSuccess)
+                    rc = CFD_E_OOM;
+                P->mask_alloc.push_back(d);
+                (m == 0 ? c->spd_nd : c->spd_cr) = d;
+            }
+            if (rc == CFD_OK && hipMalloc(&P->cmap_d[k], (size_t)slots[k] * sizeof(int)) != hipSuccess) rc = CFD_E_OOM;
+            if (rc == CFD_OK)
+                rc = hipMemcpyAsync(P->cmap_d[k], P->cmap[k].data(), (size_t)slots[k] * sizeof(int),
+                                    hipMemcpyHostToDevice, P->s) == hipSuccess ? CFD_OK : CFD_E_HIP;
+            c->bytes += 2 * (size_t)slots[k] * sizeof(SlotP) + (size_t)slots[k] * sizeof(int);
+        }
+        P->bytes += c->bytes;
+    }
+    P->last.assign(P->off[nslabs], -1);
+    if (rc == CFD_OK && !dry && hipStreamSynchronize(P->s) != hipSuccess) rc = CFD_E_HIP;
+    if (rc != CFD_OK) {
+        pack_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return CFD_OK;
+}
+
+// slot z of slab k on the device: its entry, its masked entries (with a coarser
+// level / as a coarse level), S_ACTIVE (and S_FRESH), its coarse-slot map entry
+int pack_put(Pack *P, int k, int z, bool fresh) {
+    Case *c = P->slab[k];
+    RET(put_slot(c, z, fresh));
+    SlotP nd = c->sph[z], cr = c->sph[z];
+    nd.active = (nd.active != 0.0 && nd.deep == 0.0) ? 1.0 : 0.0;
+    cr.active = (cr.active != 0.0 && cr.fas_src != 0.0) ? 1.0 : 0.0;
+    CFD_HIP(hipMemcpyAsync(c->spd_nd + z, &nd, sizeof(SlotP), hipMemcpyHostToDevice, P->s));
+    CFD_HIP(hipMemcpyAsync(c->spd_cr + z, &cr, sizeof(SlotP), hipMemcpyHostToDevice, P->s));
+    CFD_HIP(hipMemcpyAsync(P->cmap_d[k] + z, &P->cmap[k][z], sizeof(int), hipMemcpyHostToDevice, P->s));
+    CFD_HIP(hipStreamSynchronize(P->s));
+    return CFD_OK;
+}
+
+int pack_release(Pack *P, int id) {
+    if (!P || id < 0 || id >= P->off[P->ns] || P->last[id] < 0) return CFD_E_ARG;
+    int k = 0;
+    while (P->off[k + 1] <= id) k++;
+    int z = id - P->off[k];
+    const int kd = P->last[id];
+    for (int l = k; l <= kd; l++) {
+        const int zn = P->cmap[l][z];
+        P->slab[l]->sph[z].active = 0.0;
+        P->owner[l][z] = -1;
+        P->cmap[l][z] = -1;
+        RET(pack_put(P, l, z, false));
+        z = zn;
+    }
+    P->last[id] = -1;
+    return CFD_OK;
+}
+
+// A case of n0 x n1 nodes into the first free slots of its slabs; *id: the case
+// id (the top slot's).  CFD_E_FULL when a slab of its chain has no free slot.
+int pack_add(Pack *P, int n0, int n1, const double *x, const double *y, const double *xix, const double *xiy,
+             const double *etx, const double *ety, const double *alph, const double *beta, const double *gamma,
+             const double *ajac, const double *p1, const double *q1, const cfd_params *phys, int *id) {
+    if (!P || !id || !phys || !grid_args_ok(x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac)) return CFD_E_ARG;
+    int k = -1;
+    for (int l = 0; l < P->ns; l++)
+        if (P->slab[l] && P->slab[l]->n0 == n0 && P->slab[l]->n1 == n1) k = l;
+    if (k < 0) return CFD_E_ARG;
+    // the case's own hierarchy (as a batch of its own would have it): its coarsest level
+    cfd_params q = P->uprm;
+    q.Re = phys->Re; q.Ri = phys->Ri; q.Pr = phys->Pr;
+    const int coarsest = cfd_fas_coarsest(n0, n1, &q);
+    int kd = -1;
+    for (int l = k + 1; l < P->ns; l++)
+        if (P->slab[l] && std::min(P->slab[l]->n0, P->slab[l]->n1) == coarsest) kd = l;
+    if (kd < 0) return CFD_E_ARG;
+    for (int l = k; l <= kd; l++)
+        if (!P->slab[l]) return CFD_E_ARG;
+    if (small_fits(P->slab[k]) || !small_fits(P->slab[kd])) return CFD_E_ARG;
+    std::vector<int> zs(kd + 1, -1);
+    for (int l = k; l <= kd; l++) {
+        for (int z = 0; z < P->slab[l]->B; z++)
+            if (P->owner[l][z] < 0) { zs[l] = z; break; }
+        if (zs[l] < 0) return CFD_E_FULL;
+    }
+    const int cid = P->off[k] + zs[k];
+    Case *top = P->slab[k];
+    RET(load_grid(top, zs[k], x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1));
+    const SlotP base = slot_physics(n0, n1, x, y, phys, P->prm.wall_upwind);
+    const int cscheme = std::max(P->prm.wall_upwind, P->prm.fas_coarse_scheme);
+    for (int l = k; l <= kd; l++) {
+        Case *c = P->slab[l];
+        const int z = zs[l];
+        if (l > k) RET(inject_level(P->slab[l - 1], zs[l - 1], c, z));
+        SlotP &sp = c->sph[z];
+        sp = base;
+        if (l > k) {
+            sp.scheme = cscheme;
+            sp.fas_src = 1.0;
+        }
+        sp.deep = l == kd ? 1.0 : 0.0;
+        sp.active = 1.0;
+        P->owner[l][z] = cid;
+        P->cmap[l][z] = l < kd ? zs[l + 1] : -1;
+        c->iter[z] = 0;
+        c->mass0[z] = c->energy0[z] = 0.0;
+    }
+    for (int l = kd; l >= k; l--) RET(pack_put(P, l, zs[l], l == k));
+    RET(wall_bcs(top, zs[k], 1, true));
+    CFD_HIP(hipStreamSynchronize(P->s));
+    P->last[cid] = kd;
+    *id = cid;
+    return CFD_OK;
+}
+
+// one FAS cycle of slab k and the slabs below it: the pre-smoothing (on a small
+// slab, a slot at its coarsest level runs its fas_coarse iterations there), the
+// restriction of the slots that have a coarser level, the slabs below, the
+// correction, the post-smoothing
+int pack_cycle(Pack *P, int k) {
+    Case *c = P->slab[k];
+    Case *cc = k + 1 < P->ns ? P->slab[k + 1] : nullptr;
+    const cfd_params &pr = P->prm;
+    const bool small = small_fits(c);
+    if (small) RET(simple_small(c, pr.fas_pre, pr.fas_coarse));
+    else
+        for (int it = 0; it < pr.fas_pre; it++) RET(simple_body(c, true, true));
+    if (cc) {
+        RET(fas_restrict(c, cc, c->spd_nd, cc->spd_cr, P->cmap_d[k]));
+        RET(pack_cycle(P, k + 1));
+        RET(fas_prolong(c, cc, c->spd_nd, P->cmap_d[k]));
+    }
+    if (small) return simple_small(c, pr.fas_post, 0);
+    for (int it = 0; it < pr.fas_post; it++) RET(simple_body(c, true, true));
+    return CFD_OK;
+}
+
+int pack_launch(Pack *P) {
+    if (!P || P->dry) return CFD_E_ARG;
+    int k0 = 0;
+    while (k0 < P->ns && !P->slab[k0]) k0++;
+    if (k0 == P->ns) return CFD_OK;
+    for (int k = k0; k < P->ns; k++)
+        if (!P->slab[k]) return CFD_E_ARG;  // the chain below the largest slab is complete
+    if (!P->tuned) {
+        for (int k = k0; k < P->ns; k++) RET(tune_mg(P->slab[k]));
+        P->tuned = true;
+    }
+    if (!P->prm.graph) return pack_cycle(P, k0);
+    if (!P->gexec) {
+        CFD_HIP(hipStreamSynchronize(P->s));
+        CFD_HIP(hipStreamBeginCapture(P->s, hipStreamCaptureModeThreadLocal));
+        for (int k = k0; k < P->ns; k++) P->slab[k]->capturing = true;
+        const int rc = pack_cycle(P, k0);
+        for (int k = k0; k < P->ns; k++) P->slab[k]->capturing = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(P->s, &g);
+        if (rc != CFD_OK || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            fprintf(stderr, "cfd_gpu: pack cycle capture failed (rc %d, %s)\n", rc, hipGetErrorString(e));
+            return rc != CFD_OK ? rc : CFD_E_HIP;
+        }
+        P->graph = g;
+        CFD_HIP(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+    }
+    CFD_HIP(hipGraphLaunch(P->gexec, P->s));
+    return CFD_OK;
+}
+
+// statistics of every case (st: one entry per case id, zeros where there is none)
+int pack_finish(Pack *P, cfd_stats *st) {
+    if (!P || P->dry) return CFD_E_ARG;
+    CFD_HIP(hipStreamSynchronize(P->s));
+    for (int k = 0; k < P->ns; k++) {
+        Case *c = P->slab[k];
+        if (!c) continue;
+        bool tops = false;
+        for (int z = 0; z < c->B; z++) tops |= P->owner[k][z] == P->off[k] + z;
+        if (tops) RET(sync_scal(c));
+    }
+    for (int k = 0; k < P->ns; k++) {
+        Case *c = P->slab[k];
+        for (int z = 0; c && z < c->B; z++) {
+            const int id = P->off[k] + z;
+            if (P->owner[k][z] != id) {
+                if (st) st[id] = cfd_stats{};
+                continue;
+            }
+            const double *h = c->hscal + (size_t)z * SS;
+            c->iter[z]++;
+            if (c->iter[z] == 1) {
+                c->mass0[z] = h[S_MASS0];
+                c->energy0[z] = h[S_ENERGY0];
+            }
+            if (!st) continue;
+            cfd_stats &o = st[id];
+            psolve_result(c, z, &o.p_iters, &o.p_relres);
+            o.mass_l1 = h[S_MASS];
+            o.energy_l1 = h[S_ENERGY];
+            o.mom_l1 = h[S_MOM];
+            o.dmax = h[S_DMAX];
+            o.nu_hot = h[S_NUH];
+            o.nu_cold = h[S_NUC];
+            o.mass_res = c->mass0[z] > 0 ? h[S_MASS] / c->mass0[z] : 0.0;
+            o.energy_res = c->energy0[z] > 0 ? h[S_ENERGY] / c->energy0[z] : 0.0;
+            o.iter = c->iter[z];
+        }
+    }
+    return CFD_OK;
+}
+
+int pack_case(const Pack *P, int id, Case **c, int *z) {
+    if (!P || id < 0 || id >= P->off[P->ns] || P->last[id] < 0) return CFD_E_ARG;
+    int k = 0;
+    while (P->off[k + 1] <= id) k++;
+    *c = P->slab[k];
+    *z = id - P->off[k];
+    return CFD_OK;
+}
 
 }  // namespace
 
@@ -2828,6 +3223,77 @@ int cfd_batch_bytes_needed(int n0, int n1, double dxi, double deta, int slots, c
 }
 
 // ---- single cases: a batch of one slot
+
+// ---- packs
+
+int cfd_pack_create(int n0, int n1, double dxi, double deta, int nslabs, const int *slots, const cfd_params *params,
+                    cfd_pack *out) {
+    if (!out) return CFD_E_ARG;
+    Pack *P = nullptr;
+    RET(pack_create(n0, n1, dxi, deta, nslabs, slots, params, &P, false));
+    *out = reinterpret_cast<cfd_pack>(P);
+    return CFD_OK;
+}
+
+int cfd_pack_destroy(cfd_pack h) {
+    pack_destroy(reinterpret_cast<Pack *>(h));
+    return CFD_OK;
+}
+
+int cfd_pack_bytes_needed(int n0, int n1, double dxi, double deta, int nslabs, const int *slots,
+                          const cfd_params *params, size_t *bytes) {
+    if (!bytes) return CFD_E_ARG;
+    Pack *P = nullptr;
+    const int rc = pack_create(n0, n1, dxi, deta, nslabs, slots, params, &P, true);
+    if (rc == CFD_OK) *bytes = P->bytes;
+    pack_destroy(P);
+    return rc;
+}
+
+int cfd_pack_bytes(cfd_pack h, size_t *bytes) {
+    const Pack *P = reinterpret_cast<const Pack *>(h);
+    if (!P || !bytes) return CFD_E_ARG;
+    *bytes = P->bytes;
+    return CFD_OK;
+}
+
+int cfd_pack_capacity(cfd_pack h, int *ids) {
+    const Pack *P = reinterpret_cast<const Pack *>(h);
+    if (!P || !ids) return CFD_E_ARG;
+    *ids = P->off[P->ns];
+    return CFD_OK;
+}
+
+int cfd_pack_add_case(cfd_pack h, int n0, int n1, const double *x, const double *y, const double *xix,
+                      const double *xiy, const double *etx, const double *ety, const double *alph,
+                      const double *beta, const double *gamma, const double *ajac, const double *p1,
+                      const double *q1, const cfd_params *params, int *id) {
+    return pack_add(reinterpret_cast<Pack *>(h), n0, n1, x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1,
+                    params, id);
+}
+
+int cfd_pack_release(cfd_pack h, int id) { return pack_release(reinterpret_cast<Pack *>(h), id); }
+
+int cfd_pack_launch(cfd_pack h) { return pack_launch(reinterpret_cast<Pack *>(h)); }
+
+int cfd_pack_finish(cfd_pack h, cfd_stats *st) { return pack_finish(reinterpret_cast<Pack *>(h), st); }
+
+int cfd_pack_outer(cfd_pack h, int n, cfd_stats *st) {
+    Pack *P = reinterpret_cast<Pack *>(h);
+    if (!P || n < 0) return CFD_E_ARG;
+    for (int k = 0; k < n; k++) {
+        RET(pack_launch(P));
+        RET(pack_finish(P, st));
+    }
+    return CFD_OK;
+}
+
+int cfd_pack_get_fields(cfd_pack h, int id, double *u, double *v, double *T, double *p) {
+    Case *c = nullptr;
+    int z = 0;
+    RET(pack_case(reinterpret_cast<const Pack *>(h), id, &c, &z));
+    return cfd_batch_get_fields(reinterpret_cast<cfd_batch>(c), z, u, v, T, p);
+}
 
 int cfd_case_create(int n0, int n1, double dxi, double deta, const double *x, const double *y,
                     const double *xix, const double *xiy, const double *etx, const double *ety,

@@ -38,6 +38,7 @@ extern "C" {
 #define CFD_E_NOTCONV (-4)  /* iteration cap reached before the tolerance         */
 #define CFD_E_HIP (-5)      /* HIP runtime error (message on stderr)              */
 #define CFD_E_STATE (-6)    /* call out of order (e.g. fields never set)          */
+#define CFD_E_FULL (-7)     /* no free slot for the case (cfd_pack_add_case)      */
 
 /* Library / device information. */
 const char *cfd_version(void);
@@ -446,6 +447,54 @@ int cfd_batch_bytes(cfd_batch b, size_t *bytes);
 /* HBM bytes a batch of this shape would hold (no device work). */
 int cfd_batch_bytes_needed(int n0, int n1, double dxi, double deta, int slots,
                            const cfd_params *params, size_t *bytes);
+
+/* ------------------------------------------------------------------ */
+/* 3b. Packs: cavity cases of uneven size sharing every launch          */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one case per process, solver.cpp:2041-2062
+ * main).  A pack is a chain of `nslabs` slabs: slab k holds slots[k] grids of
+ * ((n0-1) >> k) + 1 x ((n1-1) >> k) + 1 nodes with spacing 2^k (dxi, deta) (n0 and
+ * n1 odd down the chain).  A case whose grid is slab k's size takes one slot of
+ * slab k for its own grid and one slot of each slab below it down to its coarsest
+ * FAS level (the level a batch of its own would have, from its physics), so the
+ * slots of one slab hold the finest grids of the cases of that size beside the
+ * coarse levels of every larger case.  One outer iteration (one FAS cycle of every
+ * case) is one launch sequence down and up the chain: every SIMPLE stage of a slab
+ * is one launch over all its slots, and each case computes what it computes in a
+ * batch of its own (the same bits).  A case's own grid must not be a small level
+ * (more than 33 x 33 nodes) and its coarsest level must be one (at most 33 x 33;
+ * Re <= 1024 with the automatic depth).  Pack-wide settings come from `params` at
+ * creation (graph: 1 captures the cycle once and replays it); each case's physics
+ * from the params given to cfd_pack_add_case. */
+typedef struct cfd_pack_s *cfd_pack;
+
+int cfd_pack_create(int n0, int n1, double dxi, double deta, int nslabs, const int *slots,
+                    const cfd_params *params, cfd_pack *out);
+int cfd_pack_destroy(cfd_pack p);
+/* HBM bytes a pack of this shape would hold (no device work). */
+int cfd_pack_bytes_needed(int n0, int n1, double dxi, double deta, int nslabs, const int *slots,
+                          const cfd_params *params, size_t *bytes);
+int cfd_pack_bytes(cfd_pack p, size_t *bytes);
+/* Number of case ids (the sum of slots): statistics arrays hold this many entries. */
+int cfd_pack_capacity(cfd_pack p, int *ids);
+/* Load a case of n0 x n1 nodes (grid arrays as for cfd_case_create) into free
+ * slots of its slabs; its fields start at rest with the wall conditions and *id
+ * receives its case id.  CFD_E_FULL: a slab of its chain has no free slot;
+ * CFD_E_ARG: no slab of its size, or a hierarchy the pack cannot hold. */
+int cfd_pack_add_case(cfd_pack p, int n0, int n1, const double *x, const double *y,
+                      const double *xix, const double *xiy, const double *etx, const double *ety,
+                      const double *alph, const double *beta, const double *gamma,
+                      const double *ajac, const double *p1, const double *q1,
+                      const cfd_params *params, int *id);
+/* The case stops taking part and its slots become free (its fields stay readable
+ * until the slots are reused). */
+int cfd_pack_release(cfd_pack p, int id);
+/* n outer iterations of every case; st (may be NULL) receives one entry per case
+ * id (cfd_pack_capacity), zeros where there is no case. */
+int cfd_pack_outer(cfd_pack p, int n, cfd_stats *st);
+int cfd_pack_launch(cfd_pack p);
+int cfd_pack_finish(cfd_pack p, cfd_stats *st);
+int cfd_pack_get_fields(cfd_pack p, int id, double *u, double *v, double *T, double *p_);
 
 /* ------------------------------------------------------------------ */
 /* 4. Serving: continuous batching of cavity solves on one device      */

@@ -80,3 +80,24 @@ def test_server_queue_needs_no_gpu():
     st = C.c_int()
     assert lib.cfd_server_poll(srv._h, C.c_long(t + 99), C.byref(st), None, None, None, None, None, None) == -1
     srv.close()
+
+
+def test_pack_shapes_need_no_gpu():
+    """A pack's byte count comes from a dry build (no device work); chains that do not
+    halve, fewer than two slabs and negative slot counts are rejected."""
+    import ctypes as C
+    from cfd_solver_amd.cavity import CavityPack, default_params
+    lib = _lib.load()
+    prm = default_params(precision=1)
+    b = CavityPack.bytes_needed(129, 129, 1 / 128, 1 / 128, [2, 4, 4, 2], prm)
+    b1 = CavityPack.bytes_needed(129, 129, 1 / 128, 1 / 128, [1, 4, 4, 2], prm)
+    one = C.c_size_t()
+    lib.cfd_batch_bytes_needed(129, 129, 1 / 128, 1 / 128, 1, C.byref(default_params(precision=1, fas_levels=1)),
+                               C.byref(one))
+    # one more 129^2 slot: a single-level batch slot plus the multigrid transfer arrays
+    assert b1 > 0 and one.value <= b - b1 < 1.5 * one.value
+    h = C.c_void_p()
+    for n, slots in ((128, [1, 1]), (129, [1]), (129, [1, -1]), (9, [1, 1, 1])):
+        arr = (C.c_int * len(slots))(*slots)
+        assert lib.cfd_pack_create(n, n, 0.01, 0.01, len(slots), arr, C.byref(prm), C.byref(h)) == -1
+    assert lib.cfd_pack_release(None, 0) == -1 and lib.cfd_pack_launch(None) == -1
