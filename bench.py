@@ -304,9 +304,90 @@ def run_batched(mixed, cap=150, tol=1e-8):
                         "cases heated from below at these Rayleigh numbers often have no steady state"}
 
 
-def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=150):
-    """Poisson-arriving solves to tol, served by continuous batching, then the same
-    requests solved one at a time on the plain FP64 path."""
+def pack_slots(n_top, chains):
+    """Slots per slab of a pack whose largest slab is n_top x n_top, for cases given
+    as (n, coarsest) chains: each case takes one slot of every slab from n down to
+    its coarsest level."""
+    sizes = []
+    n = n_top
+    while n >= 5:
+        sizes.append(n)
+        if not n & 1:
+            break
+        n = (n + 1) // 2
+    lo = min(c for _, c in chains)
+    sizes = [m for m in sizes if m >= lo]
+    return sizes, [sum(1 for n, c in chains if c <= m <= n) for m in sizes]
+
+
+def run_packed(mixed, cap=150, tol=1e-8):
+    """The 256-case mix in one pack (every launch of a cycle serves every case that
+    has a grid of that size), graph replay, every case to tol or its cap."""
+    import ctypes as C
+    from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED, load
+    from cfd_solver_amd.cavity import CavityPack, cells_per_outer, default_params
+    from cfd_solver_amd.grid import cavity_grid
+    from cfd_solver_amd.workloads import batched_cases
+    cases = batched_cases()
+    L = load()
+    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1)
+    chains = []
+    for n, skew, phys in cases:
+        p = default_params(precision=prm.precision, **phys)
+        chains.append((n, L.cfd_fas_coarsest(n, n, C.byref(p))))
+    n_top = max(n for n, _ in chains)
+    sizes, slots = pack_slots(n_top, chains)
+    g0 = cavity_grid(n_top)
+    pk = CavityPack(n_top, n_top, g0.dxi, g0.deta, slots, prm)
+    ids, cells = {}, {}
+    for idx, (n, skew, phys) in enumerate(cases):
+        cid = pk.add_case(cavity_grid(n, skew=skew), **phys)
+        ids[cid] = idx
+        cells[cid] = cells_per_outer(n, n, default_params(**phys))
+    live = set(ids)
+    states, cycles = {}, {}
+    cell_its = 0.0
+    step = 0
+    t0 = time.perf_counter()
+    while live:
+        step += 1
+        pk.launch()
+        st = pk.finish()
+        for cid in list(live):
+            s = st[cid]
+            cell_its += cells[cid]
+            fin = None
+            if not np.isfinite(s.mass_l1) or not np.isfinite(s.energy_l1):
+                fin = "failed"
+            elif s.mass_res <= tol and s.energy_res <= tol:
+                fin = "done"
+            elif s.iter >= cap:
+                fin = "notconv"
+            if fin:
+                states[ids[cid]] = fin
+                cycles[ids[cid]] = s.iter
+                live.discard(cid)
+                pk.release(cid)
+    el = time.perf_counter() - t0
+    nbytes = pk.nbytes()
+    pk.close()
+    counts = {k: sum(1 for v in states.values() if v == k) for k in ("done", "notconv", "failed")}
+    done_cycles = [cycles[i] for i, v in states.items() if v == "done"]
+    return {"value": cell_its / el, "unit": "cell-iter/s", "seconds": el, "cases": len(cases), "states": counts,
+            "cycles_done_median": float(np.median(done_cycles)) if done_cycles else None,
+            "cycle_cap": cap, "tolerance": tol, "launch_rounds": step, "hbm_bytes": nbytes,
+            "slabs": {str(m): k for m, k in zip(sizes, slots)},
+            "precision": "mixed" if mixed else "fp64",
+            "workload": "the same 256 cases as one pack: slabs of 513, 257, ..., 17 nodes per side, each case "
+                        "holding a slot in every slab from its own size down to its coarsest multigrid level; "
+                        "one FAS cycle of every live case per launch round (one captured graph), each case to "
+                        "mass and energy residual 1e-8 or 150 cycles"}
+
+
+def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=150, packed=False, solo=True):
+    """Poisson-arriving solves to tol, served by continuous batching (size-class
+    batches, or one pack when packed), then the same requests solved one at a time
+    on the plain FP64 path."""
     from cfd_solver_amd._lib import CFD_FP64, CFD_MIXED
     from cfd_solver_amd.cavity import CavityCase, default_params
     from cfd_solver_amd.grid import cavity_grid
@@ -314,8 +395,17 @@ def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=
     from cfd_solver_amd.workloads import served_cases
     arrivals = served_cases(count, lam, kmin=kmin, kmax=kmax, re=(50.0, 1000.0), ri=(0.1, 10.0), pr=(0.71, 7.0))
     grids = [cavity_grid(n, skew=skew) for _, n, skew, _ in arrivals]
-    srv = CavityServer(default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1), hbm_budget=96 << 30,
-                       slots_per_batch=slots)
+    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1)
+    if packed:
+        # slabs 2^kmax + 1 down to 17: `slots` cases of each size at once, every larger
+        # case's chain through the smaller slabs on top
+        top = 2 ** kmax + 1
+        g0 = cavity_grid(top)
+        nsl = kmax - 3
+        pslots = [slots * min(k + 1, kmax - kmin + 1) for k in range(nsl)]
+        srv = CavityServer(prm, pack=(top, top, g0.dxi, g0.deta, pslots))
+    else:
+        srv = CavityServer(prm, hbm_budget=96 << 30, slots_per_batch=slots)
     # warm a graph of every size class outside the timed run
     warm = {srv.submit(cavity_grid(2 ** k + 1), tol=tol, max_outer=2) for k in range(kmin, kmax + 1)}
     while warm:
@@ -347,8 +437,14 @@ def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=
                 its.append(r["stats"].iter)
                 states[r["state"]] = states.get(r["state"], 0) + 1
     el = time.perf_counter() - t0
+    hbm = srv.info()["bytes"]
     srv.close()
     lat = np.sort(np.array(lat))
+    if not solo:
+        return {"solves_per_s": count / el, "latency_p50_s": float(np.percentile(lat, 50)),
+                "latency_p99_s": float(np.percentile(lat, 99)), "mean_outer_iterations": float(np.mean(its)),
+                "requests": count, "arrival_rate_per_s": lam, "busy_fraction": busy / el, "states": states,
+                "elapsed_s": el, "hbm_bytes": hbm, "pack_slots": pslots if packed else None}
     # the same requests, one at a time, plain FP64 path (no batching, no graphs), each
     # to the same tolerance or cap, back to back
     t1 = time.perf_counter()
@@ -368,6 +464,7 @@ def run_served(mixed, lam, count, kmin=6, kmax=10, tol=1e-8, slots=8, max_outer=
             "requests": count, "arrival_rate_per_s": lam, "busy_fraction": busy / el, "states": states,
             "elapsed_s": el, "tolerance": tol, "precision": "mixed" if mixed else "fp64", "sizes": sizes,
             "one_at_a_time_fp64": {"solves_per_s": count / solo, "seconds": solo, "states": solo_states},
+            "hbm_bytes": hbm,
             "workload": f"Poisson arrivals (seed 5) at {lam}/s, n log-uniform over 2^k+1, k in [{kmin}, {kmax}], "
                         "A ~ U[0, 0.15], Re ~ U[50, 1000], Ri log-uniform [0.1, 10], Pr in {0.71, 7}; each solve "
                         f"to mass and energy residual {tol:g} or {max_outer} cycles; {slots} slots per batch class"}
@@ -507,12 +604,19 @@ def main():
     if "main" in sections:
         cfg["solve"] = run_solve(grid)
     if "batch" in sections:
-        cfg["batched"] = run_batched(main_prec == "mixed")
-        print(f"[bench] batched: {cfg['batched']['seconds']:.2f} s {cfg['batched']['states']}", file=sys.stderr,
+        cfg["batched"] = run_packed(main_prec == "mixed")
+        print(f"[bench] packed: {cfg['batched']['seconds']:.2f} s {cfg['batched']['states']}", file=sys.stderr,
               flush=True)
+        cfg["batched_by_size"] = run_batched(main_prec == "mixed")
+        print(f"[bench] batched by size: {cfg['batched_by_size']['seconds']:.2f} s "
+              f"{cfg['batched_by_size']['states']}", file=sys.stderr, flush=True)
     if "serve" in sections:
         cfg["served"] = run_served(main_prec == "mixed", a.serve_rate, a.serve_count)
         print(f"[bench] served: {cfg['served']['solves_per_s']:.2f} solves/s", file=sys.stderr, flush=True)
+        cfg["served"]["packed_server"] = run_served(main_prec == "mixed", a.serve_rate, a.serve_count, packed=True,
+                                                    solo=False)
+        print(f"[bench] served from one pack: {cfg['served']['packed_server']['solves_per_s']:.2f} solves/s",
+              file=sys.stderr, flush=True)
     if "piso" in sections:
         cfg["p_isolate"] = run_p_isolate()
         print("[bench] p' isolate done", file=sys.stderr, flush=True)

@@ -131,6 +131,7 @@ SIGNATURES = {
     "cfd_pack_finish": ([_p, _p], _i),
     "cfd_pack_get_fields": ([_p, _i, dp, dp, dp, dp], _i),
     "cfd_server_create": ([C.POINTER(Params), C.c_size_t, _i, C.POINTER(_p)], _i),
+    "cfd_server_create_packed": ([C.POINTER(Params), _i, _i, _d, _d, _i, C.POINTER(_i), C.POINTER(_p)], _i),
     "cfd_server_destroy": ([_p], _i),
     "cfd_server_submit": ([_p, _i, _i, _d, _d] + [dp] * 10 + [_p, _p, C.POINTER(Params), _d, _i,
                           C.POINTER(C.c_long)], _i),

@@ -16,6 +16,12 @@
 // Slots of a batch are independent (own arrays, scalars and physics), so a
 // request's result does not depend on what else is being served.
 //
+// Packed mode (cfd_server_create_packed): the server holds one pack (cfd_pack_*)
+// instead of size classes; a request takes free slots of the pack's slabs from its
+// grid size down to its coarsest level (it waits while they are taken) and every
+// scheduling round is one FAS cycle of the whole pack, so requests of every size
+// share each launch.
+//
 // The reference serves nothing (one hard-coded case per process,
 // solver.cpp:2041-2062); this is the serving layer of the batched path.
 #include <algorithm>
@@ -73,6 +79,11 @@ struct Server {
     size_t budget = 0;
     int slots = 0;
     std::vector<Class> classes;
+    // packed mode: the pack, its statistics per case id and the request per case id
+    cfd_pack pack = nullptr;
+    size_t pack_bytes = 0;
+    std::vector<cfd_stats> pst;
+    std::vector<long> powner;
     std::deque<long> queue;
     std::map<long, Request> reqs;
     long next = 1;
@@ -130,22 +141,55 @@ int make_class(Server &s, const Request &r) {
 }
 
 void finish(Server &s, Request &r, int state) {
-    Class &c = s.classes[r.cls];
     r.state = state;
     r.t_done = clk::now();
-    r.st = c.st[r.slot];
     const size_t N = (size_t)r.n0 * r.n1;
     r.fields.assign(4 * N, 0.0);
     double *f = r.fields.data();
-    if (cfd_batch_get_fields(c.b, r.slot, f, f + N, f + 2 * N, f + 3 * N) != CFD_OK) r.state = CFD_REQ_FAILED;
-    cfd_batch_release(c.b, r.slot);
-    c.owner[r.slot] = 0;
+    if (s.pack) {
+        r.st = s.pst[r.slot];
+        if (cfd_pack_get_fields(s.pack, r.slot, f, f + N, f + 2 * N, f + 3 * N) != CFD_OK) r.state = CFD_REQ_FAILED;
+        cfd_pack_release(s.pack, r.slot);
+        s.powner[r.slot] = 0;
+    } else {
+        Class &c = s.classes[r.cls];
+        r.st = c.st[r.slot];
+        if (cfd_batch_get_fields(c.b, r.slot, f, f + N, f + 2 * N, f + 3 * N) != CFD_OK) r.state = CFD_REQ_FAILED;
+        cfd_batch_release(c.b, r.slot);
+        c.owner[r.slot] = 0;
+    }
     r.grid.clear();
     r.grid.shrink_to_fit();
     s.finished++;
 }
 
+// packed mode: a request takes its chain of pack slots, or waits while they are taken
+void admit_packed(Server &s) {
+    for (auto it = s.queue.begin(); it != s.queue.end();) {
+        Request &r = s.reqs[*it];
+        const size_t N = (size_t)r.n0 * r.n1;
+        const double *g = r.grid.data();
+        int id = -1;
+        const int rc = cfd_pack_add_case(s.pack, r.n0, r.n1, g, g + N, g + 2 * N, g + 3 * N, g + 4 * N, g + 5 * N,
+                                         g + 6 * N, g + 7 * N, g + 8 * N, g + 9 * N,
+                                         r.has_p1 ? g + 10 * N : nullptr, r.has_q1 ? g + 11 * N : nullptr, &r.phys,
+                                         &id);
+        if (rc == CFD_E_FULL) { ++it; continue; }
+        if (rc != CFD_OK) {
+            r.state = CFD_REQ_FAILED;
+            r.t_done = clk::now();
+            s.finished++;
+        } else {
+            r.state = CFD_REQ_RUNNING;
+            r.slot = id;
+            s.powner[id] = r.id;
+        }
+        it = s.queue.erase(it);
+    }
+}
+
 void admit(Server &s) {
+    if (s.pack) return admit_packed(s);
     for (auto it = s.queue.begin(); it != s.queue.end();) {
         Request &r = s.reqs[*it];
         const int depth = coarsest_of(s, r);
@@ -201,9 +245,30 @@ int cfd_server_create(const cfd_params *params, size_t hbm_budget, int slots_per
     return CFD_OK;
 }
 
+int cfd_server_create_packed(const cfd_params *params, int n0, int n1, double dxi, double deta, int nslabs,
+                             const int *slots, cfd_server *out) {
+    if (!params || !out) return CFD_E_ARG;
+    cfd_pack pk = nullptr;
+    const int rc = cfd_pack_create(n0, n1, dxi, deta, nslabs, slots, params, &pk);
+    if (rc != CFD_OK) return rc;
+    Server *s = new Server();
+    s->prm = *params;
+    s->pack = pk;
+    int ids = 0;
+    cfd_pack_capacity(pk, &ids);
+    cfd_pack_bytes(pk, &s->pack_bytes);
+    s->budget = s->pack_bytes;
+    s->used = s->pack_bytes;
+    s->pst.assign(ids, cfd_stats{});
+    s->powner.assign(ids, 0);
+    *out = reinterpret_cast<cfd_server>(s);
+    return CFD_OK;
+}
+
 int cfd_server_destroy(cfd_server h) {
     Server *s = reinterpret_cast<Server *>(h);
     if (!s) return CFD_OK;
+    if (s->pack) cfd_pack_destroy(s->pack);
     for (Class &c : s->classes) cfd_batch_destroy(c.b);
     delete s;
     return CFD_OK;
@@ -242,6 +307,30 @@ int cfd_server_step(cfd_server h, int *running, int *queued) {
     Server *s = reinterpret_cast<Server *>(h);
     if (!s) return CFD_E_ARG;
     admit(*s);
+    if (s->pack) {
+        int run = 0;
+        for (long o : s->powner) run += o != 0;
+        int rc = CFD_OK;
+        if (run > 0) {
+            rc = cfd_pack_launch(s->pack);
+            if (rc == CFD_OK) rc = cfd_pack_finish(s->pack, s->pst.data());
+            for (size_t id = 0; rc == CFD_OK && id < s->powner.size(); id++) {
+                if (s->powner[id] == 0) continue;
+                Request &r = s->reqs[s->powner[id]];
+                const cfd_stats &st = s->pst[id];
+                if (!std::isfinite(st.mass_l1) || !std::isfinite(st.energy_l1)) finish(*s, r, CFD_REQ_FAILED);
+                else if (st.mass_res <= r.tol && st.energy_res <= r.tol) finish(*s, r, CFD_REQ_DONE);
+                else if (st.iter >= r.max_outer) finish(*s, r, CFD_REQ_NOTCONV);
+            }
+        }
+        if (running) {
+            int a = 0;
+            for (long o : s->powner) a += o != 0;
+            *running = a;
+        }
+        if (queued) *queued = (int)s->queue.size();
+        return rc;
+    }
     std::vector<size_t> live;
     for (size_t k = 0; k < s->classes.size(); k++)
         if (s->classes[k].active() > 0) live.push_back(k);
@@ -294,7 +383,7 @@ int cfd_server_poll(cfd_server h, long ticket, int *state, cfd_stats *st, double
 int cfd_server_info(cfd_server h, int *batches, size_t *bytes, long *finished) {
     Server *s = reinterpret_cast<Server *>(h);
     if (!s) return CFD_E_ARG;
-    if (batches) *batches = (int)s->classes.size();
+    if (batches) *batches = s->pack ? 1 : (int)s->classes.size();
     if (bytes) *bytes = s->used;
     if (finished) *finished = s->finished;
     return CFD_OK;

@@ -17,11 +17,20 @@ class CavityServer:
     """Queue solves with submit(), advance all running solves one outer iteration
     per step(), collect finished ones with poll()."""
 
-    def __init__(self, params: Params | None = None, hbm_budget: int = 64 << 30, slots_per_batch: int = 16, **kw):
+    def __init__(self, params: Params | None = None, hbm_budget: int = 64 << 30, slots_per_batch: int = 16,
+                 pack=None, **kw):
+        """pack: None (size-class batches under hbm_budget) or (n0, n1, dxi, deta, slots):
+        one pack of those slabs (cfd_server_create_packed)."""
         self.params = params if params is not None else default_params(**kw)
         self._h = C.c_void_p()
-        call("cfd_server_create", C.byref(self.params), C.c_size_t(int(hbm_budget)), int(slots_per_batch),
-             C.byref(self._h))
+        if pack is None:
+            call("cfd_server_create", C.byref(self.params), C.c_size_t(int(hbm_budget)), int(slots_per_batch),
+                 C.byref(self._h))
+        else:
+            n0, n1, dxi, deta, slots = pack
+            arr = (C.c_int * len(slots))(*slots)
+            call("cfd_server_create_packed", C.byref(self.params), n0, n1, dxi, deta, len(slots), arr,
+                 C.byref(self._h))
         self._shape = {}
 
     def close(self):

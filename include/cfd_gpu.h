@@ -519,6 +519,12 @@ typedef struct cfd_server_s *cfd_server;
 
 int cfd_server_create(const cfd_params *params, size_t hbm_budget, int slots_per_batch,
                       cfd_server *out);
+/* Packed mode: the server holds one pack (cfd_pack_create's arguments; its bytes
+ * are the server's HBM use) and every request takes free slots of it (a request
+ * waits while its slabs are full; one the pack cannot hold fails); each
+ * cfd_server_step is one outer iteration of the whole pack. */
+int cfd_server_create_packed(const cfd_params *params, int n0, int n1, double dxi, double deta, int nslabs,
+                             const int *slots, cfd_server *out);
 int cfd_server_destroy(cfd_server s);
 /* Queue a solve (the arrays are copied); *ticket identifies it. */
 int cfd_server_submit(cfd_server s, int n0, int n1, double dxi, double deta,

@@ -94,3 +94,40 @@ def test_server_iteration_cap():
     r = _drain(srv, [t])[t]
     assert r["state"] == "notconv" and r["stats"].iter == 5
     srv.close()
+
+
+def test_packed_server_alone_equals_in_mix():
+    print("\n[gpu] packed server: a request alone vs arriving into a full mix of sizes", flush=True)
+    from cfd_solver_amd.serve import CavityServer
+    g0 = cavity_grid(129)
+    shape = (129, 129, g0.dxi, g0.deta, [2, 3, 4, 2])   # slabs 129, 65, 33, 17
+    ga = cavity_grid(65, skew=0.05)
+    pa = dict(Re=100.0, Ri=1.0, Pr=0.71)
+    alone = CavityServer(pack=shape)
+    ta = alone.submit(ga, tol=1e-6, max_outer=300, **pa)
+    ra = _drain(alone, [ta])[ta]
+    alone.close()
+    assert ra["state"] == "done" and ra["stats"].mass_res <= 1e-6
+
+    mix = CavityServer(pack=shape)
+    ts = [mix.submit(cavity_grid(129, skew=0.1), tol=1e-6, max_outer=60, Re=150.0, Ri=1.0, Pr=0.71),
+          mix.submit(cavity_grid(129, skew=0.02, beta=1.3), tol=1e-6, max_outer=40, Re=600.0, Ri=0.2, Pr=7.0),
+          mix.submit(cavity_grid(65, skew=0.12), tol=1e-6, max_outer=30, Re=80.0, Ri=4.0, Pr=7.0)]
+    for _ in range(3):
+        mix.step()
+    assert mix.info()["batches"] == 1
+    t3 = mix.submit(ga, tol=1e-6, max_outer=300, **pa)   # waits: the 65 slab is full
+    t4 = mix.submit(cavity_grid(129), tol=1e-6, max_outer=20, Re=100.0)
+    res = _drain(mix, ts + [t3, t4])
+    rb = res[t3]
+    assert rb["state"] == "done"
+    assert rb["stats"].iter == ra["stats"].iter and rb["stats"].mass_l1 == ra["stats"].mass_l1
+    for k in ("u", "v", "T", "p"):
+        assert np.array_equal(rb["fields"][k], ra["fields"][k]), k
+    print("  others:", [res[t]["state"] for t in ts + [t4]], flush=True)
+    assert all(res[t]["state"] in ("done", "notconv", "failed") for t in ts + [t4])
+    # a request of a size the pack has no slab for fails at once
+    tb = mix.submit(cavity_grid(97), tol=1e-6, max_outer=10)
+    mix.step()
+    assert mix.poll(tb)["state"] == "failed"
+    mix.close()
