@@ -16,6 +16,7 @@ and the converged mixed fields are compared with the FP64 ones per field.
 
     python bench.py [--steps K] [--warmup W] [--n 1025] [--no-cpu]
                     [--sections main,batch,serve,piso,ogrid]
+                    (pack: the packed 256-case mix alone; none: the timed cycles only)
 
 Prints one JSON line.  roofline: the fine-level kernel with the largest device
 time per step, timed live with HIP events on the case's stream; its algorithmic
@@ -103,12 +104,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1025)
-    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--sections", default="main,batch,serve,piso,ogrid")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="served arrivals per second")
     ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--pack-graph", type=int, default=1, help="1: replay the pack's cycle as a captured graph; "
+                    "0: launch it eagerly (same kernels and results)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (the known-size kernel that scales "
                          "rocprof's FETCH_SIZE / WRITE_SIZE counters)")
@@ -127,20 +130,23 @@ def host_model():
 
 def cpu_baseline(grid, n_cycles):
     """The CPU oracle (serial FP64 C, gcc -O3) on the same case: one untimed cycle, then
-    n_cycles timed; cell-iterations counted as on the GPU."""
+    n_cycles cycles timed one by one (median); cell-iterations counted as on the GPU."""
     from cfd_solver_amd.cavity import cells_per_outer, default_params
     from oracle.oracle import CavityOracle, build, cav_params
     build()
     c = CavityOracle(grid, cav_params(**PHYS))
     c.step()
-    t0 = time.perf_counter()
+    runs = []
     for _ in range(n_cycles):
+        t0 = time.perf_counter()
         c.step()
-    dt = time.perf_counter() - t0
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
     cells = cells_per_outer(grid.n0, grid.n1, default_params(**PHYS))
-    return {"value": cells * n_cycles / dt, "unit": "cell-iter/s", "cores": 1, "kind": "port",
-            "sample": f"{n_cycles} FAS cycles (after one untimed) of the same {grid.n0}x{grid.n1} case, serial FP64 "
-                      f"C oracle (gcc -O3), 1 of {os.cpu_count()} host threads ({host_model()})"}
+    return {"value": cells / dt, "unit": "cell-iter/s", "cores": 1, "kind": "port",
+            "sample": f"median of {n_cycles} FAS cycles timed one by one (after one untimed) of the same "
+                      f"{grid.n0}x{grid.n1} case, serial FP64 C oracle (gcc -O3), 1 of {os.cpu_count()} host "
+                      f"threads ({host_model()})", "cycle_seconds": [round(r, 3) for r in runs]}
 
 
 def run_gpu(grid, mixed, steps, warmup, prof_steps=3):
@@ -216,6 +222,8 @@ def rel_l2(a, b):
 
 
 def run_solve(grid):
+    """FP64 and mixed solves of one case from rest to 1e-8, and the per-field relative
+    L2 of the mixed fields against the FP64 ones."""
     out = {}
     fields = {}
     for prec in ("fp64", "mixed"):
@@ -320,7 +328,7 @@ def pack_slots(n_top, chains):
     return sizes, [sum(1 for n, c in chains if c <= m <= n) for m in sizes]
 
 
-def run_packed(mixed, cap=150, tol=1e-8):
+def run_packed(mixed, cap=150, tol=1e-8, graph=1):
     """The 256-case mix in one pack (every launch of a cycle serves every case that
     has a grid of that size), graph replay, every case to tol or its cap."""
     import ctypes as C
@@ -330,7 +338,8 @@ def run_packed(mixed, cap=150, tol=1e-8):
     from cfd_solver_amd.workloads import batched_cases
     cases = batched_cases()
     L = load()
-    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=1)
+    prm = default_params(precision=CFD_MIXED if mixed else CFD_FP64, graph=graph)
+    from cfd_solver_amd.cavity import fas_hierarchy
     chains = []
     for n, skew, phys in cases:
         p = default_params(precision=prm.precision, **phys)
@@ -340,19 +349,26 @@ def run_packed(mixed, cap=150, tol=1e-8):
     g0 = cavity_grid(n_top)
     pk = CavityPack(n_top, n_top, g0.dxi, g0.deta, slots, prm)
     ids, cells = {}, {}
+    mix_bytes = 0.0  # algorithmic bytes of one cycle of every case (stage-kernel levels)
     for idx, (n, skew, phys) in enumerate(cases):
         cid = pk.add_case(cavity_grid(n, skew=skew), **phys)
         ids[cid] = idx
         cells[cid] = cells_per_outer(n, n, default_params(**phys))
+        for a, b, k in fas_hierarchy(n, n, default_params(**phys)):
+            if a > 33:  # levels of at most 33 x 33 nodes run from LDS in one workgroup
+                mix_bytes += a * b * k * simple_bytes_per_node(mixed, 2)
     live = set(ids)
     states, cycles = {}, {}
     cell_its = 0.0
     step = 0
+    round_ms = []
     t0 = time.perf_counter()
     while live:
         step += 1
+        tr = time.perf_counter()
         pk.launch()
         st = pk.finish()
+        round_ms.append((time.perf_counter() - tr) * 1e3)
         for cid in list(live):
             s = st[cid]
             cell_its += cells[cid]
@@ -377,6 +393,12 @@ def run_packed(mixed, cap=150, tol=1e-8):
             "cycles_done_median": float(np.median(done_cycles)) if done_cycles else None,
             "cycle_cap": cap, "tolerance": tol, "launch_rounds": step, "hbm_bytes": nbytes,
             "slabs": {str(m): k for m, k in zip(sizes, slots)},
+            "full_mix_cycle_ms": float(np.median(round_ms[1:4])),
+            "full_mix_roofline": {
+                "bytes_model": round(mix_bytes), "GBs": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9, 1),
+                "frac": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "note": "cycles 2-4 (every case live); algorithmic bytes of the SIMPLE iterations on levels above "
+                        "33 x 33 (two BiCGStab iterations each) over the cycle's wall time"},
             "precision": "mixed" if mixed else "fp64",
             "workload": "the same 256 cases as one pack: slabs of 513, 257, ..., 17 nodes per side, each case "
                         "holding a slot in every slab from its own size down to its coarsest multigrid level; "
@@ -603,10 +625,15 @@ def main():
         cfg["mixed_over_fp64"] = res["mixed"][0] / res["fp64"][0]
     if "main" in sections:
         cfg["solve"] = run_solve(grid)
-    if "batch" in sections:
-        cfg["batched"] = run_packed(main_prec == "mixed")
+        if n > 513:
+            cfg["solve_513"] = run_solve(cavity_grid((n + 1) // 2, skew=0.1, beta=1.5))
+            print(f"[bench] 513 solves: {cfg['solve_513']['fp64']['cycles']} / {cfg['solve_513']['mixed']['cycles']} "
+                  "cycles", file=sys.stderr, flush=True)
+    if "batch" in sections or "pack" in sections:
+        cfg["batched"] = run_packed(main_prec == "mixed", graph=a.pack_graph)
         print(f"[bench] packed: {cfg['batched']['seconds']:.2f} s {cfg['batched']['states']}", file=sys.stderr,
               flush=True)
+    if "batch" in sections:
         cfg["batched_by_size"] = run_batched(main_prec == "mixed")
         print(f"[bench] batched by size: {cfg['batched_by_size']['seconds']:.2f} s "
               f"{cfg['batched_by_size']['states']}", file=sys.stderr, flush=True)

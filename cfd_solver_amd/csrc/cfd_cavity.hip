@@ -385,6 +385,43 @@ __device__ __forceinline__ JNode<TR> jac_load(const JArgs<TR> &a, size_t P) {
 
 // one sweep at interior node (i, j); `dat(f, i, j)` reads the old correction of
 // field f (0 u, 1 v, 2 T) at a neighbour
+// the node's A_lo rows (u and v; T) and relaxed inverse diagonals
+template <typename TR>
+struct JRow {
+    TR cu[9], ct[9], iu, it;
+};
+template <typename TR>
+__device__ __forceinline__ JRow<TR> jac_row(const JNode<TR> &c, const SlotP &ph, const RecipT<TR> &r) {
+    JRow<TR> w;
+    const TR dt = (TR)ph.dt;
+    alo_coefs_r<TR>(c.al, c.be, c.ga, c.pp, c.qq, c.U, c.V, r, dt, (TR)ph.dnu, w.cu);
+    alo_coefs_r<TR>(c.al, c.be, c.ga, c.pp, c.qq, c.U, c.V, r, dt, (TR)ph.dnt, w.ct);
+    w.iu = (TR)ph.alpha_u / w.cu[4];
+    w.it = (TR)ph.alpha_t / w.ct[4];
+    return w;
+}
+// one sweep at interior node (i, j) from the node's rows
+template <typename TR, bool FIRST, typename F>
+__device__ __forceinline__ void jac_node_w(const JNode<TR> &c, const JRow<TR> &w, int i, int j, F dat, TR &ou,
+                                           TR &ov, TR &ot) {
+    TR su = c.ru, sv = c.rv, st = c.rt;
+    if (!FIRST) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+            for (int di = -1; di <= 1; di++) {
+                if (di == 0 && dj == 0) continue;
+                const int k = (dj + 1) * 3 + (di + 1);
+                su -= w.cu[k] * dat(0, i + di, j + dj);
+                sv -= w.cu[k] * dat(1, i + di, j + dj);
+                st -= w.ct[k] * dat(2, i + di, j + dj);
+            }
+    }
+    ou = su * w.iu;
+    ov = sv * w.iu;
+    ot = st * w.it;
+}
+
 template <typename TR, bool FIRST, typename F>
 __device__ __forceinline__ void jac_node(const JNode<TR> &c, const SlotP &ph, const RecipT<TR> &r, int i, int j,
                                          F dat, TR &ou, TR &ov, TR &ot) {
@@ -924,10 +961,10 @@ struct SmallArgs {
     Geom lev[CW_LEVS];      // the p' levels' metrics (level 0 is this level)
 };
 
-// totals of NV values over the workgroup, in every thread
-template <int NV>
+// totals of NV values over the workgroup of NT threads, in every thread
+template <int NT, int NV>
 __device__ __forceinline__ void sl_sums(double (&v)[NV]) {
-    __shared__ double red[NV][SL_NT / 64];
+    __shared__ double red[NV][NT / 64];
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
     if ((threadIdx.x & 63) == 0) {
@@ -938,7 +975,7 @@ __device__ __forceinline__ void sl_sums(double (&v)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double t = 0.0;
-        for (int q = 0; q < SL_NT / 64; q++) t += red[k][q];
+        for (int q = 0; q < NT / 64; q++) t += red[k][q];
         v[k] = t;
     }
     __syncthreads();
@@ -970,6 +1007,7 @@ struct SlNode {
 // zero), u* = u + du into SL_U / SL_V (no stage before the correction reads the
 // old u at a neighbour), T += dT, and u-hat, v-hat into SL_A0 / SL_A1.  Ends with
 // a block barrier.
+template <int NT>
 __device__ __forceinline__ void sl_predict(const SmallArgs *__restrict__ a, SlNode t) {
     const CGrid &g = a->g;
     const SlotP ph = g.sp[blockIdx.z];
@@ -990,7 +1028,7 @@ __device__ __forceinline__ void sl_predict(const SmallArgs *__restrict__ a, SlNo
     }
     double *da[3] = {sl_arr(SL_A0), sl_arr(SL_A1), sl_arr(SL_A2)};
     double *db[3] = {sl_arr(SL_B0), sl_arr(SL_B1), sl_arr(SL_B2)};
-    for (int b = tid; b < 2 * (n0 + n1) - 4; b += SL_NT) {
+    for (int b = tid; b < 2 * (n0 + n1) - 4; b += NT) {
         const int k = sl_border(b, n0, n1);
 #pragma unroll
         for (int f = 0; f < 3; f++) da[f][k] = db[f][k] = 0.0;
@@ -998,12 +1036,13 @@ __device__ __forceinline__ void sl_predict(const SmallArgs *__restrict__ a, SlNo
     const int sweeps = a->sweeps;
     if (sweeps <= 0 && t.own) da[0][P] = da[1][P] = da[2][P] = 0.0;
     const JNode<double> jn{c.al, c.be, c.ga, c.pp, c.qq, ro.U, ro.V, ro.ru, ro.rv, ro.rt};
+    const JRow<double> jw = jac_row<double>(jn, ph, g.r);  // the rows, once for all sweeps
     for (int sw = 0; sw < sweeps; sw++) {
         if (t.own) {
             auto dat = [&](int f, int ii, int jj) { return da[f][jj * n0 + ii]; };
             double ou, ov, ot;
-            if (sw == 0) jac_node<double, true>(jn, ph, g.r, t.i, t.j, dat, ou, ov, ot);
-            else jac_node<double, false>(jn, ph, g.r, t.i, t.j, dat, ou, ov, ot);
+            if (sw == 0) jac_node_w<double, true>(jn, jw, t.i, t.j, dat, ou, ov, ot);
+            else jac_node_w<double, false>(jn, jw, t.i, t.j, dat, ou, ov, ot);
             db[0][P] = ou;
             db[1][P] = ov;
             db[2][P] = ot;
@@ -1061,6 +1100,7 @@ __device__ __forceinline__ void sl_face_div(const SmallArgs *__restrict__ a, SlN
 // Stage 6a: the face coefficients and inverse diagonals of every p' level from the
 // level's metrics and d injected from this level (k_faces_all's arithmetic), into
 // the level tables (global memory).  Ends with a block barrier.
+template <int NT>
 __device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) {
     const double *const D_ = sl_arr(SL_D);
     const int n0 = a->g.n0, last = a->mg.nlev - 1;
@@ -1072,7 +1112,7 @@ __device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) 
         const int ln0 = gl.n0, ln1 = gl.n1;
         const size_t ol = zslot() * (size_t)ln0 * ln1;
         const double r1 = 1.0 / (gl.h1 * gl.h1), r2 = 1.0 / (gl.h2 * gl.h2), r12 = 1.0 / (4.0 * gl.h1 * gl.h2);
-        for (int k = threadIdx.x; k < ln0 * ln1; k += SL_NT) {
+        for (int k = threadIdx.x; k < ln0 * ln1; k += NT) {
             const int li = k % ln0, lj = k / ln0;
             auto fn = [&](int ii, int jj) {
                 const int ic = clampi(ii, 0, ln0 - 1), jc = clampi(jj, 0, ln1 - 1);
@@ -1115,6 +1155,7 @@ __device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) 
 // level in registers (as k_mg_coarse holds them) and the vectors at the own node in
 // the free work arrays; p' with its interior mean removed goes to SL_A1.  Ends with
 // a block barrier.
+template <int NT>
 __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNode t) {
     const int n0 = a->g.n0, P = t.P;
     const bool own = t.own;
@@ -1129,7 +1170,7 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
     double *const VV = sl_arr(SL_A2), *const X = sl_arr(SL_A0);
     const double bq = own ? R[P] : 0.0;  // the face divergence
     double ms[1] = {bq};
-    sl_sums<1>(ms);
+    sl_sums<NT, 1>(ms);
     const double r = own ? bq - ms[0] * inv_m : 0.0;
     if (own) {
         R[P] = r;
@@ -1137,7 +1178,7 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
         X[P] = 0.0;
     }
     double nb[2] = {bq * bq, r * r};
-    sl_sums<2>(nb);
+    sl_sums<NT, 2>(nb);
     const double bnorm = sqrt(nb[0]), tol = a->p_rtol * bnorm;
     if (bnorm != 0.0 && sqrt(nb[1]) > tol) {
         const double *z = sl_mg + a->mg.off[0];  // the V-cycle's result on this level
@@ -1145,7 +1186,7 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
         double rho_old = 1.0, alpha = 1.0, omega = 1.0;
         for (int it = 1; it <= maxit; it++) {
             double t1[1] = {own ? R0[P] * R[P] : 0.0};
-            sl_sums<1>(t1);
+            sl_sums<NT, 1>(t1);
             const double rho = t1[0];
             if (rho == 0.0) break;
             double pv = 0.0;
@@ -1154,26 +1195,26 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
                 PV[P] = pv;
             }
             nd[0].b = pv;
-            cw_vcycle<double>(a->mg, 0, nd, sl_mg);
+            cw_vcycle<double, NT>(a->mg, 0, nd, sl_mg);
             const double phv = own ? z[P] : 0.0;
             const double vv = own ? cw_row<double>(nd[0], z, n0) : 0.0;
             if (own) VV[P] = vv;
             double t2[1] = {own ? R0[P] * vv : 0.0};
-            sl_sums<1>(t2);
+            sl_sums<NT, 1>(t2);
             alpha = rho / t2[0];
             const double sv = own ? R[P] - alpha * vv : 0.0;
             double t3[1] = {sv * sv};
-            sl_sums<1>(t3);
+            sl_sums<NT, 1>(t3);
             if (sqrt(t3[0]) <= tol) {
                 if (own) X[P] = X[P] + alpha * phv;
                 break;
             }
             nd[0].b = sv;
-            cw_vcycle<double>(a->mg, 0, nd, sl_mg);
+            cw_vcycle<double, NT>(a->mg, 0, nd, sl_mg);
             const double shv = own ? z[P] : 0.0;
             const double tv = own ? cw_row<double>(nd[0], z, n0) : 0.0;
             double t4[2] = {tv * tv, tv * sv};
-            sl_sums<2>(t4);
+            sl_sums<NT, 2>(t4);
             omega = t4[0] > 0.0 ? t4[1] / t4[0] : 0.0;
             const double rn = sv - omega * tv;
             if (own) {
@@ -1181,14 +1222,14 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
                 R[P] = rn;
             }
             double t5[1] = {rn * rn};
-            sl_sums<1>(t5);
+            sl_sums<NT, 1>(t5);
             if (sqrt(t5[0]) <= tol || omega == 0.0) break;
             rho_old = rho;
         }
     }
     const double xv = own ? X[P] : 0.0;
     double xs[1] = {xv};
-    sl_sums<1>(xs);
+    sl_sums<NT, 1>(xs);
     if (own) sl_arr(SL_A1)[P] = xv - xs[0] * inv_m;
     __syncthreads();
 }
@@ -1198,6 +1239,7 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
 // old ones); the wall pressure (reads p at interior nodes only), the corners from
 // the new wall values (kept without sources in SL_B0), then the sources.  Ends
 // with a block barrier.
+template <int NT>
 __device__ __forceinline__ void sl_correct_walls(const SmallArgs *__restrict__ a, SlNode t) {
     const CGrid &g = a->g;
     const SlotP *const ph = g.sp + blockIdx.z;
@@ -1267,7 +1309,9 @@ __device__ __forceinline__ void sl_correct_walls(const SmallArgs *__restrict__ a
 
 // a: the level's arguments in device memory (scalar loads, no per-lane copy);
 // iters SIMPLE iterations, iters_deep on a slot that is its case's coarsest level
-__global__ __launch_bounds__(SL_NT) void k_simple_small(const SmallArgs *__restrict__ a, int iters, int iters_deep) {
+// NT threads per workgroup (at least the level's interior nodes, and 2 (n0 + n1))
+template <int NT>
+__global__ __launch_bounds__(NT) void k_simple_small(const SmallArgs *__restrict__ a, int iters, int iters_deep) {
     const SlotP &sp = a->g.sp[blockIdx.z];
     if (sp.active == 0.0) return;
     if (sp.deep != 0.0) iters = iters_deep;
@@ -1282,26 +1326,26 @@ __global__ __launch_bounds__(SL_NT) void k_simple_small(const SmallArgs *__restr
     t.i = 1 + qn % m0;
     t.j = 1 + qn / m0;
     t.P = t.j * n0 + t.i;
-    for (int k = tid; k < N; k += SL_NT) {
+    for (int k = tid; k < N; k += NT) {
         sl_arr(SL_U)[k] = a->u[o + k];
         sl_arr(SL_V)[k] = a->v[o + k];
         sl_arr(SL_T)[k] = a->T[o + k];
         sl_arr(SL_P)[k] = a->p[o + k];
     }
-    for (int k = tid; k < CN_MAX * CW_ARR; k += SL_NT) sl_mg[k] = 0.0;  // wall entries stay zero
+    for (int k = tid; k < CN_MAX * CW_ARR; k += NT) sl_mg[k] = 0.0;  // wall entries stay zero
     __syncthreads();
     // The stages run as the cases of one loop: a stage's values cannot stay live into
     // the next one, so the register demand is the largest stage's, not their sum.
     for (int k = 0; k < 5 * iters; k++) {
         switch (k % 5) {
-        case 0: sl_predict(a, t); break;
+        case 0: sl_predict<NT>(a, t); break;
         case 1: sl_face_div(a, t); break;  // the face tables' barrier orders it before the solve
-        case 2: sl_face_tables(a); break;
-        case 3: sl_pprime(a, t); break;
-        default: sl_correct_walls(a, t); break;
+        case 2: sl_face_tables<NT>(a); break;
+        case 3: sl_pprime<NT>(a, t); break;
+        default: sl_correct_walls<NT>(a, t); break;
         }
     }
-    for (int k = tid; k < N; k += SL_NT) {
+    for (int k = tid; k < N; k += NT) {
         a->u[o + k] = sl_arr(SL_U)[k];
         a->v[o + k] = sl_arr(SL_V)[k];
         a->T[o + k] = sl_arr(SL_T)[k];
@@ -2275,7 +2319,12 @@ int alloc_small(Case *c) {
 int simple_small(Case *c, int iters, int iters_deep) {
     if (iters <= 0 && iters_deep <= 0) return CFD_OK;
     if (!c->sl_args || !c->use_src) return CFD_E_ARG;
-    k_simple_small<<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters, iters_deep);
+    // a level of at most 256 interior nodes in four wavefronts (each with up to 512
+    // registers per lane), a larger one in sixteen
+    if ((c->n0 - 2) * (c->n1 - 2) <= 256 && 2 * (c->n0 + c->n1) <= 256)
+        k_simple_small<256><<<dim3(1, 1, c->B), 256, 0, c->s>>>(c->sl_args, iters, iters_deep);
+    else
+        k_simple_small<SL_NT><<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters, iters_deep);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }

@@ -813,13 +813,15 @@ struct CoarseArgs {
     int off[PS_MAXLEV];  // LDS offset of each level's arrays (levels >= l0)
 };
 
+// total of v over a workgroup of NTH threads, in every thread
+template <int NTH = CW>
 __device__ __forceinline__ double cw_sum(double v) {
-    __shared__ double red[CW / 64];
+    __shared__ double red[NTH / 64];
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double tot = 0.0;
-    for (int q = 0; q < CW / 64; q++) tot += red[q];
+    for (int q = 0; q < NTH / 64; q++) tot += red[q];
     __syncthreads();
     return tot;
 }
@@ -907,11 +909,12 @@ __device__ __forceinline__ void cw_row_fold(const CoarseArgs<T> &A, int l0, CNod
     }
 }
 
-// The V-cycle of levels l0 .. A.nlev - 1 in one workgroup of CW threads, from the
-// rows, inverse diagonals and level-l0 right-hand side in nd[] (nd[L].b of the
-// coarser levels is set here), iterates in the LDS arrays at sh + A.off[l]; the
-// result is level l0's iterate, complete and visible to every thread on return.
-template <typename T>
+// The V-cycle of levels l0 .. A.nlev - 1 in one workgroup of NTH threads (at least
+// the interior nodes of level l0), from the rows, inverse diagonals and level-l0
+// right-hand side in nd[] (nd[L].b of the coarser levels is set here), iterates in
+// the LDS arrays at sh + A.off[l]; the result is level l0's iterate, complete and
+// visible to every thread on return.
+template <typename T, int NTH = CW>
 __device__ __forceinline__ void cw_vcycle(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS], T *sh) {
     const int tid = threadIdx.x;
     const T w = A.omega;
@@ -1006,7 +1009,7 @@ __device__ __forceinline__ void cw_vcycle(const CoarseArgs<T> &A, int l0, CNode<
             __syncthreads();
             CNode<T> &cc = nd[L + 1];
             const T v = restrict_to(r, n0, cc);
-            const double mean = cw_sum(cc.k >= 0 ? (double)v : 0.0) / (double)INNER(L + 1);
+            const double mean = cw_sum<NTH>(cc.k >= 0 ? (double)v : 0.0) / (double)INNER(L + 1);
             cc.b = (T)((double)v - mean);
         }
     }

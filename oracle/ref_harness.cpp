@@ -21,6 +21,10 @@
 // solutions.
 //
 // Usage: ref_harness <workdir-with-INP.DAT> <outdir> <last_step> [--ri X] [--solve-step S]
+//        ref_harness <workdir-with-INP.DAT> <outdir> <steps> --time 1
+//   (timing mode: nothing is dumped; the wall time of each of the first <steps>
+//   steps after step 0 is printed, one per line, and the process exits)
+#include <chrono>
 #include <cstdio>
 #include <climits>
 #include <cstdlib>
@@ -43,6 +47,8 @@ int g_last = 2;
 int g_solve_step = -1;
 double g_ri = -1.0;
 int g_line = 0;
+bool g_time = false;
+std::chrono::steady_clock::time_point g_t0;
 
 void dump2(const char *tag, const char *name, double **a) {
     const int n0 = n[0], n1 = n[1];
@@ -177,6 +183,16 @@ void copy_text(const char *src, const char *dst_name) {
 }
 
 void on_step_line(int L) {
+    if (g_time) {
+        const auto t = std::chrono::steady_clock::now();
+        if (L >= 1) fprintf(stderr, "%.3f\n", std::chrono::duration<double, std::milli>(t - g_t0).count());
+        if (L >= g_last) {
+            fflush(nullptr);
+            _exit(0);
+        }
+        g_t0 = std::chrono::steady_clock::now();
+        return;
+    }
     if (L == 0) {
         dump_setup();
         // the momentum-solver inputs of step 0 start from the initial fields; keep them
@@ -225,6 +241,7 @@ int main(int argc, char **argv) {
     for (int a = 4; a + 1 < argc; a += 2) {
         if (!strcmp(argv[a], "--ri")) g_ri = atof(argv[a + 1]);
         else if (!strcmp(argv[a], "--solve-step")) g_solve_step = atoi(argv[a + 1]);
+        else if (!strcmp(argv[a], "--time")) g_time = atoi(argv[a + 1]) != 0;
     }
     if (chdir(argv[1]) != 0) { perror("chdir"); return 2; }
     static LineHook hook;
