@@ -38,7 +38,7 @@ class Params(C.Structure):
                 ("mg_coarse", _i), ("mg_omega", _d), ("precision", _i), ("graph", _i), ("mg_tune", _i),
                 ("wall_upwind", _i), ("hyb_blend", _d), ("fas_levels", _i), ("fas_pre", _i), ("fas_post", _i),
                 ("fas_coarse", _i), ("fas_min_n", _i), ("fas_coarse_scheme", _i),
-                ("fas_small", _i)]
+                ("fas_small", _i), ("fas_coarse_pmaxit", _i)]
 
 
 class Stats(C.Structure):

@@ -42,6 +42,7 @@ struct SlotP {
     double scheme;   // convection next to walls (cfd_params::wall_upwind) on this slot's level
     double fas_src;  // 1: the slot is a coarse level of a FAS hierarchy (applies the level's sources)
     double deep;     // 1: the slot is the coarsest level of its case's hierarchy
+    double pmaxit;   // > 0: the slot's p' BiCGStab iteration cap (else the level's)
 };
 
 // grid shape shared by the slots, and the slots' physics
@@ -1182,7 +1183,8 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
     const double bnorm = sqrt(nb[0]), tol = a->p_rtol * bnorm;
     if (bnorm != 0.0 && sqrt(nb[1]) > tol) {
         const double *z = sl_mg + a->mg.off[0];  // the V-cycle's result on this level
-        const int maxit = a->p_maxit;
+        const double pm = a->g.sp[blockIdx.z].pmaxit;
+        const int maxit = pm > 0.0 ? (int)pm : a->p_maxit;
         double rho_old = 1.0, alpha = 1.0, omega = 1.0;
         for (int it = 1; it <= maxit; it++) {
             double t1[1] = {own ? R0[P] * R[P] : 0.0};
@@ -2531,7 +2533,10 @@ int put_slot(Case *c, int z, bool fresh = false) {
         double *h = l->hscal + (size_t)z * SS;
         h[S_ACTIVE] = l->sph[z].active;
         h[S_FRESH] = fresh ? 1.0 : 0.0;
+        h[S_PMAX] = l->sph[z].pmaxit;
         CFD_HIP(hipMemcpyAsync(l->scal + (size_t)z * SS + S_ACTIVE, h + S_ACTIVE, sizeof(double),
+                               hipMemcpyHostToDevice, l->s));
+        CFD_HIP(hipMemcpyAsync(l->scal + (size_t)z * SS + S_PMAX, h + S_PMAX, sizeof(double),
                                hipMemcpyHostToDevice, l->s));
         if (fresh)
             CFD_HIP(hipMemcpyAsync(l->scal + (size_t)z * SS + S_FRESH, h + S_FRESH, sizeof(double),
@@ -2644,6 +2649,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     cfd_params cp = *params;
     cp.fas_levels = params->fas_levels - 1;
     if (cp.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = cp.fas_coarse_scheme;
+    if (cp.fas_coarse_pmaxit > 0) cp.p_maxit = cp.fas_coarse_pmaxit;
     if (dry) {
         if (rc == CFD_OK) rc = build_levels(c);
         if (rc == CFD_OK && is_coarse) rc = alloc_small(c);
@@ -2889,6 +2895,9 @@ int pack_create(int n0, int n1, double dxi, double deta, int nslabs, const int *
     P->prm = *params;
     P->uprm = *params;
     P->prm.fas_levels = 1;  // slabs are single levels; the pack links them
+    // p' iterations launched per SIMPLE iteration: the larger of the top and coarse
+    // levels' caps (slots with a smaller one stop early, pack_add)
+    if (params->fas_coarse_pmaxit > params->p_maxit) P->prm.p_maxit = params->fas_coarse_pmaxit;
     P->slab.assign(nslabs, nullptr);
     P->off.assign(nslabs + 1, 0);
     P->owner.resize(nslabs);
@@ -3005,15 +3014,20 @@ int pack_add(Pack *P, int n0, int n1, const double *x, const double *y, const do
     RET(load_grid(top, zs[k], x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1));
     const SlotP base = slot_physics(n0, n1, x, y, phys, P->prm.wall_upwind);
     const int cscheme = std::max(P->prm.wall_upwind, P->prm.fas_coarse_scheme);
+    // a case's levels cap their p' solves as a batch of its own does
+    const int launched = P->prm.p_maxit, tpmax = P->uprm.p_maxit;
+    const int cpmax = P->uprm.fas_coarse_pmaxit > 0 ? P->uprm.fas_coarse_pmaxit : P->uprm.p_maxit;
     for (int l = k; l <= kd; l++) {
         Case *c = P->slab[l];
         const int z = zs[l];
         if (l > k) RET(inject_level(P->slab[l - 1], zs[l - 1], c, z));
         SlotP &sp = c->sph[z];
         sp = base;
+        sp.pmaxit = tpmax < launched ? tpmax : 0.0;
         if (l > k) {
             sp.scheme = cscheme;
             sp.fas_src = 1.0;
+            sp.pmaxit = cpmax < launched ? cpmax : 0.0;
         }
         sp.deep = l == kd ? 1.0 : 0.0;
         sp.active = 1.0;
@@ -3156,6 +3170,7 @@ void cfd_params_default(cfd_params *p) {
     p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 0;
     p->fas_coarse_scheme = 2;
     p->fas_small = 1;
+    p->fas_coarse_pmaxit = 1;
 }
 
 int cfd_fas_coarsest(int n0, int n1, const cfd_params *p) {

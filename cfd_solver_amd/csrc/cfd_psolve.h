@@ -60,6 +60,7 @@ enum {
     S_ACTIVE, // slot holds a case (set by the host)
     S_MASS0, S_ENERGY0,  // residual norms of the case's first SIMPLE iteration
     S_FRESH,  // 1 until the statistics of the case's first SIMPLE iteration are finished (set by the host)
+    S_PMAX,   // > 0: the slot's BiCGStab iteration cap below the launched count (set by the host)
     S_NSCAL
 };
 constexpr int SS = 64;  // device scalars per slot
@@ -1648,6 +1649,8 @@ __device__ __forceinline__ void bi_fin_scalars(double *s, const double (&tot)[NV
         } else {
             s[S_RHO_OLD] = s[S_RHO];
             s[S_RHO] = tot[NV - 1];
+            // the slot's own cap: its remaining launched iterations are no-ops
+            if (s[S_PMAX] > 0.0 && (double)it >= s[S_PMAX]) s[S_DONE] = 1.0;
         }
     }
 }

@@ -330,6 +330,10 @@ typedef struct {
                            (and at most 4 p' multigrid levels) runs its SIMPLE
                            iterations in one workgroup per case, in FP64; 0: in the
                            stage kernels of the fine levels                        */
+    int fas_coarse_pmaxit; /* p' BiCGStab iteration cap on the coarse FAS levels
+                           (default 1: their SIMPLE iterations only smooth, and one
+                           iteration gave the same cycle counts over drawn cases at
+                           four-fifths of the cost; <= 0: p_maxit)                  */
 } cfd_params;
 
 typedef struct {

@@ -426,6 +426,7 @@ static void fas_build(or_cav_case *c, const double *x, const double *y, const do
     or_cav_params cp = c->prm;
     cp.fas_levels = c->prm.fas_levels - 1;
     if (c->prm.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = c->prm.fas_coarse_scheme;
+    if (c->prm.fas_coarse_pmaxit > 0) cp.p_maxit = c->prm.fas_coarse_pmaxit;
     const double *src[12] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1};
     double *inj[12];
     for (int k = 0; k < 12; k++) inj[k] = inject_new(src[k], n0, m0, m1);

@@ -87,7 +87,7 @@ class CavParams(C.Structure):
                 ("mg_levels", _i), ("mg_pre", _i), ("mg_post", _i), ("mg_coarse", _i),
                 ("mg_omega", _d), ("wall_upwind", _i), ("fas_levels", _i), ("fas_pre", _i),
                 ("fas_post", _i), ("fas_coarse", _i), ("fas_min_n", _i),
-                ("fas_coarse_scheme", _i), ("hyb_blend", _d), ("plant", _i)]
+                ("fas_coarse_scheme", _i), ("hyb_blend", _d), ("fas_coarse_pmaxit", _i), ("plant", _i)]
 
 
 class CavStats(C.Structure):
@@ -101,7 +101,7 @@ def cav_params(**kw):
                 lid_u=1.0, t_hot=1.0, t_cold=0.0, mom_sweeps=8, p_rtol=1e-2, p_maxit=2,
                 mg_levels=12, mg_pre=2, mg_post=2, mg_coarse=10, mg_omega=0.8,
                 wall_upwind=1, fas_levels=8, fas_pre=2, fas_post=2, fas_coarse=40, fas_min_n=0,
-                fas_coarse_scheme=2, hyb_blend=1.0, plant=0)
+                fas_coarse_scheme=2, hyb_blend=1.0, fas_coarse_pmaxit=1, plant=0)
     base.update(kw)
     return CavParams(**base)
 
