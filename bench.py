@@ -181,7 +181,12 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=3):
     bytes_launch = int(kernel_bytes(name, mixed) * n0 * n1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     levels = fas_hierarchy(n0, n1, case.params)
-    cycle_bytes = sum(a * b * k for a, b, k in levels) * simple_bytes_per_node(mixed, max(st.p_iters, 1))
+    # algorithmic bytes of the stage-kernel levels (a level of at most 33 x 33 nodes runs
+    # from LDS in one workgroup): the finest level's p' solve takes the BiCGStab
+    # iterations of the last statistics, the coarse levels' at most fas_coarse_pmaxit
+    pc = case.params.fas_coarse_pmaxit if case.params.fas_coarse_pmaxit > 0 else case.params.p_maxit
+    cycle_bytes = sum(a * b * k * simple_bytes_per_node(mixed, max(st.p_iters, 1) if i == 0 else pc)
+                      for i, (a, b, k) in enumerate(levels) if a > 33)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": hbm_traffic(mixed, name, n0, n1), "kernel": name,
             "avg_launch_ms": round(avg_ms, 5), "launches": launches, "bytes_per_launch": bytes_launch,
@@ -345,6 +350,7 @@ def run_packed(mixed, cap=150, tol=1e-8, graph=1):
         p = default_params(precision=prm.precision, **phys)
         chains.append((n, L.cfd_fas_coarsest(n, n, C.byref(p))))
     n_top = max(n for n, _ in chains)
+    pc = prm.fas_coarse_pmaxit if prm.fas_coarse_pmaxit > 0 else prm.p_maxit
     sizes, slots = pack_slots(n_top, chains)
     g0 = cavity_grid(n_top)
     pk = CavityPack(n_top, n_top, g0.dxi, g0.deta, slots, prm)
@@ -354,9 +360,9 @@ def run_packed(mixed, cap=150, tol=1e-8, graph=1):
         cid = pk.add_case(cavity_grid(n, skew=skew), **phys)
         ids[cid] = idx
         cells[cid] = cells_per_outer(n, n, default_params(**phys))
-        for a, b, k in fas_hierarchy(n, n, default_params(**phys)):
+        for i, (a, b, k) in enumerate(fas_hierarchy(n, n, default_params(**phys))):
             if a > 33:  # levels of at most 33 x 33 nodes run from LDS in one workgroup
-                mix_bytes += a * b * k * simple_bytes_per_node(mixed, 2)
+                mix_bytes += a * b * k * simple_bytes_per_node(mixed, prm.p_maxit if i == 0 else pc)
     live = set(ids)
     states, cycles = {}, {}
     cell_its = 0.0
@@ -398,7 +404,8 @@ def run_packed(mixed, cap=150, tol=1e-8, graph=1):
                 "bytes_model": round(mix_bytes), "GBs": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9, 1),
                 "frac": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "cycles 2-4 (every case live); algorithmic bytes of the SIMPLE iterations on levels above "
-                        "33 x 33 (two BiCGStab iterations each) over the cycle's wall time"},
+                        "33 x 33 (p' BiCGStab iterations at their caps: 2 on a case's own grid, 1 on its coarse "
+                        "levels) over the cycle's wall time"},
             "precision": "mixed" if mixed else "fp64",
             "workload": "the same 256 cases as one pack: slabs of 513, 257, ..., 17 nodes per side, each case "
                         "holding a slot in every slab from its own size down to its coarsest multigrid level; "
