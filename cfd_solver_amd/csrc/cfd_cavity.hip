@@ -126,6 +126,9 @@ struct Case {
     int ev_used = 0;            // events recorded since the last drain
     double prof_ms[CFD_PROF_KINDS] = {0};
     long prof_n[CFD_PROF_KINDS] = {0};
+    // residual tile rows (4 or 8) of this level, and whether the tuner has chosen them
+    int res_ty = 8;
+    bool res_tuned = false;
     // captured outer iteration
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -161,8 +164,12 @@ struct GridPtrs {
 // as are the metric and operator coefficients of the thread's own nodes, so the
 // 25-point rows read LDS instead of issuing a global load per stencil point.
 // Halo entries beyond the grid hold clamped copies that no interior row reads.
-constexpr int RT_X = 64, RT_Y = 8;
-inline dim3 grid_res(int n0, int n1, int B) { return dim3(tiles_inner(n0, RT_X), tiles_inner(n1, RT_Y), B); }
+// Tiles are RT_X wide and TY = 4 or 8 rows high (one or two nodes per thread):
+// the autotuner (tune_res) times both on each level and keeps the faster (the
+// taller tile re-reads fewer halo rows, the shorter one gives small levels more
+// blocks and a shorter chain per thread); a node's arithmetic does not depend on it.
+constexpr int RT_X = 64;
+inline dim3 grid_res(int n0, int n1, int B, int ty) { return dim3(tiles_inner(n0, RT_X), tiles_inner(n1, ty), B); }
 
 // metric and operator coefficients of one node
 struct NC {
@@ -235,7 +242,7 @@ __device__ __forceinline__ ResOut residual_node(const CGrid &g, const SlotP &ph,
     return ResOut{ru, rv, rt, UP, VP, dt * ph.alpha_u / cl[4]};
 }
 
-template <typename TR>
+template <typename TR, int TY>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_residual(
     CGrid g, const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ T,
     const double *__restrict__ p, GridPtrs gp, TR *__restrict__ Ru, TR *__restrict__ Rv, TR *__restrict__ RT,
@@ -245,13 +252,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const size_t o = zslot() * g.N;
     u += o; v += o; T += o; p += o; Ru += o; Rv += o; RT += o; dd += o; Ua += o; Va += o;
     src = src_at(src, o, ph);
-    constexpr int FX = RT_X + 4, FY = RT_Y + 4, PX = RT_X + 2, PY = RT_Y + 2;
-    constexpr int NODES = RT_Y / BY;  // nodes per thread
+    constexpr int FX = RT_X + 4, FY = TY + 4, PX = RT_X + 2, PY = TY + 2;
+    constexpr int NODES = TY / BY;  // nodes per thread
+    static_assert(TY % BY == 0, "tile rows");
     __shared__ double s_u[FX * FY], s_v[FX * FY], s_t[FX * FY], s_p[PX * PY];
     const int tid = tlin();
     const int n0 = g.n0, n1 = g.n1;
     const int2 tl = xcd_tile();
-    const int i0 = tl.x * RT_X, j0 = tl.y * RT_Y;
+    const int i0 = tl.x * RT_X, j0 = tl.y * TY;
     struct F3 {
         double u, v, t;
     };
@@ -2020,6 +2028,48 @@ int tune_mg(Case *c) {
     return CFD_OK;
 }
 
+int launch_residual(Case *c);
+
+// Autotuning of the residual tile rows: k_residual is timed with 4- and 8-row
+// tiles on the level's own buffers (one untimed launch of each, then the smaller
+// minimum over five rounds of alternating timed launches) and the faster is kept.
+// The launches write only what every SIMPLE iteration rewrites before reading it
+// (R, U, V, d and the residual partial rows), so this runs before the first outer
+// iteration (or capture).  Once per level; not while the stream is being captured.
+int tune_res(Case *c) {
+    if (!c->prm.mg_tune || c->capturing || c->res_tuned || c->dry) return CFD_OK;
+    c->res_tuned = true;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CFD_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return CFD_E_HIP;
+    }
+    const int tys[2] = {4, 8};
+    float best[2] = {3.0e38f, 3.0e38f};
+    int rc = CFD_OK;
+    auto launch = [&](int k) {
+        c->res_ty = tys[k];
+        if (launch_residual(c) != CFD_OK) rc = CFD_E_HIP;
+    };
+    launch(0);
+    launch(1);
+    for (int rep = 0; rep < 5 && rc == CFD_OK; rep++)
+        for (int k = 0; k < 2 && rc == CFD_OK; k++) {
+            float ms = 0.f;
+            if (hipEventRecord(e0, c->s) != hipSuccess) rc = CFD_E_HIP;
+            launch(k);
+            if (rc == CFD_OK && (hipEventRecord(e1, c->s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = CFD_E_HIP;
+            best[k] = std::min(best[k], ms);
+        }
+    c->res_ty = (rc == CFD_OK && best[0] < best[1]) ? 4 : 8;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = false) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
@@ -2084,12 +2134,18 @@ WallIn wall_in(const Case *c) {
                   c->p1, c->q1, c->ajac};
 }
 
+template <typename TR, int TY>
+void launch_residual_ty(Case *c, const Timed &tm) {
+    const dim3 gr = grid_res(c->n0, c->n1, c->B, TY);
+    launch_timed(tm, k_residual<TR, TY>, gr, dim3(BX, BY), c->s, cg(c), (const double *)c->u, (const double *)c->v,
+                 (const double *)c->T, (const double *)c->p, gptrs(c), (TR *)c->Ru, (TR *)c->Rv, (TR *)c->RT, c->d,
+                 (TR *)c->U, (TR *)c->V, red(c, PS_RES), srcs(c));
+}
 template <typename TR>
 int launch_residual_t(Case *c) {
-    const dim3 gr = grid_res(c->n0, c->n1, c->B);
-    launch_timed(prof_timed(c, 2), k_residual<TR>, gr, dim3(BX, BY), c->s, cg(c), (const double *)c->u,
-                 (const double *)c->v, (const double *)c->T, (const double *)c->p, gptrs(c), (TR *)c->Ru,
-                 (TR *)c->Rv, (TR *)c->RT, c->d, (TR *)c->U, (TR *)c->V, red(c, PS_RES), srcs(c));
+    const Timed tm = prof_timed(c, 2);
+    if (c->res_ty == 4) launch_residual_ty<TR, 4>(c, tm);
+    else launch_residual_ty<TR, 8>(c, tm);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
 }
@@ -2220,7 +2276,7 @@ int simple_body(Case *c, bool graph, bool stats = true) {
     RET(wall_bcs(c, 0, c->B, false));
     // 8 dmax (partials from the correction) and the wall Nusselt numbers (FP64)
     if (stats) {
-        const dim3 grr = grid_res(c->n0, c->n1, 1);
+        const dim3 grr = grid_res(c->n0, c->n1, 1, c->res_ty);
         k_dmax_nu_fin<<<dim3(1, 1, c->B), FIN_NT, 0, c->s>>>(cg(c), red(c), nblk_in(c->n0, c->n1), (int)(grr.x * grr.y),
                                                              c->T, c->ajac, c->etx, c->ety);
         CFD_LAUNCH_CHECK();
@@ -2238,10 +2294,15 @@ unsigned nblk256(size_t n) { return (unsigned)((n + 255) / 256); }
 int state_residual(Case *c, bool with_src, const SlotP *sp) {
     const bool keep = c->use_src;
     c->use_src = with_src && keep;
-    const dim3 gr = grid_res(c->n0, c->n1, c->B);
-    k_residual<double><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0],
-                                                     c->fr[1], c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES),
-                                                     srcs(c));
+    const dim3 gr = grid_res(c->n0, c->n1, c->B, c->res_ty);
+    if (c->res_ty == 4)
+        k_residual<double, 4><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0],
+                                                            c->fr[1], c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES),
+                                                            srcs(c));
+    else
+        k_residual<double, 8><<<gr, dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, c->T, c->p, gptrs(c), c->fr[0],
+                                                            c->fr[1], c->fr[2], c->d, c->uh, c->vh, red(c, PS_RES),
+                                                            srcs(c));
     CFD_LAUNCH_CHECK();
     k_face_div<double><<<grid_in(c), dim3(BX, BY), 0, c->s>>>(cg(c, sp), c->u, c->v, (const double *)nullptr,
                                                              (const double *)nullptr, c->p, c->d, c->xix, c->xiy,
@@ -2447,7 +2508,10 @@ int capture(Case *c) {
 
 // launch one outer iteration of every active slot (no synchronisation)
 int outer_launch(Case *c) {
-    for (Case *l = c; l; l = l->coarse) RET(tune_mg(l));  // once, eagerly, before any capture
+    for (Case *l = c; l; l = l->coarse) {  // once, eagerly, before any capture
+        RET(tune_mg(l));
+        RET(tune_res(l));
+    }
     // live kernel timing brackets individual launches, so a profiled iteration runs eagerly
     if (c->prm.graph && !c->prof) {
         if (!c->gexec) RET(capture(c));
@@ -3074,7 +3138,10 @@ int pack_launch(Pack *P) {
     for (int k = k0; k < P->ns; k++)
         if (!P->slab[k]) return CFD_E_ARG;  // the chain below the largest slab is complete
     if (!P->tuned) {
-        for (int k = k0; k < P->ns; k++) RET(tune_mg(P->slab[k]));
+        for (int k = k0; k < P->ns; k++) {
+            RET(tune_mg(P->slab[k]));
+            RET(tune_res(P->slab[k]));
+        }
         P->tuned = true;
     }
     if (!P->prm.graph) return pack_cycle(P, k0);
