@@ -98,6 +98,36 @@ def hbm_traffic(mixed, name, n0, n1):
         return None
 
 
+def copy_peak(mib=2048, reps=10):
+    """Device-to-device copy rate of this GPU (hipMemcpyAsync of `mib` MiB, best of
+    `reps` timed copies with HIP events), as read + write GB/s: the HBM rate a
+    streaming kernel can reach here, beside the 8 TB/s data-sheet peak."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), n) != 0 or hip.hipMalloc(C.byref(b), n) != 0:
+        return None
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    hip.hipMemset(a, 1, n)
+    best = None
+    for _ in range(reps + 1):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, n, 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = C.c_float()
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        best = ms.value if best is None else min(best, ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return round(2 * n.value / (best * 1e-3) / 1e9, 1)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -658,6 +688,10 @@ def main():
         cfg["ref_ogrid"] = run_ref_ogrid(no_cpu=a.no_cpu)
         print(f"[bench] reference O-grid step: {cfg['ref_ogrid']['ms_per_step']:.2f} ms", file=sys.stderr,
               flush=True)
+    copy_gbs = copy_peak()
+    if copy_gbs:
+        roof["measured_copy_GBs"] = copy_gbs
+        roof["frac_of_measured_copy"] = round(roof["achieved"] / copy_gbs, 4)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(grid, a.cpu_cycles)

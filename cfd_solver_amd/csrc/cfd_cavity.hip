@@ -3229,7 +3229,7 @@ void cfd_params_default(cfd_params *p) {
     p->alpha_u = 0.7; p->alpha_p = 0.3; p->alpha_t = 0.85;
     p->lid_u = 1.0; p->t_hot = 1.0; p->t_cold = 0.0;
     p->mom_sweeps = 8; p->p_rtol = 1.0e-2; p->p_maxit = 2;
-    p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 10; p->mg_omega = 0.8;
+    p->mg_levels = 12; p->mg_pre = 2; p->mg_post = 2; p->mg_coarse = 4; p->mg_omega = 0.8;
     p->precision = CFD_FP64;
     p->graph = 0;
     p->mg_tune = 1;

@@ -289,7 +289,9 @@ typedef struct {
     double p_rtol;      /* p' relative tolerance (2-norm)                        */
     int p_maxit;        /* p' BiCGStab iteration cap (default 2)                 */
     int mg_levels;      /* multigrid levels of the p' preconditioner             */
-    int mg_pre, mg_post, mg_coarse;
+    int mg_pre, mg_post, mg_coarse; /* Jacobi sweeps before / after the coarse-grid
+                           correction on each level, and on the coarsest (defaults
+                           2, 2, 4)                                                 */
     double mg_omega;    /* damping of the Jacobi smoother                        */
     int precision;      /* CFD_FP64 or CFD_MIXED                                 */
     int graph;          /* 0 (default): launch eagerly, reading the p' solver's device
