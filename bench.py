@@ -77,10 +77,14 @@ def simple_bytes_per_node(mixed, p_iters):
     predictor launches, face divergence, face coefficients of every p' level (4/3 of
     the fine level's), correction, and per BiCGStab iteration two V-cycles (fine down
     and fused up-and-apply legs, coarser levels a third more) plus the vector updates
-    (p: 4 words, s: 3, x and r: 8); the mixed path adds one FP64 defect/p' pass."""
+    (p: 4 words, s: 3, x and r: 8); the mixed path adds one FP64 defect/p' pass.
+    p_iters < 0: the p' solve is one V-cycle (start pass q -> x, r, r0; the down legs
+    and plain up legs, which skip the operator application's 1.5 words)."""
     w = 4 if mixed else 8
     fixed = (kernel_bytes("residual", mixed) + 2 * kernel_bytes("jacobi", mixed) + kernel_bytes("face_div", mixed)
              + (10 * 8 + 4 * w) * 4 / 3 + 13 * 8 + 3 * w + (13 * 8 if mixed else 0))
+    if p_iters < 0:
+        return fixed + 8 + 3 * w + (kernel_bytes("mg_down", mixed) + kernel_bytes("mg_up", mixed) - 1.5 * w) * 4 / 3
     per_it = 2 * (kernel_bytes("mg_down", mixed) + kernel_bytes("mg_up", mixed)) * 4 / 3 + 15 * w
     return fixed + per_it * max(p_iters, 1)
 
@@ -214,7 +218,8 @@ def run_gpu(grid, mixed, steps, warmup, prof_steps=3):
     # algorithmic bytes of the stage-kernel levels (a level of at most 33 x 33 nodes runs
     # from LDS in one workgroup): the finest level's p' solve takes the BiCGStab
     # iterations of the last statistics, the coarse levels' at most fas_coarse_pmaxit
-    pc = case.params.fas_coarse_pmaxit if case.params.fas_coarse_pmaxit > 0 else case.params.p_maxit
+    # (< 0: one V-cycle)
+    pc = case.params.fas_coarse_pmaxit if case.params.fas_coarse_pmaxit != 0 else case.params.p_maxit
     cycle_bytes = sum(a * b * k * simple_bytes_per_node(mixed, max(st.p_iters, 1) if i == 0 else pc)
                       for i, (a, b, k) in enumerate(levels) if a > 33)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -380,7 +385,7 @@ def run_packed(mixed, cap=150, tol=1e-8, graph=1):
         p = default_params(precision=prm.precision, **phys)
         chains.append((n, L.cfd_fas_coarsest(n, n, C.byref(p))))
     n_top = max(n for n, _ in chains)
-    pc = prm.fas_coarse_pmaxit if prm.fas_coarse_pmaxit > 0 else prm.p_maxit
+    pc = prm.fas_coarse_pmaxit if prm.fas_coarse_pmaxit != 0 else prm.p_maxit
     sizes, slots = pack_slots(n_top, chains)
     g0 = cavity_grid(n_top)
     pk = CavityPack(n_top, n_top, g0.dxi, g0.deta, slots, prm)

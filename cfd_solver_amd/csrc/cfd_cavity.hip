@@ -42,7 +42,8 @@ struct SlotP {
     double scheme;   // convection next to walls (cfd_params::wall_upwind) on this slot's level
     double fas_src;  // 1: the slot is a coarse level of a FAS hierarchy (applies the level's sources)
     double deep;     // 1: the slot is the coarsest level of its case's hierarchy
-    double pmaxit;   // > 0: the slot's p' BiCGStab iteration cap (else the level's)
+    double pmaxit;   // > 0: the slot's p' BiCGStab iteration cap; < 0: its p' is one V-cycle
+                     // (0: the level's p_maxit)
 };
 
 // grid shape shared by the slots, and the slots' physics
@@ -1189,10 +1190,16 @@ __device__ __forceinline__ void sl_pprime(const SmallArgs *__restrict__ a, SlNod
     double nb[2] = {bq * bq, r * r};
     sl_sums<NT, 2>(nb);
     const double bnorm = sqrt(nb[0]), tol = a->p_rtol * bnorm;
-    if (bnorm != 0.0 && sqrt(nb[1]) > tol) {
-        const double *z = sl_mg + a->mg.off[0];  // the V-cycle's result on this level
-        const double pm = a->g.sp[blockIdx.z].pmaxit;
-        const int maxit = pm > 0.0 ? (int)pm : a->p_maxit;
+    const double *z = sl_mg + a->mg.off[0];  // the V-cycle's result on this level
+    const double pm = a->g.sp[blockIdx.z].pmaxit;
+    const int maxit = pm != 0.0 ? (int)pm : a->p_maxit;  // < 0: one V-cycle
+    if (maxit < 0) {
+        if (bnorm != 0.0) {
+            nd[0].b = r;
+            cw_vcycle<double, NT>(a->mg, 0, nd, sl_mg);
+            if (own) X[P] = z[P];
+        }
+    } else if (bnorm != 0.0 && sqrt(nb[1]) > tol) {
         double rho_old = 1.0, alpha = 1.0, omega = 1.0;
         for (int it = 1; it <= maxit; it++) {
             double t1[1] = {own ? R0[P] * R[P] : 0.0};
@@ -1814,6 +1821,8 @@ __global__ __launch_bounds__(FIN_NT) void k_cav_fin(int gs, RedArgs ra, int npar
         s[S_PSUM] = tot[0];
     } else if (OP == CF_Q) {
         // ||q|| and ||q - mean(q)|| = sqrt(sum q^2 - sum(q) mean); refinement start
+        // (a one-V-cycle slot has no tolerance)
+        if (s[S_PMAX] < 0.0) rtol = 0.0;
         const double rr = fmax(tot[NV - 1] - tot[0] * (tot[0] / nin), 0.0);
         const double bn = sqrt(tot[NV - 1]), rn = sqrt(rr);
         s[S_BB64] = tot[NV - 1];
@@ -2070,21 +2079,26 @@ int tune_res(Case *c) {
     return rc;
 }
 
+// maxit < 0: p' is one V-cycle on the mean-free q (no Krylov iteration, no
+// tolerance), as the oracle's vcycle_solve()
 int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = false) {
     PsCtx cx = ctx(c);
     const int n0 = c->n0, n1 = c->n1;
     // the p' kernels here run one thread per node in memory order
     const dim3 gf(nflat(n0, n1), 1, c->B), bl(BX, BY);
     const int nf = nflat(n0, n1);
+    const bool vc = maxit < 0;
+    if (vc) rtol = 0.0;
     if (!c->mixed) {
         // with the face divergence's sums of q finished, the solve starts in one pass
         if (q_norms_ready) RET(cav_fin<CF_Q>(c, -1, rtol, 1.0, nblk_in(n0, n1)));
-        RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph, q_norms_ready));
+        if (vc) RET(c->ps64->vsolve(cx, c->q, c->pc, -1, q_norms_ready));
+        else RET(c->ps64->solve(cx, c->q, c->pc, nullptr, rtol, -1, maxit, -1, graph, q_norms_ready));
         k_pc_final<false><<<gf, bl, 0, c->s>>>(n0, n1, c->pc, nullptr, red(c));
         CFD_LAUNCH_CHECK();
         return cav_fin<CF_PSUM>(c, -1, 0.0, 0.5, nf);
     }
-    const int R = rtol >= 1e-4 ? 1 : 6;
+    const int R = (vc || rtol >= 1e-4) ? 1 : 6;
     const FArgs<double> a64{n0, n1, c->f64[0], c->f64[1], c->f64[2], c->f64[3]};
     if (!q_norms_ready) {
         k_q_norms<<<gf, bl, 0, c->s>>>(n0, n1, c->q, c->pc, red(c));
@@ -2104,8 +2118,10 @@ int psolve(Case *c, double rtol, int maxit, bool graph, bool q_norms_ready = fal
             }
             if (all) break;
         }
-        RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit, S_RDONE,
-                           graph, k == 0));
+        if (vc) RET(c->ps32->vsolve(cx, c->q, c->ef, S_RDONE, true));
+        else
+            RET(c->ps32->solve(cx, k == 0 ? c->q : c->r64, c->ef, k == 0 ? nullptr : c->pc, 0.0, S_RTOL, maxit,
+                               S_RDONE, graph, k == 0));
         if (R == 1) {  // defect, p' = e and its sum in one pass
             k_defect_final<<<gf, bl, 0, c->s>>>(a64, c->q, c->pc, c->ef, red(c));
             CFD_LAUNCH_CHECK();
@@ -2655,6 +2671,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     if (!out || !params) return CFD_E_ARG;
     if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0) || slots < 1 || slots > 65535) return CFD_E_ARG;
     if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
+    if (!coarse_level && params->p_maxit < 1) return CFD_E_ARG;  // (-1, one V-cycle: coarse levels)
     *out = nullptr;
     Case *c = new Case();
     c->dry = dry;
@@ -2713,7 +2730,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
     cfd_params cp = *params;
     cp.fas_levels = params->fas_levels - 1;
     if (cp.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = cp.fas_coarse_scheme;
-    if (cp.fas_coarse_pmaxit > 0) cp.p_maxit = cp.fas_coarse_pmaxit;
+    if (cp.fas_coarse_pmaxit != 0) cp.p_maxit = cp.fas_coarse_pmaxit < 0 ? -1 : cp.fas_coarse_pmaxit;
     if (dry) {
         if (rc == CFD_OK) rc = build_levels(c);
         if (rc == CFD_OK && is_coarse) rc = alloc_small(c);
@@ -3080,7 +3097,8 @@ int pack_add(Pack *P, int n0, int n1, const double *x, const double *y, const do
     const int cscheme = std::max(P->prm.wall_upwind, P->prm.fas_coarse_scheme);
     // a case's levels cap their p' solves as a batch of its own does
     const int launched = P->prm.p_maxit, tpmax = P->uprm.p_maxit;
-    const int cpmax = P->uprm.fas_coarse_pmaxit > 0 ? P->uprm.fas_coarse_pmaxit : P->uprm.p_maxit;
+    const int cfp = P->uprm.fas_coarse_pmaxit;
+    const int cpmax = cfp != 0 ? (cfp < 0 ? -1 : cfp) : P->uprm.p_maxit;  // -1: one V-cycle
     for (int l = k; l <= kd; l++) {
         Case *c = P->slab[l];
         const int z = zs[l];
@@ -3237,7 +3255,7 @@ void cfd_params_default(cfd_params *p) {
     p->fas_levels = 8; p->fas_pre = 2; p->fas_post = 2; p->fas_coarse = 40; p->fas_min_n = 0;
     p->fas_coarse_scheme = 2;
     p->fas_small = 1;
-    p->fas_coarse_pmaxit = 1;
+    p->fas_coarse_pmaxit = -1;
 }
 
 int cfd_fas_coarsest(int n0, int n1, const cfd_params *p) {

@@ -333,9 +333,9 @@ typedef struct {
                            iterations in one workgroup per case, in FP64; 0: in the
                            stage kernels of the fine levels                        */
     int fas_coarse_pmaxit; /* p' BiCGStab iteration cap on the coarse FAS levels
-                           (default 1: their SIMPLE iterations only smooth, and one
-                           iteration gave the same cycle counts over drawn cases at
-                           four-fifths of the cost; <= 0: p_maxit)                  */
+                           (their SIMPLE iterations only smooth; 0: p_maxit; < 0:
+                           no Krylov iteration, p' is one multigrid V-cycle on the
+                           mean-free q)                                            */
 } cfd_params;
 
 typedef struct {

@@ -176,8 +176,8 @@ typedef struct {
                            values; the finer of it and wall_upwind is used)         */
     double hyb_blend;   /* 0: the reference's switch at |Pe| = 2; > 0: blend from the
                            central to the upwind form over 2 < |Pe| < 2 + hyb_blend  */
-    int fas_coarse_pmaxit; /* p' BiCGStab iteration cap on the coarse FAS levels (<= 0:
-                           p_maxit)                                                 */
+    int fas_coarse_pmaxit; /* p' BiCGStab iteration cap on the coarse FAS levels (0:
+                           p_maxit; < 0: p' is one V-cycle, vcycle_solve())         */
     int plant;          /* tests only, a deliberate mistake (0: none): 1-4 flips the
                            sign of the normal-momentum term of the wall pressure on
                            wall 0-3 (j=0, j=n1-1, i=0, i=n0-1), 5 the sign of the

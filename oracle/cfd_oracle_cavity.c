@@ -347,6 +347,21 @@ static int bicgstab(or_cav_case *c, const double *bvec, double *x, double rtol, 
     return it > maxit ? maxit : it;
 }
 
+/* p' by one multigrid V-cycle from zero on the mean-free right-hand side (no
+   Krylov iteration; the coarse FAS levels' SIMPLE iterations only smooth). */
+static int vcycle_solve(or_cav_case *c, const double *bvec, double *x, double *relres)
+{
+    const int n0 = c->n0, n1 = c->n1;
+    double *r = c->bi[0];
+    for (int j = 0; j < n1; j++)
+        for (int i = 0; i < n0; i++)
+            r[ID(i, j)] = (i == 0 || j == 0 || i == n0 - 1 || j == n1 - 1) ? 0.0 : bvec[ID(i, j)];
+    project_mean(n0, n1, r);
+    precond(c, r, x);
+    if (relres) *relres = 0.0;
+    return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 static void build_levels(or_cav_case *c)
 {
@@ -427,6 +442,7 @@ static void fas_build(or_cav_case *c, const double *x, const double *y, const do
     cp.fas_levels = c->prm.fas_levels - 1;
     if (c->prm.fas_coarse_scheme > cp.wall_upwind) cp.wall_upwind = c->prm.fas_coarse_scheme;
     if (c->prm.fas_coarse_pmaxit > 0) cp.p_maxit = c->prm.fas_coarse_pmaxit;
+    if (c->prm.fas_coarse_pmaxit < 0) cp.p_maxit = -1; /* one V-cycle */
     const double *src[12] = {x, y, xix, xiy, etx, ety, alph, beta, gamma, ajac, p1, q1};
     double *inj[12];
     for (int k = 0; k < 12; k++) inj[k] = inject_new(src[k], n0, m0, m1);
@@ -1048,7 +1064,8 @@ int or_cav_outer(or_cav_case *c, or_cav_stats *st)
     inject_d(c);
     memset(c->pc, 0, N * sizeof(double));
     double relres = 0.0;
-    const int its = bicgstab(c, c->q, c->pc, pr->p_rtol, pr->p_maxit, &relres);
+    const int its = pr->p_maxit < 0 ? vcycle_solve(c, c->q, c->pc, &relres)
+                                     : bicgstab(c, c->q, c->pc, pr->p_rtol, pr->p_maxit, &relres);
     project_mean(n0, n1, c->pc);
     /* 6-7: correction and boundary conditions */
     correct(c);

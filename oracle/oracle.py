@@ -101,7 +101,7 @@ def cav_params(**kw):
                 lid_u=1.0, t_hot=1.0, t_cold=0.0, mom_sweeps=8, p_rtol=1e-2, p_maxit=2,
                 mg_levels=12, mg_pre=2, mg_post=2, mg_coarse=4, mg_omega=0.8,
                 wall_upwind=1, fas_levels=8, fas_pre=2, fas_post=2, fas_coarse=40, fas_min_n=0,
-                fas_coarse_scheme=2, hyb_blend=1.0, fas_coarse_pmaxit=1, plant=0)
+                fas_coarse_scheme=2, hyb_blend=1.0, fas_coarse_pmaxit=-1, plant=0)
     base.update(kw)
     return CavParams(**base)
 
