@@ -225,6 +225,7 @@ void or_cav_state_residual(or_cav_case *c, double *ru, double *rv, double *rT, d
 /* stage-level entry points of the cavity path (tests compare them one by one) */
 void or_cav_lo_diag(const or_cav_case *c, double *d_out);
 void or_cav_p_apply(const or_cav_case *c, const double *d, const double *x, double *y);
+/* maxit < 0: x = one V-cycle on the mean-free q (vcycle_solve), x overwritten */
 int or_cav_p_solve(or_cav_case *c, const double *d, const double *q, double *x,
                    double rtol, int maxit, double *relres);
 void or_cav_residuals(or_cav_case *c, double *Ru, double *Rv, double *RT, double *d);

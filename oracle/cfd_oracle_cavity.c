@@ -1270,6 +1270,7 @@ int or_cav_p_solve(or_cav_case *c, const double *d, const double *q, double *x,
     const size_t N = (size_t)c->n0 * c->n1;
     memcpy(c->d, d, N * sizeof(double));
     inject_d(c);
+    if (maxit < 0) return vcycle_solve(c, q, x, relres);
     return bicgstab(c, q, x, rtol, maxit, relres);
 }
 

@@ -28,6 +28,26 @@ def test_fas_cycles_match_oracle():
     assert abs(sg.nu_hot - sc.nu_hot) < 1e-9 * abs(sc.nu_hot)
 
 
+@pytest.mark.parametrize("pmaxit", [-1, 1])
+def test_fas_coarse_p_solves_match_oracle(pmaxit):
+    """129x129 with levels 129, 65, 33, 17: the 65x65 level runs the stage kernels, so its
+    p' solve is the multi-kernel one-V-cycle path (fas_coarse_pmaxit -1, the default) or
+    one BiCGStab iteration (1); five cycles equal the oracle's in both (fields to 1e-9,
+    residual ratios to 1e-7, the same fine-level p' iteration counts)."""
+    print(f"\n[gpu] FAS 129x129 fas_coarse_pmaxit {pmaxit}, 5 cycles vs oracle", flush=True)
+    gpu, cpu, g = pair(129, skew=0.1, beta=1.5, Re=400.0, Ri=1.0, Pr=0.71, fas_min_n=17,
+                       fas_coarse_pmaxit=pmaxit)
+    assert cpu.fas_depth() == 4
+    for k in range(5):
+        sg = gpu.step()
+        sc = cpu.step()
+        assert sg.p_iters == sc.p_iters, k
+        assert abs(sg.mass_res - sc.mass_res) <= 1e-7 * sc.mass_res, (k, sg.mass_res, sc.mass_res)
+    fg, fc = gpu.fields(), cpu.fields()
+    for k in ("u", "v", "T", "p"):
+        assert rel_l2(fg[k], fc[k]) < 1e-9, k
+
+
 def test_fas_converges_to_1e8_at_129():
     """129x129 config-2 case to R_m, R_T <= 1e-8 on the GPU in FP64 within 100 cycles
     (the oracle needs 64-68 from 65 to 257 nodes per side)."""

@@ -73,6 +73,45 @@ def test_p_solve_matches_dense():
     assert np.linalg.norm(xs - xd) <= 1e-9 * np.linalg.norm(xd)
 
 
+def test_p_vcycle_solve_properties():
+    """The coarse FAS levels' p' solve, one V-cycle on the mean-free q (maxit < 0):
+    linear in q, blind to q's interior mean, and a contraction of the p' residual on a
+    skewed, stretched grid with variable d (a dropped mean removal, a sign error in the
+    restriction or prolongation, or a missing smoothing sweep breaks one of these)."""
+    n = 33
+    g = cavity_grid(n, skew=0.1, beta=1.5)
+    c = CavityOracle(g, cav_params())
+    rng = np.random.default_rng(7)
+    d = 0.5 + rng.random((n, n))
+    q1, q2 = np.zeros((n, n)), np.zeros((n, n))
+    q1[1:-1, 1:-1] = rng.standard_normal((n - 2, n - 2))
+    q2[1:-1, 1:-1] = rng.standard_normal((n - 2, n - 2))
+    v = lambda q: c.p_solve(d, q, maxit=-1)[0]
+    x1, x2 = v(q1), v(q2)
+    x12 = v(2.0 * q1 - 3.0 * q2)
+    assert np.linalg.norm(x12 - (2.0 * x1 - 3.0 * x2)) <= 1e-12 * np.linalg.norm(x12)
+    qs = q1.copy()
+    qs[1:-1, 1:-1] += 5.0
+    assert np.linalg.norm(v(qs) - x1) <= 1e-12 * np.linalg.norm(x1)
+    qm = q1.copy()
+    qm[1:-1, 1:-1] -= qm[1:-1, 1:-1].mean()
+    r = qm - c.p_apply(d, x1)
+    ri = r[1:-1, 1:-1] - r[1:-1, 1:-1].mean()
+    assert np.linalg.norm(ri) < 0.3 * np.linalg.norm(qm[1:-1, 1:-1])
+    # a smooth q, which Jacobi sweeps alone barely reduce (0.99 with one level): the
+    # coarse-grid correction halves its residual
+    s_ = np.linspace(0.0, 1.0, n)
+    qs = np.zeros((n, n))
+    qs[1:-1, 1:-1] = (np.cos(np.pi * s_)[None, :] * np.cos(2 * np.pi * s_)[:, None])[1:-1, 1:-1]
+    qs[1:-1, 1:-1] -= qs[1:-1, 1:-1].mean()
+    for levels, lo, hi in ((12, 0.0, 0.6), (1, 0.9, 1.0)):
+        cl = CavityOracle(g, cav_params(mg_levels=levels))
+        rs = qs - cl.p_apply(d, cl.p_solve(d, qs, maxit=-1)[0])
+        rs = rs[1:-1, 1:-1] - rs[1:-1, 1:-1].mean()
+        assert lo < np.linalg.norm(rs) / np.linalg.norm(qs[1:-1, 1:-1]) < hi, levels
+    assert np.all(x1[0, :] == 0) and np.all(x1[-1, :] == 0) and np.all(x1[:, 0] == 0) and np.all(x1[:, -1] == 0)
+
+
 def test_residual_exact_on_quadratic():
     """The reference's scheme (central next to walls, wall_upwind 0) and the reference's
     step switch (hyb_blend 0): exact on quadratics wherever the hybrid branch is central
