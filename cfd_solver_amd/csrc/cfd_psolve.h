@@ -800,16 +800,6 @@ constexpr int CN_MAX = 1536;      // nodes of all LDS-resident levels together
 constexpr int CW_ARR = 2;         // LDS arrays per level: x and its ping-pong buffer
 constexpr int CW_LEVS = 4;        // at most this many levels in the kernel (a 33 x 33 level has four)
 constexpr int CW_WAVE = 64;       // levels with at most this many interior nodes run in wavefront 0
-// FP32: the coarse kernel's first level may have up to 65 x 65 nodes, its interior
-// nodes taken four per thread (CWF_NX more than one), with five levels at most
-constexpr int CWF_NODES = 4225, CNF_MAX = 5760, CWF_LEVS = 5, CWF_NX = 3;
-template <typename T>
-struct CwShape {
-    static constexpr bool wide = sizeof(T) == 4;
-    static constexpr int nodes = wide ? CWF_NODES : CW_NODES;  // first level's nodes at most
-    static constexpr int cn_max = wide ? CNF_MAX : CN_MAX;     // LDS nodes of all its levels
-    static constexpr int levs = wide ? CWF_LEVS : CW_LEVS;     // levels at most
-};
 
 template <typename T>
 struct CLev {
@@ -841,14 +831,13 @@ __device__ __forceinline__ double cw_sum(double v) {
 // the interior node of one level a thread owns
 template <typename T>
 struct CNode {
-    int ij;    // i | j << 16; 0: no node
+    int k;     // node index j * n0 + i; -1: none
+    int ij;    // i | j << 16
     T c[9];    // operator row, ghosts folded in
     T id;      // 1 / diag
     T b;       // right-hand side
     __device__ int i() const { return ij & 0xffff; }
     __device__ int j() const { return ij >> 16; }
-    __device__ bool on() const { return ij != 0; }
-    __device__ int k(int n0) const { return j() * n0 + i(); }  // node index
 };
 
 // row (i, j) from its faces with every clamped neighbour's coefficient moved onto
@@ -884,80 +873,51 @@ __device__ __forceinline__ T cw_row(const CNode<T> &c, const T *x, int n0) {
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++)
 #pragma unroll
-        for (int di = -1; di <= 1; di++) s += c.c[(dj + 1) * 3 + (di + 1)] * x[c.k(n0) + dj * n0 + di];
+        for (int di = -1; di <= 1; di++) s += c.c[(dj + 1) * 3 + (di + 1)] * x[c.k + dj * n0 + di];
     return s;
 }
 
 // The owned interior node of every level l0 + L of a single-workgroup V-cycle: its
 // index, the loads of its row faces and 1/diag (slot blockIdx.z of the level
 // arrays in A), b = 0; levels past the last reload the last one and own no node.
-template <typename T, int NLEV>
-__device__ __forceinline__ void cw_row_loads(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[NLEV],
-                                             RowFaces<T> (&rf)[NLEV]) {
+template <typename T>
+__device__ __forceinline__ void cw_row_loads(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS],
+                                             RowFaces<T> (&rf)[CW_LEVS]) {
     const int tid = threadIdx.x, last = A.nlev - 1;
 #pragma unroll
-    for (int L = 0; L < NLEV; L++) {
+    for (int L = 0; L < CW_LEVS; L++) {
         const int l = min(l0 + L, last);
         const CLev<T> &cl = A.L[l];
         const int m0 = cl.n0 - 2, m1 = cl.n1 - 2;
         const int q = min(tid, m0 * m1 - 1);
         const int i = 1 + q % m0, j = 1 + q / m0;
         const size_t o = zslot() * cl.n0 * cl.n1;
-        nd[L].ij = (l0 + L <= last && tid < m0 * m1) ? i | j << 16 : 0;
+        nd[L].ij = i | j << 16;
+        nd[L].k = (l0 + L <= last && tid < m0 * m1) ? j * cl.n0 + i : -1;
         rf[L] = load_row_faces<T>(cl.n0, cl.fen + o, cl.fet + o, cl.fnn + o, cl.fnt + o, i, j);
         nd[L].id = cl.invd[o + (size_t)j * cl.n0 + i];
         nd[L].b = T(0);
     }
 }
 // the rows from the loaded faces, Neumann ghosts folded in
-template <typename T, int NLEV>
-__device__ __forceinline__ void cw_row_fold(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[NLEV],
-                                            const RowFaces<T> (&rf)[NLEV]) {
+template <typename T>
+__device__ __forceinline__ void cw_row_fold(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS],
+                                            const RowFaces<T> (&rf)[CW_LEVS]) {
     const int last = A.nlev - 1;
 #pragma unroll
-    for (int L = 0; L < NLEV; L++) {
+    for (int L = 0; L < CW_LEVS; L++) {
         const int l = min(l0 + L, last);
         fold_row<T>(rf[L], A.L[l].n0, A.L[l].n1, nd[L].i(), nd[L].j(), nd[L].c);
     }
-}
-// The further interior nodes of level l0 a thread owns when that level has more
-// interior nodes than the workgroup has threads: node tid + (m + 1) NTH of ex[m]
-// (index, row faces, 1/diag, b = 0), and their rows.
-template <typename T, int NX>
-__device__ __forceinline__ void cw_row_loads_x(const CoarseArgs<T> &A, int l0, int nth, CNode<T> (&ex)[NX],
-                                               RowFaces<T> (&rf)[NX]) {
-    const CLev<T> &cl = A.L[l0];
-    const int m0 = cl.n0 - 2, m1 = cl.n1 - 2;
-    const size_t o = zslot() * cl.n0 * cl.n1;
-#pragma unroll
-    for (int m = 0; m < NX; m++) {
-        const int qq = (int)threadIdx.x + (m + 1) * nth;
-        const int q = min(qq, m0 * m1 - 1);
-        const int i = 1 + q % m0, j = 1 + q / m0;
-        ex[m].ij = qq < m0 * m1 ? i | j << 16 : 0;
-        rf[m] = load_row_faces<T>(cl.n0, cl.fen + o, cl.fet + o, cl.fnn + o, cl.fnt + o, i, j);
-        ex[m].id = cl.invd[o + (size_t)j * cl.n0 + i];
-        ex[m].b = T(0);
-    }
-}
-template <typename T, int NX>
-__device__ __forceinline__ void cw_row_fold_x(const CoarseArgs<T> &A, int l0, CNode<T> (&ex)[NX],
-                                              const RowFaces<T> (&rf)[NX]) {
-#pragma unroll
-    for (int m = 0; m < NX; m++) fold_row<T>(rf[m], A.L[l0].n0, A.L[l0].n1, ex[m].i(), ex[m].j(), ex[m].c);
 }
 
 // The V-cycle of levels l0 .. A.nlev - 1 in one workgroup of NTH threads (at least
 // the interior nodes of level l0), from the rows, inverse diagonals and level-l0
 // right-hand side in nd[] (nd[L].b of the coarser levels is set here), iterates in
 // the LDS arrays at sh + A.off[l]; the result is level l0's iterate, complete and
-// visible to every thread on return.  NLEV: levels at most (the size of nd); NX:
-// further level-l0 nodes per thread in ex (a level l0 with more interior nodes than
-// NTH; that level is then a block level and the sweeps, residual and prolongation
-// there cover every node the thread owns).
-template <typename T, int NTH, int NLEV, int NX>
-__device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[NLEV],
-                                            CNode<T> (&ex)[NX > 0 ? NX : 1], T *sh) {
+// visible to every thread on return.
+template <typename T, int NTH = CW>
+__device__ __forceinline__ void cw_vcycle(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS], T *sh) {
     const int tid = threadIdx.x;
     const T w = A.omega;
     const int last = A.nlev - 1;
@@ -977,17 +937,11 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    // fn on every node of level L the thread owns (level 0: nd[0] and ex)
-    auto each = [&](int L, auto fn) __attribute__((always_inline)) {
-        fn(nd[L]);
-        if (L == 0)
-#pragma unroll
-            for (int m = 0; m < NX; m++) fn(ex[m]);
-    };
     // `count` Jacobi sweeps of level L (from zero when from_zero), the result left in
     // the level's iterate; WAVE: run by wavefront 0 with wavefront barriers, else by
     // the block (ending with a block barrier)
-    auto sweeps = [&](auto wave_tag, int L, int count, bool from_zero) __attribute__((always_inline)) {
+    auto sweeps = [&](auto wave_tag, const CNode<T> &c, int L, int count, bool from_zero)
+                      __attribute__((always_inline)) {
         constexpr bool WAVE = decltype(wave_tag)::value;
         if (count <= 0) return;
         const int n0 = A.L[l0 + L].n0;
@@ -998,21 +952,19 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
             return from_zero ? (((count - 1 - s) & 1) ? t : x) : ((s & 1) ? x : t);
         };
         for (int s = 0; s < count; s++) {
-            T *out = out_of(s);
-            const T *in = (out == x) ? t : x;
-            each(L, [&](const CNode<T> &c) __attribute__((always_inline)) {
-                if (!c.on()) return;
-                const int k = c.k(n0);
-                if (from_zero && s == 0) out[k] = w * c.b * c.id;
-                else out[k] = in[k] + w * (c.b - cw_row<T>(c, in, n0)) * c.id;
-            });
+            if (c.k >= 0) {
+                T *out = out_of(s);
+                if (from_zero && s == 0) {
+                    out[c.k] = w * c.b * c.id;
+                } else {
+                    const T *in = (out == x) ? t : x;
+                    out[c.k] = in[c.k] + w * (c.b - cw_row<T>(c, in, n0)) * c.id;
+                }
+            }
             if (WAVE) wsync();
             else if (s + 1 < count) __syncthreads();
         }
-        if (!from_zero && (count & 1))  // the last sweep wrote t
-            each(L, [&](const CNode<T> &c) __attribute__((always_inline)) {
-                if (c.on()) x[c.k(n0)] = t[c.k(n0)];
-            });
+        if (!from_zero && (count & 1) && c.k >= 0) x[c.k] = t[c.k];  // the last sweep wrote t
         if (WAVE) wsync();
         else __syncthreads();
     };
@@ -1021,7 +973,7 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
     // full weighting of level L's residual r onto the owned node of level L + 1
     auto restrict_to = [&](const T *r, int n0, const CNode<T> &cc) __attribute__((always_inline)) {
         T v = T(0);
-        if (cc.on()) {
+        if (cc.k >= 0) {
             const int k = 2 * cc.j() * n0 + 2 * cc.i();
             v = (T(4) * r[k] + T(2) * (r[k - 1] + r[k + 1] + r[k - n0] + r[k + n0]) +
                  (r[k - n0 - 1] + r[k - n0 + 1] + r[k + n0 - 1] + r[k + n0 + 1])) /
@@ -1031,7 +983,7 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
     };
     // bilinear prolongation of level L + 1's correction added to level L's iterate
     auto prolong = [&](const CNode<T> &c, int L) __attribute__((always_inline)) {
-        if (!c.on()) return;
+        if (c.k < 0) return;
         const int m0 = A.L[l0 + L + 1].n0, m1 = A.L[l0 + L + 1].n1;
         const T *xcc = XA(L + 1);
         T *x = XA(L);
@@ -1043,24 +995,23 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
         const T a = xcc[J0 + I0], b = xcc[J0 + I1], cn = xcc[J1 + I0], d = xcc[J1 + I1];
         const bool ie = (i & 1) == 0, je = (j & 1) == 0;
         const T e = ie ? (je ? a : T(0.5) * (a + cn)) : (je ? T(0.5) * (a + b) : T(0.25) * (a + b + cn + d));
-        x[c.k(A.L[l0 + L].n0)] += e;
+        x[c.k] += e;
     };
 
     // down, block levels: smoothing, residual, full weighting onto the next level
     // (its interior mean removed)
 #pragma unroll
-    for (int L = 0; L < NLEV - 1; L++) {
+    for (int L = 0; L < CW_LEVS - 1; L++) {
         if (L < nl - 1 && L < LW) {
             const int n0 = A.L[l0 + L].n0;
+            const CNode<T> &c = nd[L];
             T *x = XA(L), *r = TA(L);
-            if (A.pre > 0) sweeps(Block{}, L, A.pre, true);  // with pre == 0 x stays zero
-            each(L, [&](const CNode<T> &c) __attribute__((always_inline)) {
-                if (c.on()) r[c.k(n0)] = c.b - cw_row<T>(c, x, n0);
-            });
+            if (A.pre > 0) sweeps(Block{}, c, L, A.pre, true);  // with pre == 0 x stays zero
+            if (c.k >= 0) r[c.k] = c.b - cw_row<T>(c, x, n0);
             __syncthreads();
             CNode<T> &cc = nd[L + 1];
             const T v = restrict_to(r, n0, cc);
-            const double mean = cw_sum<NTH>(cc.on() ? (double)v : 0.0) / (double)INNER(L + 1);
+            const double mean = cw_sum<NTH>(cc.k >= 0 ? (double)v : 0.0) / (double)INNER(L + 1);
             cc.b = (T)((double)v - mean);
         }
     }
@@ -1068,29 +1019,29 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
     if (LW < nl) {
         if (tid < 64) {
 #pragma unroll
-            for (int L = 0; L < NLEV - 1; L++) {
+            for (int L = 0; L < CW_LEVS - 1; L++) {
                 if (L >= LW && L < nl - 1) {
                     const int n0 = A.L[l0 + L].n0;
                     const CNode<T> &c = nd[L];
                     T *x = XA(L), *r = TA(L);
-                    if (A.pre > 0) sweeps(Wave{}, L, A.pre, true);
-                    if (c.on()) r[c.k(n0)] = c.b - cw_row<T>(c, x, n0);
+                    if (A.pre > 0) sweeps(Wave{}, c, L, A.pre, true);
+                    if (c.k >= 0) r[c.k] = c.b - cw_row<T>(c, x, n0);
                     wsync();
                     CNode<T> &cc = nd[L + 1];
                     const T v = restrict_to(r, n0, cc);
-                    const double tot = wave_sum(cc.on() ? (double)v : 0.0);
+                    const double tot = wave_sum(cc.k >= 0 ? (double)v : 0.0);
                     cc.b = (T)((double)v - tot / (double)INNER(L + 1));
                 }
             }
 #pragma unroll
-            for (int L = 0; L < NLEV; L++)
-                if (L == nl - 1) sweeps(Wave{}, L, A.coarse, true);  // with coarse == 0 x stays zero
+            for (int L = 0; L < CW_LEVS; L++)
+                if (L == nl - 1) sweeps(Wave{}, nd[L], L, A.coarse, true);  // with coarse == 0 x stays zero
 #pragma unroll
-            for (int L = NLEV - 2; L >= 0; L--) {
+            for (int L = CW_LEVS - 2; L >= 0; L--) {
                 if (L >= LW && L < nl - 1) {
                     prolong(nd[L], L);
                     wsync();
-                    sweeps(Wave{}, L, A.post, false);
+                    sweeps(Wave{}, nd[L], L, A.post, false);
                 }
             }
         }
@@ -1098,36 +1049,25 @@ __device__ __forceinline__ void cw_vcycle_x(const CoarseArgs<T> &A, int l0, CNod
     } else {
         // the coarsest level is a block level
 #pragma unroll
-        for (int L = 0; L < NLEV; L++)
-            if (L == nl - 1) sweeps(Block{}, L, A.coarse, true);
+        for (int L = 0; L < CW_LEVS; L++)
+            if (L == nl - 1) sweeps(Block{}, nd[L], L, A.coarse, true);
     }
     // up, block levels: bilinear prolongation of the correction, post-smoothing
 #pragma unroll
-    for (int L = NLEV - 2; L >= 0; L--) {
+    for (int L = CW_LEVS - 2; L >= 0; L--) {
         if (L < nl - 1 && L < LW) {
-            each(L, [&](const CNode<T> &c) __attribute__((always_inline)) { prolong(c, L); });
+            prolong(nd[L], L);
             __syncthreads();
-            sweeps(Block{}, L, A.post, false);
+            sweeps(Block{}, nd[L], L, A.post, false);
         }
     }
-}
-// one node per thread on every level
-template <typename T, int NTH = CW>
-__device__ __forceinline__ void cw_vcycle(const CoarseArgs<T> &A, int l0, CNode<T> (&nd)[CW_LEVS], T *sh) {
-    CNode<T> none[1];
-    none[0].ij = 0;
-    cw_vcycle_x<T, NTH, CW_LEVS, 0>(A, l0, nd, none, sh);
 }
 
 // b0: right-hand side of level l0 (b0part: nb0p block partials of its interior sum,
 // whose mean is removed; null: used as is); the level-l0 result is written to x0out.
-// NX: further level-l0 nodes per thread (0: at most CW_NODES nodes on level l0 and
-// CW_LEVS levels in CN_MAX LDS nodes; else the FP32 shape CWF_*).
-template <typename T, int NX>
+template <typename T>
 __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, CoarseArgs<T> A, int l0, const T *b0,
                                                   const double *b0part, int nb0p, size_t lps, T *x0out) {
-    constexpr int NLEV = NX > 0 ? CWF_LEVS : CW_LEVS;
-    constexpr int NSH = NX > 0 ? CNF_MAX : CN_MAX;
     if (slot_skip(scal + zslot() * SS, gs)) return;
     {
         const size_t o0 = zslot() * A.L[l0].n0 * A.L[l0].n1;
@@ -1135,46 +1075,32 @@ __global__ __launch_bounds__(CW) void k_mg_coarse(const double *scal, int gs, Co
         x0out += o0;
         if (b0part) b0part += zslot() * lps;
     }
-    __shared__ T sh[NSH * CW_ARR];
+    __shared__ T sh[CN_MAX * CW_ARR];
     const int tid = threadIdx.x;
     const int last = A.nlev - 1;
     auto NN = [&](int l) { return A.L[l].n0 * A.L[l].n1; };
     auto INNER = [&](int L) { return (A.L[l0 + L].n0 - 2) * (A.L[l0 + L].n1 - 2); };
 
-    // set-up loads: row faces and 1/diag of the owned nodes of every level
-    CNode<T> nd[NLEV];
-    CNode<T> ex[NX > 0 ? NX : 1];
-    T braw, bx[NX > 0 ? NX : 1];
+    // set-up loads: row faces and 1/diag of the owned node of every level
+    CNode<T> nd[CW_LEVS];
+    RowFaces<T> rf[CW_LEVS];
+    cw_row_loads<T>(A, l0, nd, rf);
+    const T braw = b0[nd[0].j() * A.L[l0].n0 + nd[0].i()];
+    double part = 0.0;
+    if (b0part)
+        for (int t = tid; t < nb0p; t += CW) part += b0part[t];
     {
-        RowFaces<T> rf[NLEV];
-        cw_row_loads<T, NLEV>(A, l0, nd, rf);
-        braw = b0[nd[0].j() * A.L[l0].n0 + nd[0].i()];
-        if constexpr (NX > 0) {
-            RowFaces<T> rx[NX];
-            cw_row_loads_x<T, NX>(A, l0, CW, ex, rx);
-#pragma unroll
-            for (int m = 0; m < NX; m++) bx[m] = b0[ex[m].j() * A.L[l0].n0 + ex[m].i()];
-            cw_row_fold_x<T, NX>(A, l0, ex, rx);
-        } else {
-            ex[0].ij = 0;
-        }
-        double part = 0.0;
-        if (b0part)
-            for (int t = tid; t < nb0p; t += CW) part += b0part[t];
-        {
-            int tot = 0;
-            for (int l = l0; l <= last; l++) tot += CW_ARR * NN(l);
-            for (int k = tid; k < tot; k += CW) sh[k] = T(0);
-        }
-        cw_row_fold<T, NLEV>(A, l0, nd, rf);
+        int tot = 0;
+        for (int l = l0; l <= last; l++) tot += CW_ARR * NN(l);
+        for (int k = tid; k < tot; k += CW) sh[k] = T(0);
+    }
+    cw_row_fold<T>(A, l0, nd, rf);
+    {
         const double mean = cw_sum(part) / (double)INNER(0);  // also orders the zero fill
         nd[0].b = b0part ? (T)((double)braw - mean) : braw;
-        if constexpr (NX > 0)
-#pragma unroll
-            for (int m = 0; m < NX; m++) ex[m].b = b0part ? (T)((double)bx[m] - mean) : bx[m];
     }
 
-    cw_vcycle_x<T, CW, NLEV, NX>(A, l0, nd, ex, sh);
+    cw_vcycle<T>(A, l0, nd, sh);
     {
         const T *x = sh + A.off[l0];
         for (int k = tid; k < NN(l0); k += CW) x0out[k] = x[k];
@@ -1893,7 +1819,6 @@ struct PSolver {
     // the launch-size rule or timed by tune()
     int wdn[PS_MAXLEV] = {}, wup[PS_MAXLEV] = {};
     bool tuned = false;
-    bool cwide = false;  // the coarse kernel takes several first-level nodes per thread
 
     int init(PsCtx &cx, const Geom *g, int nlev_, const cfd_params &prm) {
         nlev = nlev_;
@@ -1903,29 +1828,17 @@ struct PSolver {
         omega = prm.mg_omega;
         if (pre < 0 || pre > 2 || post < 0 || post > 2 || coarse < 0) return CFD_E_ARG;
         // the LDS-resident coarse kernel takes every level from the first one with
-        // at most CwShape<T>::nodes nodes whose levels (at most CwShape<T>::levs)
-        // together fit its LDS arrays
+        // at most CW_NODES nodes whose levels together fit its LDS arrays
         lc = nlev - 1;
-        while (lc > 0 && g[lc - 1].n0 * g[lc - 1].n1 <= CwShape<T>::nodes) lc--;
+        while (lc > 0 && g[lc - 1].n0 * g[lc - 1].n1 <= CW_NODES) lc--;
         for (;;) {
             int tot = 0;
             for (int l = lc; l < nlev; l++) tot += g[l].n0 * g[l].n1;
-            if (tot <= CwShape<T>::cn_max && nlev - lc <= CwShape<T>::levs) break;
+            if (tot <= CN_MAX) break;
             if (lc + 1 >= nlev) return CFD_E_ARG;
             lc++;
         }
-        // the one-node-per-thread form where the first level allows it
-        cwide = (g[lc].n0 - 2) * (g[lc].n1 - 2) > CW || nlev - lc > CW_LEVS;
-        if (cwide) {
-            int tot = 0;
-            for (int l = lc; l < nlev; l++) tot += g[l].n0 * g[l].n1;
-            if (!CwShape<T>::wide || (g[lc].n0 - 2) * (g[lc].n1 - 2) > (CWF_NX + 1) * CW || tot > CNF_MAX)
-                return CFD_E_ARG;
-        } else {
-            int tot = 0;
-            for (int l = lc; l < nlev; l++) tot += g[l].n0 * g[l].n1;
-            if (tot > CN_MAX) return CFD_E_ARG;
-        }
+        if (nlev - lc > CW_LEVS) return CFD_E_ARG;
         B = cx.B;
         for (int l = 0; l < nlev; l++) {
             OpLevel<T> &L = lev[l];
@@ -2168,15 +2081,8 @@ struct PSolver {
             CFD_LAUNCH_CHECK();
         }
         T *cout = lc == 0 ? z : lev[lc].x;
-        const T *cb = lc == 0 ? y : lev[lc].t;
-        const double *cpart = lc == 0 ? nullptr : lpart + lofs[lc];
-        if (cwide) {
-            if constexpr (CwShape<T>::wide)
-                k_mg_coarse<T, CWF_NX><<<dim3(1, 1, B), CW, 0, cx.s>>>(cx.scal, gs, cargs, lc, cb, cpart, lnp[lc],
-                                                                       lps, cout);
-        } else {
-            k_mg_coarse<T, 0><<<dim3(1, 1, B), CW, 0, cx.s>>>(cx.scal, gs, cargs, lc, cb, cpart, lnp[lc], lps, cout);
-        }
+        k_mg_coarse<T><<<dim3(1, 1, B), CW, 0, cx.s>>>(cx.scal, gs, cargs, lc, lc == 0 ? y : lev[lc].t,
+                                                       lc == 0 ? nullptr : lpart + lofs[lc], lnp[lc], lps, cout);
         CFD_LAUNCH_CHECK();
         for (int l = lc - 1; l >= 0; l--) {
             OpLevel<T> &L = lev[l];
