@@ -439,8 +439,8 @@ def run_packed(mixed, cap=150, tol=1e-8, graph=1):
                 "bytes_model": round(mix_bytes), "GBs": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9, 1),
                 "frac": round(mix_bytes / (np.median(round_ms[1:4]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "note": "cycles 2-4 (every case live); algorithmic bytes of the SIMPLE iterations on levels above "
-                        "33 x 33 (p' BiCGStab iterations at their caps: 2 on a case's own grid, 1 on its coarse "
-                        "levels) over the cycle's wall time"},
+                        "33 x 33 (the p' solve at its cap, 2 BiCGStab iterations, on a case's own grid and one "
+                        "V-cycle on its coarse levels) over the cycle's wall time"},
             "precision": "mixed" if mixed else "fp64",
             "workload": "the same 256 cases as one pack: slabs of 513, 257, ..., 17 nodes per side, each case "
                         "holding a slot in every slab from its own size down to its coarsest multigrid level; "
