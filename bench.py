@@ -136,7 +136,9 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the default window, cycles 26-55 of a solve that converges in 59, lies late in
+    # the solve (its per-cycle time matches the early cycles' within 1%)
+    ap.add_argument("--warmup", type=int, default=25)
     ap.add_argument("--n", type=int, default=1025)
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
