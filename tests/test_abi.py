@@ -62,6 +62,12 @@ def test_batch_argument_errors_need_no_gpu():
     assert lib.cfd_batch_create(4, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
     prm.precision = 7
     assert lib.cfd_batch_create(65, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
+    # a case's own p' solve needs at least one BiCGStab iteration (the one-V-cycle
+    # solve, p_maxit -1, is for its coarse levels: fas_coarse_pmaxit)
+    lib.cfd_params_default(C.byref(prm))
+    assert prm.fas_coarse_pmaxit == -1
+    prm.p_maxit = -1
+    assert lib.cfd_batch_create(65, 65, 0.1, 0.1, 2, C.byref(prm), C.byref(h)) == -1
     assert lib.cfd_batch_release(None, 0) == -1
 
 
