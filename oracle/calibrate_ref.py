@@ -1,6 +1,6 @@
 """Timing calibration of the CPU baseline against the compiled reference -- TEST
-INFRASTRUCTURE ONLY; runs only where the reference checkout is present (this
-container), never on a GPU box.
+INFRASTRUCTURE ONLY; runs only on a machine that holds the reference checkout
+(the reference's sources are read from there and compiled by the oracle's Makefile).
 
 The reference (solver.cpp, compiled from where it lies by `make -C oracle ref` at
 -O0 as its own Makefile builds it, and `make -C oracle ref-o3` at -O3) and the
