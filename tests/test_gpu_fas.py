@@ -48,6 +48,24 @@ def test_fas_coarse_p_solves_match_oracle(pmaxit):
         assert rel_l2(fg[k], fc[k]) < 1e-9, k
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fas_nonsquare_matches_oracle(prec):
+    """A 129 x 65 grid (levels 129 x 65, 65 x 33, 33 x 17): five cycles against the
+    oracle, FP64 fields to 1e-9 and mixed to 1e-6, with the one-V-cycle p' solves on
+    the non-square coarse levels."""
+    print(f"\n[gpu] FAS 129x65, precision {prec}, 5 cycles vs oracle", flush=True)
+    gpu, cpu, g = pair(129, n1=65, skew=0.1, beta=1.5, Re=400.0, Ri=1.0, Pr=0.71, fas_min_n=17, precision=prec)
+    assert cpu.fas_depth() == 3
+    for k in range(5):
+        sg = gpu.step()
+        sc = cpu.step()
+        if prec == 0:
+            assert sg.p_iters == sc.p_iters, k
+    fg, fc = gpu.fields(), cpu.fields()
+    for k in ("u", "v", "T", "p"):
+        assert rel_l2(fg[k], fc[k]) < (1e-9 if prec == 0 else 1e-6), (k, rel_l2(fg[k], fc[k]))
+
+
 def test_fas_converges_to_1e8_at_129():
     """129x129 config-2 case to R_m, R_T <= 1e-8 on the GPU in FP64 within 100 cycles
     (the oracle needs 64-68 from 65 to 257 nodes per side)."""
