@@ -1005,6 +1005,13 @@ __device__ __forceinline__ int sl_border(int b, int n0, int n1) {
 __shared__ double sl_sa[SL_NARR * SL_NODES];
 __shared__ double sl_mg[CN_MAX * CW_ARR];
 __device__ __forceinline__ double *sl_arr(int k) { return sl_sa + k * SL_NODES; }
+// The 256-thread kernel (at most SL_FF_MAX nodes on all its p' levels together)
+// also keeps the metric factors of every node's east and north faces of every p'
+// level (p9_face_ab: four arrays, the levels one after the other), which the
+// level's iterations do not change, so each iteration's face tables read them
+// and d from LDS instead of five metric arrays at five nodes from global memory.
+constexpr int SL_FF_MAX = 768;
+__shared__ double sl_ff[4 * SL_FF_MAX];
 
 // a thread's node: interior node (i, j) = index P when own
 struct SlNode {
@@ -1107,9 +1114,63 @@ __device__ __forceinline__ void sl_face_div(const SmallArgs *__restrict__ a, SlN
     sl_arr(SL_B0)[P] = qv;
 }
 
+// node k = (li, lj) of a p' level: the faces touching a wall node zeroed, the four
+// face tables written, and at an interior node 1 / diag of the clamped operator
+__device__ __forceinline__ void sl_face_store(const CLev<double> &cl, size_t ol, int k, int li, int lj, int ln0,
+                                              int ln1, double en, double et, double nn, double nt, double wn,
+                                              double wt, double sn, double st) {
+    auto ev = [&](int ii, int jj) { return ii >= 1 && ii <= ln0 - 3 && jj >= 1 && jj <= ln1 - 2; };
+    auto nv = [&](int ii, int jj) { return jj >= 1 && jj <= ln1 - 3 && ii >= 1 && ii <= ln0 - 2; };
+    if (!ev(li, lj)) en = et = 0.0;
+    if (!nv(li, lj)) nn = nt = 0.0;
+    if (!ev(li - 1, lj)) wn = wt = 0.0;
+    if (!nv(li, lj - 1)) sn = st = 0.0;
+    const_cast<double *>(cl.fen)[ol + k] = en;
+    const_cast<double *>(cl.fet)[ol + k] = et;
+    const_cast<double *>(cl.fnn)[ol + k] = nn;
+    const_cast<double *>(cl.fnt)[ol + k] = nt;
+    if (!inner_node(li, lj, ln0, ln1)) return;
+    double cr[9];
+    p9_row_from<double>(RowFaces<double>{en, et, wn, wt, nn, nt, sn, st}, cr);
+    double dg = 0.0;
+    for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++)
+            if (clampi(li + di, 1, ln0 - 2) == li && clampi(lj + dj, 1, ln1 - 2) == lj) dg += cr[(dj + 1) * 3 + (di + 1)];
+    const_cast<double *>(cl.invd)[ol + k] = 1.0 / dg;
+}
+
+// the metric factors of the p' levels' faces into sl_ff (the 256-thread kernel, once
+// per launch; p9_face_ab of (P, E) and (P, N), neighbours clamped into the grid as
+// in the face tables)
+template <int NT>
+__device__ __forceinline__ void sl_face_factors(const SmallArgs *__restrict__ a) {
+    const int last = a->mg.nlev - 1;
+    int fo = 0;  // the level's first entry
+#pragma unroll
+    for (int L = 0; L < CW_LEVS; L++) {
+        if (L > last) break;
+        const Geom &gl = a->lev[L];
+        const int ln0 = gl.n0, ln1 = gl.n1, nn = ln0 * ln1;
+        const size_t ol = zslot() * (size_t)nn;
+        for (int k = threadIdx.x; k < nn; k += NT) {
+            const int li = k % ln0, lj = k / ln0;
+            auto fn = [&](int ii, int jj) {
+                const int ic = clampi(ii, 0, ln0 - 1), jc = clampi(jj, 0, ln1 - 1);
+                const size_t Q = ol + (size_t)jc * ln0 + ic;
+                return FNode{gl.xix[Q], gl.xiy[Q], gl.etx[Q], gl.ety[Q], gl.jac[Q], 0.0};
+            };
+            const FNode nP = fn(li, lj), nE = fn(li + 1, lj), nN = fn(li, lj + 1);
+            p9_face_ab(nP, nE, 0, sl_ff[fo + k], sl_ff[SL_FF_MAX + fo + k]);
+            p9_face_ab(nP, nN, 1, sl_ff[2 * SL_FF_MAX + fo + k], sl_ff[3 * SL_FF_MAX + fo + k]);
+        }
+        fo += nn;
+    }
+}
+
 // Stage 6a: the face coefficients and inverse diagonals of every p' level from the
 // level's metrics and d injected from this level (k_faces_all's arithmetic), into
-// the level tables (global memory).  Ends with a block barrier.
+// the level tables (global memory).  Ends with a block barrier.  The 256-thread
+// kernel takes each face's metric factors from sl_ff (sl_face_tables_ff).
 template <int NT>
 __device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) {
     const double *const D_ = sl_arr(SL_D);
@@ -1136,26 +1197,43 @@ __device__ __forceinline__ void sl_face_tables(const SmallArgs *__restrict__ a) 
             p9_face_of(nP, nN, 1, r1, r2, r12, nn, nt);
             p9_face_of(nW, nP, 0, r1, r2, r12, wn, wt);
             p9_face_of(nS, nP, 1, r1, r2, r12, sn, st);
-            auto ev = [&](int ii, int jj) { return ii >= 1 && ii <= ln0 - 3 && jj >= 1 && jj <= ln1 - 2; };
-            auto nv = [&](int ii, int jj) { return jj >= 1 && jj <= ln1 - 3 && ii >= 1 && ii <= ln0 - 2; };
-            if (!ev(li, lj)) en = et = 0.0;
-            if (!nv(li, lj)) nn = nt = 0.0;
-            if (!ev(li - 1, lj)) wn = wt = 0.0;
-            if (!nv(li, lj - 1)) sn = st = 0.0;
-            const_cast<double *>(cl.fen)[ol + k] = en;
-            const_cast<double *>(cl.fet)[ol + k] = et;
-            const_cast<double *>(cl.fnn)[ol + k] = nn;
-            const_cast<double *>(cl.fnt)[ol + k] = nt;
-            if (!inner_node(li, lj, ln0, ln1)) continue;
-            double cr[9];
-            p9_row_from<double>(RowFaces<double>{en, et, wn, wt, nn, nt, sn, st}, cr);
-            double dg = 0.0;
-            for (int dj = -1; dj <= 1; dj++)
-                for (int di = -1; di <= 1; di++)
-                    if (clampi(li + di, 1, ln0 - 2) == li && clampi(lj + dj, 1, ln1 - 2) == lj)
-                        dg += cr[(dj + 1) * 3 + (di + 1)];
-            const_cast<double *>(cl.invd)[ol + k] = 1.0 / dg;
+            sl_face_store(cl, ol, k, li, lj, ln0, ln1, en, et, nn, nt, wn, wt, sn, st);
         }
+    }
+    __syncthreads();
+}
+template <int NT>
+__device__ __forceinline__ void sl_face_tables_ff(const SmallArgs *__restrict__ a) {
+    const double *const D_ = sl_arr(SL_D);
+    const int n0 = a->g.n0, last = a->mg.nlev - 1;
+    int fo = 0;  // the level's first entry in sl_ff
+#pragma unroll
+    for (int L = 0; L < CW_LEVS; L++) {
+        if (L > last) break;
+        const Geom &gl = a->lev[L];
+        const CLev<double> &cl = a->mg.L[L];
+        const int ln0 = gl.n0, ln1 = gl.n1;
+        const size_t ol = zslot() * (size_t)ln0 * ln1;
+        const double r1 = 1.0 / (gl.h1 * gl.h1), r2 = 1.0 / (gl.h2 * gl.h2), r12 = 1.0 / (4.0 * gl.h1 * gl.h2);
+        const double *const f = sl_ff + fo;
+        for (int k = threadIdx.x; k < ln0 * ln1; k += NT) {
+            const int li = k % ln0, lj = k / ln0;
+            // d at the clamped nodes; a clamped west or south neighbour's face is not
+            // this node's, but every such face touches a wall node and is zeroed
+            auto dat = [&](int ii, int jj) {
+                const int ic = clampi(ii, 0, ln0 - 1), jc = clampi(jj, 0, ln1 - 1);
+                return D_[(jc << L) * n0 + (ic << L)];
+            };
+            const int kw = lj * ln0 + clampi(li - 1, 0, ln0 - 1), ks = clampi(lj - 1, 0, ln1 - 1) * ln0 + li;
+            const double dP = dat(li, lj);
+            double en, et, nn, nt, wn, wt, sn, st;
+            p9_face_d(f[k], f[SL_FF_MAX + k], dP, dat(li + 1, lj), 0, r1, r2, r12, en, et);
+            p9_face_d(f[2 * SL_FF_MAX + k], f[3 * SL_FF_MAX + k], dP, dat(li, lj + 1), 1, r1, r2, r12, nn, nt);
+            p9_face_d(f[kw], f[SL_FF_MAX + kw], dat(li - 1, lj), dP, 0, r1, r2, r12, wn, wt);
+            p9_face_d(f[2 * SL_FF_MAX + ks], f[3 * SL_FF_MAX + ks], dat(li, lj - 1), dP, 1, r1, r2, r12, sn, st);
+            sl_face_store(cl, ol, k, li, lj, ln0, ln1, en, et, nn, nt, wn, wt, sn, st);
+        }
+        fo += ln0 * ln1;
     }
     __syncthreads();
 }
@@ -1350,6 +1428,7 @@ __global__ __launch_bounds__(NT) void k_simple_small(const SmallArgs *__restrict
         sl_arr(SL_P)[k] = a->p[o + k];
     }
     for (int k = tid; k < CN_MAX * CW_ARR; k += NT) sl_mg[k] = 0.0;  // wall entries stay zero
+    if constexpr (NT == 256) sl_face_factors<NT>(a);
     __syncthreads();
     // The stages run as the cases of one loop: a stage's values cannot stay live into
     // the next one, so the register demand is the largest stage's, not their sum.
@@ -1357,7 +1436,10 @@ __global__ __launch_bounds__(NT) void k_simple_small(const SmallArgs *__restrict
         switch (k % 5) {
         case 0: sl_predict<NT>(a, t); break;
         case 1: sl_face_div(a, t); break;  // the face tables' barrier orders it before the solve
-        case 2: sl_face_tables<NT>(a); break;
+        case 2:
+            if constexpr (NT == 256) sl_face_tables_ff<NT>(a);
+            else sl_face_tables<NT>(a);
+            break;
         case 3: sl_pprime<NT>(a, t); break;
         default: sl_correct_walls<NT>(a, t); break;
         }
@@ -2398,9 +2480,12 @@ int alloc_small(Case *c) {
 int simple_small(Case *c, int iters, int iters_deep) {
     if (iters <= 0 && iters_deep <= 0) return CFD_OK;
     if (!c->sl_args || !c->use_src) return CFD_E_ARG;
-    // a level of at most 256 interior nodes in four wavefronts (each with up to 512
-    // registers per lane), a larger one in sixteen
-    if ((c->n0 - 2) * (c->n1 - 2) <= 256 && 2 * (c->n0 + c->n1) <= 256)
+    // a level of at most 256 interior nodes (and its p' levels' face factors within
+    // SL_FF_MAX) in four wavefronts (each with up to 512 registers per lane), a
+    // larger one in sixteen
+    size_t pnodes = 0;
+    for (int l = 0; l < c->nlev; l++) pnodes += (size_t)c->geom[l].n0 * c->geom[l].n1;
+    if ((c->n0 - 2) * (c->n1 - 2) <= 256 && 2 * (c->n0 + c->n1) <= 256 && pnodes <= (size_t)SL_FF_MAX)
         k_simple_small<256><<<dim3(1, 1, c->B), 256, 0, c->s>>>(c->sl_args, iters, iters_deep);
     else
         k_simple_small<SL_NT><<<dim3(1, 1, c->B), SL_NT, 0, c->s>>>(c->sl_args, iters, iters_deep);

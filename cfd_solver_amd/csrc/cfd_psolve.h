@@ -322,6 +322,28 @@ __device__ __forceinline__ FNode fnode_load(const double *__restrict__ xix, cons
 // host-computed reciprocals (ih1sq = 1/h1^2, ih2sq = 1/h2^2, i4 = 1/(4 h1 h2)); d
 // is level 0's d injected at (i << lv, j << lv).  The caller zeroes faces that
 // touch a wall node.
+// The face's metric factors (a: normal, b: tangential; d does not enter them).
+__device__ __forceinline__ void p9_face_ab(const FNode &P, const FNode &Q, int dir, double &a, double &b) {
+    const double half = 0.5;
+    if (dir == 0) {
+        const double mx = half * (P.j * P.x + Q.j * Q.x), my = half * (P.j * P.y + Q.j * Q.y);
+        a = mx * (P.x + Q.x) + my * (P.y + Q.y);
+        b = mx * (P.e + Q.e) + my * (P.f + Q.f);
+    } else {
+        const double mx = half * (P.j * P.e + Q.j * Q.e), my = half * (P.j * P.f + Q.j * Q.f);
+        a = mx * (P.e + Q.e) + my * (P.f + Q.f);
+        b = mx * (P.x + Q.x) + my * (P.y + Q.y);
+    }
+}
+// the face from its metric factors and the d of its two nodes
+__device__ __forceinline__ void p9_face_d(double a, double b, double dP, double dQ, int dir, double ih1sq,
+                                          double ih2sq, double i4, double &kn, double &kt) {
+    const double half = 0.5;
+    const double de = half * (dP + dQ);
+    kn = half * de * a * (dir == 0 ? ih1sq : ih2sq);
+    kt = half * de * b * i4;
+}
+// (p9_face_ab then p9_face_d, written out)
 __device__ __forceinline__ void p9_face_of(const FNode &P, const FNode &Q, int dir, double ih1sq, double ih2sq,
                                            double i4, double &kn, double &kt) {
     const double half = 0.5;
