@@ -2755,6 +2755,7 @@ int batch_create(int n0, int n1, double dxi, double deta, int slots, const cfd_p
                  bool dry = false, hipStream_t share = nullptr, bool coarse_level = false) {
     if (!out || !params) return CFD_E_ARG;
     if (n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0) || slots < 1 || slots > 65535) return CFD_E_ARG;
+    if ((long long)n0 * n1 > MAX_LEVEL_NODES) return CFD_E_ARG;
     if (params->precision != CFD_FP64 && params->precision != CFD_MIXED) return CFD_E_ARG;
     if (!coarse_level && params->p_maxit < 1) return CFD_E_ARG;  // (-1, one V-cycle: coarse levels)
     *out = nullptr;
@@ -3045,6 +3046,7 @@ int pack_create(int n0, int n1, double dxi, double deta, int nslabs, const int *
                 Pack **out, bool dry) {
     if (!out || !params || !slots || nslabs < 2 || nslabs > 16 || n0 < 5 || n1 < 5 || !(dxi > 0) || !(deta > 0))
         return CFD_E_ARG;
+    if ((long long)n0 * n1 > MAX_LEVEL_NODES) return CFD_E_ARG;
     *out = nullptr;
     // the chain's shapes first (no device work for a bad one)
     for (int k = 0, a = n0, b = n1; k < nslabs; k++) {

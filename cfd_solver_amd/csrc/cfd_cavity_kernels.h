@@ -20,7 +20,12 @@
 
 namespace cfd {
 
-__device__ __forceinline__ size_t idx(int i, int j, int n0) { return (size_t)j * (size_t)n0 + (size_t)i; }
+// Node (i, j) of a level with n0 nodes per row, as an offset from the slot's first
+// node: 32-bit, as no level holds more than MAX_LEVEL_NODES nodes (checked where
+// cases and packs are created), so the address is one 32-bit multiply-add and a
+// widening add, not a 64-bit multiply.
+constexpr long long MAX_LEVEL_NODES = 1LL << 28;
+__device__ __forceinline__ unsigned idx(int i, int j, int n0) { return (unsigned)j * (unsigned)n0 + (unsigned)i; }
 
 struct KGrid {
     int n0, n1;

@@ -79,7 +79,8 @@ inline dim3 grid2d(int n0, int n1) { return dim3((n0 + BX - 1) / BX, (n1 + BY - 
 // to the launch.
 inline int tiles_inner(int n, int w) { return (n - 1 + w - 1) / w; }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped into [lo, hi] (lo <= hi): one min and one max
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(lo, min(v, hi)); }
 
 // Wavefront (64 lanes) reductions of a double with DPP lane moves instead of LDS
 // permutes (a dependent ds_bpermute costs an LDS round trip; a DPP move is a VALU

@@ -230,6 +230,30 @@ __device__ __forceinline__ T frow_f(const FArgs<T> &a, int i, int j, F xat) {
     return frow_v<T>(row_faces<T>(a, i, j), a.n0, a.n1, i, j, xat);
 }
 
+// The same row over an LDS tile of row stride S whose every entry holds the value
+// of the node it covers clamped into the interior (the Neumann ghost already in
+// place): the neighbours are read at fixed offsets from `c`, the entry of the
+// row's own node, with the terms added in frow_v's order, so the result has
+// frow_v's bits.
+template <typename T, int S>
+__device__ __forceinline__ T frow_tile(const RowFaces<T> &f, const T *c) {
+    T w[9];
+    p9_row_from<T>(f, w);
+    T s = T(0);
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+        for (int di = -1; di <= 1; di++) s += w[(dj + 1) * 3 + (di + 1)] * c[dj * S + di];
+    return s;
+}
+
+// entry of node (gi, gj) in an LDS tile of row stride S whose first entry is node
+// (fi0 - H, fj0 - H)
+template <int H, int S>
+__device__ __forceinline__ int tile_at(int gi, int gj, int fi0, int fj0) {
+    return (gj - fj0 + H) * S + (gi - fi0 + H);
+}
+
 // Loads of a strided block loop issued together: element k = tid + m * NT of a
 // COUNT-element region gets load(k) for every m before any is used, so a staging
 // phase costs one memory round trip instead of one per iteration.  The loads are
@@ -591,32 +615,29 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
         __syncthreads();
         mean = s_mean;
     }
+    // Every entry of s_b, s_x1, s_xs and s_r holds the value of its node clamped into
+    // the interior (the loads are at those nodes), so the sweeps and the residual
+    // read their neighbours at fixed offsets (frow_tile); the restriction reads
+    // interior nodes only, and the wall values written to memory stay zero.
     // mean-free right-hand side; first sweep from zero (no neighbours) in place
     batch_run<TL::NB, NTB>(tid, lb, [&](int k, const BI &r) {
-        const int gi = fi0 - TL::HB + k % TL::BWX, gj = fj0 - TL::HB + k / TL::BWX;
-        const bool in = inner_node(gi, gj, n0, n1);
-        const T bv = in ? (MEAN ? (T)((double)r.b - mean) : r.b) : T(0);
+        const T bv = MEAN ? (T)((double)r.b - mean) : r.b;
         s_b[k] = bv;
-        if (NPRE == 2) s_x1[k] = in ? w * bv * r.id : T(0);
+        if (NPRE == 2) s_x1[k] = w * bv * r.id;
     });
     if (NPRE >= 1 && !mg_early<T>()) lx = xs_load();
     __syncthreads();
     if (NPRE >= 1) {
         constexpr int HB = TL::HB, BWX = TL::BWX;
         batch_run<NXR, NTB>(tid, lx, [&](int k, const XR &r) {
-            const int li = k % TL::SX, lj = k / TL::SX;
-            const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
-            const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
-            T v = T(0);
-            if (inner_node(gi, gj, n0, n1)) {
-                if (NPRE == 1) {
-                    v = w * s_b[kb] * r.id;
-                } else {
-                    const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                        return s_x1[(jj - fj0 + HB) * BWX + (ii - fi0 + HB)];
-                    });
-                    v = s_x1[kb] + w * (s_b[kb] - ax) * r.id;
-                }
+            const int gi = clampi(fi0 - 2 + k % TL::SX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / TL::SX, 1, n1 - 2);
+            const int kb = tile_at<HB, BWX>(gi, gj, fi0, fj0);
+            T v;
+            if (NPRE == 1) {
+                v = w * s_b[kb] * r.id;
+            } else {
+                const T *c = s_x1 + kb;
+                v = c[0] + w * (s_b[kb] - frow_tile<T, BWX>(r.f, c)) * r.id;
             }
             s_xs[k] = v;
         });
@@ -625,16 +646,9 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
         batch_run<NXR, NTB>(tid, lx, [&](int k, const XR &r) {
             const int li = k % TL::SX, lj = k / TL::SX;
             if (li < 1 || li > TL::SX - 2 || lj < 1 || lj > TL::SY - 2) return;
-            const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
-            const int kb = (lj + HB - 2) * BWX + (li + HB - 2);
-            T v = T(0);
-            if (inner_node(gi, gj, n0, n1)) {
-                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_xs[(jj - fj0 + 2) * TL::SX + (ii - fi0 + 2)];
-                });
-                v = s_b[kb] - ax;
-            }
-            s_r[(lj - 1) * TL::RX + (li - 1)] = v;
+            const int gi = clampi(fi0 - 2 + li, 1, n0 - 2), gj = clampi(fj0 - 2 + lj, 1, n1 - 2);
+            const T ax = frow_tile<T, TL::SX>(r.f, s_xs + tile_at<2, TL::SX>(gi, gj, fi0, fj0));
+            s_r[(lj - 1) * TL::RX + (li - 1)] = s_b[tile_at<HB, BWX>(gi, gj, fi0, fj0)] - ax;
         });
     }
     // owned nodes: smoothed x and the mean-free right-hand side
@@ -643,8 +657,9 @@ __global__ __launch_bounds__(NTB) void k_mg_down(DownArgs<T> A) {
         const int gi = fi0 + li, gj = fj0 + lj;
         if (gi >= n0 || gj >= n1) continue;
         const size_t P = idx(gi, gj, n0);
-        A.x[P] = NPRE >= 1 ? s_xs[(lj + 2) * TL::SX + (li + 2)] : T(0);
-        if (A.bout) A.bout[P] = s_b[(lj + TL::HB) * TL::BWX + (li + TL::HB)];
+        const bool in = inner_node(gi, gj, n0, n1);
+        A.x[P] = NPRE >= 1 && in ? s_xs[(lj + 2) * TL::SX + (li + 2)] : T(0);
+        if (A.bout) A.bout[P] = in ? s_b[(lj + TL::HB) * TL::BWX + (li + TL::HB)] : T(0);
     }
     edge_walls<MG_FX, MG_FY>(tid, NTB, fi0, fj0, n0, n1, [&](size_t P) {
         A.x[P] = T(0);
@@ -742,19 +757,20 @@ __global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
     if (NPOST >= 1 && mg_early<T>()) lq = q_load();
     if (slot_skip(guard, A.gs)) return;
     edge_walls<MG_FX, MG_FY>(tid, NTB, fi0, fj0, n0, n1, [&](size_t P) { A.out[P] = T(0); });
+    // Every entry of s_y and s_z holds the value of its node clamped into the
+    // interior (the loads are at those nodes), so the sweeps read their neighbours
+    // at fixed offsets (frow_tile); the wall values written to memory stay zero.
     batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
-        const int gi = fi0 - NPOST + k % YX, gj = fj0 - NPOST + k / YX;
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            T e;
-            if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
-            else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
-            else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
-            else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
-            v = r.x + e;
-        }
+        const int ui = fi0 - NPOST + k % YX, uj = fj0 - NPOST + k / YX;
+        const int gi = clampi(ui, 1, n0 - 2), gj = clampi(uj, 1, n1 - 2);
+        T e;
+        if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
+        else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
+        else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
+        else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
+        const T v = r.x + e;
         if (NPOST == 0) {
-            if (gi < n0 && gj < n1) A.out[idx(gi, gj, n0)] = v;
+            if (ui < n0 && uj < n1) A.out[idx(ui, uj, n0)] = inner_node(ui, uj, n0, n1) ? v : T(0);
         } else {
             s_y[k] = v;
         }
@@ -765,16 +781,9 @@ __global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
     // first post sweep on the Q region (NPOST == 2), output on its inner part
     if (NPOST == 2) {
         batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
-            const int li = k % QX, lj = k / QX;
-            const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
-            T v = T(0);
-            if (inner_node(gi, gj, n0, n1)) {
-                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_y[(jj - fj0 + 2) * YX + (ii - fi0 + 2)];
-                });
-                v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
-            }
-            s_z[k] = v;
+            const int gi = clampi(fi0 - 1 + k % QX, 1, n0 - 2), gj = clampi(fj0 - 1 + k / QX, 1, n1 - 2);
+            const T *c = s_y + tile_at<2, YX>(gi, gj, fi0, fj0);
+            s_z[k] = c[0] + w * (r.b - frow_tile<T, YX>(r.f, c)) * r.id;
         });
         __syncthreads();
     }
@@ -786,17 +795,9 @@ __global__ __launch_bounds__(NTB) void k_mg_up(UpArgs<T> A) {
         if (gi >= n0 || gj >= n1) return;
         T v = T(0);
         if (inner_node(gi, gj, n0, n1)) {
-            if (NPOST == 1) {
-                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_y[(jj - fj0 + 1) * YX + (ii - fi0 + 1)];
-                });
-                v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
-            } else {
-                const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_z[(jj - fj0 + 1) * QX + (ii - fi0 + 1)];
-                });
-                v = s_z[(lj + 1) * QX + (li + 1)] + w * (r.b - ax) * r.id;
-            }
+            const T *c = NPOST == 1 ? s_y + tile_at<1, YX>(gi, gj, fi0, fj0) : s_z + tile_at<1, QX>(gi, gj, fi0, fj0);
+            const T ax = NPOST == 1 ? frow_tile<T, YX>(r.f, c) : frow_tile<T, QX>(r.f, c);
+            v = c[0] + w * (r.b - ax) * r.id;
         }
         A.out[idx(gi, gj, n0)] = v;
     });
@@ -1357,49 +1358,39 @@ __global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict_
         A.out[P] = T(0);
         yout[P] = T(0);
     });
+    // Every entry of s_y, s_q and s_o holds the value of its node clamped into the
+    // interior (the loads above are at those nodes), so the sweeps and the operator
+    // read their neighbours at fixed offsets (frow_tile); the wall values written
+    // to memory stay zero.
     batch_run<YX * YY, NTB>(tid, ly, [&](int k, const YR &r) {
-        const int gi = fi0 - 3 + k % YX, gj = fj0 - 3 + k / YX;
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            T e;
-            if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
-            else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
-            else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
-            else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
-            v = r.x + e;
-        }
-        s_y[k] = v;
+        const int gi = clampi(fi0 - 3 + k % YX, 1, n0 - 2), gj = clampi(fj0 - 3 + k / YX, 1, n1 - 2);
+        T e;
+        if ((gi & 1) == 0 && (gj & 1) == 0) e = r.c00;
+        else if ((gj & 1) == 0) e = T(0.5) * (r.c00 + r.c10);
+        else if ((gi & 1) == 0) e = T(0.5) * (r.c00 + r.c01);
+        else e = T(0.25) * (r.c00 + r.c10 + r.c01 + r.c11);
+        s_y[k] = r.x + e;
     });
     if (!mg_early<T>()) lq = q_load();
     __syncthreads();
     // first post-sweep on the halo-2 region
     batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
-        const int li = k % QX, lj = k / QX;
-        const int gi = fi0 - 2 + li, gj = fj0 - 2 + lj;
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                return s_y[(jj - fj0 + 3) * YX + (ii - fi0 + 3)];
-            });
-            v = s_y[(lj + 1) * YX + (li + 1)] + w * (r.b - ax) * r.id;
-        }
-        s_q[k] = v;
+        const int gi = clampi(fi0 - 2 + k % QX, 1, n0 - 2), gj = clampi(fj0 - 2 + k / QX, 1, n1 - 2);
+        const T *c = s_y + tile_at<3, YX>(gi, gj, fi0, fj0);
+        s_q[k] = c[0] + w * (r.b - frow_tile<T, YX>(r.f, c)) * r.id;
     });
     __syncthreads();
     // second post-sweep on the halo-1 region: z, written on the tile
     batch_run<QX * QY, NTB>(tid, lq, [&](int k, const PR &r) {
         const int li = k % QX - 1, lj = k / QX - 1;
         if (li < 0 || li >= ZX || lj < 0 || lj >= ZY) return;
-        const int gi = fi0 - 1 + li, gj = fj0 - 1 + lj;
-        T v = T(0);
-        if (inner_node(gi, gj, n0, n1)) {
-            const T ax = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                return s_q[(jj - fj0 + 2) * QX + (ii - fi0 + 2)];
-            });
-            v = s_q[(lj + 1) * QX + (li + 1)] + w * (r.b - ax) * r.id;
-        }
+        const int ui = fi0 - 1 + li, uj = fj0 - 1 + lj;
+        const int gi = clampi(ui, 1, n0 - 2), gj = clampi(uj, 1, n1 - 2);
+        const T *c = s_q + tile_at<2, QX>(gi, gj, fi0, fj0);
+        const T v = c[0] + w * (r.b - frow_tile<T, QX>(r.f, c)) * r.id;
         s_o[lj * ZX + li] = v;
-        if (li >= 1 && li <= MG_FX && lj >= 1 && lj <= MG_FY && gi < n0 && gj < n1) A.out[idx(gi, gj, n0)] = v;
+        if (li >= 1 && li <= MG_FX && lj >= 1 && lj <= MG_FY && ui < n0 && uj < n1)
+            A.out[idx(ui, uj, n0)] = inner_node(ui, uj, n0, n1) ? v : T(0);
     });
     __syncthreads();
     // y = L z on the tile and the dot products: each node's terms go to LDS and the
@@ -1412,9 +1403,7 @@ __global__ __launch_bounds__(NTB) void k_mg_up_apply(UpArgs<T> A, T *__restrict_
         if (gi < n0 && gj < n1) {
             T y = T(0);
             if (inner_node(gi, gj, n0, n1)) {
-                y = frow_v<T>(r.f, n0, n1, gi, gj, [&](int ii, int jj) {
-                    return s_o[(jj - fj0 + 1) * ZX + (ii - fi0 + 1)];
-                });
+                y = frow_tile<T, ZX>(r.f, s_o + tile_at<1, ZX>(gi, gj, fi0, fj0));
                 d0 = MODE == MGA_V ? (double)r.r0 * (double)y : (double)y * (double)r.b;
                 d1 = (double)y * (double)y;
             }
