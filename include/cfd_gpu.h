@@ -363,7 +363,8 @@ int cfd_fas_coarsest(int n0, int n1, const cfd_params *p);
 /* Create a cavity case on the current device from INP.DAT-style grid arrays
  * (x, y, metrics, alph/beta/gamma, Jacobian) plus the Laplacian control
  * functions p1 = lap(xi), q1 = lap(eta) (NULL = 0, as the reference reads them).
- * Fields start at rest with the wall conditions applied. */
+ * Fields start at rest with the wall conditions applied.  n0, n1 >= 5 and
+ * n0 * n1 <= 2^28 (cases, batches and packs alike; CFD_E_ARG otherwise). */
 int cfd_case_create(int n0, int n1, double dxi, double deta,
                     const double *x, const double *y,
                     const double *xix, const double *xiy, const double *etx, const double *ety,

@@ -106,4 +106,10 @@ def test_pack_shapes_need_no_gpu():
     for n, slots in ((128, [1, 1]), (129, [1]), (129, [1, -1]), (9, [1, 1, 1])):
         arr = (C.c_int * len(slots))(*slots)
         assert lib.cfd_pack_create(n, n, 0.01, 0.01, len(slots), arr, C.byref(prm), C.byref(h)) == -1
+    # a level of more than 2^28 nodes is refused (node offsets within a slot are 32-bit)
+    big = C.c_size_t()
+    arr = (C.c_int * 2)(1, 1)
+    assert lib.cfd_pack_bytes_needed(16385, 16385, 1e-4, 1e-4, 2, arr, C.byref(prm), C.byref(big)) == -1
+    assert lib.cfd_batch_bytes_needed(16385, 16385, 1e-4, 1e-4, 1, C.byref(prm), C.byref(big)) == -1
+    assert lib.cfd_batch_bytes_needed(8193, 16385, 1e-4, 1e-4, 1, C.byref(prm), C.byref(big)) == 0 and big.value > 0
     assert lib.cfd_pack_release(None, 0) == -1 and lib.cfd_pack_launch(None) == -1
