@@ -803,16 +803,18 @@ __device__ __forceinline__ WallIn wall_at(WallIn w, size_t o) {
     return w;
 }
 
-// Tw: the wall node's temperature (the side walls' new values are passed in before
-// they are written back)
-__device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j, double Tw) {
+// Component k (0: x, from u; 1: y, from v, with the buoyancy term) of the pressure
+// gradient the momentum equation gives at wall node (i, j) of wall `side`.  Tw: the
+// wall node's temperature (the side walls' new values are passed in before they
+// are written back)
+__device__ __forceinline__ double wall_dp(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j,
+                                          double Tw, int k) {
     const int n0 = g.n0;
     const double h1 = g.h1, h2 = g.h2;
     const size_t P = idx(i, j, n0);
     const double U = w.xix[P] * w.u[P] + w.xiy[P] * w.v[P];
     const double V = w.etx[P] * w.u[P] + w.ety[P] * w.v[P];
-    double dpx = 0.0, dpy = 0.0;
-    for (int k = 0; k < 2; k++) {
+    {
         const double *f = k == 0 ? w.u : w.v;
 #define F(ii, jj) f[idx(ii, jj, n0)]
         double d2u, conv;
@@ -844,9 +846,16 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
             conv = conv + U * s * (-3 * F(i, j) + 4 * F(i + s, j) - F(i + 2 * s, j)) / (2 * h1);
         }
 #undef F
-        if (k == 0) dpx = d2u / Re - conv;
-        else dpy = d2u / Re - conv + Ri * Tw;
+        return k == 0 ? d2u / Re - conv : d2u / Re - conv + Ri * Tw;
     }
+}
+// the wall value from the two components: p at the next node inward minus the
+// normal derivative times the spacing
+__device__ __forceinline__ double wall_p_from(const WallIn &w, KGrid g, int side, int i, int j, double dpx,
+                                              double dpy) {
+    const int n0 = g.n0;
+    const double h1 = g.h1, h2 = g.h2;
+    const size_t P = idx(i, j, n0);
     const double J = w.ajac[P];
     if (side <= 1) {
         const double tx = -w.xiy[P] * J, ty = w.xix[P] * J;
@@ -856,6 +865,10 @@ __device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, i
     const double tx = w.ety[P] * J, ty = -w.etx[P] * J;
     const double dpn = (dpx * tx + dpy * ty) * h1;
     return side == 2 ? w.p[idx(i + 1, j, n0)] - dpn : w.p[idx(i - 1, j, n0)] + dpn;
+}
+__device__ double wall_p_value(const WallIn &w, KGrid g, double Re, double Ri, int side, int i, int j, double Tw) {
+    const double dpx = wall_dp(w, g, Re, Ri, side, i, j, Tw, 0), dpy = wall_dp(w, g, Re, Ri, side, i, j, Tw, 1);
+    return wall_p_from(w, g, side, i, j, dpx, dpy);
 }
 
 // Wall pressure written in place: wall_p_value reads p only at interior nodes, so
@@ -882,7 +895,10 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
     double *const pw = pout ? pout + zslot() * cg.N : p;
     scratch += zslot() * scr;
     const int side = blockIdx.y;
-    const int t = blockIdx.x * WB + threadIdx.x;
+    // a lane pair per wall node: lane part 0 forms the x component of the pressure
+    // gradient, lane part 1 the y component (two chains of half the length), and
+    // part 0 combines them
+    const int t = (int)(blockIdx.x * WB + threadIdx.x) >> 1, part = threadIdx.x & 1;
     const int n0 = g.n0, n1 = g.n1;
     // temperature of wall node (i, j) of wall `s`: new side-wall value or T
     auto Tw = [&](int s, int i, int j) {
@@ -895,23 +911,30 @@ __global__ __launch_bounds__(WB) void k_p_walls(CGrid cg, WallIn w, double *__re
         const int j = side == 0 ? 0 : (side == 1 ? n1 - 1 : t);
         const double tw = Tw(side, i, j);
         const size_t P = idx(i, j, n0);
-        pw[P] = wall_p_value(w, g, Re, Ri, side, i, j, tw) + (sm ? sm[P] : 0.0);
-        if (side >= 2 && !pout) T[P] = tw;
+        const double dp = wall_dp(w, g, Re, Ri, side, i, j, tw, part);
+        const double dq = __shfl_xor(dp, 1, 64);  // the pair's other component
+        if (part == 0) {
+            pw[P] = wall_p_from(w, g, side, i, j, dp, dq) + (sm ? sm[P] : 0.0);
+            if (side >= 2 && !pout) T[P] = tw;
+        }
         return;
     }
-    if (blockIdx.x != 0 || threadIdx.x >= 8) return;
-    // corner c = threadIdx.x / 2 (bit 0: east, bit 1: north); lane 0 of the pair
-    // takes the bottom/top wall value next to it, lane 1 the side wall value
-    const int c = threadIdx.x >> 1, h = threadIdx.x & 1;
+    if (blockIdx.x != 0 || threadIdx.x >= 16) return;
+    // corner c = threadIdx.x / 4 (bit 0: east, bit 1: north); lanes h = 0 of it take
+    // the bottom/top wall value next to it, h = 1 the side wall value, each value
+    // formed by a lane pair as above
+    const int c = threadIdx.x >> 2, h = (threadIdx.x >> 1) & 1;
     const bool east = c & 1, north = c & 2;
     const int ci = east ? n0 - 1 : 0, cj = north ? n1 - 1 : 0;
     const int ws = h == 0 ? (north ? 1 : 0) : (east ? 3 : 2);
     const int wi = h == 0 ? (east ? n0 - 2 : 1) : ci;
     const int wj = h == 0 ? cj : (north ? n1 - 2 : 1);
-    const double v = wall_p_value(w, g, Re, Ri, ws, wi, wj, Tw(ws, wi, wj));
-    const double o = __shfl_xor(v, 1, 64);
+    const double dp = wall_dp(w, g, Re, Ri, ws, wi, wj, Tw(ws, wi, wj), part);
+    const double dq = __shfl_xor(dp, 1, 64);
+    const double v = wall_p_from(w, g, ws, wi, wj, part == 0 ? dp : dq, part == 0 ? dq : dp);
+    const double o = __shfl_xor(v, 2, 64);
     const size_t C = idx(ci, cj, n0);
-    if (h == 0) pw[C] = 0.5 * (v + o) + (sm ? sm[C] : 0.0);
+    if (h == 0 && part == 0) pw[C] = 0.5 * (v + o) + (sm ? sm[C] : 0.0);
 }
 
 // velocity and temperature wall values (oracle: velocity_temperature_bcs)
@@ -2319,7 +2342,7 @@ int wall_bcs(Case *c, int z0, int nz, bool uvt, const SlotP *sp = nullptr) {
     if (uvt) k_uvt_walls<<<g1, 256, 0, c->s>>>(g, c->u + o, c->v + o, c->T + o);
     const dim3 gs((c->n1 + WB - 1) / WB, 1, nz);
     k_t_sides<<<gs, WB, 0, c->s>>>(g, c->T + o, c->alph + o, c->beta + o, c->scratch + z0 * c->scr, c->scr, sT);
-    k_p_walls<<<dim3((m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o, c->T + o, c->scratch + z0 * c->scr,
+    k_p_walls<<<dim3((2 * m + WB - 1) / WB, 5, nz), WB, 0, c->s>>>(g, w, c->p + o, c->T + o, c->scratch + z0 * c->scr,
                                                                c->scr, sm, nullptr);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
@@ -2423,7 +2446,7 @@ int side_walls(Case *c, bool with_src, const SlotP *sp) {
 // scratch) into fr[3]
 int wall_p_out(Case *c, bool with_src, const SlotP *sp) {
     const int m = std::max(c->n0, c->n1);
-    k_p_walls<<<dim3((m + WB - 1) / WB, 5, c->B), WB, 0, c->s>>>(cg(c, sp), wall_in(c), c->p, c->T, c->scratch,
+    k_p_walls<<<dim3((2 * m + WB - 1) / WB, 5, c->B), WB, 0, c->s>>>(cg(c, sp), wall_in(c), c->p, c->T, c->scratch,
                                                                  c->scr, with_src ? srcs(c).m : nullptr, c->fr[3]);
     CFD_LAUNCH_CHECK();
     return CFD_OK;
