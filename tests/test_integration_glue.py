@@ -1,6 +1,7 @@
 """The C++ glue that binds the reference's Solver to the C ABI (the first ``cpp``
-block of INTEGRATION.md) compiles against include/cfd_gpu.h from a class that holds
-the grid size as a member, as PSolver.cpp:13 does, and links to libcfd_gpu.so."""
+block of the repository's integration guide) compiles against include/cfd_gpu.h
+from a class that holds the grid size as a member, as PSolver.cpp:13 does, and
+links to libcfd_gpu.so."""
 import os
 import re
 import shutil
